@@ -1,0 +1,28 @@
+"""fastbact_amd: MI355X-native (gfx950) k-mer spectrum path of fastBact.
+
+Rolling canonical k-mers over 2-bit packed reads, the Count-Min Sketch,
+HyperLogLog registers, the genome estimator and the solid-k-mer set, as HIP
+kernels behind the C ABI of include/fastbact_gpu.h.
+"""
+from .api import (  # noqa: F401
+    BitSeqArena,
+    Context,
+    CountMinSketch,
+    DeviceBank,
+    FastBactError,
+    GenomeEstimate,
+    HyperLogLog,
+    ReadBank,
+    SolidSet,
+    build_solid_handle,
+    build_solid_kmers,
+    compute_dynamic_kmers,
+    count_stats,
+    estimate_genome,
+    hll_estimate_registers,
+    splitmix64,
+    synth_isolate,
+    synth_params,
+)
+
+__version__ = "0.1.0"

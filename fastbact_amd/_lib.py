@@ -1,0 +1,189 @@
+"""ctypes binding of libfastbact_gpu.so (include/fastbact_gpu.h, include/fastbact_synth.h).
+
+The library is built in-tree by ``python -m fastbact_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfastbact_gpu.so"
+
+FB_OK = 0
+FB_ERR_INVALID = -1
+FB_ERR_OOM = -2
+FB_ERR_OVERFLOW = -3
+FB_ERR_HIP = -4
+FB_ERR_EMPTY = -5
+FB_ERR_NODEV = -6
+FB_ERR_BUSY = -7
+
+ERR_NAMES = {
+    FB_ERR_INVALID: "FB_ERR_INVALID",
+    FB_ERR_OOM: "FB_ERR_OOM",
+    FB_ERR_OVERFLOW: "FB_ERR_OVERFLOW",
+    FB_ERR_HIP: "FB_ERR_HIP",
+    FB_ERR_EMPTY: "FB_ERR_EMPTY",
+    FB_ERR_NODEV: "FB_ERR_NODEV",
+    FB_ERR_BUSY: "FB_ERR_BUSY",
+}
+
+u8p = C.POINTER(C.c_uint8)
+u16p = C.POINTER(C.c_uint16)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+vp = C.c_void_p
+
+
+class ArenaView(C.Structure):
+    _fields_ = [
+        ("data", u64p),
+        ("n_words", C.c_uint64),
+        ("offsets", u64p),
+        ("lengths", u32p),
+        ("n_reads", C.c_uint64),
+    ]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint32),
+        ("target_solid_cov", C.c_int32),
+        ("genome_size", C.c_uint64),
+        ("cms_width", C.c_uint64),
+        ("counter_bits", C.c_int32),
+        ("hll_p", C.c_int32),
+        ("sort", C.c_uint32),
+        ("split", C.c_uint32),
+    ]
+
+
+class CountStats(C.Structure):
+    _fields_ = [
+        ("exact_read_pairs", C.c_uint64),
+        ("total_bases_r1", C.c_uint64),
+        ("total_bases_r2", C.c_uint64),
+        ("total_bases", C.c_uint64),
+        ("min_read_len", C.c_uint64),
+        ("max_read_len", C.c_uint64),
+        ("mean_read_len", C.c_double),
+        ("median_read_len", C.c_uint64),
+    ]
+
+
+class GenomeEstimateC(C.Structure):
+    _fields_ = [
+        ("genome_size", C.c_uint64),
+        ("exact_read_pairs", C.c_uint64),
+        ("total_bases", C.c_uint64),
+        ("median_read_len", C.c_uint64),
+        ("max_read_len", C.c_uint64),
+        ("estimated_cov", C.c_double),
+        ("subsample_factor", C.c_double),
+        ("kmers", C.c_uint32 * 8),
+        ("n_kmers", C.c_uint32),
+        ("confidence", C.c_int32),
+        ("u_ge_1", C.c_uint64),
+        ("u_ge_2", C.c_uint64),
+        ("sample_kmer_depth", C.c_double),
+        ("pairs_sampled", C.c_uint64),
+        ("m_sample", C.c_uint64),
+    ]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("genome_len", C.c_uint64),
+        ("gc", C.c_double),
+        ("operon_copies", C.c_uint32),
+        ("operon_len", C.c_uint32),
+        ("tandem_arrays", C.c_uint32),
+        ("tandem_unit_min", C.c_uint32),
+        ("tandem_unit_max", C.c_uint32),
+        ("tandem_copies_min", C.c_uint32),
+        ("tandem_copies_max", C.c_uint32),
+        ("coverage", C.c_double),
+        ("read_len", C.c_uint32),
+        ("insert_mean", C.c_double),
+        ("insert_sd", C.c_double),
+        ("sub_rate", C.c_double),
+        ("indel_rate", C.c_double),
+    ]
+
+
+# (name, restype, argtypes): every symbol include/fastbact_gpu.h and
+# include/fastbact_synth.h declare.
+SIGNATURES = [
+    ("fb_ctx_create", C.c_int, [C.c_int, C.c_uint64, C.POINTER(vp)]),
+    ("fb_ctx_destroy", None, [vp]),
+    ("fb_last_error", C.c_char_p, [vp]),
+    ("fb_ctx_synchronize", C.c_int, [vp]),
+    ("fb_ctx_device_bytes", C.c_uint64, [vp]),
+    ("fb_version", C.c_char_p, []),
+    ("fb_bank_upload", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(vp)]),
+    ("fb_bank_destroy", None, [vp]),
+    ("fb_bank_total_bases", C.c_uint64, [vp]),
+    ("fb_bank_total_reads", C.c_uint64, [vp]),
+    ("fb_bank_bases_for_k", C.c_uint64, [vp, C.c_uint32]),
+    ("fb_cms_create", C.c_int, [vp, C.c_uint64, C.c_int, C.POINTER(vp)]),
+    ("fb_cms_destroy", None, [vp]),
+    ("fb_cms_width", C.c_uint64, [vp]),
+    ("fb_cms_width_for_genome_size", C.c_uint64, [C.c_uint64]),
+    ("fb_cms_clear", C.c_int, [vp]),
+    ("fb_cms_add_keys", C.c_int, [vp, u64p, C.c_uint64]),
+    ("fb_cms_query", C.c_int, [vp, u64p, C.c_uint64, u32p]),
+    ("fb_cms_add_bank", C.c_int, [vp, vp, C.c_uint32]),
+    ("fb_cms_table_bytes", C.c_uint64, [vp]),
+    ("fb_cms_download", C.c_int, [vp, vp, C.c_uint64]),
+    ("fb_hll_create", C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+    ("fb_hll_destroy", None, [vp]),
+    ("fb_hll_clear", C.c_int, [vp]),
+    ("fb_hll_add_hashes", C.c_int, [vp, u64p, C.c_uint64]),
+    ("fb_hll_add_bank", C.c_int, [vp, vp, C.c_uint32]),
+    ("fb_hll_registers", C.c_int, [vp, u8p]),
+    ("fb_hll_estimate", C.c_int, [vp, C.POINTER(C.c_double)]),
+    ("fb_hll_estimate_registers", C.c_double, [u8p, C.c_int]),
+    ("fb_build_params_default", None, [C.POINTER(BuildParams), C.c_uint32]),
+    ("fb_build_solid", C.c_int, [vp, vp, u8p, u32p, C.c_uint64, C.POINTER(BuildParams), C.POINTER(vp)]),
+    ("fb_solid_destroy", None, [vp]),
+    ("fb_solid_count", C.c_uint64, [vp]),
+    ("fb_solid_key_words", C.c_int, [vp]),
+    ("fb_solid_download", C.c_int, [vp, u64p, u16p, u8p]),
+    ("fb_solid_cms_bytes", C.c_uint64, [vp]),
+    ("fb_solid_cms_download", C.c_int, [vp, vp, C.c_uint64]),
+    ("fb_solid_hll_registers", C.c_int, [vp, u8p]),
+    ("fb_solid_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("fb_solid_stats", C.c_int, [vp, u64p]),
+    ("fb_count_stats_from_lengths", C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(CountStats)]),
+    ("fb_compute_dynamic_kmers", C.c_uint32, [C.c_uint64, C.c_int64, u32p]),
+    ("fb_estimate_genome", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.POINTER(CountStats),
+                                     C.c_uint64, C.c_double, C.POINTER(GenomeEstimateC)]),
+    ("fb_pack_ascii", C.c_int, [u8p, u32p, C.c_uint64, u64p, u64p]),
+    ("fb_splitmix64", C.c_uint64, [C.c_uint64]),
+    ("fb_synth_params_default", None, [C.POINTER(SynthParams)]),
+    ("fb_synth_genome", C.c_int, [C.POINTER(SynthParams), u8p]),
+    ("fb_synth_n_pairs", C.c_uint64, [C.POINTER(SynthParams)]),
+    ("fb_synth_reads", C.c_int, [C.POINTER(SynthParams), u8p, C.c_uint64, C.c_uint64, u8p, u8p]),
+]
+
+_LIB = None
+
+
+def load() -> C.CDLL:
+    """Loads the built library once; raises when it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP extension with `python -m fastbact_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib

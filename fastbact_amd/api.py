@@ -1,0 +1,485 @@
+"""Host-side mirror of fastBact's k-mer spectrum interface over the C ABI.
+
+Names and argument meaning follow the reference:
+  ReadBank / BitSeqArena            src/common/bitseq.rs:145-399
+  CountMinSketch                    src/common/sketch.rs:121-254
+  HyperLogLog                       src/common/sketch.rs:24-115
+  build_solid_kmers                 src/graph/build.rs:37-328 (passes 1-3 of
+                                    build_cdbg_with_contigs_ext)
+  estimate_genome, count_stats, compute_dynamic_kmers
+                                    src/ingest/estimator.rs:104-440
+Every call runs on the GPU through libfastbact_gpu.so; errors raise
+FastBactError carrying the library's status and message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArenaView, BuildParams, CountStats, GenomeEstimateC, SynthParams
+
+
+class FastBactError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_lib.ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None and a.size else None
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def splitmix64(z: int) -> int:
+    return int(_lib.load().fb_splitmix64(C.c_uint64(z & 0xFFFFFFFFFFFFFFFF)))
+
+
+# ---------------------------------------------------------------------------
+# Arenas
+
+
+@dataclass
+class BitSeqArena:
+    """2-bit packed reads (bitseq.rs:145-149): base g at bits 2*(g&31) of data[g>>5]."""
+
+    data: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    offsets: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    lengths: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+
+    @classmethod
+    def from_ascii(cls, seqs: Sequence[bytes]) -> "BitSeqArena":
+        lens = np.array([len(s) for s in seqs], dtype=np.uint32)
+        return cls.from_concat(b"".join(seqs), lens)
+
+    @classmethod
+    def from_concat(cls, ascii_concat, lengths) -> "BitSeqArena":
+        """Packs concatenated ASCII reads (BitSeqArena::push_seq for each)."""
+        lens = _u32(lengths)
+        buf = np.frombuffer(ascii_concat, dtype=np.uint8) if isinstance(ascii_concat, (bytes, bytearray)) \
+            else np.ascontiguousarray(ascii_concat, dtype=np.uint8)
+        total = int(lens.sum(dtype=np.uint64))
+        if buf.size < total:
+            raise ValueError("ascii shorter than the sum of lengths")
+        data = np.zeros((total + 31) // 32, dtype=np.uint64)
+        offs = np.zeros(lens.size, dtype=np.uint64)
+        rc = _lib.load().fb_pack_ascii(_ptr(buf, C.c_uint8), _ptr(lens, C.c_uint32), lens.size,
+                                       _ptr(data, C.c_uint64), _ptr(offs, C.c_uint64))
+        if rc:
+            raise FastBactError(rc, "fb_pack_ascii failed")
+        return cls(data, offs, lens)
+
+    def __len__(self) -> int:
+        return int(self.lengths.size)
+
+    def total_bases(self) -> int:
+        return int(self.lengths.sum(dtype=np.uint64))
+
+    def view(self) -> ArenaView:
+        self.data = _u64(self.data)
+        self.offsets = _u64(self.offsets)
+        self.lengths = _u32(self.lengths)
+        return ArenaView(_ptr(self.data, C.c_uint64), self.data.size, _ptr(self.offsets, C.c_uint64),
+                         _ptr(self.lengths, C.c_uint32), self.lengths.size)
+
+
+@dataclass
+class ReadBank:
+    """{merged_reads, pe_forward, pe_reverse} (bitseq.rs:350-354)."""
+
+    merged_reads: BitSeqArena = field(default_factory=BitSeqArena)
+    pe_forward: BitSeqArena = field(default_factory=BitSeqArena)
+    pe_reverse: BitSeqArena = field(default_factory=BitSeqArena)
+
+    def arenas(self):
+        return [self.merged_reads, self.pe_forward, self.pe_reverse]
+
+    def total_bases(self) -> int:
+        return sum(a.total_bases() for a in self.arenas())
+
+    def total_reads(self) -> int:
+        return sum(len(a) for a in self.arenas())
+
+
+# ---------------------------------------------------------------------------
+# Device context and objects
+
+
+class Context:
+    """One GPU context: a HIP stream and a device memory pool."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rc = self.lib.fb_ctx_create(device, hbm_budget, C.byref(h))
+        if rc:
+            raise FastBactError(rc, f"fb_ctx_create(device={device}) failed")
+        self.h = h
+
+    def check(self, rc: int, what: str) -> None:
+        if rc:
+            raise FastBactError(rc, f"{what}: {self.lib.fb_last_error(self.h).decode(errors='replace')}")
+
+    def synchronize(self) -> None:
+        self.check(self.lib.fb_ctx_synchronize(self.h), "fb_ctx_synchronize")
+
+    def device_bytes(self) -> int:
+        return int(self.lib.fb_ctx_device_bytes(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.fb_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Handle:
+    _destroy = ""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            getattr(self.ctx.lib, self._destroy)(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBank(_Handle):
+    """A ReadBank resident in HBM."""
+
+    _destroy = "fb_bank_destroy"
+
+    def __init__(self, ctx: Context, bank: ReadBank):
+        views = (ArenaView * 3)(*[a.view() for a in bank.arenas()])
+        h = C.c_void_p()
+        ctx.check(ctx.lib.fb_bank_upload(ctx.h, views, C.byref(h)), "fb_bank_upload")
+        super().__init__(ctx, h)
+
+    def total_bases(self) -> int:
+        return int(self.ctx.lib.fb_bank_total_bases(self.h))
+
+    def total_reads(self) -> int:
+        return int(self.ctx.lib.fb_bank_total_reads(self.h))
+
+    def bases_for_k(self, k: int) -> int:
+        return int(self.ctx.lib.fb_bank_bases_for_k(self.h, k))
+
+
+class CountMinSketch(_Handle):
+    """CountMinSketch (sketch.rs:121-254); counter_bits 4 = reference nibbles, 16 = u16."""
+
+    _destroy = "fb_cms_destroy"
+
+    def __init__(self, ctx: Context, width: int, counter_bits: int = 4):
+        h = C.c_void_p()
+        ctx.check(ctx.lib.fb_cms_create(ctx.h, width, counter_bits, C.byref(h)), "fb_cms_create")
+        super().__init__(ctx, h)
+        self.counter_bits = counter_bits
+
+    @staticmethod
+    def width_for_genome_size(g: int) -> int:
+        return int(_lib.load().fb_cms_width_for_genome_size(g))
+
+    @property
+    def width(self) -> int:
+        return int(self.ctx.lib.fb_cms_width(self.h))
+
+    def add(self, keys) -> None:
+        k = _u64(np.atleast_1d(keys))
+        self.ctx.check(self.ctx.lib.fb_cms_add_keys(self.h, _ptr(k, C.c_uint64), k.size), "fb_cms_add_keys")
+
+    def estimate(self, keys) -> np.ndarray:
+        k = _u64(np.atleast_1d(keys))
+        out = np.zeros(k.size, np.uint32)
+        self.ctx.check(self.ctx.lib.fb_cms_query(self.h, _ptr(k, C.c_uint64), k.size, _ptr(out, C.c_uint32)),
+                       "fb_cms_query")
+        return out
+
+    def add_bank(self, bank: DeviceBank, k: int) -> None:
+        self.ctx.check(self.ctx.lib.fb_cms_add_bank(self.h, bank.h, k), "fb_cms_add_bank")
+
+    def clear(self) -> None:
+        self.ctx.check(self.ctx.lib.fb_cms_clear(self.h), "fb_cms_clear")
+
+    def table(self) -> np.ndarray:
+        """Counters in the reference layout (uint8 nibble bytes or uint16)."""
+        nbytes = int(self.ctx.lib.fb_cms_table_bytes(self.h))
+        out = np.zeros(nbytes, np.uint8)
+        self.ctx.check(self.ctx.lib.fb_cms_download(self.h, out.ctypes.data, nbytes), "fb_cms_download")
+        return out if self.counter_bits == 4 else out.view(np.uint16)
+
+
+class HyperLogLog(_Handle):
+    """HyperLogLog (sketch.rs:24-115) at precision p."""
+
+    _destroy = "fb_hll_destroy"
+
+    def __init__(self, ctx: Context, p: int = 14):
+        h = C.c_void_p()
+        ctx.check(ctx.lib.fb_hll_create(ctx.h, p, C.byref(h)), "fb_hll_create")
+        super().__init__(ctx, h)
+        self.p = p
+
+    def add_hash(self, hashes) -> None:
+        a = _u64(np.atleast_1d(hashes))
+        self.ctx.check(self.ctx.lib.fb_hll_add_hashes(self.h, _ptr(a, C.c_uint64), a.size), "fb_hll_add_hashes")
+
+    def add_bank(self, bank: DeviceBank, k: int) -> None:
+        self.ctx.check(self.ctx.lib.fb_hll_add_bank(self.h, bank.h, k), "fb_hll_add_bank")
+
+    def registers(self) -> np.ndarray:
+        out = np.zeros(1 << self.p, np.uint8)
+        self.ctx.check(self.ctx.lib.fb_hll_registers(self.h, _ptr(out, C.c_uint8)), "fb_hll_registers")
+        return out
+
+    def estimate(self) -> float:
+        v = C.c_double()
+        self.ctx.check(self.ctx.lib.fb_hll_estimate(self.h, C.byref(v)), "fb_hll_estimate")
+        return v.value
+
+    def clear(self) -> None:
+        self.ctx.check(self.ctx.lib.fb_hll_clear(self.h), "fb_hll_clear")
+
+
+def hll_estimate_registers(regs: np.ndarray, p: int) -> float:
+    r = np.ascontiguousarray(regs, dtype=np.uint8)
+    return float(_lib.load().fb_hll_estimate_registers(_ptr(r, C.c_uint8), p))
+
+
+@dataclass
+class SolidSet:
+    """Solid k-mers {K -> (cov, mask)} of one pass, sorted by K.
+
+    keys: (n, words) uint64, words most significant first (1: u64, 2: u128
+    {hi, lo}, 4: [u64;4]) as the reference's KmerInt types store them."""
+
+    k: int
+    keys: np.ndarray
+    cov: np.ndarray
+    mask: np.ndarray
+    cms_table: Optional[np.ndarray] = None
+    hll_registers: Optional[np.ndarray] = None
+    timings_ms: tuple = ()
+
+    def __len__(self) -> int:
+        return int(self.cov.size)
+
+
+class SolidHandle(_Handle):
+    """Device-resident result of one pass (CMS, HLL, solid set)."""
+
+    _destroy = "fb_solid_destroy"
+
+    def count(self) -> int:
+        return int(self.ctx.lib.fb_solid_count(self.h))
+
+    def key_words(self) -> int:
+        return int(self.ctx.lib.fb_solid_key_words(self.h))
+
+    def timings(self) -> tuple:
+        t = (C.c_float * 4)()
+        self.ctx.check(self.ctx.lib.fb_solid_timings(self.h, t), "fb_solid_timings")
+        return tuple(float(x) for x in t)
+
+    def stats(self) -> dict:
+        t = (C.c_uint64 * 4)()
+        self.ctx.check(self.ctx.lib.fb_solid_stats(self.h, t), "fb_solid_stats")
+        return {"windows": int(t[0]), "records": int(t[1]), "keys": int(t[2]), "slots": int(t[3])}
+
+    def download(self, k: int, with_cms: bool = True, counter_bits: int = 4, hll_p: int = 0) -> SolidSet:
+        n, nw = self.count(), self.key_words()
+        keys = np.zeros((n, nw), np.uint64)
+        cov = np.zeros(n, np.uint16)
+        mask = np.zeros(n, np.uint8)
+        self.ctx.check(self.ctx.lib.fb_solid_download(self.h, _ptr(keys, C.c_uint64), _ptr(cov, C.c_uint16),
+                                                      _ptr(mask, C.c_uint8)), "fb_solid_download")
+        cms = None
+        if with_cms:
+            nb = int(self.ctx.lib.fb_solid_cms_bytes(self.h))
+            buf = np.zeros(nb, np.uint8)
+            self.ctx.check(self.ctx.lib.fb_solid_cms_download(self.h, buf.ctypes.data, nb), "fb_solid_cms_download")
+            cms = buf if counter_bits == 4 else buf.view(np.uint16)
+        regs = None
+        if hll_p:
+            regs = np.zeros(1 << hll_p, np.uint8)
+            self.ctx.check(self.ctx.lib.fb_solid_hll_registers(self.h, _ptr(regs, C.c_uint8)),
+                           "fb_solid_hll_registers")
+        return SolidSet(k, keys, cov, mask, cms, regs, self.timings())
+
+
+def _pack_contigs(contigs: Optional[Sequence[bytes]]):
+    if not contigs:
+        return None, None, 0
+    lens = _u32([len(c) for c in contigs])
+    buf = np.frombuffer(b"".join(contigs), dtype=np.uint8).copy()
+    return buf, lens, len(contigs)
+
+
+def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Optional[int] = None,
+                 cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
+                 split: int = 0) -> BuildParams:
+    p = BuildParams()
+    _lib.load().fb_build_params_default(C.byref(p), k)
+    p.target_solid_cov = -1 if target_solid_cov is None else int(target_solid_cov)
+    p.genome_size = 0 if genome_size is None else int(genome_size)
+    p.cms_width = 0 if cms_width is None else int(cms_width)
+    p.counter_bits = counter_bits
+    p.hll_p = hll_p
+    p.sort = 1 if sort else 0
+    p.split = split
+    return p
+
+
+def build_solid_handle(ctx: Context, bank: DeviceBank, k: int, contigs: Optional[Sequence[bytes]] = None,
+                       **kw) -> SolidHandle:
+    """Runs passes 1-3 on the device and keeps the result resident."""
+    p = build_params(k, **kw)
+    buf, lens, n = _pack_contigs(contigs)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_build_solid(ctx.h, bank.h, _ptr(buf, C.c_uint8), _ptr(lens, C.c_uint32), n,
+                                     C.byref(p), C.byref(h)), "fb_build_solid")
+    return SolidHandle(ctx, h)
+
+
+def build_solid_kmers(ctx: Context, bank: DeviceBank, k: int, target_solid_cov: Optional[int] = None,
+                      genome_size: Optional[int] = None, contigs: Optional[Sequence[bytes]] = None,
+                      cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0,
+                      with_cms: bool = True, split: int = 0) -> SolidSet:
+    """build_cdbg_with_contigs_ext passes 1-3 (build.rs:67-328): the solid set,
+    the pass's CMS table and (hll_p > 0) the pass's HLL registers."""
+    h = build_solid_handle(ctx, bank, k, contigs, target_solid_cov=target_solid_cov, genome_size=genome_size,
+                           cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, split=split)
+    try:
+        return h.download(k, with_cms=with_cms, counter_bits=counter_bits, hll_p=hll_p)
+    finally:
+        h.close()
+
+
+# ---------------------------------------------------------------------------
+# Estimator
+
+CONFIDENCE = {0: "HIGH", 1: "LOW_COVERAGE_WARNING", 2: "UNRELIABLE"}
+
+
+@dataclass
+class GenomeEstimate:
+    genome_size: int
+    exact_read_pairs: int
+    total_bases: int
+    median_read_len: int
+    max_read_len: int
+    estimated_cov: float
+    subsample_factor: float
+    kmers: list
+    confidence: str
+    u_ge_1: int
+    u_ge_2: int
+    sample_kmer_depth: float
+    pairs_sampled: int
+    m_sample: int
+
+
+def count_stats(len_r1, len_r2) -> CountStats:
+    """count_fastx_exact's statistics from the read lengths of the pairs."""
+    a, b = _u32(len_r1), _u32(len_r2)
+    if a.size != b.size:
+        raise FastBactError(_lib.FB_ERR_INVALID, "R1 and R2 hold different numbers of reads")
+    st = CountStats()
+    rc = _lib.load().fb_count_stats_from_lengths(_ptr(a, C.c_uint32), _ptr(b, C.c_uint32), a.size, C.byref(st))
+    if rc:
+        raise FastBactError(rc, "Input FASTQ files contain 0 valid read pairs" if rc == _lib.FB_ERR_EMPTY
+                            else "fb_count_stats_from_lengths failed")
+    return st
+
+
+def compute_dynamic_kmers(median_len: int, target_max_k: Optional[int] = None) -> list:
+    out = (C.c_uint32 * 8)()
+    n = _lib.load().fb_compute_dynamic_kmers(median_len, -1 if target_max_k is None else target_max_k, out)
+    return [int(out[i]) for i in range(n)]
+
+
+def estimate_genome(ctx: Context, r1: BitSeqArena, r2: BitSeqArena, max_pairs: int, target_cov: float,
+                    stats: Optional[CountStats] = None) -> GenomeEstimate:
+    """estimate_genome (estimator.rs:286-440) over R1[i]/R2[i] pairs."""
+    if stats is None:
+        stats = count_stats(r1.lengths, r2.lengths)
+    v1, v2 = r1.view(), r2.view()
+    out = GenomeEstimateC()
+    ctx.check(ctx.lib.fb_estimate_genome(ctx.h, C.byref(v1), C.byref(v2), C.byref(stats), max_pairs,
+                                         float(target_cov), C.byref(out)), "fb_estimate_genome")
+    return GenomeEstimate(out.genome_size, out.exact_read_pairs, out.total_bases, out.median_read_len,
+                          out.max_read_len, out.estimated_cov, out.subsample_factor,
+                          [int(out.kmers[i]) for i in range(out.n_kmers)], CONFIDENCE[out.confidence],
+                          out.u_ge_1, out.u_ge_2, out.sample_kmer_depth, out.pairs_sampled, out.m_sample)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic isolates
+
+
+def synth_params(**kw) -> SynthParams:
+    p = SynthParams()
+    _lib.load().fb_synth_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+@dataclass
+class SynthIsolate:
+    genome: np.ndarray      # ASCII uint8
+    r1: np.ndarray          # (n_pairs, read_len) ASCII
+    r2: np.ndarray
+
+    def bank(self) -> ReadBank:
+        """Unmerged pairs: R1 -> pe_forward, R2 -> pe_reverse."""
+        n, L = self.r1.shape
+        lens = np.full(n, L, np.uint32)
+        return ReadBank(BitSeqArena(), BitSeqArena.from_concat(self.r1.reshape(-1), lens),
+                        BitSeqArena.from_concat(self.r2.reshape(-1), lens))
+
+
+def synth_isolate(n_pairs: Optional[int] = None, **kw) -> SynthIsolate:
+    lib = _lib.load()
+    p = synth_params(**kw)
+    g = np.zeros(p.genome_len, np.uint8)
+    rc = lib.fb_synth_genome(C.byref(p), _ptr(g, C.c_uint8))
+    if rc:
+        raise FastBactError(rc, "fb_synth_genome failed")
+    n = int(lib.fb_synth_n_pairs(C.byref(p))) if n_pairs is None else int(n_pairs)
+    r1 = np.zeros((n, p.read_len), np.uint8)
+    r2 = np.zeros((n, p.read_len), np.uint8)
+    rc = lib.fb_synth_reads(C.byref(p), _ptr(g, C.c_uint8), g.size, n, _ptr(r1, C.c_uint8), _ptr(r2, C.c_uint8))
+    if rc:
+        raise FastBactError(rc, "fb_synth_reads failed")
+    return SynthIsolate(g, r1, r2)

@@ -1,0 +1,107 @@
+// fb_internal.hpp -- launchers shared between the kernels and the runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fb {
+
+constexpr uint64_t kEmpty = ~0ull;     // empty key word of the solid table
+constexpr int kArenaPadWords = 8;      // zero words after every device arena
+
+// One device read list: bases packed LSB-first, 32 per word (the reference's
+// BitSeqArena layout, src/common/bitseq.rs:145-149).  Several host arenas are
+// concatenated word-aligned; offs[] are global base indices.
+struct DevReads {
+  const uint64_t* data = nullptr;
+  const uint64_t* offs = nullptr;
+  const uint32_t* lens = nullptr;
+  uint64_t n_reads = 0;
+};
+
+// Count-Min table: rows*width u32 exact counts (index = row*width + col).
+struct DevCms {
+  uint32_t* counts = nullptr;
+  uint32_t log2w = 0;
+};
+
+struct DevHll {
+  uint32_t* regs = nullptr;  // 2^p u32 registers
+  uint32_t p = 0;
+};
+
+// Open-addressing solid table.  Slot = kw key words (k-mer packed 31 bases
+// per 62-bit word, least significant word first; all-ones = empty), then u32
+// cov, u32 pad, u32 edge[8].
+struct DevSolidTable {
+  uint64_t* slots = nullptr;
+  uint64_t cap = 0;           // power of two
+  uint32_t kw = 1;            // key words
+  uint32_t stride = 0;        // u64 words per slot = kw + 5
+  uint32_t max_probe = 0;
+  uint32_t* status = nullptr; // [0] = overflow flag, [1] = claimed slots
+};
+
+struct Pass1Args {
+  DevReads reads;
+  uint32_t k;
+  uint32_t split;        // threads per read
+  uint32_t weight;       // 1 for reads, 10 for prior contigs (build.rs:124,133)
+  DevCms cms;
+  DevHll hll;            // regs == nullptr: no HLL
+};
+
+struct Pass2Args {
+  DevReads reads;
+  uint32_t k;
+  uint32_t split;
+  const uint32_t* solid_bits;  // rows*width bits: count >= t_solid
+  uint32_t log2w;
+  DevSolidTable table;
+  uint32_t min_trans_cov;      // edge counters stop being incremented at this value
+  uint32_t contig_mode;        // 1: contig backbone (+100 / +1000, no CMS gate)
+};
+
+// Pass 1 (build.rs:93-138) with the per-pass HLL over splitmix64(ntHash).
+hipError_t launch_pass1(const Pass1Args& a, hipStream_t s);
+// Bits of count >= t (saturating-counter semantics: cap 15 or 65535).
+hipError_t launch_solid_bits(const uint32_t* counts, uint64_t n_counters, uint32_t t, uint32_t cap,
+                             uint32_t* bits, hipStream_t s);
+// Reference-layout packing of the exact counts.
+hipError_t launch_pack_nibble(const uint32_t* counts, uint64_t n_counters, uint8_t* out, hipStream_t s);
+hipError_t launch_pack_u16(const uint32_t* counts, uint64_t n_counters, uint16_t* out, hipStream_t s);
+hipError_t launch_pack_u8(const uint32_t* regs, uint64_t n, uint8_t* out, hipStream_t s);
+// Table initialisation (all key words empty, counters zero).
+hipError_t launch_table_clear(const DevSolidTable& t, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+// Pass 2 (build.rs:140-289).
+hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
+// Pass 3 (build.rs:291-328): compaction into reference key words.
+// out_keys: n_ref_words words per record, most significant first.
+hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t n_ref_words, uint32_t min_trans_cov,
+                        uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, uint32_t* out_count,
+                        unsigned long long* rec_sum, hipStream_t s);
+// Sort the compacted records by k-mer (integer order).  Uses scratch from
+// the caller: tmp_bytes queried with keys == nullptr.
+hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t n_ref_words,
+                      uint32_t k, void* tmp, size_t* tmp_bytes, hipStream_t s);
+
+// CountMinSketch::add / estimate on explicit 64-bit keys.
+hipError_t launch_cms_add_keys(DevCms c, const uint64_t* keys, uint64_t n, uint32_t weight, hipStream_t s);
+hipError_t launch_cms_query_keys(DevCms c, const uint64_t* keys, uint64_t n, uint32_t cap, uint32_t* out,
+                                 hipStream_t s);
+hipError_t launch_hll_add_hashes(DevHll h, const uint64_t* hashes, uint64_t n, hipStream_t s);
+
+// Estimator (estimator.rs:296-379), first-insertion-time form.  Reads are
+// R1[0], R2[0], R1[1], R2[1], ... in the order the reference visits them.
+struct EstArgs {
+  DevReads reads;               // interleaved sample reads
+  const uint32_t* occ_base;     // per read: global index of its first window
+  uint32_t n_reads;
+  uint32_t* first_ins;          // 4 * 2^23 cells, initialised to ~0
+  uint32_t* first_total;        // 2^14 * 64 (register, rho) -> first pair index
+  uint32_t* first_solid;
+};
+hipError_t launch_est_first_insert(const EstArgs& a, hipStream_t s);
+hipError_t launch_est_solid(const EstArgs& a, hipStream_t s);
+
+}  // namespace fb

@@ -1,0 +1,644 @@
+// fb_kernels.hip -- HIP kernels of the k-mer spectrum path (gfx950).
+//
+// Pass 1  (src/graph/build.rs:93-138): canonical k-mer -> to_u64 fold -> 4-row
+//         Count-Min increments.  Counts are exact u32 cells; the reference's
+//         saturating nibble / u16 counters are min(count, cap) of them, which
+//         is order-independent (every update is a non-negative add).
+// Pass 2  (build.rs:140-289): adjacent windows whose CMS estimates are both
+//         >= t_solid add (canonical, edge) records to an open-addressing table.
+// Pass 3  (build.rs:291-328): edge mask, cov = max(min(cov,65535)/2, 1), compaction.
+// Estimator (src/ingest/estimator.rs:296-379) in first-insertion-time form.
+#include <hipcub/hipcub.hpp>
+
+#include "fb_device.hpp"
+#include "fb_internal.hpp"
+
+namespace fb {
+using namespace fbk;
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(uint64_t threads) {
+  uint64_t g = (threads + kBlock - 1) / kBlock;
+  return (unsigned)(g ? g : 1);
+}
+
+// Splits the `units` of read r evenly over `split` threads; returns false when
+// this thread's share is empty.
+__device__ __forceinline__ bool read_share(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k,
+                                           uint32_t extra, uint64_t& r, uint32_t& u0, uint32_t& u1) {
+  r = gid / split;
+  uint32_t part = (uint32_t)(gid - r * split);
+  if (r >= R.n_reads) return false;
+  uint32_t len = R.lens[r];
+  if (len < k + extra) return false;
+  uint32_t units = len - k + 1 - extra;
+  u0 = (uint32_t)((uint64_t)part * units / split);
+  u1 = (uint32_t)((uint64_t)(part + 1) * units / split);
+  return u0 < u1;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1
+template <int L, bool CMS, bool HLL>
+__global__ __launch_bounds__(kBlock) void k_pass1(Pass1Args a) {
+  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t r;
+  uint32_t w0, w1;
+  const uint32_t k = a.k;
+  if (!read_share(a.reads, gid, a.split, k, 0, r, w0, w1)) return;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r] + w0;
+  Kmer<L> win = window_bits<L>(data, g, k);
+  Kmer<L> fwd = fwd_from_window<L>(win, k);
+  Kmer<L> rev = rc_from_window<L>(win, k);
+  BaseStream in;
+  in.init(data, g + k);
+  NtHash nh;
+  BaseStream outs;
+  if (HLL) {
+    nh.f = 0;
+    nh.r = 0;
+    BaseStream s0;
+    s0.init(data, g);
+    for (uint32_t j = 0; j < k; ++j) nh.push(s0.next(), k);
+    outs.init(data, g);
+  }
+  const uint32_t lw = a.cms.log2w;
+  const uint64_t wmask = (1ull << lw) - 1ull;
+  uint32_t* __restrict__ counts = a.cms.counts;
+  for (uint32_t w = w0;;) {
+    if (CMS) {
+      const uint64_t key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
+#pragma unroll
+      for (int row = 0; row < 4; ++row) {
+        uint64_t col = splitmix64(key ^ cms_seed(row)) & wmask;
+        atomicAdd(counts + (((uint64_t)row << lw) | col), a.weight);
+      }
+    }
+    if (HLL) {
+      uint32_t idx, rho;
+      hll_index_rho(splitmix64(nh.canonical()), a.hll.p, idx, rho);
+      if (rho > a.hll.regs[idx]) atomicMax(a.hll.regs + idx, rho);
+    }
+    if (++w >= w1) break;
+    const uint32_t b = in.next();
+    if (CMS) kmer_roll<L>(fwd, rev, b, k);
+    if (HLL) nh.roll(outs.next(), b, k);
+  }
+}
+
+hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
+  if (a.reads.n_reads == 0) return hipSuccess;
+  const uint32_t L = (2 * a.k + 63) / 64;
+  unsigned grid = grid_for(a.reads.n_reads * (uint64_t)a.split);
+  const bool h = a.hll.regs != nullptr;
+  const bool c = a.cms.counts != nullptr;
+  if (!h && !c) return hipSuccess;
+#define P1(LL)                                                                          \
+  if (h && c) hipLaunchKernelGGL((k_pass1<LL, true, true>), dim3(grid), dim3(kBlock), 0, s, a); \
+  else if (c) hipLaunchKernelGGL((k_pass1<LL, true, false>), dim3(grid), dim3(kBlock), 0, s, a); \
+  else hipLaunchKernelGGL((k_pass1<LL, false, true>), dim3(grid), dim3(kBlock), 0, s, a)
+  switch (L) {
+    case 1: P1(1); break;
+    case 2: P1(2); break;
+    case 3: P1(3); break;
+    default: P1(4); break;
+  }
+#undef P1
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Threshold bitmap and reference-layout packing
+
+__global__ __launch_bounds__(kBlock) void k_solid_bits(const uint32_t* __restrict__ counts, uint64_t n,
+                                                       uint32_t t, uint32_t cap, uint32_t* __restrict__ bits) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool on = false;
+  if (i < n) {
+    uint32_t c = counts[i];
+    c = c < cap ? c : cap;
+    on = c >= t;
+  }
+  uint64_t m = __ballot(on);
+  uint32_t lane = threadIdx.x & 63u;
+  uint64_t wbase = (i - lane) >> 5;  // first u32 word of this wave
+  uint64_t nwords = (n + 31) >> 5;
+  if (lane == 0 && wbase < nwords) bits[wbase] = (uint32_t)m;
+  if (lane == 0 && wbase + 1 < nwords) bits[wbase + 1] = (uint32_t)(m >> 32);
+}
+
+hipError_t launch_solid_bits(const uint32_t* counts, uint64_t n, uint32_t t, uint32_t cap, uint32_t* bits,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_solid_bits, dim3(grid_for(n)), dim3(kBlock), 0, s, counts, n, t, cap, bits);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_nibble(const uint32_t* __restrict__ counts, uint64_t nbytes,
+                                                        uint8_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbytes) return;
+  uint32_t lo = counts[2 * i], hi = counts[2 * i + 1];
+  lo = lo < 15u ? lo : 15u;
+  hi = hi < 15u ? hi : 15u;
+  out[i] = (uint8_t)(lo | (hi << 4));
+}
+hipError_t launch_pack_nibble(const uint32_t* counts, uint64_t n, uint8_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_nibble, dim3(grid_for(n / 2)), dim3(kBlock), 0, s, counts, n / 2, out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_u16(const uint32_t* __restrict__ counts, uint64_t n,
+                                                     uint16_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t c = counts[i];
+  out[i] = (uint16_t)(c < 65535u ? c : 65535u);
+}
+hipError_t launch_pack_u16(const uint32_t* counts, uint64_t n, uint16_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_u16, dim3(grid_for(n)), dim3(kBlock), 0, s, counts, n, out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_u8(const uint32_t* __restrict__ regs, uint64_t n,
+                                                    uint8_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint8_t)regs[i];
+}
+hipError_t launch_pack_u8(const uint32_t* regs, uint64_t n, uint8_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_u8, dim3(grid_for(n)), dim3(kBlock), 0, s, regs, n, out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g == 0) g = 1;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)g), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_table_clear(uint64_t* __restrict__ slots, uint64_t cap,
+                                                        uint32_t stride, uint32_t kw) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t n = cap * stride;
+  uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += step) {
+    uint32_t w = (uint32_t)(i % stride);
+    slots[i] = w < kw ? kEmpty : 0ull;
+  }
+}
+hipError_t launch_table_clear(const DevSolidTable& t, hipStream_t s) {
+  uint64_t g = (t.cap * t.stride + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_table_clear, dim3((unsigned)g), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, t.kw);
+  if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+  return launch_fill_u32(t.status, 2, 0, s);
+}
+
+// ---------------------------------------------------------------------------
+// Solid table
+
+// Packs the 2k-bit k-mer integer into KW words of 62 bits (bits 62j..62j+61
+// in word j).  Two spare bits per word keep the all-ones empty marker out of
+// reach of any key.
+template <int L, int KW>
+__device__ __forceinline__ void pack62(const Kmer<L>& x, uint64_t (&w)[KW]) {
+#pragma unroll
+  for (int j = 0; j < KW; ++j) {
+    const int bit = 62 * j;
+    const int limb = bit / 64, off = bit % 64;
+    uint64_t v = limb < L ? (x.a[limb] >> off) : 0ull;
+    if (off > 2 && limb + 1 < L) v |= x.a[limb + 1] << (64 - off);
+    w[j] = v & 0x3fffffffffffffffull;
+  }
+}
+
+// Finds or claims the slot of `key`.  Key words are write-once (empty ->
+// value), claimed word by word with compare-and-swap in order; a slot whose
+// prefix matches but whose next word holds another value belongs to another
+// key.  A plain load that returns a stale empty word only sends the thread to
+// the compare-and-swap, which returns the stored word.
+template <int KW>
+__device__ __forceinline__ uint32_t* table_find(const DevSolidTable& t, const uint64_t (&key)[KW], uint64_t h) {
+  const uint64_t cmask = t.cap - 1ull;
+  uint64_t slot = h & cmask;
+  for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
+    uint64_t* s = t.slots + slot * t.stride;
+    bool match = true;
+#pragma unroll
+    for (int j = 0; j < KW; ++j) {
+      if (!match) break;
+      uint64_t v = s[j];
+      if (v == kEmpty) {
+        uint64_t old = atomicCAS((unsigned long long*)(s + j), (unsigned long long)kEmpty,
+                                 (unsigned long long)key[j]);
+        if (old == kEmpty) {
+          v = key[j];
+          if (j == 0) atomicAdd(t.status + 1, 1u);
+        } else {
+          v = old;
+        }
+      }
+      match = (v == key[j]);
+    }
+    if (match) return reinterpret_cast<uint32_t*>(s + KW);
+    slot = (slot + 1ull) & cmask;
+  }
+  atomicOr(t.status, 1u);
+  return nullptr;
+}
+
+__device__ __forceinline__ bool solid_lookup(const uint32_t* __restrict__ bits, uint32_t lw, uint64_t key) {
+  const uint64_t wmask = (1ull << lw) - 1ull;
+  bool ok = true;
+#pragma unroll
+  for (int row = 0; row < 4; ++row) {
+    uint64_t i = ((uint64_t)row << lw) | (splitmix64(key ^ cms_seed(row)) & wmask);
+    ok &= (bits[i >> 5] >> (i & 31u)) & 1u;
+  }
+  return ok;
+}
+
+__device__ __forceinline__ uint64_t slot_hash(uint64_t fold) { return splitmix64(fold ^ 0x9e3779b97f4a7c15ull); }
+
+// One record batch for window x: cov += n*cov_w; edge counters below the edge
+// gate get +edge_w (contig weights are always applied).
+__device__ __forceinline__ void add_records(uint32_t* c, uint32_t nrec, bool left, uint32_t e_left, bool right,
+                                            uint32_t e_right, uint32_t cov_w, uint32_t edge_w, uint32_t gate,
+                                            bool always) {
+  atomicAdd(c, nrec * cov_w);
+  if (left && (always || c[2 + e_left] < gate)) atomicAdd(c + 2 + e_left, edge_w);
+  if (right && (always || c[2 + e_right] < gate)) atomicAdd(c + 2 + e_right, edge_w);
+}
+
+template <int L, int KW>
+__global__ __launch_bounds__(kBlock) void k_pass2(Pass2Args a) {
+  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t r;
+  uint32_t p0, p1;  // pairs [p0, p1): pair x joins windows x and x+1
+  const uint32_t k = a.k;
+  if (!read_share(a.reads, gid, a.split, k, 1, r, p0, p1)) return;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r] + p0;
+  const bool contig = a.contig_mode != 0;
+  const uint32_t cov_w = contig ? 100u : 1u, edge_w = contig ? 1000u : 1u;
+  Kmer<L> win = window_bits<L>(data, g, k);
+  Kmer<L> fwd = fwd_from_window<L>(win, k);
+  Kmer<L> rev = rc_from_window<L>(win, k);
+  BaseStream in, first;
+  in.init(data, g + k);
+  first.init(data, g);
+  bool cur_rc = kmer_lt<L>(rev, fwd);
+  Kmer<L> cur = cur_rc ? rev : fwd;
+  uint64_t cur_fold = kmer_fold<L>(cur);
+  bool cur_solid = contig || solid_lookup(a.solid_bits, a.log2w, cur_fold);
+  bool left = false;       // pair (x-1, x) is solid
+  uint32_t left_rcp = 0;   // complement of the first base of window x-1
+  for (uint32_t x = p0;; ++x) {
+    bool right = false;
+    uint32_t b = 0;
+    bool nxt_rc = false, nxt_solid = false;
+    Kmer<L> nxt;
+    uint64_t nxt_fold = 0;
+    if (x < p1) {
+      b = in.next();
+      kmer_roll<L>(fwd, rev, b, k);
+      nxt_rc = kmer_lt<L>(rev, fwd);
+      nxt = nxt_rc ? rev : fwd;
+      nxt_fold = kmer_fold<L>(nxt);
+      nxt_solid = contig || solid_lookup(a.solid_bits, a.log2w, nxt_fold);
+      right = cur_solid && nxt_solid;
+    }
+    const uint32_t first_base = first.next();
+    const uint32_t nrec = (uint32_t)left + (uint32_t)right;
+    if (nrec) {
+      uint64_t kwds[KW];
+      pack62<L, KW>(cur, kwds);
+      uint32_t* c = table_find<KW>(a.table, kwds, slot_hash(cur_fold));
+      if (c) {
+        // edge_v of pair (x-1, x) and edge_u of pair (x, x+1), build.rs:268-275
+        const uint32_t e_left = cur_rc ? left_rcp : left_rcp + 4u;
+        const uint32_t e_right = cur_rc ? b + 4u : b;
+        add_records(c, nrec, left, e_left, right, e_right, cov_w, edge_w, a.min_trans_cov, contig);
+      }
+    }
+    if (x >= p1) break;
+    left = right;
+    left_rcp = 3u - first_base;
+    cur = nxt;
+    cur_rc = nxt_rc;
+    cur_fold = nxt_fold;
+    cur_solid = nxt_solid;
+  }
+}
+
+static uint32_t key_words(uint32_t k) { return (k + 30) / 31; }
+
+hipError_t launch_pass2(const Pass2Args& a, hipStream_t s) {
+  if (a.reads.n_reads == 0) return hipSuccess;
+  const uint32_t L = (2 * a.k + 63) / 64;
+  const uint32_t KW = key_words(a.k);
+  if (KW != a.table.kw) return hipErrorInvalidValue;
+  unsigned grid = grid_for(a.reads.n_reads * (uint64_t)a.split);
+#define P2(LL, KK) hipLaunchKernelGGL((k_pass2<LL, KK>), dim3(grid), dim3(kBlock), 0, s, a)
+  if (L == 1 && KW == 1) P2(1, 1);
+  else if (L == 1) P2(1, 2);
+  else if (L == 2 && KW == 2) P2(2, 2);
+  else if (L == 2) P2(2, 3);
+  else if (L == 3 && KW == 3) P2(3, 3);
+  else if (L == 3) P2(3, 4);
+  else if (KW == 4) P2(4, 4);
+  else P2(4, 5);
+#undef P2
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3
+
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_pass3(const uint64_t* __restrict__ slots, uint64_t cap, uint32_t stride,
+                                                  uint32_t k, uint32_t nref, uint32_t gate,
+                                                  uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
+                                                  uint8_t* __restrict__ out_mask, uint32_t* __restrict__ count,
+                                                  unsigned long long* __restrict__ rec_sum) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  uint32_t mask = 0, cov = 0;
+  unsigned long long raw_cov = 0;
+  uint64_t wk[KW];
+  if (i < cap) {
+    const uint64_t* s = slots + i * stride;
+#pragma unroll
+    for (int j = 0; j < KW; ++j) wk[j] = s[j];
+    if (wk[0] != kEmpty) {
+      const uint32_t* c = reinterpret_cast<const uint32_t*>(s + KW);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        uint32_t v = c[2 + e];
+        v = v < 65535u ? v : 65535u;
+        if (v >= gate) mask |= 1u << e;
+      }
+      raw_cov = c[0];
+      uint32_t cv = c[0] < 65535u ? c[0] : 65535u;
+      cov = cv / 2u;
+      if (cov < 1u) cov = 1u;
+      keep = mask != 0u;
+    }
+  }
+  // records added to the table (sum of raw cov), one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) raw_cov += __shfl_xor(raw_cov, off);
+  uint32_t lane = threadIdx.x & 63u;
+  if (lane == 0 && raw_cov) atomicAdd(rec_sum, raw_cov);
+  // wave-aggregated output slot
+  uint64_t ballot = __ballot(keep);
+  uint32_t leader = ballot ? (uint32_t)__builtin_ctzll(ballot) : 0u;
+  uint32_t base = 0;
+  if (ballot && lane == leader) base = atomicAdd(count, (uint32_t)__builtin_popcountll(ballot));
+  base = __shfl(base, (int)leader);
+  if (!keep) return;
+  uint32_t pos = base + (uint32_t)__builtin_popcountll(ballot & ((1ull << lane) - 1ull));
+  // 62-bit words -> 64-bit limbs (least significant first)
+  uint64_t limb[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < KW; ++j) {
+    const int bit = 62 * j, li = bit / 64, off = bit % 64;
+    if (li < 4) limb[li] |= wk[j] << off;
+    if (off > 2 && li + 1 < 4) limb[li + 1] |= wk[j] >> (64 - off);
+  }
+  // reference words, most significant first
+  for (uint32_t w = 0; w < nref; ++w) out_keys[(uint64_t)pos * nref + w] = limb[nref - 1 - w];
+  out_cov[pos] = (uint16_t)cov;
+  out_mask[pos] = (uint8_t)mask;
+  (void)k;
+}
+
+hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t nref, uint32_t gate, uint64_t* out_keys,
+                        uint16_t* out_cov, uint8_t* out_mask, uint32_t* out_count, unsigned long long* rec_sum,
+                        hipStream_t s) {
+  unsigned grid = grid_for(t.cap);
+#define P3(KK)                                                                                        \
+  hipLaunchKernelGGL((k_pass3<KK>), dim3(grid), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, k, nref, gate, \
+                     out_keys, out_cov, out_mask, out_count, rec_sum)
+  switch (t.kw) {
+    case 1: P3(1); break;
+    case 2: P3(2); break;
+    case 3: P3(3); break;
+    case 4: P3(4); break;
+    default: P3(5); break;
+  }
+#undef P3
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Sorting the solid set by k-mer: stable LSD passes over the significant
+// 64-bit limbs with a permutation, then one gather.
+
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+__global__ __launch_bounds__(kBlock) void k_gather_limb(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                                                        uint32_t n, uint32_t nref, uint32_t word,
+                                                        uint64_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = keys[(uint64_t)perm[i] * nref + word];
+}
+__global__ __launch_bounds__(kBlock) void k_gather_records(const uint64_t* __restrict__ keys,
+                                                           const uint16_t* __restrict__ cov,
+                                                           const uint8_t* __restrict__ mask,
+                                                           const uint32_t* __restrict__ perm, uint32_t n,
+                                                           uint32_t nref, uint64_t* __restrict__ okeys,
+                                                           uint16_t* __restrict__ ocov, uint8_t* __restrict__ omask) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t j = perm[i];
+  for (uint32_t w = 0; w < nref; ++w) okeys[(uint64_t)i * nref + w] = keys[(uint64_t)j * nref + w];
+  ocov[i] = cov[j];
+  omask[i] = mask[j];
+}
+
+// Scratch layout: [limb keys x2][perm x2][record copy][cub temp].
+hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t nref, uint32_t k, void* tmp,
+                      size_t* tmp_bytes, hipStream_t s) {
+  size_t cub_bytes = 0;
+  hipcub::DoubleBuffer<uint64_t> dk(nullptr, nullptr);
+  hipcub::DoubleBuffer<uint32_t> dv(nullptr, nullptr);
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, dk, dv, (int)(n ? n : 1), 0, 64, s);
+  if (e != hipSuccess) return e;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t nn = n ? n : 1;
+  const size_t need = al(nn * 8) * 2 + al(nn * 4) * 2 + al(nn * nref * 8) + al(nn * 2) + al(nn) + al(cub_bytes);
+  if (keys == nullptr) {
+    *tmp_bytes = need;
+    return hipSuccess;
+  }
+  if (*tmp_bytes < need) return hipErrorInvalidValue;
+  if (n <= 1) return hipSuccess;
+  char* p = (char*)tmp;
+  uint64_t* k0 = (uint64_t*)p; p += al(nn * 8);
+  uint64_t* k1 = (uint64_t*)p; p += al(nn * 8);
+  uint32_t* v0 = (uint32_t*)p; p += al(nn * 4);
+  uint32_t* v1 = (uint32_t*)p; p += al(nn * 4);
+  uint64_t* ck = (uint64_t*)p; p += al(nn * nref * 8);
+  uint16_t* cc = (uint16_t*)p; p += al(nn * 2);
+  uint8_t* cm = (uint8_t*)p; p += al(nn);
+  void* ctmp = p;
+  const unsigned grid = grid_for(n);
+  hipLaunchKernelGGL(k_iota, dim3(grid), dim3(kBlock), 0, s, v0, n);
+  // Significant limbs: L = ceil(2k/64); the top limb has 2k - 64(L-1) bits.
+  const uint32_t L = (2 * k + 63) / 64;
+  uint32_t* perm = v0;
+  uint32_t* perm_alt = v1;
+  for (uint32_t limb = 0; limb < L; ++limb) {
+    const uint32_t word = nref - 1 - limb;  // reference words are most significant first
+    hipLaunchKernelGGL(k_gather_limb, dim3(grid), dim3(kBlock), 0, s, keys, perm, n, nref, word, k0);
+    const int end_bit = (limb == L - 1) ? (int)(2 * k - 64 * (L - 1)) : 64;
+    hipcub::DoubleBuffer<uint64_t> bk(k0, k1);
+    hipcub::DoubleBuffer<uint32_t> bv(perm, perm_alt);
+    size_t cb = cub_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(ctmp, cb, bk, bv, (int)n, 0, end_bit, s);
+    if (e != hipSuccess) return e;
+    if (bv.Current() != perm) {
+      perm_alt = perm;
+      perm = bv.Current();
+    }
+  }
+  hipMemcpyAsync(ck, keys, (size_t)n * nref * 8, hipMemcpyDeviceToDevice, s);
+  hipMemcpyAsync(cc, cov, (size_t)n * 2, hipMemcpyDeviceToDevice, s);
+  hipMemcpyAsync(cm, mask, (size_t)n, hipMemcpyDeviceToDevice, s);
+  hipLaunchKernelGGL(k_gather_records, dim3(grid), dim3(kBlock), 0, s, ck, cc, cm, perm, n, nref, keys, cov, mask);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Explicit-key sketch operations (CountMinSketch::add / estimate, HyperLogLog::add_hash)
+
+__global__ __launch_bounds__(kBlock) void k_cms_add_keys(DevCms c, const uint64_t* __restrict__ keys, uint64_t n,
+                                                         uint32_t weight) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = keys[i];
+  const uint64_t wmask = (1ull << c.log2w) - 1ull;
+#pragma unroll
+  for (int row = 0; row < 4; ++row)
+    atomicAdd(c.counts + (((uint64_t)row << c.log2w) | (splitmix64(key ^ cms_seed(row)) & wmask)), weight);
+}
+hipError_t launch_cms_add_keys(DevCms c, const uint64_t* keys, uint64_t n, uint32_t weight, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_cms_add_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, c, keys, n, weight);
+  return hipGetLastError();
+}
+
+// sketch.rs:227-246: min over rows of the saturating counters.
+__global__ __launch_bounds__(kBlock) void k_cms_query_keys(DevCms c, const uint64_t* __restrict__ keys, uint64_t n,
+                                                           uint32_t cap, uint32_t* __restrict__ out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = keys[i];
+  const uint64_t wmask = (1ull << c.log2w) - 1ull;
+  uint32_t mn = cap;
+#pragma unroll
+  for (int row = 0; row < 4; ++row) {
+    uint32_t v = c.counts[((uint64_t)row << c.log2w) | (splitmix64(key ^ cms_seed(row)) & wmask)];
+    mn = v < mn ? v : mn;
+  }
+  out[i] = mn;
+}
+hipError_t launch_cms_query_keys(DevCms c, const uint64_t* keys, uint64_t n, uint32_t cap, uint32_t* out,
+                                 hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_cms_query_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, c, keys, n, cap, out);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_hll_add_hashes(DevHll h, const uint64_t* __restrict__ hashes, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t idx, rho;
+  hll_index_rho(hashes[i], h.p, idx, rho);
+  if (rho > h.regs[idx]) atomicMax(h.regs + idx, rho);
+}
+hipError_t launch_hll_add_hashes(DevHll h, const uint64_t* hashes, uint64_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_hll_add_hashes, dim3(grid_for(n)), dim3(kBlock), 0, s, h, hashes, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Estimator.  Occurrence t of a k-mer sets cms "c >= 2" at insertion time iff
+// every row's cell already had an insertion before t, i.e. max over rows of
+// the cell's first insertion index is < t.  First insertion indices are an
+// atomicMin per cell; HLL registers are recovered at any pair prefix from the
+// first pair index at which each (register, rank) appeared.
+
+constexpr uint32_t kEstK = 21;
+constexpr uint32_t kEstLog2W = 23;
+
+template <bool SOLID>
+__global__ __launch_bounds__(kBlock) void k_est(EstArgs a) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n_reads) return;
+  const uint32_t len = a.reads.lens[r];
+  if (len < kEstK) return;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r];
+  const uint32_t pair = r >> 1;
+  uint32_t t = a.occ_base[r];
+  NtHash nh;
+  nh.f = 0;
+  nh.r = 0;
+  BaseStream in, outs;
+  in.init(data, g);
+  for (uint32_t j = 0; j < kEstK; ++j) nh.push(in.next(), kEstK);
+  outs.init(data, g);
+  const uint64_t wmask = (1ull << kEstLog2W) - 1ull;
+  const uint32_t nwin = len - kEstK + 1;
+  for (uint32_t w = 0;; ++w, ++t) {
+    const uint64_t raw = nh.canonical();
+    uint32_t idx, rho;
+    hll_index_rho(splitmix64(raw), 14, idx, rho);
+    if (!SOLID) {
+#pragma unroll
+      for (int row = 0; row < 4; ++row)
+        atomicMin(a.first_ins + (((uint64_t)row << kEstLog2W) | (splitmix64(raw ^ cms_seed(row)) & wmask)), t);
+      uint32_t* f = a.first_total + idx * 64u + rho;
+      if (pair < *f) atomicMin(f, pair);
+    } else {
+      uint32_t mx = 0;
+#pragma unroll
+      for (int row = 0; row < 4; ++row) {
+        uint32_t v = a.first_ins[((uint64_t)row << kEstLog2W) | (splitmix64(raw ^ cms_seed(row)) & wmask)];
+        mx = v > mx ? v : mx;
+      }
+      if (mx < t) {
+        uint32_t* f = a.first_solid + idx * 64u + rho;
+        if (pair < *f) atomicMin(f, pair);
+      }
+    }
+    if (w + 1 >= nwin) break;
+    nh.roll(outs.next(), in.next(), kEstK);
+  }
+}
+
+hipError_t launch_est_first_insert(const EstArgs& a, hipStream_t s) {
+  if (!a.n_reads) return hipSuccess;
+  hipLaunchKernelGGL((k_est<false>), dim3(grid_for(a.n_reads)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_est_solid(const EstArgs& a, hipStream_t s) {
+  if (!a.n_reads) return hipSuccess;
+  hipLaunchKernelGGL((k_est<true>), dim3(grid_for(a.n_reads)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace fb

@@ -1,0 +1,867 @@
+// fb_runtime.hip -- context, device memory, objects and the C ABI of
+// include/fastbact_gpu.h.  One HIP stream per context; device buffers come
+// from a per-context pool so repeated passes reuse memory instead of calling
+// hipMalloc/hipFree.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fastbact_gpu.h"
+#include "fb_host.hpp"
+#include "fb_internal.hpp"
+
+using namespace fb;
+
+// ---------------------------------------------------------------------------
+// Context and pool
+
+struct PoolBuf {
+  void* p;
+  size_t bytes;
+};
+
+struct fb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;  // 0 = unlimited
+  uint64_t held = 0;    // bytes allocated from HIP (in use + pooled)
+  std::string err;
+  std::vector<PoolBuf> pool;  // free buffers
+  hipEvent_t ev[8] = {};
+};
+
+static int set_err(fb_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                              \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return set_err((ctx), FB_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                         __FILE__, __LINE__);                                           \
+  } while (0)
+
+static int ctx_current(fb_ctx* c) {
+  HIP_TRY(c, hipSetDevice(c->device));
+  return FB_OK;
+}
+
+// Takes a pooled buffer of at least `bytes` (and at most 2x) or allocates.
+static int pool_get(fb_ctx* c, size_t bytes, void** out) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes == 0) bytes = 256;
+  size_t best = (size_t)-1;
+  for (size_t i = 0; i < c->pool.size(); ++i) {
+    const PoolBuf& b = c->pool[i];
+    if (b.bytes >= bytes && b.bytes <= 2 * bytes && (best == (size_t)-1 || b.bytes < c->pool[best].bytes)) best = i;
+  }
+  if (best != (size_t)-1) {
+    *out = c->pool[best].p;
+    c->pool.erase(c->pool.begin() + best);
+    return FB_OK;
+  }
+  // free pooled buffers first if the budget would be exceeded
+  while (c->budget && c->held + bytes > c->budget && !c->pool.empty()) {
+    (void)hipFree(c->pool.back().p);
+    c->held -= c->pool.back().bytes;
+    c->pool.pop_back();
+  }
+  if (c->budget && c->held + bytes > c->budget)
+    return set_err(c, FB_ERR_OOM, "HBM budget exceeded: need %zu bytes, %llu of %llu held", bytes,
+                   (unsigned long long)c->held, (unsigned long long)c->budget);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // release the pool and retry once
+    for (auto& b : c->pool) {
+      (void)hipFree(b.p);
+      c->held -= b.bytes;
+    }
+    c->pool.clear();
+    (void)hipGetLastError();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return set_err(c, FB_ERR_OOM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+  }
+  c->held += bytes;
+  *out = p;
+  return FB_OK;
+}
+static void pool_put(fb_ctx* c, void* p, size_t bytes) {
+  if (!p) return;
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (bytes == 0) bytes = 256;
+  c->pool.push_back({p, bytes});
+}
+
+// RAII device buffer from the pool.
+struct DBuf {
+  fb_ctx* c = nullptr;
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  int get(fb_ctx* ctx, size_t n) {
+    release();
+    c = ctx;
+    bytes = n;
+    return pool_get(ctx, n, &p);
+  }
+  void release() {
+    if (p && c) pool_put(c, p, bytes);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+extern "C" int fb_ctx_create(int device, uint64_t budget, fb_ctx** out) {
+  if (!out) return FB_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return FB_ERR_NODEV;
+  }
+  if (device < 0 || device >= n) return FB_ERR_INVALID;
+  fb_ctx* c = new fb_ctx();
+  c->device = device;
+  c->budget = budget;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FB_ERR_HIP;
+  }
+  for (auto& e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return FB_ERR_HIP;
+    }
+  }
+  *out = c;
+  return FB_OK;
+}
+
+extern "C" void fb_ctx_destroy(fb_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& b : c->pool) (void)hipFree(b.p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* fb_last_error(const fb_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int fb_ctx_synchronize(fb_ctx* c) {
+  if (!c) return FB_ERR_INVALID;
+  if (int r = ctx_current(c)) return r;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+
+extern "C" uint64_t fb_ctx_device_bytes(const fb_ctx* c) { return c ? c->held : 0; }
+
+// ---------------------------------------------------------------------------
+// ReadBank
+
+struct fb_bank {
+  fb_ctx* ctx = nullptr;
+  DBuf data, offs, lens;
+  uint64_t n_reads = 0;
+  uint64_t total_bases = 0;
+  std::vector<uint64_t> len_hist;  // bases summed per read length (index = length), last bin open
+  uint32_t max_len = 0;
+  DevReads view() const {
+    DevReads r;
+    r.data = data.as<uint64_t>();
+    r.offs = offs.as<uint64_t>();
+    r.lens = lens.as<uint32_t>();
+    r.n_reads = n_reads;
+    return r;
+  }
+};
+
+// Concatenates arenas word-aligned into one device read list (offsets become
+// global base indices) with kArenaPadWords zero words after the data.
+static int upload_reads(fb_ctx* c, const fb_arena_view* arenas, int n_arenas, DBuf& data, DBuf& offs, DBuf& lens,
+                        uint64_t* n_reads_out, std::vector<uint64_t>* hist, uint32_t* max_len, uint64_t* bases) {
+  uint64_t words = 0, nr = 0;
+  for (int a = 0; a < n_arenas; ++a) {
+    const fb_arena_view& v = arenas[a];
+    if (v.n_reads && (!v.offsets || !v.lengths || !v.data))
+      return set_err(c, FB_ERR_INVALID, "arena %d: null pointer with %llu reads", a, (unsigned long long)v.n_reads);
+    // every read must lie inside data
+    for (uint64_t i = 0; i < v.n_reads; ++i) {
+      uint64_t end = v.offsets[i] + v.lengths[i];
+      if (v.lengths[i] && (end + 31) / 32 > v.n_words)
+        return set_err(c, FB_ERR_INVALID, "arena %d read %llu ends past data (%llu words)", a, (unsigned long long)i,
+                       (unsigned long long)v.n_words);
+    }
+    words += v.n_words;
+    nr += v.n_reads;
+  }
+  if (int r = data.get(c, (words + kArenaPadWords) * 8)) return r;
+  if (int r = offs.get(c, (nr ? nr : 1) * 8)) return r;
+  if (int r = lens.get(c, (nr ? nr : 1) * 4)) return r;
+  std::vector<uint64_t> h_offs(nr);
+  std::vector<uint32_t> h_lens(nr);
+  uint64_t wpos = 0, rpos = 0, tot = 0;
+  uint32_t mx = 0;
+  for (int a = 0; a < n_arenas; ++a) {
+    const fb_arena_view& v = arenas[a];
+    if (v.n_words) HIP_TRY(c, hipMemcpyAsync(data.as<uint64_t>() + wpos, v.data, v.n_words * 8, hipMemcpyHostToDevice, c->stream));
+    for (uint64_t i = 0; i < v.n_reads; ++i) {
+      h_offs[rpos] = v.offsets[i] + wpos * 32;
+      h_lens[rpos] = v.lengths[i];
+      tot += v.lengths[i];
+      mx = std::max(mx, v.lengths[i]);
+      ++rpos;
+    }
+    wpos += v.n_words;
+  }
+  HIP_TRY(c, hipMemsetAsync(data.as<uint64_t>() + wpos, 0, kArenaPadWords * 8, c->stream));
+  if (nr) {
+    HIP_TRY(c, hipMemcpyAsync(offs.p, h_offs.data(), nr * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(lens.p, h_lens.data(), nr * 4, hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (hist) {
+    hist->assign((size_t)mx + 1, 0);
+    for (uint64_t i = 0; i < nr; ++i) (*hist)[h_lens[i]] += h_lens[i];
+  }
+  if (max_len) *max_len = mx;
+  if (bases) *bases = tot;
+  *n_reads_out = nr;
+  return FB_OK;
+}
+
+extern "C" int fb_bank_upload(fb_ctx* c, const fb_arena_view arenas[3], fb_bank** out) {
+  if (!c || !arenas || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (int r = ctx_current(c)) return r;
+  fb_bank* b = new fb_bank();
+  b->ctx = c;
+  int r = upload_reads(c, arenas, 3, b->data, b->offs, b->lens, &b->n_reads, &b->len_hist, &b->max_len, &b->total_bases);
+  if (r) {
+    delete b;
+    return r;
+  }
+  *out = b;
+  return FB_OK;
+}
+
+extern "C" void fb_bank_destroy(fb_bank* b) { delete b; }
+extern "C" uint64_t fb_bank_total_bases(const fb_bank* b) { return b ? b->total_bases : 0; }
+extern "C" uint64_t fb_bank_total_reads(const fb_bank* b) { return b ? b->n_reads : 0; }
+// Windows of a k pass: sum over reads with len >= k of len - k + 1.
+static uint64_t bank_windows(const fb_bank* b, uint32_t k) {
+  uint64_t w = 0;
+  for (size_t len = k; len < b->len_hist.size(); ++len)
+    if (b->len_hist[len]) w += (b->len_hist[len] / len) * (len - k + 1);
+  return w;
+}
+
+extern "C" uint64_t fb_bank_bases_for_k(const fb_bank* b, uint32_t k) {
+  if (!b) return 0;
+  uint64_t s = 0;
+  for (size_t len = k; len < b->len_hist.size(); ++len) s += b->len_hist[len];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// CountMinSketch
+
+struct fb_cms {
+  fb_ctx* ctx = nullptr;
+  DBuf counts;
+  uint64_t width = 0;
+  uint32_t log2w = 0;
+  int bits = 4;
+  DevCms dev() const {
+    DevCms d;
+    d.counts = counts.as<uint32_t>();
+    d.log2w = log2w;
+    return d;
+  }
+  uint32_t cap() const { return bits == 4 ? 15u : 65535u; }
+};
+
+static uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+static int cms_init(fb_ctx* c, fb_cms* m, uint64_t width, int bits) {
+  if (bits != 4 && bits != 16) return set_err(c, FB_ERR_INVALID, "counter_bits must be 4 or 16 (got %d)", bits);
+  if (width == 0 || width > (1ull << 32)) return set_err(c, FB_ERR_INVALID, "CMS width %llu out of range", (unsigned long long)width);
+  m->ctx = c;
+  m->width = next_pow2(width);
+  m->log2w = (uint32_t)__builtin_ctzll(m->width);
+  m->bits = bits;
+  if (int r = m->counts.get(c, m->width * 4 * 4)) return r;
+  HIP_TRY(c, launch_fill_u32(m->counts.as<uint32_t>(), m->width * 4, 0, c->stream));
+  return FB_OK;
+}
+
+extern "C" int fb_cms_create(fb_ctx* c, uint64_t width, int bits, fb_cms** out) {
+  if (!c || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (int r = ctx_current(c)) return r;
+  fb_cms* m = new fb_cms();
+  if (int r = cms_init(c, m, width, bits)) {
+    delete m;
+    return r;
+  }
+  *out = m;
+  return FB_OK;
+}
+extern "C" void fb_cms_destroy(fb_cms* m) { delete m; }
+extern "C" uint64_t fb_cms_width(const fb_cms* m) { return m ? m->width : 0; }
+extern "C" int fb_cms_clear(fb_cms* m) {
+  if (!m) return FB_ERR_INVALID;
+  if (int r = ctx_current(m->ctx)) return r;
+  HIP_TRY(m->ctx, launch_fill_u32(m->counts.as<uint32_t>(), m->width * 4, 0, m->ctx->stream));
+  return FB_OK;
+}
+
+// Copies a host array to a pooled device buffer.
+static int to_device(fb_ctx* c, DBuf& d, const void* src, size_t bytes) {
+  if (int r = d.get(c, bytes ? bytes : 8)) return r;
+  if (bytes) HIP_TRY(c, hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return FB_OK;
+}
+
+// Total adds must stay below 2^32 so the exact u32 cells cannot wrap.
+static int check_add_budget(fb_ctx* c, uint64_t adds) {
+  if (adds >= 0xFFFFFFFFull)
+    return set_err(c, FB_ERR_INVALID, "%llu sketch additions exceed the 2^32-1 exact-counter limit of one pass",
+                   (unsigned long long)adds);
+  return FB_OK;
+}
+
+extern "C" int fb_cms_add_keys(fb_cms* m, const uint64_t* keys, uint64_t n) {
+  if (!m || (n && !keys)) return FB_ERR_INVALID;
+  fb_ctx* c = m->ctx;
+  if (int r = ctx_current(c)) return r;
+  if (int r = check_add_budget(c, n)) return r;
+  DBuf d;
+  if (int r = to_device(c, d, keys, n * 8)) return r;
+  HIP_TRY(c, launch_cms_add_keys(m->dev(), d.as<uint64_t>(), n, 1, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+
+extern "C" int fb_cms_query(fb_cms* m, const uint64_t* keys, uint64_t n, uint32_t* out) {
+  if (!m || (n && (!keys || !out))) return FB_ERR_INVALID;
+  fb_ctx* c = m->ctx;
+  if (int r = ctx_current(c)) return r;
+  DBuf d, o;
+  if (int r = to_device(c, d, keys, n * 8)) return r;
+  if (int r = o.get(c, (n ? n : 1) * 4)) return r;
+  HIP_TRY(c, launch_cms_query_keys(m->dev(), d.as<uint64_t>(), n, m->cap(), o.as<uint32_t>(), c->stream));
+  if (n) HIP_TRY(c, hipMemcpyAsync(out, o.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+
+static uint32_t default_split(uint32_t k, uint32_t max_len) {
+  // about 48 windows per thread
+  uint32_t win = max_len >= k ? max_len - k + 1 : 1;
+  uint32_t s = (win + 47) / 48;
+  return std::max(1u, std::min(s, 8u));
+}
+
+extern "C" int fb_cms_add_bank(fb_cms* m, const fb_bank* b, uint32_t k) {
+  if (!m || !b) return FB_ERR_INVALID;
+  fb_ctx* c = m->ctx;
+  if (k < 1 || k > 127) return set_err(c, FB_ERR_INVALID, "k=%u outside 1..127", k);
+  if (int r = ctx_current(c)) return r;
+  if (int r = check_add_budget(c, fb_bank_bases_for_k(b, k))) return r;
+  Pass1Args a{};
+  a.reads = b->view();
+  a.k = k;
+  a.split = default_split(k, b->max_len);
+  a.weight = 1;
+  a.cms = m->dev();
+  HIP_TRY(c, launch_pass1(a, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+
+extern "C" uint64_t fb_cms_table_bytes(const fb_cms* m) {
+  if (!m) return 0;
+  return m->bits == 4 ? m->width * 2 : m->width * 8;
+}
+
+static int cms_download(fb_ctx* c, const DevCms& d, uint64_t width, int bits, void* out, uint64_t bytes) {
+  const uint64_t need = bits == 4 ? width * 2 : width * 8;
+  if (!out || bytes < need) return set_err(c, FB_ERR_INVALID, "CMS download needs %llu bytes", (unsigned long long)need);
+  DBuf tmp;
+  if (int r = tmp.get(c, need)) return r;
+  if (bits == 4) HIP_TRY(c, launch_pack_nibble(d.counts, width * 4, tmp.as<uint8_t>(), c->stream));
+  else HIP_TRY(c, launch_pack_u16(d.counts, width * 4, tmp.as<uint16_t>(), c->stream));
+  HIP_TRY(c, hipMemcpyAsync(out, tmp.p, need, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+
+extern "C" int fb_cms_download(fb_cms* m, void* out, uint64_t bytes) {
+  if (!m) return FB_ERR_INVALID;
+  if (int r = ctx_current(m->ctx)) return r;
+  return cms_download(m->ctx, m->dev(), m->width, m->bits, out, bytes);
+}
+
+// ---------------------------------------------------------------------------
+// HyperLogLog
+
+struct fb_hll {
+  fb_ctx* ctx = nullptr;
+  DBuf regs;
+  int p = 14;
+  DevHll dev() const {
+    DevHll d;
+    d.regs = regs.as<uint32_t>();
+    d.p = (uint32_t)p;
+    return d;
+  }
+};
+
+static int hll_init(fb_ctx* c, fb_hll* h, int p) {
+  if (p < 4 || p > 18) return set_err(c, FB_ERR_INVALID, "HLL precision %d outside 4..18", p);
+  h->ctx = c;
+  h->p = p;
+  if (int r = h->regs.get(c, ((size_t)1 << p) * 4)) return r;
+  HIP_TRY(c, launch_fill_u32(h->regs.as<uint32_t>(), (uint64_t)1 << p, 0, c->stream));
+  return FB_OK;
+}
+
+extern "C" int fb_hll_create(fb_ctx* c, int p, fb_hll** out) {
+  if (!c || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (int r = ctx_current(c)) return r;
+  fb_hll* h = new fb_hll();
+  if (int r = hll_init(c, h, p)) {
+    delete h;
+    return r;
+  }
+  *out = h;
+  return FB_OK;
+}
+extern "C" void fb_hll_destroy(fb_hll* h) { delete h; }
+extern "C" int fb_hll_clear(fb_hll* h) {
+  if (!h) return FB_ERR_INVALID;
+  if (int r = ctx_current(h->ctx)) return r;
+  HIP_TRY(h->ctx, launch_fill_u32(h->regs.as<uint32_t>(), (uint64_t)1 << h->p, 0, h->ctx->stream));
+  return FB_OK;
+}
+extern "C" int fb_hll_add_hashes(fb_hll* h, const uint64_t* hashes, uint64_t n) {
+  if (!h || (n && !hashes)) return FB_ERR_INVALID;
+  fb_ctx* c = h->ctx;
+  if (int r = ctx_current(c)) return r;
+  DBuf d;
+  if (int r = to_device(c, d, hashes, n * 8)) return r;
+  HIP_TRY(c, launch_hll_add_hashes(h->dev(), d.as<uint64_t>(), n, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+extern "C" int fb_hll_add_bank(fb_hll* h, const fb_bank* b, uint32_t k) {
+  if (!h || !b) return FB_ERR_INVALID;
+  fb_ctx* c = h->ctx;
+  if (k < 1 || k > 127) return set_err(c, FB_ERR_INVALID, "k=%u outside 1..127", k);
+  if (int r = ctx_current(c)) return r;
+  Pass1Args a{};
+  a.reads = b->view();
+  a.k = k;
+  a.split = default_split(k, b->max_len);
+  a.weight = 1;
+  a.hll = h->dev();
+  HIP_TRY(c, launch_pass1(a, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+static int hll_download(fb_ctx* c, const DevHll& d, uint8_t* out) {
+  if (!out) return FB_ERR_INVALID;
+  const uint64_t m = (uint64_t)1 << d.p;
+  DBuf tmp;
+  if (int r = tmp.get(c, m)) return r;
+  HIP_TRY(c, launch_pack_u8(d.regs, m, tmp.as<uint8_t>(), c->stream));
+  HIP_TRY(c, hipMemcpyAsync(out, tmp.p, m, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+extern "C" int fb_hll_registers(fb_hll* h, uint8_t* out) {
+  if (!h) return FB_ERR_INVALID;
+  if (int r = ctx_current(h->ctx)) return r;
+  return hll_download(h->ctx, h->dev(), out);
+}
+extern "C" int fb_hll_estimate(fb_hll* h, double* out) {
+  if (!h || !out) return FB_ERR_INVALID;
+  std::vector<uint8_t> regs((size_t)1 << h->p);
+  if (int r = fb_hll_registers(h, regs.data())) return r;
+  *out = fb_hll_estimate_registers(regs.data(), h->p);
+  return FB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Solid set
+
+struct fb_solid {
+  fb_ctx* ctx = nullptr;
+  uint32_t k = 0;
+  uint32_t nref = 1;
+  uint64_t n = 0;
+  DBuf keys, cov, mask;
+  fb_cms cms;
+  fb_hll hll;
+  bool has_hll = false;
+  float ms[4] = {0, 0, 0, 0};
+  uint64_t stats[4] = {0, 0, 0, 0};  // windows, read records, claimed keys, table slots
+};
+
+extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->k = k;
+  p->target_solid_cov = -1;
+  p->counter_bits = 4;
+  p->sort = 1;
+}
+
+static uint32_t ref_words(uint32_t k) { return k <= 31 ? 1u : (k <= 63 ? 2u : 4u); }
+
+extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
+                              uint64_t n_contigs, const fb_build_params* prm, fb_solid** out) {
+  if (!c || !bank || !prm || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const uint32_t k = prm->k;
+  if (k < 1 || k > 127) return set_err(c, FB_ERR_INVALID, "k=%u outside 1..127", k);
+  if (prm->counter_bits != 4 && prm->counter_bits != 16)
+    return set_err(c, FB_ERR_INVALID, "counter_bits must be 4 or 16 (got %d)", prm->counter_bits);
+  if (prm->hll_p != 0 && (prm->hll_p < 4 || prm->hll_p > 18))
+    return set_err(c, FB_ERR_INVALID, "hll_p %d outside 4..18", prm->hll_p);
+  if (prm->target_solid_cov > 255) return set_err(c, FB_ERR_INVALID, "target_solid_cov is a u8");
+  if (n_contigs && (!contigs || !contig_lens)) return set_err(c, FB_ERR_INVALID, "contigs: null pointer");
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+
+  // build.rs:74-84: genome size source and CMS width
+  uint64_t est_g = prm->genome_size ? std::max<uint64_t>(prm->genome_size, 50000)
+                                    : std::max<uint64_t>(bank->total_bases / 60, 50000);
+  uint64_t width = prm->cms_width ? prm->cms_width : fb_cms_width_for_genome_size(est_g);
+  const uint32_t t_solid = prm->target_solid_cov >= 0 ? (uint32_t)prm->target_solid_cov : 2u;
+  const uint32_t gate = prm->target_solid_cov >= 0 ? (uint32_t)prm->target_solid_cov : 3u;
+
+  uint64_t contig_bases = 0;
+  for (uint64_t i = 0; i < n_contigs; ++i) contig_bases += contig_lens[i];
+  if (int r = check_add_budget(c, fb_bank_bases_for_k(bank, k) + 10 * contig_bases)) return r;
+
+  fb_solid* so = new fb_solid();
+  so->ctx = c;
+  so->k = k;
+  so->nref = ref_words(k);
+  auto fail = [&](int r) {
+    delete so;
+    return r;
+  };
+  if (int r = cms_init(c, &so->cms, width, prm->counter_bits)) return fail(r);
+  if (prm->hll_p) {
+    if (int r = hll_init(c, &so->hll, prm->hll_p)) return fail(r);
+    so->has_hll = true;
+  }
+  // prior contigs as a packed read list (weights applied by the kernels)
+  DBuf cdata, coffs, clens;
+  uint64_t c_reads = 0;
+  if (n_contigs) {
+    std::vector<uint64_t> words((contig_bases + 31) / 32 + 1, 0), offs(n_contigs);
+    fb_pack_ascii(contigs, contig_lens, n_contigs, words.data(), offs.data());
+    fb_arena_view v{words.data(), (uint64_t)words.size(), offs.data(), contig_lens, n_contigs};
+    if (int r = upload_reads(c, &v, 1, cdata, coffs, clens, &c_reads, nullptr, nullptr, nullptr)) return fail(r);
+  }
+  DevReads cview;
+  cview.data = cdata.as<uint64_t>();
+  cview.offs = coffs.as<uint64_t>();
+  cview.lens = clens.as<uint32_t>();
+  cview.n_reads = c_reads;
+  const uint32_t split = prm->split ? prm->split : default_split(k, bank->max_len);
+
+  HIP_TRY(c, hipEventRecord(c->ev[0], s));
+  // pass 1
+  Pass1Args p1{};
+  p1.reads = bank->view();
+  p1.k = k;
+  p1.split = split;
+  p1.weight = 1;
+  p1.cms = so->cms.dev();
+  if (so->has_hll) p1.hll = so->hll.dev();
+  HIP_TRY(c, launch_pass1(p1, s));
+  if (c_reads) {
+    Pass1Args pc = p1;
+    pc.reads = cview;
+    pc.split = 1;
+    pc.weight = 10;
+    pc.hll = DevHll{};
+    HIP_TRY(c, launch_pass1(pc, s));
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[1], s));
+
+  // threshold bitmap of the pass-2 gate
+  const uint64_t ncnt = so->cms.width * 4;
+  DBuf bits;
+  if (int r = bits.get(c, ((ncnt + 31) / 32) * 4)) return fail(r);
+  HIP_TRY(c, launch_solid_bits(so->cms.counts.as<uint32_t>(), ncnt, t_solid, so->cms.cap(), bits.as<uint32_t>(), s));
+
+  // pass 2 into the solid table; grows and re-runs on overflow
+  DevSolidTable tab;
+  tab.kw = (k + 30) / 31;
+  tab.stride = tab.kw + 5;
+  tab.max_probe = 4096;
+  uint64_t cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(est_g * 2.5)));
+  DBuf tbuf, status;
+  if (int r = status.get(c, 16)) return fail(r);
+  tab.status = status.as<uint32_t>();
+  uint32_t st[2] = {0, 0};
+  for (int attempt = 0;; ++attempt) {
+    tab.cap = cap;
+    if (int r = tbuf.get(c, cap * tab.stride * 8)) return fail(r);
+    tab.slots = tbuf.as<uint64_t>();
+    HIP_TRY(c, launch_table_clear(tab, s));
+    HIP_TRY(c, hipEventRecord(c->ev[2], s));
+    Pass2Args p2{};
+    p2.reads = bank->view();
+    p2.k = k;
+    p2.split = split;
+    p2.solid_bits = bits.as<uint32_t>();
+    p2.log2w = so->cms.log2w;
+    p2.table = tab;
+    p2.min_trans_cov = gate;
+    p2.contig_mode = 0;
+    HIP_TRY(c, launch_pass2(p2, s));
+    if (c_reads) {
+      Pass2Args pc = p2;
+      pc.reads = cview;
+      pc.split = 1;
+      pc.contig_mode = 1;
+      HIP_TRY(c, launch_pass2(pc, s));
+    }
+    HIP_TRY(c, hipEventRecord(c->ev[3], s));
+    HIP_TRY(c, hipMemcpyAsync(st, tab.status, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (!st[0]) break;
+    if (attempt >= 6) return fail(set_err(c, FB_ERR_OVERFLOW, "solid table overflow at %llu slots", (unsigned long long)cap));
+    cap *= 2;
+  }
+  bits.release();
+
+  // pass 3
+  const uint64_t claimed = st[1];
+  const uint64_t maxn = std::max<uint64_t>(claimed, 1);
+  if (int r = so->keys.get(c, maxn * so->nref * 8)) return fail(r);
+  if (int r = so->cov.get(c, maxn * 2)) return fail(r);
+  if (int r = so->mask.get(c, maxn)) return fail(r);
+  DBuf cnt;
+  if (int r = cnt.get(c, 16)) return fail(r);
+  HIP_TRY(c, launch_fill_u32(cnt.as<uint32_t>(), 4, 0, s));
+  HIP_TRY(c, hipEventRecord(c->ev[4], s));
+  HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
+                          so->mask.as<uint8_t>(), cnt.as<uint32_t>(),
+                          reinterpret_cast<unsigned long long*>(cnt.as<uint32_t>() + 2), s));
+  uint32_t cbuf[4] = {0, 0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(cbuf, cnt.p, 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const uint32_t n_out = cbuf[0];
+  uint64_t rec_sum = 0;
+  memcpy(&rec_sum, cbuf + 2, 8);
+  so->stats[0] = bank_windows(bank, k);
+  so->stats[1] = rec_sum;
+  so->stats[2] = claimed;
+  so->stats[3] = cap;
+  tbuf.release();
+  so->n = n_out;
+  if (prm->sort && n_out > 1) {
+    size_t tb = 0;
+    HIP_TRY(c, sort_solid(nullptr, nullptr, nullptr, n_out, so->nref, k, nullptr, &tb, s));
+    DBuf tmp;
+    if (int r = tmp.get(c, tb)) return fail(r);
+    HIP_TRY(c, sort_solid(so->keys.as<uint64_t>(), so->cov.as<uint16_t>(), so->mask.as<uint8_t>(), n_out, so->nref, k,
+                          tmp.p, &tb, s));
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[5], s));
+  HIP_TRY(c, hipEventSynchronize(c->ev[5]));
+  (void)hipEventElapsedTime(&so->ms[0], c->ev[0], c->ev[1]);
+  (void)hipEventElapsedTime(&so->ms[1], c->ev[2], c->ev[3]);
+  (void)hipEventElapsedTime(&so->ms[2], c->ev[4], c->ev[5]);
+  (void)hipEventElapsedTime(&so->ms[3], c->ev[0], c->ev[5]);
+  *out = so;
+  return FB_OK;
+}
+
+extern "C" void fb_solid_destroy(fb_solid* s) { delete s; }
+extern "C" uint64_t fb_solid_count(const fb_solid* s) { return s ? s->n : 0; }
+extern "C" int fb_solid_key_words(const fb_solid* s) { return s ? (int)s->nref : 0; }
+extern "C" int fb_solid_download(const fb_solid* s, uint64_t* keys, uint16_t* cov, uint8_t* mask) {
+  if (!s) return FB_ERR_INVALID;
+  fb_ctx* c = s->ctx;
+  if (int r = ctx_current(c)) return r;
+  if (s->n) {
+    if (keys) HIP_TRY(c, hipMemcpyAsync(keys, s->keys.p, s->n * s->nref * 8, hipMemcpyDeviceToHost, c->stream));
+    if (cov) HIP_TRY(c, hipMemcpyAsync(cov, s->cov.p, s->n * 2, hipMemcpyDeviceToHost, c->stream));
+    if (mask) HIP_TRY(c, hipMemcpyAsync(mask, s->mask.p, s->n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+extern "C" uint64_t fb_solid_cms_bytes(const fb_solid* s) { return s ? fb_cms_table_bytes(&s->cms) : 0; }
+extern "C" int fb_solid_cms_download(const fb_solid* s, void* out, uint64_t bytes) {
+  if (!s) return FB_ERR_INVALID;
+  if (int r = ctx_current(s->ctx)) return r;
+  return cms_download(s->ctx, s->cms.dev(), s->cms.width, s->cms.bits, out, bytes);
+}
+extern "C" int fb_solid_hll_registers(const fb_solid* s, uint8_t* out) {
+  if (!s || !s->has_hll) return FB_ERR_INVALID;
+  if (int r = ctx_current(s->ctx)) return r;
+  return hll_download(s->ctx, s->hll.dev(), out);
+}
+extern "C" int fb_solid_timings(const fb_solid* s, float out[4]) {
+  if (!s || !out) return FB_ERR_INVALID;
+  memcpy(out, s->ms, sizeof s->ms);
+  return FB_OK;
+}
+extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[4]) {
+  if (!s || !out) return FB_ERR_INVALID;
+  memcpy(out, s->stats, sizeof s->stats);
+  return FB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Genome estimator
+
+extern "C" int fb_estimate_genome(fb_ctx* c, const fb_arena_view* r1, const fb_arena_view* r2,
+                                  const fb_count_stats* st, uint64_t max_pairs, double target_cov,
+                                  fb_genome_estimate* out) {
+  if (!c || !r1 || !r2 || !st || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  memset(out, 0, sizeof *out);
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+  // estimator.rs:305: sample_target = min(max_pairs, 150000, exact pairs);
+  // the loop also ends when either reader runs dry.
+  uint64_t n = std::min<uint64_t>(std::min<uint64_t>(max_pairs, 150000), st->exact_read_pairs);
+  n = std::min<uint64_t>(n, std::min(r1->n_reads, r2->n_reads));
+  if (n == 0) return set_err(c, FB_ERR_EMPTY, "Input FASTQ files contain 0 valid read pairs");
+  const uint32_t K = 21;
+  // interleaved read list R1[0], R2[0], R1[1], ... and occurrence bases
+  fb_arena_view parts[2] = {*r1, *r2};
+  parts[0].n_reads = n;
+  parts[1].n_reads = n;
+  DBuf data, offs0, lens0;
+  uint64_t nr = 0;
+  if (int r = upload_reads(c, parts, 2, data, offs0, lens0, &nr, nullptr, nullptr, nullptr)) return r;
+  std::vector<uint64_t> h_offs(2 * n);
+  std::vector<uint32_t> h_lens(2 * n), occ(2 * n);
+  std::vector<uint64_t> m_prefix(n + 1, 0);  // windows of pairs [0, i)
+  uint64_t t = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    h_offs[2 * i] = r1->offsets[i];
+    h_offs[2 * i + 1] = r2->offsets[i] + r1->n_words * 32;
+    h_lens[2 * i] = r1->lengths[i];
+    h_lens[2 * i + 1] = r2->lengths[i];
+    for (int j = 0; j < 2; ++j) {
+      occ[2 * i + j] = (uint32_t)t;
+      const uint32_t len = h_lens[2 * i + j];
+      if (len >= K) t += len - K + 1;
+    }
+    m_prefix[i + 1] = t;
+    if (t >= 0xFFFFFFFFull) return set_err(c, FB_ERR_INVALID, "estimator sample exceeds 2^32 k-mers");
+  }
+  DBuf offs, lens, occd, first_ins, ftot, fsol;
+  if (int r = to_device(c, offs, h_offs.data(), h_offs.size() * 8)) return r;
+  if (int r = to_device(c, lens, h_lens.data(), h_lens.size() * 4)) return r;
+  if (int r = to_device(c, occd, occ.data(), occ.size() * 4)) return r;
+  const uint64_t ncell = 4ull << 23, nreg = (1u << 14) * 64u;
+  if (int r = first_ins.get(c, ncell * 4)) return r;
+  if (int r = ftot.get(c, nreg * 4)) return r;
+  if (int r = fsol.get(c, nreg * 4)) return r;
+  HIP_TRY(c, launch_fill_u32(first_ins.as<uint32_t>(), ncell, 0xFFFFFFFFu, s));
+  HIP_TRY(c, launch_fill_u32(ftot.as<uint32_t>(), nreg, 0xFFFFFFFFu, s));
+  HIP_TRY(c, launch_fill_u32(fsol.as<uint32_t>(), nreg, 0xFFFFFFFFu, s));
+  EstArgs a{};
+  a.reads.data = data.as<uint64_t>();
+  a.reads.offs = offs.as<uint64_t>();
+  a.reads.lens = lens.as<uint32_t>();
+  a.reads.n_reads = 2 * n;
+  a.occ_base = occd.as<uint32_t>();
+  a.n_reads = (uint32_t)(2 * n);
+  a.first_ins = first_ins.as<uint32_t>();
+  a.first_total = ftot.as<uint32_t>();
+  a.first_solid = fsol.as<uint32_t>();
+  HIP_TRY(c, launch_est_first_insert(a, s));
+  HIP_TRY(c, launch_est_solid(a, s));
+  std::vector<uint32_t> ft(nreg), fs(nreg);
+  HIP_TRY(c, hipMemcpyAsync(ft.data(), ftot.p, nreg * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(fs.data(), fsol.p, nreg * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+
+  // Register timelines: events (pair, register, rank) sorted by pair.
+  struct Ev {
+    uint32_t pair, idx, rho;
+  };
+  auto events = [&](const std::vector<uint32_t>& f) {
+    std::vector<Ev> e;
+    for (uint32_t idx = 0; idx < (1u << 14); ++idx)
+      for (uint32_t rho = 1; rho < 64; ++rho)
+        if (f[idx * 64 + rho] != 0xFFFFFFFFu) e.push_back({f[idx * 64 + rho], idx, rho});
+    std::sort(e.begin(), e.end(), [](const Ev& x, const Ev& y) { return x.pair < y.pair; });
+    return e;
+  };
+  std::vector<Ev> et = events(ft), es = events(fs);
+  std::vector<uint8_t> regt(1u << 14, 0), regs(1u << 14, 0);
+  size_t it = 0, is = 0;
+  auto advance = [](const std::vector<Ev>& e, size_t& i, std::vector<uint8_t>& reg, uint64_t upto) {
+    while (i < e.size() && e[i].pair < upto) {
+      if (e[i].rho > reg[e[i].idx]) reg[e[i].idx] = (uint8_t)e[i].rho;
+      ++i;
+    }
+  };
+  uint64_t stop = n;
+  for (uint64_t cp = 6000; cp <= n; cp += 2000) {  // pair_count >= 5000 && pair_count % 2000 == 0
+    advance(et, it, regt, cp);
+    advance(es, is, regs, cp);
+    uint64_t u1 = f64_to_u64(round(fb_hll_estimate_registers(regt.data(), 14)));
+    uint64_t u2 = f64_to_u64(round(fb_hll_estimate_registers(regs.data(), 14)));
+    if (checkpoint_stop(u1, u2, m_prefix[cp])) {
+      stop = cp;
+      break;
+    }
+  }
+  advance(et, it, regt, stop);
+  advance(es, is, regs, stop);
+  uint64_t u1 = f64_to_u64(round(fb_hll_estimate_registers(regt.data(), 14)));
+  uint64_t u2 = f64_to_u64(round(fb_hll_estimate_registers(regs.data(), 14)));
+  estimate_finish(u1, u2, m_prefix[stop], st, target_cov, out);
+  out->pairs_sampled = stop;
+  return FB_OK;
+}

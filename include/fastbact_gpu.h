@@ -1,0 +1,204 @@
+/*
+ * fastbact_gpu.h -- C ABI of the MI355X (gfx950) k-mer spectrum path.
+ *
+ * A drop-in for fastBact's k-mer spectrum hot path: rolling canonical k-mers
+ * over 2-bit packed reads, the Count-Min Sketch, HyperLogLog registers, the
+ * genome estimator's sketch loop and the solid-k-mer set that feeds unitig
+ * compaction.  Plain pointers and sizes only.  Every call returns an int
+ * status (FB_OK or a negative FB_ERR_*); fb_last_error() names the cause.  No
+ * exception crosses the boundary.
+ *
+ * Threading: a context owns one HIP stream.  Calls on one context must not
+ * overlap (lock externally when sharing a context between host threads).
+ * Objects created from a context are used only with that context.
+ *
+ * Reference interfaces replaced (paths in the fastBact source tree):
+ *   fb_bank_*           ReadBank / BitSeqArena           src/common/bitseq.rs:145-399
+ *   fb_cms_*            CountMinSketch                   src/common/sketch.rs:121-254
+ *   fb_hll_*            HyperLogLog                      src/common/sketch.rs:24-115
+ *   fb_build_solid      build_cdbg_with_contigs_ext<K>   src/graph/build.rs:67-328
+ *                       (passes 1-3; unitig compaction, build.rs:330-597, is downstream)
+ *   fb_estimate_genome  estimate_genome                  src/ingest/estimator.rs:286-440
+ *   fb_count_stats_from_lengths  count_fastx_exact (length part)  src/ingest/estimator.rs:167-283
+ *   fb_compute_dynamic_kmers                             src/ingest/estimator.rs:104-161
+ */
+#ifndef FASTBACT_GPU_H
+#define FASTBACT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FB_OK 0
+#define FB_ERR_INVALID -1   /* bad argument (k out of range, null pointer, ...)    */
+#define FB_ERR_OOM -2       /* device allocation failed or HBM budget exceeded      */
+#define FB_ERR_OVERFLOW -3  /* solid table could not hold the keys (after growth)   */
+#define FB_ERR_HIP -4       /* HIP runtime error                                    */
+#define FB_ERR_EMPTY -5     /* no read pairs: "Input FASTQ files contain 0 valid read pairs" */
+#define FB_ERR_NODEV -6     /* no usable GPU                                        */
+#define FB_ERR_BUSY -7      /* serving queue full                                   */
+
+typedef struct fb_ctx fb_ctx;
+typedef struct fb_bank fb_bank;
+typedef struct fb_cms fb_cms;
+typedef struct fb_hll fb_hll;
+typedef struct fb_solid fb_solid;
+
+/* One BitSeqArena (bitseq.rs:145-149): base g of the concatenated stream at
+ * bits 2*(g&31) of data[g>>5]; reads contiguous, offsets[i] = base index. */
+typedef struct {
+  const uint64_t* data;
+  uint64_t n_words;
+  const uint64_t* offsets;
+  const uint32_t* lengths;
+  uint64_t n_reads;
+} fb_arena_view;
+
+/* ---- context -------------------------------------------------------------
+ * hbm_budget_bytes caps this context's device allocations (0 = no cap). */
+int fb_ctx_create(int device, uint64_t hbm_budget_bytes, fb_ctx** out);
+void fb_ctx_destroy(fb_ctx* ctx);
+const char* fb_last_error(const fb_ctx* ctx);
+int fb_ctx_synchronize(fb_ctx* ctx);
+/* Device bytes currently held by the context's objects and caches. */
+uint64_t fb_ctx_device_bytes(const fb_ctx* ctx);
+/* Library version string. */
+const char* fb_version(void);
+
+/* ---- ReadBank (bitseq.rs:350-399) -----------------------------------------
+ * arenas[0] = merged_reads, [1] = pe_forward, [2] = pe_reverse; any may be
+ * empty (n_reads = 0).  Host buffers are copied; the caller keeps ownership. */
+int fb_bank_upload(fb_ctx* ctx, const fb_arena_view arenas[3], fb_bank** out);
+void fb_bank_destroy(fb_bank* bank);
+/* ReadBank::total_bases / total_reads */
+uint64_t fb_bank_total_bases(const fb_bank* bank);
+uint64_t fb_bank_total_reads(const fb_bank* bank);
+/* Sum of lengths of reads with len >= k (the bases a k pass counts). */
+uint64_t fb_bank_bases_for_k(const fb_bank* bank, uint32_t k);
+
+/* ---- CountMinSketch (sketch.rs:121-254) -----------------------------------
+ * width is rounded up to a power of two (sketch.rs:141-143).  counter_bits 4
+ * is the reference's nibble sketch; 16 is the same sketch with saturating
+ * u16 counters.  Counts are kept exact on the device; reads of the sketch
+ * see min(count, 15) or min(count, 65535). */
+int fb_cms_create(fb_ctx* ctx, uint64_t width, int counter_bits, fb_cms** out);
+void fb_cms_destroy(fb_cms* cms);
+uint64_t fb_cms_width(const fb_cms* cms);
+/* CountMinSketch::width_for_genome_size (sketch.rs:165-168) */
+uint64_t fb_cms_width_for_genome_size(uint64_t genome_size);
+int fb_cms_clear(fb_cms* cms);
+/* CountMinSketch::add for each key (host array). */
+int fb_cms_add_keys(fb_cms* cms, const uint64_t* keys, uint64_t n);
+/* CountMinSketch::estimate for each key. */
+int fb_cms_query(fb_cms* cms, const uint64_t* keys, uint64_t n, uint32_t* out);
+/* Pass 1 of build_cdbg_with_contigs_ext (build.rs:93-116): every canonical
+ * k-mer of every read with len >= k, key = KmerInt::to_u64. */
+int fb_cms_add_bank(fb_cms* cms, const fb_bank* bank, uint32_t k);
+/* Table in the reference layout: nibble -> width*4/2 bytes (counter idx =
+ * row*width+col, byte idx>>1, high nibble for odd idx); u16 -> width*4 u16. */
+uint64_t fb_cms_table_bytes(const fb_cms* cms);
+int fb_cms_download(fb_cms* cms, void* out, uint64_t bytes);
+
+/* ---- HyperLogLog (sketch.rs:24-115), precision 4..18 ---------------------- */
+int fb_hll_create(fb_ctx* ctx, int p, fb_hll** out);
+void fb_hll_destroy(fb_hll* hll);
+int fb_hll_clear(fb_hll* hll);
+/* HyperLogLog::add_hash for each hash. */
+int fb_hll_add_hashes(fb_hll* hll, const uint64_t* hashes, uint64_t n);
+/* add_hash(splitmix64(canonical ntHash_k)) over every window of every read
+ * with len >= k (HyperLogLog::add_canonical_hash, sketch.rs:68-70). */
+int fb_hll_add_bank(fb_hll* hll, const fb_bank* bank, uint32_t k);
+int fb_hll_registers(fb_hll* hll, uint8_t* out /* 2^p bytes */);
+/* HyperLogLog::estimate (sketch.rs:83-114), generalised to m = 2^p. */
+int fb_hll_estimate(fb_hll* hll, double* out);
+/* The same estimator on a host register array. */
+double fb_hll_estimate_registers(const uint8_t* regs, int p);
+
+/* ---- Solid k-mer set: build_cdbg_with_contigs_ext passes 1-3 --------------- */
+typedef struct {
+  uint32_t k;                 /* 1..127 */
+  int32_t target_solid_cov;   /* Option<u8>: < 0 = None (t_solid 2, edge gate 3) */
+  uint64_t genome_size;       /* Option<u64>: 0 = None (bank bases / 60, build.rs:74-83) */
+  uint64_t cms_width;         /* 0 = CountMinSketch::for_genome_size(est_g); else explicit */
+  int32_t counter_bits;       /* 4 (reference nibble) or 16 (saturating u16) */
+  int32_t hll_p;              /* 0 = no per-pass HLL; else precision of the pass HLL */
+  uint32_t sort;              /* 1 = records sorted by k-mer (parity form) */
+  uint32_t split;             /* threads per read; 0 = tuned default */
+} fb_build_params;
+
+void fb_build_params_default(fb_build_params* p, uint32_t k);
+
+/* contigs: prior unitig sequences as ASCII (concatenated) + lengths; may be
+ * null/0.  Weights follow build.rs:119-138 (x10 CMS adds) and 203-240
+ * (+100 cov, +1000 edge).  The result owns the pass's CMS, HLL and solid
+ * set on the device. */
+int fb_build_solid(fb_ctx* ctx, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
+                   uint64_t n_contigs, const fb_build_params* params, fb_solid** out);
+void fb_solid_destroy(fb_solid* s);
+uint64_t fb_solid_count(const fb_solid* s);
+/* Words per key: 1 (k<=31, u64), 2 (k<=63, u128 {hi,lo}), 4 (k<=127, [u64;4]),
+ * most significant first as the reference stores them. */
+int fb_solid_key_words(const fb_solid* s);
+int fb_solid_download(const fb_solid* s, uint64_t* keys, uint16_t* cov, uint8_t* mask);
+/* The pass's sketches: table bytes as fb_cms_download, registers as fb_hll_registers. */
+uint64_t fb_solid_cms_bytes(const fb_solid* s);
+int fb_solid_cms_download(const fb_solid* s, void* out, uint64_t bytes);
+int fb_solid_hll_registers(const fb_solid* s, uint8_t* out);
+/* Timing of the build (HIP events on the context stream around the kernels),
+ * milliseconds: [0] pass 1 kernel (CMS + HLL), [1] pass 2 kernel, [2] pass 3
+ * + sort, [3] whole build. */
+int fb_solid_timings(const fb_solid* s, float out[4]);
+/* Work counts of the build: [0] k-mer windows of pass 1, [1] pass-2 records
+ * (sum of raw cov), [2] keys in the solid table, [3] table slots. */
+int fb_solid_stats(const fb_solid* s, uint64_t out[4]);
+
+/* ---- Genome estimator (estimator.rs) -------------------------------------- */
+typedef struct {
+  uint64_t exact_read_pairs;
+  uint64_t total_bases_r1, total_bases_r2, total_bases;
+  uint64_t min_read_len, max_read_len;
+  double mean_read_len;
+  uint64_t median_read_len;
+} fb_count_stats;
+
+typedef struct {
+  uint64_t genome_size;
+  uint64_t exact_read_pairs;
+  uint64_t total_bases;
+  uint64_t median_read_len;
+  uint64_t max_read_len;
+  double estimated_cov;
+  double subsample_factor;
+  uint32_t kmers[8];
+  uint32_t n_kmers;
+  int32_t confidence;  /* 0 HIGH, 1 LOW_COVERAGE_WARNING, 2 UNRELIABLE */
+  uint64_t u_ge_1, u_ge_2;
+  double sample_kmer_depth;
+  uint64_t pairs_sampled;
+  uint64_t m_sample;
+} fb_genome_estimate;
+
+/* count_fastx_exact's statistics from per-read lengths of the pairs.
+ * FB_ERR_EMPTY when n_pairs == 0. */
+int fb_count_stats_from_lengths(const uint32_t* len_r1, const uint32_t* len_r2, uint64_t n_pairs, fb_count_stats* out);
+/* compute_dynamic_kmers; target_max_k < 0 = None.  Returns the count (<= 8). */
+uint32_t fb_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uint32_t* out8);
+/* estimate_genome over R1[i]/R2[i] pairs (ASCII->2-bit is lossless for the
+ * estimator: N maps to A before hashing, bitseq.rs:67-81). */
+int fb_estimate_genome(fb_ctx* ctx, const fb_arena_view* r1, const fb_arena_view* r2, const fb_count_stats* stats,
+                       uint64_t max_pairs, double target_cov, fb_genome_estimate* out);
+
+/* ---- Host helpers ---------------------------------------------------------- */
+/* BitSeqArena::push_seq for many sequences: data must hold ceil(total/32)
+ * zeroed words; offsets receives n entries. */
+int fb_pack_ascii(const uint8_t* ascii, const uint32_t* lengths, uint64_t n, uint64_t* data, uint64_t* offsets);
+/* splitmix64 (sketch.rs:13-18) */
+uint64_t fb_splitmix64(uint64_t z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1256 @@
+/*
+ * fb_oracle.c -- CPU restatement of fastBact's k-mer spectrum path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker and the timed CPU baseline
+ * ("port" kind).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it.  The GPU product never links, loads or calls it.
+ *
+ * Citations are to the fastBact source tree (src/...).  How each part is
+ * pinned (tests/test_oracle_*.py):
+ *   splitmix64            closed form vs an independent Python big-int restatement
+ *   rev_comp_word         golden vectors of bitseq.rs:434-446 + brute force on strings
+ *   KmerInt (1/2/4 words) from_ascii/to_ascii round trip, rev_comp vs string RC,
+ *                         rolling == fresh, fwd/RC canonical agreement for
+ *                         k = 21/55/99/127 (kmer.rs:543-651), fold vs big-int
+ *   ntHash                closed-form definition in Python, roll == fresh (kmer.rs:653-664)
+ *   CountMinSketch        0/5/15 and 4x3=12 (sketch.rs:304-342), clamped per-cell
+ *                         histogram computed by brute force
+ *   HyperLogLog           per-hash register closed form, p16 -> p14 fold, error
+ *                         gates of sketch.rs:261-407
+ *   estimator             count/ladder known answers (estimator.rs:463-546), Poisson tail
+ *                         recomputed in Python, order-dependent hll_solid vs a
+ *                         sequential Python emulation on small inputs
+ *   solid set             brute force with exact counts at a collision-free width,
+ *                         and the 10 kb circular-repeat case of build.rs:604-687
+ *                         (all genome k-mers solid, errors filtered, 3 unitigs)
+ *   f64 HLL estimate      "parity unpinned" against the Rust binary (cannot be run
+ *                         here); sequential sum in register order with glibc log/exp.
+ */
+#define _GNU_SOURCE
+#include "fb_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+#define FORCE_INLINE static inline __attribute__((always_inline))
+
+/* ------------------------------------------------------------------------- */
+/* sketch.rs:13-18                                                           */
+uint64_t orc_splitmix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+FORCE_INLINE uint64_t sm64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+/* bitseq.rs:67-81: A/a=0 C/c=1 G/g=2 T/t=3, everything else 0 */
+static uint8_t A2B[256];
+static void init_a2b(void) __attribute__((constructor));
+static void init_a2b(void) {
+    memset(A2B, 0, sizeof A2B);
+    A2B['C'] = A2B['c'] = 1;
+    A2B['G'] = A2B['g'] = 2;
+    A2B['T'] = A2B['t'] = 3;
+}
+uint8_t orc_ascii_to_2bit(uint8_t c) { return A2B[c]; }
+static const uint8_t TWO_BIT_TO_ASCII[4] = {'A', 'C', 'G', 'T'};
+
+/* bitseq.rs:120-138 */
+FORCE_INLINE uint64_t rcw(uint64_t word, uint32_t nb) {
+    if (nb == 0) return 0;
+    uint64_t w = __builtin_bswap64(word);
+    w = ((w & 0x0f0f0f0f0f0f0f0fULL) << 4) | ((w & 0xf0f0f0f0f0f0f0f0ULL) >> 4);
+    w = ((w & 0x3333333333333333ULL) << 2) | ((w & 0xccccccccccccccccULL) >> 2);
+    w = ~w;
+    if (nb < 32) {
+        uint32_t shift = (32 - nb) * 2;
+        uint64_t mask = (1ULL << (nb * 2)) - 1;
+        return (w >> shift) & mask;
+    }
+    return w;
+}
+uint64_t orc_rev_comp_word(uint64_t word, uint32_t nb) { return rcw(word, nb); }
+
+/* bitseq.rs:195-222 */
+void orc_arena_pack(const uint8_t* ascii, const uint32_t* lengths, uint64_t n, uint64_t* data,
+                    uint64_t* offsets) {
+    uint64_t pos = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        offsets[i] = pos;
+        for (uint32_t j = 0; j < lengths[i]; j++) {
+            uint64_t g = pos + j;
+            data[g >> 5] |= (uint64_t)A2B[*ascii++] << ((g & 31) << 1);
+        }
+        pos += lengths[i];
+    }
+}
+/* bitseq.rs:286-299 */
+void orc_arena_extract_ascii(const orc_arena* a, uint64_t id, uint8_t* out) {
+    uint64_t start = a->offsets[id];
+    uint32_t len = a->lengths[id];
+    for (uint32_t i = 0; i < len; i++) {
+        uint64_t g = start + i;
+        out[i] = TWO_BIT_TO_ASCII[(a->data[g >> 5] >> ((g & 31) << 1)) & 3];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* KmerInt.  km_t holds nw words, most significant first.                     */
+typedef struct { uint64_t w[4]; } km_t;
+
+int orc_kmer_words_for_k(uint32_t k) { return k <= 31 ? 1 : (k <= 63 ? 2 : 4); }
+
+FORCE_INLINE u128 k2_get(km_t x) { return ((u128)x.w[0] << 64) | x.w[1]; }
+FORCE_INLINE km_t k2_put(u128 v) { km_t r = {{(uint64_t)(v >> 64), (uint64_t)v, 0, 0}}; return r; }
+
+/* kmer.rs:73-81, 132-140, 210-222 */
+FORCE_INLINE km_t km_from_ascii(const uint8_t* s, uint32_t k, int nw) {
+    km_t r = {{0, 0, 0, 0}};
+    if (nw == 1) {
+        uint64_t v = 0;
+        for (uint32_t i = 0; i < k; i++) v = (v << 2) | A2B[s[i]];
+        r.w[0] = v;
+    } else if (nw == 2) {
+        u128 v = 0;
+        for (uint32_t i = 0; i < k; i++) v = (v << 2) | A2B[s[i]];
+        r = k2_put(v);
+    } else {
+        for (uint32_t i = 0; i < k; i++) {
+            uint64_t code = A2B[s[i]];
+            r.w[0] = (r.w[0] << 2) | (r.w[1] >> 62);
+            r.w[1] = (r.w[1] << 2) | (r.w[2] >> 62);
+            r.w[2] = (r.w[2] << 2) | (r.w[3] >> 62);
+            r.w[3] = (r.w[3] << 2) | code;
+        }
+    }
+    return r;
+}
+
+/* kmer.rs:245-257, 291-302: mask a 4-word value to 2k bits */
+FORCE_INLINE void k4_mask(uint64_t* w, uint32_t k) {
+    uint32_t total_bits = 2 * k;
+    for (int j = 0; j < 4; j++) {
+        uint32_t word_min_bit = (uint32_t)(3 - j) * 64;
+        if (total_bits <= word_min_bit) {
+            w[j] = 0;
+        } else if (total_bits < word_min_bit + 64) {
+            uint32_t bits = total_bits - word_min_bit;
+            w[j] &= (1ULL << bits) - 1;
+        }
+    }
+}
+
+/* kmer.rs:99-101, 158-178, 260-305 */
+FORCE_INLINE km_t km_rev_comp(km_t x, uint32_t k, int nw) {
+    km_t r = {{0, 0, 0, 0}};
+    if (nw == 1) {
+        r.w[0] = rcw(x.w[0], k);
+    } else if (nw == 2) {
+        if (k == 0) return r;
+        uint64_t hi = x.w[0], lo = x.w[1];
+        u128 full = ((u128)rcw(lo, 32) << 64) | (u128)rcw(hi, 32);
+        if (k < 64) {
+            uint32_t shift = (64 - k) * 2;
+            u128 mask = (((u128)1) << (2 * k)) - 1;
+            full = (full >> shift) & mask;
+        }
+        r = k2_put(full);
+    } else {
+        if (k == 0) return r;
+        uint64_t full[4] = {rcw(x.w[3], 32), rcw(x.w[2], 32), rcw(x.w[1], 32), rcw(x.w[0], 32)};
+        if (k == 128) {
+            memcpy(r.w, full, sizeof full);
+            return r;
+        }
+        uint32_t s = (128 - k) * 2;
+        uint32_t w_shift = s / 64, b_shift = s % 64;
+        uint64_t sh[4] = {0, 0, 0, 0};
+        for (uint32_t i = 0; i < 4; i++) {
+            if (i + w_shift < 4) {
+                sh[i + w_shift] |= full[i] >> b_shift;
+                if (b_shift > 0 && i + w_shift + 1 < 4) sh[i + w_shift + 1] |= full[i] << (64 - b_shift);
+            }
+        }
+        k4_mask(sh, k);
+        memcpy(r.w, sh, sizeof sh);
+    }
+    return r;
+}
+
+/* kmer.rs:104-112, 181-189, 307-329 */
+FORCE_INLINE void km_roll(km_t* fwd, km_t* rev, uint8_t base, uint32_t k, int nw) {
+    uint64_t b = base & 3;
+    uint64_t cb = 3 - b; /* COMPLEMENT_2BIT */
+    if (nw == 1) {
+        uint64_t mask = k >= 32 ? ~0ULL : ((1ULL << (2 * k)) - 1);
+        fwd->w[0] = ((fwd->w[0] << 2) | b) & mask;
+        rev->w[0] = (rev->w[0] >> 2) | (cb << (2 * (k - 1)));
+    } else if (nw == 2) {
+        u128 mask = k >= 64 ? ~(u128)0 : ((((u128)1) << (2 * k)) - 1);
+        u128 f = ((k2_get(*fwd) << 2) | b) & mask;
+        u128 r = (k2_get(*rev) >> 2) | ((u128)cb << (2 * (k - 1)));
+        *fwd = k2_put(f);
+        *rev = k2_put(r);
+    } else {
+        uint64_t* f = fwd->w;
+        uint64_t nf[4] = {(f[0] << 2) | (f[1] >> 62), (f[1] << 2) | (f[2] >> 62),
+                          (f[2] << 2) | (f[3] >> 62), (f[3] << 2) | b};
+        k4_mask(nf, k);
+        memcpy(f, nf, sizeof nf);
+        uint64_t* r = rev->w;
+        uint64_t nr[4] = {r[0] >> 2, (r[1] >> 2) | (r[0] << 62), (r[2] >> 2) | (r[1] << 62),
+                          (r[3] >> 2) | (r[2] << 62)};
+        uint32_t bit_pos = 2 * (k - 1);
+        nr[3 - bit_pos / 64] |= cb << (bit_pos % 64);
+        memcpy(r, nr, sizeof nr);
+    }
+}
+
+/* derived Ord on u64 / u128 / [u64;4]: lexicographic over MSB-first words */
+FORCE_INLINE int km_lt(const km_t* a, const km_t* b, int nw) {
+    for (int i = 0; i < nw; i++)
+        if (a->w[i] != b->w[i]) return a->w[i] < b->w[i];
+    return 0;
+}
+FORCE_INLINE int km_eq(const km_t* a, const km_t* b, int nw) {
+    for (int i = 0; i < nw; i++)
+        if (a->w[i] != b->w[i]) return 0;
+    return 1;
+}
+/* kmer.rs:115-117, 192-194, 332-334 */
+FORCE_INLINE uint64_t km_fold(const km_t* x, int nw) {
+    uint64_t f = x->w[0];
+    for (int i = 1; i < nw; i++) f ^= x->w[i];
+    return f;
+}
+
+static km_t km_load(const uint64_t* p, int nw) {
+    km_t r = {{0, 0, 0, 0}};
+    memcpy(r.w, p, (size_t)nw * 8);
+    return r;
+}
+void orc_kmer_from_ascii(const uint8_t* ascii, uint32_t k, int nw, uint64_t* out) {
+    km_t r = km_from_ascii(ascii, k, nw);
+    memcpy(out, r.w, (size_t)nw * 8);
+}
+void orc_kmer_rev_comp(const uint64_t* x, uint32_t k, int nw, uint64_t* out) {
+    km_t r = km_rev_comp(km_load(x, nw), k, nw);
+    memcpy(out, r.w, (size_t)nw * 8);
+}
+void orc_kmer_roll_next(const uint64_t* fwd, const uint64_t* rev, uint8_t base, uint32_t k, int nw,
+                        uint64_t* out_fwd, uint64_t* out_rev) {
+    km_t f = km_load(fwd, nw), r = km_load(rev, nw);
+    km_roll(&f, &r, base, k, nw);
+    memcpy(out_fwd, f.w, (size_t)nw * 8);
+    memcpy(out_rev, r.w, (size_t)nw * 8);
+}
+uint64_t orc_kmer_to_u64(const uint64_t* x, int nw) {
+    km_t v = km_load(x, nw);
+    return km_fold(&v, nw);
+}
+void orc_kmer_canonical_windows(const uint8_t* s, uint32_t len, uint32_t k, int nw, uint64_t* out,
+                                uint8_t* is_rc) {
+    if (len < k || k == 0) return;
+    km_t fwd = km_from_ascii(s, k, nw);
+    km_t rev = km_rev_comp(fwd, k, nw);
+    for (uint32_t i = k;; i++) {
+        int rc = km_lt(&rev, &fwd, nw);
+        const km_t* c = rc ? &rev : &fwd;
+        uint64_t w = i - k;
+        memcpy(out + w * nw, c->w, (size_t)nw * 8);
+        is_rc[w] = (uint8_t)rc;
+        if (i >= len) break;
+        km_roll(&fwd, &rev, A2B[s[i]], k, nw);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* ntHash, kmer.rs:385-477                                                    */
+static const uint64_t NT_FWD[4] = {0x3c8bfbb395c60474ULL, 0x3193c18562a02b4cULL,
+                                   0x20323ee046061444ULL, 0x295c504fac61002eULL};
+static const uint64_t NT_REV[4] = {0x295c504fac61002eULL, 0x20323ee046061444ULL,
+                                   0x3193c18562a02b4cULL, 0x3c8bfbb395c60474ULL};
+FORCE_INLINE uint64_t rol64(uint64_t x, uint32_t r) { r &= 63; return r ? (x << r) | (x >> (64 - r)) : x; }
+FORCE_INLINE uint64_t ror64(uint64_t x, uint32_t r) { r &= 63; return r ? (x >> r) | (x << (64 - r)) : x; }
+
+uint64_t orc_nthash_fwd(const uint8_t* s, uint32_t k) {
+    uint64_t h = 0;
+    for (uint32_t i = 0; i < k; i++) h ^= rol64(NT_FWD[A2B[s[i]]], (k - 1 - i) % 64);
+    return h;
+}
+uint64_t orc_nthash_rev(const uint8_t* s, uint32_t k) {
+    uint64_t h = 0;
+    for (uint32_t i = 0; i < k; i++) h ^= rol64(NT_REV[A2B[s[i]]], i % 64);
+    return h;
+}
+uint64_t orc_nthash_canonical(const uint8_t* s, uint32_t k) {
+    uint64_t f = orc_nthash_fwd(s, k), r = orc_nthash_rev(s, k);
+    return f < r ? f : r;
+}
+typedef struct { uint32_t k; uint64_t f, r; } nth_t;
+FORCE_INLINE uint64_t nth_roll(nth_t* h, uint8_t out_c, uint8_t in_c) {
+    uint32_t kmod = h->k % 64, km1 = (h->k - 1) % 64;
+    h->f = rol64(h->f, 1) ^ rol64(NT_FWD[A2B[out_c]], kmod) ^ NT_FWD[A2B[in_c]];
+    h->r = ror64(h->r, 1) ^ ror64(NT_REV[A2B[out_c]], 1) ^ rol64(NT_REV[A2B[in_c]], km1);
+    return h->f < h->r ? h->f : h->r;
+}
+void orc_nthash_rolling(const uint8_t* s, uint32_t len, uint32_t k, uint64_t* out) {
+    if (len < k || k == 0) return;
+    nth_t h = {k, orc_nthash_fwd(s, k), orc_nthash_rev(s, k)};
+    out[0] = h.f < h.r ? h.f : h.r;
+    for (uint32_t i = k; i < len; i++) out[i - k + 1] = nth_roll(&h, s[i - k], s[i]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* CountMinSketch, sketch.rs:121-254                                          */
+static const uint64_t CMS_SEEDS[4] = {0x243f6a8885a308d3ULL, 0x13198a2e03707344ULL,
+                                      0xa4093822299f31d0ULL, 0x082efa98ec4e6c89ULL};
+struct orc_cms {
+    uint64_t width, mask;
+    int bits;
+    uint8_t* nib;
+    uint16_t* u16;
+};
+
+orc_cms* orc_cms_new(uint64_t width, int bits) {
+    if (width == 0) width = 1;
+    if (width & (width - 1)) { /* next_power_of_two (sketch.rs:141-143) */
+        uint64_t p = 1;
+        while (p < width) p <<= 1;
+        width = p;
+    }
+    orc_cms* c = (orc_cms*)calloc(1, sizeof *c);
+    c->width = width;
+    c->mask = width - 1;
+    c->bits = bits;
+    if (bits == 4) c->nib = (uint8_t*)calloc(width * 4 / 2 + 1, 1);
+    else c->u16 = (uint16_t*)calloc(width * 4, 2);
+    return c;
+}
+void orc_cms_free(orc_cms* c) {
+    if (!c) return;
+    free(c->nib);
+    free(c->u16);
+    free(c);
+}
+uint64_t orc_cms_width(const orc_cms* c) { return c->width; }
+size_t orc_cms_table_bytes(const orc_cms* c) { return c->bits == 4 ? c->width * 2 : c->width * 8; }
+const void* orc_cms_table(const orc_cms* c) { return c->bits == 4 ? (const void*)c->nib : (const void*)c->u16; }
+uint64_t orc_cms_width_for_genome_size(uint64_t g) {
+    uint64_t t = g * 16, p = 1;
+    while (p < t) p <<= 1; /* next_power_of_two; 0 -> 1 */
+    if (p < (1ULL << 21)) p = 1ULL << 21;
+    if (p > (1ULL << 27)) p = 1ULL << 27;
+    return p;
+}
+
+/* sketch.rs:191-223; atomic variant so the same code serves the threaded baseline */
+FORCE_INLINE void cms_add(orc_cms* c, uint64_t key) {
+    for (int r = 0; r < 4; r++) {
+        uint64_t col = sm64(key ^ CMS_SEEDS[r]) & c->mask;
+        uint64_t idx = (uint64_t)r * c->width + col;
+        if (c->bits == 4) {
+            uint8_t* p = &c->nib[idx >> 1];
+            int hi = (int)(idx & 1);
+            uint8_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+            for (;;) {
+                uint8_t cnt = hi ? (cur >> 4) & 0x0F : cur & 0x0F;
+                if (cnt >= 15) break;
+                uint8_t nc = cnt + 1;
+                uint8_t nv = hi ? (uint8_t)((cur & 0x0F) | (nc << 4)) : (uint8_t)((cur & 0xF0) | nc);
+                if (__atomic_compare_exchange_n(p, &cur, nv, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) break;
+            }
+        } else {
+            uint16_t* p = &c->u16[idx];
+            uint16_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+            for (;;) {
+                if (cur == 0xFFFF) break;
+                if (__atomic_compare_exchange_n(p, &cur, (uint16_t)(cur + 1), 1, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED))
+                    break;
+            }
+        }
+    }
+}
+/* sketch.rs:227-246 (min starts at the counter maximum, early out at 0) */
+FORCE_INLINE uint32_t cms_est(const orc_cms* c, uint64_t key) {
+    uint32_t mn = c->bits == 4 ? 15u : 65535u;
+    for (int r = 0; r < 4; r++) {
+        uint64_t col = sm64(key ^ CMS_SEEDS[r]) & c->mask;
+        uint64_t idx = (uint64_t)r * c->width + col;
+        uint32_t cnt;
+        if (c->bits == 4) {
+            uint8_t cur = __atomic_load_n(&c->nib[idx >> 1], __ATOMIC_RELAXED);
+            cnt = (idx & 1) ? (cur >> 4) & 0x0F : cur & 0x0F;
+        } else {
+            cnt = __atomic_load_n(&c->u16[idx], __ATOMIC_RELAXED);
+        }
+        if (cnt < mn) mn = cnt;
+        if (mn == 0) return 0;
+    }
+    return mn;
+}
+void orc_cms_add(orc_cms* c, uint64_t key) { cms_add(c, key); }
+uint32_t orc_cms_estimate(const orc_cms* c, uint64_t key) { return cms_est(c, key); }
+uint32_t orc_cms_add_and_estimate(orc_cms* c, uint64_t key) {
+    cms_add(c, key);
+    return cms_est(c, key);
+}
+
+/* ------------------------------------------------------------------------- */
+/* HyperLogLog, sketch.rs:24-115, generalised to precision p                  */
+FORCE_INLINE void hll_add(uint8_t* regs, int p, uint64_t h) {
+    uint64_t idx = h >> (64 - p);
+    uint64_t w = h & ((1ULL << (64 - p)) - 1);
+    uint8_t rho = w == 0 ? (uint8_t)(64 - p + 1) : (uint8_t)(__builtin_clzll(w) - p + 1);
+    if (rho > regs[idx]) regs[idx] = rho;
+}
+void orc_hll_add_hash(uint8_t* regs, int p, uint64_t h) { hll_add(regs, p, h); }
+void orc_hll_merge(uint8_t* regs, const uint8_t* other, int p) {
+    size_t m = (size_t)1 << p;
+    for (size_t i = 0; i < m; i++)
+        if (other[i] > regs[i]) regs[i] = other[i];
+}
+double orc_hll_estimate(const uint8_t* regs, int p) {
+    size_t mi = (size_t)1 << p;
+    double m = (double)mi;
+    double alpha_m = 0.7213 / (1.0 + 1.079 / m);
+    double sum = 0.0;
+    size_t zeros = 0;
+    for (size_t i = 0; i < mi; i++) {
+        sum += ldexp(1.0, -(int)regs[i]); /* 2.0f64.powi(-v) is exact */
+        if (regs[i] == 0) zeros++;
+    }
+    double raw = alpha_m * m * m / sum;
+    if (raw <= 2.5 * m) {
+        if (zeros > 0) return m * log(m / (double)zeros);
+        return raw;
+    } else if (raw <= (1.0 / 30.0) * 18446744073709551616.0) {
+        return raw;
+    } else {
+        double two64 = 18446744073709551616.0;
+        return -two64 * log(1.0 - raw / two64);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Estimator                                                                  */
+uint64_t orc_f64_to_u64(double x) {
+    if (!(x > 0.0)) return 0; /* negative and NaN */
+    if (x >= 18446744073709551616.0) return UINT64_MAX;
+    return (uint64_t)x;
+}
+static uint64_t sat_sub(uint64_t a, uint64_t b) { return a > b ? a - b : 0; }
+
+/* estimator.rs:104-161 */
+uint32_t orc_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uint32_t* out) {
+    uint64_t min_k = median_len < 75 ? 21 : 31;
+    uint64_t effective_len = (median_len * 16) / 10;
+    uint64_t auto_max = orc_f64_to_u64((double)effective_len * 0.85);
+    uint64_t max_k = target_max_k >= 0 ? (uint64_t)target_max_k : auto_max;
+    if (max_k < min_k) max_k = min_k;
+    if (max_k > 127) max_k = 127;
+    if (max_k % 2 == 0) max_k -= 1;
+    uint32_t n = 0;
+    if (max_k <= min_k) {
+        out[n++] = (uint32_t)min_k;
+        return n;
+    }
+    if (median_len >= 100) {
+        uint64_t mid_k = 63;
+        if (max_k > mid_k + 10) {
+            out[n++] = (uint32_t)min_k;
+            out[n++] = (uint32_t)mid_k;
+            out[n++] = (uint32_t)max_k;
+        } else {
+            out[n++] = (uint32_t)min_k;
+            out[n++] = (uint32_t)max_k;
+        }
+        return n;
+    }
+    uint64_t step = (max_k - min_k) / 2;
+    if (step % 2 == 1) step += 1;
+    if (step < 6) step = 6;
+    for (uint64_t k = min_k; k <= max_k && n < 8; k += step) out[n++] = (uint32_t)k;
+    if (out[n - 1] != max_k && max_k > out[n - 1]) {
+        if (max_k - out[n - 1] >= 10) out[n++] = (uint32_t)max_k;
+        else out[n - 1] = (uint32_t)max_k;
+    }
+    /* dedup of consecutive equal values */
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < n; i++)
+        if (m == 0 || out[m - 1] != out[i]) out[m++] = out[i];
+    return m;
+}
+
+/* estimator.rs:167-283 restricted to what the lengths determine */
+int orc_count_stats_from_lengths(const uint32_t* l1, const uint32_t* l2, uint64_t n,
+                                 orc_count_stats* o) {
+    memset(o, 0, sizeof *o);
+    if (n == 0) return -1;
+    uint64_t* hist = (uint64_t*)calloc(4096, sizeof(uint64_t));
+    uint64_t* over = (uint64_t*)malloc(sizeof(uint64_t) * 2 * n);
+    uint64_t n_over = 0;
+    uint64_t mn = UINT64_MAX, mx = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t a = l1[i], b = l2[i];
+        o->total_bases_r1 += a;
+        o->total_bases_r2 += b;
+        if (a < mn) mn = a;
+        if (b < mn) mn = b;
+        if (a > mx) mx = a;
+        if (b > mx) mx = b;
+        if (a < 4096) hist[a]++; else over[n_over++] = a;
+        if (b < 4096) hist[b]++; else over[n_over++] = b;
+    }
+    o->exact_read_pairs = n;
+    o->total_bases = o->total_bases_r1 + o->total_bases_r2;
+    o->min_read_len = mn;
+    o->max_read_len = mx;
+    uint64_t total_reads = n * 2;
+    o->mean_read_len = (double)o->total_bases / (double)total_reads;
+    uint64_t target = (total_reads + 1) / 2, cum = 0, med = 0;
+    for (uint64_t len = 0; len < 4096; len++) {
+        cum += hist[len];
+        if (cum >= target) {
+            med = len;
+            break;
+        }
+    }
+    if (med == 0 && n_over > 0) {
+        /* sort_unstable then the middle element */
+        for (uint64_t i = 1; i < n_over; i++) {
+            uint64_t v = over[i], j = i;
+            while (j > 0 && over[j - 1] > v) {
+                over[j] = over[j - 1];
+                j--;
+            }
+            over[j] = v;
+        }
+        uint64_t idx = n_over / 2;
+        if (idx > n_over - 1) idx = n_over - 1;
+        med = over[idx];
+    }
+    o->median_read_len = med;
+    free(hist);
+    free(over);
+    return 0;
+}
+
+/* The sequential sketch loop body for one read (estimator.rs:319-339). */
+static void est_read(const orc_arena* a, uint64_t id, uint8_t* buf, orc_cms* cms, uint8_t* ht,
+                     uint8_t* hs, uint64_t* m_sample) {
+    const uint32_t k = 21;
+    uint32_t len = a->lengths[id];
+    if (len < k) return;
+    orc_arena_extract_ascii(a, id, buf);
+    *m_sample += len - k + 1;
+    nth_t h = {k, orc_nthash_fwd(buf, k), orc_nthash_rev(buf, k)};
+    uint64_t raw = h.f < h.r ? h.f : h.r;
+    for (uint32_t i = k;; i++) {
+        uint32_t c = orc_cms_add_and_estimate(cms, raw);
+        uint64_t mixed = sm64(raw);
+        hll_add(ht, 14, mixed);
+        if (c >= 2) hll_add(hs, 14, mixed);
+        if (i >= len) break;
+        raw = nth_roll(&h, buf[i - k], buf[i]);
+    }
+}
+
+static uint64_t arena_max_len(const orc_arena* a) {
+    uint64_t m = 0;
+    for (uint64_t i = 0; i < a->n_reads; i++)
+        if (a->lengths[i] > m) m = a->lengths[i];
+    return m;
+}
+
+void orc_estimate_finish(uint64_t u_ge_1, uint64_t u_ge_2, uint64_t m_sample,
+                         const orc_count_stats* st, double target_cov, orc_genome_estimate* o) {
+    const double k = 21.0;
+    uint64_t e_1 = sat_sub(u_ge_1, u_ge_2);
+    uint64_t m_true = sat_sub(m_sample, e_1);
+    double lambda_s = u_ge_2 > 0 ? (double)m_true / (double)u_ge_2 : 0.0;
+    double p_ge_2 = 1.0 - (1.0 + lambda_s) * exp(-lambda_s);
+    uint64_t est_g;
+    if (p_ge_2 > 0.05) est_g = orc_f64_to_u64(round((double)u_ge_2 / p_ge_2));
+    else est_g = u_ge_2 > 10000 ? u_ge_2 : 10000;
+    if (est_g < 10000) est_g = 10000;
+    double estimated_cov = (double)st->total_bases / (double)est_g;
+    double d_k = estimated_cov * ((st->mean_read_len - k + 1.0) / st->mean_read_len);
+    double solid_ratio = u_ge_1 > 0 ? (double)u_ge_2 / (double)u_ge_1 : 0.0;
+    int conf;
+    if (d_k < 5.0 || solid_ratio < 0.05 || u_ge_2 < 1000) conf = 2;
+    else if (d_k < 20.0 || solid_ratio < 0.15) conf = 1;
+    else conf = 0;
+    double sub = 1.0;
+    if (target_cov > 0.0 && estimated_cov > 1.5 * target_cov) {
+        sub = target_cov / estimated_cov;
+        if (sub < 0.01) sub = 0.01;
+        if (sub > 1.0) sub = 1.0;
+    }
+    o->genome_size = est_g;
+    o->exact_read_pairs = st->exact_read_pairs;
+    o->total_bases = st->total_bases;
+    o->median_read_len = st->median_read_len;
+    o->max_read_len = st->max_read_len;
+    o->estimated_cov = estimated_cov;
+    o->subsample_factor = sub;
+    o->n_kmers = orc_compute_dynamic_kmers(st->median_read_len, -1, o->kmers);
+    o->confidence = conf;
+    o->u_ge_1 = u_ge_1;
+    o->u_ge_2 = u_ge_2;
+    o->sample_kmer_depth = lambda_s;
+    o->m_sample = m_sample;
+}
+
+/* estimator.rs:286-440 */
+int orc_estimate_genome(const orc_arena* r1, const orc_arena* r2, const orc_count_stats* st,
+                        uint64_t max_pairs, double target_cov, orc_genome_estimate* o) {
+    memset(o, 0, sizeof *o);
+    uint64_t avail = r1->n_reads < r2->n_reads ? r1->n_reads : r2->n_reads;
+    uint64_t target = max_pairs;
+    if (target > 150000) target = 150000;
+    if (target > st->exact_read_pairs) target = st->exact_read_pairs;
+    uint8_t* ht = (uint8_t*)calloc(1 << 14, 1);
+    uint8_t* hs = (uint8_t*)calloc(1 << 14, 1);
+    orc_cms* cms = orc_cms_new(1ULL << 23, 4);
+    uint64_t ml = arena_max_len(r1), ml2 = arena_max_len(r2);
+    uint8_t* buf = (uint8_t*)malloc((ml > ml2 ? ml : ml2) + 1);
+    uint64_t m_sample = 0, pc = 0;
+    while (pc < target) {
+        if (pc >= avail) break; /* `_ => break` when a reader runs dry */
+        est_read(r1, pc, buf, cms, ht, hs, &m_sample);
+        est_read(r2, pc, buf, cms, ht, hs, &m_sample);
+        pc++;
+        if (pc >= 5000 && pc % 2000 == 0) {
+            uint64_t u1 = orc_f64_to_u64(round(orc_hll_estimate(ht, 14)));
+            uint64_t u2 = orc_f64_to_u64(round(orc_hll_estimate(hs, 14)));
+            uint64_t e1 = sat_sub(u1, u2);
+            uint64_t m_tr = sat_sub(m_sample, e1);
+            double lam = u2 > 0 ? (double)m_tr / (double)u2 : 0.0;
+            if (lam >= 15.0) break;
+        }
+    }
+    int rc = 0;
+    if (pc == 0) {
+        rc = -1;
+    } else {
+        uint64_t u1 = orc_f64_to_u64(round(orc_hll_estimate(ht, 14)));
+        uint64_t u2 = orc_f64_to_u64(round(orc_hll_estimate(hs, 14)));
+        orc_estimate_finish(u1, u2, m_sample, st, target_cov, o);
+        o->pairs_sampled = pc;
+    }
+    free(buf);
+    orc_cms_free(cms);
+    free(ht);
+    free(hs);
+    return rc;
+}
+
+void orc_estimator_sketch(const orc_arena* r1, const orc_arena* r2, uint64_t n_pairs, uint8_t* ht,
+                          uint8_t* hs, uint64_t* m_sample) {
+    orc_cms* cms = orc_cms_new(1ULL << 23, 4);
+    uint64_t ml = arena_max_len(r1), ml2 = arena_max_len(r2);
+    uint8_t* buf = (uint8_t*)malloc((ml > ml2 ? ml : ml2) + 1);
+    *m_sample = 0;
+    for (uint64_t pc = 0; pc < n_pairs; pc++) {
+        est_read(r1, pc, buf, cms, ht, hs, m_sample);
+        est_read(r2, pc, buf, cms, ht, hs, m_sample);
+    }
+    free(buf);
+    orc_cms_free(cms);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Solid-set accumulator: open addressing over keys of nw words.             */
+typedef struct {
+    uint16_t cov;
+    uint16_t edge[8];
+} acc_t;
+
+typedef struct {
+    uint64_t cap, n;
+    int nw;
+    uint64_t* keys;
+    acc_t* vals;
+    uint8_t* used;
+} accmap_t;
+
+static void am_init(accmap_t* m, int nw, uint64_t cap) {
+    uint64_t c = 1024;
+    while (c < cap) c <<= 1;
+    m->cap = c;
+    m->n = 0;
+    m->nw = nw;
+    m->keys = (uint64_t*)malloc(c * nw * 8);
+    m->vals = (acc_t*)calloc(c, sizeof(acc_t));
+    m->used = (uint8_t*)calloc(c, 1);
+}
+static void am_free(accmap_t* m) {
+    free(m->keys);
+    free(m->vals);
+    free(m->used);
+    memset(m, 0, sizeof *m);
+}
+FORCE_INLINE uint64_t am_slot_hash(const km_t* key, int nw) { return sm64(km_fold(key, nw) ^ 0x9e3779b97f4a7c15ULL); }
+static acc_t* am_entry(accmap_t* m, const km_t* key);
+static void am_grow(accmap_t* m) {
+    accmap_t n2;
+    am_init(&n2, m->nw, m->cap * 2);
+    for (uint64_t i = 0; i < m->cap; i++) {
+        if (!m->used[i]) continue;
+        km_t k = km_load(m->keys + i * m->nw, m->nw);
+        *am_entry(&n2, &k) = m->vals[i];
+    }
+    am_free(m);
+    *m = n2;
+}
+static acc_t* am_entry(accmap_t* m, const km_t* key) {
+    if ((m->n + 1) * 2 > m->cap) am_grow(m);
+    const int nw = m->nw;
+    uint64_t s = am_slot_hash(key, nw) & (m->cap - 1);
+    for (;;) {
+        if (!m->used[s]) {
+            m->used[s] = 1;
+            memcpy(m->keys + s * nw, key->w, (size_t)nw * 8);
+            m->n++;
+            return &m->vals[s];
+        }
+        if (memcmp(m->keys + s * nw, key->w, (size_t)nw * 8) == 0) return &m->vals[s];
+        s = (s + 1) & (m->cap - 1);
+    }
+}
+FORCE_INLINE uint16_t sat_add16(uint16_t a, uint32_t b) {
+    uint32_t s = (uint32_t)a + b;
+    return s > 65535 ? 65535 : (uint16_t)s;
+}
+/* build.rs:183-187, 218-231 */
+FORCE_INLINE void acc_push(accmap_t* m, const km_t* key, uint8_t edge, uint32_t cov_w, uint32_t edge_w) {
+    acc_t* a = am_entry(m, key);
+    a->cov = sat_add16(a->cov, cov_w);
+    a->edge[edge] = sat_add16(a->edge[edge], edge_w);
+}
+
+void orc_build_params_default(orc_build_params* p, int target, uint64_t width) {
+    p->width = width;
+    p->counter_bits = 4;
+    p->t_solid = target >= 0 ? target : 2;
+    p->min_trans_cov = target >= 0 ? target : 3;
+    p->hll_p = 0;
+}
+
+static uint64_t arenas_max_len(const orc_arena* arenas, int n) {
+    uint64_t m = 0;
+    for (int i = 0; i < n; i++) {
+        uint64_t x = arena_max_len(&arenas[i]);
+        if (x > m) m = x;
+    }
+    return m;
+}
+
+/* Pass 1 for one read (build.rs:97-113) plus the per-pass HLL over
+ * splitmix64(canonical ntHash) of the same windows. */
+FORCE_INLINE void pass1_read(const uint8_t* seq, uint32_t len, uint32_t k, int nw, orc_cms* cms,
+                             uint8_t* hll, int hp) {
+    km_t fwd = km_from_ascii(seq, k, nw);
+    km_t rev = km_rev_comp(fwd, k, nw);
+    const km_t* can = km_lt(&rev, &fwd, nw) ? &rev : &fwd;
+    cms_add(cms, km_fold(can, nw));
+    for (uint32_t i = k; i < len; i++) {
+        km_roll(&fwd, &rev, A2B[seq[i]], k, nw);
+        can = km_lt(&rev, &fwd, nw) ? &rev : &fwd;
+        cms_add(cms, km_fold(can, nw));
+    }
+    if (hll) {
+        nth_t h = {k, orc_nthash_fwd(seq, k), orc_nthash_rev(seq, k)};
+        hll_add(hll, hp, sm64(h.f < h.r ? h.f : h.r));
+        for (uint32_t i = k; i < len; i++) hll_add(hll, hp, sm64(nth_roll(&h, seq[i - k], seq[i])));
+    }
+}
+
+/* Contig weight in pass 1 (build.rs:119-138): every k-mer added 10 times. */
+static void pass1_contigs(const uint8_t* contigs, const uint32_t* lens, uint64_t n, uint32_t k, int nw,
+                          orc_cms* cms) {
+    const uint8_t* c = contigs;
+    for (uint64_t ci = 0; ci < n; ci++) {
+        uint32_t len = lens[ci];
+        if (len >= k) {
+            km_t fwd = km_from_ascii(c, k, nw);
+            km_t rev = km_rev_comp(fwd, k, nw);
+            const km_t* can = km_lt(&rev, &fwd, nw) ? &rev : &fwd;
+            for (int t = 0; t < 10; t++) cms_add(cms, km_fold(can, nw));
+            for (uint32_t i = k; i < len; i++) {
+                km_roll(&fwd, &rev, A2B[c[i]], k, nw);
+                can = km_lt(&rev, &fwd, nw) ? &rev : &fwd;
+                for (int t = 0; t < 10; t++) cms_add(cms, km_fold(can, nw));
+            }
+        }
+        c += len;
+    }
+}
+
+/* Pass 2 contig backbone (build.rs:203-240) */
+static void pass2_contigs(const uint8_t* contigs, const uint32_t* lens, uint64_t n, uint32_t k, int nw,
+                          void (*push)(void*, const km_t*, uint8_t, uint32_t, uint32_t), void* ctx) {
+    const uint8_t* c = contigs;
+    for (uint64_t ci = 0; ci < n; ci++) {
+        uint32_t len = lens[ci];
+        if (len >= k + 1) {
+            km_t uf = km_from_ascii(c, k, nw);
+            km_t ur = km_rev_comp(uf, k, nw);
+            int u_rc = km_lt(&ur, &uf, nw);
+            for (uint32_t i = 0; i + k + 1 <= len; i++) {
+                uint8_t base = A2B[c[i + k]];
+                km_t vf = uf, vr = ur;
+                km_roll(&vf, &vr, base, k, nw);
+                int v_rc = km_lt(&vr, &vf, nw);
+                uint8_t edge_u = !u_rc ? base : (uint8_t)(base + 4);
+                push(ctx, u_rc ? &ur : &uf, edge_u, 100, 1000);
+                uint8_t rc_p = (uint8_t)(3 - A2B[c[i]]);
+                uint8_t edge_v = !v_rc ? (uint8_t)(rc_p + 4) : rc_p;
+                push(ctx, v_rc ? &vr : &vf, edge_v, 100, 1000);
+                uf = vf;
+                ur = vr;
+                u_rc = v_rc;
+            }
+        }
+        c += len;
+    }
+}
+static void push_single(void* ctx, const km_t* key, uint8_t e, uint32_t cw, uint32_t ew) {
+    acc_push((accmap_t*)ctx, key, e, cw, ew);
+}
+
+/* Pass 2 for one read (build.rs:253-285); emit(u, edge) for both records. */
+#define PASS2_READ(seq, len, k, nw, cms, t_solid, EMIT)                                   \
+    do {                                                                                 \
+        km_t uf = km_from_ascii(seq, k, nw);                                             \
+        km_t ur = km_rev_comp(uf, k, nw);                                                \
+        int u_rc = km_lt(&ur, &uf, nw);                                                  \
+        uint32_t u_cnt = cms_est(cms, km_fold(u_rc ? &ur : &uf, nw));                    \
+        for (uint32_t i = 0; i + k + 1 <= (len); i++) {                                  \
+            uint8_t base = A2B[seq[i + k]];                                              \
+            km_t vf = uf, vr = ur;                                                       \
+            km_roll(&vf, &vr, base, k, nw);                                              \
+            int v_rc = km_lt(&vr, &vf, nw);                                              \
+            uint32_t v_cnt = cms_est(cms, km_fold(v_rc ? &vr : &vf, nw));                \
+            if (u_cnt >= (uint32_t)(t_solid) && v_cnt >= (uint32_t)(t_solid)) {          \
+                uint8_t edge_u = !u_rc ? base : (uint8_t)(base + 4);                     \
+                EMIT(u_rc ? &ur : &uf, edge_u);                                          \
+                uint8_t rc_p = (uint8_t)(3 - A2B[seq[i]]);                               \
+                uint8_t edge_v = !v_rc ? (uint8_t)(rc_p + 4) : rc_p;                     \
+                EMIT(v_rc ? &vr : &vf, edge_v);                                          \
+            }                                                                            \
+            uf = vf;                                                                     \
+            ur = vr;                                                                     \
+            u_rc = v_rc;                                                                 \
+            u_cnt = v_cnt;                                                               \
+        }                                                                                \
+    } while (0)
+
+/* Pass 3 (build.rs:302-328) into a record list. */
+typedef struct {
+    uint64_t n, cap;
+    int nw;
+    uint64_t* keys;
+    uint16_t* cov;
+    uint8_t* mask;
+} rec_list;
+static void rl_push(rec_list* r, const uint64_t* key, uint16_t cov, uint8_t mask) {
+    if (r->n == r->cap) {
+        r->cap = r->cap ? r->cap * 2 : 1024;
+        r->keys = (uint64_t*)realloc(r->keys, r->cap * r->nw * 8);
+        r->cov = (uint16_t*)realloc(r->cov, r->cap * 2);
+        r->mask = (uint8_t*)realloc(r->mask, r->cap);
+    }
+    memcpy(r->keys + r->n * r->nw, key, (size_t)r->nw * 8);
+    r->cov[r->n] = cov;
+    r->mask[r->n] = mask;
+    r->n++;
+}
+static void pass3_map(const accmap_t* m, int min_trans_cov, rec_list* out) {
+    for (uint64_t i = 0; i < m->cap; i++) {
+        if (!m->used[i]) continue;
+        const acc_t* a = &m->vals[i];
+        uint8_t mask = 0;
+        for (int e = 0; e < 8; e++)
+            if (a->edge[e] >= (uint16_t)min_trans_cov) mask |= (uint8_t)(1u << e);
+        if (mask != 0) {
+            uint16_t cov = (uint16_t)(a->cov / 2);
+            if (cov < 1) cov = 1;
+            rl_push(out, m->keys + i * m->nw, cov, mask);
+        }
+    }
+}
+
+/* Sort the record list by k-mer (parity form). */
+static int memcmp_words(const uint64_t* x, const uint64_t* y, int n) {
+    for (int w = 0; w < n; w++)
+        if (x[w] != y[w]) return x[w] < y[w] ? -1 : 1;
+    return 0;
+}
+typedef struct { const uint64_t* keys; int nw; } sort_ctx;
+static int cmp_idx_r(const void* a, const void* b, void* cp) {
+    const sort_ctx* c = (const sort_ctx*)cp;
+    uint64_t i = *(const uint64_t*)a, j = *(const uint64_t*)b;
+    return memcmp_words(c->keys + i * c->nw, c->keys + j * c->nw, c->nw);
+}
+static void rl_to_sorted_set(rec_list* r, orc_solid_set* out) {
+    uint64_t n = r->n;
+    int nw = r->nw;
+    uint64_t* idx = (uint64_t*)malloc((n ? n : 1) * 8);
+    for (uint64_t i = 0; i < n; i++) idx[i] = i;
+    sort_ctx sc = {r->keys, nw};
+    qsort_r(idx, n, 8, cmp_idx_r, &sc);
+    out->n = n;
+    out->nw = nw;
+    out->keys = (uint64_t*)malloc((n ? n : 1) * nw * 8);
+    out->cov = (uint16_t*)malloc((n ? n : 1) * 2);
+    out->mask = (uint8_t*)malloc(n ? n : 1);
+    for (uint64_t i = 0; i < n; i++) {
+        memcpy(out->keys + i * nw, r->keys + idx[i] * nw, (size_t)nw * 8);
+        out->cov[i] = r->cov[idx[i]];
+        out->mask[i] = r->mask[idx[i]];
+    }
+    free(idx);
+    free(r->keys);
+    free(r->cov);
+    free(r->mask);
+    r->keys = NULL;
+    r->cov = NULL;
+    r->mask = NULL;
+    r->n = r->cap = 0;
+}
+
+void orc_solid_free(orc_solid_set* s) {
+    free(s->keys);
+    free(s->cov);
+    free(s->mask);
+    memset(s, 0, sizeof *s);
+}
+
+static int check_k(uint32_t k) { return k >= 1 && k <= 127; }
+
+int orc_build_solid(const orc_arena* arenas, int n_arenas, const uint8_t* contigs,
+                    const uint32_t* contig_lens, uint64_t n_contigs, uint32_t k,
+                    const orc_build_params* p, orc_cms** cms_out, uint8_t* hll_out, orc_solid_set* out) {
+    if (!check_k(k)) return -1;
+    const int nw = orc_kmer_words_for_k(k);
+    orc_cms* cms = orc_cms_new(p->width, p->counter_bits);
+    uint8_t* buf = (uint8_t*)malloc(arenas_max_len(arenas, n_arenas) + 1);
+    if (p->hll_p <= 0) hll_out = NULL;
+    /* pass 1 */
+    for (int a = 0; a < n_arenas; a++) {
+        const orc_arena* ar = &arenas[a];
+        for (uint64_t id = 0; id < ar->n_reads; id++) {
+            if (ar->lengths[id] < k) continue;
+            orc_arena_extract_ascii(ar, id, buf);
+            switch (nw) {
+            case 1: pass1_read(buf, ar->lengths[id], k, 1, cms, hll_out, p->hll_p); break;
+            case 2: pass1_read(buf, ar->lengths[id], k, 2, cms, hll_out, p->hll_p); break;
+            default: pass1_read(buf, ar->lengths[id], k, 4, cms, hll_out, p->hll_p); break;
+            }
+        }
+    }
+    if (n_contigs) pass1_contigs(contigs, contig_lens, n_contigs, k, nw, cms);
+    /* pass 2 */
+    accmap_t m;
+    am_init(&m, nw, 1 << 16);
+    if (n_contigs) pass2_contigs(contigs, contig_lens, n_contigs, k, nw, push_single, &m);
+#define EMIT_SINGLE(key, e) acc_push(&m, (key), (e), 1, 1)
+    for (int a = 0; a < n_arenas; a++) {
+        const orc_arena* ar = &arenas[a];
+        for (uint64_t id = 0; id < ar->n_reads; id++) {
+            uint32_t len = ar->lengths[id];
+            if (len < k + 1) continue;
+            orc_arena_extract_ascii(ar, id, buf);
+            switch (nw) {
+            case 1: PASS2_READ(buf, len, k, 1, cms, p->t_solid, EMIT_SINGLE); break;
+            case 2: PASS2_READ(buf, len, k, 2, cms, p->t_solid, EMIT_SINGLE); break;
+            default: PASS2_READ(buf, len, k, 4, cms, p->t_solid, EMIT_SINGLE); break;
+            }
+        }
+    }
+#undef EMIT_SINGLE
+    /* pass 3 */
+    rec_list rl = {0, 0, nw, NULL, NULL, NULL};
+    pass3_map(&m, p->min_trans_cov, &rl);
+    am_free(&m);
+    rl_to_sorted_set(&rl, out);
+    free(buf);
+    if (cms_out) *cms_out = cms;
+    else orc_cms_free(cms);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Reference-shaped threaded form (build.rs:93-116, 143-200, 243-289, 303-328) */
+#define NUM_SHARDS 64
+#define TBUF_CAP 2048
+#define P2_CHUNK 4096
+#define P1_CHUNK 256
+
+typedef struct {
+    pthread_mutex_t mu;
+    accmap_t map;
+} shard_t;
+
+typedef struct {
+    const orc_arena* arenas;
+    int n_arenas;
+    uint32_t k;
+    int nw;
+    const orc_build_params* p;
+    orc_cms* cms;
+    shard_t* shards;
+    uint64_t maxlen;
+    _Atomic uint64_t next;
+    pthread_barrier_t bar;
+    int nthreads;
+    uint8_t** thread_hll;
+    rec_list* shard_out;
+    _Atomic int next_shard;
+} mt_ctx;
+
+typedef struct {
+    km_t key[TBUF_CAP];
+    uint8_t edge[TBUF_CAP];
+    int n;
+} tbuf_t;
+
+/* ThreadBuffer::flush_shard (build.rs:177-189) */
+static void tb_flush(mt_ctx* c, tbuf_t* tb, int s) {
+    if (tb->n == 0) return;
+    pthread_mutex_lock(&c->shards[s].mu);
+    for (int i = 0; i < tb->n; i++) acc_push(&c->shards[s].map, &tb->key[i], tb->edge[i], 1, 1);
+    pthread_mutex_unlock(&c->shards[s].mu);
+    tb->n = 0;
+}
+
+typedef struct {
+    mt_ctx* c;
+    int tid;
+} mt_arg;
+
+/* All threads drain one arena, then meet; thread 0 resets the cursor. */
+static void mt_arena_done(mt_ctx* c, int tid) {
+    pthread_barrier_wait(&c->bar);
+    if (tid == 0) atomic_store(&c->next, 0);
+    pthread_barrier_wait(&c->bar);
+}
+
+static void* mt_worker(void* argp) {
+    mt_arg* arg = (mt_arg*)argp;
+    mt_ctx* c = arg->c;
+    const uint32_t k = c->k;
+    const int nw = c->nw;
+    uint8_t* buf = (uint8_t*)malloc(c->maxlen + 1);
+    uint8_t* hll = c->thread_hll ? c->thread_hll[arg->tid] : NULL;
+    /* pass 1: each arena in turn, dynamic chunks of reads */
+    for (int a = 0; a < c->n_arenas; a++) {
+        const orc_arena* ar = &c->arenas[a];
+        for (;;) {
+            uint64_t start = atomic_fetch_add(&c->next, P1_CHUNK);
+            if (start >= ar->n_reads) break;
+            uint64_t end = start + P1_CHUNK < ar->n_reads ? start + P1_CHUNK : ar->n_reads;
+            for (uint64_t id = start; id < end; id++) {
+                if (ar->lengths[id] < k) continue;
+                orc_arena_extract_ascii(ar, id, buf);
+                switch (nw) {
+                case 1: pass1_read(buf, ar->lengths[id], k, 1, c->cms, hll, c->p->hll_p); break;
+                case 2: pass1_read(buf, ar->lengths[id], k, 2, c->cms, hll, c->p->hll_p); break;
+                default: pass1_read(buf, ar->lengths[id], k, 4, c->cms, hll, c->p->hll_p); break;
+                }
+            }
+        }
+        mt_arena_done(c, arg->tid);
+    }
+    /* pass 2: 4096-read chunks, one thread buffer per chunk, flush_all at chunk end */
+    tbuf_t* tbs = (tbuf_t*)malloc(sizeof(tbuf_t) * NUM_SHARDS);
+    for (int a = 0; a < c->n_arenas; a++) {
+        const orc_arena* ar = &c->arenas[a];
+        for (;;) {
+            uint64_t start = atomic_fetch_add(&c->next, P2_CHUNK);
+            if (start >= ar->n_reads) break;
+            uint64_t end = start + P2_CHUNK < ar->n_reads ? start + P2_CHUNK : ar->n_reads;
+            for (int s = 0; s < NUM_SHARDS; s++) tbs[s].n = 0;
+#define EMIT_MT(keyp, e)                                                      \
+    do {                                                                      \
+        const km_t* kp_ = (keyp);                                             \
+        int s_ = (int)(sm64(km_fold(kp_, nw)) & (NUM_SHARDS - 1));            \
+        tbs[s_].key[tbs[s_].n] = *kp_;                                        \
+        tbs[s_].edge[tbs[s_].n] = (e);                                        \
+        if (++tbs[s_].n >= TBUF_CAP) tb_flush(c, &tbs[s_], s_);               \
+    } while (0)
+            for (uint64_t id = start; id < end; id++) {
+                uint32_t len = ar->lengths[id];
+                if (len < k + 1) continue;
+                orc_arena_extract_ascii(ar, id, buf);
+                switch (nw) {
+                case 1: PASS2_READ(buf, len, k, 1, c->cms, c->p->t_solid, EMIT_MT); break;
+                case 2: PASS2_READ(buf, len, k, 2, c->cms, c->p->t_solid, EMIT_MT); break;
+                default: PASS2_READ(buf, len, k, 4, c->cms, c->p->t_solid, EMIT_MT); break;
+                }
+            }
+#undef EMIT_MT
+            for (int s = 0; s < NUM_SHARDS; s++) tb_flush(c, &tbs[s], s);
+        }
+        mt_arena_done(c, arg->tid);
+    }
+    free(tbs);
+    /* pass 3: per shard, each shard's records sorted; merged by the caller */
+    for (;;) {
+        int s = atomic_fetch_add(&c->next_shard, 1);
+        if (s >= NUM_SHARDS) break;
+        rec_list* r = &c->shard_out[s];
+        pass3_map(&c->shards[s].map, c->p->min_trans_cov, r);
+        am_free(&c->shards[s].map);
+        orc_solid_set tmp;
+        rl_to_sorted_set(r, &tmp);
+        r->n = tmp.n;
+        r->cap = tmp.n;
+        r->keys = tmp.keys;
+        r->cov = tmp.cov;
+        r->mask = tmp.mask;
+    }
+    free(buf);
+    return NULL;
+}
+
+static void heap_sift(int* heap, int hn, int j, const rec_list* sh, const uint64_t* pos, int nw) {
+    for (;;) {
+        int l = 2 * j + 1, r = l + 1, m = j;
+        if (l < hn && memcmp_words(sh[heap[l]].keys + pos[heap[l]] * nw,
+                                   sh[heap[m]].keys + pos[heap[m]] * nw, nw) < 0)
+            m = l;
+        if (r < hn && memcmp_words(sh[heap[r]].keys + pos[heap[r]] * nw,
+                                   sh[heap[m]].keys + pos[heap[m]] * nw, nw) < 0)
+            m = r;
+        if (m == j) break;
+        int t = heap[j];
+        heap[j] = heap[m];
+        heap[m] = t;
+        j = m;
+    }
+}
+
+/* k-way merge of the sorted shard lists into one sorted set */
+static void merge_sorted_shards(const rec_list* sh, int nsh, int nw, orc_solid_set* out) {
+    uint64_t total = 0;
+    for (int s = 0; s < nsh; s++) total += sh[s].n;
+    out->n = total;
+    out->nw = nw;
+    out->keys = (uint64_t*)malloc((total ? total : 1) * nw * 8);
+    out->cov = (uint16_t*)malloc((total ? total : 1) * 2);
+    out->mask = (uint8_t*)malloc(total ? total : 1);
+    int heap[NUM_SHARDS];
+    uint64_t pos[NUM_SHARDS];
+    int hn = 0;
+    for (int s = 0; s < nsh; s++) {
+        pos[s] = 0;
+        if (sh[s].n) heap[hn++] = s;
+    }
+    for (int i = hn / 2 - 1; i >= 0; i--) heap_sift(heap, hn, i, sh, pos, nw);
+    uint64_t o = 0;
+    while (hn > 0) {
+        int s = heap[0];
+        memcpy(out->keys + o * nw, sh[s].keys + pos[s] * nw, (size_t)nw * 8);
+        out->cov[o] = sh[s].cov[pos[s]];
+        out->mask[o] = sh[s].mask[pos[s]];
+        o++;
+        pos[s]++;
+        if (pos[s] >= sh[s].n) heap[0] = heap[--hn];
+        heap_sift(heap, hn, 0, sh, pos, nw);
+    }
+}
+
+int orc_build_solid_mt(const orc_arena* arenas, int n_arenas, const uint8_t* contigs,
+                       const uint32_t* contig_lens, uint64_t n_contigs, uint32_t k,
+                       const orc_build_params* p, int threads, orc_cms** cms_out, uint8_t* hll_out,
+                       orc_solid_set* out) {
+    if (!check_k(k)) return -1;
+    if (threads < 1) threads = 1;
+    mt_ctx c;
+    memset(&c, 0, sizeof c);
+    c.arenas = arenas;
+    c.n_arenas = n_arenas;
+    c.k = k;
+    c.nw = orc_kmer_words_for_k(k);
+    c.p = p;
+    c.cms = orc_cms_new(p->width, p->counter_bits);
+    c.maxlen = arenas_max_len(arenas, n_arenas);
+    c.nthreads = threads;
+    c.shards = (shard_t*)calloc(NUM_SHARDS, sizeof(shard_t));
+    c.shard_out = (rec_list*)calloc(NUM_SHARDS, sizeof(rec_list));
+    for (int s = 0; s < NUM_SHARDS; s++) {
+        pthread_mutex_init(&c.shards[s].mu, NULL);
+        am_init(&c.shards[s].map, c.nw, 1 << 12);
+        c.shard_out[s].nw = c.nw;
+    }
+    if (hll_out && p->hll_p > 0) {
+        c.thread_hll = (uint8_t**)calloc(threads, sizeof(uint8_t*));
+        for (int t = 0; t < threads; t++) c.thread_hll[t] = (uint8_t*)calloc((size_t)1 << p->hll_p, 1);
+    }
+    atomic_store(&c.next, 0);
+    atomic_store(&c.next_shard, 0);
+    pthread_barrier_init(&c.bar, NULL, (unsigned)threads);
+    /* Contigs are serial in the reference (build.rs:119,203).  Their pass-1
+     * adds only raise CMS counters and their pass-2 records do not read the
+     * CMS, so both can run before the threaded read passes: saturating adds
+     * commute, and the read passes see the same final CMS. */
+    if (n_contigs) {
+        pass1_contigs(contigs, contig_lens, n_contigs, k, c.nw, c.cms);
+        accmap_t tmp;
+        am_init(&tmp, c.nw, 1 << 12);
+        pass2_contigs(contigs, contig_lens, n_contigs, k, c.nw, push_single, &tmp);
+        for (uint64_t i = 0; i < tmp.cap; i++) {
+            if (!tmp.used[i]) continue;
+            km_t key = km_load(tmp.keys + i * c.nw, c.nw);
+            int s = (int)(sm64(km_fold(&key, c.nw)) & (NUM_SHARDS - 1));
+            acc_t* a = am_entry(&c.shards[s].map, &key);
+            a->cov = sat_add16(a->cov, tmp.vals[i].cov);
+            for (int e = 0; e < 8; e++) a->edge[e] = sat_add16(a->edge[e], tmp.vals[i].edge[e]);
+        }
+        am_free(&tmp);
+    }
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+    mt_arg* args = (mt_arg*)malloc(sizeof(mt_arg) * threads);
+    for (int t = 0; t < threads; t++) {
+        args[t].c = &c;
+        args[t].tid = t;
+        pthread_create(&th[t], NULL, mt_worker, &args[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    pthread_barrier_destroy(&c.bar);
+    merge_sorted_shards(c.shard_out, NUM_SHARDS, c.nw, out);
+    for (int s = 0; s < NUM_SHARDS; s++) {
+        free(c.shard_out[s].keys);
+        free(c.shard_out[s].cov);
+        free(c.shard_out[s].mask);
+        pthread_mutex_destroy(&c.shards[s].mu);
+    }
+    if (c.thread_hll) {
+        memset(hll_out, 0, (size_t)1 << p->hll_p);
+        for (int t = 0; t < threads; t++) {
+            orc_hll_merge(hll_out, c.thread_hll[t], p->hll_p);
+            free(c.thread_hll[t]);
+        }
+        free(c.thread_hll);
+    }
+    free(c.shards);
+    free(c.shard_out);
+    free(th);
+    free(args);
+    if (cms_out) *cms_out = c.cms;
+    else orc_cms_free(c.cms);
+    return 0;
+}

@@ -1,0 +1,175 @@
+/*
+ * fb_oracle.h -- CPU restatement of fastBact's k-mer spectrum path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the timed
+ * CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it.  The product (fastbact_amd, libfastbact_gpu.so) never
+ * links, loads or calls it.
+ *
+ * Every function restates the reference algorithm it names (paths relative
+ * to the fastBact source tree).  Pinning of each function is listed in the
+ * header comment of fb_oracle.c.
+ */
+#ifndef FB_ORACLE_H
+#define FB_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/common/sketch.rs:13-18 */
+uint64_t orc_splitmix64(uint64_t z);
+/* src/common/bitseq.rs:120-138 */
+uint64_t orc_rev_comp_word(uint64_t word, uint32_t num_bases);
+/* src/common/bitseq.rs:67-81 */
+uint8_t orc_ascii_to_2bit(uint8_t c);
+
+/* One BitSeqArena (src/common/bitseq.rs:145-149): data packs base g of the
+ * concatenated stream at bits 2*(g&31) of data[g>>5]. */
+typedef struct {
+    const uint64_t* data;
+    const uint64_t* offsets;
+    const uint32_t* lengths;
+    uint64_t n_reads;
+} orc_arena;
+
+/* Packs ASCII sequences into an arena exactly as BitSeqArena::push_seq
+ * (src/common/bitseq.rs:195-222).  data must hold ceil(total/32) zeroed words. */
+void orc_arena_pack(const uint8_t* ascii_concat, const uint32_t* lengths, uint64_t n,
+                    uint64_t* data, uint64_t* offsets);
+/* BitSeqArena::extract_ascii_into (src/common/bitseq.rs:286-299). */
+void orc_arena_extract_ascii(const orc_arena* a, uint64_t id, uint8_t* out);
+
+/* ---- KmerInt (src/common/kmer.rs:18-335) -----------------------------------
+ * nw selects the reference type: 1 = u64 (k<=31), 2 = u128 (k<=63),
+ * 4 = [u64;4] (k<=127).  Words are most-significant first: nw=2 stores
+ * {hi, lo}; nw=4 stores words[0..4] as the reference does. */
+int  orc_kmer_words_for_k(uint32_t k);
+void orc_kmer_from_ascii(const uint8_t* ascii, uint32_t k, int nw, uint64_t* out);
+void orc_kmer_rev_comp(const uint64_t* x, uint32_t k, int nw, uint64_t* out);
+void orc_kmer_roll_next(const uint64_t* fwd, const uint64_t* rev, uint8_t base, uint32_t k,
+                        int nw, uint64_t* out_fwd, uint64_t* out_rev);
+uint64_t orc_kmer_to_u64(const uint64_t* x, int nw);
+/* Canonical k-mers of every window of an ASCII sequence by the reference's
+ * rolling loop (src/graph/build.rs:100-112): out[w*nw ..], is_rc[w]. */
+void orc_kmer_canonical_windows(const uint8_t* ascii, uint32_t len, uint32_t k, int nw,
+                                uint64_t* out, uint8_t* is_rc);
+
+/* ---- ntHash (src/common/kmer.rs:385-477) ---------------------------------- */
+uint64_t orc_nthash_fwd(const uint8_t* ascii, uint32_t k);
+uint64_t orc_nthash_rev(const uint8_t* ascii, uint32_t k);
+uint64_t orc_nthash_canonical(const uint8_t* ascii, uint32_t k);
+/* Rolling canonical ntHash over every window (RollingNtHash::new + roll). */
+void orc_nthash_rolling(const uint8_t* ascii, uint32_t len, uint32_t k, uint64_t* out);
+
+/* ---- CountMinSketch (src/common/sketch.rs:121-254) -------------------------
+ * counter_bits 4 is the reference layout (two nibbles per byte, low nibble =
+ * even counter); counter_bits 16 is the same sketch with saturating u16
+ * counters.  Table layout: counter idx = row*width + col. */
+typedef struct orc_cms orc_cms;
+orc_cms* orc_cms_new(uint64_t width, int counter_bits);
+void     orc_cms_free(orc_cms* c);
+uint64_t orc_cms_width(const orc_cms* c);
+void     orc_cms_add(orc_cms* c, uint64_t key);
+uint32_t orc_cms_estimate(const orc_cms* c, uint64_t key);
+uint32_t orc_cms_add_and_estimate(orc_cms* c, uint64_t key);
+/* Raw table bytes: width*4/2 (nibble) or width*4*2 (u16). */
+size_t   orc_cms_table_bytes(const orc_cms* c);
+const void* orc_cms_table(const orc_cms* c);
+/* sketch.rs:165-168 */
+uint64_t orc_cms_width_for_genome_size(uint64_t g);
+
+/* ---- HyperLogLog (src/common/sketch.rs:24-115), precision p ---------------- */
+void   orc_hll_add_hash(uint8_t* regs, int p, uint64_t h);
+void   orc_hll_merge(uint8_t* regs, const uint8_t* other, int p);
+double orc_hll_estimate(const uint8_t* regs, int p);
+
+/* ---- Genome estimator (src/ingest/estimator.rs:104-161,167-283,286-440) --- */
+typedef struct {
+    uint64_t exact_read_pairs;
+    uint64_t total_bases_r1, total_bases_r2, total_bases;
+    uint64_t min_read_len, max_read_len;
+    double   mean_read_len;
+    uint64_t median_read_len;
+} orc_count_stats;
+
+typedef struct {
+    uint64_t genome_size;
+    uint64_t exact_read_pairs;
+    uint64_t total_bases;
+    uint64_t median_read_len;
+    uint64_t max_read_len;
+    double   estimated_cov;
+    double   subsample_factor;
+    uint32_t kmers[8];
+    uint32_t n_kmers;
+    int32_t  confidence;        /* 0 High, 1 LowCoverageWarning, 2 Unreliable */
+    uint64_t u_ge_1, u_ge_2;
+    double   sample_kmer_depth;
+    uint64_t pairs_sampled;     /* pairs consumed before the loop ended */
+    uint64_t m_sample;
+} orc_genome_estimate;
+
+/* Length statistics of count_fastx_exact from per-read lengths.  Returns 0,
+ * or -1 when n_pairs == 0 (the reference's "0 valid read pairs" error). */
+int orc_count_stats_from_lengths(const uint32_t* len_r1, const uint32_t* len_r2, uint64_t n_pairs,
+                                 orc_count_stats* out);
+/* compute_dynamic_kmers; target_max_k < 0 means None.  Returns count. */
+uint32_t orc_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uint32_t* out8);
+/* estimate_genome over the first pairs of two arenas (R1[i] pairs with R2[i]).
+ * Returns 0, or -1 when no pair is available. */
+int orc_estimate_genome(const orc_arena* r1, const orc_arena* r2, const orc_count_stats* stats,
+                        uint64_t max_pairs, double target_cov, orc_genome_estimate* out);
+/* Checkpoint-free view of the estimator's sketch loop over exactly n_pairs
+ * pairs: final registers of hll_total and hll_solid (p=14). */
+void orc_estimator_sketch(const orc_arena* r1, const orc_arena* r2, uint64_t n_pairs,
+                          uint8_t* hll_total, uint8_t* hll_solid, uint64_t* m_sample);
+/* The Poisson/confidence tail of estimate_genome (estimator.rs:385-439). */
+void orc_estimate_finish(uint64_t u_ge_1, uint64_t u_ge_2, uint64_t m_sample,
+                         const orc_count_stats* stats, double target_cov, orc_genome_estimate* out);
+/* Rust's `f64 as u64` (saturating, NaN -> 0). */
+uint64_t orc_f64_to_u64(double x);
+
+/* ---- cDBG build passes 1-3 (src/graph/build.rs:67-328) --------------------
+ * Output record of the solid set: the canonical k-mer (nw words, MSB first)
+ * followed by cov (u16) and mask (u8); the parity form is sorted by k-mer. */
+typedef struct {
+    uint64_t width;              /* CMS row width (power of two) */
+    int      counter_bits;       /* 4 (reference) or 16 */
+    int      t_solid;            /* CMS gate for both ends of a pair */
+    int      min_trans_cov;      /* edge gate */
+    int      hll_p;              /* 0 = no per-pass HLL */
+} orc_build_params;
+
+typedef struct {
+    uint64_t n;                  /* solid k-mers */
+    int      nw;
+    uint64_t* keys;              /* n*nw words */
+    uint16_t* cov;
+    uint8_t*  mask;
+} orc_solid_set;
+
+/* Reference defaults: t_solid = target.unwrap_or(2), min_trans_cov =
+ * target.unwrap_or(3) (build.rs:141,302).  target < 0 means None. */
+void orc_build_params_default(orc_build_params* p, int target_solid_cov, uint64_t width);
+
+/* contigs: ASCII concatenation + lengths (build.rs:119-138,203-240). */
+int orc_build_solid(const orc_arena* arenas, int n_arenas,
+                    const uint8_t* contigs, const uint32_t* contig_lens, uint64_t n_contigs,
+                    uint32_t k, const orc_build_params* p,
+                    orc_cms** cms_out, uint8_t* hll_out, orc_solid_set* out);
+/* Reference-shaped multi-threaded form of the same passes: atomic CAS CMS,
+ * 64 mutex-sharded maps fed by 2048-entry thread buffers, 4096-read chunks. */
+int orc_build_solid_mt(const orc_arena* arenas, int n_arenas,
+                       const uint8_t* contigs, const uint32_t* contig_lens, uint64_t n_contigs,
+                       uint32_t k, const orc_build_params* p, int threads,
+                       orc_cms** cms_out, uint8_t* hll_out, orc_solid_set* out);
+void orc_solid_free(orc_solid_set* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

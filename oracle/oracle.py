@@ -1,0 +1,374 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the parity checker and the timed CPU baseline.
+Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline leg import
+this module.  Build with `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+u8p = C.POINTER(C.c_uint8)
+u16p = C.POINTER(C.c_uint16)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+vp = C.c_void_p
+
+
+class Arena(C.Structure):
+    _fields_ = [("data", u64p), ("offsets", u64p), ("lengths", u32p), ("n_reads", C.c_uint64)]
+
+
+class CountStats(C.Structure):
+    _fields_ = [
+        ("exact_read_pairs", C.c_uint64),
+        ("total_bases_r1", C.c_uint64),
+        ("total_bases_r2", C.c_uint64),
+        ("total_bases", C.c_uint64),
+        ("min_read_len", C.c_uint64),
+        ("max_read_len", C.c_uint64),
+        ("mean_read_len", C.c_double),
+        ("median_read_len", C.c_uint64),
+    ]
+
+
+class GenomeEstimate(C.Structure):
+    _fields_ = [
+        ("genome_size", C.c_uint64),
+        ("exact_read_pairs", C.c_uint64),
+        ("total_bases", C.c_uint64),
+        ("median_read_len", C.c_uint64),
+        ("max_read_len", C.c_uint64),
+        ("estimated_cov", C.c_double),
+        ("subsample_factor", C.c_double),
+        ("kmers", C.c_uint32 * 8),
+        ("n_kmers", C.c_uint32),
+        ("confidence", C.c_int32),
+        ("u_ge_1", C.c_uint64),
+        ("u_ge_2", C.c_uint64),
+        ("sample_kmer_depth", C.c_double),
+        ("pairs_sampled", C.c_uint64),
+        ("m_sample", C.c_uint64),
+    ]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("width", C.c_uint64), ("counter_bits", C.c_int), ("t_solid", C.c_int),
+                ("min_trans_cov", C.c_int), ("hll_p", C.c_int)]
+
+
+class SolidSetC(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("nw", C.c_int), ("keys", u64p), ("cov", u16p), ("mask", u8p)]
+
+
+_SIGS = [
+    ("orc_splitmix64", C.c_uint64, [C.c_uint64]),
+    ("orc_rev_comp_word", C.c_uint64, [C.c_uint64, C.c_uint32]),
+    ("orc_ascii_to_2bit", C.c_uint8, [C.c_uint8]),
+    ("orc_arena_pack", None, [u8p, u32p, C.c_uint64, u64p, u64p]),
+    ("orc_arena_extract_ascii", None, [C.POINTER(Arena), C.c_uint64, u8p]),
+    ("orc_kmer_words_for_k", C.c_int, [C.c_uint32]),
+    ("orc_kmer_from_ascii", None, [u8p, C.c_uint32, C.c_int, u64p]),
+    ("orc_kmer_rev_comp", None, [u64p, C.c_uint32, C.c_int, u64p]),
+    ("orc_kmer_roll_next", None, [u64p, u64p, C.c_uint8, C.c_uint32, C.c_int, u64p, u64p]),
+    ("orc_kmer_to_u64", C.c_uint64, [u64p, C.c_int]),
+    ("orc_kmer_canonical_windows", None, [u8p, C.c_uint32, C.c_uint32, C.c_int, u64p, u8p]),
+    ("orc_nthash_fwd", C.c_uint64, [u8p, C.c_uint32]),
+    ("orc_nthash_rev", C.c_uint64, [u8p, C.c_uint32]),
+    ("orc_nthash_canonical", C.c_uint64, [u8p, C.c_uint32]),
+    ("orc_nthash_rolling", None, [u8p, C.c_uint32, C.c_uint32, u64p]),
+    ("orc_cms_new", vp, [C.c_uint64, C.c_int]),
+    ("orc_cms_free", None, [vp]),
+    ("orc_cms_width", C.c_uint64, [vp]),
+    ("orc_cms_add", None, [vp, C.c_uint64]),
+    ("orc_cms_estimate", C.c_uint32, [vp, C.c_uint64]),
+    ("orc_cms_add_and_estimate", C.c_uint32, [vp, C.c_uint64]),
+    ("orc_cms_table_bytes", C.c_size_t, [vp]),
+    ("orc_cms_table", vp, [vp]),
+    ("orc_cms_width_for_genome_size", C.c_uint64, [C.c_uint64]),
+    ("orc_hll_add_hash", None, [u8p, C.c_int, C.c_uint64]),
+    ("orc_hll_merge", None, [u8p, u8p, C.c_int]),
+    ("orc_hll_estimate", C.c_double, [u8p, C.c_int]),
+    ("orc_count_stats_from_lengths", C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(CountStats)]),
+    ("orc_compute_dynamic_kmers", C.c_uint32, [C.c_uint64, C.c_int64, u32p]),
+    ("orc_estimate_genome", C.c_int, [C.POINTER(Arena), C.POINTER(Arena), C.POINTER(CountStats), C.c_uint64,
+                                      C.c_double, C.POINTER(GenomeEstimate)]),
+    ("orc_estimator_sketch", None, [C.POINTER(Arena), C.POINTER(Arena), C.c_uint64, u8p, u8p, u64p]),
+    ("orc_estimate_finish", None, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(CountStats), C.c_double,
+                                   C.POINTER(GenomeEstimate)]),
+    ("orc_f64_to_u64", C.c_uint64, [C.c_double]),
+    ("orc_build_params_default", None, [C.POINTER(BuildParams), C.c_int, C.c_uint64]),
+    ("orc_build_solid", C.c_int, [C.POINTER(Arena), C.c_int, u8p, u32p, C.c_uint64, C.c_uint32,
+                                  C.POINTER(BuildParams), C.POINTER(vp), u8p, C.POINTER(SolidSetC)]),
+    ("orc_build_solid_mt", C.c_int, [C.POINTER(Arena), C.c_int, u8p, u32p, C.c_uint64, C.c_uint32,
+                                     C.POINTER(BuildParams), C.c_int, C.POINTER(vp), u8p, C.POINTER(SolidSetC)]),
+    ("orc_solid_free", None, [C.POINTER(SolidSetC)]),
+]
+
+_LIB = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None and a.size else None
+
+
+def _bytes(s) -> np.ndarray:
+    return np.frombuffer(bytes(s), dtype=np.uint8).copy() if not isinstance(s, np.ndarray) else \
+        np.ascontiguousarray(s, dtype=np.uint8)
+
+
+# ---- primitives ------------------------------------------------------------
+
+def splitmix64(z: int) -> int:
+    return int(lib().orc_splitmix64(z & 0xFFFFFFFFFFFFFFFF))
+
+
+def rev_comp_word(w: int, nb: int) -> int:
+    return int(lib().orc_rev_comp_word(w, nb))
+
+
+def words_for_k(k: int) -> int:
+    return int(lib().orc_kmer_words_for_k(k))
+
+
+def kmer_from_ascii(s: bytes, nw: int | None = None) -> np.ndarray:
+    k = len(s)
+    nw = nw or words_for_k(k)
+    out = np.zeros(nw, np.uint64)
+    b = _bytes(s)
+    lib().orc_kmer_from_ascii(_p(b, C.c_uint8), k, nw, _p(out, C.c_uint64))
+    return out
+
+
+def kmer_rev_comp(x: np.ndarray, k: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.uint64)
+    out = np.zeros_like(x)
+    lib().orc_kmer_rev_comp(_p(x, C.c_uint64), k, x.size, _p(out, C.c_uint64))
+    return out
+
+
+def kmer_roll_next(fwd, rev, base: int, k: int):
+    fwd = np.ascontiguousarray(fwd, np.uint64)
+    rev = np.ascontiguousarray(rev, np.uint64)
+    of, orv = np.zeros_like(fwd), np.zeros_like(rev)
+    lib().orc_kmer_roll_next(_p(fwd, C.c_uint64), _p(rev, C.c_uint64), base, k, fwd.size, _p(of, C.c_uint64),
+                             _p(orv, C.c_uint64))
+    return of, orv
+
+
+def kmer_to_u64(x) -> int:
+    x = np.ascontiguousarray(x, np.uint64)
+    return int(lib().orc_kmer_to_u64(_p(x, C.c_uint64), x.size))
+
+
+def canonical_windows(seq: bytes, k: int, nw: int | None = None):
+    nw = nw or words_for_k(k)
+    n = max(0, len(seq) - k + 1)
+    out = np.zeros((n, nw), np.uint64)
+    rc = np.zeros(n, np.uint8)
+    b = _bytes(seq)
+    lib().orc_kmer_canonical_windows(_p(b, C.c_uint8), len(seq), k, nw, _p(out, C.c_uint64), _p(rc, C.c_uint8))
+    return out, rc
+
+
+def nthash_fwd(s: bytes) -> int:
+    b = _bytes(s)
+    return int(lib().orc_nthash_fwd(_p(b, C.c_uint8), len(s)))
+
+
+def nthash_rev(s: bytes) -> int:
+    b = _bytes(s)
+    return int(lib().orc_nthash_rev(_p(b, C.c_uint8), len(s)))
+
+
+def nthash_canonical(s: bytes) -> int:
+    b = _bytes(s)
+    return int(lib().orc_nthash_canonical(_p(b, C.c_uint8), len(s)))
+
+
+def nthash_rolling(seq: bytes, k: int) -> np.ndarray:
+    n = max(0, len(seq) - k + 1)
+    out = np.zeros(n, np.uint64)
+    b = _bytes(seq)
+    lib().orc_nthash_rolling(_p(b, C.c_uint8), len(seq), k, _p(out, C.c_uint64))
+    return out
+
+
+# ---- sketches --------------------------------------------------------------
+
+class Cms:
+    """CountMinSketch (sketch.rs:121-254), nibble (4) or u16 (16) counters."""
+
+    def __init__(self, width: int, counter_bits: int = 4, handle=None):
+        self.h = handle if handle is not None else lib().orc_cms_new(width, counter_bits)
+        self.counter_bits = counter_bits
+
+    @property
+    def width(self) -> int:
+        return int(lib().orc_cms_width(self.h))
+
+    def add(self, key: int) -> None:
+        lib().orc_cms_add(self.h, key & 0xFFFFFFFFFFFFFFFF)
+
+    def estimate(self, key: int) -> int:
+        return int(lib().orc_cms_estimate(self.h, key & 0xFFFFFFFFFFFFFFFF))
+
+    def add_and_estimate(self, key: int) -> int:
+        return int(lib().orc_cms_add_and_estimate(self.h, key & 0xFFFFFFFFFFFFFFFF))
+
+    def table(self) -> np.ndarray:
+        n = int(lib().orc_cms_table_bytes(self.h))
+        raw = np.ctypeslib.as_array(C.cast(lib().orc_cms_table(self.h), u8p), shape=(n,)).copy()
+        return raw if self.counter_bits == 4 else raw.view(np.uint16)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_cms_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def cms_width_for_genome_size(g: int) -> int:
+    return int(lib().orc_cms_width_for_genome_size(g))
+
+
+def hll_add_hashes(regs: np.ndarray, p: int, hashes) -> None:
+    for h in np.asarray(hashes, dtype=np.uint64).tolist():
+        lib().orc_hll_add_hash(_p(regs, C.c_uint8), p, h)
+
+
+def hll_estimate(regs: np.ndarray, p: int) -> float:
+    r = np.ascontiguousarray(regs, np.uint8)
+    return float(lib().orc_hll_estimate(_p(r, C.c_uint8), p))
+
+
+# ---- arenas ----------------------------------------------------------------
+
+def pack(seqs) -> tuple:
+    """BitSeqArena::push_seq for each sequence -> (data, offsets, lengths)."""
+    lens = np.array([len(s) for s in seqs], np.uint32)
+    return pack_concat(b"".join(bytes(s) for s in seqs), lens)
+
+
+def pack_concat(ascii_concat, lengths) -> tuple:
+    lens = np.ascontiguousarray(lengths, np.uint32)
+    buf = _bytes(ascii_concat)
+    total = int(lens.sum(dtype=np.uint64))
+    data = np.zeros((total + 31) // 32 + 1, np.uint64)
+    offs = np.zeros(lens.size, np.uint64)
+    lib().orc_arena_pack(_p(buf, C.c_uint8), _p(lens, C.c_uint32), lens.size, _p(data, C.c_uint64),
+                         _p(offs, C.c_uint64))
+    return data, offs, lens
+
+
+def arena(data, offsets, lengths) -> Arena:
+    a = Arena(_p(data, C.c_uint64), _p(offsets, C.c_uint64), _p(lengths, C.c_uint32), lengths.size)
+    a._keep = (data, offsets, lengths)
+    return a
+
+
+# ---- estimator -------------------------------------------------------------
+
+def count_stats(l1, l2) -> CountStats | None:
+    a = np.ascontiguousarray(l1, np.uint32)
+    b = np.ascontiguousarray(l2, np.uint32)
+    st = CountStats()
+    rc = lib().orc_count_stats_from_lengths(_p(a, C.c_uint32), _p(b, C.c_uint32), a.size, C.byref(st))
+    return None if rc else st
+
+
+def compute_dynamic_kmers(median_len: int, target_max_k: int | None = None) -> list:
+    out = (C.c_uint32 * 8)()
+    n = lib().orc_compute_dynamic_kmers(median_len, -1 if target_max_k is None else target_max_k, out)
+    return [int(out[i]) for i in range(n)]
+
+
+def estimate_genome(r1: tuple, r2: tuple, stats: CountStats, max_pairs: int, target_cov: float):
+    a1, a2 = arena(*r1), arena(*r2)
+    out = GenomeEstimate()
+    rc = lib().orc_estimate_genome(C.byref(a1), C.byref(a2), C.byref(stats), max_pairs, float(target_cov),
+                                   C.byref(out))
+    return None if rc else out
+
+
+def estimator_sketch(r1: tuple, r2: tuple, n_pairs: int):
+    a1, a2 = arena(*r1), arena(*r2)
+    ht = np.zeros(1 << 14, np.uint8)
+    hs = np.zeros(1 << 14, np.uint8)
+    m = C.c_uint64()
+    lib().orc_estimator_sketch(C.byref(a1), C.byref(a2), n_pairs, _p(ht, C.c_uint8), _p(hs, C.c_uint8), C.byref(m))
+    return ht, hs, int(m.value)
+
+
+def estimate_finish(u1: int, u2: int, m_sample: int, stats: CountStats, target_cov: float):
+    out = GenomeEstimate()
+    lib().orc_estimate_finish(u1, u2, m_sample, C.byref(stats), float(target_cov), C.byref(out))
+    return out
+
+
+# ---- solid set -------------------------------------------------------------
+
+def build_solid(arenas: list, k: int, target_solid_cov: int | None = None, width: int = 1 << 22,
+                counter_bits: int = 4, hll_p: int = 0, contigs=None, threads: int = 0):
+    """build_cdbg_with_contigs_ext passes 1-3 -> dict(keys, cov, mask, cms, hll).
+
+    arenas: list of (data, offsets, lengths); threads 0 = sequential form."""
+    p = BuildParams()
+    lib().orc_build_params_default(C.byref(p), -1 if target_solid_cov is None else target_solid_cov, width)
+    p.counter_bits = counter_bits
+    p.hll_p = hll_p
+    arr = (Arena * max(1, len(arenas)))(*[arena(*a) for a in arenas])
+    keep = [arena(*a) for a in arenas]  # noqa: F841 keep buffers alive
+    if contigs:
+        cl = np.array([len(c) for c in contigs], np.uint32)
+        cb = _bytes(b"".join(contigs))
+    else:
+        cl, cb = None, None
+    cms_h = vp()
+    regs = np.zeros(1 << hll_p, np.uint8) if hll_p else None
+    ss = SolidSetC()
+    if threads:
+        rc = lib().orc_build_solid_mt(arr, len(arenas), _p(cb, C.c_uint8), _p(cl, C.c_uint32),
+                                      0 if cl is None else cl.size, k, C.byref(p), threads, C.byref(cms_h),
+                                      _p(regs, C.c_uint8), C.byref(ss))
+    else:
+        rc = lib().orc_build_solid(arr, len(arenas), _p(cb, C.c_uint8), _p(cl, C.c_uint32),
+                                   0 if cl is None else cl.size, k, C.byref(p), C.byref(cms_h),
+                                   _p(regs, C.c_uint8), C.byref(ss))
+    if rc:
+        raise ValueError(f"orc_build_solid failed ({rc})")
+    n, nw = int(ss.n), int(ss.nw)
+    keys = np.ctypeslib.as_array(ss.keys, shape=(max(n, 1) * nw,))[: n * nw].copy().reshape(n, nw)
+    cov = np.ctypeslib.as_array(ss.cov, shape=(max(n, 1),))[:n].copy()
+    mask = np.ctypeslib.as_array(ss.mask, shape=(max(n, 1),))[:n].copy()
+    lib().orc_solid_free(C.byref(ss))
+    cms = Cms(width, counter_bits, handle=cms_h.value)
+    return {"keys": keys, "cov": cov, "mask": mask, "cms": cms, "hll": regs}

@@ -1,0 +1,297 @@
+"""Independent pure-Python restatement of the path for small inputs.
+
+Written from the reference's definitions with Python big integers and
+strings (no limb shifting, no rolling updates), so that a mistake in the C
+oracle's word-level code shows up as a disagreement.  Test code only.
+"""
+from __future__ import annotations
+
+import math
+
+M64 = (1 << 64) - 1
+CMS_SEEDS = [0x243F6A8885A308D3, 0x13198A2E03707344, 0xA4093822299F31D0, 0x082EFA98EC4E6C89]
+NT = {"A": 0x3C8BFBB395C60474, "C": 0x3193C18562A02B4C, "G": 0x20323EE046061444, "T": 0x295C504FAC61002E}
+COMP = {"A": "T", "C": "G", "G": "C", "T": "A"}
+
+
+def splitmix64(z: int) -> int:
+    z &= M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def norm(seq: bytes) -> str:
+    """ASCII -> ACGT with every non-ACGT byte read as A (bitseq.rs:67-81)."""
+    out = []
+    for c in seq:
+        ch = chr(c).upper()
+        out.append(ch if ch in "ACGT" else "A")
+    return "".join(out)
+
+
+def revcomp(s: str) -> str:
+    return "".join(COMP[c] for c in reversed(s))
+
+
+def kmer_int(s: str) -> int:
+    v = 0
+    for c in s:
+        v = v * 4 + "ACGT".index(c)
+    return v
+
+
+def canonical(s: str) -> tuple[int, bool]:
+    f, r = kmer_int(s), kmer_int(revcomp(s))
+    return (r, True) if r < f else (f, False)
+
+
+def fold(x: int) -> int:
+    """to_u64: XOR of the 64-bit limbs of the k-mer integer."""
+    f = 0
+    while x:
+        f ^= x & M64
+        x >>= 64
+    return f
+
+
+def ref_words(x: int, k: int) -> list:
+    nw = 1 if k <= 31 else (2 if k <= 63 else 4)
+    return [(x >> (64 * (nw - 1 - i))) & M64 for i in range(nw)]
+
+
+def rol(x: int, r: int) -> int:
+    r %= 64
+    return ((x << r) | (x >> (64 - r))) & M64 if r else x
+
+
+def nthash_fwd(s: str) -> int:
+    k, h = len(s), 0
+    for i, c in enumerate(s):
+        h ^= rol(NT[c], k - 1 - i)
+    return h
+
+
+def nthash_rev(s: str) -> int:
+    h = 0
+    for i, c in enumerate(s):
+        h ^= rol(NT[COMP[c]], i)
+    return h
+
+
+def nthash_canonical(s: str) -> int:
+    return min(nthash_fwd(s), nthash_rev(s))
+
+
+def cms_cells(key: int, width: int) -> list:
+    return [r * width + (splitmix64(key ^ CMS_SEEDS[r]) & (width - 1)) for r in range(4)]
+
+
+def hll_index_rho(h: int, p: int) -> tuple[int, int]:
+    idx = h >> (64 - p)
+    w = h & ((1 << (64 - p)) - 1)
+    rho = (64 - p + 1) if w == 0 else ((64 - p) - w.bit_length()) + 1
+    return idx, rho
+
+
+def hll_estimate(regs, p: int) -> float:
+    m = float(1 << p)
+    alpha = 0.7213 / (1.0 + 1.079 / m)
+    s = sum(2.0 ** -int(v) for v in regs)
+    zeros = sum(1 for v in regs if v == 0)
+    raw = alpha * m * m / s
+    if raw <= 2.5 * m:
+        return m * math.log(m / zeros) if zeros else raw
+    return raw
+
+
+# ---- CMS table from exact per-cell counts (order-free) ---------------------
+
+def cms_table_from_counts(cells: dict, width: int, bits: int) -> bytes | list:
+    cap = 15 if bits == 4 else 65535
+    if bits == 16:
+        t = [0] * (4 * width)
+        for c, n in cells.items():
+            t[c] = min(n, cap)
+        return t
+    t = bytearray(2 * width)
+    for c, n in cells.items():
+        v = min(n, cap)
+        if c & 1:
+            t[c >> 1] |= v << 4
+        else:
+            t[c >> 1] |= v
+    return bytes(t)
+
+
+def cms_estimate(cells: dict, key: int, width: int, bits: int) -> int:
+    cap = 15 if bits == 4 else 65535
+    return min(min(cells.get(c, 0), cap) for c in cms_cells(key, width))
+
+
+# ---- passes 1-3 (build.rs:93-328) by brute force ---------------------------
+
+def build_solid(reads: list, k: int, width: int, target=None, bits: int = 4, contigs=()) -> tuple:
+    """Returns (sorted [(kmer_int, cov, mask)], cell counts)."""
+    t_solid = 2 if target is None else target
+    gate = 3 if target is None else target
+    cells: dict = {}
+    seqs = [norm(r) for r in reads]
+    cseqs = [norm(c) for c in contigs]
+    for s in seqs:
+        for i in range(len(s) - k + 1):
+            for c in cms_cells(fold(canonical(s[i:i + k])[0]), width):
+                cells[c] = cells.get(c, 0) + 1
+    for s in cseqs:
+        for i in range(len(s) - k + 1):
+            for c in cms_cells(fold(canonical(s[i:i + k])[0]), width):
+                cells[c] = cells.get(c, 0) + 10
+    acc: dict = {}
+
+    def push(key, edge, cw, ew):
+        a = acc.setdefault(key, [0, [0] * 8])
+        a[0] += cw
+        a[1][edge] += ew
+
+    def pairs(s, gated):
+        for i in range(len(s) - k):
+            u, v = s[i:i + k], s[i + 1:i + k + 1]
+            (uc, urc), (vc, vrc) = canonical(u), canonical(v)
+            if gated and not (cms_estimate(cells, fold(uc), width, bits) >= t_solid
+                              and cms_estimate(cells, fold(vc), width, bits) >= t_solid):
+                continue
+            b = "ACGT".index(s[i + k])
+            p = "ACGT".index(s[i])
+            yield uc, (b + 4 if urc else b), vc, ((3 - p) if vrc else (3 - p) + 4)
+
+    for s in cseqs:
+        for uc, eu, vc, ev in pairs(s, False):
+            push(uc, eu, 100, 1000)
+            push(vc, ev, 100, 1000)
+    for s in seqs:
+        for uc, eu, vc, ev in pairs(s, True):
+            push(uc, eu, 1, 1)
+            push(vc, ev, 1, 1)
+    out = []
+    for key, (cov, edges) in acc.items():
+        mask = 0
+        for e in range(8):
+            if min(edges[e], 65535) >= gate:
+                mask |= 1 << e
+        if mask:
+            out.append((key, max(min(cov, 65535) // 2, 1), mask))
+    out.sort()
+    return out, cells
+
+
+# ---- estimator sketch loop (estimator.rs:309-379), sequential -------------
+
+def estimator_registers(pairs: list, n_pairs: int, width: int = 1 << 23):
+    """Final hll_total / hll_solid registers (p=14) after n_pairs pairs, with
+    the order-dependent `c >= 2` test evaluated at insertion time."""
+    k = 21
+    cells: dict = {}
+    ht, hs = [0] * (1 << 14), [0] * (1 << 14)
+    m = 0
+    for r1, r2 in pairs[:n_pairs]:
+        for s in (norm(r1), norm(r2)):
+            if len(s) < k:
+                continue
+            m += len(s) - k + 1
+            for i in range(len(s) - k + 1):
+                raw = nthash_canonical(s[i:i + k])
+                cs = cms_cells(raw, width)
+                for c in cs:
+                    cells[c] = min(cells.get(c, 0) + 1, 15)
+                cnt = min(cells[c] for c in cs)
+                mixed = splitmix64(raw)
+                idx, rho = hll_index_rho(mixed, 14)
+                ht[idx] = max(ht[idx], rho)
+                if cnt >= 2:
+                    hs[idx] = max(hs[idx], rho)
+    return ht, hs, m
+
+
+# ---- unitig count of a solid set (walk of build.rs:332-549) ----------------
+
+def count_unitigs(solid: list, k: int) -> int:
+    """Number of unitigs the reference's compaction walk emits for a solid
+    set [(kmer_int, cov, mask)]: walks from boundary directed k-mers plus
+    isolated cycles (build.rs:332-549)."""
+    nodes = {key: [mask, 0] for key, _, mask in solid}  # mask, visited
+    full = (1 << (2 * k)) - 1
+
+    def rc_int(x):
+        return kmer_int(revcomp(to_str(x)))
+
+    def to_str(x):
+        return "".join("ACGT"[(x >> (2 * (k - 1 - i))) & 3] for i in range(k))
+
+    def step(x_fwd, b):
+        nf = ((x_fwd << 2) | b) & full
+        nr = rc_int(nf)
+        return (nr, True) if nr < nf else (nf, False)
+
+    def first_base(x_fwd):
+        return (x_fwd >> (2 * (k - 1))) & 3
+
+    count = 0
+    boundary = []
+    for key, (mask, _) in nodes.items():
+        fo, fi = bin(mask & 15).count("1"), bin(mask >> 4).count("1")
+        if fi != 1 or fo != 1:
+            boundary.append((key, False))
+        if fo != 1 or fi != 1:
+            boundary.append((key, True))
+
+    def walk_from(ukey, urc, b):
+        fwd = rc_int(ukey) if urc else ukey
+        rc_first = 3 - first_base(fwd)
+        ckey, crc = step(fwd, b)
+        if ckey in nodes:
+            nodes[ckey][1] |= 1 << (rc_first + 4 if not crc else rc_first)
+        steps = 0
+        while True:
+            if ckey not in nodes:
+                break
+            m = nodes[ckey][0]
+            out_m = (m & 15) if not crc else (m >> 4)
+            in_m = (m >> 4) if not crc else (m & 15)
+            if bin(in_m).count("1") == 1 and bin(out_m).count("1") == 1:
+                nb = (out_m & -out_m).bit_length() - 1
+                eb = nb if not crc else nb + 4
+                if nodes[ckey][1] & (1 << eb):
+                    break
+                nodes[ckey][1] |= 1 << eb
+                cf = rc_int(ckey) if crc else ckey
+                rp = 3 - first_base(cf)
+                nkey, nrc = step(cf, nb)
+                if nkey in nodes:
+                    nodes[nkey][1] |= 1 << (rp + 4 if not nrc else rp)
+                ckey, crc = nkey, nrc
+                steps += 1
+            else:
+                break
+        return ckey, crc
+
+    for ukey, urc in boundary:
+        for b in range(4):
+            if ukey not in nodes:
+                break
+            m, vis = nodes[ukey]
+            out_m = (m & 15) if not urc else (m >> 4)
+            v = (vis & 15) if not urc else (vis >> 4)
+            if out_m & (1 << b) and not v & (1 << b):
+                nodes[ukey][1] |= 1 << (b if not urc else b + 4)
+                walk_from(ukey, urc, b)
+                count += 1
+    for key in list(nodes):
+        m, vis = nodes[key]
+        if bin(m & 15).count("1") == 1 and bin(m >> 4).count("1") == 1:
+            b = (m & 15).bit_length() - 1
+            if not vis & (1 << b):
+                nodes[key][1] |= 1 << b
+                end, _ = walk_from(key, False, b)
+                if end == key:
+                    count += 1
+    return count

@@ -1,0 +1,225 @@
+"""GPU parity: the HIP path against the CPU oracle, bit-exact.
+
+Every test calls through the C ABI (libfastbact_gpu.so) on cuda:0 and
+compares with oracle/liboracle.so on the same seeded inputs.
+"""
+import numpy as np
+import pytest
+
+import fastbact_amd as F
+from tests.helpers import isolate, oracle_arenas, ragged_bank
+
+pytestmark = pytest.mark.gpu
+
+
+def _solid_equal(gpu: F.SolidSet, ref: dict):
+    assert gpu.keys.shape == ref["keys"].shape, (gpu.keys.shape, ref["keys"].shape)
+    assert np.array_equal(gpu.keys, ref["keys"])
+    assert np.array_equal(gpu.cov, ref["cov"])
+    assert np.array_equal(gpu.mask, ref["mask"])
+
+
+@pytest.fixture(scope="module")
+def small_iso():
+    return isolate(seed=21, genome_len=30_000, n_pairs=2500)
+
+
+@pytest.mark.parametrize("k", [21, 31, 32, 33, 55, 63, 64, 65, 77, 99, 127])
+def test_build_solid_matches_oracle_all_widths(ctx, orc, small_iso, k):
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    width = 1 << 18
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=3, cms_width=width, hll_p=14)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=3, width=width, hll_p=14)
+    _solid_equal(g, r)
+    assert np.array_equal(g.cms_table, r["cms"].table())
+    assert np.array_equal(g.hll_registers, r["hll"])
+    assert len(g) > 1000
+
+
+@pytest.mark.parametrize("k,target,bits", [(31, None, 4), (31, 5, 16), (55, 4, 4), (77, 4, 16), (127, 2, 4)])
+def test_build_solid_thresholds_and_counter_modes(ctx, orc, small_iso, k, target, bits):
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    width = 1 << 16  # small width: collisions inflate estimates, exercised bit-exactly
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=target, cms_width=width, counter_bits=bits)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=target, width=width, counter_bits=bits)
+    _solid_equal(g, r)
+    assert np.array_equal(g.cms_table, r["cms"].table())
+
+
+def test_reference_cms_width_from_genome_size(ctx, orc, small_iso):
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    # genome_size None -> total_bases/60 fallback; explicit -> max(g, 50000) (build.rs:74-84)
+    for gs in (None, 30_000, 400_000):
+        g = F.build_solid_kmers(ctx, dev, 31, genome_size=gs)
+        est = max(gs, 50_000) if gs else max(bank.total_bases() // 60, 50_000)
+        w = orc.cms_width_for_genome_size(est)
+        r = orc.build_solid(oracle_arenas(bank), 31, width=w)
+        _solid_equal(g, r)
+        assert g.cms_table.size == w * 2
+
+
+@pytest.mark.parametrize("k", [21, 31, 55, 77, 127])
+def test_ragged_reads_and_non_acgt(ctx, orc, k):
+    bank = ragged_bank(seed=k)
+    dev = F.DeviceBank(ctx, bank)
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=1, cms_width=1 << 14, hll_p=12)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=1, width=1 << 14, hll_p=12)
+    _solid_equal(g, r)
+    assert np.array_equal(g.cms_table, r["cms"].table())
+    assert np.array_equal(g.hll_registers, r["hll"])
+
+
+@pytest.mark.parametrize("k", [31, 55, 99])
+def test_prior_contigs_weights(ctx, orc, small_iso, k):
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    gen = small_iso.genome.tobytes()
+    contigs = [gen[1000:5000], gen[12000:12000 + k], gen[20000:20000 + k + 1], gen[25000:26500]]
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=4, cms_width=1 << 17, contigs=contigs)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=4, width=1 << 17, contigs=contigs)
+    _solid_equal(g, r)
+    assert np.array_equal(g.cms_table, r["cms"].table())
+
+
+def test_empty_bank(ctx, orc):
+    bank = F.ReadBank()
+    dev = F.DeviceBank(ctx, bank)
+    g = F.build_solid_kmers(ctx, dev, 31, cms_width=1 << 12, hll_p=10)
+    assert len(g) == 0
+    assert not g.cms_table.any()
+    assert not g.hll_registers.any()
+
+
+def test_saturation_nibble_and_u16(ctx, orc):
+    # one k-mer repeated many times: counters saturate at 15 / 65535, the solid
+    # cov at 65535 // 2 (build.rs:185-186,319)
+    seq = b"A" * 20000
+    bank = F.ReadBank(F.BitSeqArena.from_ascii([seq] * 5), F.BitSeqArena(), F.BitSeqArena())
+    dev = F.DeviceBank(ctx, bank)
+    for bits in (4, 16):
+        g = F.build_solid_kmers(ctx, dev, 31, cms_width=1 << 12, counter_bits=bits)
+        r = orc.build_solid(oracle_arenas(bank), 31, width=1 << 12, counter_bits=bits)
+        _solid_equal(g, r)
+        assert np.array_equal(g.cms_table, r["cms"].table())
+        assert len(g) == 1 and g.cov[0] == 65535 // 2
+
+
+def test_cms_reference_known_answers(ctx):
+    # sketch.rs:304-319: 0 -> 5 -> saturates at 15
+    cms = F.CountMinSketch(ctx, 1024, 4)
+    assert cms.estimate(42)[0] == 0
+    for _ in range(5):
+        cms.add(42)
+    assert cms.estimate(42)[0] == 5
+    cms.add(np.full(20, 42, np.uint64))
+    assert cms.estimate(42)[0] == 15
+    # sketch.rs:321-342: 4 threads x 3 adds = 12 (here 12 concurrent GPU adds)
+    cms2 = F.CountMinSketch(ctx, 4096, 4)
+    cms2.add(np.full(12, 12345, np.uint64))
+    assert cms2.estimate(12345)[0] == 12
+
+
+def test_cms_keys_and_table_match_oracle(ctx, orc):
+    rng = np.random.default_rng(3)
+    keys = rng.integers(0, 2**63, size=50_000, dtype=np.uint64)
+    keys = np.concatenate([keys, keys[:5000], keys[:100].repeat(30)])
+    for bits in (4, 16):
+        cms = F.CountMinSketch(ctx, 1 << 12, bits)
+        cms.add(keys)
+        o = orc.Cms(1 << 12, bits)
+        for x in keys.tolist():
+            o.add(x)
+        assert np.array_equal(cms.table(), o.table())
+        q = keys[:2000]
+        assert np.array_equal(cms.estimate(q), np.array([o.estimate(int(x)) for x in q.tolist()], np.uint32))
+
+
+def test_hll_hashes_and_bank_match_oracle(ctx, orc, small_iso):
+    rng = np.random.default_rng(4)
+    hs = rng.integers(0, 2**64 - 1, size=100_000, dtype=np.uint64)
+    hs[:10] = 0  # w == 0 -> rho = 64 - p + 1
+    for p in (10, 14, 16):
+        h = F.HyperLogLog(ctx, p)
+        h.add_hash(hs)
+        regs = np.zeros(1 << p, np.uint8)
+        orc.hll_add_hashes(regs, p, hs)
+        assert np.array_equal(h.registers(), regs)
+        assert h.estimate() == orc.hll_estimate(regs, p)
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    h = F.HyperLogLog(ctx, 14)
+    h.add_bank(dev, 21)
+    r = orc.build_solid(oracle_arenas(bank), 21, width=1 << 12, hll_p=14)
+    assert np.array_equal(h.registers(), r["hll"])
+
+
+def test_cms_add_bank_matches_pass1(ctx, orc, small_iso):
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    for k in (21, 63, 127):
+        cms = F.CountMinSketch(ctx, 1 << 15, 16)
+        cms.add_bank(dev, k)
+        r = orc.build_solid(oracle_arenas(bank), k, width=1 << 15, counter_bits=16)
+        assert np.array_equal(cms.table(), r["cms"].table())
+
+
+def _est_equal(g: F.GenomeEstimate, o):
+    assert g.genome_size == o.genome_size
+    assert g.u_ge_1 == o.u_ge_1 and g.u_ge_2 == o.u_ge_2
+    assert g.pairs_sampled == o.pairs_sampled and g.m_sample == o.m_sample
+    assert g.estimated_cov == o.estimated_cov
+    assert g.subsample_factor == o.subsample_factor
+    assert g.sample_kmer_depth == o.sample_kmer_depth
+    assert g.confidence == F.api.CONFIDENCE[o.confidence]
+    assert g.kmers == [o.kmers[i] for i in range(o.n_kmers)]
+
+
+@pytest.mark.parametrize("genome_len,n_pairs,max_pairs", [
+    (50_000, 20_000, 250_000),    # early stop at the first checkpoint (lambda_s >= 15)
+    (1_000_000, 12_000, 250_000),  # no early stop: every sampled pair counts
+    (200_000, 9_000, 7_000),       # max_pairs cuts the sample
+    (300_000, 3_000, 250_000),     # fewer pairs than the first checkpoint
+])
+def test_estimate_genome_matches_oracle(ctx, orc, genome_len, n_pairs, max_pairs):
+    iso = isolate(seed=genome_len % 97, genome_len=genome_len, n_pairs=n_pairs, sub_rate=0.004)
+    bank = iso.bank()
+    r1, r2 = bank.pe_forward, bank.pe_reverse
+    g = F.estimate_genome(ctx, r1, r2, max_pairs, 100.0)
+    st = orc.count_stats(r1.lengths, r2.lengths)
+    o = orc.estimate_genome((r1.data, r1.offsets, r1.lengths), (r2.data, r2.offsets, r2.lengths), st, max_pairs,
+                            100.0)
+    _est_equal(g, o)
+
+
+def test_estimate_genome_ragged_and_empty(ctx, orc):
+    bank = ragged_bank(seed=9, n=3000, max_len=300)
+    r1, r2 = bank.pe_forward, bank.pe_reverse
+    g = F.estimate_genome(ctx, r1, r2, 250_000, 50.0)
+    st = orc.count_stats(r1.lengths, r2.lengths)
+    o = orc.estimate_genome((r1.data, r1.offsets, r1.lengths), (r2.data, r2.offsets, r2.lengths), st, 250_000, 50.0)
+    _est_equal(g, o)
+    with pytest.raises(F.FastBactError) as e:
+        F.estimate_genome(ctx, F.BitSeqArena(), F.BitSeqArena(), 1000, 50.0)
+    assert "0 valid read pairs" in str(e.value)
+
+
+def test_cfg1_scale_plain_isolate(ctx, orc):
+    """1 Mbp at 50% GC, 100k PE150 pairs (30x), 0.5% errors, k=31, CMS 4 x 2^22
+    u16 counters, HLL p=14, solid threshold 3: bit-exact against the oracle."""
+    iso = F.synth_isolate(seed=0xFB01, genome_len=1_000_000, gc=0.5, coverage=30.0, insert_mean=350,
+                          insert_sd=30, sub_rate=0.005)
+    bank = iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    g = F.build_solid_kmers(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 22, counter_bits=16, hll_p=14)
+    r = orc.build_solid(oracle_arenas(bank), 31, target_solid_cov=3, width=1 << 22, counter_bits=16, hll_p=14,
+                        threads=8)
+    _solid_equal(g, r)
+    assert np.array_equal(g.cms_table, r["cms"].table())
+    assert np.array_equal(g.hll_registers, r["hll"])
+    # HLL of distinct 31-mers within the reference's 3% gate of the exact count
+    est = F.hll_estimate_registers(g.hll_registers, 14)
+    assert 0.5e6 < len(g) < 1.1e6
+    assert est > len(g)

@@ -70,11 +70,12 @@ def make_isolate():
     return iso, bank
 
 
-def run_step(ctx, dev, F):
+def run_step(ctx, dev, F, placement=0):
     handles = []
     for k, tgt in PASSES:
         handles.append(F.build_solid_handle(ctx, dev, k, target_solid_cov=tgt, genome_size=GENOME["genome_len"],
-                                            cms_width=WIDTH, counter_bits=COUNTER_BITS, hll_p=HLL_P))
+                                            cms_width=WIDTH, counter_bits=COUNTER_BITS, hll_p=HLL_P,
+                                            placement=placement))
     ctx.synchronize()
     info = [(h.timings(), h.stats(), h.count()) for h in handles]
     for h in handles:
@@ -115,6 +116,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-pairs", type=int, default=300_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--placement", type=int, default=0, help="pass-1 sketch placement: 0 auto, 1 global, 2 LDS bins")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the k-mer spectrum path is measured per GPU); got --gpus {args.gpus}",
@@ -132,14 +134,14 @@ def main(argv=None) -> int:
     log(f"bank in HBM ({bank.total_bases() / 4e6:.0f} MB packed, upload {h2d_s:.2f} s); {bases_step} bases per step")
 
     for i in range(args.warmup):
-        run_step(ctx, dev, F)
+        run_step(ctx, dev, F, args.placement)
         log(f"warmup step {i + 1}/{args.warmup} done")
     ctx.synchronize()
     step_ms, infos = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         ts = time.perf_counter()
-        infos.append(run_step(ctx, dev, F))
+        infos.append(run_step(ctx, dev, F, args.placement))
         step_ms.append((time.perf_counter() - ts) * 1e3)
         log(f"step {i + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
     ctx.synchronize()

@@ -56,6 +56,7 @@ class BuildParams(C.Structure):
         ("hll_p", C.c_int32),
         ("sort", C.c_uint32),
         ("split", C.c_uint32),
+        ("placement", C.c_uint32),
     ]
 
 

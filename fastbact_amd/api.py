@@ -345,7 +345,7 @@ def _pack_contigs(contigs: Optional[Sequence[bytes]]):
 
 def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Optional[int] = None,
                  cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
-                 split: int = 0) -> BuildParams:
+                 split: int = 0, placement: int = 0) -> BuildParams:
     p = BuildParams()
     _lib.load().fb_build_params_default(C.byref(p), k)
     p.target_solid_cov = -1 if target_solid_cov is None else int(target_solid_cov)
@@ -355,6 +355,7 @@ def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Op
     p.hll_p = hll_p
     p.sort = 1 if sort else 0
     p.split = split
+    p.placement = placement
     return p
 
 
@@ -372,11 +373,12 @@ def build_solid_handle(ctx: Context, bank: DeviceBank, k: int, contigs: Optional
 def build_solid_kmers(ctx: Context, bank: DeviceBank, k: int, target_solid_cov: Optional[int] = None,
                       genome_size: Optional[int] = None, contigs: Optional[Sequence[bytes]] = None,
                       cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0,
-                      with_cms: bool = True, split: int = 0) -> SolidSet:
+                      with_cms: bool = True, split: int = 0, placement: int = 0) -> SolidSet:
     """build_cdbg_with_contigs_ext passes 1-3 (build.rs:67-328): the solid set,
     the pass's CMS table and (hll_p > 0) the pass's HLL registers."""
     h = build_solid_handle(ctx, bank, k, contigs, target_solid_cov=target_solid_cov, genome_size=genome_size,
-                           cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, split=split)
+                           cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, split=split,
+                           placement=placement)
     try:
         return h.download(k, with_cms=with_cms, counter_bits=counter_bits, hll_p=hll_p)
     finally:

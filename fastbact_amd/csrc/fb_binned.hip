@@ -1,0 +1,181 @@
+// fb_binned.hip -- pass 1 with LDS-privatised Count-Min slices.
+//
+// The Count-Min table (4 x width cells) is cut into bins of 2^sb cells, each
+// small enough to count in one workgroup's LDS.
+//   Phase A (k_bin_scatter): each workgroup walks its reads twice.  The first
+//     walk counts the workgroup's updates per bin in LDS (and feeds the HLL).
+//     One global atomic per bin then reserves a run in that bin's region.  The
+//     second walk writes each update's 16-bit cell offset into the run.  An
+//     update that finds its bin region full is added to the table directly
+//     with a global atomic, so any input (low-complexity reads included)
+//     stays exact.
+//   Phase B (k_bin_count): one workgroup per bin counts the bin's offsets
+//     with LDS atomics, adds them to the table (the bin's cells belong to this
+//     workgroup alone) and writes the pass-2 threshold bits of those cells.
+// Every update is +1, so the final counters equal the direct-atomic ones.
+#include "fb_device.hpp"
+#include "fb_internal.hpp"
+
+namespace fb {
+using namespace fbk;
+
+constexpr int kScatterBlock = 512;
+constexpr int kCountBlock = 1024;
+
+__device__ __forceinline__ bool share_of(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
+                                         uint32_t& w0, uint32_t& w1) {
+  r = gid / split;
+  uint32_t part = (uint32_t)(gid - r * split);
+  if (r >= R.n_reads) return false;
+  uint32_t len = R.lens[r];
+  if (len < k) return false;
+  uint32_t units = len - k + 1;
+  w0 = (uint32_t)((uint64_t)part * units / split);
+  w1 = (uint32_t)((uint64_t)(part + 1) * units / split);
+  return w0 < w1;
+}
+
+// Calls f(cell) for the 4 cells of every window of this thread's share and,
+// when HLL, updates the HLL registers.
+template <int L, bool HLL, class F>
+__device__ __forceinline__ void walk_cells(const BinArgs& a, uint64_t r, uint32_t w0, uint32_t w1, F&& f) {
+  const uint32_t k = a.k;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r] + w0;
+  Kmer<L> win = window_bits<L>(data, g, k);
+  Kmer<L> fwd = fwd_from_window<L>(win, k);
+  Kmer<L> rev = rc_from_window<L>(win, k);
+  BaseStream in;
+  in.init(data, g + k);
+  NtHash nh;
+  BaseStream outs;
+  if (HLL) {
+    nh.f = 0;
+    nh.r = 0;
+    BaseStream s0;
+    s0.init(data, g);
+    for (uint32_t j = 0; j < k; ++j) nh.push(s0.next(), k);
+    outs.init(data, g);
+  }
+  const uint32_t lw = a.log2w;
+  const uint64_t wmask = (1ull << lw) - 1ull;
+  for (uint32_t w = w0;;) {
+    const uint64_t key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
+#pragma unroll
+    for (int row = 0; row < 4; ++row) f(((uint64_t)row << lw) | (splitmix64(key ^ cms_seed(row)) & wmask));
+    if (HLL) {
+      uint32_t idx, rho;
+      hll_index_rho(splitmix64(nh.canonical()), a.hll.p, idx, rho);
+      if (rho > a.hll.regs[idx]) atomicMax(a.hll.regs + idx, rho);
+    }
+    if (++w >= w1) break;
+    const uint32_t b = in.next();
+    kmer_roll<L>(fwd, rev, b, k);
+    if (HLL) nh.roll(outs.next(), b, k);
+  }
+}
+
+template <int L, bool HLL>
+__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinArgs a) {
+  extern __shared__ uint32_t sh[];
+  uint32_t* hist = sh;
+  uint32_t* cur = sh + a.nbins;
+  for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t r = 0;
+  uint32_t w0 = 0, w1 = 0;
+  const bool valid = share_of(a.reads, gid, a.split, a.k, r, w0, w1);
+  const uint32_t sb = a.sb;
+  if (valid) walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(hist + (cell >> sb), 1u); });
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
+    const uint32_t c = hist[b];
+    cur[b] = c ? atomicAdd(a.cursor + b, c) : 0u;
+  }
+  __syncthreads();
+  if (valid) {
+    const uint64_t cap = a.cap;
+    const uint32_t smask = (1u << sb) - 1u;
+    uint16_t* __restrict__ out = a.data;
+    uint32_t* __restrict__ counts = a.counts;
+    walk_cells<L, false>(a, r, w0, w1, [&](uint64_t cell) {
+      const uint32_t b = (uint32_t)(cell >> sb);
+      const uint32_t pos = atomicAdd(cur + b, 1u);
+      if (pos < cap) out[(uint64_t)b * cap + pos] = (uint16_t)(cell & smask);
+      else atomicAdd(counts + cell, 1u);
+    });
+  }
+}
+
+__global__ __launch_bounds__(kCountBlock) void k_bin_count(BinArgs a) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t b = blockIdx.x;
+  const uint32_t S = 1u << a.sb;
+  const uint64_t first = (uint64_t)b << a.sb;
+  const uint32_t ncells = (uint32_t)((a.n_cells - first) < S ? (a.n_cells - first) : S);
+  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) lds[i] = 0;
+  __syncthreads();
+  const uint32_t filled = a.cursor[b];
+  const uint32_t n = filled < a.cap ? filled : (uint32_t)a.cap;
+  const uint16_t* __restrict__ d = a.data + (uint64_t)b * a.cap;  // cap is a multiple of 4
+  const uint32_t n4 = n & ~3u;
+  for (uint32_t i = threadIdx.x * 4u; i < n4; i += blockDim.x * 4u) {
+    const uint2 v = *reinterpret_cast<const uint2*>(d + i);
+    atomicAdd(lds + (v.x & 0xFFFFu), 1u);
+    atomicAdd(lds + (v.x >> 16), 1u);
+    atomicAdd(lds + (v.y & 0xFFFFu), 1u);
+    atomicAdd(lds + (v.y >> 16), 1u);
+  }
+  for (uint32_t i = n4 + threadIdx.x; i < n; i += blockDim.x) atomicAdd(lds + d[i], 1u);
+  __syncthreads();
+  // add into the table and emit the threshold bits, 64 cells per wave step
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (uint32_t base = wave * 64u; base < ncells; base += nwaves * 64u) {
+    const uint32_t i = base + lane;
+    bool on = false;
+    if (i < ncells) {
+      const uint64_t cell = first + i;
+      const uint32_t v = a.counts[cell] + lds[i];
+      a.counts[cell] = v;
+      const uint32_t c = v < a.cap_value ? v : a.cap_value;
+      on = c >= a.t_solid;
+    }
+    const uint64_t m = __ballot(on);
+    if (a.bits && lane == 0) {
+      const uint64_t word = (first + base) >> 5;
+      const uint64_t nwords = (a.n_cells + 31) >> 5;
+      if (word < nwords) a.bits[word] = (uint32_t)m;
+      if (word + 1 < nwords) a.bits[word + 1] = (uint32_t)(m >> 32);
+    }
+  }
+}
+
+size_t bin_scatter_lds(uint32_t nbins) { return (size_t)nbins * 2 * sizeof(uint32_t); }
+
+hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
+  if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
+  if (a.reads.n_reads) {
+    const uint32_t L = (2 * a.k + 63) / 64;
+    const uint64_t threads = a.reads.n_reads * (uint64_t)a.split;
+    const unsigned grid = (unsigned)((threads + kScatterBlock - 1) / kScatterBlock);
+    const size_t lds = bin_scatter_lds(a.nbins);
+    const bool h = a.hll.regs != nullptr;
+#define BS(LL)                                                                                              \
+  if (h) hipLaunchKernelGGL((k_bin_scatter<LL, true>), dim3(grid), dim3(kScatterBlock), lds, s, a);        \
+  else hipLaunchKernelGGL((k_bin_scatter<LL, false>), dim3(grid), dim3(kScatterBlock), lds, s, a)
+    switch (L) {
+      case 1: BS(1); break;
+      case 2: BS(2); break;
+      case 3: BS(3); break;
+      default: BS(4); break;
+    }
+#undef BS
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_bin_count, dim3(a.nbins), dim3(kCountBlock), (size_t)(1u << a.sb) * 4, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace fb

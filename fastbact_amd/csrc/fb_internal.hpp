@@ -63,6 +63,27 @@ struct Pass2Args {
 
 // Pass 1 (build.rs:93-138) with the per-pass HLL over splitmix64(ntHash).
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s);
+
+// Pass 1 with LDS-privatised Count-Min slices (fb_binned.hip).
+struct BinArgs {
+  DevReads reads;
+  uint32_t k = 0;
+  uint32_t split = 1;
+  uint32_t log2w = 0;
+  uint64_t n_cells = 0;        // 4 * width
+  uint32_t* counts = nullptr;  // exact u32 cells
+  DevHll hll;
+  uint32_t sb = 14;            // log2 cells per bin (<= 16)
+  uint32_t nbins = 0;
+  uint64_t cap = 0;            // offsets per bin region, multiple of 4
+  uint16_t* data = nullptr;    // nbins * cap cell offsets
+  uint32_t* cursor = nullptr;  // nbins fill counters, zeroed before phase A
+  uint32_t* bits = nullptr;    // optional threshold bits of count >= t_solid
+  uint32_t t_solid = 0;
+  uint32_t cap_value = 65535;  // counter saturation (15 or 65535)
+};
+size_t bin_scatter_lds(uint32_t nbins);
+hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s);
 // Bits of count >= t (saturating-counter semantics: cap 15 or 65535).
 hipError_t launch_solid_bits(const uint32_t* counts, uint64_t n_counters, uint32_t t, uint32_t cap,
                              uint32_t* bits, hipStream_t s);

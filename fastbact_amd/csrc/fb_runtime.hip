@@ -609,31 +609,67 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   cview.n_reads = c_reads;
   const uint32_t split = prm->split ? prm->split : default_split(k, bank->max_len);
 
+  // pass 1: LDS-binned Count-Min slices or direct global atomics
+  const uint64_t ncnt = so->cms.width * 4;
+  const uint64_t windows = bank_windows(bank, k);
+  uint32_t sb = 14;
+  while (sb > 6 && (1ull << sb) > ncnt) --sb;
+  const uint64_t nbins = std::max<uint64_t>(1, ncnt >> sb);
+  const bool binned = prm->placement == 2 || (prm->placement == 0 && nbins <= 8192 && windows >= (1u << 20));
+  if (prm->placement == 2 && nbins > 8192)
+    return fail(set_err(c, FB_ERR_INVALID, "LDS placement needs at most 8192 bins (width %llu)",
+                        (unsigned long long)so->cms.width));
+  DBuf bits, bdata, bcur;
+  if (int r = bits.get(c, ((ncnt + 31) / 32) * 4)) return fail(r);
+  BinArgs ba;
+  if (binned) {
+    const double expected = 4.0 * (double)windows / (double)nbins;
+    ba.cap = ((uint64_t)(expected * 1.1) + 256 + 3) & ~3ull;
+    ba.sb = sb;
+    ba.nbins = (uint32_t)nbins;
+    if (int r = bdata.get(c, nbins * ba.cap * 2)) return fail(r);
+    if (int r = bcur.get(c, nbins * 4)) return fail(r);
+    HIP_TRY(c, launch_fill_u32(bcur.as<uint32_t>(), nbins, 0, s));
+  }
   HIP_TRY(c, hipEventRecord(c->ev[0], s));
-  // pass 1
-  Pass1Args p1{};
-  p1.reads = bank->view();
-  p1.k = k;
-  p1.split = split;
-  p1.weight = 1;
-  p1.cms = so->cms.dev();
-  if (so->has_hll) p1.hll = so->hll.dev();
-  HIP_TRY(c, launch_pass1(p1, s));
-  if (c_reads) {
-    Pass1Args pc = p1;
+  if (c_reads) {  // prior contigs: every k-mer added 10 times (build.rs:119-138)
+    Pass1Args pc{};
     pc.reads = cview;
+    pc.k = k;
     pc.split = 1;
     pc.weight = 10;
-    pc.hll = DevHll{};
+    pc.cms = so->cms.dev();
     HIP_TRY(c, launch_pass1(pc, s));
   }
+  if (binned) {
+    ba.reads = bank->view();
+    ba.k = k;
+    ba.split = split;
+    ba.log2w = so->cms.log2w;
+    ba.n_cells = ncnt;
+    ba.counts = so->cms.counts.as<uint32_t>();
+    if (so->has_hll) ba.hll = so->hll.dev();
+    ba.data = bdata.as<uint16_t>();
+    ba.cursor = bcur.as<uint32_t>();
+    ba.bits = bits.as<uint32_t>();
+    ba.t_solid = t_solid;
+    ba.cap_value = so->cms.cap();
+    HIP_TRY(c, launch_bin_pass1(ba, s));
+  } else {
+    Pass1Args p1{};
+    p1.reads = bank->view();
+    p1.k = k;
+    p1.split = split;
+    p1.weight = 1;
+    p1.cms = so->cms.dev();
+    if (so->has_hll) p1.hll = so->hll.dev();
+    HIP_TRY(c, launch_pass1(p1, s));
+  }
   HIP_TRY(c, hipEventRecord(c->ev[1], s));
-
-  // threshold bitmap of the pass-2 gate
-  const uint64_t ncnt = so->cms.width * 4;
-  DBuf bits;
-  if (int r = bits.get(c, ((ncnt + 31) / 32) * 4)) return fail(r);
-  HIP_TRY(c, launch_solid_bits(so->cms.counts.as<uint32_t>(), ncnt, t_solid, so->cms.cap(), bits.as<uint32_t>(), s));
+  if (!binned)  // threshold bitmap of the pass-2 gate
+    HIP_TRY(c, launch_solid_bits(so->cms.counts.as<uint32_t>(), ncnt, t_solid, so->cms.cap(), bits.as<uint32_t>(), s));
+  bdata.release();
+  bcur.release();
 
   // pass 2 into the solid table; grows and re-runs on overflow
   DevSolidTable tab;

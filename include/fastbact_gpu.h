@@ -127,6 +127,7 @@ typedef struct {
   int32_t hll_p;              /* 0 = no per-pass HLL; else precision of the pass HLL */
   uint32_t sort;              /* 1 = records sorted by k-mer (parity form) */
   uint32_t split;             /* threads per read; 0 = tuned default */
+  uint32_t placement;         /* pass-1 sketch placement: 0 auto, 1 global atomics, 2 LDS bins */
 } fb_build_params;
 
 void fb_build_params_default(fb_build_params* p, uint32_t k);

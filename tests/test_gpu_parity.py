@@ -24,12 +24,13 @@ def small_iso():
     return isolate(seed=21, genome_len=30_000, n_pairs=2500)
 
 
+@pytest.mark.parametrize("placement", [1, 2])
 @pytest.mark.parametrize("k", [21, 31, 32, 33, 55, 63, 64, 65, 77, 99, 127])
-def test_build_solid_matches_oracle_all_widths(ctx, orc, small_iso, k):
+def test_build_solid_matches_oracle_all_widths(ctx, orc, small_iso, k, placement):
     bank = small_iso.bank()
     dev = F.DeviceBank(ctx, bank)
     width = 1 << 18
-    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=3, cms_width=width, hll_p=14)
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=3, cms_width=width, hll_p=14, placement=placement)
     r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=3, width=width, hll_p=14)
     _solid_equal(g, r)
     assert np.array_equal(g.cms_table, r["cms"].table())
@@ -61,11 +62,12 @@ def test_reference_cms_width_from_genome_size(ctx, orc, small_iso):
         assert g.cms_table.size == w * 2
 
 
+@pytest.mark.parametrize("placement", [1, 2])
 @pytest.mark.parametrize("k", [21, 31, 55, 77, 127])
-def test_ragged_reads_and_non_acgt(ctx, orc, k):
+def test_ragged_reads_and_non_acgt(ctx, orc, k, placement):
     bank = ragged_bank(seed=k)
     dev = F.DeviceBank(ctx, bank)
-    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=1, cms_width=1 << 14, hll_p=12)
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=1, cms_width=1 << 14, hll_p=12, placement=placement)
     r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=1, width=1 << 14, hll_p=12)
     _solid_equal(g, r)
     assert np.array_equal(g.cms_table, r["cms"].table())
@@ -99,9 +101,11 @@ def test_saturation_nibble_and_u16(ctx, orc):
     seq = b"A" * 20000
     bank = F.ReadBank(F.BitSeqArena.from_ascii([seq] * 5), F.BitSeqArena(), F.BitSeqArena())
     dev = F.DeviceBank(ctx, bank)
-    for bits in (4, 16):
-        g = F.build_solid_kmers(ctx, dev, 31, cms_width=1 << 12, counter_bits=bits)
-        r = orc.build_solid(oracle_arenas(bank), 31, width=1 << 12, counter_bits=bits)
+    # width 2^16 with LDS bins: every update lands in at most 4 of 16 bins, so
+    # the bin regions overflow and the direct-atomic spill path runs
+    for bits, width, placement in ((4, 1 << 12, 1), (16, 1 << 12, 2), (4, 1 << 16, 2), (16, 1 << 16, 1)):
+        g = F.build_solid_kmers(ctx, dev, 31, cms_width=width, counter_bits=bits, placement=placement)
+        r = orc.build_solid(oracle_arenas(bank), 31, width=width, counter_bits=bits)
         _solid_equal(g, r)
         assert np.array_equal(g.cms_table, r["cms"].table())
         assert len(g) == 1 and g.cov[0] == 65535 // 2
@@ -213,12 +217,14 @@ def test_cfg1_scale_plain_isolate(ctx, orc):
                           insert_sd=30, sub_rate=0.005)
     bank = iso.bank()
     dev = F.DeviceBank(ctx, bank)
-    g = F.build_solid_kmers(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 22, counter_bits=16, hll_p=14)
     r = orc.build_solid(oracle_arenas(bank), 31, target_solid_cov=3, width=1 << 22, counter_bits=16, hll_p=14,
                         threads=8)
-    _solid_equal(g, r)
-    assert np.array_equal(g.cms_table, r["cms"].table())
-    assert np.array_equal(g.hll_registers, r["hll"])
+    for placement in (1, 2):
+        g = F.build_solid_kmers(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 22, counter_bits=16, hll_p=14,
+                                placement=placement)
+        _solid_equal(g, r)
+        assert np.array_equal(g.cms_table, r["cms"].table())
+        assert np.array_equal(g.hll_registers, r["hll"])
     # HLL of distinct 31-mers within the reference's 3% gate of the exact count
     est = F.hll_estimate_registers(g.hll_registers, 14)
     assert 0.5e6 < len(g) < 1.1e6

@@ -5,6 +5,7 @@ HyperLogLog registers, the genome estimator and the solid-k-mer set, as HIP
 kernels behind the C ABI of include/fastbact_gpu.h.
 """
 from .api import (  # noqa: F401
+    Batch,
     BitSeqArena,
     Context,
     CountMinSketch,
@@ -14,6 +15,7 @@ from .api import (  # noqa: F401
     HyperLogLog,
     ReadBank,
     SolidSet,
+    batch_footprint,
     build_solid_handle,
     build_solid_kmers,
     compute_dynamic_kmers,

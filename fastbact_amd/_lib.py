@@ -157,6 +157,18 @@ SIGNATURES = [
     ("fb_solid_hll_registers", C.c_int, [vp, u8p]),
     ("fb_solid_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_solid_stats", C.c_int, [vp, u64p]),
+    ("fb_batch_create", C.c_int, [vp, C.POINTER(vp), u64p, C.c_uint32, C.POINTER(BuildParams), C.POINTER(vp)]),
+    ("fb_batch_destroy", None, [vp]),
+    ("fb_batch_footprint", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(BuildParams), C.c_uint32]),
+    ("fb_batch_run", C.c_int, [vp, C.c_uint32, C.c_int32]),
+    ("fb_batch_size", C.c_uint32, [vp]),
+    ("fb_batch_count", C.c_uint64, [vp, C.c_uint32]),
+    ("fb_batch_key_words", C.c_int, [vp]),
+    ("fb_batch_download", C.c_int, [vp, C.c_uint32, u64p, u16p, u8p]),
+    ("fb_batch_cms_bytes", C.c_uint64, [vp, C.c_uint32]),
+    ("fb_batch_cms_download", C.c_int, [vp, C.c_uint32, vp, C.c_uint64]),
+    ("fb_batch_hll_registers", C.c_int, [vp, C.c_uint32, u8p]),
+    ("fb_batch_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_count_stats_from_lengths", C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(CountStats)]),
     ("fb_compute_dynamic_kmers", C.c_uint32, [C.c_uint64, C.c_int64, u32p]),
     ("fb_estimate_genome", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.POINTER(CountStats),

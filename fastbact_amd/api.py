@@ -386,6 +386,68 @@ def build_solid_kmers(ctx: Context, bank: DeviceBank, k: int, target_solid_cov: 
 
 
 # ---------------------------------------------------------------------------
+# Batched form
+
+
+class Batch(_Handle):
+    """Many isolates per launch (include/fastbact_gpu.h, fb_batch_*): one
+    launch per kernel covers every isolate and each pass replays a HIP graph."""
+
+    _destroy = "fb_batch_destroy"
+
+    def __init__(self, ctx: Context, banks: Sequence[DeviceBank], genome_sizes: Sequence[int],
+                 cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
+                 split: int = 0, placement: int = 0):
+        p = build_params(0, cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, sort=sort, split=split,
+                         placement=placement)
+        arr = (C.c_void_p * len(banks))(*[b.h.value if hasattr(b.h, "value") else b.h for b in banks])
+        gs = _u64(genome_sizes)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.fb_batch_create(ctx.h, arr, _ptr(gs, C.c_uint64), len(banks), C.byref(p), C.byref(h)),
+                  "fb_batch_create")
+        super().__init__(ctx, h)
+        self.banks = list(banks)  # keep the device banks alive
+        self.counter_bits, self.hll_p = counter_bits, hll_p
+
+    def __len__(self) -> int:
+        return int(self.ctx.lib.fb_batch_size(self.h))
+
+    def run(self, k: int, target_solid_cov: Optional[int] = None) -> None:
+        self.ctx.check(self.ctx.lib.fb_batch_run(self.h, k, -1 if target_solid_cov is None else target_solid_cov),
+                       "fb_batch_run")
+
+    def timings(self) -> tuple:
+        t = (C.c_float * 4)()
+        self.ctx.check(self.ctx.lib.fb_batch_timings(self.h, t), "fb_batch_timings")
+        return tuple(float(x) for x in t)
+
+    def result(self, i: int, k: int, with_cms: bool = True) -> SolidSet:
+        n = int(self.ctx.lib.fb_batch_count(self.h, i))
+        nw = int(self.ctx.lib.fb_batch_key_words(self.h))
+        keys = np.zeros((n, nw), np.uint64)
+        cov = np.zeros(n, np.uint16)
+        mask = np.zeros(n, np.uint8)
+        self.ctx.check(self.ctx.lib.fb_batch_download(self.h, i, _ptr(keys, C.c_uint64), _ptr(cov, C.c_uint16),
+                                                      _ptr(mask, C.c_uint8)), "fb_batch_download")
+        cms = regs = None
+        if with_cms:
+            nb = int(self.ctx.lib.fb_batch_cms_bytes(self.h, i))
+            buf = np.zeros(nb, np.uint8)
+            self.ctx.check(self.ctx.lib.fb_batch_cms_download(self.h, i, buf.ctypes.data, nb), "fb_batch_cms_download")
+            cms = buf if self.counter_bits == 4 else buf.view(np.uint16)
+        if self.hll_p:
+            regs = np.zeros(1 << self.hll_p, np.uint8)
+            self.ctx.check(self.ctx.lib.fb_batch_hll_registers(self.h, i, _ptr(regs, C.c_uint8)),
+                           "fb_batch_hll_registers")
+        return SolidSet(k, keys, cov, mask, cms, regs, self.timings())
+
+
+def batch_footprint(genome_size: int, bases: int, reads: int, k: int, **kw) -> int:
+    p = build_params(k, **kw)
+    return int(_lib.load().fb_batch_footprint(genome_size, bases, reads, C.byref(p), k))
+
+
+# ---------------------------------------------------------------------------
 # Estimator
 
 CONFIDENCE = {0: "HIGH", 1: "LOW_COVERAGE_WARNING", 2: "UNRELIABLE"}

@@ -76,13 +76,11 @@ __device__ __forceinline__ void walk_cells(const BinArgs& a, uint64_t r, uint32_
 }
 
 template <int L, bool HLL>
-__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinArgs a) {
-  extern __shared__ uint32_t sh[];
+__device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t gid, uint32_t* sh) {
   uint32_t* hist = sh;
   uint32_t* cur = sh + a.nbins;
   for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
   __syncthreads();
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t r = 0;
   uint32_t w0 = 0, w1 = 0;
   const bool valid = share_of(a.reads, gid, a.split, a.k, r, w0, w1);
@@ -108,9 +106,21 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinArgs a) {
   }
 }
 
-__global__ __launch_bounds__(kCountBlock) void k_bin_count(BinArgs a) {
-  extern __shared__ uint32_t lds[];
-  const uint32_t b = blockIdx.x;
+template <int L, bool HLL>
+__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinArgs a) {
+  extern __shared__ uint32_t sh[];
+  bin_scatter_body<L, HLL>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, sh);
+}
+
+template <int L, bool HLL>
+__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter_m(const BinArgs* __restrict__ args,
+                                                                 const uint32_t* __restrict__ blk, uint32_t n_iso) {
+  extern __shared__ uint32_t sh[];
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  bin_scatter_body<L, HLL>(args[iso], (uint64_t)(blockIdx.x - blk[iso]) * blockDim.x + threadIdx.x, sh);
+}
+
+__device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uint32_t* lds) {
   const uint32_t S = 1u << a.sb;
   const uint64_t first = (uint64_t)b << a.sb;
   const uint32_t ncells = (uint32_t)((a.n_cells - first) < S ? (a.n_cells - first) : S);
@@ -151,6 +161,18 @@ __global__ __launch_bounds__(kCountBlock) void k_bin_count(BinArgs a) {
   }
 }
 
+__global__ __launch_bounds__(kCountBlock) void k_bin_count(BinArgs a) {
+  extern __shared__ uint32_t lds[];
+  bin_count_body(a, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(kCountBlock) void k_bin_count_m(const BinArgs* __restrict__ args,
+                                                             const uint32_t* __restrict__ blk, uint32_t n_iso) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  bin_count_body(args[iso], blockIdx.x - blk[iso], lds);
+}
+
 size_t bin_scatter_lds(uint32_t nbins) { return (size_t)nbins * 2 * sizeof(uint32_t); }
 
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
@@ -175,6 +197,32 @@ hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_bin_count, dim3(a.nbins), dim3(kCountBlock), (size_t)(1u << a.sb) * 4, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s) {
+  if (!m.total_blocks) return hipSuccess;
+  const uint32_t L = (2 * k + 63) / 64;
+  const BinArgs* a = static_cast<const BinArgs*>(m.args);
+  const size_t lds = bin_scatter_lds(max_nbins);
+#define BSM(LL)                                                                                                   \
+  if (hll) hipLaunchKernelGGL((k_bin_scatter_m<LL, true>), dim3(m.total_blocks), dim3(kScatterBlock), lds, s, a, m.blk, \
+                              m.n_iso);                                                                           \
+  else hipLaunchKernelGGL((k_bin_scatter_m<LL, false>), dim3(m.total_blocks), dim3(kScatterBlock), lds, s, a, m.blk, m.n_iso)
+  switch (L) {
+    case 1: BSM(1); break;
+    case 2: BSM(2); break;
+    case 3: BSM(3); break;
+    default: BSM(4); break;
+  }
+#undef BSM
+  return hipGetLastError();
+}
+
+hipError_t launch_bin_count_multi(const MultiLaunch& m, uint32_t max_sb, hipStream_t s) {
+  if (!m.total_blocks) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_count_m, dim3(m.total_blocks), dim3(kCountBlock), (size_t)(1u << max_sb) * 4, s,
+                     static_cast<const BinArgs*>(m.args), m.blk, m.n_iso);
   return hipGetLastError();
 }
 

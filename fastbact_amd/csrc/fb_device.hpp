@@ -200,4 +200,16 @@ __device__ __forceinline__ void hll_index_rho(uint64_t h, uint32_t p, uint32_t& 
   rho = w ? (uint32_t)__builtin_clzll(w) - p + 1u : 64u - p + 1u;
 }
 
+// Isolate of block b in a multi-isolate launch: the largest i with blk[i] <= b
+// (blk holds n+1 block prefixes, blk[0] = 0).
+__device__ __forceinline__ uint32_t find_iso(const uint32_t* __restrict__ blk, uint32_t n, uint32_t b) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (blk[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace fbk

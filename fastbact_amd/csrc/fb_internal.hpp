@@ -64,6 +64,19 @@ struct Pass2Args {
 // Pass 1 (build.rs:93-138) with the per-pass HLL over splitmix64(ntHash).
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s);
 
+// A launch over many isolates: args is a device array of per-isolate
+// argument structs, blk a device array of n_iso+1 block prefixes; block b
+// works for the isolate i with blk[i] <= b < blk[i+1], so no tile crosses
+// isolates.
+struct MultiLaunch {
+  const void* args = nullptr;
+  const uint32_t* blk = nullptr;
+  uint32_t n_iso = 0;
+  uint32_t total_blocks = 0;
+};
+constexpr uint32_t kPassBlock = 256;   // block size of k_pass1 / k_pass2
+hipError_t launch_pass1_multi(const MultiLaunch& m, uint32_t k, bool cms, bool hll, hipStream_t s);
+
 // Pass 1 with LDS-privatised Count-Min slices (fb_binned.hip).
 struct BinArgs {
   DevReads reads;
@@ -84,9 +97,15 @@ struct BinArgs {
 };
 size_t bin_scatter_lds(uint32_t nbins);
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s);
+constexpr uint32_t kScatterBlockSize = 512;  // block size of the bin scatter kernel
+hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s);
+// m.blk counts bins: one block per bin.
+hipError_t launch_bin_count_multi(const MultiLaunch& m, uint32_t max_sb, hipStream_t s);
 // Bits of count >= t (saturating-counter semantics: cap 15 or 65535).
 hipError_t launch_solid_bits(const uint32_t* counts, uint64_t n_counters, uint32_t t, uint32_t cap,
                              uint32_t* bits, hipStream_t s);
+// Sum of set bits of n u32 words added to *out (zeroed by the caller).
+hipError_t launch_popcount(const uint32_t* words, uint64_t n, unsigned long long* out, hipStream_t s);
 // Reference-layout packing of the exact counts.
 hipError_t launch_pack_nibble(const uint32_t* counts, uint64_t n_counters, uint8_t* out, hipStream_t s);
 hipError_t launch_pack_u16(const uint32_t* counts, uint64_t n_counters, uint16_t* out, hipStream_t s);
@@ -96,15 +115,37 @@ hipError_t launch_table_clear(const DevSolidTable& t, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 // Pass 2 (build.rs:140-289).
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
+hipError_t launch_pass2_multi(const MultiLaunch& m, uint32_t k, hipStream_t s);
 // Pass 3 (build.rs:291-328): compaction into reference key words.
 // out_keys: n_ref_words words per record, most significant first.
+// scratch (pass3_scratch_bytes()): u32 [0] records kept, u64 at byte 8 the sum
+// of raw cov (records added to the table), then per-block partials.
+size_t pass3_scratch_bytes();
 hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t n_ref_words, uint32_t min_trans_cov,
-                        uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, uint32_t* out_count,
-                        unsigned long long* rec_sum, hipStream_t s);
+                        uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s);
 // Sort the compacted records by k-mer (integer order).  Uses scratch from
 // the caller: tmp_bytes queried with keys == nullptr.
 hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t n_ref_words,
                       uint32_t k, void* tmp, size_t* tmp_bytes, hipStream_t s);
+
+// Multi-table forms for the batched pass (blockIdx.y = isolate).
+struct P3Args {
+  DevSolidTable t;
+  uint32_t nref = 1;
+  uint32_t gate = 0;
+  uint64_t* out_keys = nullptr;
+  uint16_t* out_cov = nullptr;
+  uint8_t* out_mask = nullptr;
+  void* scratch = nullptr;  // pass3_scratch_bytes()
+};
+hipError_t launch_pass3_multi(const P3Args* args, uint32_t n, uint32_t kw, hipStream_t s);
+struct FillRange {
+  uint32_t* p;
+  uint64_t n;
+  uint32_t v;
+};
+hipError_t launch_fill_ranges(const FillRange* r, uint32_t n, hipStream_t s);
+hipError_t launch_table_clear_multi(const DevSolidTable* tabs, uint32_t n, hipStream_t s);
 
 // CountMinSketch::add / estimate on explicit 64-bit keys.
 hipError_t launch_cms_add_keys(DevCms c, const uint64_t* keys, uint64_t n, uint32_t weight, hipStream_t s);

@@ -41,8 +41,7 @@ __device__ __forceinline__ bool read_share(const DevReads& R, uint64_t gid, uint
 // ---------------------------------------------------------------------------
 // Pass 1
 template <int L, bool CMS, bool HLL>
-__global__ __launch_bounds__(kBlock) void k_pass1(Pass1Args a) {
-  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pass1_body(const Pass1Args& a, uint64_t gid) {
   uint64_t r;
   uint32_t w0, w1;
   const uint32_t k = a.k;
@@ -88,6 +87,19 @@ __global__ __launch_bounds__(kBlock) void k_pass1(Pass1Args a) {
   }
 }
 
+template <int L, bool CMS, bool HLL>
+__global__ __launch_bounds__(kBlock) void k_pass1(Pass1Args a) {
+  pass1_body<L, CMS, HLL>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// One launch over many isolates: block b belongs to isolate find_iso(b).
+template <int L, bool CMS, bool HLL>
+__global__ __launch_bounds__(kBlock) void k_pass1_m(const Pass1Args* __restrict__ args, const uint32_t* __restrict__ blk,
+                                                    uint32_t n_iso) {
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  pass1_body<L, CMS, HLL>(args[iso], (uint64_t)(blockIdx.x - blk[iso]) * blockDim.x + threadIdx.x);
+}
+
 hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
   if (a.reads.n_reads == 0) return hipSuccess;
   const uint32_t L = (2 * a.k + 63) / 64;
@@ -106,6 +118,24 @@ hipError_t launch_pass1(const Pass1Args& a, hipStream_t s) {
     default: P1(4); break;
   }
 #undef P1
+  return hipGetLastError();
+}
+
+hipError_t launch_pass1_multi(const MultiLaunch& m, uint32_t k, bool cms, bool hll, hipStream_t s) {
+  if (!m.total_blocks) return hipSuccess;
+  const uint32_t L = (2 * k + 63) / 64;
+  const Pass1Args* a = static_cast<const Pass1Args*>(m.args);
+#define P1M(LL)                                                                                                       \
+  if (hll && cms) hipLaunchKernelGGL((k_pass1_m<LL, true, true>), dim3(m.total_blocks), dim3(kBlock), 0, s, a, m.blk, m.n_iso); \
+  else if (cms) hipLaunchKernelGGL((k_pass1_m<LL, true, false>), dim3(m.total_blocks), dim3(kBlock), 0, s, a, m.blk, m.n_iso); \
+  else hipLaunchKernelGGL((k_pass1_m<LL, false, true>), dim3(m.total_blocks), dim3(kBlock), 0, s, a, m.blk, m.n_iso)
+  switch (L) {
+    case 1: P1M(1); break;
+    case 2: P1M(2); break;
+    case 3: P1M(3); break;
+    default: P1M(4); break;
+  }
+#undef P1M
   return hipGetLastError();
 }
 
@@ -132,6 +162,31 @@ __global__ __launch_bounds__(kBlock) void k_solid_bits(const uint32_t* __restric
 hipError_t launch_solid_bits(const uint32_t* counts, uint64_t n, uint32_t t, uint32_t cap, uint32_t* bits,
                              hipStream_t s) {
   hipLaunchKernelGGL(k_solid_bits, dim3(grid_for(n)), dim3(kBlock), 0, s, counts, n, t, cap, bits);
+  return hipGetLastError();
+}
+
+// Popcount of n words, one atomic per block.
+__global__ __launch_bounds__(kBlock) void k_popcount(const uint32_t* __restrict__ w, uint64_t n,
+                                                     unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[kBlock / 64];
+  unsigned long long c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    c += (unsigned long long)__builtin_popcount(w[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / 64; ++i) t += part[i];
+    if (t) atomicAdd(out, t);
+  }
+}
+hipError_t launch_popcount(const uint32_t* words, uint64_t n, unsigned long long* out, hipStream_t s) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 1024) g = 1024;
+  if (g == 0) g = 1;
+  hipLaunchKernelGGL(k_popcount, dim3((unsigned)g), dim3(kBlock), 0, s, words, n, out);
   return hipGetLastError();
 }
 
@@ -279,8 +334,7 @@ __device__ __forceinline__ void add_records(uint32_t* c, uint32_t nrec, bool lef
 }
 
 template <int L, int KW>
-__global__ __launch_bounds__(kBlock) void k_pass2(Pass2Args a) {
-  uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pass2_body(const Pass2Args& a, uint64_t gid) {
   uint64_t r;
   uint32_t p0, p1;  // pairs [p0, p1): pair x joins windows x and x+1
   const uint32_t k = a.k;
@@ -339,6 +393,18 @@ __global__ __launch_bounds__(kBlock) void k_pass2(Pass2Args a) {
   }
 }
 
+template <int L, int KW>
+__global__ __launch_bounds__(kBlock) void k_pass2(Pass2Args a) {
+  pass2_body<L, KW>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int L, int KW>
+__global__ __launch_bounds__(kBlock) void k_pass2_m(const Pass2Args* __restrict__ args, const uint32_t* __restrict__ blk,
+                                                    uint32_t n_iso) {
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  pass2_body<L, KW>(args[iso], (uint64_t)(blockIdx.x - blk[iso]) * blockDim.x + threadIdx.x);
+}
+
 static uint32_t key_words(uint32_t k) { return (k + 30) / 31; }
 
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s) {
@@ -360,74 +426,290 @@ hipError_t launch_pass2(const Pass2Args& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// Pass 3
-
-template <int KW>
-__global__ __launch_bounds__(kBlock) void k_pass3(const uint64_t* __restrict__ slots, uint64_t cap, uint32_t stride,
-                                                  uint32_t k, uint32_t nref, uint32_t gate,
-                                                  uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
-                                                  uint8_t* __restrict__ out_mask, uint32_t* __restrict__ count,
-                                                  unsigned long long* __restrict__ rec_sum) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool keep = false;
-  uint32_t mask = 0, cov = 0;
-  unsigned long long raw_cov = 0;
-  uint64_t wk[KW];
-  if (i < cap) {
-    const uint64_t* s = slots + i * stride;
-#pragma unroll
-    for (int j = 0; j < KW; ++j) wk[j] = s[j];
-    if (wk[0] != kEmpty) {
-      const uint32_t* c = reinterpret_cast<const uint32_t*>(s + KW);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        uint32_t v = c[2 + e];
-        v = v < 65535u ? v : 65535u;
-        if (v >= gate) mask |= 1u << e;
-      }
-      raw_cov = c[0];
-      uint32_t cv = c[0] < 65535u ? c[0] : 65535u;
-      cov = cv / 2u;
-      if (cov < 1u) cov = 1u;
-      keep = mask != 0u;
-    }
-  }
-  // records added to the table (sum of raw cov), one atomic per wave
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) raw_cov += __shfl_xor(raw_cov, off);
-  uint32_t lane = threadIdx.x & 63u;
-  if (lane == 0 && raw_cov) atomicAdd(rec_sum, raw_cov);
-  // wave-aggregated output slot
-  uint64_t ballot = __ballot(keep);
-  uint32_t leader = ballot ? (uint32_t)__builtin_ctzll(ballot) : 0u;
-  uint32_t base = 0;
-  if (ballot && lane == leader) base = atomicAdd(count, (uint32_t)__builtin_popcountll(ballot));
-  base = __shfl(base, (int)leader);
-  if (!keep) return;
-  uint32_t pos = base + (uint32_t)__builtin_popcountll(ballot & ((1ull << lane) - 1ull));
-  // 62-bit words -> 64-bit limbs (least significant first)
-  uint64_t limb[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int j = 0; j < KW; ++j) {
-    const int bit = 62 * j, li = bit / 64, off = bit % 64;
-    if (li < 4) limb[li] |= wk[j] << off;
-    if (off > 2 && li + 1 < 4) limb[li + 1] |= wk[j] >> (64 - off);
-  }
-  // reference words, most significant first
-  for (uint32_t w = 0; w < nref; ++w) out_keys[(uint64_t)pos * nref + w] = limb[nref - 1 - w];
-  out_cov[pos] = (uint16_t)cov;
-  out_mask[pos] = (uint8_t)mask;
-  (void)k;
+hipError_t launch_pass2_multi(const MultiLaunch& m, uint32_t k, hipStream_t s) {
+  if (!m.total_blocks) return hipSuccess;
+  const uint32_t L = (2 * k + 63) / 64;
+  const uint32_t KW = key_words(k);
+  const Pass2Args* a = static_cast<const Pass2Args*>(m.args);
+#define P2M(LL, KK) hipLaunchKernelGGL((k_pass2_m<LL, KK>), dim3(m.total_blocks), dim3(kBlock), 0, s, a, m.blk, m.n_iso)
+  if (L == 1 && KW == 1) P2M(1, 1);
+  else if (L == 1) P2M(1, 2);
+  else if (L == 2 && KW == 2) P2M(2, 2);
+  else if (L == 2) P2M(2, 3);
+  else if (L == 3 && KW == 3) P2M(3, 3);
+  else if (L == 3) P2M(3, 4);
+  else if (KW == 4) P2M(4, 4);
+  else P2M(4, 5);
+#undef P2M
+  return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Pass 3.  Three launches so no counter is contended: per-block counts of
+// kept slots over contiguous slot ranges, one small exclusive scan, then the
+// write pass (records leave in slot order).
+
+constexpr uint32_t kP3Blocks = 2048;
+
+template <int KW>
+__device__ __forceinline__ bool slot_record(const uint64_t* __restrict__ slots, uint64_t i, uint32_t stride,
+                                            uint32_t gate, uint64_t (&wk)[KW], uint32_t& mask, uint32_t& cov,
+                                            uint64_t& raw_cov) {
+  const uint64_t* s = slots + i * stride;
+#pragma unroll
+  for (int j = 0; j < KW; ++j) wk[j] = s[j];
+  mask = 0;
+  cov = 0;
+  raw_cov = 0;
+  if (wk[0] == kEmpty) return false;
+  const uint32_t* c = reinterpret_cast<const uint32_t*>(s + KW);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    uint32_t v = c[2 + e];
+    v = v < 65535u ? v : 65535u;
+    if (v >= gate) mask |= 1u << e;
+  }
+  raw_cov = c[0];
+  const uint32_t cv = c[0] < 65535u ? c[0] : 65535u;
+  cov = cv / 2u;
+  if (cov < 1u) cov = 1u;
+  return mask != 0u;
+}
+
+template <int KW>
+__device__ __forceinline__ void pass3_count_body(const uint64_t* __restrict__ slots, uint64_t cap, uint32_t stride,
+                                                 uint32_t gate, uint32_t* __restrict__ bcount,
+                                                 unsigned long long* __restrict__ brec) {
+  __shared__ uint32_t s_cnt[kBlock / 64];
+  __shared__ unsigned long long s_rec[kBlock / 64];
+  const uint64_t per = (cap + kP3Blocks - 1) / kP3Blocks;
+  const uint64_t lo = (uint64_t)blockIdx.x * per, hi = lo + per < cap ? lo + per : cap;
+  uint32_t n = 0;
+  unsigned long long rec = 0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    uint64_t wk[KW];
+    uint32_t mask, cov;
+    uint64_t raw;
+    n += slot_record<KW>(slots, i, stride, gate, wk, mask, cov, raw) ? 1u : 0u;
+    rec += raw;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    n += __shfl_xor(n, off);
+    rec += __shfl_xor(rec, off);
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    s_cnt[threadIdx.x >> 6] = n;
+    s_rec[threadIdx.x >> 6] = rec;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    unsigned long long r = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      t += s_cnt[w];
+      r += s_rec[w];
+    }
+    bcount[blockIdx.x] = t;
+    brec[blockIdx.x] = r;
+  }
+}
+
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_pass3_count(const uint64_t* __restrict__ slots, uint64_t cap,
+                                                        uint32_t stride, uint32_t gate, uint32_t* __restrict__ bcount,
+                                                        unsigned long long* __restrict__ brec) {
+  pass3_count_body<KW>(slots, cap, stride, gate, bcount, brec);
+}
+
+// One block: exclusive scan of kP3Blocks counts in place; totals to out[0]
+// (records kept) and rec_sum.
+__device__ __forceinline__ void pass3_scan_body(uint32_t* __restrict__ bcount,
+                                                const unsigned long long* __restrict__ brec,
+                                                uint32_t* __restrict__ out_count,
+                                                unsigned long long* __restrict__ rec_sum) {
+  __shared__ uint32_t s[1024];
+  __shared__ unsigned long long sr[1024];
+  constexpr uint32_t per = kP3Blocks / 1024;
+  uint32_t local[per];
+  uint32_t tot = 0;
+  unsigned long long rt = 0;
+  for (uint32_t j = 0; j < per; ++j) {
+    local[j] = bcount[threadIdx.x * per + j];
+    tot += local[j];
+    rt += brec[threadIdx.x * per + j];
+  }
+  s[threadIdx.x] = tot;
+  sr[threadIdx.x] = rt;
+  __syncthreads();
+  // Hillis-Steele inclusive scan over 1024 partial sums
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    uint32_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+    unsigned long long w = threadIdx.x >= off ? sr[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    sr[threadIdx.x] += w;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - tot;
+  for (uint32_t j = 0; j < per; ++j) {
+    bcount[threadIdx.x * per + j] = run;
+    run += local[j];
+  }
+  if (threadIdx.x == 1023) {
+    out_count[0] = s[1023];
+    rec_sum[0] = sr[1023];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_pass3_scan(uint32_t* __restrict__ bcount,
+                                                     const unsigned long long* __restrict__ brec,
+                                                     uint32_t* __restrict__ out_count,
+                                                     unsigned long long* __restrict__ rec_sum) {
+  pass3_scan_body(bcount, brec, out_count, rec_sum);
+}
+
+template <int KW>
+__device__ __forceinline__ void pass3_write_body(const uint64_t* __restrict__ slots, uint64_t cap, uint32_t stride,
+                                                 uint32_t nref, uint32_t gate, const uint32_t* __restrict__ boffset,
+                                                 uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
+                                                 uint8_t* __restrict__ out_mask) {
+  __shared__ uint32_t s_w[kBlock / 64];
+  const uint64_t per = (cap + kP3Blocks - 1) / kP3Blocks;
+  const uint64_t lo = (uint64_t)blockIdx.x * per, hi = lo + per < cap ? lo + per : cap;
+  uint32_t base = boffset[blockIdx.x];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint64_t c0 = lo; c0 < hi; c0 += blockDim.x) {
+    const uint64_t i = c0 + threadIdx.x;
+    uint64_t wk[KW];
+    uint32_t mask = 0, cov = 0;
+    uint64_t raw;
+    const bool keep = i < hi && slot_record<KW>(slots, i, stride, gate, wk, mask, cov, raw);
+    const uint64_t ballot = __ballot(keep);
+    if (lane == 0) s_w[wave] = (uint32_t)__builtin_popcountll(ballot);
+    __syncthreads();
+    uint32_t before = 0, chunk = 0;
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+      if (w < wave) before += s_w[w];
+      chunk += s_w[w];
+    }
+    if (keep) {
+      const uint32_t pos = base + before + (uint32_t)__builtin_popcountll(ballot & ((1ull << lane) - 1ull));
+      uint64_t limb[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < KW; ++j) {
+        const int bit = 62 * j, li = bit / 64, off = bit % 64;
+        if (li < 4) limb[li] |= wk[j] << off;
+        if (off > 2 && li + 1 < 4) limb[li + 1] |= wk[j] >> (64 - off);
+      }
+      for (uint32_t w = 0; w < nref; ++w) out_keys[(uint64_t)pos * nref + w] = limb[nref - 1 - w];
+      out_cov[pos] = (uint16_t)cov;
+      out_mask[pos] = (uint8_t)mask;
+    }
+    base += chunk;
+    __syncthreads();
+  }
+}
+
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_pass3_write(const uint64_t* __restrict__ slots, uint64_t cap,
+                                                        uint32_t stride, uint32_t nref, uint32_t gate,
+                                                        const uint32_t* __restrict__ boffset,
+                                                        uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
+                                                        uint8_t* __restrict__ out_mask) {
+  pass3_write_body<KW>(slots, cap, stride, nref, gate, boffset, out_keys, out_cov, out_mask);
+}
+
+// Pass 3 over many tables: blockIdx.y selects the table.
+__device__ __forceinline__ void p3_scratch(void* scratch, uint32_t*& out_count, unsigned long long*& rec_sum,
+                                           unsigned long long*& brec, uint32_t*& bcount) {
+  out_count = static_cast<uint32_t*>(scratch);
+  rec_sum = reinterpret_cast<unsigned long long*>(out_count + 2);
+  brec = reinterpret_cast<unsigned long long*>(out_count + 4);
+  bcount = reinterpret_cast<uint32_t*>(brec + kP3Blocks);
+}
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_pass3_count_m(const P3Args* __restrict__ args) {
+  const P3Args& a = args[blockIdx.y];
+  uint32_t *oc, *bc;
+  unsigned long long *rs, *br;
+  p3_scratch(a.scratch, oc, rs, br, bc);
+  pass3_count_body<KW>(a.t.slots, a.t.cap, a.t.stride, a.gate, bc, br);
+}
+__global__ __launch_bounds__(1024) void k_pass3_scan_m(const P3Args* __restrict__ args) {
+  const P3Args& a = args[blockIdx.y];
+  uint32_t *oc, *bc;
+  unsigned long long *rs, *br;
+  p3_scratch(a.scratch, oc, rs, br, bc);
+  pass3_scan_body(bc, br, oc, rs);
+}
+template <int KW>
+__global__ __launch_bounds__(kBlock) void k_pass3_write_m(const P3Args* __restrict__ args) {
+  const P3Args& a = args[blockIdx.y];
+  uint32_t *oc, *bc;
+  unsigned long long *rs, *br;
+  p3_scratch(a.scratch, oc, rs, br, bc);
+  pass3_write_body<KW>(a.t.slots, a.t.cap, a.t.stride, a.nref, a.gate, bc, a.out_keys, a.out_cov, a.out_mask);
+}
+
+hipError_t launch_pass3_multi(const P3Args* args, uint32_t n, uint32_t kw, hipStream_t s) {
+  if (!n) return hipSuccess;
+#define P3M(KK)                                                                                   \
+  hipLaunchKernelGGL((k_pass3_count_m<KK>), dim3(kP3Blocks, n), dim3(kBlock), 0, s, args);        \
+  hipLaunchKernelGGL(k_pass3_scan_m, dim3(1, n), dim3(1024), 0, s, args);                         \
+  hipLaunchKernelGGL((k_pass3_write_m<KK>), dim3(kP3Blocks, n), dim3(kBlock), 0, s, args)
+  switch (kw) {
+    case 1: P3M(1); break;
+    case 2: P3M(2); break;
+    case 3: P3M(3); break;
+    case 4: P3M(4); break;
+    default: P3M(5); break;
+  }
+#undef P3M
+  return hipGetLastError();
+}
+
+// Fills many u32 ranges in one launch: blockIdx.y selects the range.
+__global__ __launch_bounds__(kBlock) void k_fill_ranges(const FillRange* __restrict__ r) {
+  const FillRange f = r[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n; i += (uint64_t)gridDim.x * blockDim.x)
+    f.p[i] = f.v;
+}
+hipError_t launch_fill_ranges(const FillRange* r, uint32_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_ranges, dim3(2048, n), dim3(kBlock), 0, s, r);
+  return hipGetLastError();
+}
+
+// Clears many solid tables in one launch: blockIdx.y selects the table.
+__global__ __launch_bounds__(kBlock) void k_table_clear_m(const DevSolidTable* __restrict__ tabs) {
+  const DevSolidTable& t = tabs[blockIdx.y];
+  const uint64_t n = t.cap * t.stride;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    t.slots[i] = (uint32_t)(i % t.stride) < t.kw ? kEmpty : 0ull;
+  if (blockIdx.x == 0 && threadIdx.x < 2) t.status[threadIdx.x] = 0u;
+}
+hipError_t launch_table_clear_multi(const DevSolidTable* tabs, uint32_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_table_clear_m, dim3(4096, n), dim3(kBlock), 0, s, tabs);
+  return hipGetLastError();
+}
+
+size_t pass3_scratch_bytes() { return 16 + (size_t)kP3Blocks * 12 + 64; }
+
 hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t nref, uint32_t gate, uint64_t* out_keys,
-                        uint16_t* out_cov, uint8_t* out_mask, uint32_t* out_count, unsigned long long* rec_sum,
-                        hipStream_t s) {
-  unsigned grid = grid_for(t.cap);
-#define P3(KK)                                                                                        \
-  hipLaunchKernelGGL((k_pass3<KK>), dim3(grid), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, k, nref, gate, \
-                     out_keys, out_cov, out_mask, out_count, rec_sum)
+                        uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s) {
+  (void)k;
+  uint32_t* out_count = static_cast<uint32_t*>(scratch);
+  unsigned long long* rec_sum = reinterpret_cast<unsigned long long*>(out_count + 2);
+  unsigned long long* brec = reinterpret_cast<unsigned long long*>(out_count + 4);
+  uint32_t* bcount = reinterpret_cast<uint32_t*>(brec + kP3Blocks);
+#define P3(KK)                                                                                                  \
+  hipLaunchKernelGGL((k_pass3_count<KK>), dim3(kP3Blocks), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, gate, \
+                     bcount, brec);                                                                             \
+  hipLaunchKernelGGL(k_pass3_scan, dim3(1), dim3(1024), 0, s, bcount, brec, out_count, rec_sum);               \
+  hipLaunchKernelGGL((k_pass3_write<KK>), dim3(kP3Blocks), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, nref,  \
+                     gate, bcount, out_keys, out_cov, out_mask)
   switch (t.kw) {
     case 1: P3(1); break;
     case 2: P3(2); break;

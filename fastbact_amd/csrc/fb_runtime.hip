@@ -9,6 +9,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -553,6 +555,9 @@ extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
 }
 
 static uint32_t ref_words(uint32_t k) { return k <= 31 ? 1u : (k <= 63 ? 2u : 4u); }
+// Solid-table slot layout: kw key words of 62 bits, then the counters.
+static uint32_t key_words_for_k(uint32_t k) { return (k + 30) / 31; }
+static uint32_t table_stride(uint32_t kw) { return kw + 5; }
 
 extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
                               uint64_t n_contigs, const fb_build_params* prm, fb_solid** out) {
@@ -673,10 +678,27 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
 
   // pass 2 into the solid table; grows and re-runs on overflow
   DevSolidTable tab;
-  tab.kw = (k + 30) / 31;
-  tab.stride = tab.kw + 5;
+  tab.kw = key_words_for_k(k);
+  tab.stride = table_stride(tab.kw);
   tab.max_probe = 4096;
-  uint64_t cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(est_g * 2.5)));
+  // Capacity from the sketch itself: occ = row-0 cells whose count passes the
+  // gate; linear counting N = -W ln(1 - occ/W) estimates the distinct k-mers
+  // landing in such cells, an upper-side estimate of the keys pass 2 inserts.
+  uint64_t cap;
+  {
+    DBuf pc;
+    if (int r = pc.get(c, 8)) return fail(r);
+    HIP_TRY(c, launch_fill_u32(pc.as<uint32_t>(), 2, 0, s));
+    HIP_TRY(c, launch_popcount(bits.as<uint32_t>(), so->cms.width / 32 ? so->cms.width / 32 : 1,
+                               pc.as<unsigned long long>(), s));
+    unsigned long long occ = 0;
+    HIP_TRY(c, hipMemcpyAsync(&occ, pc.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    const double W = (double)so->cms.width;
+    double n_est = occ >= so->cms.width ? (double)windows : -W * log(1.0 - (double)occ / W);
+    n_est = std::min(n_est, (double)windows + 1.0);
+    cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * n_est) + 1024));
+  }
   DBuf tbuf, status;
   if (int r = status.get(c, 16)) return fail(r);
   tab.status = status.as<uint32_t>();
@@ -720,12 +742,10 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   if (int r = so->cov.get(c, maxn * 2)) return fail(r);
   if (int r = so->mask.get(c, maxn)) return fail(r);
   DBuf cnt;
-  if (int r = cnt.get(c, 16)) return fail(r);
-  HIP_TRY(c, launch_fill_u32(cnt.as<uint32_t>(), 4, 0, s));
+  if (int r = cnt.get(c, pass3_scratch_bytes())) return fail(r);
   HIP_TRY(c, hipEventRecord(c->ev[4], s));
   HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
-                          so->mask.as<uint8_t>(), cnt.as<uint32_t>(),
-                          reinterpret_cast<unsigned long long*>(cnt.as<uint32_t>() + 2), s));
+                          so->mask.as<uint8_t>(), cnt.p, s));
   uint32_t cbuf[4] = {0, 0, 0, 0};
   HIP_TRY(c, hipMemcpyAsync(cbuf, cnt.p, 16, hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
@@ -899,5 +919,440 @@ extern "C" int fb_estimate_genome(fb_ctx* c, const fb_arena_view* r1, const fb_a
   uint64_t u2 = f64_to_u64(round(fb_hll_estimate_registers(regs.data(), 14)));
   estimate_finish(u1, u2, m_prefix[stop], st, target_cov, out);
   out->pairs_sampled = stop;
+  return FB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: all isolates of a batch in each launch, passes captured as a
+// HIP graph per k and replayed.
+
+struct BatchIso {
+  const fb_bank* bank = nullptr;
+  uint64_t est_g = 0;
+  fb_cms cms;
+  fb_hll hll;
+  bool has_hll = false;
+  DBuf bits, bdata, bcur, slots, keys, cov, mask, p3;
+  uint64_t bins_cap = 0;
+  uint32_t sb = 14, nbins = 1;
+  bool binned = false;
+  uint64_t tab_cap = 0;
+  uint32_t tab_stride = 0;
+  uint64_t n_out = 0;
+  uint32_t split = 1;
+  uint64_t windows = 0;
+  bool overflow = false;
+};
+
+struct fb_batch {
+  fb_ctx* ctx = nullptr;
+  fb_build_params prm{};
+  std::vector<std::unique_ptr<BatchIso>> iso;
+  uint32_t k = 0;
+  uint32_t nref = 1;
+  DBuf d_p1, d_bin, d_p2, d_p3, d_fill, d_tabs, d_blk1, d_blkc, d_blk2, d_status;
+  MultiLaunch m1, mc, m2;
+  uint32_t max_nbins = 1, max_sb = 6;
+  uint32_t n_fill = 0;
+  bool any_binned = false, all_binned = false;
+  std::map<uint32_t, hipGraphExec_t> graphs;
+  std::map<uint32_t, hipGraph_t> graph_defs;
+  float ms[4] = {0, 0, 0, 0};
+  ~fb_batch() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : graph_defs) (void)hipGraphDestroy(g.second);
+  }
+};
+
+static uint64_t batch_cms_width(const fb_build_params* p, uint64_t est_g) {
+  return next_pow2(p->cms_width ? p->cms_width : fb_cms_width_for_genome_size(est_g));
+}
+
+// Table capacity from the genome size: the keys pass 2 inserts run to
+// about 2-3.5x the genome's distinct k-mers at the widths used here (error
+// k-mers lifted over the gate by Count-Min collisions); load stays <= 0.5.
+static uint64_t batch_table_cap(uint64_t est_g, uint32_t t_solid) {
+  const double keys = (t_solid <= 2 ? 3.5 : 2.5) * (double)est_g;
+  return next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * keys)));
+}
+
+extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uint64_t reads,
+                                       const fb_build_params* p, uint32_t k) {
+  if (!p) return 0;
+  const uint64_t est_g = std::max<uint64_t>(genome_size, 50000);
+  const uint64_t W = batch_cms_width(p, est_g);
+  const uint32_t t = p->target_solid_cov >= 0 ? (uint32_t)p->target_solid_cov : 2u;
+  const uint64_t cap = batch_table_cap(est_g, t);
+  const uint64_t stride = table_stride(key_words_for_k(std::max<uint32_t>(k, 1)));
+  const uint64_t windows = bases;  // upper bound of windows at any k
+  uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;             // counts + gate bits
+  b += p->hll_p ? (4ull << p->hll_p) : 0;                      // HLL registers
+  b += cap * stride * 8;                                       // solid table
+  b += cap * (ref_words(std::max<uint32_t>(k, 1)) * 8 + 3);   // outputs
+  b += (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536;    // bin offsets
+  b += bases / 4 + reads * 12 + 4096;                          // the bank itself
+  return b;
+}
+
+extern "C" int fb_batch_create(fb_ctx* c, const fb_bank* const* banks, const uint64_t* genome_sizes, uint32_t n,
+                               const fb_build_params* prm, fb_batch** out) {
+  if (!c || !banks || !genome_sizes || !prm || !out || n == 0) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (prm->counter_bits != 4 && prm->counter_bits != 16)
+    return set_err(c, FB_ERR_INVALID, "counter_bits must be 4 or 16 (got %d)", prm->counter_bits);
+  if (prm->hll_p != 0 && (prm->hll_p < 4 || prm->hll_p > 18))
+    return set_err(c, FB_ERR_INVALID, "hll_p %d outside 4..18", prm->hll_p);
+  if (int r = ctx_current(c)) return r;
+  auto b = std::make_unique<fb_batch>();
+  b->ctx = c;
+  b->prm = *prm;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!banks[i]) return set_err(c, FB_ERR_INVALID, "bank %u is null", i);
+    auto it = std::make_unique<BatchIso>();
+    it->bank = banks[i];
+    it->est_g = std::max<uint64_t>(genome_sizes[i], 50000);
+    if (int r = cms_init(c, &it->cms, batch_cms_width(prm, it->est_g), prm->counter_bits)) return r;
+    if (prm->hll_p) {
+      if (int r = hll_init(c, &it->hll, prm->hll_p)) return r;
+      it->has_hll = true;
+    }
+    if (int r = it->bits.get(c, ((it->cms.width * 4 + 31) / 32) * 4)) return r;
+    if (int r = it->p3.get(c, pass3_scratch_bytes())) return r;
+    b->iso.push_back(std::move(it));
+  }
+  const size_t nn = n;
+  if (int r = b->d_p1.get(c, nn * sizeof(Pass1Args))) return r;
+  if (int r = b->d_bin.get(c, nn * sizeof(BinArgs))) return r;
+  if (int r = b->d_p2.get(c, nn * sizeof(Pass2Args))) return r;
+  if (int r = b->d_p3.get(c, nn * sizeof(P3Args))) return r;
+  if (int r = b->d_tabs.get(c, nn * sizeof(DevSolidTable))) return r;
+  if (int r = b->d_fill.get(c, nn * 4 * sizeof(FillRange))) return r;
+  if (int r = b->d_blk1.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blkc.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blk2.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_status.get(c, nn * 16)) return r;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *out = b.release();
+  return FB_OK;
+}
+
+extern "C" void fb_batch_destroy(fb_batch* b) { delete b; }
+extern "C" uint32_t fb_batch_size(const fb_batch* b) { return b ? (uint32_t)b->iso.size() : 0; }
+extern "C" int fb_batch_key_words(const fb_batch* b) { return b ? (int)b->nref : 0; }
+extern "C" uint64_t fb_batch_count(const fb_batch* b, uint32_t i) {
+  return (b && i < b->iso.size()) ? b->iso[i]->n_out : 0;
+}
+
+// Lays out per-isolate buffers and argument arrays for a pass at k.
+static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gate, bool* layout_changed) {
+  fb_ctx* c = b->ctx;
+  const uint32_t n = (uint32_t)b->iso.size();
+  const uint32_t kw = key_words_for_k(k), stride = table_stride(kw);
+  std::vector<Pass1Args> p1(n);
+  std::vector<BinArgs> ba(n);
+  std::vector<Pass2Args> p2(n);
+  std::vector<P3Args> p3(n);
+  std::vector<DevSolidTable> tabs(n);
+  std::vector<FillRange> fills;
+  std::vector<uint32_t> blk1(n + 1, 0), blkc(n + 1, 0), blk2(n + 1, 0);
+  b->max_nbins = 1;
+  b->max_sb = 6;
+  b->any_binned = false;
+  b->all_binned = true;
+  *layout_changed = false;
+  for (uint32_t i = 0; i < n; ++i) {
+    BatchIso& it = *b->iso[i];
+    const uint64_t ncnt = it.cms.width * 4;
+    it.windows = bank_windows(it.bank, k);
+    it.split = b->prm.split ? b->prm.split : default_split(k, it.bank->max_len);
+    uint32_t sb = 14;
+    while (sb > 6 && (1ull << sb) > ncnt) --sb;
+    const uint64_t nbins = std::max<uint64_t>(1, ncnt >> sb);
+    const bool binned = b->prm.placement != 1 && nbins <= 8192;
+    if (binned != it.binned || sb != it.sb) *layout_changed = true;
+    it.binned = binned;
+    it.sb = sb;
+    it.nbins = (uint32_t)nbins;
+    if (binned) {
+      const uint64_t cap = ((uint64_t)(4.0 * (double)it.windows / (double)nbins * 1.1) + 256 + 3) & ~3ull;
+      if (cap > it.bins_cap || !it.bdata.p) {
+        if (int r = it.bdata.get(c, nbins * cap * 2)) return r;
+        it.bins_cap = cap;
+      }
+      if (!it.bcur.p)
+        if (int r = it.bcur.get(c, 8192 * 4)) return r;
+      b->max_nbins = std::max<uint32_t>(b->max_nbins, it.nbins);
+      b->max_sb = std::max<uint32_t>(b->max_sb, sb);
+      b->any_binned = true;
+    } else {
+      b->all_binned = false;
+    }
+    const uint64_t tcap = std::max(it.tab_cap, batch_table_cap(it.est_g, t_solid));
+    if (tcap != it.tab_cap || stride != it.tab_stride || !it.slots.p) {
+      if (int r = it.slots.get(c, tcap * stride * 8)) return r;
+      if (int r = it.keys.get(c, tcap * ref_words(k) * 8)) return r;
+      if (int r = it.cov.get(c, tcap * 2)) return r;
+      if (int r = it.mask.get(c, tcap)) return r;
+      it.tab_cap = tcap;
+      it.tab_stride = stride;
+      *layout_changed = true;
+    }
+    DevReads rv = it.bank->view();
+    const uint64_t ncells = ncnt;
+    // fills: counts, HLL, bin cursors
+    fills.push_back({it.cms.counts.as<uint32_t>(), ncells, 0u});
+    if (it.has_hll) fills.push_back({it.hll.regs.as<uint32_t>(), (uint64_t)1 << it.hll.p, 0u});
+    if (binned) fills.push_back({it.bcur.as<uint32_t>(), nbins, 0u});
+    // pass 1
+    Pass1Args& a1 = p1[i];
+    a1.reads = rv;
+    a1.k = k;
+    a1.split = it.split;
+    a1.weight = 1;
+    a1.cms = it.cms.dev();
+    if (it.has_hll) a1.hll = it.hll.dev();
+    BinArgs& bi = ba[i];
+    bi.reads = rv;
+    bi.k = k;
+    bi.split = it.split;
+    bi.log2w = it.cms.log2w;
+    bi.n_cells = ncells;
+    bi.counts = it.cms.counts.as<uint32_t>();
+    if (it.has_hll) bi.hll = it.hll.dev();
+    bi.sb = sb;
+    bi.nbins = it.nbins;
+    bi.cap = it.bins_cap;
+    bi.data = it.bdata.as<uint16_t>();
+    bi.cursor = it.bcur.as<uint32_t>();
+    bi.bits = it.bits.as<uint32_t>();
+    bi.t_solid = t_solid;
+    bi.cap_value = it.cms.cap();
+    // table and passes 2-3
+    DevSolidTable& t = tabs[i];
+    t.slots = it.slots.as<uint64_t>();
+    t.cap = it.tab_cap;
+    t.kw = kw;
+    t.stride = stride;
+    t.max_probe = 4096;
+    t.status = b->d_status.as<uint32_t>() + 4 * i;
+    Pass2Args& a2 = p2[i];
+    a2.reads = rv;
+    a2.k = k;
+    a2.split = it.split;
+    a2.solid_bits = it.bits.as<uint32_t>();
+    a2.log2w = it.cms.log2w;
+    a2.table = t;
+    a2.min_trans_cov = gate;
+    a2.contig_mode = 0;
+    P3Args& q = p3[i];
+    q.t = t;
+    q.nref = ref_words(k);
+    q.gate = gate;
+    q.out_keys = it.keys.as<uint64_t>();
+    q.out_cov = it.cov.as<uint16_t>();
+    q.out_mask = it.mask.as<uint8_t>();
+    q.scratch = it.p3.p;
+    const uint64_t thr = it.bank->n_reads * (uint64_t)it.split;
+    blk1[i + 1] = blk1[i] + (uint32_t)((thr + (binned ? kScatterBlockSize : kPassBlock) - 1) /
+                                       (binned ? kScatterBlockSize : kPassBlock));
+    blkc[i + 1] = blkc[i] + (binned ? it.nbins : 0u);
+    blk2[i + 1] = blk2[i] + (uint32_t)((thr + kPassBlock - 1) / kPassBlock);
+  }
+  if (b->any_binned && !b->all_binned)
+    return set_err(c, FB_ERR_INVALID, "a batch needs one pass-1 placement for all isolates (widths differ too much)");
+  if (fills.size() > 4 * (size_t)n) return set_err(c, FB_ERR_INVALID, "internal: fill ranges");
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipMemcpyAsync(b->d_p1.p, p1.data(), n * sizeof(Pass1Args), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_bin.p, ba.data(), n * sizeof(BinArgs), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_p2.p, p2.data(), n * sizeof(Pass2Args), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_p3.p, p3.data(), n * sizeof(P3Args), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_tabs.p, tabs.data(), n * sizeof(DevSolidTable), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_fill.p, fills.data(), fills.size() * sizeof(FillRange), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blk1.p, blk1.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blkc.p, blkc.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blk2.p, blk2.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipStreamSynchronize(s));  // host vectors go out of scope
+  b->m1 = MultiLaunch{b->d_p1.p, b->d_blk1.as<uint32_t>(), n, blk1[n]};
+  if (b->any_binned) b->m1.args = b->d_bin.p;
+  b->mc = MultiLaunch{b->d_bin.p, b->d_blkc.as<uint32_t>(), n, blkc[n]};
+  b->m2 = MultiLaunch{b->d_p2.p, b->d_blk2.as<uint32_t>(), n, blk2[n]};
+  b->n_fill = (uint32_t)fills.size();
+  b->nref = ref_words(k);
+  return FB_OK;
+}
+
+// The captured sequence: clears, pass 1, gate bits, table clear, pass 2, pass 3.
+static int batch_enqueue(fb_batch* b, uint32_t k, uint32_t t_solid) {
+  fb_ctx* c = b->ctx;
+  hipStream_t s = c->stream;
+  const uint32_t n = (uint32_t)b->iso.size();
+  const bool hll = b->prm.hll_p != 0;
+  HIP_TRY(c, launch_fill_ranges(b->d_fill.as<FillRange>(), b->n_fill, s));
+  if (b->any_binned) {
+    HIP_TRY(c, launch_bin_scatter_multi(b->m1, k, hll, b->max_nbins, s));
+    HIP_TRY(c, launch_bin_count_multi(b->mc, b->max_sb, s));
+  } else {
+    HIP_TRY(c, launch_pass1_multi(b->m1, k, true, hll, s));
+    for (uint32_t i = 0; i < n; ++i) {
+      BatchIso& it = *b->iso[i];
+      HIP_TRY(c, launch_solid_bits(it.cms.counts.as<uint32_t>(), it.cms.width * 4, t_solid, it.cms.cap(),
+                                   it.bits.as<uint32_t>(), s));
+    }
+  }
+  HIP_TRY(c, launch_table_clear_multi(b->d_tabs.as<DevSolidTable>(), n, s));
+  HIP_TRY(c, launch_pass2_multi(b->m2, k, s));
+  HIP_TRY(c, launch_pass3_multi(b->d_p3.as<P3Args>(), n, key_words_for_k(k), s));
+  return FB_OK;
+}
+
+extern "C" int fb_batch_run(fb_batch* b, uint32_t k, int32_t target) {
+  if (!b) return FB_ERR_INVALID;
+  fb_ctx* c = b->ctx;
+  if (k < 1 || k > 127) return set_err(c, FB_ERR_INVALID, "k=%u outside 1..127", k);
+  if (target > 255) return set_err(c, FB_ERR_INVALID, "target_solid_cov is a u8");
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+  const uint32_t t_solid = target >= 0 ? (uint32_t)target : 2u;
+  const uint32_t gate = target >= 0 ? (uint32_t)target : 3u;
+  const uint32_t n = (uint32_t)b->iso.size();
+  for (auto& it : b->iso)
+    if (int r = check_add_budget(c, fb_bank_bases_for_k(it->bank, k))) return r;
+  bool changed = false;
+  if (int r = batch_prepare(b, k, t_solid, gate, &changed)) return r;
+  // graph per k; re-captured when buffers moved
+  auto g = b->graphs.find(k);
+  if (changed && g != b->graphs.end()) {
+    (void)hipGraphExecDestroy(g->second);
+    (void)hipGraphDestroy(b->graph_defs[k]);
+    b->graphs.erase(g);
+    b->graph_defs.erase(k);
+    g = b->graphs.end();
+  }
+  if (g == b->graphs.end()) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int r = batch_enqueue(b, k, t_solid);
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    HIP_TRY(c, e);
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    b->graphs[k] = exec;
+    b->graph_defs[k] = graph;
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[0], s));
+  HIP_TRY(c, hipGraphLaunch(b->graphs[k], s));
+  HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  std::vector<uint32_t> st(4 * n);
+  HIP_TRY(c, hipMemcpyAsync(st.data(), b->d_status.p, 16ull * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  // isolates whose table overflowed: double the table and re-run passes 2-3 alone
+  HIP_TRY(c, hipEventRecord(c->ev[2], s));
+  const uint32_t kw = key_words_for_k(k), stride = table_stride(kw);
+  for (uint32_t i = 0; i < n; ++i) {
+    BatchIso& it = *b->iso[i];
+    it.overflow = st[4 * i] != 0;
+    for (int attempt = 0; it.overflow; ++attempt) {
+      if (attempt >= 6) return set_err(c, FB_ERR_OVERFLOW, "isolate %u: solid table overflow", i);
+      it.tab_cap *= 2;
+      if (int r = it.slots.get(c, it.tab_cap * stride * 8)) return r;
+      if (int r = it.keys.get(c, it.tab_cap * ref_words(k) * 8)) return r;
+      if (int r = it.cov.get(c, it.tab_cap * 2)) return r;
+      if (int r = it.mask.get(c, it.tab_cap)) return r;
+      DevSolidTable t;
+      t.slots = it.slots.as<uint64_t>();
+      t.cap = it.tab_cap;
+      t.kw = kw;
+      t.stride = stride;
+      t.max_probe = 4096;
+      t.status = b->d_status.as<uint32_t>() + 4 * i;
+      HIP_TRY(c, launch_table_clear(t, s));
+      Pass2Args a2{};
+      a2.reads = it.bank->view();
+      a2.k = k;
+      a2.split = it.split;
+      a2.solid_bits = it.bits.as<uint32_t>();
+      a2.log2w = it.cms.log2w;
+      a2.table = t;
+      a2.min_trans_cov = gate;
+      HIP_TRY(c, launch_pass2(a2, s));
+      HIP_TRY(c, launch_pass3(t, k, ref_words(k), gate, it.keys.as<uint64_t>(), it.cov.as<uint16_t>(),
+                              it.mask.as<uint8_t>(), it.p3.p, s));
+      uint32_t one[4];
+      HIP_TRY(c, hipMemcpyAsync(one, t.status, 16, hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipStreamSynchronize(s));
+      it.overflow = one[0] != 0;
+      it.tab_stride = stride;
+    }
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[3], s));
+  // per-isolate record counts, then sorting
+  std::vector<uint32_t> counts(n);
+  for (uint32_t i = 0; i < n; ++i)
+    HIP_TRY(c, hipMemcpyAsync(&counts[i], b->iso[i]->p3.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  HIP_TRY(c, hipEventRecord(c->ev[4], s));
+  if (b->prm.sort) {
+    size_t need = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      size_t tb = 0;
+      HIP_TRY(c, sort_solid(nullptr, nullptr, nullptr, counts[i], ref_words(k), k, nullptr, &tb, s));
+      need = std::max(need, tb);
+    }
+    DBuf tmp;
+    if (int r = tmp.get(c, need)) return r;
+    for (uint32_t i = 0; i < n; ++i) {
+      BatchIso& it = *b->iso[i];
+      if (counts[i] > 1) {
+        size_t tb = need;
+        HIP_TRY(c, sort_solid(it.keys.as<uint64_t>(), it.cov.as<uint16_t>(), it.mask.as<uint8_t>(), counts[i],
+                              ref_words(k), k, tmp.p, &tb, s));
+      }
+    }
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[5], s));
+  HIP_TRY(c, hipEventSynchronize(c->ev[5]));
+  for (uint32_t i = 0; i < n; ++i) b->iso[i]->n_out = counts[i];
+  b->k = k;
+  (void)hipEventElapsedTime(&b->ms[0], c->ev[0], c->ev[1]);
+  (void)hipEventElapsedTime(&b->ms[1], c->ev[2], c->ev[3]);
+  (void)hipEventElapsedTime(&b->ms[2], c->ev[4], c->ev[5]);
+  (void)hipEventElapsedTime(&b->ms[3], c->ev[0], c->ev[5]);
+  return FB_OK;
+}
+
+extern "C" int fb_batch_download(const fb_batch* b, uint32_t i, uint64_t* keys, uint16_t* cov, uint8_t* mask) {
+  if (!b || i >= b->iso.size()) return FB_ERR_INVALID;
+  fb_ctx* c = b->ctx;
+  if (int r = ctx_current(c)) return r;
+  const BatchIso& it = *b->iso[i];
+  if (it.n_out) {
+    if (keys) HIP_TRY(c, hipMemcpyAsync(keys, it.keys.p, it.n_out * b->nref * 8, hipMemcpyDeviceToHost, c->stream));
+    if (cov) HIP_TRY(c, hipMemcpyAsync(cov, it.cov.p, it.n_out * 2, hipMemcpyDeviceToHost, c->stream));
+    if (mask) HIP_TRY(c, hipMemcpyAsync(mask, it.mask.p, it.n_out, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+extern "C" uint64_t fb_batch_cms_bytes(const fb_batch* b, uint32_t i) {
+  return (b && i < b->iso.size()) ? fb_cms_table_bytes(&b->iso[i]->cms) : 0;
+}
+extern "C" int fb_batch_cms_download(const fb_batch* b, uint32_t i, void* out, uint64_t bytes) {
+  if (!b || i >= b->iso.size()) return FB_ERR_INVALID;
+  if (int r = ctx_current(b->ctx)) return r;
+  const BatchIso& it = *b->iso[i];
+  return cms_download(b->ctx, it.cms.dev(), it.cms.width, it.cms.bits, out, bytes);
+}
+extern "C" int fb_batch_hll_registers(const fb_batch* b, uint32_t i, uint8_t* out) {
+  if (!b || i >= b->iso.size() || !b->iso[i]->has_hll) return FB_ERR_INVALID;
+  if (int r = ctx_current(b->ctx)) return r;
+  return hll_download(b->ctx, b->iso[i]->hll.dev(), out);
+}
+extern "C" int fb_batch_timings(const fb_batch* b, float out[4]) {
+  if (!b || !out) return FB_ERR_INVALID;
+  memcpy(out, b->ms, sizeof b->ms);
   return FB_OK;
 }

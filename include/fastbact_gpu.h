@@ -156,6 +156,35 @@ int fb_solid_timings(const fb_solid* s, float out[4]);
  * (sum of raw cov), [2] keys in the solid table, [3] table slots. */
 int fb_solid_stats(const fb_solid* s, uint64_t out[4]);
 
+/* ---- Batched form: many isolates per launch --------------------------------
+ * The reference builds one isolate per process run (main.rs:262).  A batch
+ * holds n isolates' banks and per-isolate sketches and tables; one
+ * fb_batch_run executes passes 1-3 for every isolate with one launch per
+ * kernel (blocks map to isolates through a prefix table, no tile crosses
+ * isolates).  The pass is captured as a HIP graph on its first run for a
+ * given k and replayed afterwards.  Per-isolate results equal
+ * fb_build_solid's on the same isolate.  genome_sizes[i] sizes isolate i's CMS
+ * when params->cms_width is 0 (build.rs:74-84); params->k is ignored. */
+typedef struct fb_batch fb_batch;
+int fb_batch_create(fb_ctx* ctx, const fb_bank* const* banks, const uint64_t* genome_sizes, uint32_t n,
+                    const fb_build_params* params, fb_batch** out);
+void fb_batch_destroy(fb_batch* b);
+/* Device bytes one isolate of this shape needs at k (sketches, table,
+ * outputs, scratch) -- the admission footprint of the serving loop. */
+uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uint64_t reads, const fb_build_params* params,
+                            uint32_t k);
+int fb_batch_run(fb_batch* b, uint32_t k, int32_t target_solid_cov);
+uint32_t fb_batch_size(const fb_batch* b);
+uint64_t fb_batch_count(const fb_batch* b, uint32_t iso);
+int fb_batch_key_words(const fb_batch* b);
+int fb_batch_download(const fb_batch* b, uint32_t iso, uint64_t* keys, uint16_t* cov, uint8_t* mask);
+uint64_t fb_batch_cms_bytes(const fb_batch* b, uint32_t iso);
+int fb_batch_cms_download(const fb_batch* b, uint32_t iso, void* out, uint64_t bytes);
+int fb_batch_hll_registers(const fb_batch* b, uint32_t iso, uint8_t* out);
+/* Last run, milliseconds: [0] graph (passes 1-3, all isolates), [1] growth
+ * re-runs, [2] sort, [3] whole run. */
+int fb_batch_timings(const fb_batch* b, float out[4]);
+
 /* ---- Genome estimator (estimator.rs) -------------------------------------- */
 typedef struct {
   uint64_t exact_read_pairs;

@@ -1,0 +1,67 @@
+"""GPU parity of the batched form: every isolate's result from one batched
+launch equals the oracle's on that isolate alone."""
+import numpy as np
+import pytest
+
+import fastbact_amd as F
+from tests.helpers import isolate, oracle_arenas
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def isolates():
+    out = []
+    for i, (g, n, gc) in enumerate([(20_000, 1500, 0.35), (35_000, 2500, 0.5), (12_000, 900, 0.66),
+                                    (50_000, 3000, 0.45), (8_000, 0, 0.5), (25_000, 2000, 0.6)]):
+        out.append(isolate(seed=100 + i, genome_len=g, n_pairs=n, gc=gc) if n else None)
+    return out
+
+
+def _bank(iso):
+    return iso.bank() if iso is not None else F.ReadBank()
+
+
+@pytest.mark.parametrize("placement", [1, 2])
+@pytest.mark.parametrize("k,target", [(21, None), (55, 4), (77, 3)])
+def test_batch_equals_oracle_per_isolate(ctx, orc, isolates, k, target, placement):
+    banks = [_bank(x) for x in isolates]
+    dev = [F.DeviceBank(ctx, b) for b in banks]
+    gs = [50_000] * len(banks)
+    width = 1 << 16
+    batch = F.Batch(ctx, dev, gs, cms_width=width, counter_bits=16, hll_p=12, placement=placement)
+    for rep in range(2):  # second run replays the captured graph
+        batch.run(k, target)
+        for i, b in enumerate(banks):
+            got = batch.result(i, k)
+            ref = orc.build_solid(oracle_arenas(b), k, target_solid_cov=target, width=width, counter_bits=16,
+                                  hll_p=12)
+            assert np.array_equal(got.keys, ref["keys"]), (rep, i)
+            assert np.array_equal(got.cov, ref["cov"]) and np.array_equal(got.mask, ref["mask"])
+            assert np.array_equal(got.cms_table, ref["cms"].table())
+            assert np.array_equal(got.hll_registers, ref["hll"])
+
+
+def test_batch_multi_k_reuses_buffers_and_matches_single(ctx, isolates):
+    banks = [_bank(x) for x in isolates[:4]]
+    dev = [F.DeviceBank(ctx, b) for b in banks]
+    batch = F.Batch(ctx, dev, [60_000] * 4, cms_width=1 << 17, counter_bits=4)
+    for k, t in ((21, None), (55, 4), (77, 4), (21, None)):
+        batch.run(k, t)
+        for i in range(4):
+            single = F.build_solid_kmers(ctx, dev[i], k, target_solid_cov=t, cms_width=1 << 17, counter_bits=4)
+            got = batch.result(i, k)
+            assert np.array_equal(got.keys, single.keys) and np.array_equal(got.cov, single.cov)
+            assert np.array_equal(got.cms_table, single.cms_table)
+
+
+def test_batch_small_table_grows_exactly(ctx, orc, isolates):
+    # tiny genome sizes force the per-isolate tables to overflow and grow
+    banks = [_bank(x) for x in isolates[:3]]
+    dev = [F.DeviceBank(ctx, b) for b in banks]
+    batch = F.Batch(ctx, dev, [1, 1, 1], cms_width=1 << 15, counter_bits=16)
+    batch.run(31, 1)
+    for i, b in enumerate(banks):
+        ref = orc.build_solid(oracle_arenas(b), 31, target_solid_cov=1, width=1 << 15, counter_bits=16)
+        got = batch.result(i, 31)
+        assert np.array_equal(got.keys, ref["keys"]) and np.array_equal(got.cov, ref["cov"])

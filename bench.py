@@ -166,7 +166,8 @@ def main(argv=None) -> int:
         p2_bytes += b2 * args.steps
         per_pass.append({"k": k, "pass1_ms": t1 / args.steps, "pass2_ms": t2 / args.steps,
                          "pass3_sort_ms": t3 / args.steps, "windows": st["windows"], "records": st["records"],
-                         "solid_kmers": infos[-1][j][2],
+                         "solid_kmers": infos[-1][j][2], "pass2_route": st["pass2_route"],
+                         "lds_bins": st["lds_bins"], "est_keys": st["est_keys"],
                          "pass1_GBps": b1 / (t1 / args.steps) / 1e6, "pass2_GBps": b2 / (t2 / args.steps) / 1e6})
     if p1_ms >= p2_ms:
         kname, kms, kbytes = "k_pass1 (CMS update + HLL)", p1_ms, p1_bytes

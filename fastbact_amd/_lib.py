@@ -57,6 +57,7 @@ class BuildParams(C.Structure):
         ("sort", C.c_uint32),
         ("split", C.c_uint32),
         ("placement", C.c_uint32),
+        ("bins2", C.c_uint32),
     ]
 
 

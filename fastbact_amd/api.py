@@ -310,9 +310,11 @@ class SolidHandle(_Handle):
         return tuple(float(x) for x in t)
 
     def stats(self) -> dict:
-        t = (C.c_uint64 * 4)()
+        t = (C.c_uint64 * 8)()
         self.ctx.check(self.ctx.lib.fb_solid_stats(self.h, t), "fb_solid_stats")
-        return {"windows": int(t[0]), "records": int(t[1]), "keys": int(t[2]), "slots": int(t[3])}
+        return {"windows": int(t[0]), "records": int(t[1]), "keys": int(t[2]), "slots": int(t[3]),
+                "pass2_route": ("global", "lds", "lds_region_full", "lds_spill_full")[int(t[4])],
+                "lds_bins": int(t[5]), "lds_bin_keys": int(t[6]), "est_keys": int(t[7])}
 
     def download(self, k: int, with_cms: bool = True, counter_bits: int = 4, hll_p: int = 0) -> SolidSet:
         n, nw = self.count(), self.key_words()
@@ -345,7 +347,7 @@ def _pack_contigs(contigs: Optional[Sequence[bytes]]):
 
 def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Optional[int] = None,
                  cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
-                 split: int = 0, placement: int = 0) -> BuildParams:
+                 split: int = 0, placement: int = 0, bins2: int = 0) -> BuildParams:
     p = BuildParams()
     _lib.load().fb_build_params_default(C.byref(p), k)
     p.target_solid_cov = -1 if target_solid_cov is None else int(target_solid_cov)
@@ -356,6 +358,7 @@ def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Op
     p.sort = 1 if sort else 0
     p.split = split
     p.placement = placement
+    p.bins2 = bins2
     return p
 
 
@@ -373,12 +376,12 @@ def build_solid_handle(ctx: Context, bank: DeviceBank, k: int, contigs: Optional
 def build_solid_kmers(ctx: Context, bank: DeviceBank, k: int, target_solid_cov: Optional[int] = None,
                       genome_size: Optional[int] = None, contigs: Optional[Sequence[bytes]] = None,
                       cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0,
-                      with_cms: bool = True, split: int = 0, placement: int = 0) -> SolidSet:
+                      with_cms: bool = True, split: int = 0, placement: int = 0, bins2: int = 0) -> SolidSet:
     """build_cdbg_with_contigs_ext passes 1-3 (build.rs:67-328): the solid set,
     the pass's CMS table and (hll_p > 0) the pass's HLL registers."""
     h = build_solid_handle(ctx, bank, k, contigs, target_solid_cov=target_solid_cov, genome_size=genome_size,
                            cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, split=split,
-                           placement=placement)
+                           placement=placement, bins2=bins2)
     try:
         return h.download(k, with_cms=with_cms, counter_bits=counter_bits, hll_p=hll_p)
     finally:

@@ -121,12 +121,46 @@ hipError_t launch_pass2_multi(const MultiLaunch& m, uint32_t k, hipStream_t s);
 // scratch (pass3_scratch_bytes()): u32 [0] records kept, u64 at byte 8 the sum
 // of raw cov (records added to the table), then per-block partials.
 size_t pass3_scratch_bytes();
+// out_base (nullable): device u32 added to every output position.
 hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t n_ref_words, uint32_t min_trans_cov,
-                        uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s);
+                        uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s,
+                        const uint32_t* out_base = nullptr);
 // Sort the compacted records by k-mer (integer order).  Uses scratch from
 // the caller: tmp_bytes queried with keys == nullptr.
 hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t n_ref_words,
                       uint32_t k, void* tmp, size_t* tmp_bytes, hipStream_t s);
+
+// Pass 2 + 3 with LDS-privatised solid tables (fb_pass2_binned.hip).
+constexpr uint32_t kP2LdsBytes = 160u * 1024u;  // workgroup LDS of the bin kernel
+struct P2BinArgs {
+  DevReads reads;
+  uint32_t k = 0;
+  uint32_t split = 1;            // pairs per thread must stay <= 63
+  const uint32_t* solid_bits = nullptr;
+  uint32_t log2w = 0;
+  uint32_t nbins = 1;
+  uint32_t log2_nbins = 0;
+  uint64_t bin_cap = 0;          // records per bin region
+  uint32_t* cursor = nullptr;    // nbins, zeroed
+  uint64_t* rec_keys = nullptr;  // nbins * kw * bin_cap
+  uint8_t* rec_info = nullptr;   // nbins * bin_cap
+  uint32_t* status = nullptr;    // [0] region overflow, zeroed
+  uint32_t lds_cap = 0;          // LDS table entries
+  uint32_t min_trans_cov = 0;
+  DevSolidTable table;           // keys that find the LDS table full
+  uint32_t nref = 1;
+  uint64_t* out_keys = nullptr;
+  uint16_t* out_cov = nullptr;
+  uint8_t* out_mask = nullptr;
+  uint32_t* out_count = nullptr;  // zeroed; records emitted from LDS
+  unsigned long long* stats = nullptr;  // zeroed; [0] records (sum of raw cov), [1] keys held in LDS
+};
+uint32_t p2_lds_cap(uint32_t kw);
+uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
+size_t p2_lds_bytes(uint32_t kw, uint32_t cap);
+hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s);
+hipError_t launch_p2_binned_multi(const MultiLaunch& scatter, const MultiLaunch& bins, uint32_t k,
+                                  uint32_t max_nbins, uint32_t lds_cap, hipStream_t s);
 
 // Multi-table forms for the batched pass (blockIdx.y = isolate).
 struct P3Args {
@@ -137,6 +171,7 @@ struct P3Args {
   uint16_t* out_cov = nullptr;
   uint8_t* out_mask = nullptr;
   void* scratch = nullptr;  // pass3_scratch_bytes()
+  const uint32_t* out_base = nullptr;
 };
 hipError_t launch_pass3_multi(const P3Args* args, uint32_t n, uint32_t kw, hipStream_t s);
 struct FillRange {

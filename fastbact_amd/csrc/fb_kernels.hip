@@ -12,6 +12,7 @@
 
 #include "fb_device.hpp"
 #include "fb_internal.hpp"
+#include "fb_table.hpp"
 
 namespace fb {
 using namespace fbk;
@@ -275,41 +276,6 @@ __device__ __forceinline__ void pack62(const Kmer<L>& x, uint64_t (&w)[KW]) {
   }
 }
 
-// Finds or claims the slot of `key`.  Key words are write-once (empty ->
-// value), claimed word by word with compare-and-swap in order; a slot whose
-// prefix matches but whose next word holds another value belongs to another
-// key.  A plain load that returns a stale empty word only sends the thread to
-// the compare-and-swap, which returns the stored word.
-template <int KW>
-__device__ __forceinline__ uint32_t* table_find(const DevSolidTable& t, const uint64_t (&key)[KW], uint64_t h) {
-  const uint64_t cmask = t.cap - 1ull;
-  uint64_t slot = h & cmask;
-  for (uint32_t probe = 0; probe < t.max_probe; ++probe) {
-    uint64_t* s = t.slots + slot * t.stride;
-    bool match = true;
-#pragma unroll
-    for (int j = 0; j < KW; ++j) {
-      if (!match) break;
-      uint64_t v = s[j];
-      if (v == kEmpty) {
-        uint64_t old = atomicCAS((unsigned long long*)(s + j), (unsigned long long)kEmpty,
-                                 (unsigned long long)key[j]);
-        if (old == kEmpty) {
-          v = key[j];
-          if (j == 0) atomicAdd(t.status + 1, 1u);
-        } else {
-          v = old;
-        }
-      }
-      match = (v == key[j]);
-    }
-    if (match) return reinterpret_cast<uint32_t*>(s + KW);
-    slot = (slot + 1ull) & cmask;
-  }
-  atomicOr(t.status, 1u);
-  return nullptr;
-}
-
 __device__ __forceinline__ bool solid_lookup(const uint32_t* __restrict__ bits, uint32_t lw, uint64_t key) {
   const uint64_t wmask = (1ull << lw) - 1ull;
   bool ok = true;
@@ -375,7 +341,7 @@ __device__ __forceinline__ void pass2_body(const Pass2Args& a, uint64_t gid) {
     if (nrec) {
       uint64_t kwds[KW];
       pack62<L, KW>(cur, kwds);
-      uint32_t* c = table_find<KW>(a.table, kwds, slot_hash(cur_fold));
+      uint32_t* c = table_find_global<KW>(a.table, kwds, slot_hash(cur_fold));
       if (c) {
         // edge_v of pair (x-1, x) and edge_u of pair (x, x+1), build.rs:268-275
         const uint32_t e_left = cur_rc ? left_rcp : left_rcp + 4u;
@@ -573,11 +539,11 @@ template <int KW>
 __device__ __forceinline__ void pass3_write_body(const uint64_t* __restrict__ slots, uint64_t cap, uint32_t stride,
                                                  uint32_t nref, uint32_t gate, const uint32_t* __restrict__ boffset,
                                                  uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
-                                                 uint8_t* __restrict__ out_mask) {
+                                                 uint8_t* __restrict__ out_mask, const uint32_t* __restrict__ out_base) {
   __shared__ uint32_t s_w[kBlock / 64];
   const uint64_t per = (cap + kP3Blocks - 1) / kP3Blocks;
   const uint64_t lo = (uint64_t)blockIdx.x * per, hi = lo + per < cap ? lo + per : cap;
-  uint32_t base = boffset[blockIdx.x];
+  uint32_t base = boffset[blockIdx.x] + (out_base ? *out_base : 0u);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   for (uint64_t c0 = lo; c0 < hi; c0 += blockDim.x) {
     const uint64_t i = c0 + threadIdx.x;
@@ -616,8 +582,8 @@ __global__ __launch_bounds__(kBlock) void k_pass3_write(const uint64_t* __restri
                                                         uint32_t stride, uint32_t nref, uint32_t gate,
                                                         const uint32_t* __restrict__ boffset,
                                                         uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
-                                                        uint8_t* __restrict__ out_mask) {
-  pass3_write_body<KW>(slots, cap, stride, nref, gate, boffset, out_keys, out_cov, out_mask);
+                                                        uint8_t* __restrict__ out_mask, const uint32_t* __restrict__ out_base) {
+  pass3_write_body<KW>(slots, cap, stride, nref, gate, boffset, out_keys, out_cov, out_mask, out_base);
 }
 
 // Pass 3 over many tables: blockIdx.y selects the table.
@@ -649,7 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_pass3_write_m(const P3Args* __restri
   uint32_t *oc, *bc;
   unsigned long long *rs, *br;
   p3_scratch(a.scratch, oc, rs, br, bc);
-  pass3_write_body<KW>(a.t.slots, a.t.cap, a.t.stride, a.nref, a.gate, bc, a.out_keys, a.out_cov, a.out_mask);
+  pass3_write_body<KW>(a.t.slots, a.t.cap, a.t.stride, a.nref, a.gate, bc, a.out_keys, a.out_cov, a.out_mask,
+                       a.out_base);
 }
 
 hipError_t launch_pass3_multi(const P3Args* args, uint32_t n, uint32_t kw, hipStream_t s) {
@@ -698,7 +665,7 @@ hipError_t launch_table_clear_multi(const DevSolidTable* tabs, uint32_t n, hipSt
 size_t pass3_scratch_bytes() { return 16 + (size_t)kP3Blocks * 12 + 64; }
 
 hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t nref, uint32_t gate, uint64_t* out_keys,
-                        uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s) {
+                        uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s, const uint32_t* out_base) {
   (void)k;
   uint32_t* out_count = static_cast<uint32_t*>(scratch);
   unsigned long long* rec_sum = reinterpret_cast<unsigned long long*>(out_count + 2);
@@ -709,7 +676,7 @@ hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t nref, uint3
                      bcount, brec);                                                                             \
   hipLaunchKernelGGL(k_pass3_scan, dim3(1), dim3(1024), 0, s, bcount, brec, out_count, rec_sum);               \
   hipLaunchKernelGGL((k_pass3_write<KK>), dim3(kP3Blocks), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, nref,  \
-                     gate, bcount, out_keys, out_cov, out_mask)
+                     gate, bcount, out_keys, out_cov, out_mask, out_base)
   switch (t.kw) {
     case 1: P3(1); break;
     case 2: P3(2); break;

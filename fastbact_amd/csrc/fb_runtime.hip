@@ -542,7 +542,9 @@ struct fb_solid {
   fb_hll hll;
   bool has_hll = false;
   float ms[4] = {0, 0, 0, 0};
-  uint64_t stats[4] = {0, 0, 0, 0};  // windows, read records, claimed keys, table slots
+  // windows, read records, claimed keys, table slots, pass-2 route, LDS bins,
+  // keys per LDS bin table, estimated keys
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
@@ -685,6 +687,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   // gate; linear counting N = -W ln(1 - occ/W) estimates the distinct k-mers
   // landing in such cells, an upper-side estimate of the keys pass 2 inserts.
   uint64_t cap;
+  double n_est_keys = 0.0;
   {
     DBuf pc;
     if (int r = pc.get(c, 8)) return fail(r);
@@ -698,64 +701,157 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     double n_est = occ >= so->cms.width ? (double)windows : -W * log(1.0 - (double)occ / W);
     n_est = std::min(n_est, (double)windows + 1.0);
     cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * n_est) + 1024));
+    n_est_keys = n_est;
   }
   DBuf tbuf, status;
   if (int r = status.get(c, 16)) return fail(r);
   tab.status = status.as<uint32_t>();
-  uint32_t st[2] = {0, 0};
-  for (int attempt = 0;; ++attempt) {
-    tab.cap = cap;
-    if (int r = tbuf.get(c, cap * tab.stride * 8)) return fail(r);
+  uint32_t n_out = 0;
+  uint64_t rec_sum = 0, claimed = 0;
+  uint64_t route = 0;
+  bool done = false;
+  // LDS-privatised pass 2+3 (reads only; prior contigs use the global table)
+  const uint32_t kw = tab.kw;
+  const uint32_t lds_cap = p2_lds_cap(kw);
+  uint64_t nb2 = 1;
+  while ((double)nb2 * 0.5 * (double)lds_cap < n_est_keys) nb2 <<= 1;
+  if (prm->bins2) {
+    if (prm->bins2 > p2_max_bins() || (prm->bins2 & (prm->bins2 - 1)))
+      return fail(set_err(c, FB_ERR_INVALID, "bins2 must be a power of two <= %u (got %u)", p2_max_bins(), prm->bins2));
+    nb2 = prm->bins2;
+  }
+  so->stats[5] = nb2;
+  so->stats[6] = lds_cap;
+  so->stats[7] = (uint64_t)n_est_keys;
+  if (!c_reads && prm->placement != 1 && nb2 <= p2_max_bins() && windows > 0) {
+    const uint32_t pairs_max = bank->max_len > k ? bank->max_len - k : 1;
+    const uint32_t split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
+    const uint64_t bin_cap = (((uint64_t)((double)windows / (double)nb2 * 1.15)) + 1024 + 3) & ~3ull;
+    // keys beyond what the LDS tables are expected to hold, plus slack
+    const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
+    const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est_keys / 4)));
+    const uint64_t out_cap = nb2 * lds_cap + spill_cap;
+    DBuf rk, ri, cur, st2, p3s;
+    int r = 0;
+    if (!r) r = rk.get(c, nb2 * kw * bin_cap * 8);
+    if (!r) r = ri.get(c, nb2 * bin_cap);
+    if (!r) r = cur.get(c, nb2 * 4);
+    if (!r) r = st2.get(c, 64);
+    if (!r) r = p3s.get(c, pass3_scratch_bytes());
+    if (!r) r = tbuf.get(c, spill_cap * tab.stride * 8);
+    if (!r) r = so->keys.get(c, out_cap * so->nref * 8);
+    if (!r) r = so->cov.get(c, out_cap * 2);
+    if (!r) r = so->mask.get(c, out_cap);
+    if (r) return fail(r);
+    tab.cap = spill_cap;
     tab.slots = tbuf.as<uint64_t>();
     HIP_TRY(c, launch_table_clear(tab, s));
+    HIP_TRY(c, launch_fill_u32(cur.as<uint32_t>(), nb2, 0, s));
+    HIP_TRY(c, launch_fill_u32(st2.as<uint32_t>(), 16, 0, s));
+    P2BinArgs pa;
+    pa.reads = bank->view();
+    pa.k = k;
+    pa.split = split2;
+    pa.solid_bits = bits.as<uint32_t>();
+    pa.log2w = so->cms.log2w;
+    pa.nbins = (uint32_t)nb2;
+    pa.log2_nbins = (uint32_t)__builtin_ctzll(nb2);
+    pa.bin_cap = bin_cap;
+    pa.cursor = cur.as<uint32_t>();
+    pa.rec_keys = rk.as<uint64_t>();
+    pa.rec_info = ri.as<uint8_t>();
+    pa.status = st2.as<uint32_t>();            // [0] region overflow
+    pa.out_count = st2.as<uint32_t>() + 2;     // records emitted from LDS
+    pa.stats = reinterpret_cast<unsigned long long*>(st2.as<uint32_t>() + 4);
+    pa.lds_cap = lds_cap;
+    pa.min_trans_cov = gate;
+    pa.table = tab;
+    pa.nref = so->nref;
+    pa.out_keys = so->keys.as<uint64_t>();
+    pa.out_cov = so->cov.as<uint16_t>();
+    pa.out_mask = so->mask.as<uint8_t>();
     HIP_TRY(c, hipEventRecord(c->ev[2], s));
-    Pass2Args p2{};
-    p2.reads = bank->view();
-    p2.k = k;
-    p2.split = split;
-    p2.solid_bits = bits.as<uint32_t>();
-    p2.log2w = so->cms.log2w;
-    p2.table = tab;
-    p2.min_trans_cov = gate;
-    p2.contig_mode = 0;
-    HIP_TRY(c, launch_pass2(p2, s));
-    if (c_reads) {
-      Pass2Args pc = p2;
-      pc.reads = cview;
-      pc.split = 1;
-      pc.contig_mode = 1;
-      HIP_TRY(c, launch_pass2(pc, s));
-    }
+    HIP_TRY(c, launch_p2_binned(pa, s));
     HIP_TRY(c, hipEventRecord(c->ev[3], s));
-    HIP_TRY(c, hipMemcpyAsync(st, tab.status, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipEventRecord(c->ev[4], s));
+    HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
+                            so->mask.as<uint8_t>(), p3s.p, s, pa.out_count));
+    uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0};
+    HIP_TRY(c, hipMemcpyAsync(hb, st2.p, 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(sp, p3s.p, 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(ts, tab.status, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    if (!st[0]) break;
-    if (attempt >= 6) return fail(set_err(c, FB_ERR_OVERFLOW, "solid table overflow at %llu slots", (unsigned long long)cap));
-    cap *= 2;
+    if (!hb[0] && !ts[0]) {
+      uint64_t bins_rec = 0, bins_keys = 0, sp_rec = 0;
+      memcpy(&bins_rec, hb + 4, 8);
+      memcpy(&bins_keys, hb + 6, 8);
+      memcpy(&sp_rec, sp + 2, 8);
+      n_out = hb[2] + sp[0];
+      rec_sum = bins_rec + sp_rec;
+      claimed = bins_keys + ts[1];
+      cap = spill_cap;
+      route = 1;
+      done = true;
+    } else {
+      route = hb[0] ? 2 : 3;
+    }
+  }
+  if (!done) {
+    uint32_t st[2] = {0, 0};
+    for (int attempt = 0;; ++attempt) {
+      tab.cap = cap;
+      if (int r = tbuf.get(c, cap * tab.stride * 8)) return fail(r);
+      tab.slots = tbuf.as<uint64_t>();
+      HIP_TRY(c, launch_table_clear(tab, s));
+      HIP_TRY(c, hipEventRecord(c->ev[2], s));
+      Pass2Args p2{};
+      p2.reads = bank->view();
+      p2.k = k;
+      p2.split = split;
+      p2.solid_bits = bits.as<uint32_t>();
+      p2.log2w = so->cms.log2w;
+      p2.table = tab;
+      p2.min_trans_cov = gate;
+      p2.contig_mode = 0;
+      HIP_TRY(c, launch_pass2(p2, s));
+      if (c_reads) {
+        Pass2Args pc = p2;
+        pc.reads = cview;
+        pc.split = 1;
+        pc.contig_mode = 1;
+        HIP_TRY(c, launch_pass2(pc, s));
+      }
+      HIP_TRY(c, hipEventRecord(c->ev[3], s));
+      HIP_TRY(c, hipMemcpyAsync(st, tab.status, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipStreamSynchronize(s));
+      if (!st[0]) break;
+      if (attempt >= 6)
+        return fail(set_err(c, FB_ERR_OVERFLOW, "solid table overflow at %llu slots", (unsigned long long)cap));
+      cap *= 2;
+    }
+    // pass 3
+    claimed = st[1];
+    const uint64_t maxn = std::max<uint64_t>(claimed, 1);
+    if (int r = so->keys.get(c, maxn * so->nref * 8)) return fail(r);
+    if (int r = so->cov.get(c, maxn * 2)) return fail(r);
+    if (int r = so->mask.get(c, maxn)) return fail(r);
+    DBuf cnt;
+    if (int r = cnt.get(c, pass3_scratch_bytes())) return fail(r);
+    HIP_TRY(c, hipEventRecord(c->ev[4], s));
+    HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
+                            so->mask.as<uint8_t>(), cnt.p, s));
+    uint32_t cbuf[4] = {0, 0, 0, 0};
+    HIP_TRY(c, hipMemcpyAsync(cbuf, cnt.p, 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    n_out = cbuf[0];
+    memcpy(&rec_sum, cbuf + 2, 8);
   }
   bits.release();
-
-  // pass 3
-  const uint64_t claimed = st[1];
-  const uint64_t maxn = std::max<uint64_t>(claimed, 1);
-  if (int r = so->keys.get(c, maxn * so->nref * 8)) return fail(r);
-  if (int r = so->cov.get(c, maxn * 2)) return fail(r);
-  if (int r = so->mask.get(c, maxn)) return fail(r);
-  DBuf cnt;
-  if (int r = cnt.get(c, pass3_scratch_bytes())) return fail(r);
-  HIP_TRY(c, hipEventRecord(c->ev[4], s));
-  HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
-                          so->mask.as<uint8_t>(), cnt.p, s));
-  uint32_t cbuf[4] = {0, 0, 0, 0};
-  HIP_TRY(c, hipMemcpyAsync(cbuf, cnt.p, 16, hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
-  const uint32_t n_out = cbuf[0];
-  uint64_t rec_sum = 0;
-  memcpy(&rec_sum, cbuf + 2, 8);
-  so->stats[0] = bank_windows(bank, k);
+  so->stats[0] = windows;
   so->stats[1] = rec_sum;
   so->stats[2] = claimed;
   so->stats[3] = cap;
+  so->stats[4] = route;
   tbuf.release();
   so->n = n_out;
   if (prm->sort && n_out > 1) {
@@ -807,7 +903,7 @@ extern "C" int fb_solid_timings(const fb_solid* s, float out[4]) {
   memcpy(out, s->ms, sizeof s->ms);
   return FB_OK;
 }
-extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[4]) {
+extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[8]) {
   if (!s || !out) return FB_ERR_INVALID;
   memcpy(out, s->stats, sizeof s->stats);
   return FB_OK;

@@ -127,7 +127,8 @@ typedef struct {
   int32_t hll_p;              /* 0 = no per-pass HLL; else precision of the pass HLL */
   uint32_t sort;              /* 1 = records sorted by k-mer (parity form) */
   uint32_t split;             /* threads per read; 0 = tuned default */
-  uint32_t placement;         /* pass-1 sketch placement: 0 auto, 1 global atomics, 2 LDS bins */
+  uint32_t placement;         /* sketch and solid-table placement: 0 auto, 1 global tables, 2 LDS bins */
+  uint32_t bins2;             /* pass-2 LDS bins (power of two <= 16384); 0 = sized from the sketch */
 } fb_build_params;
 
 void fb_build_params_default(fb_build_params* p, uint32_t k);
@@ -153,8 +154,11 @@ int fb_solid_hll_registers(const fb_solid* s, uint8_t* out);
  * + sort, [3] whole build. */
 int fb_solid_timings(const fb_solid* s, float out[4]);
 /* Work counts of the build: [0] k-mer windows of pass 1, [1] pass-2 records
- * (sum of raw cov), [2] keys in the solid table, [3] table slots. */
-int fb_solid_stats(const fb_solid* s, uint64_t out[4]);
+ * (sum of raw cov), [2] keys in the solid table, [3] table slots, [4] pass-2
+ * route (0 global table, 1 LDS bins, 2 LDS bins abandoned for a full record
+ * region, 3 abandoned for a full spill table), [5] pass-2 LDS bins, [6] keys
+ * per LDS bin table, [7] keys estimated from the pass-1 gate bits. */
+int fb_solid_stats(const fb_solid* s, uint64_t out[8]);
 
 /* ---- Batched form: many isolates per launch --------------------------------
  * The reference builds one isolate per process run (main.rs:262).  A batch

@@ -229,3 +229,31 @@ def test_cfg1_scale_plain_isolate(ctx, orc):
     est = F.hll_estimate_registers(g.hll_registers, 14)
     assert 0.5e6 < len(g) < 1.1e6
     assert est > len(g)
+
+
+@pytest.mark.parametrize("k,bins2", [(31, 1), (55, 2), (77, 1), (127, 4)])
+def test_lds_table_full_spills_to_global_table(ctx, orc, k, bins2):
+    """Few pass-2 bins force keys past the LDS table into the global table;
+    each key lives in one of them, so the set is still exact."""
+    iso = isolate(seed=77, genome_len=120_000, n_pairs=9000)
+    bank = iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=2, cms_width=1 << 18, placement=2, bins2=bins2)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=2, width=1 << 18)
+    _solid_equal(g, r)
+    h = F.build_solid_handle(ctx, dev, k, target_solid_cov=2, cms_width=1 << 18, placement=2, bins2=bins2)
+    st = h.stats()
+    h.close()
+    assert st["keys"] > 6000 * bins2  # more keys than the LDS tables hold
+
+
+@pytest.mark.parametrize("placement", [1, 2])
+def test_pass2_placements_agree_on_stats(ctx, small_iso, placement):
+    dev = F.DeviceBank(ctx, small_iso.bank())
+    h = F.build_solid_handle(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 18, placement=placement)
+    st = h.stats()
+    h.close()
+    h1 = F.build_solid_handle(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 18, placement=1)
+    st1 = h1.stats()
+    h1.close()
+    assert st["records"] == st1["records"] and st["keys"] == st1["keys"] and st["windows"] == st1["windows"]

@@ -4,13 +4,13 @@
 // small enough to count in one workgroup's LDS.
 //   Phase A (k_bin_scatter): each workgroup walks its reads twice.  The first
 //     walk counts the workgroup's updates per bin in LDS (and feeds the HLL).
-//     One global atomic per bin then reserves a run in that bin's region.  The
-//     second walk writes each update's 16-bit cell offset into the run.  An
-//     update that finds its bin region full is added to the table directly
-//     with a global atomic, so any input (low-complexity reads included)
-//     stays exact.
-//   Phase B (k_bin_count): one workgroup per bin counts the bin's offsets
-//     with LDS atomics, adds them to the table (the bin's cells belong to this
+//     One global atomic per bin then reserves a run in that bin's region,
+//     in the slice that belongs to the workgroup's XCD.  The second walk
+//     writes each update's 16-bit cell offset into the run.  An update that
+//     finds its slice full is added to the table directly with a global
+//     atomic, so any input (low-complexity reads included) stays exact.
+//   Phase B (k_bin_count): one workgroup per bin counts the offsets of the
+//     bin's slices with LDS atomics, adds them to the table (the bin's cells belong to this
 //     workgroup alone) and writes the pass-2 threshold bits of those cells.
 // Every update is +1, so the final counters equal the direct-atomic ones.
 #include "fb_device.hpp"
@@ -21,6 +21,7 @@ using namespace fbk;
 
 constexpr int kScatterBlock = 512;
 constexpr int kCountBlock = 1024;
+static_assert(kSlices == kBinSlices, "slice count of device and host code");
 
 __device__ __forceinline__ bool share_of(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
                                          uint32_t& w0, uint32_t& w1) {
@@ -87,20 +88,21 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t gid,
   const uint32_t sb = a.sb;
   if (valid) walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(hist + (cell >> sb), 1u); });
   __syncthreads();
+  const uint32_t sl = xcc_slice();
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
     const uint32_t c = hist[b];
-    cur[b] = c ? atomicAdd(a.cursor + b, c) : 0u;
+    cur[b] = c ? atomicAdd(a.cursor + (uint64_t)b * kSlices + sl, c) : 0u;
   }
   __syncthreads();
   if (valid) {
     const uint64_t cap = a.cap;
     const uint32_t smask = (1u << sb) - 1u;
-    uint16_t* __restrict__ out = a.data;
+    uint16_t* __restrict__ out = a.data + sl * cap;
     uint32_t* __restrict__ counts = a.counts;
     walk_cells<L, false>(a, r, w0, w1, [&](uint64_t cell) {
       const uint32_t b = (uint32_t)(cell >> sb);
       const uint32_t pos = atomicAdd(cur + b, 1u);
-      if (pos < cap) out[(uint64_t)b * cap + pos] = (uint16_t)(cell & smask);
+      if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = (uint16_t)(cell & smask);
       else atomicAdd(counts + cell, 1u);
     });
   }
@@ -126,18 +128,20 @@ __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uin
   const uint32_t ncells = (uint32_t)((a.n_cells - first) < S ? (a.n_cells - first) : S);
   for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) lds[i] = 0;
   __syncthreads();
-  const uint32_t filled = a.cursor[b];
-  const uint32_t n = filled < a.cap ? filled : (uint32_t)a.cap;
-  const uint16_t* __restrict__ d = a.data + (uint64_t)b * a.cap;  // cap is a multiple of 4
-  const uint32_t n4 = n & ~3u;
-  for (uint32_t i = threadIdx.x * 4u; i < n4; i += blockDim.x * 4u) {
-    const uint2 v = *reinterpret_cast<const uint2*>(d + i);
-    atomicAdd(lds + (v.x & 0xFFFFu), 1u);
-    atomicAdd(lds + (v.x >> 16), 1u);
-    atomicAdd(lds + (v.y & 0xFFFFu), 1u);
-    atomicAdd(lds + (v.y >> 16), 1u);
+  for (uint32_t sl = 0; sl < kSlices; ++sl) {
+    const uint32_t filled = a.cursor[(uint64_t)b * kSlices + sl];
+    const uint32_t n = filled < a.cap ? filled : (uint32_t)a.cap;
+    const uint16_t* __restrict__ d = a.data + ((uint64_t)b * kSlices + sl) * a.cap;  // cap is a multiple of 4
+    const uint32_t n4 = n & ~3u;
+    for (uint32_t i = threadIdx.x * 4u; i < n4; i += blockDim.x * 4u) {
+      const uint2 v = *reinterpret_cast<const uint2*>(d + i);
+      atomicAdd(lds + (v.x & 0xFFFFu), 1u);
+      atomicAdd(lds + (v.x >> 16), 1u);
+      atomicAdd(lds + (v.y & 0xFFFFu), 1u);
+      atomicAdd(lds + (v.y >> 16), 1u);
+    }
+    for (uint32_t i = n4 + threadIdx.x; i < n; i += blockDim.x) atomicAdd(lds + d[i], 1u);
   }
-  for (uint32_t i = n4 + threadIdx.x; i < n; i += blockDim.x) atomicAdd(lds + d[i], 1u);
   __syncthreads();
   // add into the table and emit the threshold bits, 64 cells per wave step
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;

@@ -200,6 +200,18 @@ __device__ __forceinline__ void hll_index_rho(uint64_t h, uint32_t p, uint32_t& 
   rho = w ? (uint32_t)__builtin_clzll(w) - p + 1u : 64u - p + 1u;
 }
 
+// Slices per bin region: one per XCD (accelerator complex die) of the MI355X.
+constexpr uint32_t kSlices = 8;
+
+// XCD the calling wave runs on (hardware register XCC_ID, bits 3:0), folded
+// into 0..kSlices-1.  All waves of a workgroup share one CU and so one XCD.
+// A workgroup writes its share of a bin into its XCD's slice, so every line
+// that is still being filled is held in one L2 only.  Results never depend
+// on the value: any slice choice is exact, only the write traffic changes.
+__device__ __forceinline__ uint32_t xcc_slice() {
+  return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kSlices - 1u);
+}
+
 // Isolate of block b in a multi-isolate launch: the largest i with blk[i] <= b
 // (blk holds n+1 block prefixes, blk[0] = 0).
 __device__ __forceinline__ uint32_t find_iso(const uint32_t* __restrict__ blk, uint32_t n, uint32_t b) {

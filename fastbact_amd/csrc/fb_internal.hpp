@@ -7,6 +7,7 @@ namespace fb {
 
 constexpr uint64_t kEmpty = ~0ull;     // empty key word of the solid table
 constexpr int kArenaPadWords = 8;      // zero words after every device arena
+constexpr uint32_t kBinSlices = 8;     // XCD slices of every bin region (one per XCD)
 
 // One device read list: bases packed LSB-first, 32 per word (the reference's
 // BitSeqArena layout, src/common/bitseq.rs:145-149).  Several host arenas are
@@ -88,9 +89,9 @@ struct BinArgs {
   DevHll hll;
   uint32_t sb = 14;            // log2 cells per bin (<= 16)
   uint32_t nbins = 0;
-  uint64_t cap = 0;            // offsets per bin region, multiple of 4
-  uint16_t* data = nullptr;    // nbins * cap cell offsets
-  uint32_t* cursor = nullptr;  // nbins fill counters, zeroed before phase A
+  uint64_t cap = 0;            // offsets per (bin, XCD slice) region, multiple of 4
+  uint16_t* data = nullptr;    // nbins * kBinSlices * cap cell offsets
+  uint32_t* cursor = nullptr;  // nbins * kBinSlices fill counters, zeroed before phase A
   uint32_t* bits = nullptr;    // optional threshold bits of count >= t_solid
   uint32_t t_solid = 0;
   uint32_t cap_value = 65535;  // counter saturation (15 or 65535)
@@ -140,11 +141,15 @@ struct P2BinArgs {
   uint32_t log2w = 0;
   uint32_t nbins = 1;
   uint32_t log2_nbins = 0;
-  uint64_t bin_cap = 0;          // records per bin region
-  uint32_t* cursor = nullptr;    // nbins, zeroed
-  uint64_t* rec_keys = nullptr;  // nbins * kw * bin_cap
-  uint8_t* rec_info = nullptr;   // nbins * bin_cap
-  uint32_t* status = nullptr;    // [0] region overflow, zeroed
+  uint64_t* gate_masks = nullptr;  // n_reads * split, gate bit of window p0+i at bit i
+  uint32_t rw = 1;               // 64-bit words per record: ceil((2k + 8) / 64)
+  // nbins * kBinSlices + 1 run counters, zeroed: record counts per (bin,
+  // slice), then (after the scan) region starts, then (after phase A) region
+  // ends; the last entry holds the record total.
+  uint32_t* runs = nullptr;
+  uint64_t* recs = nullptr;      // records of rw words, one per window at most
+  uint64_t rec_cap = 0;          // records the buffer holds
+  uint32_t* status = nullptr;    // [0] record buffer overflow (never for exact runs), zeroed
   uint32_t lds_cap = 0;          // LDS table entries
   uint32_t min_trans_cov = 0;
   DevSolidTable table;           // keys that find the LDS table full
@@ -159,8 +164,10 @@ uint32_t p2_lds_cap(uint32_t kw);
 uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap);
 hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s);
-hipError_t launch_p2_binned_multi(const MultiLaunch& scatter, const MultiLaunch& bins, uint32_t k,
-                                  uint32_t max_nbins, uint32_t lds_cap, hipStream_t s);
+// gate and scatter: one block per kP2GateBlock / kP2ScatterBlock threads of
+// each isolate; bins: one block per bin.
+hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
+                                  uint32_t k, uint32_t max_nbins, uint32_t lds_cap, hipStream_t s);
 
 // Multi-table forms for the batched pass (blockIdx.y = isolate).
 struct P3Args {

@@ -4,13 +4,22 @@
 // enough that one workgroup holds the bin's distinct keys in LDS
 // (src/graph/build.rs:140-328: the accumulation is a set of saturating adds per
 // key, so any grouping of the records gives the same result).
-//   Phase A (k_p2_scatter): each thread walks its windows twice.  The first
-//     walk gates windows against the Count-Min bitmap, keeps the gate bits in
-//     a register and counts the workgroup's records per bin in LDS; one global
-//     atomic per bin reserves a run.  The second walk writes one record per
-//     window that has any (the key's 62-bit words and an edge byte).  A run
-//     that would pass its bin's region sets an overflow flag and writes
-//     nothing; the host then runs the global-table pass instead.
+//   Gate (k_p2_gate, one launch per Count-Min row): each thread walks its
+//     windows and clears, in a 64-bit mask, the window bits whose cell in
+//     this row is below the threshold.  One row's threshold bits (width/8
+//     bytes: 2 MB at width 2^24) stay resident in every XCD's L2 for a whole
+//     launch; the four rows together do not fit one L2, and probing them all
+//     per window is bound by the misses.  The launch of the last row
+//     (k_p2_count) also counts the records per (bin, slice); a one-block scan
+//     (k_p2_scan) turns the counts into exact region starts, so no region
+//     can overflow whatever the skew of the k-mer spectrum.
+//   Phase A (k_p2_scatter): the first walk counts the workgroup's records per
+//     bin in LDS; one global atomic per bin reserves a run in the region of
+//     (bin, slice), slice = workgroup index mod 8.  Workgroups are handed to
+//     the 8 XCDs in turn, so each region is filled from one XCD and its partly
+//     written lines stay in one L2 until they are whole.  The second walk
+//     writes one record per window that has any: the canonical k-mer's 2k
+//     bits and the edge byte, packed densely into rw 64-bit words.
 //   Phase B (k_p2_bins): one workgroup per bin builds the bin's table in LDS
 //     (write-once keys claimed word by word with compare-and-swap, as the
 //     global table does), then emits the solid k-mers (pass 3) directly.  A
@@ -24,6 +33,7 @@
 namespace fb {
 using namespace fbk;
 
+constexpr int kP2GateBlock = 256;
 constexpr int kP2ScatterBlock = 512;
 constexpr int kP2BinBlock = 1024;
 
@@ -41,71 +51,38 @@ __device__ __forceinline__ void pack62b(const Kmer<L>& x, uint64_t (&w)[KW]) {
   }
 }
 
-// One thread's walk over windows p0..p1 of read r.  cb(x, key, fold, left,
-// e_left, right, e_right) is called for every window with a record; gates
-// holds (PASS 0) or supplies (PASS 1) the gate bit of window p0+i at bit i.
-template <int L, int PASS, class CB>
-__device__ __forceinline__ void p2_walk(const P2BinArgs& a, uint64_t r, uint32_t p0, uint32_t p1, uint64_t& gates,
-                                        CB&& cb) {
-  const uint32_t k = a.k;
-  const uint64_t* __restrict__ data = a.reads.data;
-  const uint64_t g = a.reads.offs[r] + p0;
-  Kmer<L> win = window_bits<L>(data, g, k);
-  Kmer<L> fwd = fwd_from_window<L>(win, k);
-  Kmer<L> rev = rc_from_window<L>(win, k);
-  BaseStream in, first;
-  in.init(data, g + k);
-  first.init(data, g);
-  bool cur_rc = kmer_lt<L>(rev, fwd);
-  Kmer<L> cur = cur_rc ? rev : fwd;
-  uint64_t cur_fold = kmer_fold<L>(cur);
-  auto gate_of = [&](uint32_t i, uint64_t fold) -> bool {
-    if (PASS == 0) {
-      const uint64_t wmask = (1ull << a.log2w) - 1ull;
-      bool ok = true;
+// Record layout: bits [0, 2k) hold the k-mer integer (limbs least significant
+// first), bits [2k, 2k+8) the edge byte; rw = ceil((2k+8)/64) words.
+template <int L>
+__device__ __forceinline__ void pack_record(const Kmer<L>& key, uint32_t info, uint32_t k, uint64_t (&w)[L + 1]) {
+  const uint32_t q = (2u * k) >> 6, s = (2u * k) & 63u;
 #pragma unroll
-      for (int row = 0; row < 4; ++row) {
-        const uint64_t c = ((uint64_t)row << a.log2w) | (splitmix64(fold ^ cms_seed(row)) & wmask);
-        ok &= (a.solid_bits[c >> 5] >> (c & 31u)) & 1u;
-      }
-      if (ok) gates |= 1ull << i;
-      return ok;
-    }
-    return (gates >> i) & 1ull;
-  };
-  bool cur_solid = gate_of(0, cur_fold);
-  bool left = false;
-  uint32_t left_rcp = 0;
-  for (uint32_t x = p0;; ++x) {
-    bool right = false;
-    uint32_t b = 0;
-    bool nxt_rc = false, nxt_solid = false;
-    Kmer<L> nxt;
-    uint64_t nxt_fold = 0;
-    if (x < p1) {
-      b = in.next();
-      kmer_roll<L>(fwd, rev, b, k);
-      nxt_rc = kmer_lt<L>(rev, fwd);
-      nxt = nxt_rc ? rev : fwd;
-      nxt_fold = kmer_fold<L>(nxt);
-      nxt_solid = gate_of(x + 1 - p0, nxt_fold);
-      right = cur_solid && nxt_solid;
-    }
-    const uint32_t first_base = first.next();
-    if (left || right) {
-      // edge_v of pair (x-1, x) and edge_u of pair (x, x+1), build.rs:268-275
-      const uint32_t e_left = cur_rc ? left_rcp : left_rcp + 4u;
-      const uint32_t e_right = cur_rc ? b + 4u : b;
-      cb(cur, cur_fold, left, e_left, right, e_right);
-    }
-    if (x >= p1) break;
-    left = right;
-    left_rcp = 3u - first_base;
-    cur = nxt;
-    cur_rc = nxt_rc;
-    cur_fold = nxt_fold;
-    cur_solid = nxt_solid;
+  for (int j = 0; j <= L; ++j) {
+    uint64_t v = j < L ? key.a[j] : 0ull;
+    if ((uint32_t)j == q) v |= (uint64_t)info << s;
+    if (s > 56u && (uint32_t)j == q + 1u) v |= (uint64_t)info >> (64u - s);
+    w[j] = v;
   }
+}
+template <int L>
+__device__ __forceinline__ void unpack_record(const uint64_t* __restrict__ src, uint32_t rw, uint32_t k, Kmer<L>& key,
+                                              uint32_t& info) {
+  uint64_t w[L + 1];
+#pragma unroll
+  for (int j = 0; j <= L; ++j) w[j] = (uint32_t)j < rw ? src[j] : 0ull;
+  const uint32_t q = (2u * k) >> 6, s = (2u * k) & 63u;
+  uint64_t lo = 0, hi = 0;
+#pragma unroll
+  for (int j = 0; j <= L; ++j) {
+    if ((uint32_t)j == q) lo = w[j];
+    if ((uint32_t)j == q + 1u) hi = w[j];
+  }
+  uint64_t v = lo >> s;
+  if (s > 56u) v |= hi << (64u - s);
+  info = (uint32_t)v & 0xffu;
+#pragma unroll
+  for (int j = 0; j < L; ++j) key.a[j] = w[j];
+  key.a[L - 1] &= top_mask<L>(k);
 }
 
 __device__ __forceinline__ bool p2_share(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
@@ -121,8 +98,195 @@ __device__ __forceinline__ bool p2_share(const DevReads& R, uint64_t gid, uint32
   return p0 < p1;
 }
 
+// Gate of one Count-Min row over the windows p0..p1 of read r (p1 - p0 <=
+// 63): clears bit i of m when window p0+i's cell in this row is below the
+// threshold.
+template <int L>
+__device__ __forceinline__ uint64_t gate_row(const P2BinArgs& a, uint64_t r, uint32_t p0, uint32_t p1, uint64_t m,
+                                             uint32_t row) {
+  const uint32_t k = a.k;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r] + p0;
+  Kmer<L> win = window_bits<L>(data, g, k);
+  Kmer<L> fwd = fwd_from_window<L>(win, k);
+  Kmer<L> rev = rc_from_window<L>(win, k);
+  BaseStream in;
+  in.init(data, g + k);
+  const uint64_t wmask = (1ull << a.log2w) - 1ull;
+  const uint64_t rbase = (uint64_t)row << a.log2w;
+  const uint64_t seed = cms_seed((int)row);
+  const uint32_t last = p1 - p0;
+  for (uint32_t i = 0;; ++i) {
+    if ((m >> i) & 1ull) {
+      const uint64_t fold = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
+      const uint64_t c = rbase | (splitmix64(fold ^ seed) & wmask);
+      if (!((a.solid_bits[c >> 5] >> (c & 31u)) & 1u)) m &= ~(1ull << i);
+    }
+    if (i == last) break;
+    kmer_roll<L>(fwd, rev, in.next(), k);
+  }
+  return m;
+}
+
+// Rows 0..2: row 0 starts from all ones, rows 1 and 2 narrow the stored mask.
+template <int L>
+__device__ __forceinline__ void p2_gate_body(const P2BinArgs& a, uint64_t gid, uint32_t row) {
+  uint64_t r = 0;
+  uint32_t p0 = 0, p1 = 0;
+  if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) return;
+  const uint64_t m = row ? a.gate_masks[gid] : ~0ull;
+  a.gate_masks[gid] = m ? gate_row<L>(a, r, p0, p1, m, row) : 0ull;
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2GateBlock) void k_p2_gate(P2BinArgs a, uint32_t row) {
+  p2_gate_body<L>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, row);
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2GateBlock) void k_p2_gate_m(const P2BinArgs* __restrict__ args,
+                                                            const uint32_t* __restrict__ blk, uint32_t n_iso,
+                                                            uint32_t row) {
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  p2_gate_body<L>(args[iso], (uint64_t)(blockIdx.x - blk[iso]) * blockDim.x + threadIdx.x, row);
+}
+
+// One thread's walk over windows p0..p1 of read r with the gate bits of
+// window p0+i at bit i of gates.  cb(key, fold, left, e_left, right, e_right)
+// is called for every window with a record (build.rs:253-285).
+template <int L, class CB>
+__device__ __forceinline__ void p2_walk(const P2BinArgs& a, uint64_t r, uint32_t p0, uint32_t p1, uint64_t gates,
+                                        CB&& cb) {
+  const uint32_t k = a.k;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r] + p0;
+  Kmer<L> win = window_bits<L>(data, g, k);
+  Kmer<L> fwd = fwd_from_window<L>(win, k);
+  Kmer<L> rev = rc_from_window<L>(win, k);
+  BaseStream in, first;
+  in.init(data, g + k);
+  first.init(data, g);
+  bool cur_rc = kmer_lt<L>(rev, fwd);
+  Kmer<L> cur = cur_rc ? rev : fwd;
+  bool cur_solid = gates & 1ull;
+  bool left = false;
+  uint32_t left_rcp = 0;
+  for (uint32_t x = p0;; ++x) {
+    bool right = false;
+    uint32_t b = 0;
+    bool nxt_rc = false, nxt_solid = false;
+    Kmer<L> nxt;
+    if (x < p1) {
+      b = in.next();
+      kmer_roll<L>(fwd, rev, b, k);
+      nxt_rc = kmer_lt<L>(rev, fwd);
+      nxt = nxt_rc ? rev : fwd;
+      nxt_solid = (gates >> (x + 1 - p0)) & 1ull;
+      right = cur_solid && nxt_solid;
+    }
+    const uint32_t first_base = first.next();
+    if (left || right) {
+      // edge_v of pair (x-1, x) and edge_u of pair (x, x+1), build.rs:268-275
+      const uint32_t e_left = cur_rc ? left_rcp : left_rcp + 4u;
+      const uint32_t e_right = cur_rc ? b + 4u : b;
+      cb(cur, kmer_fold<L>(cur), left, e_left, right, e_right);
+    }
+    if (x >= p1) break;
+    left = right;
+    left_rcp = 3u - first_base;
+    cur = nxt;
+    cur_rc = nxt_rc;
+    cur_solid = nxt_solid;
+  }
+}
+
+__device__ __forceinline__ uint32_t p2_bin_of(const P2BinArgs& a, uint64_t fold) {
+  return a.log2_nbins ? (uint32_t)(slot_hash2(fold) >> (64u - a.log2_nbins)) : 0u;
+}
+
+// Row 3 of the gate, then the records of the final mask counted per bin for
+// the scatter workgroup that will hold these threads (slice = its index mod
+// kSlices).  A gate block lies inside one scatter block.
+static_assert(kP2ScatterBlock % kP2GateBlock == 0, "gate blocks nest in scatter blocks");
+template <int L>
+__device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t gid, uint32_t sl, uint32_t* hist) {
+  for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
+  __syncthreads();
+  uint64_t r = 0;
+  uint32_t p0 = 0, p1 = 0;
+  uint64_t m = 0;
+  if (p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) {
+    m = a.gate_masks[gid];
+    if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
+    a.gate_masks[gid] = m;
+  }
+  if (m)
+    p2_walk<L>(a, r, p0, p1, m, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
+      atomicAdd(hist + p2_bin_of(a, fold), 1u);
+    });
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
+    const uint32_t c = hist[b];
+    if (c) atomicAdd(a.runs + (uint64_t)b * kSlices + sl, c);
+  }
+}
+
+__device__ __forceinline__ uint32_t slice_of_gate_block(uint32_t blk) {
+  return (uint32_t)(((uint64_t)blk * kP2GateBlock / kP2ScatterBlock) & (kSlices - 1u));
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2GateBlock) void k_p2_count(P2BinArgs a) {
+  extern __shared__ uint32_t sh[];
+  p2_count_body<L>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, slice_of_gate_block(blockIdx.x), sh);
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2GateBlock) void k_p2_count_m(const P2BinArgs* __restrict__ args,
+                                                             const uint32_t* __restrict__ blk, uint32_t n_iso) {
+  extern __shared__ uint32_t sh[];
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  const uint32_t lb = blockIdx.x - blk[iso];
+  p2_count_body<L>(args[iso], (uint64_t)lb * blockDim.x + threadIdx.x, slice_of_gate_block(lb), sh);
+}
+
+// Exclusive scan of runs[0..n) in place, runs[n] = total; one block.
+__device__ __forceinline__ void p2_scan_body(uint32_t* runs, uint32_t n) {
+  __shared__ uint32_t s_wave[16];
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
+  const uint32_t hi = lo + per < n ? lo + per : n;
+  uint32_t sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += runs[i];
+  uint32_t incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off);
+    if ((threadIdx.x & 63u) >= (uint32_t)off) incl += v;
+  }
+  if ((threadIdx.x & 63u) == 63u) s_wave[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) before += s_wave[w];
+  uint32_t run = before + incl - sum;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t v = runs[i];
+    runs[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == blockDim.x - 1) runs[n] = run;
+}
+
+constexpr int kP2ScanBlock = 1024;
+__global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan(uint32_t* runs, uint32_t n) { p2_scan_body(runs, n); }
+
+__global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan_m(const P2BinArgs* __restrict__ args, uint32_t n_iso) {
+  const P2BinArgs& a = args[blockIdx.x];
+  if (blockIdx.x < n_iso && a.runs) p2_scan_body(a.runs, a.nbins * kSlices);
+}
+
 template <int L, int KW>
-__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t gid, uint32_t* sh) {
+__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t gid, uint32_t sl, uint32_t* sh) {
   uint32_t* hist = sh;
   uint32_t* cur = sh + a.nbins;
   for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
@@ -130,43 +294,42 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t gid
   uint64_t r = 0;
   uint32_t p0 = 0, p1 = 0;
   const bool valid = p2_share(a.reads, gid, a.split, a.k, r, p0, p1);
-  const uint32_t shift = 64u - a.log2_nbins;
-  uint64_t gates = 0;
-  if (valid)
-    p2_walk<L, 0>(a, r, p0, p1, gates, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
-      const uint32_t bin = a.log2_nbins ? (uint32_t)(slot_hash2(fold) >> shift) : 0u;
-      atomicAdd(hist + bin, 1u);
+  const uint64_t gates = valid ? a.gate_masks[gid] : 0ull;
+  if (gates)
+    p2_walk<L>(a, r, p0, p1, gates, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
+      atomicAdd(hist + p2_bin_of(a, fold), 1u);
     });
   __syncthreads();
+  // absolute start of this workgroup's run in the (bin, slice) region
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
     const uint32_t c = hist[b];
-    cur[b] = c ? atomicAdd(a.cursor + b, c) : 0u;
+    cur[b] = c ? atomicAdd(a.runs + (uint64_t)b * kSlices + sl, c) : 0u;
   }
   __syncthreads();
-  if (valid) {
-    const uint64_t cap = a.bin_cap;
-    p2_walk<L, 1>(a, r, p0, p1, gates,
-                  [&](const Kmer<L>& key, uint64_t fold, bool left, uint32_t el, bool right, uint32_t er) {
-                    const uint32_t bin = a.log2_nbins ? (uint32_t)(slot_hash2(fold) >> shift) : 0u;
-                    const uint32_t pos = atomicAdd(cur + bin, 1u);
-                    if (pos >= cap) {
-                      a.status[0] = 1u;  // region full: the host re-runs this pass on the global table
-                      return;
-                    }
-                    uint64_t w[KW];
-                    pack62b<L, KW>(key, w);
+  if (gates) {
+    const uint64_t cap = a.rec_cap;
+    const uint32_t rw = a.rw, k = a.k;
+    p2_walk<L>(a, r, p0, p1, gates,
+               [&](const Kmer<L>& key, uint64_t fold, bool left, uint32_t el, bool right, uint32_t er) {
+                 const uint32_t pos = atomicAdd(cur + p2_bin_of(a, fold), 1u);
+                 if (pos >= cap) {
+                   a.status[0] = 1u;  // runs disagree with the counts: the host falls back
+                   return;
+                 }
+                 uint64_t w[L + 1];
+                 pack_record<L>(key, (left ? 1u : 0u) | (el << 1) | (right ? 16u : 0u) | (er << 5), k, w);
+                 uint64_t* dst = a.recs + (uint64_t)pos * rw;
 #pragma unroll
-                    for (int j = 0; j < KW; ++j) a.rec_keys[((uint64_t)bin * KW + j) * cap + pos] = w[j];
-                    a.rec_info[(uint64_t)bin * cap + pos] =
-                        (uint8_t)((left ? 1u : 0u) | (el << 1) | (right ? 16u : 0u) | (er << 5));
-                  });
+                 for (int j = 0; j <= L; ++j)
+                   if ((uint32_t)j < rw) dst[j] = w[j];
+               });
   }
 }
 
 template <int L, int KW>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter(P2BinArgs a) {
   extern __shared__ uint32_t sh[];
-  p2_scatter_body<L, KW>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, sh);
+  p2_scatter_body<L, KW>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x & (kSlices - 1u), sh);
 }
 
 template <int L, int KW>
@@ -174,7 +337,8 @@ __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_m(const P2BinArg
                                                                   const uint32_t* __restrict__ blk, uint32_t n_iso) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  p2_scatter_body<L, KW>(args[iso], (uint64_t)(blockIdx.x - blk[iso]) * blockDim.x + threadIdx.x, sh);
+  const uint32_t lb = blockIdx.x - blk[iso];
+  p2_scatter_body<L, KW>(args[iso], (uint64_t)lb * blockDim.x + threadIdx.x, lb & (kSlices - 1u), sh);
 }
 
 // LDS hash of a packed key (placement inside a bin; independent of the bin bits).
@@ -210,7 +374,7 @@ __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, 
   return -1;
 }
 
-template <int KW>
+template <int L, int KW>
 __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, unsigned char* smem) {
   const uint32_t cap = a.lds_cap;
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(smem);      // KW * cap
@@ -223,15 +387,20 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   for (uint32_t i = threadIdx.x; i < 5 * cap; i += blockDim.x) cov[i] = 0u;
   if (threadIdx.x < 2) s_stat[threadIdx.x] = 0ull;
   __syncthreads();
-  const uint32_t filled = a.cursor[bin];
-  const uint64_t rcap = a.bin_cap;
-  const uint32_t n = filled < rcap ? filled : (uint32_t)rcap;
+  // the bin's kSlices regions are adjacent: [end of the previous bin, end of
+  // this bin's last slice)
+  const uint64_t e0 = bin ? a.runs[(uint64_t)bin * kSlices - 1] : 0ull;
+  uint64_t e1 = a.runs[(uint64_t)bin * kSlices + kSlices - 1];
+  if (e1 > a.rec_cap) e1 = a.rec_cap;
+  const uint32_t n = e1 > e0 ? (uint32_t)(e1 - e0) : 0u;
+  const uint32_t rw = a.rw;
   const uint32_t gate = a.min_trans_cov;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    Kmer<L> key;
+    uint32_t info;
+    unpack_record<L>(a.recs + (e0 + i) * rw, rw, a.k, key, info);
     uint64_t w[KW];
-#pragma unroll
-    for (int j = 0; j < KW; ++j) w[j] = a.rec_keys[((uint64_t)bin * KW + j) * rcap + i];
-    const uint32_t info = a.rec_info[(uint64_t)bin * rcap + i];
+    pack62b<L, KW>(key, w);
     const bool left = info & 1u, right = (info >> 4) & 1u;
     const uint32_t el = (info >> 1) & 7u, er = (info >> 5) & 7u;
     const int e = lds_find<KW>(keys, cap, w);
@@ -335,18 +504,18 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   }
 }
 
-template <int KW>
+template <int L, int KW>
 __global__ __launch_bounds__(kP2BinBlock) void k_p2_bins(P2BinArgs a) {
   extern __shared__ unsigned char smem[];
-  p2_bins_body<KW>(a, blockIdx.x, smem);
+  p2_bins_body<L, KW>(a, blockIdx.x, smem);
 }
 
-template <int KW>
+template <int L, int KW>
 __global__ __launch_bounds__(kP2BinBlock) void k_p2_bins_m(const P2BinArgs* __restrict__ args,
                                                            const uint32_t* __restrict__ blk, uint32_t n_iso) {
   extern __shared__ unsigned char smem[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  p2_bins_body<KW>(args[iso], blockIdx.x - blk[iso], smem);
+  p2_bins_body<L, KW>(args[iso], blockIdx.x - blk[iso], smem);
 }
 
 // LDS the bin kernels may use: the full 160 KB once the attribute is granted,
@@ -361,34 +530,27 @@ static hipError_t set_attr(const void* f) {
   return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kDynLds);
 }
 
-template <int KW>
-static hipError_t set_bins_attr() {
-  hipError_t e = set_attr(reinterpret_cast<const void*>(&k_p2_bins<KW>));
-  if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_bins_m<KW>));
-  return e;
-}
 template <int L, int KW>
-static hipError_t set_scatter_attr() {
-  hipError_t e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter<L, KW>));
+static hipError_t set_pair_attr() {
+  hipError_t e = set_attr(reinterpret_cast<const void*>(&k_p2_count<L>));
+  if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_count_m<L>));
+  if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_bins<L, KW>));
+  if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_bins_m<L, KW>));
+  if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter<L, KW>));
   if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_m<L, KW>));
   return e;
 }
 
 static uint32_t lds_budget() {
   if (!g_lds_bytes) {
-    hipError_t e = set_bins_attr<1>();
-    if (e == hipSuccess) e = set_bins_attr<2>();
-    if (e == hipSuccess) e = set_bins_attr<3>();
-    if (e == hipSuccess) e = set_bins_attr<4>();
-    if (e == hipSuccess) e = set_bins_attr<5>();
-    if (e == hipSuccess) e = set_scatter_attr<1, 1>();
-    if (e == hipSuccess) e = set_scatter_attr<1, 2>();
-    if (e == hipSuccess) e = set_scatter_attr<2, 2>();
-    if (e == hipSuccess) e = set_scatter_attr<2, 3>();
-    if (e == hipSuccess) e = set_scatter_attr<3, 3>();
-    if (e == hipSuccess) e = set_scatter_attr<3, 4>();
-    if (e == hipSuccess) e = set_scatter_attr<4, 4>();
-    if (e == hipSuccess) e = set_scatter_attr<4, 5>();
+    hipError_t e = set_pair_attr<1, 1>();
+    if (e == hipSuccess) e = set_pair_attr<1, 2>();
+    if (e == hipSuccess) e = set_pair_attr<2, 2>();
+    if (e == hipSuccess) e = set_pair_attr<2, 3>();
+    if (e == hipSuccess) e = set_pair_attr<3, 3>();
+    if (e == hipSuccess) e = set_pair_attr<3, 4>();
+    if (e == hipSuccess) e = set_pair_attr<4, 4>();
+    if (e == hipSuccess) e = set_pair_attr<4, 5>();
     if (e != hipSuccess) (void)hipGetLastError();
     g_lds_bytes = e == hipSuccess ? kP2LdsBytes : 65536u;
   }
@@ -404,76 +566,88 @@ uint32_t p2_lds_cap(uint32_t kw) {
 }
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap) { return (size_t)cap * (kw * 8u + 4u + 16u); }
 
-hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s) {
-  const uint32_t L = (2 * a.k + 63) / 64;
-  const uint32_t KW = (a.k + 30) / 31;
-  const size_t scat_lds = (size_t)a.nbins * 8;
-  if (a.reads.n_reads) {
-    const uint64_t threads = a.reads.n_reads * (uint64_t)a.split;
-    const unsigned grid = (unsigned)((threads + kP2ScatterBlock - 1) / kP2ScatterBlock);
-#define SC(LL, KK) hipLaunchKernelGGL((k_p2_scatter<LL, KK>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, a)
-    if (L == 1 && KW == 1) SC(1, 1);
-    else if (L == 1) SC(1, 2);
-    else if (L == 2 && KW == 2) SC(2, 2);
-    else if (L == 2) SC(2, 3);
-    else if (L == 3 && KW == 3) SC(3, 3);
-    else if (L == 3) SC(3, 4);
-    else if (KW == 4) SC(4, 4);
-    else SC(4, 5);
-#undef SC
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  const size_t lds = p2_lds_bytes(KW, a.lds_cap);
-  if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
-#define BN(KK) hipLaunchKernelGGL((k_p2_bins<KK>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, a)
-  switch (KW) {
-    case 1: BN(1); break;
-    case 2: BN(2); break;
-    case 3: BN(3); break;
-    case 4: BN(4); break;
-    default: BN(5); break;
-  }
-#undef BN
-  return hipGetLastError();
-}
-
-hipError_t launch_p2_binned_multi(const MultiLaunch& scatter, const MultiLaunch& bins, uint32_t k,
-                                  uint32_t max_nbins, uint32_t lds_cap, hipStream_t s) {
+// Calls F<L, KW>() for the template pair of k.
+template <class F>
+static hipError_t dispatch_lkw(uint32_t k, F&& f) {
   const uint32_t L = (2 * k + 63) / 64;
   const uint32_t KW = (k + 30) / 31;
-  const P2BinArgs* a = static_cast<const P2BinArgs*>(scatter.args);
-  const size_t scat_lds = (size_t)max_nbins * 8;
-  if (scatter.total_blocks) {
-#define SCM(LL, KK)                                                                                              \
-  hipLaunchKernelGGL((k_p2_scatter_m<LL, KK>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s, a, \
-                     scatter.blk, scatter.n_iso)
-    if (L == 1 && KW == 1) SCM(1, 1);
-    else if (L == 1) SCM(1, 2);
-    else if (L == 2 && KW == 2) SCM(2, 2);
-    else if (L == 2) SCM(2, 3);
-    else if (L == 3 && KW == 3) SCM(3, 3);
-    else if (L == 3) SCM(3, 4);
-    else if (KW == 4) SCM(4, 4);
-    else SCM(4, 5);
-#undef SCM
+  if (L == 1 && KW == 1) return f.template operator()<1, 1>();
+  if (L == 1) return f.template operator()<1, 2>();
+  if (L == 2 && KW == 2) return f.template operator()<2, 2>();
+  if (L == 2) return f.template operator()<2, 3>();
+  if (L == 3 && KW == 3) return f.template operator()<3, 3>();
+  if (L == 3) return f.template operator()<3, 4>();
+  if (KW == 4) return f.template operator()<4, 4>();
+  return f.template operator()<4, 5>();
+}
+
+struct P2Launch {
+  const P2BinArgs& a;
+  uint64_t threads;
+  size_t scat_lds, lds;
+  hipStream_t s;
+  template <int L, int KW>
+  hipError_t operator()() {
+    if (threads) {
+      const unsigned ggrid = (unsigned)((threads + kP2GateBlock - 1) / kP2GateBlock);
+      for (uint32_t row = 0; row < 3; ++row)
+        hipLaunchKernelGGL((k_p2_gate<L>), dim3(ggrid), dim3(kP2GateBlock), 0, s, a, row);
+      hipLaunchKernelGGL((k_p2_count<L>), dim3(ggrid), dim3(kP2GateBlock), (size_t)a.nbins * 4, s, a);
+    }
+    hipLaunchKernelGGL(k_p2_scan, dim3(1), dim3(kP2ScanBlock), 0, s, a.runs, a.nbins * kSlices);
+    if (threads) {
+      const unsigned grid = (unsigned)((threads + kP2ScatterBlock - 1) / kP2ScatterBlock);
+      hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_p2_bins<L, KW>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, a);
+    return hipGetLastError();
+  }
+};
+
+hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s) {
+  const uint32_t KW = (a.k + 30) / 31;
+  const size_t scat_lds = (size_t)a.nbins * 8;
+  const size_t lds = p2_lds_bytes(KW, a.lds_cap);
+  if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
+  if (a.rw != (2 * a.k + 8 + 63) / 64) return hipErrorInvalidValue;
+  return dispatch_lkw(a.k, P2Launch{a, a.reads.n_reads * (uint64_t)a.split, scat_lds, lds, s});
+}
+
+struct P2LaunchMulti {
+  const MultiLaunch &gate, &scatter, &bins;
+  size_t scat_lds, lds;
+  hipStream_t s;
+  template <int L, int KW>
+  hipError_t operator()() {
+    const P2BinArgs* ga = static_cast<const P2BinArgs*>(gate.args);
+    if (gate.total_blocks) {
+      for (uint32_t row = 0; row < 3; ++row)
+        hipLaunchKernelGGL((k_p2_gate_m<L>), dim3(gate.total_blocks), dim3(kP2GateBlock), 0, s, ga, gate.blk,
+                           gate.n_iso, row);
+      hipLaunchKernelGGL((k_p2_count_m<L>), dim3(gate.total_blocks), dim3(kP2GateBlock), scat_lds / 2, s, ga,
+                         gate.blk, gate.n_iso);
+    }
+    if (gate.n_iso) hipLaunchKernelGGL(k_p2_scan_m, dim3(gate.n_iso), dim3(kP2ScanBlock), 0, s, ga, gate.n_iso);
+    if (scatter.total_blocks)
+      hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,
+                         static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || !bins.total_blocks) return e;
+    hipLaunchKernelGGL((k_p2_bins_m<L, KW>), dim3(bins.total_blocks), dim3(kP2BinBlock), lds, s,
+                       static_cast<const P2BinArgs*>(bins.args), bins.blk, bins.n_iso);
+    return hipGetLastError();
   }
-  if (!bins.total_blocks) return hipSuccess;
+};
+
+hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
+                                  uint32_t k, uint32_t max_nbins, uint32_t lds_cap, hipStream_t s) {
+  const uint32_t KW = (k + 30) / 31;
+  const size_t scat_lds = (size_t)max_nbins * 8;
   const size_t lds = p2_lds_bytes(KW, lds_cap);
-  const P2BinArgs* ab = static_cast<const P2BinArgs*>(bins.args);
-#define BNM(KK) \
-  hipLaunchKernelGGL((k_p2_bins_m<KK>), dim3(bins.total_blocks), dim3(kP2BinBlock), lds, s, ab, bins.blk, bins.n_iso)
-  switch (KW) {
-    case 1: BNM(1); break;
-    case 2: BNM(2); break;
-    case 3: BNM(3); break;
-    case 4: BNM(4); break;
-    default: BNM(5); break;
-  }
-#undef BNM
-  return hipGetLastError();
+  if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
+  return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, scat_lds, lds, s});
 }
 
 }  // namespace fb

@@ -559,6 +559,17 @@ extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
 static uint32_t ref_words(uint32_t k) { return k <= 31 ? 1u : (k <= 63 ? 2u : 4u); }
 // Solid-table slot layout: kw key words of 62 bits, then the counters.
 static uint32_t key_words_for_k(uint32_t k) { return (k + 30) / 31; }
+
+// Entries per (bin, XCD slice) region when per_bin entries land in a bin on
+// average from n_wg workgroups: the even share of one slice plus 10%, and room
+// for two more workgroups' shares (few workgroups split unevenly over the
+// XCDs).  A full region spills (pass 1) or falls back (pass 2), so the size
+// only matters for speed.
+static uint64_t bin_slice_cap(double per_bin, uint64_t n_wg, uint64_t align) {
+  const double wgs = (double)std::max<uint64_t>(n_wg, 1);
+  const uint64_t c = (uint64_t)(per_bin * (1.1 / kBinSlices + 2.0 / wgs)) + 64;
+  return (c + align - 1) / align * align;
+}
 static uint32_t table_stride(uint32_t kw) { return kw + 5; }
 
 extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
@@ -630,13 +641,13 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   if (int r = bits.get(c, ((ncnt + 31) / 32) * 4)) return fail(r);
   BinArgs ba;
   if (binned) {
-    const double expected = 4.0 * (double)windows / (double)nbins;
-    ba.cap = ((uint64_t)(expected * 1.1) + 256 + 3) & ~3ull;
+    const uint64_t n_wg = (bank->n_reads * (uint64_t)split + kScatterBlockSize - 1) / kScatterBlockSize;
+    ba.cap = bin_slice_cap(4.0 * (double)windows / (double)nbins, n_wg, 4);
     ba.sb = sb;
     ba.nbins = (uint32_t)nbins;
-    if (int r = bdata.get(c, nbins * ba.cap * 2)) return fail(r);
-    if (int r = bcur.get(c, nbins * 4)) return fail(r);
-    HIP_TRY(c, launch_fill_u32(bcur.as<uint32_t>(), nbins, 0, s));
+    if (int r = bdata.get(c, nbins * kBinSlices * ba.cap * 2)) return fail(r);
+    if (int r = bcur.get(c, nbins * kBinSlices * 4)) return fail(r);
+    HIP_TRY(c, launch_fill_u32(bcur.as<uint32_t>(), nbins * kBinSlices, 0, s));
   }
   HIP_TRY(c, hipEventRecord(c->ev[0], s));
   if (c_reads) {  // prior contigs: every k-mer added 10 times (build.rs:119-138)
@@ -723,19 +734,22 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   so->stats[5] = nb2;
   so->stats[6] = lds_cap;
   so->stats[7] = (uint64_t)n_est_keys;
-  if (!c_reads && prm->placement != 1 && nb2 <= p2_max_bins() && windows > 0) {
-    const uint32_t pairs_max = bank->max_len > k ? bank->max_len - k : 1;
-    const uint32_t split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
-    const uint64_t bin_cap = (((uint64_t)((double)windows / (double)nb2 * 1.15)) + 1024 + 3) & ~3ull;
+  const uint32_t pairs_max = bank->max_len > k ? bank->max_len - k : 1;
+  const uint32_t split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
+  const uint64_t threads2 = bank->n_reads * (uint64_t)split2;
+  // a window at a share boundary can give one record to each of two threads
+  const uint64_t rec_cap = windows + threads2;
+  if (!c_reads && prm->placement != 1 && nb2 <= p2_max_bins() && windows > 0 && rec_cap < (1ull << 32)) {
+    const uint32_t rw = (2 * k + 8 + 63) / 64;
     // keys beyond what the LDS tables are expected to hold, plus slack
     const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
     const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est_keys / 4)));
     const uint64_t out_cap = nb2 * lds_cap + spill_cap;
-    DBuf rk, ri, cur, st2, p3s;
+    DBuf recs, gm, runs, st2, p3s;
     int r = 0;
-    if (!r) r = rk.get(c, nb2 * kw * bin_cap * 8);
-    if (!r) r = ri.get(c, nb2 * bin_cap);
-    if (!r) r = cur.get(c, nb2 * 4);
+    if (!r) r = recs.get(c, rec_cap * rw * 8);
+    if (!r) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
+    if (!r) r = runs.get(c, (nb2 * kBinSlices + 1) * 4);
     if (!r) r = st2.get(c, 64);
     if (!r) r = p3s.get(c, pass3_scratch_bytes());
     if (!r) r = tbuf.get(c, spill_cap * tab.stride * 8);
@@ -746,7 +760,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     tab.cap = spill_cap;
     tab.slots = tbuf.as<uint64_t>();
     HIP_TRY(c, launch_table_clear(tab, s));
-    HIP_TRY(c, launch_fill_u32(cur.as<uint32_t>(), nb2, 0, s));
+    HIP_TRY(c, launch_fill_u32(runs.as<uint32_t>(), nb2 * kBinSlices + 1, 0, s));
     HIP_TRY(c, launch_fill_u32(st2.as<uint32_t>(), 16, 0, s));
     P2BinArgs pa;
     pa.reads = bank->view();
@@ -756,11 +770,12 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.log2w = so->cms.log2w;
     pa.nbins = (uint32_t)nb2;
     pa.log2_nbins = (uint32_t)__builtin_ctzll(nb2);
-    pa.bin_cap = bin_cap;
-    pa.cursor = cur.as<uint32_t>();
-    pa.rec_keys = rk.as<uint64_t>();
-    pa.rec_info = ri.as<uint8_t>();
-    pa.status = st2.as<uint32_t>();            // [0] region overflow
+    pa.gate_masks = gm.as<uint64_t>();
+    pa.rw = rw;
+    pa.runs = runs.as<uint32_t>();
+    pa.recs = recs.as<uint64_t>();
+    pa.rec_cap = rec_cap;
+    pa.status = st2.as<uint32_t>();            // [0] record buffer overflow
     pa.out_count = st2.as<uint32_t>() + 2;     // records emitted from LDS
     pa.stats = reinterpret_cast<unsigned long long*>(st2.as<uint32_t>() + 4);
     pa.lds_cap = lds_cap;
@@ -1170,13 +1185,14 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     it.sb = sb;
     it.nbins = (uint32_t)nbins;
     if (binned) {
-      const uint64_t cap = ((uint64_t)(4.0 * (double)it.windows / (double)nbins * 1.1) + 256 + 3) & ~3ull;
+      const uint64_t n_wg = (it.bank->n_reads * (uint64_t)it.split + kScatterBlockSize - 1) / kScatterBlockSize;
+      const uint64_t cap = bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg, 4);
       if (cap > it.bins_cap || !it.bdata.p) {
-        if (int r = it.bdata.get(c, nbins * cap * 2)) return r;
+        if (int r = it.bdata.get(c, nbins * kBinSlices * cap * 2)) return r;
         it.bins_cap = cap;
       }
       if (!it.bcur.p)
-        if (int r = it.bcur.get(c, 8192 * 4)) return r;
+        if (int r = it.bcur.get(c, 8192 * kBinSlices * 4)) return r;
       b->max_nbins = std::max<uint32_t>(b->max_nbins, it.nbins);
       b->max_sb = std::max<uint32_t>(b->max_sb, sb);
       b->any_binned = true;
@@ -1198,7 +1214,7 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     // fills: counts, HLL, bin cursors
     fills.push_back({it.cms.counts.as<uint32_t>(), ncells, 0u});
     if (it.has_hll) fills.push_back({it.hll.regs.as<uint32_t>(), (uint64_t)1 << it.hll.p, 0u});
-    if (binned) fills.push_back({it.bcur.as<uint32_t>(), nbins, 0u});
+    if (binned) fills.push_back({it.bcur.as<uint32_t>(), nbins * kBinSlices, 0u});
     // pass 1
     Pass1Args& a1 = p1[i];
     a1.reads = rv;

@@ -19,7 +19,7 @@
 namespace fb {
 using namespace fbk;
 
-constexpr int kScatterBlock = 512;
+constexpr int kScatterBlock = (int)kScatterBlockSize;
 constexpr int kCountBlock = 1024;
 static_assert(kSlices == kBinSlices, "slice count of device and host code");
 
@@ -76,17 +76,21 @@ __device__ __forceinline__ void walk_cells(const BinArgs& a, uint64_t r, uint32_
   }
 }
 
+// Block blk covers thread shares [blk * chunks * blockDim, (blk + 1) * chunks * blockDim).
 template <int L, bool HLL>
-__device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t gid, uint32_t* sh) {
+__device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
   uint32_t* hist = sh;
   uint32_t* cur = sh + a.nbins;
   for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
   __syncthreads();
-  uint64_t r = 0;
-  uint32_t w0 = 0, w1 = 0;
-  const bool valid = share_of(a.reads, gid, a.split, a.k, r, w0, w1);
+  const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
   const uint32_t sb = a.sb;
-  if (valid) walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(hist + (cell >> sb), 1u); });
+  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
+    uint64_t r = 0;
+    uint32_t w0 = 0, w1 = 0;
+    if (share_of(a.reads, gid0 + (uint64_t)ch * blockDim.x, a.split, a.k, r, w0, w1))
+      walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(hist + (cell >> sb), 1u); });
+  }
   __syncthreads();
   const uint32_t sl = xcc_slice();
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
@@ -94,24 +98,27 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t gid,
     cur[b] = c ? atomicAdd(a.cursor + (uint64_t)b * kSlices + sl, c) : 0u;
   }
   __syncthreads();
-  if (valid) {
-    const uint64_t cap = a.cap;
-    const uint32_t smask = (1u << sb) - 1u;
-    uint16_t* __restrict__ out = a.data + sl * cap;
-    uint32_t* __restrict__ counts = a.counts;
-    walk_cells<L, false>(a, r, w0, w1, [&](uint64_t cell) {
-      const uint32_t b = (uint32_t)(cell >> sb);
-      const uint32_t pos = atomicAdd(cur + b, 1u);
-      if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = (uint16_t)(cell & smask);
-      else atomicAdd(counts + cell, 1u);
-    });
+  const uint64_t cap = a.cap;
+  const uint32_t smask = (1u << sb) - 1u;
+  uint16_t* __restrict__ out = a.data + sl * cap;
+  uint32_t* __restrict__ counts = a.counts;
+  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
+    uint64_t r = 0;
+    uint32_t w0 = 0, w1 = 0;
+    if (share_of(a.reads, gid0 + (uint64_t)ch * blockDim.x, a.split, a.k, r, w0, w1))
+      walk_cells<L, false>(a, r, w0, w1, [&](uint64_t cell) {
+        const uint32_t b = (uint32_t)(cell >> sb);
+        const uint32_t pos = atomicAdd(cur + b, 1u);
+        if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = (uint16_t)(cell & smask);
+        else atomicAdd(counts + cell, 1u);
+      });
   }
 }
 
 template <int L, bool HLL>
 __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinArgs a) {
   extern __shared__ uint32_t sh[];
-  bin_scatter_body<L, HLL>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, sh);
+  bin_scatter_body<L, HLL>(a, blockIdx.x, sh);
 }
 
 template <int L, bool HLL>
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter_m(const BinArgs* 
                                                                  const uint32_t* __restrict__ blk, uint32_t n_iso) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  bin_scatter_body<L, HLL>(args[iso], (uint64_t)(blockIdx.x - blk[iso]) * blockDim.x + threadIdx.x, sh);
+  bin_scatter_body<L, HLL>(args[iso], blockIdx.x - blk[iso], sh);
 }
 
 __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uint32_t* lds) {
@@ -183,8 +190,7 @@ hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
   if (a.reads.n_reads) {
     const uint32_t L = (2 * a.k + 63) / 64;
-    const uint64_t threads = a.reads.n_reads * (uint64_t)a.split;
-    const unsigned grid = (unsigned)((threads + kScatterBlock - 1) / kScatterBlock);
+    const unsigned grid = (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks);
     const size_t lds = bin_scatter_lds(a.nbins);
     const bool h = a.hll.regs != nullptr;
 #define BS(LL)                                                                                              \

@@ -83,6 +83,7 @@ struct BinArgs {
   DevReads reads;
   uint32_t k = 0;
   uint32_t split = 1;
+  uint32_t chunks = 1;         // thread shares per scatter-block thread (blocks = threads / (block * chunks))
   uint32_t log2w = 0;
   uint64_t n_cells = 0;        // 4 * width
   uint32_t* counts = nullptr;  // exact u32 cells
@@ -98,7 +99,18 @@ struct BinArgs {
 };
 size_t bin_scatter_lds(uint32_t nbins);
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s);
-constexpr uint32_t kScatterBlockSize = 512;  // block size of the bin scatter kernel
+constexpr uint32_t kScatterBlockSize = 512;  // block size of the bin scatter kernels (passes 1 and 2)
+constexpr uint32_t kP2GateBlockSize = 256;   // block size of the pass-2 gate kernel
+// Thread shares per scatter-block thread: enough blocks to fill the chip a few
+// times over, each flushing its bin histogram once for more work.
+inline uint32_t scatter_chunks(uint64_t threads) {
+  const uint64_t c = threads / ((uint64_t)kScatterBlockSize * 2048u);
+  return c < 1 ? 1u : (c > 16 ? 16u : (uint32_t)c);
+}
+inline uint64_t scatter_blocks(uint64_t threads, uint32_t chunks) {
+  const uint64_t per = (uint64_t)kScatterBlockSize * chunks;
+  return (threads + per - 1) / per;
+}
 hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s);
 // m.blk counts bins: one block per bin.
 hipError_t launch_bin_count_multi(const MultiLaunch& m, uint32_t max_sb, hipStream_t s);
@@ -137,6 +149,7 @@ struct P2BinArgs {
   DevReads reads;
   uint32_t k = 0;
   uint32_t split = 1;            // pairs per thread must stay <= 63
+  uint32_t chunks = 1;           // thread shares per count/scatter-block thread
   const uint32_t* solid_bits = nullptr;
   uint32_t log2w = 0;
   uint32_t nbins = 1;
@@ -164,8 +177,9 @@ uint32_t p2_lds_cap(uint32_t kw);
 uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap);
 hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s);
-// gate and scatter: one block per kP2GateBlock / kP2ScatterBlock threads of
-// each isolate; bins: one block per bin.
+// gate: one block per kP2GateBlock threads of each isolate; scatter (the
+// count and scatter kernels): one block per kScatterBlockSize * chunks
+// threads; bins: one block per bin.
 hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
                                   uint32_t k, uint32_t max_nbins, uint32_t lds_cap, hipStream_t s);
 

@@ -10,9 +10,10 @@
 //     bytes: 2 MB at width 2^24) stay resident in every XCD's L2 for a whole
 //     launch; the four rows together do not fit one L2, and probing them all
 //     per window is bound by the misses.  The launch of the last row
-//     (k_p2_count) also counts the records per (bin, slice); a one-block scan
-//     (k_p2_scan) turns the counts into exact region starts, so no region
-//     can overflow whatever the skew of the k-mer spectrum.
+//     (k_p2_count, with the block layout of the scatter kernel) also counts
+//     the records per (bin, slice); a one-block scan (k_p2_scan) turns the
+//     counts into exact region starts, so no region can overflow whatever the
+//     skew of the k-mer spectrum.
 //   Phase A (k_p2_scatter): the first walk counts the workgroup's records per
 //     bin in LDS; one global atomic per bin reserves a run in the region of
 //     (bin, slice), slice = workgroup index mod 8.  Workgroups are handed to
@@ -33,8 +34,8 @@
 namespace fb {
 using namespace fbk;
 
-constexpr int kP2GateBlock = 256;
-constexpr int kP2ScatterBlock = 512;
+constexpr int kP2GateBlock = (int)kP2GateBlockSize;
+constexpr int kP2ScatterBlock = (int)kScatterBlockSize;
 constexpr int kP2BinBlock = 1024;
 
 __device__ __forceinline__ uint64_t slot_hash2(uint64_t fold) { return splitmix64(fold ^ 0x9e3779b97f4a7c15ull); }
@@ -205,49 +206,46 @@ __device__ __forceinline__ uint32_t p2_bin_of(const P2BinArgs& a, uint64_t fold)
 }
 
 // Row 3 of the gate, then the records of the final mask counted per bin for
-// the scatter workgroup that will hold these threads (slice = its index mod
-// kSlices).  A gate block lies inside one scatter block.
-static_assert(kP2ScatterBlock % kP2GateBlock == 0, "gate blocks nest in scatter blocks");
+// slice sl.  Block blk covers the thread shares of scatter block blk:
+// [blk * chunks * blockDim, (blk + 1) * chunks * blockDim).
 template <int L>
-__device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t gid, uint32_t sl, uint32_t* hist) {
+__device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, uint32_t* hist) {
   for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
   __syncthreads();
-  uint64_t r = 0;
-  uint32_t p0 = 0, p1 = 0;
-  uint64_t m = 0;
-  if (p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) {
-    m = a.gate_masks[gid];
+  const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
+  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
+    const uint64_t gid = gid0 + (uint64_t)ch * blockDim.x;
+    uint64_t r = 0;
+    uint32_t p0 = 0, p1 = 0;
+    if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
+    uint64_t m = a.gate_masks[gid];
     if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
     a.gate_masks[gid] = m;
+    if (m)
+      p2_walk<L>(a, r, p0, p1, m, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
+        atomicAdd(hist + p2_bin_of(a, fold), 1u);
+      });
   }
-  if (m)
-    p2_walk<L>(a, r, p0, p1, m, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
-      atomicAdd(hist + p2_bin_of(a, fold), 1u);
-    });
   __syncthreads();
+  const uint32_t sl = (uint32_t)(blk & (kSlices - 1u));
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
     const uint32_t c = hist[b];
     if (c) atomicAdd(a.runs + (uint64_t)b * kSlices + sl, c);
   }
 }
 
-__device__ __forceinline__ uint32_t slice_of_gate_block(uint32_t blk) {
-  return (uint32_t)(((uint64_t)blk * kP2GateBlock / kP2ScatterBlock) & (kSlices - 1u));
-}
-
 template <int L>
-__global__ __launch_bounds__(kP2GateBlock) void k_p2_count(P2BinArgs a) {
+__global__ __launch_bounds__(kP2ScatterBlock) void k_p2_count(P2BinArgs a) {
   extern __shared__ uint32_t sh[];
-  p2_count_body<L>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, slice_of_gate_block(blockIdx.x), sh);
+  p2_count_body<L>(a, blockIdx.x, sh);
 }
 
 template <int L>
-__global__ __launch_bounds__(kP2GateBlock) void k_p2_count_m(const P2BinArgs* __restrict__ args,
-                                                             const uint32_t* __restrict__ blk, uint32_t n_iso) {
+__global__ __launch_bounds__(kP2ScatterBlock) void k_p2_count_m(const P2BinArgs* __restrict__ args,
+                                                                const uint32_t* __restrict__ blk, uint32_t n_iso) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  const uint32_t lb = blockIdx.x - blk[iso];
-  p2_count_body<L>(args[iso], (uint64_t)lb * blockDim.x + threadIdx.x, slice_of_gate_block(lb), sh);
+  p2_count_body<L>(args[iso], blockIdx.x - blk[iso], sh);
 }
 
 // Exclusive scan of runs[0..n) in place, runs[n] = total; one block.
@@ -285,30 +283,44 @@ __global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan_m(const P2BinArgs* __r
   if (blockIdx.x < n_iso && a.runs) p2_scan_body(a.runs, a.nbins * kSlices);
 }
 
+// Block blk covers the same thread shares as count block blk and writes
+// into slice blk mod kSlices; workgroups go to the XCDs in turn, so the
+// slice is the XCD's.
 template <int L, int KW>
-__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t gid, uint32_t sl, uint32_t* sh) {
+__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk, uint32_t* sh) {
   uint32_t* hist = sh;
   uint32_t* cur = sh + a.nbins;
   for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
   __syncthreads();
-  uint64_t r = 0;
-  uint32_t p0 = 0, p1 = 0;
-  const bool valid = p2_share(a.reads, gid, a.split, a.k, r, p0, p1);
-  const uint64_t gates = valid ? a.gate_masks[gid] : 0ull;
-  if (gates)
-    p2_walk<L>(a, r, p0, p1, gates, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
-      atomicAdd(hist + p2_bin_of(a, fold), 1u);
-    });
+  const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
+  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
+    const uint64_t gid = gid0 + (uint64_t)ch * blockDim.x;
+    uint64_t r = 0;
+    uint32_t p0 = 0, p1 = 0;
+    if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
+    const uint64_t gates = a.gate_masks[gid];
+    if (gates)
+      p2_walk<L>(a, r, p0, p1, gates, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
+        atomicAdd(hist + p2_bin_of(a, fold), 1u);
+      });
+  }
   __syncthreads();
   // absolute start of this workgroup's run in the (bin, slice) region
+  const uint32_t sl = (uint32_t)(blk & (kSlices - 1u));
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
     const uint32_t c = hist[b];
     cur[b] = c ? atomicAdd(a.runs + (uint64_t)b * kSlices + sl, c) : 0u;
   }
   __syncthreads();
-  if (gates) {
-    const uint64_t cap = a.rec_cap;
-    const uint32_t rw = a.rw, k = a.k;
+  const uint64_t cap = a.rec_cap;
+  const uint32_t rw = a.rw, k = a.k;
+  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
+    const uint64_t gid = gid0 + (uint64_t)ch * blockDim.x;
+    uint64_t r = 0;
+    uint32_t p0 = 0, p1 = 0;
+    if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
+    const uint64_t gates = a.gate_masks[gid];
+    if (!gates) continue;
     p2_walk<L>(a, r, p0, p1, gates,
                [&](const Kmer<L>& key, uint64_t fold, bool left, uint32_t el, bool right, uint32_t er) {
                  const uint32_t pos = atomicAdd(cur + p2_bin_of(a, fold), 1u);
@@ -329,7 +341,7 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t gid
 template <int L, int KW>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter(P2BinArgs a) {
   extern __shared__ uint32_t sh[];
-  p2_scatter_body<L, KW>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x & (kSlices - 1u), sh);
+  p2_scatter_body<L, KW>(a, blockIdx.x, sh);
 }
 
 template <int L, int KW>
@@ -337,8 +349,7 @@ __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_m(const P2BinArg
                                                                   const uint32_t* __restrict__ blk, uint32_t n_iso) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  const uint32_t lb = blockIdx.x - blk[iso];
-  p2_scatter_body<L, KW>(args[iso], (uint64_t)lb * blockDim.x + threadIdx.x, lb & (kSlices - 1u), sh);
+  p2_scatter_body<L, KW>(args[iso], blockIdx.x - blk[iso], sh);
 }
 
 // LDS hash of a packed key (placement inside a bin; independent of the bin bits).
@@ -592,11 +603,12 @@ struct P2Launch {
       const unsigned ggrid = (unsigned)((threads + kP2GateBlock - 1) / kP2GateBlock);
       for (uint32_t row = 0; row < 3; ++row)
         hipLaunchKernelGGL((k_p2_gate<L>), dim3(ggrid), dim3(kP2GateBlock), 0, s, a, row);
-      hipLaunchKernelGGL((k_p2_count<L>), dim3(ggrid), dim3(kP2GateBlock), (size_t)a.nbins * 4, s, a);
+      hipLaunchKernelGGL((k_p2_count<L>), dim3(scatter_blocks(threads, a.chunks)), dim3(kP2ScatterBlock),
+                         (size_t)a.nbins * 4, s, a);
     }
     hipLaunchKernelGGL(k_p2_scan, dim3(1), dim3(kP2ScanBlock), 0, s, a.runs, a.nbins * kSlices);
     if (threads) {
-      const unsigned grid = (unsigned)((threads + kP2ScatterBlock - 1) / kP2ScatterBlock);
+      const unsigned grid = (unsigned)scatter_blocks(threads, a.chunks);
       hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
@@ -626,9 +638,10 @@ struct P2LaunchMulti {
       for (uint32_t row = 0; row < 3; ++row)
         hipLaunchKernelGGL((k_p2_gate_m<L>), dim3(gate.total_blocks), dim3(kP2GateBlock), 0, s, ga, gate.blk,
                            gate.n_iso, row);
-      hipLaunchKernelGGL((k_p2_count_m<L>), dim3(gate.total_blocks), dim3(kP2GateBlock), scat_lds / 2, s, ga,
-                         gate.blk, gate.n_iso);
     }
+    if (scatter.total_blocks)
+      hipLaunchKernelGGL((k_p2_count_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds / 2, s,
+                         static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso);
     if (gate.n_iso) hipLaunchKernelGGL(k_p2_scan_m, dim3(gate.n_iso), dim3(kP2ScanBlock), 0, s, ga, gate.n_iso);
     if (scatter.total_blocks)
       hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,

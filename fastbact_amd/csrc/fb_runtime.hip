@@ -641,7 +641,8 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   if (int r = bits.get(c, ((ncnt + 31) / 32) * 4)) return fail(r);
   BinArgs ba;
   if (binned) {
-    const uint64_t n_wg = (bank->n_reads * (uint64_t)split + kScatterBlockSize - 1) / kScatterBlockSize;
+    const uint64_t thr1 = bank->n_reads * (uint64_t)split;
+    const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1));
     ba.cap = bin_slice_cap(4.0 * (double)windows / (double)nbins, n_wg, 4);
     ba.sb = sb;
     ba.nbins = (uint32_t)nbins;
@@ -663,6 +664,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     ba.reads = bank->view();
     ba.k = k;
     ba.split = split;
+    ba.chunks = scatter_chunks(bank->n_reads * (uint64_t)split);
     ba.log2w = so->cms.log2w;
     ba.n_cells = ncnt;
     ba.counts = so->cms.counts.as<uint32_t>();
@@ -766,6 +768,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.reads = bank->view();
     pa.k = k;
     pa.split = split2;
+    pa.chunks = scatter_chunks(threads2);
     pa.solid_bits = bits.as<uint32_t>();
     pa.log2w = so->cms.log2w;
     pa.nbins = (uint32_t)nb2;
@@ -1185,7 +1188,8 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     it.sb = sb;
     it.nbins = (uint32_t)nbins;
     if (binned) {
-      const uint64_t n_wg = (it.bank->n_reads * (uint64_t)it.split + kScatterBlockSize - 1) / kScatterBlockSize;
+      const uint64_t thr1 = it.bank->n_reads * (uint64_t)it.split;
+      const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1));
       const uint64_t cap = bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg, 4);
       if (cap > it.bins_cap || !it.bdata.p) {
         if (int r = it.bdata.get(c, nbins * kBinSlices * cap * 2)) return r;
@@ -1227,6 +1231,7 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     bi.reads = rv;
     bi.k = k;
     bi.split = it.split;
+    bi.chunks = scatter_chunks(it.bank->n_reads * (uint64_t)it.split);
     bi.log2w = it.cms.log2w;
     bi.n_cells = ncells;
     bi.counts = it.cms.counts.as<uint32_t>();
@@ -1265,8 +1270,7 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     q.out_mask = it.mask.as<uint8_t>();
     q.scratch = it.p3.p;
     const uint64_t thr = it.bank->n_reads * (uint64_t)it.split;
-    blk1[i + 1] = blk1[i] + (uint32_t)((thr + (binned ? kScatterBlockSize : kPassBlock) - 1) /
-                                       (binned ? kScatterBlockSize : kPassBlock));
+    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(thr, bi.chunks) : (thr + kPassBlock - 1) / kPassBlock);
     blkc[i + 1] = blkc[i] + (binned ? it.nbins : 0u);
     blk2[i + 1] = blk2[i] + (uint32_t)((thr + kPassBlock - 1) / kPassBlock);
   }

@@ -5,10 +5,13 @@
 //   Phase A (k_bin_scatter): each workgroup walks its reads twice.  The first
 //     walk counts the workgroup's updates per bin in LDS (and feeds the HLL).
 //     One global atomic per bin then reserves a run in that bin's region,
-//     in the slice that belongs to the workgroup's XCD.  The second walk
-//     writes each update's 16-bit cell offset into the run.  An update that
-//     finds its slice full is added to the table directly with a global
-//     atomic, so any input (low-complexity reads included) stays exact.
+//     in the slice that belongs to the workgroup's XCD.  The second walk goes
+//     in rounds of kRound windows per thread: the round's cells are counted
+//     per bin, sorted by bin into an LDS stage (a counting sort), and written
+//     out in stage order, so a wave's store covers a few contiguous runs
+//     instead of 64 scattered lines.  An update that finds its slice full is
+//     added to the table directly with a global atomic, so any input
+//     (low-complexity reads included) stays exact.
 //   Phase B (k_bin_count): one workgroup per bin counts the offsets of the
 //     bin's slices with LDS atomics, adds them to the table (the bin's cells belong to this
 //     workgroup alone) and writes the pass-2 threshold bits of those cells.
@@ -21,6 +24,8 @@ using namespace fbk;
 
 constexpr int kScatterBlock = (int)kScatterBlockSize;
 constexpr int kCountBlock = 1024;
+constexpr int kRound = 8;                                     // windows per thread per staging round
+constexpr uint32_t kStage = 4u * kRound * kScatterBlockSize;  // staged cells per round
 static_assert(kSlices == kBinSlices, "slice count of device and host code");
 
 __device__ __forceinline__ bool share_of(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
@@ -76,12 +81,88 @@ __device__ __forceinline__ void walk_cells(const BinArgs& a, uint64_t r, uint32_
   }
 }
 
+// The windows of a thread's shares gid0, gid0 + blockDim, ... (chunks of
+// them) one at a time; a copy of the state replays the same windows.
+template <int L>
+struct CellWalker {
+  uint64_t gid;     // next share to open
+  uint32_t left;    // shares not yet opened
+  uint32_t w, w1;   // next window of the open share and its end; w == w1: none open
+  Kmer<L> fwd, rev;
+  BaseStream in;
+
+  __device__ __forceinline__ void init(uint64_t gid0, uint32_t chunks) {
+    gid = gid0;
+    left = chunks;
+    w = w1 = 0;
+  }
+  // Canonical fold of the next window; false once every share is done.
+  __device__ __forceinline__ bool next(const BinArgs& a, uint64_t& key) {
+    if (w >= w1) {
+      bool opened = false;
+      while (left && !opened) {
+        uint64_t r = 0;
+        uint32_t w0 = 0, e = 0;
+        opened = share_of(a.reads, gid, a.split, a.k, r, w0, e);
+        gid += blockDim.x;
+        --left;
+        if (opened) {
+          const uint64_t g = a.reads.offs[r] + w0;
+          const Kmer<L> win = window_bits<L>(a.reads.data, g, a.k);
+          fwd = fwd_from_window<L>(win, a.k);
+          rev = rc_from_window<L>(win, a.k);
+          in.init(a.reads.data, g + a.k);
+          w = w0;
+          w1 = e;
+        }
+      }
+      if (!opened) return false;
+    }
+    key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
+    if (++w < w1) kmer_roll<L>(fwd, rev, in.next(), a.k);
+    return true;
+  }
+};
+
+__device__ __forceinline__ uint64_t cell_of(const BinArgs& a, uint64_t key, int row) {
+  return ((uint64_t)row << a.log2w) | (splitmix64(key ^ cms_seed(row)) & ((1ull << a.log2w) - 1ull));
+}
+
+// Exclusive scan of v[0..n) in place; one block.
+__device__ __forceinline__ void block_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+  const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
+  const uint32_t hi = lo + per < n ? lo + per : n;
+  uint32_t sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += v[i];
+  uint32_t incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off);
+    if ((threadIdx.x & 63u) >= (uint32_t)off) incl += t;
+  }
+  if ((threadIdx.x & 63u) == 63u) s_wave[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  uint32_t run = incl - sum;
+  for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) run += s_wave[w];
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t t = v[i];
+    v[i] = run;
+    run += t;
+  }
+}
+
 // Block blk covers thread shares [blk * chunks * blockDim, (blk + 1) * chunks * blockDim).
+// LDS: gcur[nbins] (the block's next position per bin), rcur[nbins] (round
+// counts, then run ends), stage[kStage].
 template <int L, bool HLL>
 __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
-  uint32_t* hist = sh;
-  uint32_t* cur = sh + a.nbins;
-  for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
+  __shared__ uint32_t s_wave[kScatterBlock / 64];
+  const uint32_t nb = a.nbins;
+  uint32_t* gcur = sh;
+  uint32_t* rcur = sh + nb;
+  uint32_t* stage = sh + 2 * nb;
+  for (uint32_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) sh[i] = 0;
   __syncthreads();
   const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
   const uint32_t sb = a.sb;
@@ -89,29 +170,58 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
     uint64_t r = 0;
     uint32_t w0 = 0, w1 = 0;
     if (share_of(a.reads, gid0 + (uint64_t)ch * blockDim.x, a.split, a.k, r, w0, w1))
-      walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(hist + (cell >> sb), 1u); });
+      walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(gcur + (cell >> sb), 1u); });
   }
   __syncthreads();
   const uint32_t sl = xcc_slice();
-  for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
-    const uint32_t c = hist[b];
-    cur[b] = c ? atomicAdd(a.cursor + (uint64_t)b * kSlices + sl, c) : 0u;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    const uint32_t c = gcur[b];
+    gcur[b] = c ? atomicAdd(a.cursor + (uint64_t)b * kSlices + sl, c) : 0u;
   }
-  __syncthreads();
+  // (the first round's barrier orders these writes before their use)
   const uint64_t cap = a.cap;
   const uint32_t smask = (1u << sb) - 1u;
   uint16_t* __restrict__ out = a.data + sl * cap;
   uint32_t* __restrict__ counts = a.counts;
-  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
-    uint64_t r = 0;
-    uint32_t w0 = 0, w1 = 0;
-    if (share_of(a.reads, gid0 + (uint64_t)ch * blockDim.x, a.split, a.k, r, w0, w1))
-      walk_cells<L, false>(a, r, w0, w1, [&](uint64_t cell) {
-        const uint32_t b = (uint32_t)(cell >> sb);
-        const uint32_t pos = atomicAdd(cur + b, 1u);
-        if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = (uint16_t)(cell & smask);
-        else atomicAdd(counts + cell, 1u);
-      });
+  CellWalker<L> wk;
+  wk.init(gid0, a.chunks);
+  for (;;) {
+    const CellWalker<L> start = wk;
+    uint32_t n = 0;
+    for (; n < (uint32_t)kRound; ++n) {
+      uint64_t key;
+      if (!wk.next(a, key)) break;
+#pragma unroll
+      for (int row = 0; row < 4; ++row) atomicAdd(rcur + (cell_of(a, key, row) >> sb), 1u);
+    }
+    if (!__syncthreads_or(n)) break;
+    block_exclusive_scan(rcur, nb, s_wave);
+    __syncthreads();
+    CellWalker<L> re = start;
+    for (uint32_t j = 0; j < n; ++j) {
+      uint64_t key;
+      re.next(a, key);
+#pragma unroll
+      for (int row = 0; row < 4; ++row) {
+        const uint32_t cell = (uint32_t)cell_of(a, key, row);
+        stage[atomicAdd(rcur + (cell >> sb), 1u)] = cell;
+      }
+    }
+    __syncthreads();
+    // rcur[b] is now the end of bin b's run in the stage
+    const uint32_t total = rcur[nb - 1];
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+      const uint32_t cell = stage[i];
+      const uint32_t b = cell >> sb;
+      const uint32_t pos = gcur[b] + i - (b ? rcur[b - 1] : 0u);
+      if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = (uint16_t)(cell & smask);
+      else atomicAdd(counts + cell, 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gcur[b] += rcur[b] - (b ? rcur[b - 1] : 0u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) rcur[b] = 0;
+    __syncthreads();
   }
 }
 
@@ -184,10 +294,50 @@ __global__ __launch_bounds__(kCountBlock) void k_bin_count_m(const BinArgs* __re
   bin_count_body(args[iso], blockIdx.x - blk[iso], lds);
 }
 
-size_t bin_scatter_lds(uint32_t nbins) { return (size_t)nbins * 2 * sizeof(uint32_t); }
+size_t bin_scatter_lds(uint32_t nbins) { return ((size_t)nbins * 2 + kStage) * sizeof(uint32_t); }
+
+// Dynamic LDS beyond the 64 KB default for the scatter (bins + stage) and
+// count (2^sb counters) kernels; the limit less 1 KB of static LDS.
+static hipError_t pass1_lds_attrs() {
+  static hipError_t done = hipErrorNotReady;
+  if (done == hipErrorNotReady) {
+    const int bytes = (int)kP2LdsBytes - 1024;
+    const void* f[] = {reinterpret_cast<const void*>(&k_bin_scatter<1, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<1, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<2, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<2, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<3, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<3, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<4, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter<4, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<1, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<1, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<2, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<2, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<3, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<3, false>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<4, true>),
+                       reinterpret_cast<const void*>(&k_bin_scatter_m<4, false>),
+                       reinterpret_cast<const void*>(&k_bin_count),
+                       reinterpret_cast<const void*>(&k_bin_count_m)};
+    hipError_t e = hipSuccess;
+    for (const void* fn : f)
+      if (e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    done = e;
+  }
+  return done;
+}
+
+hipError_t check_bin_pass1(uint32_t nbins, uint32_t sb) {
+  if (hipError_t e = pass1_lds_attrs()) return e;
+  if (bin_scatter_lds(nbins) + 1024 > kP2LdsBytes || ((size_t)4 << sb) + 1024 > kP2LdsBytes)
+    return hipErrorInvalidConfiguration;
+  return hipSuccess;
+}
 
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
+  if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
   if (a.reads.n_reads) {
     const uint32_t L = (2 * a.k + 63) / 64;
     const unsigned grid = (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks);
@@ -212,6 +362,7 @@ hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
 
 hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s) {
   if (!m.total_blocks) return hipSuccess;
+  if (hipError_t e = check_bin_pass1(max_nbins, 6)) return e;  // count LDS checked by its launcher
   const uint32_t L = (2 * k + 63) / 64;
   const BinArgs* a = static_cast<const BinArgs*>(m.args);
   const size_t lds = bin_scatter_lds(max_nbins);
@@ -231,6 +382,7 @@ hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, 
 
 hipError_t launch_bin_count_multi(const MultiLaunch& m, uint32_t max_sb, hipStream_t s) {
   if (!m.total_blocks) return hipSuccess;
+  if (hipError_t e = check_bin_pass1(1, max_sb)) return e;
   hipLaunchKernelGGL(k_bin_count_m, dim3(m.total_blocks), dim3(kCountBlock), (size_t)(1u << max_sb) * 4, s,
                      static_cast<const BinArgs*>(m.args), m.blk, m.n_iso);
   return hipGetLastError();

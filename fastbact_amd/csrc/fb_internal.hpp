@@ -88,7 +88,7 @@ struct BinArgs {
   uint64_t n_cells = 0;        // 4 * width
   uint32_t* counts = nullptr;  // exact u32 cells
   DevHll hll;
-  uint32_t sb = 14;            // log2 cells per bin (<= 16)
+  uint32_t sb = 15;            // log2 cells per bin (<= 15: 4 << sb bytes of LDS counters)
   uint32_t nbins = 0;
   uint64_t cap = 0;            // offsets per (bin, XCD slice) region, multiple of 4
   uint16_t* data = nullptr;    // nbins * kBinSlices * cap cell offsets
@@ -100,6 +100,7 @@ struct BinArgs {
 size_t bin_scatter_lds(uint32_t nbins);
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s);
 constexpr uint32_t kScatterBlockSize = 512;  // block size of the bin scatter kernels (passes 1 and 2)
+constexpr uint32_t kBinLog2Cells = 15;       // pass-1 bin: 2^15 Count-Min cells (128 KB of LDS counters)
 constexpr uint32_t kP2GateBlockSize = 256;   // block size of the pass-2 gate kernel
 // Thread shares per scatter-block thread: enough blocks to fill the chip a few
 // times over, each flushing its bin histogram once for more work.

@@ -630,7 +630,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   // pass 1: LDS-binned Count-Min slices or direct global atomics
   const uint64_t ncnt = so->cms.width * 4;
   const uint64_t windows = bank_windows(bank, k);
-  uint32_t sb = 14;
+  uint32_t sb = kBinLog2Cells;
   while (sb > 6 && (1ull << sb) > ncnt) --sb;
   const uint64_t nbins = std::max<uint64_t>(1, ncnt >> sb);
   const bool binned = prm->placement == 2 || (prm->placement == 0 && nbins <= 8192 && windows >= (1u << 20));
@@ -1179,7 +1179,7 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     const uint64_t ncnt = it.cms.width * 4;
     it.windows = bank_windows(it.bank, k);
     it.split = b->prm.split ? b->prm.split : default_split(k, it.bank->max_len);
-    uint32_t sb = 14;
+    uint32_t sb = kBinLog2Cells;
     while (sb > 6 && (1ull << sb) > ncnt) --sb;
     const uint64_t nbins = std::max<uint64_t>(1, ncnt >> sb);
     const bool binned = b->prm.placement != 1 && nbins <= 8192;

@@ -22,10 +22,10 @@
 namespace fb {
 using namespace fbk;
 
-constexpr int kScatterBlock = (int)kScatterBlockSize;
+constexpr int kScatterBlock = (int)kP1BlockSize;
 constexpr int kCountBlock = 1024;
-constexpr int kRound = 8;                                     // windows per thread per staging round
-constexpr uint32_t kStage = 4u * kRound * kScatterBlockSize;  // staged cells per round
+constexpr int kRound = 7;                                     // windows per thread per staging round
+constexpr uint32_t kStage = 4u * kRound * kP1BlockSize;  // staged cells per round
 static_assert(kSlices == kBinSlices, "slice count of device and host code");
 
 __device__ __forceinline__ bool share_of(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
@@ -82,7 +82,7 @@ __device__ __forceinline__ void walk_cells(const BinArgs& a, uint64_t r, uint32_
 }
 
 // The windows of a thread's shares gid0, gid0 + blockDim, ... (chunks of
-// them) one at a time; a copy of the state replays the same windows.
+// them), one share at a time.
 template <int L>
 struct CellWalker {
   uint64_t gid;     // next share to open
@@ -96,31 +96,32 @@ struct CellWalker {
     left = chunks;
     w = w1 = 0;
   }
-  // Canonical fold of the next window; false once every share is done.
-  __device__ __forceinline__ bool next(const BinArgs& a, uint64_t& key) {
-    if (w >= w1) {
-      bool opened = false;
-      while (left && !opened) {
-        uint64_t r = 0;
-        uint32_t w0 = 0, e = 0;
-        opened = share_of(a.reads, gid, a.split, a.k, r, w0, e);
-        gid += blockDim.x;
-        --left;
-        if (opened) {
-          const uint64_t g = a.reads.offs[r] + w0;
-          const Kmer<L> win = window_bits<L>(a.reads.data, g, a.k);
-          fwd = fwd_from_window<L>(win, a.k);
-          rev = rc_from_window<L>(win, a.k);
-          in.init(a.reads.data, g + a.k);
-          w = w0;
-          w1 = e;
-        }
+  // Opens the next share with windows, if any.
+  __device__ __forceinline__ void open(const BinArgs& a) {
+    while (left) {
+      uint64_t r = 0;
+      uint32_t w0 = 0, e = 0;
+      const bool ok = share_of(a.reads, gid, a.split, a.k, r, w0, e);
+      gid += blockDim.x;
+      --left;
+      if (ok) {
+        const uint64_t g = a.reads.offs[r] + w0;
+        const Kmer<L> win = window_bits<L>(a.reads.data, g, a.k);
+        fwd = fwd_from_window<L>(win, a.k);
+        rev = rc_from_window<L>(win, a.k);
+        in.init(a.reads.data, g + a.k);
+        w = w0;
+        w1 = e;
+        return;
       }
-      if (!opened) return false;
     }
-    key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
+  }
+  __device__ __forceinline__ bool has() const { return w < w1; }
+  // Canonical fold of the next window of the open share (has() must hold).
+  __device__ __forceinline__ uint64_t take(const BinArgs& a) {
+    const uint64_t key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
     if (++w < w1) kmer_roll<L>(fwd, rev, in.next(), a.k);
-    return true;
+    return key;
   }
 };
 
@@ -186,25 +187,33 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
   CellWalker<L> wk;
   wk.init(gid0, a.chunks);
   for (;;) {
-    const CellWalker<L> start = wk;
+    // a round takes up to kRound windows of one share; its cells stay in registers
+    if (!wk.has()) wk.open(a);
+    uint32_t cells[4 * kRound];
     uint32_t n = 0;
-    for (; n < (uint32_t)kRound; ++n) {
-      uint64_t key;
-      if (!wk.next(a, key)) break;
 #pragma unroll
-      for (int row = 0; row < 4; ++row) atomicAdd(rcur + (cell_of(a, key, row) >> sb), 1u);
+    for (int j = 0; j < kRound; ++j) {
+      if (wk.has()) {
+        const uint64_t key = wk.take(a);
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+          cells[4 * j + row] = (uint32_t)cell_of(a, key, row);
+          atomicAdd(rcur + (cells[4 * j + row] >> sb), 1u);
+        }
+        n = j + 1;
+      }
     }
     if (!__syncthreads_or(n)) break;
     block_exclusive_scan(rcur, nb, s_wave);
     __syncthreads();
-    CellWalker<L> re = start;
-    for (uint32_t j = 0; j < n; ++j) {
-      uint64_t key;
-      re.next(a, key);
 #pragma unroll
-      for (int row = 0; row < 4; ++row) {
-        const uint32_t cell = (uint32_t)cell_of(a, key, row);
-        stage[atomicAdd(rcur + (cell >> sb), 1u)] = cell;
+    for (int j = 0; j < kRound; ++j) {
+      if ((uint32_t)j < n) {
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+          const uint32_t cell = cells[4 * j + row];
+          stage[atomicAdd(rcur + (cell >> sb), 1u)] = cell;
+        }
       }
     }
     __syncthreads();
@@ -340,7 +349,7 @@ hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
   if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
   if (a.reads.n_reads) {
     const uint32_t L = (2 * a.k + 63) / 64;
-    const unsigned grid = (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks);
+    const unsigned grid = (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks, kP1BlockSize);
     const size_t lds = bin_scatter_lds(a.nbins);
     const bool h = a.hll.regs != nullptr;
 #define BS(LL)                                                                                              \

@@ -99,17 +99,18 @@ struct BinArgs {
 };
 size_t bin_scatter_lds(uint32_t nbins);
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s);
-constexpr uint32_t kScatterBlockSize = 512;  // block size of the bin scatter kernels (passes 1 and 2)
+constexpr uint32_t kScatterBlockSize = 512;  // block size of the pass-2 count and scatter kernels
+constexpr uint32_t kP1BlockSize = 512;       // block size of the pass-1 bin scatter kernel
 constexpr uint32_t kBinLog2Cells = 15;       // pass-1 bin: 2^15 Count-Min cells (128 KB of LDS counters)
 constexpr uint32_t kP2GateBlockSize = 256;   // block size of the pass-2 gate kernel
 // Thread shares per scatter-block thread: enough blocks to fill the chip a few
 // times over, each flushing its bin histogram once for more work.
-inline uint32_t scatter_chunks(uint64_t threads) {
-  const uint64_t c = threads / ((uint64_t)kScatterBlockSize * 2048u);
+inline uint32_t scatter_chunks(uint64_t threads, uint32_t block = kScatterBlockSize) {
+  const uint64_t c = threads / ((uint64_t)block * 2048u);
   return c < 1 ? 1u : (c > 16 ? 16u : (uint32_t)c);
 }
-inline uint64_t scatter_blocks(uint64_t threads, uint32_t chunks) {
-  const uint64_t per = (uint64_t)kScatterBlockSize * chunks;
+inline uint64_t scatter_blocks(uint64_t threads, uint32_t chunks, uint32_t block = kScatterBlockSize) {
+  const uint64_t per = (uint64_t)block * chunks;
   return (threads + per - 1) / per;
 }
 hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s);
@@ -158,9 +159,13 @@ struct P2BinArgs {
   uint64_t* gate_masks = nullptr;  // n_reads * split, gate bit of window p0+i at bit i
   uint32_t rw = 1;               // 64-bit words per record: ceil((2k + 8) / 64)
   // nbins * kBinSlices + 1 run counters, zeroed: record counts per (bin,
-  // slice), then (after the scan) region starts, then (after phase A) region
-  // ends; the last entry holds the record total.
+  // slice), then (after the scan) region starts; the last entry holds the
+  // record total.
   uint32_t* runs = nullptr;
+  // n_blocks * nbins: each scatter block's record count per bin, then (after
+  // the column scan) the absolute start of its run.
+  uint32_t* blk_hist = nullptr;
+  uint32_t n_blocks = 0;         // scatter blocks: scatter_blocks(n_reads * split, chunks)
   uint64_t* recs = nullptr;      // records of rw words, one per window at most
   uint64_t rec_cap = 0;          // records the buffer holds
   uint32_t* status = nullptr;    // [0] record buffer overflow (never for exact runs), zeroed

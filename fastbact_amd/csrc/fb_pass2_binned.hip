@@ -14,13 +14,15 @@
 //     the records per (bin, slice); a one-block scan (k_p2_scan) turns the
 //     counts into exact region starts, so no region can overflow whatever the
 //     skew of the k-mer spectrum.
-//   Phase A (k_p2_scatter): the first walk counts the workgroup's records per
-//     bin in LDS; one global atomic per bin reserves a run in the region of
-//     (bin, slice), slice = workgroup index mod 8.  Workgroups are handed to
-//     the 8 XCDs in turn, so each region is filled from one XCD and its partly
-//     written lines stay in one L2 until they are whole.  The second walk
-//     writes one record per window that has any: the canonical k-mer's 2k
-//     bits and the edge byte, packed densely into rw 64-bit words.
+//   Phase A (k_p2_scatter): block b writes into the region of (bin, slice),
+//     slice = b mod 8.  Workgroups are handed to the 8 XCDs in turn, so each
+//     region is filled from one XCD and its partly written lines stay in one
+//     L2 until they are whole.  The count kernel stores every block's
+//     per-bin record counts; a column scan over the blocks of each slice
+//     (k_p2_block_offsets) turns them into each block's run starts, so the
+//     scatter walks its windows once, writing one record per window that has
+//     any: the canonical k-mer's 2k bits and the edge byte, packed densely
+//     into rw 64-bit words.
 //   Phase B (k_p2_bins): one workgroup per bin builds the bin's table in LDS
 //     (write-once keys claimed word by word with compare-and-swap, as the
 //     global table does), then emits the solid k-mers (pass 3) directly.  A
@@ -228,8 +230,10 @@ __device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, 
   }
   __syncthreads();
   const uint32_t sl = (uint32_t)(blk & (kSlices - 1u));
+  uint32_t* __restrict__ row = a.blk_hist + blk * a.nbins;
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
     const uint32_t c = hist[b];
+    row[b] = c;
     if (c) atomicAdd(a.runs + (uint64_t)b * kSlices + sl, c);
   }
 }
@@ -283,35 +287,61 @@ __global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan_m(const P2BinArgs* __r
   if (blockIdx.x < n_iso && a.runs) p2_scan_body(a.runs, a.nbins * kSlices);
 }
 
+// Thread i = sl * nbins + b: the blocks of slice sl (sl, sl + 8, ...) take
+// consecutive runs of bin b's region in block order.
+constexpr int kP2OffBlock = 256;
+
+// LDS requested by the scatter kernel: its bin cursors need nbins * 4 bytes,
+// but it asks for 96 KB so that one block runs per CU.  Every running block
+// keeps a partly written line open in each bin; with 2-4 blocks per CU those
+// open lines outgrow the caches and the record stores slow down by a third
+// (measured at 8192 bins: 6.6 vs 10.8 ms for 433M windows).
+constexpr size_t kP2ScatterLds = 96u * 1024u;
+inline size_t p2_scatter_lds(uint32_t nbins) {
+  return (size_t)nbins * 4 > kP2ScatterLds ? (size_t)nbins * 4 : kP2ScatterLds;
+}
+__device__ __forceinline__ void p2_offsets_body(const P2BinArgs& a, uint64_t i) {
+  if (i >= (uint64_t)a.nbins * kSlices) return;
+  const uint32_t sl = (uint32_t)(i / a.nbins), b = (uint32_t)(i % a.nbins);
+  uint32_t run = a.runs[(uint64_t)b * kSlices + sl];
+  uint32_t* __restrict__ h = a.blk_hist + b;
+  const uint64_t stride = (uint64_t)kSlices * a.nbins;
+  uint64_t blk = sl;
+  for (; blk + 3 * kSlices < a.n_blocks; blk += 4 * kSlices) {
+    const uint32_t t0 = h[blk * a.nbins], t1 = h[blk * a.nbins + stride];
+    const uint32_t t2 = h[blk * a.nbins + 2 * stride], t3 = h[blk * a.nbins + 3 * stride];
+    h[blk * a.nbins] = run;
+    h[blk * a.nbins + stride] = run + t0;
+    h[blk * a.nbins + 2 * stride] = run + t0 + t1;
+    h[blk * a.nbins + 3 * stride] = run + t0 + t1 + t2;
+    run += t0 + t1 + t2 + t3;
+  }
+  for (; blk < a.n_blocks; blk += kSlices) {
+    const uint32_t t = h[blk * a.nbins];
+    h[blk * a.nbins] = run;
+    run += t;
+  }
+}
+
+__global__ __launch_bounds__(kP2OffBlock) void k_p2_block_offsets(P2BinArgs a) {
+  p2_offsets_body(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// blockIdx.y = isolate
+__global__ __launch_bounds__(kP2OffBlock) void k_p2_block_offsets_m(const P2BinArgs* __restrict__ args) {
+  p2_offsets_body(args[blockIdx.y], (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Block blk covers the same thread shares as count block blk and writes
 // into slice blk mod kSlices; workgroups go to the XCDs in turn, so the
 // slice is the XCD's.
 template <int L, int KW>
-__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk, uint32_t* sh) {
-  uint32_t* hist = sh;
-  uint32_t* cur = sh + a.nbins;
-  for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
+__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk, uint32_t* cur) {
+  // absolute start of this block's run in every bin's (bin, slice) region
+  const uint32_t* __restrict__ row = a.blk_hist + blk * a.nbins;
+  for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) cur[b] = row[b];
   __syncthreads();
   const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
-  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
-    const uint64_t gid = gid0 + (uint64_t)ch * blockDim.x;
-    uint64_t r = 0;
-    uint32_t p0 = 0, p1 = 0;
-    if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
-    const uint64_t gates = a.gate_masks[gid];
-    if (gates)
-      p2_walk<L>(a, r, p0, p1, gates, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
-        atomicAdd(hist + p2_bin_of(a, fold), 1u);
-      });
-  }
-  __syncthreads();
-  // absolute start of this workgroup's run in the (bin, slice) region
-  const uint32_t sl = (uint32_t)(blk & (kSlices - 1u));
-  for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) {
-    const uint32_t c = hist[b];
-    cur[b] = c ? atomicAdd(a.runs + (uint64_t)b * kSlices + sl, c) : 0u;
-  }
-  __syncthreads();
   const uint64_t cap = a.rec_cap;
   const uint32_t rw = a.rw, k = a.k;
   for (uint32_t ch = 0; ch < a.chunks; ++ch) {
@@ -398,10 +428,10 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   for (uint32_t i = threadIdx.x; i < 5 * cap; i += blockDim.x) cov[i] = 0u;
   if (threadIdx.x < 2) s_stat[threadIdx.x] = 0ull;
   __syncthreads();
-  // the bin's kSlices regions are adjacent: [end of the previous bin, end of
-  // this bin's last slice)
-  const uint64_t e0 = bin ? a.runs[(uint64_t)bin * kSlices - 1] : 0ull;
-  uint64_t e1 = a.runs[(uint64_t)bin * kSlices + kSlices - 1];
+  // the bin's kSlices regions are adjacent: [start of its first slice, start
+  // of the next bin's first slice)
+  const uint64_t e0 = a.runs[(uint64_t)bin * kSlices];
+  uint64_t e1 = a.runs[(uint64_t)(bin + 1) * kSlices];
   if (e1 > a.rec_cap) e1 = a.rec_cap;
   const uint32_t n = e1 > e0 ? (uint32_t)(e1 - e0) : 0u;
   const uint32_t rw = a.rw;
@@ -568,7 +598,7 @@ static uint32_t lds_budget() {
   return g_lds_bytes;
 }
 
-uint32_t p2_max_bins() { return (lds_budget() - 1024u) / 8u >= 16384u ? 16384u : 8192u; }
+uint32_t p2_max_bins() { return (lds_budget() - 1024u) / 4u >= 16384u ? 16384u : 8192u; }
 
 uint32_t p2_lds_cap(uint32_t kw) {
   // keys + cov + 8 u16 edges per entry within the workgroup's LDS
@@ -608,6 +638,8 @@ struct P2Launch {
     }
     hipLaunchKernelGGL(k_p2_scan, dim3(1), dim3(kP2ScanBlock), 0, s, a.runs, a.nbins * kSlices);
     if (threads) {
+      const unsigned ogrid = (unsigned)(((uint64_t)a.nbins * kSlices + kP2OffBlock - 1) / kP2OffBlock);
+      hipLaunchKernelGGL(k_p2_block_offsets, dim3(ogrid), dim3(kP2OffBlock), 0, s, a);
       const unsigned grid = (unsigned)scatter_blocks(threads, a.chunks);
       hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, a);
       hipError_t e = hipGetLastError();
@@ -620,16 +652,17 @@ struct P2Launch {
 
 hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s) {
   const uint32_t KW = (a.k + 30) / 31;
-  const size_t scat_lds = (size_t)a.nbins * 8;
+  const size_t scat_lds = p2_scatter_lds(a.nbins);
   const size_t lds = p2_lds_bytes(KW, a.lds_cap);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (a.rw != (2 * a.k + 8 + 63) / 64) return hipErrorInvalidValue;
+  if (a.n_blocks != scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks)) return hipErrorInvalidValue;
   return dispatch_lkw(a.k, P2Launch{a, a.reads.n_reads * (uint64_t)a.split, scat_lds, lds, s});
 }
 
 struct P2LaunchMulti {
   const MultiLaunch &gate, &scatter, &bins;
-  size_t scat_lds, lds;
+  size_t count_lds, scat_lds, lds;
   hipStream_t s;
   template <int L, int KW>
   hipError_t operator()() {
@@ -640,9 +673,13 @@ struct P2LaunchMulti {
                            gate.n_iso, row);
     }
     if (scatter.total_blocks)
-      hipLaunchKernelGGL((k_p2_count_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds / 2, s,
+      hipLaunchKernelGGL((k_p2_count_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), count_lds, s,
                          static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso);
-    if (gate.n_iso) hipLaunchKernelGGL(k_p2_scan_m, dim3(gate.n_iso), dim3(kP2ScanBlock), 0, s, ga, gate.n_iso);
+    if (gate.n_iso) {
+      hipLaunchKernelGGL(k_p2_scan_m, dim3(gate.n_iso), dim3(kP2ScanBlock), 0, s, ga, gate.n_iso);
+      const unsigned ogrid = (unsigned)((count_lds / 4 * kSlices + kP2OffBlock - 1) / kP2OffBlock);
+      hipLaunchKernelGGL(k_p2_block_offsets_m, dim3(ogrid, gate.n_iso), dim3(kP2OffBlock), 0, s, ga);
+    }
     if (scatter.total_blocks)
       hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,
                          static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso);
@@ -657,10 +694,10 @@ struct P2LaunchMulti {
 hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
                                   uint32_t k, uint32_t max_nbins, uint32_t lds_cap, hipStream_t s) {
   const uint32_t KW = (k + 30) / 31;
-  const size_t scat_lds = (size_t)max_nbins * 8;
+  const size_t scat_lds = p2_scatter_lds(max_nbins);
   const size_t lds = p2_lds_bytes(KW, lds_cap);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
-  return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, scat_lds, lds, s});
+  return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, (size_t)max_nbins * 4, scat_lds, lds, s});
 }
 
 }  // namespace fb

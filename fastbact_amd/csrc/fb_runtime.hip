@@ -642,7 +642,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   BinArgs ba;
   if (binned) {
     const uint64_t thr1 = bank->n_reads * (uint64_t)split;
-    const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1));
+    const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1, kP1BlockSize), kP1BlockSize);
     ba.cap = bin_slice_cap(4.0 * (double)windows / (double)nbins, n_wg, 4);
     ba.sb = sb;
     ba.nbins = (uint32_t)nbins;
@@ -664,7 +664,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     ba.reads = bank->view();
     ba.k = k;
     ba.split = split;
-    ba.chunks = scatter_chunks(bank->n_reads * (uint64_t)split);
+    ba.chunks = scatter_chunks(bank->n_reads * (uint64_t)split, kP1BlockSize);
     ba.log2w = so->cms.log2w;
     ba.n_cells = ncnt;
     ba.counts = so->cms.counts.as<uint32_t>();
@@ -747,8 +747,11 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
     const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est_keys / 4)));
     const uint64_t out_cap = nb2 * lds_cap + spill_cap;
-    DBuf recs, gm, runs, st2, p3s;
+    const uint32_t chunks2 = scatter_chunks(threads2);
+    const uint64_t n_blocks2 = scatter_blocks(threads2, chunks2);
+    DBuf recs, gm, runs, bh, st2, p3s;
     int r = 0;
+    if (!r) r = bh.get(c, std::max<uint64_t>(n_blocks2 * nb2, 1) * 4);
     if (!r) r = recs.get(c, rec_cap * rw * 8);
     if (!r) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
     if (!r) r = runs.get(c, (nb2 * kBinSlices + 1) * 4);
@@ -768,7 +771,9 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.reads = bank->view();
     pa.k = k;
     pa.split = split2;
-    pa.chunks = scatter_chunks(threads2);
+    pa.chunks = chunks2;
+    pa.blk_hist = bh.as<uint32_t>();
+    pa.n_blocks = (uint32_t)n_blocks2;
     pa.solid_bits = bits.as<uint32_t>();
     pa.log2w = so->cms.log2w;
     pa.nbins = (uint32_t)nb2;
@@ -1189,7 +1194,7 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     it.nbins = (uint32_t)nbins;
     if (binned) {
       const uint64_t thr1 = it.bank->n_reads * (uint64_t)it.split;
-      const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1));
+      const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1, kP1BlockSize), kP1BlockSize);
       const uint64_t cap = bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg, 4);
       if (cap > it.bins_cap || !it.bdata.p) {
         if (int r = it.bdata.get(c, nbins * kBinSlices * cap * 2)) return r;
@@ -1231,7 +1236,7 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     bi.reads = rv;
     bi.k = k;
     bi.split = it.split;
-    bi.chunks = scatter_chunks(it.bank->n_reads * (uint64_t)it.split);
+    bi.chunks = scatter_chunks(it.bank->n_reads * (uint64_t)it.split, kP1BlockSize);
     bi.log2w = it.cms.log2w;
     bi.n_cells = ncells;
     bi.counts = it.cms.counts.as<uint32_t>();
@@ -1270,7 +1275,8 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     q.out_mask = it.mask.as<uint8_t>();
     q.scratch = it.p3.p;
     const uint64_t thr = it.bank->n_reads * (uint64_t)it.split;
-    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(thr, bi.chunks) : (thr + kPassBlock - 1) / kPassBlock);
+    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(thr, bi.chunks, kP1BlockSize)
+                                              : (thr + kPassBlock - 1) / kPassBlock);
     blkc[i + 1] = blkc[i] + (binned ? it.nbins : 0u);
     blk2[i + 1] = blk2[i] + (uint32_t)((thr + kPassBlock - 1) / kPassBlock);
   }

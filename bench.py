@@ -46,19 +46,41 @@ def ref_words(k: int) -> int:
     return 1 if k <= 31 else (2 if k <= 63 else 4)
 
 
-def pass_bytes(k: int, windows: int, records: int, n_reads: int, bases: int) -> tuple[float, float]:
-    """Algorithmic HBM bytes of the pass-1 and pass-2 kernels.
+def kernel_bytes(k: int, st: dict, bases: int, solid: int) -> dict:
+    """Algorithmic bytes one launch of each kernel must move for pass k.
 
-    Per read: packed bases L/4 plus 12 bytes of offset and length.  Per
-    window, pass 1: the 4-row CMS read-modify-write (2 x 4 x c, c = counter
-    bytes) and the HLL register read-modify-write (2).  Pass 2: the 4-row CMS
-    query (4 x c) per window and, per adjacency record, the key plus its
-    cov/edge u16 read-modify-write (K_B + 4)."""
+    stream = packed bases (L/4) plus 12 bytes of offset and length per read.
+    k_bin_scatter: stream + per window the 4 u16 cell offsets it emits (8) and
+      the HLL register read-modify-write (2).
+    k_bin_count: the offsets read back (8 per window), the u32 counter
+      read-modify-write of every cell (8) and the gate bit per cell (1/8).
+    k_p2_scatter: stream + per record its rw 64-bit words.
+    k_p2_bins: the records read back plus the solid set written
+      (8 * ref_words + 3 per solid k-mer)."""
+    rw = (2 * k + 8 + 63) // 64
+    stream = bases / 4.0 + 12.0 * st["reads_used"]
+    cells = 4.0 * WIDTH
+    recs = st["pass2_records"] * rw * 8.0
+    return {"bin_scatter": stream + st["windows"] * 10.0,
+            "bin_count": st["windows"] * 8.0 + cells * (8.0 + 1.0 / 8.0),
+            "p2_scatter": stream + recs,
+            "p2_bins": recs + solid * (8.0 * ref_words(k) + 3.0)}
+
+
+KERNEL_NAMES = {"bin_scatter": "k_bin_scatter (pass-1 Count-Min bin scatter + HLL)",
+                "bin_count": "k_bin_count (pass-1 LDS counts + gate bits)",
+                "p2_scatter": "k_p2_scatter (pass-2 adjacency record scatter)",
+                "p2_bins": "k_p2_bins (pass-2 LDS tables + pass 3)"}
+
+
+def pass_model_bytes(k: int, st: dict, bases: int, solid: int) -> float:
+    """Whole-pass algorithmic bytes: per read L/4 + 12; per window the 4-row
+    CMS update (8c) and query (4c) and the HLL RMW (2); per adjacency record
+    the key and its cov/edge RMW (K_B + 4); per solid k-mer K_B + 3."""
     c = 2 if COUNTER_BITS == 16 else 1
-    stream = bases / 4.0 + 12.0 * n_reads
-    p1 = stream + windows * (8 * c + 2)
-    p2 = stream + windows * 4 * c + records * (8 * ref_words(k) + 4)
-    return p1, p2
+    kb = 8 * ref_words(k)
+    stream = bases / 4.0 + 12.0 * st["reads_used"]
+    return stream + st["windows"] * (12 * c + 2) + st["records"] * (kb + 4) + solid * (kb + 3)
 
 
 def make_isolate():
@@ -77,10 +99,32 @@ def run_step(ctx, dev, F, placement=0):
                                             cms_width=WIDTH, counter_bits=COUNTER_BITS, hll_p=HLL_P,
                                             placement=placement))
     ctx.synchronize()
-    info = [(h.timings(), h.stats(), h.count()) for h in handles]
+    info = [(h.timings(), h.stats(), h.count(), h.kernel_timings()) for h in handles]
     for h in handles:
         h.close()
     return info
+
+
+def copy_calibration(ctx, mib: int) -> None:
+    """One k_copy_calibration launch: reads and writes mib MiB."""
+    ctx.check(ctx.lib.fb_diag_copy(ctx.h, mib << 20), "fb_diag_copy")
+    log(f"copy calibration: {mib} MiB device-to-device (k_copy_calibration)")
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_full_isolate_k21_55_77.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration), or None when no summary covers it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        v = t["kernels"][kernel]["hbm_bytes_per_launch"]
+        return float(v), os.path.relpath(TRAFFIC_FILE, ROOT)
+    except (OSError, KeyError, ValueError, TypeError):
+        return None, None
 
 
 def cpu_baseline(bank, pairs: int, threads: int) -> dict:
@@ -117,6 +161,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--placement", type=int, default=0, help="pass-1 sketch placement: 0 auto, 1 global, 2 LDS bins")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the steps, copy a buffer of this many MiB device to device once (a known-size "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the k-mer spectrum path is measured per GPU); got --gpus {args.gpus}",
@@ -149,36 +196,42 @@ def main(argv=None) -> int:
     value = bases_step * args.steps / total_s / 1e9
 
     # per-kernel event times and algorithmic bytes over the timed steps
-    n_reads = dev.total_reads()
-    p1_ms = p2_ms = p3_ms = 0.0
-    p1_bytes = p2_bytes = 0.0
     per_pass = []
-    for (k, _), j in zip(PASSES, range(len(PASSES))):
-        t1 = sum(info[j][0][0] for info in infos)
-        t2 = sum(info[j][0][1] for info in infos)
-        t3 = sum(info[j][0][2] for info in infos)
-        st = infos[-1][j][1]
-        b1, b2 = pass_bytes(k, st["windows"], st["records"], n_reads, dev.bases_for_k(k))
-        p1_ms += t1
-        p2_ms += t2
-        p3_ms += t3
-        p1_bytes += b1 * args.steps
-        p2_bytes += b2 * args.steps
-        per_pass.append({"k": k, "pass1_ms": t1 / args.steps, "pass2_ms": t2 / args.steps,
-                         "pass3_sort_ms": t3 / args.steps, "windows": st["windows"], "records": st["records"],
-                         "solid_kmers": infos[-1][j][2], "pass2_route": st["pass2_route"],
-                         "lds_bins": st["lds_bins"], "est_keys": st["est_keys"],
-                         "pass1_GBps": b1 / (t1 / args.steps) / 1e6, "pass2_GBps": b2 / (t2 / args.steps) / 1e6})
-    if p1_ms >= p2_ms:
-        kname, kms, kbytes = "k_pass1 (CMS update + HLL)", p1_ms, p1_bytes
-    else:
-        kname, kms, kbytes = "k_pass2 (CMS gate + solid table)", p2_ms, p2_bytes
+    k_ms = {n: 0.0 for n in KERNEL_NAMES}
+    k_bytes = {n: 0.0 for n in KERNEL_NAMES}
+    model_bytes = 0.0
+    for j, (k, _) in enumerate(PASSES):
+        st, solid = infos[-1][j][1], infos[-1][j][2]
+        bases = dev.bases_for_k(k)
+        kb = kernel_bytes(k, st, bases, solid)
+        kt = {n: sum(info[j][3][n] for info in infos) / args.steps for n in KERNEL_NAMES}
+        for n in KERNEL_NAMES:
+            k_ms[n] += kt[n] * args.steps
+            k_bytes[n] += kb[n] * args.steps
+        model_bytes += pass_model_bytes(k, st, bases, solid) * args.steps
+        per_pass.append({"k": k,
+                         "pass1_ms": sum(info[j][0][0] for info in infos) / args.steps,
+                         "pass2_ms": sum(info[j][0][1] for info in infos) / args.steps,
+                         "pass3_sort_ms": sum(info[j][0][2] for info in infos) / args.steps,
+                         "kernel_ms": {n: round(v, 3) for n, v in kt.items()},
+                         "windows": st["windows"], "pass2_records": st["pass2_records"],
+                         "raw_cov_sum": st["records"], "solid_kmers": solid, "pass2_route": st["pass2_route"],
+                         "lds_bins": st["lds_bins"], "est_keys": st["est_keys"]})
+    dom = max(KERNEL_NAMES, key=lambda n: k_ms[n])
     launches = args.steps * len(PASSES)
-    achieved = kbytes / launches / (kms / launches / 1e3) / 1e9
+    avg_ms = k_ms[dom] / launches
+    achieved = k_bytes[dom] / (k_ms[dom] / 1e3) / 1e9 if k_ms[dom] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
-                "avg_launch_ms": round(kms / launches, 3),
-                "algorithmic_bytes_per_launch": round(kbytes / launches)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": KERNEL_NAMES[dom],
+                "avg_launch_ms": round(avg_ms, 3), "launches": launches,
+                "algorithmic_bytes_per_launch": round(k_bytes[dom] / launches)}
+    model_gbs = model_bytes / total_s / 1e9
+    pass_model = {"achieved": round(model_gbs, 1), "unit": "GB/s", "frac": round(model_gbs / HBM_PEAK_GBS, 4),
+                  "bytes_per_step": round(model_bytes / args.steps)}
+
+    if args.copy_calibration_mib:
+        copy_calibration(ctx, args.copy_calibration_mib)
+    roofline["traffic"], roofline["traffic_source"] = measured_traffic(dom)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -211,6 +264,7 @@ def main(argv=None) -> int:
             "passes": per_pass,
         },
         "roofline": roofline,
+        "pass_model_roofline": pass_model,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -157,6 +157,8 @@ SIGNATURES = [
     ("fb_solid_cms_download", C.c_int, [vp, vp, C.c_uint64]),
     ("fb_solid_hll_registers", C.c_int, [vp, u8p]),
     ("fb_solid_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("fb_solid_kernel_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("fb_diag_copy", C.c_int, [vp, C.c_uint64]),
     ("fb_solid_stats", C.c_int, [vp, u64p]),
     ("fb_batch_create", C.c_int, [vp, C.POINTER(vp), u64p, C.c_uint32, C.POINTER(BuildParams), C.POINTER(vp)]),
     ("fb_batch_destroy", None, [vp]),

@@ -344,7 +344,7 @@ hipError_t check_bin_pass1(uint32_t nbins, uint32_t sb) {
   return hipSuccess;
 }
 
-hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
+hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
   if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
   if (a.reads.n_reads) {
@@ -365,6 +365,8 @@ hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  if (mid)
+    if (hipError_t e = hipEventRecord(mid, s)) return e;
   hipLaunchKernelGGL(k_bin_count, dim3(a.nbins), dim3(kCountBlock), (size_t)(1u << a.sb) * 4, s, a);
   return hipGetLastError();
 }

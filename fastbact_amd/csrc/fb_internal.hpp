@@ -98,7 +98,8 @@ struct BinArgs {
   uint32_t cap_value = 65535;  // counter saturation (15 or 65535)
 };
 size_t bin_scatter_lds(uint32_t nbins);
-hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s);
+// mid (optional) is recorded between the scatter and count kernels.
+hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid = nullptr);
 constexpr uint32_t kScatterBlockSize = 512;  // block size of the pass-2 count and scatter kernels
 constexpr uint32_t kP1BlockSize = 512;       // block size of the pass-1 bin scatter kernel
 constexpr uint32_t kBinLog2Cells = 15;       // pass-1 bin: 2^15 Count-Min cells (128 KB of LDS counters)
@@ -128,6 +129,7 @@ hipError_t launch_pack_u8(const uint32_t* regs, uint64_t n, uint8_t* out, hipStr
 // Table initialisation (all key words empty, counters zero).
 hipError_t launch_table_clear(const DevSolidTable& t, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_copy_calibration(const void* src, void* dst, uint64_t n16, hipStream_t s);
 // Pass 2 (build.rs:140-289).
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
 hipError_t launch_pass2_multi(const MultiLaunch& m, uint32_t k, hipStream_t s);
@@ -182,7 +184,10 @@ struct P2BinArgs {
 uint32_t p2_lds_cap(uint32_t kw);
 uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap);
-hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s);
+// gated (optional) is recorded after the gate, count, scan and offset kernels,
+// scattered after the scatter kernel.
+hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated = nullptr,
+                            hipEvent_t scattered = nullptr);
 // gate: one block per kP2GateBlock threads of each isolate; scatter (the
 // count and scatter kernels): one block per kScatterBlockSize * chunks
 // threads; bins: one block per bin.

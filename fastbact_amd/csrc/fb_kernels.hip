@@ -240,6 +240,22 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Plain copy of n 16-byte words: reads and writes 16 n bytes, nothing else.
+__global__ __launch_bounds__(kBlock) void k_copy_calibration(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                             uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+hipError_t launch_copy_calibration(const void* src, void* dst, uint64_t n16, hipStream_t s) {
+  uint64_t g = (n16 + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g == 0) g = 1;
+  hipLaunchKernelGGL(k_copy_calibration, dim3((unsigned)g), dim3(kBlock), 0, s, static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), n16);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kBlock) void k_table_clear(uint64_t* __restrict__ slots, uint64_t cap,
                                                         uint32_t stride, uint32_t kw) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -627,6 +627,7 @@ struct P2Launch {
   uint64_t threads;
   size_t scat_lds, lds;
   hipStream_t s;
+  hipEvent_t gated, scattered;
   template <int L, int KW>
   hipError_t operator()() {
     if (threads) {
@@ -640,24 +641,30 @@ struct P2Launch {
     if (threads) {
       const unsigned ogrid = (unsigned)(((uint64_t)a.nbins * kSlices + kP2OffBlock - 1) / kP2OffBlock);
       hipLaunchKernelGGL(k_p2_block_offsets, dim3(ogrid), dim3(kP2OffBlock), 0, s, a);
+    }
+    if (gated)
+      if (hipError_t e = hipEventRecord(gated, s)) return e;
+    if (threads) {
       const unsigned grid = (unsigned)scatter_blocks(threads, a.chunks);
       hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, a);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
     }
+    if (scattered)
+      if (hipError_t e = hipEventRecord(scattered, s)) return e;
     hipLaunchKernelGGL((k_p2_bins<L, KW>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, a);
     return hipGetLastError();
   }
 };
 
-hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s) {
+hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated, hipEvent_t scattered) {
   const uint32_t KW = (a.k + 30) / 31;
   const size_t scat_lds = p2_scatter_lds(a.nbins);
   const size_t lds = p2_lds_bytes(KW, a.lds_cap);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (a.rw != (2 * a.k + 8 + 63) / 64) return hipErrorInvalidValue;
   if (a.n_blocks != scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks)) return hipErrorInvalidValue;
-  return dispatch_lkw(a.k, P2Launch{a, a.reads.n_reads * (uint64_t)a.split, scat_lds, lds, s});
+  return dispatch_lkw(a.k, P2Launch{a, a.reads.n_reads * (uint64_t)a.split, scat_lds, lds, s, gated, scattered});
 }
 
 struct P2LaunchMulti {

@@ -36,7 +36,7 @@ struct fb_ctx {
   uint64_t held = 0;    // bytes allocated from HIP (in use + pooled)
   std::string err;
   std::vector<PoolBuf> pool;  // free buffers
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};
 };
 
 static int set_err(fb_ctx* c, int code, const char* fmt, ...) {
@@ -186,6 +186,22 @@ extern "C" int fb_ctx_synchronize(fb_ctx* c) {
 }
 
 extern "C" uint64_t fb_ctx_device_bytes(const fb_ctx* c) { return c ? c->held : 0; }
+
+extern "C" int fb_diag_copy(fb_ctx* c, uint64_t bytes) {
+  if (!c) return FB_ERR_INVALID;
+  if (int r = ctx_current(c)) return r;
+  const uint64_t n16 = (bytes + 15) / 16;
+  if (!n16) return FB_OK;
+  DBuf src, dst;
+  if (int r = src.get(c, n16 * 16)) return r;
+  if (int r = dst.get(c, n16 * 16)) return r;
+  hipStream_t s = c->stream;
+  HIP_TRY(c, launch_fill_u32(src.as<uint32_t>(), n16 * 4, 0x5a5a5a5au, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  HIP_TRY(c, launch_copy_calibration(src.p, dst.p, n16, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return FB_OK;
+}
 
 // ---------------------------------------------------------------------------
 // ReadBank
@@ -542,9 +558,11 @@ struct fb_solid {
   fb_hll hll;
   bool has_hll = false;
   float ms[4] = {0, 0, 0, 0};
+  // pass-1 scatter, pass-1 count, pass-2 gate+count, pass-2 scatter, pass-2 bin tables
+  float kms[5] = {0, 0, 0, 0, 0};
   // windows, read records, claimed keys, table slots, pass-2 route, LDS bins,
-  // keys per LDS bin table, estimated keys
-  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // keys per LDS bin table, estimated keys, LDS-route records, reads >= k
+  uint64_t stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
@@ -674,7 +692,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     ba.bits = bits.as<uint32_t>();
     ba.t_solid = t_solid;
     ba.cap_value = so->cms.cap();
-    HIP_TRY(c, launch_bin_pass1(ba, s));
+    HIP_TRY(c, launch_bin_pass1(ba, s, c->ev[8]));
   } else {
     Pass1Args p1{};
     p1.reads = bank->view();
@@ -794,13 +812,14 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.out_cov = so->cov.as<uint16_t>();
     pa.out_mask = so->mask.as<uint8_t>();
     HIP_TRY(c, hipEventRecord(c->ev[2], s));
-    HIP_TRY(c, launch_p2_binned(pa, s));
+    HIP_TRY(c, launch_p2_binned(pa, s, c->ev[9], c->ev[10]));
     HIP_TRY(c, hipEventRecord(c->ev[3], s));
     HIP_TRY(c, hipEventRecord(c->ev[4], s));
     HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
                             so->mask.as<uint8_t>(), p3s.p, s, pa.out_count));
-    uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0};
+    uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0}, n_rec = 0;
     HIP_TRY(c, hipMemcpyAsync(hb, st2.p, 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&n_rec, runs.as<uint32_t>() + nb2 * kBinSlices, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(sp, p3s.p, 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(ts, tab.status, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
@@ -814,6 +833,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
       claimed = bins_keys + ts[1];
       cap = spill_cap;
       route = 1;
+      so->stats[8] = n_rec;
       done = true;
     } else {
       route = hb[0] ? 2 : 3;
@@ -875,6 +895,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   so->stats[2] = claimed;
   so->stats[3] = cap;
   so->stats[4] = route;
+  for (size_t len = k; len < bank->len_hist.size(); ++len) so->stats[9] += bank->len_hist[len] / len;
   tbuf.release();
   so->n = n_out;
   if (prm->sort && n_out > 1) {
@@ -891,6 +912,15 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   (void)hipEventElapsedTime(&so->ms[1], c->ev[2], c->ev[3]);
   (void)hipEventElapsedTime(&so->ms[2], c->ev[4], c->ev[5]);
   (void)hipEventElapsedTime(&so->ms[3], c->ev[0], c->ev[5]);
+  if (binned) {
+    (void)hipEventElapsedTime(&so->kms[0], c->ev[0], c->ev[8]);
+    (void)hipEventElapsedTime(&so->kms[1], c->ev[8], c->ev[1]);
+  }
+  if (route == 1) {
+    (void)hipEventElapsedTime(&so->kms[2], c->ev[2], c->ev[9]);
+    (void)hipEventElapsedTime(&so->kms[3], c->ev[9], c->ev[10]);
+    (void)hipEventElapsedTime(&so->kms[4], c->ev[10], c->ev[3]);
+  }
   *out = so;
   return FB_OK;
 }
@@ -926,7 +956,13 @@ extern "C" int fb_solid_timings(const fb_solid* s, float out[4]) {
   memcpy(out, s->ms, sizeof s->ms);
   return FB_OK;
 }
-extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[8]) {
+
+extern "C" int fb_solid_kernel_timings(const fb_solid* s, float out[5]) {
+  if (!s || !out) return FB_ERR_INVALID;
+  memcpy(out, s->kms, sizeof s->kms);
+  return FB_OK;
+}
+extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[10]) {
   if (!s || !out) return FB_ERR_INVALID;
   memcpy(out, s->stats, sizeof s->stats);
   return FB_OK;

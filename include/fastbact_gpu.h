@@ -65,6 +65,10 @@ const char* fb_last_error(const fb_ctx* ctx);
 int fb_ctx_synchronize(fb_ctx* ctx);
 /* Device bytes currently held by the context's objects and caches. */
 uint64_t fb_ctx_device_bytes(const fb_ctx* ctx);
+/* Diagnostics, no counterpart in the reference: one plain device-to-device
+ * copy of `bytes` (kernel k_copy_calibration reads and writes bytes rounded
+ * up to 16), for scaling hardware traffic counters against a known size. */
+int fb_diag_copy(fb_ctx* ctx, uint64_t bytes);
 /* Library version string. */
 const char* fb_version(void);
 
@@ -153,12 +157,20 @@ int fb_solid_hll_registers(const fb_solid* s, uint8_t* out);
  * milliseconds: [0] pass 1 kernel (CMS + HLL), [1] pass 2 kernel, [2] pass 3
  * + sort, [3] whole build. */
 int fb_solid_timings(const fb_solid* s, float out[4]);
+/* Kernel-level breakdown of the LDS-binned route, milliseconds (0 where the
+ * build took another route): [0] pass-1 bin scatter, [1] pass-1 bin count
+ * and gate bits, [2] pass-2 gate rows, record count and run offsets, [3]
+ * pass-2 record scatter, [4] pass-2 bin tables with the fused pass 3.  No
+ * counterpart in the reference (diagnostics of this implementation). */
+int fb_solid_kernel_timings(const fb_solid* s, float out[5]);
 /* Work counts of the build: [0] k-mer windows of pass 1, [1] pass-2 records
  * (sum of raw cov), [2] keys in the solid table, [3] table slots, [4] pass-2
  * route (0 global table, 1 LDS bins, 2 LDS bins abandoned for a full record
- * region, 3 abandoned for a full spill table), [5] pass-2 LDS bins, [6] keys
- * per LDS bin table, [7] keys estimated from the pass-1 gate bits. */
-int fb_solid_stats(const fb_solid* s, uint64_t out[8]);
+ * buffer, 3 abandoned for a full spill table), [5] pass-2 LDS bins, [6] keys
+ * per LDS bin table, [7] keys estimated from the pass-1 gate bits, [8]
+ * records the LDS route wrote (one per window with an edge, 0 on another
+ * route), [9] reads of length >= k. */
+int fb_solid_stats(const fb_solid* s, uint64_t out[10]);
 
 /* ---- Batched form: many isolates per launch --------------------------------
  * The reference builds one isolate per process run (main.rs:262).  A batch

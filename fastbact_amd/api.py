@@ -317,12 +317,12 @@ class SolidHandle(_Handle):
         return {n: float(x) for n, x in zip(names, t)}
 
     def stats(self) -> dict:
-        t = (C.c_uint64 * 10)()
+        t = (C.c_uint64 * 11)()
         self.ctx.check(self.ctx.lib.fb_solid_stats(self.h, t), "fb_solid_stats")
         return {"windows": int(t[0]), "records": int(t[1]), "keys": int(t[2]), "slots": int(t[3]),
                 "pass2_route": ("global", "lds", "lds_region_full", "lds_spill_full")[int(t[4])],
                 "lds_bins": int(t[5]), "lds_bin_keys": int(t[6]), "est_keys": int(t[7]),
-                "pass2_records": int(t[8]), "reads_used": int(t[9])}
+                "pass2_records": int(t[8]), "reads_used": int(t[9]), "pass2_rounds": int(t[10])}
 
     def download(self, k: int, with_cms: bool = True, counter_bits: int = 4, hll_p: int = 0) -> SolidSet:
         n, nw = self.count(), self.key_words()

@@ -156,13 +156,18 @@ struct P2BinArgs {
   uint32_t chunks = 1;           // thread shares per count/scatter-block thread
   const uint32_t* solid_bits = nullptr;
   uint32_t log2w = 0;
-  uint32_t nbins = 1;
+  uint32_t nbins = 1;             // bins per round
   uint32_t log2_nbins = 0;
+  // Keys are split over 2^log2_rounds rounds by the top bits of their bin
+  // hash; each round counts, scatters and tables only its own records, so the
+  // distinct keys of one round fit the LDS tables of nbins bins.
+  uint32_t log2_rounds = 0;
+  uint32_t round = 0;            // round of this launch sequence
   uint64_t* gate_masks = nullptr;  // n_reads * split, gate bit of window p0+i at bit i
   uint32_t rw = 1;               // 64-bit words per record: ceil((2k + 8) / 64)
-  // nbins * kBinSlices + 1 run counters, zeroed: record counts per (bin,
-  // slice), then (after the scan) region starts; the last entry holds the
-  // record total.
+  // nbins * kBinSlices + 1 run counters, zeroed before every round: record
+  // counts per (bin, slice), then (after the scan) region starts; the last
+  // entry holds the round's record total.
   uint32_t* runs = nullptr;
   // n_blocks * nbins: each scatter block's record count per bin, then (after
   // the column scan) the absolute start of its run.
@@ -170,7 +175,7 @@ struct P2BinArgs {
   uint32_t n_blocks = 0;         // scatter blocks: scatter_blocks(n_reads * split, chunks)
   uint64_t* recs = nullptr;      // records of rw words, one per window at most
   uint64_t rec_cap = 0;          // records the buffer holds
-  uint32_t* status = nullptr;    // [0] record buffer overflow (never for exact runs), zeroed
+  uint32_t* status = nullptr;    // zeroed: [0] record buffer overflow (never for exact runs), [1] records of all rounds
   uint32_t lds_cap = 0;          // LDS table entries
   uint32_t min_trans_cov = 0;
   DevSolidTable table;           // keys that find the LDS table full
@@ -190,9 +195,14 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated 
                             hipEvent_t scattered = nullptr);
 // gate: one block per kP2GateBlock threads of each isolate; scatter (the
 // count and scatter kernels): one block per kScatterBlockSize * chunks
-// threads; bins: one block per bin.
+// threads; bins: one block per bin of a round.  runs_fill clears every
+// isolate's run counters before each of the max_rounds rounds (an isolate
+// with fewer rounds skips the later ones).
+struct FillRange;
 hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
-                                  uint32_t k, uint32_t max_nbins, uint32_t lds_cap, hipStream_t s);
+                                  const FillRange* runs_fill, uint32_t n_fill, uint32_t k, uint32_t max_nbins,
+                                  uint32_t max_rounds, uint32_t lds_cap, hipStream_t s);
+constexpr uint32_t kP2MaxLog2Rounds = 6;  // at most 64 pass-2 rounds
 
 // Multi-table forms for the batched pass (blockIdx.y = isolate).
 struct P3Args {

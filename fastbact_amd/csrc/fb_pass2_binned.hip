@@ -203,15 +203,22 @@ __device__ __forceinline__ void p2_walk(const P2BinArgs& a, uint64_t r, uint32_t
   }
 }
 
+// Bin of a key within its round: the top log2_rounds bits of the bin hash
+// pick the round, the next log2_nbins bits the bin.
 __device__ __forceinline__ uint32_t p2_bin_of(const P2BinArgs& a, uint64_t fold) {
-  return a.log2_nbins ? (uint32_t)(slot_hash2(fold) >> (64u - a.log2_nbins)) : 0u;
+  return a.log2_nbins ? (uint32_t)((slot_hash2(fold) << a.log2_rounds) >> (64u - a.log2_nbins)) : 0u;
+}
+__device__ __forceinline__ bool p2_in_round(const P2BinArgs& a, uint64_t fold, uint32_t round) {
+  return a.log2_rounds == 0 || (uint32_t)(slot_hash2(fold) >> (64u - a.log2_rounds)) == round;
 }
 
-// Row 3 of the gate, then the records of the final mask counted per bin for
-// slice sl.  Block blk covers the thread shares of scatter block blk:
+// Round 0 applies row 3 of the gate and stores the final mask; every round
+// then counts its records of the final mask per bin for slice sl.  Block blk
+// covers the thread shares of scatter block blk:
 // [blk * chunks * blockDim, (blk + 1) * chunks * blockDim).
 template <int L>
-__device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, uint32_t* hist) {
+__device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, uint32_t* hist, uint32_t round) {
+  if ((round >> a.log2_rounds) != 0u) return;  // this isolate has fewer rounds
   for (uint32_t i = threadIdx.x; i < a.nbins; i += blockDim.x) hist[i] = 0;
   __syncthreads();
   const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
@@ -221,11 +228,13 @@ __device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, 
     uint32_t p0 = 0, p1 = 0;
     if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
     uint64_t m = a.gate_masks[gid];
-    if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
-    a.gate_masks[gid] = m;
+    if (round == 0) {
+      if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
+      a.gate_masks[gid] = m;
+    }
     if (m)
       p2_walk<L>(a, r, p0, p1, m, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
-        atomicAdd(hist + p2_bin_of(a, fold), 1u);
+        if (p2_in_round(a, fold, round)) atomicAdd(hist + p2_bin_of(a, fold), 1u);
       });
   }
   __syncthreads();
@@ -241,19 +250,21 @@ __device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, 
 template <int L>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_count(P2BinArgs a) {
   extern __shared__ uint32_t sh[];
-  p2_count_body<L>(a, blockIdx.x, sh);
+  p2_count_body<L>(a, blockIdx.x, sh, a.round);
 }
 
 template <int L>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_count_m(const P2BinArgs* __restrict__ args,
-                                                                const uint32_t* __restrict__ blk, uint32_t n_iso) {
+                                                                const uint32_t* __restrict__ blk, uint32_t n_iso,
+                                                                uint32_t round) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  p2_count_body<L>(args[iso], blockIdx.x - blk[iso], sh);
+  p2_count_body<L>(args[iso], blockIdx.x - blk[iso], sh, round);
 }
 
-// Exclusive scan of runs[0..n) in place, runs[n] = total; one block.
-__device__ __forceinline__ void p2_scan_body(uint32_t* runs, uint32_t n) {
+// Exclusive scan of runs[0..n) in place, runs[n] = total (also added to
+// *acc, the records of all rounds, when acc is set); one block.
+__device__ __forceinline__ void p2_scan_body(uint32_t* runs, uint32_t n, uint32_t* acc) {
   __shared__ uint32_t s_wave[16];
   const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
   const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
@@ -276,15 +287,20 @@ __device__ __forceinline__ void p2_scan_body(uint32_t* runs, uint32_t n) {
     runs[i] = run;
     run += v;
   }
-  if (threadIdx.x == blockDim.x - 1) runs[n] = run;
+  if (threadIdx.x == blockDim.x - 1) {
+    runs[n] = run;
+    if (acc) atomicAdd(acc, run);
+  }
 }
 
 constexpr int kP2ScanBlock = 1024;
-__global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan(uint32_t* runs, uint32_t n) { p2_scan_body(runs, n); }
+__global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan(uint32_t* runs, uint32_t n, uint32_t* acc) {
+  p2_scan_body(runs, n, acc);
+}
 
 __global__ __launch_bounds__(kP2ScanBlock) void k_p2_scan_m(const P2BinArgs* __restrict__ args, uint32_t n_iso) {
   const P2BinArgs& a = args[blockIdx.x];
-  if (blockIdx.x < n_iso && a.runs) p2_scan_body(a.runs, a.nbins * kSlices);
+  if (blockIdx.x < n_iso && a.runs) p2_scan_body(a.runs, a.nbins * kSlices, a.status ? a.status + 1 : nullptr);
 }
 
 // Thread i = sl * nbins + b: the blocks of slice sl (sl, sl + 8, ...) take
@@ -336,7 +352,8 @@ __global__ __launch_bounds__(kP2OffBlock) void k_p2_block_offsets_m(const P2BinA
 // into slice blk mod kSlices; workgroups go to the XCDs in turn, so the
 // slice is the XCD's.
 template <int L, int KW>
-__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk, uint32_t* cur) {
+__device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk, uint32_t* cur, uint32_t round) {
+  if ((round >> a.log2_rounds) != 0u) return;
   // absolute start of this block's run in every bin's (bin, slice) region
   const uint32_t* __restrict__ row = a.blk_hist + blk * a.nbins;
   for (uint32_t b = threadIdx.x; b < a.nbins; b += blockDim.x) cur[b] = row[b];
@@ -353,6 +370,7 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk
     if (!gates) continue;
     p2_walk<L>(a, r, p0, p1, gates,
                [&](const Kmer<L>& key, uint64_t fold, bool left, uint32_t el, bool right, uint32_t er) {
+                 if (!p2_in_round(a, fold, round)) return;
                  const uint32_t pos = atomicAdd(cur + p2_bin_of(a, fold), 1u);
                  if (pos >= cap) {
                    a.status[0] = 1u;  // runs disagree with the counts: the host falls back
@@ -371,15 +389,16 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk
 template <int L, int KW>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter(P2BinArgs a) {
   extern __shared__ uint32_t sh[];
-  p2_scatter_body<L, KW>(a, blockIdx.x, sh);
+  p2_scatter_body<L, KW>(a, blockIdx.x, sh, a.round);
 }
 
 template <int L, int KW>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_m(const P2BinArgs* __restrict__ args,
-                                                                  const uint32_t* __restrict__ blk, uint32_t n_iso) {
+                                                                  const uint32_t* __restrict__ blk, uint32_t n_iso,
+                                                                  uint32_t round) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  p2_scatter_body<L, KW>(args[iso], blockIdx.x - blk[iso], sh);
+  p2_scatter_body<L, KW>(args[iso], blockIdx.x - blk[iso], sh, round);
 }
 
 // LDS hash of a packed key (placement inside a bin; independent of the bin bits).
@@ -416,7 +435,8 @@ __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, 
 }
 
 template <int L, int KW>
-__device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, unsigned char* smem) {
+__device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, unsigned char* smem, uint32_t round) {
+  if ((round >> a.log2_rounds) != 0u) return;  // this isolate has fewer rounds
   const uint32_t cap = a.lds_cap;
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(smem);      // KW * cap
   uint32_t* cov = reinterpret_cast<uint32_t*>(keys + (uint64_t)KW * cap);       // cap
@@ -548,15 +568,16 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
 template <int L, int KW>
 __global__ __launch_bounds__(kP2BinBlock) void k_p2_bins(P2BinArgs a) {
   extern __shared__ unsigned char smem[];
-  p2_bins_body<L, KW>(a, blockIdx.x, smem);
+  p2_bins_body<L, KW>(a, blockIdx.x, smem, a.round);
 }
 
 template <int L, int KW>
 __global__ __launch_bounds__(kP2BinBlock) void k_p2_bins_m(const P2BinArgs* __restrict__ args,
-                                                           const uint32_t* __restrict__ blk, uint32_t n_iso) {
+                                                           const uint32_t* __restrict__ blk, uint32_t n_iso,
+                                                           uint32_t round) {
   extern __shared__ unsigned char smem[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  p2_bins_body<L, KW>(args[iso], blockIdx.x - blk[iso], smem);
+  p2_bins_body<L, KW>(args[iso], blockIdx.x - blk[iso], smem, round);
 }
 
 // LDS the bin kernels may use: the full 160 KB once the attribute is granted,
@@ -634,26 +655,35 @@ struct P2Launch {
       const unsigned ggrid = (unsigned)((threads + kP2GateBlock - 1) / kP2GateBlock);
       for (uint32_t row = 0; row < 3; ++row)
         hipLaunchKernelGGL((k_p2_gate<L>), dim3(ggrid), dim3(kP2GateBlock), 0, s, a, row);
-      hipLaunchKernelGGL((k_p2_count<L>), dim3(scatter_blocks(threads, a.chunks)), dim3(kP2ScatterBlock),
-                         (size_t)a.nbins * 4, s, a);
     }
-    hipLaunchKernelGGL(k_p2_scan, dim3(1), dim3(kP2ScanBlock), 0, s, a.runs, a.nbins * kSlices);
-    if (threads) {
-      const unsigned ogrid = (unsigned)(((uint64_t)a.nbins * kSlices + kP2OffBlock - 1) / kP2OffBlock);
-      hipLaunchKernelGGL(k_p2_block_offsets, dim3(ogrid), dim3(kP2OffBlock), 0, s, a);
+    const uint32_t rounds = 1u << a.log2_rounds;
+    for (uint32_t round = 0; round < rounds; ++round) {
+      P2BinArgs ar = a;
+      ar.round = round;
+      if (hipError_t e = launch_fill_u32(a.runs, (uint64_t)a.nbins * kSlices + 1, 0u, s)) return e;
+      if (threads)
+        hipLaunchKernelGGL((k_p2_count<L>), dim3(scatter_blocks(threads, a.chunks)), dim3(kP2ScatterBlock),
+                           (size_t)a.nbins * 4, s, ar);
+      hipLaunchKernelGGL(k_p2_scan, dim3(1), dim3(kP2ScanBlock), 0, s, a.runs, a.nbins * kSlices,
+                         a.status ? a.status + 1 : nullptr);
+      if (threads) {
+        const unsigned ogrid = (unsigned)(((uint64_t)a.nbins * kSlices + kP2OffBlock - 1) / kP2OffBlock);
+        hipLaunchKernelGGL(k_p2_block_offsets, dim3(ogrid), dim3(kP2OffBlock), 0, s, ar);
+      }
+      if (gated && round == 0)
+        if (hipError_t e = hipEventRecord(gated, s)) return e;
+      if (threads) {
+        const unsigned grid = (unsigned)scatter_blocks(threads, a.chunks);
+        hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, ar);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+      }
+      if (scattered && round == 0)
+        if (hipError_t e = hipEventRecord(scattered, s)) return e;
+      hipLaunchKernelGGL((k_p2_bins<L, KW>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, ar);
+      if (hipError_t e = hipGetLastError()) return e;
     }
-    if (gated)
-      if (hipError_t e = hipEventRecord(gated, s)) return e;
-    if (threads) {
-      const unsigned grid = (unsigned)scatter_blocks(threads, a.chunks);
-      hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, a);
-      hipError_t e = hipGetLastError();
-      if (e != hipSuccess) return e;
-    }
-    if (scattered)
-      if (hipError_t e = hipEventRecord(scattered, s)) return e;
-    hipLaunchKernelGGL((k_p2_bins<L, KW>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, a);
-    return hipGetLastError();
+    return hipSuccess;
   }
 };
 
@@ -664,11 +694,14 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated,
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (a.rw != (2 * a.k + 8 + 63) / 64) return hipErrorInvalidValue;
   if (a.n_blocks != scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks)) return hipErrorInvalidValue;
+  if (a.log2_rounds > kP2MaxLog2Rounds || a.nbins != (1u << a.log2_nbins)) return hipErrorInvalidValue;
   return dispatch_lkw(a.k, P2Launch{a, a.reads.n_reads * (uint64_t)a.split, scat_lds, lds, s, gated, scattered});
 }
 
 struct P2LaunchMulti {
   const MultiLaunch &gate, &scatter, &bins;
+  const FillRange* runs_fill;
+  uint32_t n_fill, max_rounds;
   size_t count_lds, scat_lds, lds;
   hipStream_t s;
   template <int L, int KW>
@@ -679,32 +712,39 @@ struct P2LaunchMulti {
         hipLaunchKernelGGL((k_p2_gate_m<L>), dim3(gate.total_blocks), dim3(kP2GateBlock), 0, s, ga, gate.blk,
                            gate.n_iso, row);
     }
-    if (scatter.total_blocks)
-      hipLaunchKernelGGL((k_p2_count_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), count_lds, s,
-                         static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso);
-    if (gate.n_iso) {
-      hipLaunchKernelGGL(k_p2_scan_m, dim3(gate.n_iso), dim3(kP2ScanBlock), 0, s, ga, gate.n_iso);
-      const unsigned ogrid = (unsigned)((count_lds / 4 * kSlices + kP2OffBlock - 1) / kP2OffBlock);
-      hipLaunchKernelGGL(k_p2_block_offsets_m, dim3(ogrid, gate.n_iso), dim3(kP2OffBlock), 0, s, ga);
+    for (uint32_t round = 0; round < max_rounds; ++round) {
+      if (hipError_t e = launch_fill_ranges(runs_fill, n_fill, s)) return e;
+      if (scatter.total_blocks)
+        hipLaunchKernelGGL((k_p2_count_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), count_lds, s,
+                           static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso, round);
+      if (gate.n_iso) {
+        hipLaunchKernelGGL(k_p2_scan_m, dim3(gate.n_iso), dim3(kP2ScanBlock), 0, s, ga, gate.n_iso);
+        const unsigned ogrid = (unsigned)((count_lds / 4 * kSlices + kP2OffBlock - 1) / kP2OffBlock);
+        hipLaunchKernelGGL(k_p2_block_offsets_m, dim3(ogrid, gate.n_iso), dim3(kP2OffBlock), 0, s, ga);
+      }
+      if (scatter.total_blocks)
+        hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,
+                           static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso, round);
+      if (hipError_t e = hipGetLastError()) return e;
+      if (bins.total_blocks)
+        hipLaunchKernelGGL((k_p2_bins_m<L, KW>), dim3(bins.total_blocks), dim3(kP2BinBlock), lds, s,
+                           static_cast<const P2BinArgs*>(bins.args), bins.blk, bins.n_iso, round);
+      if (hipError_t e = hipGetLastError()) return e;
     }
-    if (scatter.total_blocks)
-      hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,
-                         static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !bins.total_blocks) return e;
-    hipLaunchKernelGGL((k_p2_bins_m<L, KW>), dim3(bins.total_blocks), dim3(kP2BinBlock), lds, s,
-                       static_cast<const P2BinArgs*>(bins.args), bins.blk, bins.n_iso);
-    return hipGetLastError();
+    return hipSuccess;
   }
 };
 
 hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
-                                  uint32_t k, uint32_t max_nbins, uint32_t lds_cap, hipStream_t s) {
+                                  const FillRange* runs_fill, uint32_t n_fill, uint32_t k, uint32_t max_nbins,
+                                  uint32_t max_rounds, uint32_t lds_cap, hipStream_t s) {
   const uint32_t KW = (k + 30) / 31;
   const size_t scat_lds = p2_scatter_lds(max_nbins);
   const size_t lds = p2_lds_bytes(KW, lds_cap);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
-  return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, (size_t)max_nbins * 4, scat_lds, lds, s});
+  if (max_rounds == 0 || max_rounds > (1u << kP2MaxLog2Rounds)) return hipErrorInvalidValue;
+  return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, runs_fill, n_fill, max_rounds, (size_t)max_nbins * 4,
+                                       scat_lds, lds, s});
 }
 
 }  // namespace fb

@@ -561,8 +561,9 @@ struct fb_solid {
   // pass-1 scatter, pass-1 count, pass-2 gate+count, pass-2 scatter, pass-2 bin tables
   float kms[5] = {0, 0, 0, 0, 0};
   // windows, read records, claimed keys, table slots, pass-2 route, LDS bins,
-  // keys per LDS bin table, estimated keys, LDS-route records, reads >= k
-  uint64_t stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // keys per LDS bin table, estimated keys, LDS-route records, reads >= k,
+  // pass-2 rounds
+  uint64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
@@ -744,13 +745,20 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   // LDS-privatised pass 2+3 (reads only; prior contigs use the global table)
   const uint32_t kw = tab.kw;
   const uint32_t lds_cap = p2_lds_cap(kw);
+  // bins over all rounds: the estimated keys at half the LDS table capacity
   uint64_t nb2 = 1;
   while ((double)nb2 * 0.5 * (double)lds_cap < n_est_keys) nb2 <<= 1;
+  const uint64_t max_bins = (uint64_t)p2_max_bins() << kP2MaxLog2Rounds;
   if (prm->bins2) {
-    if (prm->bins2 > p2_max_bins() || (prm->bins2 & (prm->bins2 - 1)))
-      return fail(set_err(c, FB_ERR_INVALID, "bins2 must be a power of two <= %u (got %u)", p2_max_bins(), prm->bins2));
+    if (prm->bins2 > max_bins || (prm->bins2 & (prm->bins2 - 1)))
+      return fail(set_err(c, FB_ERR_INVALID, "bins2 must be a power of two <= %llu (got %u)",
+                          (unsigned long long)max_bins, prm->bins2));
     nb2 = prm->bins2;
   }
+  // rounds of at most p2_max_bins() bins each
+  uint32_t log2_rounds = 0;
+  while ((nb2 >> log2_rounds) > p2_max_bins()) ++log2_rounds;
+  const uint64_t nbr = nb2 >> log2_rounds;
   so->stats[5] = nb2;
   so->stats[6] = lds_cap;
   so->stats[7] = (uint64_t)n_est_keys;
@@ -759,7 +767,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   const uint64_t threads2 = bank->n_reads * (uint64_t)split2;
   // a window at a share boundary can give one record to each of two threads
   const uint64_t rec_cap = windows + threads2;
-  if (!c_reads && prm->placement != 1 && nb2 <= p2_max_bins() && windows > 0 && rec_cap < (1ull << 32)) {
+  if (!c_reads && prm->placement != 1 && nb2 <= max_bins && windows > 0 && rec_cap < (1ull << 32)) {
     const uint32_t rw = (2 * k + 8 + 63) / 64;
     // keys beyond what the LDS tables are expected to hold, plus slack
     const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
@@ -769,10 +777,10 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     const uint64_t n_blocks2 = scatter_blocks(threads2, chunks2);
     DBuf recs, gm, runs, bh, st2, p3s;
     int r = 0;
-    if (!r) r = bh.get(c, std::max<uint64_t>(n_blocks2 * nb2, 1) * 4);
+    if (!r) r = bh.get(c, std::max<uint64_t>(n_blocks2 * nbr, 1) * 4);
     if (!r) r = recs.get(c, rec_cap * rw * 8);
     if (!r) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
-    if (!r) r = runs.get(c, (nb2 * kBinSlices + 1) * 4);
+    if (!r) r = runs.get(c, (nbr * kBinSlices + 1) * 4);
     if (!r) r = st2.get(c, 64);
     if (!r) r = p3s.get(c, pass3_scratch_bytes());
     if (!r) r = tbuf.get(c, spill_cap * tab.stride * 8);
@@ -783,7 +791,6 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     tab.cap = spill_cap;
     tab.slots = tbuf.as<uint64_t>();
     HIP_TRY(c, launch_table_clear(tab, s));
-    HIP_TRY(c, launch_fill_u32(runs.as<uint32_t>(), nb2 * kBinSlices + 1, 0, s));
     HIP_TRY(c, launch_fill_u32(st2.as<uint32_t>(), 16, 0, s));
     P2BinArgs pa;
     pa.reads = bank->view();
@@ -794,14 +801,15 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.n_blocks = (uint32_t)n_blocks2;
     pa.solid_bits = bits.as<uint32_t>();
     pa.log2w = so->cms.log2w;
-    pa.nbins = (uint32_t)nb2;
-    pa.log2_nbins = (uint32_t)__builtin_ctzll(nb2);
+    pa.nbins = (uint32_t)nbr;
+    pa.log2_nbins = (uint32_t)__builtin_ctzll(nbr);
+    pa.log2_rounds = log2_rounds;
     pa.gate_masks = gm.as<uint64_t>();
     pa.rw = rw;
     pa.runs = runs.as<uint32_t>();
     pa.recs = recs.as<uint64_t>();
     pa.rec_cap = rec_cap;
-    pa.status = st2.as<uint32_t>();            // [0] record buffer overflow
+    pa.status = st2.as<uint32_t>();            // [0] record buffer overflow, [1] records
     pa.out_count = st2.as<uint32_t>() + 2;     // records emitted from LDS
     pa.stats = reinterpret_cast<unsigned long long*>(st2.as<uint32_t>() + 4);
     pa.lds_cap = lds_cap;
@@ -817,9 +825,8 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     HIP_TRY(c, hipEventRecord(c->ev[4], s));
     HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
                             so->mask.as<uint8_t>(), p3s.p, s, pa.out_count));
-    uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0}, n_rec = 0;
+    uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0};
     HIP_TRY(c, hipMemcpyAsync(hb, st2.p, 32, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipMemcpyAsync(&n_rec, runs.as<uint32_t>() + nb2 * kBinSlices, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(sp, p3s.p, 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(ts, tab.status, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
@@ -833,7 +840,8 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
       claimed = bins_keys + ts[1];
       cap = spill_cap;
       route = 1;
-      so->stats[8] = n_rec;
+      so->stats[8] = hb[1];
+      so->stats[10] = 1ull << log2_rounds;
       done = true;
     } else {
       route = hb[0] ? 2 : 3;
@@ -962,7 +970,7 @@ extern "C" int fb_solid_kernel_timings(const fb_solid* s, float out[5]) {
   memcpy(out, s->kms, sizeof s->kms);
   return FB_OK;
 }
-extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[10]) {
+extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[11]) {
   if (!s || !out) return FB_ERR_INVALID;
   memcpy(out, s->stats, sizeof s->stats);
   return FB_OK;

@@ -132,7 +132,7 @@ typedef struct {
   uint32_t sort;              /* 1 = records sorted by k-mer (parity form) */
   uint32_t split;             /* threads per read; 0 = tuned default */
   uint32_t placement;         /* sketch and solid-table placement: 0 auto, 1 global tables, 2 LDS bins */
-  uint32_t bins2;             /* pass-2 LDS bins (power of two <= 16384); 0 = sized from the sketch */
+  uint32_t bins2;             /* pass-2 LDS bins over all rounds (power of two <= 2^20); 0 = sized from the sketch */
 } fb_build_params;
 
 void fb_build_params_default(fb_build_params* p, uint32_t k);
@@ -166,11 +166,13 @@ int fb_solid_kernel_timings(const fb_solid* s, float out[5]);
 /* Work counts of the build: [0] k-mer windows of pass 1, [1] pass-2 records
  * (sum of raw cov), [2] keys in the solid table, [3] table slots, [4] pass-2
  * route (0 global table, 1 LDS bins, 2 LDS bins abandoned for a full record
- * buffer, 3 abandoned for a full spill table), [5] pass-2 LDS bins, [6] keys
- * per LDS bin table, [7] keys estimated from the pass-1 gate bits, [8]
- * records the LDS route wrote (one per window with an edge, 0 on another
- * route), [9] reads of length >= k. */
-int fb_solid_stats(const fb_solid* s, uint64_t out[10]);
+ * buffer, 3 abandoned for a full spill table), [5] pass-2 LDS bins over all
+ * rounds, [6] keys per LDS bin table, [7] keys estimated from the pass-1
+ * gate bits, [8] records the LDS route wrote (one per window with an edge,
+ * 0 on another route), [9] reads of length >= k, [10] pass-2 rounds (the
+ * LDS route splits the bins into rounds of at most 16384; 0 on another
+ * route). */
+int fb_solid_stats(const fb_solid* s, uint64_t out[11]);
 
 /* ---- Batched form: many isolates per launch --------------------------------
  * The reference builds one isolate per process run (main.rs:262).  A batch

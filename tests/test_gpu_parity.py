@@ -257,3 +257,25 @@ def test_pass2_placements_agree_on_stats(ctx, small_iso, placement):
     st1 = h1.stats()
     h1.close()
     assert st["records"] == st1["records"] and st["keys"] == st1["keys"] and st["windows"] == st1["windows"]
+
+
+@pytest.mark.parametrize("k,bins2", [(21, 1 << 15), (55, 1 << 16), (127, 1 << 18)])
+def test_pass2_rounds_match_oracle(ctx, orc, small_iso, k, bins2):
+    """More pass-2 bins than one launch of the bin kernels takes: the keys are
+    split into rounds by the top bits of their bin hash, each round counted,
+    scattered and tabled alone.  The set is the same exact set."""
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=2, cms_width=1 << 18, placement=2, bins2=bins2)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=2, width=1 << 18)
+    _solid_equal(g, r)
+    h = F.build_solid_handle(ctx, dev, k, target_solid_cov=2, cms_width=1 << 18, placement=2, bins2=bins2)
+    st = h.stats()
+    h.close()
+    h1 = F.build_solid_handle(ctx, dev, k, target_solid_cov=2, cms_width=1 << 18, placement=2)
+    st1 = h1.stats()
+    h1.close()
+    assert st["pass2_route"] == "lds" and st["pass2_rounds"] == bins2 // 16384
+    assert st1["pass2_rounds"] == 1
+    # every round writes its own records: together they are the one-round records
+    assert st["pass2_records"] == st1["pass2_records"] and st["records"] == st1["records"]

@@ -172,6 +172,7 @@ SIGNATURES = [
     ("fb_batch_cms_download", C.c_int, [vp, C.c_uint32, vp, C.c_uint64]),
     ("fb_batch_hll_registers", C.c_int, [vp, C.c_uint32, u8p]),
     ("fb_batch_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("fb_batch_routes", C.c_int, [vp, u32p]),
     ("fb_count_stats_from_lengths", C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(CountStats)]),
     ("fb_compute_dynamic_kmers", C.c_uint32, [C.c_uint64, C.c_int64, u32p]),
     ("fb_estimate_genome", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.POINTER(CountStats),

@@ -274,6 +274,11 @@ def hll_estimate_registers(regs: np.ndarray, p: int) -> float:
     return float(_lib.load().fb_hll_estimate_registers(_ptr(r, C.c_uint8), p))
 
 
+
+# pass-2 routes (fb_solid_stats[4]): the abandoned LDS routes finish on the global table
+ROUTES = ("global", "lds", "lds_region_full", "lds_spill_full", "lds_output_full")
+
+
 @dataclass
 class SolidSet:
     """Solid k-mers {K -> (cov, mask)} of one pass, sorted by K.
@@ -320,7 +325,7 @@ class SolidHandle(_Handle):
         t = (C.c_uint64 * 11)()
         self.ctx.check(self.ctx.lib.fb_solid_stats(self.h, t), "fb_solid_stats")
         return {"windows": int(t[0]), "records": int(t[1]), "keys": int(t[2]), "slots": int(t[3]),
-                "pass2_route": ("global", "lds", "lds_region_full", "lds_spill_full")[int(t[4])],
+                "pass2_route": ROUTES[int(t[4])],
                 "lds_bins": int(t[5]), "lds_bin_keys": int(t[6]), "est_keys": int(t[7]),
                 "pass2_records": int(t[8]), "reads_used": int(t[9]), "pass2_rounds": int(t[10])}
 
@@ -408,9 +413,9 @@ class Batch(_Handle):
 
     def __init__(self, ctx: Context, banks: Sequence[DeviceBank], genome_sizes: Sequence[int],
                  cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
-                 split: int = 0, placement: int = 0):
+                 split: int = 0, placement: int = 0, bins2: int = 0):
         p = build_params(0, cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, sort=sort, split=split,
-                         placement=placement)
+                         placement=placement, bins2=bins2)
         arr = (C.c_void_p * len(banks))(*[b.h.value if hasattr(b.h, "value") else b.h for b in banks])
         gs = _u64(genome_sizes)
         h = C.c_void_p()
@@ -431,6 +436,12 @@ class Batch(_Handle):
         t = (C.c_float * 4)()
         self.ctx.check(self.ctx.lib.fb_batch_timings(self.h, t), "fb_batch_timings")
         return tuple(float(x) for x in t)
+
+    def routes(self) -> list:
+        """Pass-2 route of every isolate in the last run."""
+        r = np.zeros(len(self), np.uint32)
+        self.ctx.check(self.ctx.lib.fb_batch_routes(self.h, _ptr(r, C.c_uint32)), "fb_batch_routes")
+        return [ROUTES[int(x)] for x in r]
 
     def result(self, i: int, k: int, with_cms: bool = True) -> SolidSet:
         n = int(self.ctx.lib.fb_batch_count(self.h, i))

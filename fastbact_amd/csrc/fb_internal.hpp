@@ -138,10 +138,11 @@ hipError_t launch_pass2_multi(const MultiLaunch& m, uint32_t k, hipStream_t s);
 // scratch (pass3_scratch_bytes()): u32 [0] records kept, u64 at byte 8 the sum
 // of raw cov (records added to the table), then per-block partials.
 size_t pass3_scratch_bytes();
-// out_base (nullable): device u32 added to every output position.
+// out_base (nullable): device u32 added to every output position; positions
+// at or past out_cap are not written (the count in scratch still has them).
 hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t n_ref_words, uint32_t min_trans_cov,
                         uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s,
-                        const uint32_t* out_base = nullptr);
+                        const uint32_t* out_base = nullptr, uint32_t out_cap = 0xFFFFFFFFu);
 // Sort the compacted records by k-mer (integer order).  Uses scratch from
 // the caller: tmp_bytes queried with keys == nullptr.
 hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t n_ref_words,
@@ -183,7 +184,8 @@ struct P2BinArgs {
   uint64_t* out_keys = nullptr;
   uint16_t* out_cov = nullptr;
   uint8_t* out_mask = nullptr;
-  uint32_t* out_count = nullptr;  // zeroed; records emitted from LDS
+  uint32_t* out_count = nullptr;  // zeroed; records emitted from LDS (counted also past out_cap)
+  uint32_t out_cap = 0xFFFFFFFFu; // output entries; positions past it are not written
   unsigned long long* stats = nullptr;  // zeroed; [0] records (sum of raw cov), [1] keys held in LDS
 };
 uint32_t p2_lds_cap(uint32_t kw);
@@ -214,6 +216,7 @@ struct P3Args {
   uint8_t* out_mask = nullptr;
   void* scratch = nullptr;  // pass3_scratch_bytes()
   const uint32_t* out_base = nullptr;
+  uint32_t out_cap = 0xFFFFFFFFu;
 };
 hipError_t launch_pass3_multi(const P3Args* args, uint32_t n, uint32_t kw, hipStream_t s);
 struct FillRange {

@@ -555,7 +555,8 @@ template <int KW>
 __device__ __forceinline__ void pass3_write_body(const uint64_t* __restrict__ slots, uint64_t cap, uint32_t stride,
                                                  uint32_t nref, uint32_t gate, const uint32_t* __restrict__ boffset,
                                                  uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
-                                                 uint8_t* __restrict__ out_mask, const uint32_t* __restrict__ out_base) {
+                                                 uint8_t* __restrict__ out_mask, const uint32_t* __restrict__ out_base,
+                                                 uint32_t out_cap) {
   __shared__ uint32_t s_w[kBlock / 64];
   const uint64_t per = (cap + kP3Blocks - 1) / kP3Blocks;
   const uint64_t lo = (uint64_t)blockIdx.x * per, hi = lo + per < cap ? lo + per : cap;
@@ -575,8 +576,8 @@ __device__ __forceinline__ void pass3_write_body(const uint64_t* __restrict__ sl
       if (w < wave) before += s_w[w];
       chunk += s_w[w];
     }
-    if (keep) {
-      const uint32_t pos = base + before + (uint32_t)__builtin_popcountll(ballot & ((1ull << lane) - 1ull));
+    const uint32_t pos = base + before + (uint32_t)__builtin_popcountll(ballot & ((1ull << lane) - 1ull));
+    if (keep && pos < out_cap) {
       uint64_t limb[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < KW; ++j) {
@@ -598,8 +599,9 @@ __global__ __launch_bounds__(kBlock) void k_pass3_write(const uint64_t* __restri
                                                         uint32_t stride, uint32_t nref, uint32_t gate,
                                                         const uint32_t* __restrict__ boffset,
                                                         uint64_t* __restrict__ out_keys, uint16_t* __restrict__ out_cov,
-                                                        uint8_t* __restrict__ out_mask, const uint32_t* __restrict__ out_base) {
-  pass3_write_body<KW>(slots, cap, stride, nref, gate, boffset, out_keys, out_cov, out_mask, out_base);
+                                                        uint8_t* __restrict__ out_mask, const uint32_t* __restrict__ out_base,
+                                                        uint32_t out_cap) {
+  pass3_write_body<KW>(slots, cap, stride, nref, gate, boffset, out_keys, out_cov, out_mask, out_base, out_cap);
 }
 
 // Pass 3 over many tables: blockIdx.y selects the table.
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(kBlock) void k_pass3_write_m(const P3Args* __restri
   unsigned long long *rs, *br;
   p3_scratch(a.scratch, oc, rs, br, bc);
   pass3_write_body<KW>(a.t.slots, a.t.cap, a.t.stride, a.nref, a.gate, bc, a.out_keys, a.out_cov, a.out_mask,
-                       a.out_base);
+                       a.out_base, a.out_cap);
 }
 
 hipError_t launch_pass3_multi(const P3Args* args, uint32_t n, uint32_t kw, hipStream_t s) {
@@ -681,7 +683,8 @@ hipError_t launch_table_clear_multi(const DevSolidTable* tabs, uint32_t n, hipSt
 size_t pass3_scratch_bytes() { return 16 + (size_t)kP3Blocks * 12 + 64; }
 
 hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t nref, uint32_t gate, uint64_t* out_keys,
-                        uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s, const uint32_t* out_base) {
+                        uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s, const uint32_t* out_base,
+                        uint32_t out_cap) {
   (void)k;
   uint32_t* out_count = static_cast<uint32_t*>(scratch);
   unsigned long long* rec_sum = reinterpret_cast<unsigned long long*>(out_count + 2);
@@ -692,7 +695,7 @@ hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t nref, uint3
                      bcount, brec);                                                                             \
   hipLaunchKernelGGL(k_pass3_scan, dim3(1), dim3(1024), 0, s, bcount, brec, out_count, rec_sum);               \
   hipLaunchKernelGGL((k_pass3_write<KK>), dim3(kP3Blocks), dim3(kBlock), 0, s, t.slots, t.cap, t.stride, nref,  \
-                     gate, bcount, out_keys, out_cov, out_mask, out_base)
+                     gate, bcount, out_keys, out_cov, out_mask, out_base, out_cap)
   switch (t.kw) {
     case 1: P3(1); break;
     case 2: P3(2); break;

@@ -548,6 +548,7 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   for (uint32_t e = lo; e < hi; ++e) {
     uint32_t m, cv;
     if (!keep_of(e, m, cv)) continue;
+    if (pos >= a.out_cap) break;  // the host sees out_count > out_cap and re-runs the isolate
     uint64_t wk[KW];
 #pragma unroll
     for (int j = 0; j < KW; ++j) wk[j] = keys[(uint64_t)j * cap + e];

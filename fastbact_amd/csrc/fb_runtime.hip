@@ -772,7 +772,10 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     // keys beyond what the LDS tables are expected to hold, plus slack
     const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
     const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est_keys / 4)));
-    const uint64_t out_cap = nb2 * lds_cap + spill_cap;
+    // solid keys from the bins: at most the keys they hold; sized from the
+    // estimate, with a count past the end sending the build to the global route
+    const uint64_t lds_out = std::min<uint64_t>(nb2 * lds_cap, (uint64_t)(1.25 * n_est_keys) + 65536);
+    const uint64_t out_cap = std::min<uint64_t>(lds_out + spill_cap, 0xFFFFFFFFull);
     const uint32_t chunks2 = scatter_chunks(threads2);
     const uint64_t n_blocks2 = scatter_blocks(threads2, chunks2);
     DBuf recs, gm, runs, bh, st2, p3s;
@@ -811,6 +814,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.rec_cap = rec_cap;
     pa.status = st2.as<uint32_t>();            // [0] record buffer overflow, [1] records
     pa.out_count = st2.as<uint32_t>() + 2;     // records emitted from LDS
+    pa.out_cap = (uint32_t)out_cap;
     pa.stats = reinterpret_cast<unsigned long long*>(st2.as<uint32_t>() + 4);
     pa.lds_cap = lds_cap;
     pa.min_trans_cov = gate;
@@ -824,13 +828,14 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     HIP_TRY(c, hipEventRecord(c->ev[3], s));
     HIP_TRY(c, hipEventRecord(c->ev[4], s));
     HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
-                            so->mask.as<uint8_t>(), p3s.p, s, pa.out_count));
+                            so->mask.as<uint8_t>(), p3s.p, s, pa.out_count, (uint32_t)out_cap));
     uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0};
     HIP_TRY(c, hipMemcpyAsync(hb, st2.p, 32, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(sp, p3s.p, 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipMemcpyAsync(ts, tab.status, 8, hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    if (!hb[0] && !ts[0]) {
+    const bool out_full = (uint64_t)hb[2] + sp[0] > out_cap;
+    if (!hb[0] && !ts[0] && !out_full) {
       uint64_t bins_rec = 0, bins_keys = 0, sp_rec = 0;
       memcpy(&bins_rec, hb + 4, 8);
       memcpy(&bins_keys, hb + 6, 8);
@@ -844,7 +849,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
       so->stats[10] = 1ull << log2_rounds;
       done = true;
     } else {
-      route = hb[0] ? 2 : 3;
+      route = hb[0] ? 2 : (ts[0] ? 3 : 4);
     }
   }
   if (!done) {
@@ -1086,8 +1091,13 @@ extern "C" int fb_estimate_genome(fb_ctx* c, const fb_arena_view* r1, const fb_a
 }
 
 // ---------------------------------------------------------------------------
-// Batched form: all isolates of a batch in each launch, passes captured as a
-// HIP graph per k and replayed.
+// Batched form: all isolates of a batch in each launch.  A pass at k is two
+// captured HIP graphs, replayed while the layout holds: (1) the sketch clears,
+// pass 1 on LDS bins and the gate occupancy of every isolate; (2) after the
+// host has sized each isolate's pass 2 from that occupancy, pass 2 and the
+// fused pass 3 on the LDS route (in rounds) and pass 3 of the spill tables.
+// An isolate whose LDS route overflows its record, spill or output buffer
+// re-runs pass 2 alone on the global table, which grows until it fits.
 
 struct BatchIso {
   const fb_bank* bank = nullptr;
@@ -1095,16 +1105,21 @@ struct BatchIso {
   fb_cms cms;
   fb_hll hll;
   bool has_hll = false;
-  DBuf bits, bdata, bcur, slots, keys, cov, mask, p3;
-  uint64_t bins_cap = 0;
-  uint32_t sb = 14, nbins = 1;
+  // gate bits; pass-1 bins, then pass-2 records; bin cursors; gate occupancy
+  DBuf bits, scratch, bcur, occ;
+  DBuf gm, bh, runs, st2, p3, slots, keys, cov, mask;
+  // pass-1 layout
+  uint32_t split = 1, sb = kBinLog2Cells, nbins = 1, chunks1 = 1;
+  uint64_t bins_cap = 0, windows = 0, threads1 = 0;
   bool binned = false;
-  uint64_t tab_cap = 0;
-  uint32_t tab_stride = 0;
+  // pass-2 layout of the LDS route
+  bool lds = false;
+  uint32_t split2 = 1, chunks2 = 1, log2_rounds = 0;
+  uint64_t nb2 = 1, nbr = 1, n_blocks2 = 0, rec_cap = 0, spill_cap = 0, out_cap = 0;
+  double est_keys = 0.0;
+  // results of the last run
   uint64_t n_out = 0;
-  uint32_t split = 1;
-  uint64_t windows = 0;
-  bool overflow = false;
+  uint32_t route = 0;  // as fb_solid_stats[4]
 };
 
 struct fb_batch {
@@ -1113,17 +1128,40 @@ struct fb_batch {
   std::vector<std::unique_ptr<BatchIso>> iso;
   uint32_t k = 0;
   uint32_t nref = 1;
-  DBuf d_p1, d_bin, d_p2, d_p3, d_fill, d_tabs, d_blk1, d_blkc, d_blk2, d_status;
-  MultiLaunch m1, mc, m2;
-  uint32_t max_nbins = 1, max_sb = 6;
-  uint32_t n_fill = 0;
-  bool any_binned = false, all_binned = false;
-  std::map<uint32_t, hipGraphExec_t> graphs;
-  std::map<uint32_t, hipGraph_t> graph_defs;
+  // pass 1
+  DBuf d_p1, d_bin, d_fill1, d_blk1, d_blkc;
+  MultiLaunch m1, mc;
+  uint32_t n_fill1 = 0, max_nbins = 1, max_sb = 6;
+  bool binned = false;
+  // pass 2 (LDS isolates only, in lds_idx order)
+  std::vector<uint32_t> lds_idx;
+  DBuf d_p2, d_p3, d_tabs, d_fill2, d_runs_fill, d_blkg, d_blks, d_blkb;
+  MultiLaunch mg, ms2, mb;
+  uint32_t n_fill2 = 0, max_nbr = 1, max_rounds = 1, lds_cap = 0;
+  // A graph reads the per-isolate pointers from the argument arrays, which
+  // are re-uploaded before every launch; what it holds fixed are the launch
+  // shapes, recorded here per k.
+  std::map<uint32_t, hipGraphExec_t> g1, g2;
+  std::map<uint32_t, hipGraph_t> g1_def, g2_def;
+  std::map<uint32_t, std::vector<uint64_t>> sig1, sig2;
   float ms[4] = {0, 0, 0, 0};
+  void drop(std::map<uint32_t, hipGraphExec_t>& ge, std::map<uint32_t, hipGraph_t>& gd, uint32_t kk) {
+    auto e = ge.find(kk);
+    if (e != ge.end()) {
+      (void)hipGraphExecDestroy(e->second);
+      ge.erase(e);
+    }
+    auto d = gd.find(kk);
+    if (d != gd.end()) {
+      (void)hipGraphDestroy(d->second);
+      gd.erase(d);
+    }
+  }
   ~fb_batch() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
-    for (auto& g : graph_defs) (void)hipGraphDestroy(g.second);
+    for (auto& g : g1) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : g2) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : g1_def) (void)hipGraphDestroy(g.second);
+    for (auto& g : g2_def) (void)hipGraphDestroy(g.second);
   }
 };
 
@@ -1131,29 +1169,65 @@ static uint64_t batch_cms_width(const fb_build_params* p, uint64_t est_g) {
   return next_pow2(p->cms_width ? p->cms_width : fb_cms_width_for_genome_size(est_g));
 }
 
-// Table capacity from the genome size: the keys pass 2 inserts run to
-// about 2-3.5x the genome's distinct k-mers at the widths used here (error
-// k-mers lifted over the gate by Count-Min collisions); load stays <= 0.5.
-static uint64_t batch_table_cap(uint64_t est_g, uint32_t t_solid) {
-  const double keys = (t_solid <= 2 ? 3.5 : 2.5) * (double)est_g;
-  return next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * keys)));
+// Pass-2 sizing of the LDS route from the estimated keys (as fb_build_solid).
+struct P2Plan {
+  uint64_t nb2 = 1, nbr = 1, spill_cap = 0, out_cap = 0, rec_cap = 0, n_blocks2 = 0, threads2 = 0;
+  uint32_t log2_rounds = 0, split2 = 1, chunks2 = 1, lds_cap = 0;
+  bool ok = false;
+};
+static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint32_t max_len, uint32_t k,
+                         uint32_t split, uint32_t bins2) {
+  P2Plan q;
+  const uint32_t kw = key_words_for_k(k);
+  q.lds_cap = p2_lds_cap(kw);
+  while ((double)q.nb2 * 0.5 * (double)q.lds_cap < n_est) q.nb2 <<= 1;
+  const uint64_t max_bins = (uint64_t)p2_max_bins() << kP2MaxLog2Rounds;
+  if (bins2) q.nb2 = bins2;
+  while ((q.nb2 >> q.log2_rounds) > p2_max_bins()) ++q.log2_rounds;
+  q.nbr = q.nb2 >> q.log2_rounds;
+  const uint32_t pairs_max = max_len > k ? max_len - k : 1;
+  q.split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
+  q.threads2 = n_reads * (uint64_t)q.split2;
+  q.rec_cap = windows + q.threads2;
+  q.chunks2 = scatter_chunks(q.threads2);
+  q.n_blocks2 = scatter_blocks(q.threads2, q.chunks2);
+  const double excess = std::max(0.0, n_est - (double)q.nb2 * 0.5 * (double)q.lds_cap);
+  q.spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est / 4)));
+  const uint64_t lds_out = std::min<uint64_t>(q.nb2 * q.lds_cap, (uint64_t)(1.25 * n_est) + 65536);
+  q.out_cap = std::min<uint64_t>(lds_out + q.spill_cap, 0xFFFFFFFFull);
+  q.ok = q.nb2 <= max_bins && windows > 0 && q.rec_cap < (1ull << 32);
+  return q;
+}
+
+// Linear-counting estimate of the distinct keys landing in gate-passing
+// row-0 cells: an upper-side estimate of the keys pass 2 inserts.
+static double est_keys_from_occ(uint64_t occ, uint64_t width, uint64_t windows) {
+  const double W = (double)width;
+  double n = occ >= width ? (double)windows : -W * log(1.0 - (double)occ / W);
+  return std::min(n, (double)windows + 1.0);
 }
 
 extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uint64_t reads,
                                        const fb_build_params* p, uint32_t k) {
   if (!p) return 0;
+  k = std::max<uint32_t>(k, 1);
   const uint64_t est_g = std::max<uint64_t>(genome_size, 50000);
   const uint64_t W = batch_cms_width(p, est_g);
-  const uint32_t t = p->target_solid_cov >= 0 ? (uint32_t)p->target_solid_cov : 2u;
-  const uint64_t cap = batch_table_cap(est_g, t);
-  const uint64_t stride = table_stride(key_words_for_k(std::max<uint32_t>(k, 1)));
-  const uint64_t windows = bases;  // upper bound of windows at any k
-  uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;             // counts + gate bits
-  b += p->hll_p ? (4ull << p->hll_p) : 0;                      // HLL registers
-  b += cap * stride * 8;                                       // solid table
-  b += cap * (ref_words(std::max<uint32_t>(k, 1)) * 8 + 3);   // outputs
-  b += (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536;    // bin offsets
-  b += bases / 4 + reads * 12 + 4096;                          // the bank itself
+  const uint64_t windows = bases;  // upper bound of the windows at any k
+  const uint32_t max_len = reads ? (uint32_t)std::min<uint64_t>(bases / reads + 1, 1u << 20) : 1u;
+  // keys of pass 2: at most the windows, and the linear-counting estimate
+  // stays below 8 keys per row-0 cell for the sketches in use
+  const double n_est = (double)std::min<uint64_t>(windows, 8 * W);
+  const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2);
+  const uint32_t rw = (2 * k + 8 + 63) / 64;
+  uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;            // exact counts + gate bits
+  b += p->hll_p ? (4ull << p->hll_p) : 0;                     // HLL registers
+  const uint64_t bins = (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536;
+  b += std::max<uint64_t>(bins, q.rec_cap * rw * 8);          // pass-1 bins, then pass-2 records
+  b += q.threads2 * 8 + q.n_blocks2 * q.nbr * 4;              // gate masks, run matrix
+  b += q.spill_cap * table_stride(key_words_for_k(k)) * 8;    // spill table
+  b += q.out_cap * (ref_words(k) * 8 + 3);                    // solid set
+  b += bases / 4 + reads * 12 + 4096;                         // the bank itself
   return b;
 }
 
@@ -1165,6 +1239,8 @@ extern "C" int fb_batch_create(fb_ctx* c, const fb_bank* const* banks, const uin
     return set_err(c, FB_ERR_INVALID, "counter_bits must be 4 or 16 (got %d)", prm->counter_bits);
   if (prm->hll_p != 0 && (prm->hll_p < 4 || prm->hll_p > 18))
     return set_err(c, FB_ERR_INVALID, "hll_p %d outside 4..18", prm->hll_p);
+  if (prm->bins2 && (prm->bins2 & (prm->bins2 - 1)))
+    return set_err(c, FB_ERR_INVALID, "bins2 must be a power of two (got %u)", prm->bins2);
   if (int r = ctx_current(c)) return r;
   auto b = std::make_unique<fb_batch>();
   b->ctx = c;
@@ -1180,20 +1256,25 @@ extern "C" int fb_batch_create(fb_ctx* c, const fb_bank* const* banks, const uin
       it->has_hll = true;
     }
     if (int r = it->bits.get(c, ((it->cms.width * 4 + 31) / 32) * 4)) return r;
+    if (int r = it->occ.get(c, 8)) return r;
     if (int r = it->p3.get(c, pass3_scratch_bytes())) return r;
+    if (int r = it->st2.get(c, 64)) return r;
     b->iso.push_back(std::move(it));
   }
   const size_t nn = n;
   if (int r = b->d_p1.get(c, nn * sizeof(Pass1Args))) return r;
   if (int r = b->d_bin.get(c, nn * sizeof(BinArgs))) return r;
-  if (int r = b->d_p2.get(c, nn * sizeof(Pass2Args))) return r;
-  if (int r = b->d_p3.get(c, nn * sizeof(P3Args))) return r;
-  if (int r = b->d_tabs.get(c, nn * sizeof(DevSolidTable))) return r;
-  if (int r = b->d_fill.get(c, nn * 4 * sizeof(FillRange))) return r;
+  if (int r = b->d_fill1.get(c, nn * 4 * sizeof(FillRange))) return r;
   if (int r = b->d_blk1.get(c, (nn + 1) * 4)) return r;
   if (int r = b->d_blkc.get(c, (nn + 1) * 4)) return r;
-  if (int r = b->d_blk2.get(c, (nn + 1) * 4)) return r;
-  if (int r = b->d_status.get(c, nn * 16)) return r;
+  if (int r = b->d_p2.get(c, nn * sizeof(P2BinArgs))) return r;
+  if (int r = b->d_p3.get(c, nn * sizeof(P3Args))) return r;
+  if (int r = b->d_tabs.get(c, nn * sizeof(DevSolidTable))) return r;
+  if (int r = b->d_fill2.get(c, nn * sizeof(FillRange))) return r;
+  if (int r = b->d_runs_fill.get(c, nn * sizeof(FillRange))) return r;
+  if (int r = b->d_blkg.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blks.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blkb.get(c, (nn + 1) * 4)) return r;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   *out = b.release();
   return FB_OK;
@@ -1206,69 +1287,58 @@ extern "C" uint64_t fb_batch_count(const fb_batch* b, uint32_t i) {
   return (b && i < b->iso.size()) ? b->iso[i]->n_out : 0;
 }
 
-// Lays out per-isolate buffers and argument arrays for a pass at k.
-static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gate, bool* layout_changed) {
+// Pass-1 layout and argument arrays; *changed when a launch shape differs
+// from the one the graph of this k was captured with.
+static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* changed) {
   fb_ctx* c = b->ctx;
   const uint32_t n = (uint32_t)b->iso.size();
-  const uint32_t kw = key_words_for_k(k), stride = table_stride(kw);
   std::vector<Pass1Args> p1(n);
   std::vector<BinArgs> ba(n);
-  std::vector<Pass2Args> p2(n);
-  std::vector<P3Args> p3(n);
-  std::vector<DevSolidTable> tabs(n);
   std::vector<FillRange> fills;
-  std::vector<uint32_t> blk1(n + 1, 0), blkc(n + 1, 0), blk2(n + 1, 0);
+  std::vector<uint32_t> blk1(n + 1, 0), blkc(n + 1, 0);
   b->max_nbins = 1;
   b->max_sb = 6;
-  b->any_binned = false;
-  b->all_binned = true;
-  *layout_changed = false;
+  // one pass-1 placement for the whole batch: LDS bins unless asked for
+  // global atomics or a sketch has more bins than the scatter kernel holds
+  bool binned = b->prm.placement != 1;
+  for (auto& it : b->iso) {
+    uint32_t sb = kBinLog2Cells;
+    while (sb > 6 && (1ull << sb) > it->cms.width * 4) --sb;
+    if (((it->cms.width * 4) >> sb) > 8192) binned = false;
+  }
+  if (b->prm.placement == 2 && !binned)
+    return set_err(c, FB_ERR_INVALID, "LDS placement needs at most 8192 pass-1 bins per isolate");
+  b->binned = binned;
   for (uint32_t i = 0; i < n; ++i) {
     BatchIso& it = *b->iso[i];
     const uint64_t ncnt = it.cms.width * 4;
     it.windows = bank_windows(it.bank, k);
     it.split = b->prm.split ? b->prm.split : default_split(k, it.bank->max_len);
+    it.threads1 = it.bank->n_reads * (uint64_t)it.split;
     uint32_t sb = kBinLog2Cells;
     while (sb > 6 && (1ull << sb) > ncnt) --sb;
     const uint64_t nbins = std::max<uint64_t>(1, ncnt >> sb);
-    const bool binned = b->prm.placement != 1 && nbins <= 8192;
-    if (binned != it.binned || sb != it.sb) *layout_changed = true;
-    it.binned = binned;
     it.sb = sb;
     it.nbins = (uint32_t)nbins;
+    it.binned = binned;
+    it.chunks1 = scatter_chunks(it.threads1, kP1BlockSize);
     if (binned) {
-      const uint64_t thr1 = it.bank->n_reads * (uint64_t)it.split;
-      const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1, kP1BlockSize), kP1BlockSize);
+      const uint64_t n_wg = scatter_blocks(it.threads1, it.chunks1, kP1BlockSize);
       const uint64_t cap = bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg, 4);
-      if (cap > it.bins_cap || !it.bdata.p) {
-        if (int r = it.bdata.get(c, nbins * kBinSlices * cap * 2)) return r;
-        it.bins_cap = cap;
-      }
+      const uint64_t need = nbins * kBinSlices * cap * 2;
+      if (need > it.scratch.bytes || !it.scratch.p)
+        if (int r = it.scratch.get(c, need)) return r;
+      it.bins_cap = cap;
       if (!it.bcur.p)
         if (int r = it.bcur.get(c, 8192 * kBinSlices * 4)) return r;
       b->max_nbins = std::max<uint32_t>(b->max_nbins, it.nbins);
       b->max_sb = std::max<uint32_t>(b->max_sb, sb);
-      b->any_binned = true;
-    } else {
-      b->all_binned = false;
-    }
-    const uint64_t tcap = std::max(it.tab_cap, batch_table_cap(it.est_g, t_solid));
-    if (tcap != it.tab_cap || stride != it.tab_stride || !it.slots.p) {
-      if (int r = it.slots.get(c, tcap * stride * 8)) return r;
-      if (int r = it.keys.get(c, tcap * ref_words(k) * 8)) return r;
-      if (int r = it.cov.get(c, tcap * 2)) return r;
-      if (int r = it.mask.get(c, tcap)) return r;
-      it.tab_cap = tcap;
-      it.tab_stride = stride;
-      *layout_changed = true;
     }
     DevReads rv = it.bank->view();
-    const uint64_t ncells = ncnt;
-    // fills: counts, HLL, bin cursors
-    fills.push_back({it.cms.counts.as<uint32_t>(), ncells, 0u});
+    fills.push_back({it.cms.counts.as<uint32_t>(), ncnt, 0u});
+    fills.push_back({it.occ.as<uint32_t>(), 2, 0u});
     if (it.has_hll) fills.push_back({it.hll.regs.as<uint32_t>(), (uint64_t)1 << it.hll.p, 0u});
     if (binned) fills.push_back({it.bcur.as<uint32_t>(), nbins * kBinSlices, 0u});
-    // pass 1
     Pass1Args& a1 = p1[i];
     a1.reads = rv;
     a1.k = k;
@@ -1280,94 +1350,276 @@ static int batch_prepare(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gat
     bi.reads = rv;
     bi.k = k;
     bi.split = it.split;
-    bi.chunks = scatter_chunks(it.bank->n_reads * (uint64_t)it.split, kP1BlockSize);
+    bi.chunks = it.chunks1;
     bi.log2w = it.cms.log2w;
-    bi.n_cells = ncells;
+    bi.n_cells = ncnt;
     bi.counts = it.cms.counts.as<uint32_t>();
     if (it.has_hll) bi.hll = it.hll.dev();
     bi.sb = sb;
     bi.nbins = it.nbins;
     bi.cap = it.bins_cap;
-    bi.data = it.bdata.as<uint16_t>();
+    bi.data = it.scratch.as<uint16_t>();
     bi.cursor = it.bcur.as<uint32_t>();
     bi.bits = it.bits.as<uint32_t>();
     bi.t_solid = t_solid;
     bi.cap_value = it.cms.cap();
-    // table and passes 2-3
-    DevSolidTable& t = tabs[i];
+    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(it.threads1, it.chunks1, kP1BlockSize)
+                                              : (it.threads1 + kPassBlock - 1) / kPassBlock);
+    blkc[i + 1] = blkc[i] + (binned ? it.nbins : 0u);
+  }
+  if (fills.size() > 4 * (size_t)n) return set_err(c, FB_ERR_INVALID, "internal: fill ranges");
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipMemcpyAsync(b->d_p1.p, p1.data(), n * sizeof(Pass1Args), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_bin.p, ba.data(), n * sizeof(BinArgs), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_fill1.p, fills.data(), fills.size() * sizeof(FillRange), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blk1.p, blk1.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blkc.p, blkc.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipStreamSynchronize(s));  // host vectors go out of scope
+  b->m1 = MultiLaunch{binned ? b->d_bin.p : b->d_p1.p, b->d_blk1.as<uint32_t>(), n, blk1[n]};
+  b->mc = MultiLaunch{b->d_bin.p, b->d_blkc.as<uint32_t>(), n, blkc[n]};
+  b->n_fill1 = (uint32_t)fills.size();
+  const std::vector<uint64_t> sig = {binned, b->n_fill1, blk1[n], blkc[n], b->max_nbins, b->max_sb};
+  if (b->sig1[k] != sig) *changed = true;
+  b->sig1[k] = sig;
+  return FB_OK;
+}
+
+// Graph 1: clears, pass 1, gate bits, gate occupancy of row 0.
+static int batch_enqueue1(fb_batch* b, uint32_t k, uint32_t t_solid) {
+  fb_ctx* c = b->ctx;
+  hipStream_t s = c->stream;
+  const bool hll = b->prm.hll_p != 0;
+  HIP_TRY(c, launch_fill_ranges(b->d_fill1.as<FillRange>(), b->n_fill1, s));
+  if (b->binned) {
+    HIP_TRY(c, launch_bin_scatter_multi(b->m1, k, hll, b->max_nbins, s));
+    HIP_TRY(c, launch_bin_count_multi(b->mc, b->max_sb, s));
+  } else {
+    HIP_TRY(c, launch_pass1_multi(b->m1, k, true, hll, s));
+    for (auto& it : b->iso)
+      HIP_TRY(c, launch_solid_bits(it->cms.counts.as<uint32_t>(), it->cms.width * 4, t_solid, it->cms.cap(),
+                                   it->bits.as<uint32_t>(), s));
+  }
+  for (auto& it : b->iso)
+    HIP_TRY(c, launch_popcount(it->bits.as<uint32_t>(), it->cms.width / 32 ? it->cms.width / 32 : 1,
+                               it->occ.as<unsigned long long>(), s));
+  return FB_OK;
+}
+
+// Pass-2 layout from the gate occupancies; *changed as for pass 1.
+static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vector<uint64_t>& occ, bool* changed) {
+  fb_ctx* c = b->ctx;
+  const uint32_t n = (uint32_t)b->iso.size();
+  const uint32_t kw = key_words_for_k(k), stride = table_stride(kw), nref = ref_words(k);
+  const uint32_t rw = (2 * k + 8 + 63) / 64;
+  std::vector<uint32_t> lds_idx;
+  std::vector<P2BinArgs> p2;
+  std::vector<P3Args> p3;
+  std::vector<DevSolidTable> tabs;
+  std::vector<FillRange> fill2, runs_fill;
+  std::vector<uint32_t> blkg(1, 0), blks(1, 0), blkb(1, 0);
+  b->max_nbr = 1;
+  b->max_rounds = 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    BatchIso& it = *b->iso[i];
+    it.est_keys = est_keys_from_occ(occ[i], it.cms.width, it.windows);
+    const P2Plan q = plan_pass2(it.est_keys, it.windows, it.bank->n_reads, it.bank->max_len, k, it.split,
+                                b->prm.bins2);
+    const bool lds = q.ok && b->prm.placement != 1;
+    it.lds = lds;
+    if (!lds) continue;
+    it.nb2 = q.nb2;
+    it.nbr = q.nbr;
+    it.log2_rounds = q.log2_rounds;
+    it.split2 = q.split2;
+    it.chunks2 = q.chunks2;
+    it.n_blocks2 = q.n_blocks2;
+    it.rec_cap = q.rec_cap;
+    b->lds_cap = q.lds_cap;
+    auto grow = [&](DBuf& d, uint64_t bytes) -> int { return (d.p && d.bytes >= bytes) ? FB_OK : d.get(c, bytes); };
+    int r = grow(it.scratch, q.rec_cap * rw * 8);
+    if (!r) r = grow(it.gm, std::max<uint64_t>(q.threads2, 1) * 8);
+    if (!r) r = grow(it.bh, std::max<uint64_t>(q.n_blocks2 * q.nbr, 1) * 4);
+    if (!r) r = grow(it.runs, (q.nbr * kBinSlices + 1) * 4);
+    if (!r) r = grow(it.slots, q.spill_cap * stride * 8);
+    it.spill_cap = q.spill_cap;
+    if (!r && q.out_cap > it.out_cap) {
+      r = it.keys.get(c, q.out_cap * nref * 8);
+      if (!r) r = it.cov.get(c, q.out_cap * 2);
+      if (!r) r = it.mask.get(c, q.out_cap);
+      it.out_cap = q.out_cap;
+    }
+    if (r) return r;
+    DevSolidTable t;
     t.slots = it.slots.as<uint64_t>();
-    t.cap = it.tab_cap;
+    t.cap = it.spill_cap;
     t.kw = kw;
     t.stride = stride;
     t.max_probe = 4096;
-    t.status = b->d_status.as<uint32_t>() + 4 * i;
-    Pass2Args& a2 = p2[i];
-    a2.reads = rv;
+    t.status = it.st2.as<uint32_t>() + 8;  // [8] spill overflow, [9] claimed
+    P2BinArgs pa;
+    pa.reads = it.bank->view();
+    pa.k = k;
+    pa.split = it.split2;
+    pa.chunks = it.chunks2;
+    pa.blk_hist = it.bh.as<uint32_t>();
+    pa.n_blocks = (uint32_t)it.n_blocks2;
+    pa.solid_bits = it.bits.as<uint32_t>();
+    pa.log2w = it.cms.log2w;
+    pa.nbins = (uint32_t)it.nbr;
+    pa.log2_nbins = (uint32_t)__builtin_ctzll(it.nbr);
+    pa.log2_rounds = it.log2_rounds;
+    pa.gate_masks = it.gm.as<uint64_t>();
+    pa.rw = rw;
+    pa.runs = it.runs.as<uint32_t>();
+    pa.recs = it.scratch.as<uint64_t>();
+    pa.rec_cap = it.rec_cap;
+    pa.status = it.st2.as<uint32_t>();          // [0] record overflow, [1] records
+    pa.out_count = it.st2.as<uint32_t>() + 2;   // [2] keys emitted from LDS
+    pa.stats = reinterpret_cast<unsigned long long*>(it.st2.as<uint32_t>() + 4);  // [4..7]
+    pa.lds_cap = q.lds_cap;
+    pa.min_trans_cov = gate;
+    pa.table = t;
+    pa.nref = nref;
+    pa.out_keys = it.keys.as<uint64_t>();
+    pa.out_cov = it.cov.as<uint16_t>();
+    pa.out_mask = it.mask.as<uint8_t>();
+    pa.out_cap = (uint32_t)it.out_cap;
+    p2.push_back(pa);
+    P3Args q3;
+    q3.t = t;
+    q3.nref = nref;
+    q3.gate = gate;
+    q3.out_keys = pa.out_keys;
+    q3.out_cov = pa.out_cov;
+    q3.out_mask = pa.out_mask;
+    q3.scratch = it.p3.p;
+    q3.out_base = pa.out_count;
+    q3.out_cap = pa.out_cap;
+    p3.push_back(q3);
+    tabs.push_back(t);
+    fill2.push_back({it.st2.as<uint32_t>(), 8, 0u});  // [8..9] are cleared with the table
+    runs_fill.push_back({it.runs.as<uint32_t>(), it.nbr * kBinSlices + 1, 0u});
+    const uint64_t thr2 = it.bank->n_reads * (uint64_t)it.split2;
+    blkg.push_back(blkg.back() + (uint32_t)((thr2 + kP2GateBlockSize - 1) / kP2GateBlockSize));
+    blks.push_back(blks.back() + (uint32_t)it.n_blocks2);
+    blkb.push_back(blkb.back() + (uint32_t)it.nbr);
+    b->max_nbr = std::max<uint32_t>(b->max_nbr, (uint32_t)it.nbr);
+    b->max_rounds = std::max<uint32_t>(b->max_rounds, 1u << it.log2_rounds);
+    lds_idx.push_back(i);
+  }
+  b->lds_idx = lds_idx;
+  const uint32_t m = (uint32_t)lds_idx.size();
+  b->n_fill2 = m;
+  b->nref = nref;
+  const std::vector<uint64_t> sig = {m, blkg.back(), blks.back(), blkb.back(), b->max_nbr, b->max_rounds, b->lds_cap};
+  if (b->sig2[k] != sig) *changed = true;
+  b->sig2[k] = sig;
+  if (!m) return FB_OK;
+  hipStream_t s = c->stream;
+  HIP_TRY(c, hipMemcpyAsync(b->d_p2.p, p2.data(), m * sizeof(P2BinArgs), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_p3.p, p3.data(), m * sizeof(P3Args), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_tabs.p, tabs.data(), m * sizeof(DevSolidTable), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_fill2.p, fill2.data(), m * sizeof(FillRange), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_runs_fill.p, runs_fill.data(), m * sizeof(FillRange), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blkg.p, blkg.data(), (m + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blks.p, blks.data(), (m + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(b->d_blkb.p, blkb.data(), (m + 1) * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  b->mg = MultiLaunch{b->d_p2.p, b->d_blkg.as<uint32_t>(), m, blkg[m]};
+  b->ms2 = MultiLaunch{b->d_p2.p, b->d_blks.as<uint32_t>(), m, blks[m]};
+  b->mb = MultiLaunch{b->d_p2.p, b->d_blkb.as<uint32_t>(), m, blkb[m]};
+  return FB_OK;
+}
+
+// Graph 2: passes 2 and 3 of the LDS isolates.
+static int batch_enqueue2(fb_batch* b, uint32_t k) {
+  fb_ctx* c = b->ctx;
+  hipStream_t s = c->stream;
+  const uint32_t m = (uint32_t)b->lds_idx.size();
+  HIP_TRY(c, launch_fill_ranges(b->d_fill2.as<FillRange>(), m, s));
+  HIP_TRY(c, launch_table_clear_multi(b->d_tabs.as<DevSolidTable>(), m, s));
+  HIP_TRY(c, launch_p2_binned_multi(b->mg, b->ms2, b->mb, b->d_runs_fill.as<FillRange>(), m, k, b->max_nbr,
+                                    b->max_rounds, b->lds_cap, s));
+  HIP_TRY(c, launch_pass3_multi(b->d_p3.as<P3Args>(), m, key_words_for_k(k), s));
+  return FB_OK;
+}
+
+// Captures (when absent) and launches the graph of one phase.
+template <class F>
+static int run_graph(fb_batch* b, std::map<uint32_t, hipGraphExec_t>& ge, std::map<uint32_t, hipGraph_t>& gd,
+                     uint32_t k, bool changed, F&& enqueue) {
+  fb_ctx* c = b->ctx;
+  hipStream_t s = c->stream;
+  if (changed) b->drop(ge, gd, k);
+  if (ge.find(k) == ge.end()) {
+    hipGraph_t graph = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int r = enqueue();
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (r) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    HIP_TRY(c, e);
+    hipGraphExec_t exec = nullptr;
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(c, ei);
+    }
+    ge[k] = exec;
+    gd[k] = graph;
+  }
+  HIP_TRY(c, hipGraphLaunch(ge[k], s));
+  return FB_OK;
+}
+
+// Pass 2 and 3 of one isolate on the global table, grown until it fits.
+static int batch_global_route(fb_ctx* c, BatchIso& it, uint32_t k, uint32_t gate) {
+  hipStream_t s = c->stream;
+  const uint32_t kw = key_words_for_k(k), stride = table_stride(kw), nref = ref_words(k);
+  uint64_t cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * it.est_keys) + 1024));
+  DBuf tbuf, status;
+  if (int r = status.get(c, 16)) return r;
+  DevSolidTable t;
+  t.kw = kw;
+  t.stride = stride;
+  t.max_probe = 4096;
+  t.status = status.as<uint32_t>();
+  uint32_t st[2] = {0, 0};
+  for (int attempt = 0;; ++attempt) {
+    t.cap = cap;
+    if (int r = tbuf.get(c, cap * stride * 8)) return r;
+    t.slots = tbuf.as<uint64_t>();
+    HIP_TRY(c, launch_table_clear(t, s));
+    Pass2Args a2{};
+    a2.reads = it.bank->view();
     a2.k = k;
     a2.split = it.split;
     a2.solid_bits = it.bits.as<uint32_t>();
     a2.log2w = it.cms.log2w;
     a2.table = t;
     a2.min_trans_cov = gate;
-    a2.contig_mode = 0;
-    P3Args& q = p3[i];
-    q.t = t;
-    q.nref = ref_words(k);
-    q.gate = gate;
-    q.out_keys = it.keys.as<uint64_t>();
-    q.out_cov = it.cov.as<uint16_t>();
-    q.out_mask = it.mask.as<uint8_t>();
-    q.scratch = it.p3.p;
-    const uint64_t thr = it.bank->n_reads * (uint64_t)it.split;
-    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(thr, bi.chunks, kP1BlockSize)
-                                              : (thr + kPassBlock - 1) / kPassBlock);
-    blkc[i + 1] = blkc[i] + (binned ? it.nbins : 0u);
-    blk2[i + 1] = blk2[i] + (uint32_t)((thr + kPassBlock - 1) / kPassBlock);
+    HIP_TRY(c, launch_pass2(a2, s));
+    HIP_TRY(c, hipMemcpyAsync(st, t.status, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (!st[0]) break;
+    if (attempt >= 6) return set_err(c, FB_ERR_OVERFLOW, "solid table overflow at %llu slots", (unsigned long long)cap);
+    cap *= 2;
   }
-  if (b->any_binned && !b->all_binned)
-    return set_err(c, FB_ERR_INVALID, "a batch needs one pass-1 placement for all isolates (widths differ too much)");
-  if (fills.size() > 4 * (size_t)n) return set_err(c, FB_ERR_INVALID, "internal: fill ranges");
-  hipStream_t s = c->stream;
-  HIP_TRY(c, hipMemcpyAsync(b->d_p1.p, p1.data(), n * sizeof(Pass1Args), hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_bin.p, ba.data(), n * sizeof(BinArgs), hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_p2.p, p2.data(), n * sizeof(Pass2Args), hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_p3.p, p3.data(), n * sizeof(P3Args), hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_tabs.p, tabs.data(), n * sizeof(DevSolidTable), hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_fill.p, fills.data(), fills.size() * sizeof(FillRange), hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_blk1.p, blk1.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_blkc.p, blkc.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipMemcpyAsync(b->d_blk2.p, blk2.data(), (n + 1) * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(c, hipStreamSynchronize(s));  // host vectors go out of scope
-  b->m1 = MultiLaunch{b->d_p1.p, b->d_blk1.as<uint32_t>(), n, blk1[n]};
-  if (b->any_binned) b->m1.args = b->d_bin.p;
-  b->mc = MultiLaunch{b->d_bin.p, b->d_blkc.as<uint32_t>(), n, blkc[n]};
-  b->m2 = MultiLaunch{b->d_p2.p, b->d_blk2.as<uint32_t>(), n, blk2[n]};
-  b->n_fill = (uint32_t)fills.size();
-  b->nref = ref_words(k);
-  return FB_OK;
-}
-
-// The captured sequence: clears, pass 1, gate bits, table clear, pass 2, pass 3.
-static int batch_enqueue(fb_batch* b, uint32_t k, uint32_t t_solid) {
-  fb_ctx* c = b->ctx;
-  hipStream_t s = c->stream;
-  const uint32_t n = (uint32_t)b->iso.size();
-  const bool hll = b->prm.hll_p != 0;
-  HIP_TRY(c, launch_fill_ranges(b->d_fill.as<FillRange>(), b->n_fill, s));
-  if (b->any_binned) {
-    HIP_TRY(c, launch_bin_scatter_multi(b->m1, k, hll, b->max_nbins, s));
-    HIP_TRY(c, launch_bin_count_multi(b->mc, b->max_sb, s));
-  } else {
-    HIP_TRY(c, launch_pass1_multi(b->m1, k, true, hll, s));
-    for (uint32_t i = 0; i < n; ++i) {
-      BatchIso& it = *b->iso[i];
-      HIP_TRY(c, launch_solid_bits(it.cms.counts.as<uint32_t>(), it.cms.width * 4, t_solid, it.cms.cap(),
-                                   it.bits.as<uint32_t>(), s));
-    }
+  const uint64_t maxn = std::max<uint64_t>(st[1], 1);
+  if (maxn > it.out_cap || !it.keys.p) {
+    if (int r = it.keys.get(c, maxn * nref * 8)) return r;
+    if (int r = it.cov.get(c, maxn * 2)) return r;
+    if (int r = it.mask.get(c, maxn)) return r;
+    it.out_cap = maxn;
   }
-  HIP_TRY(c, launch_table_clear_multi(b->d_tabs.as<DevSolidTable>(), n, s));
-  HIP_TRY(c, launch_pass2_multi(b->m2, k, s));
-  HIP_TRY(c, launch_pass3_multi(b->d_p3.as<P3Args>(), n, key_words_for_k(k), s));
+  HIP_TRY(c, launch_pass3(t, k, nref, gate, it.keys.as<uint64_t>(), it.cov.as<uint16_t>(), it.mask.as<uint8_t>(),
+                          it.p3.p, s));
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(cnt, it.p3.p, 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  it.n_out = cnt[0];
   return FB_OK;
 }
 
@@ -1383,106 +1635,70 @@ extern "C" int fb_batch_run(fb_batch* b, uint32_t k, int32_t target) {
   const uint32_t n = (uint32_t)b->iso.size();
   for (auto& it : b->iso)
     if (int r = check_add_budget(c, fb_bank_bases_for_k(it->bank, k))) return r;
-  bool changed = false;
-  if (int r = batch_prepare(b, k, t_solid, gate, &changed)) return r;
-  // graph per k; re-captured when buffers moved
-  auto g = b->graphs.find(k);
-  if (changed && g != b->graphs.end()) {
-    (void)hipGraphExecDestroy(g->second);
-    (void)hipGraphDestroy(b->graph_defs[k]);
-    b->graphs.erase(g);
-    b->graph_defs.erase(k);
-    g = b->graphs.end();
-  }
-  if (g == b->graphs.end()) {
-    hipGraph_t graph = nullptr;
-    HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int r = batch_enqueue(b, k, t_solid);
-    hipError_t e = hipStreamEndCapture(s, &graph);
-    if (r) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return r;
-    }
-    HIP_TRY(c, e);
-    hipGraphExec_t exec = nullptr;
-    HIP_TRY(c, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    b->graphs[k] = exec;
-    b->graph_defs[k] = graph;
-  }
   HIP_TRY(c, hipEventRecord(c->ev[0], s));
-  HIP_TRY(c, hipGraphLaunch(b->graphs[k], s));
-  HIP_TRY(c, hipEventRecord(c->ev[1], s));
-  std::vector<uint32_t> st(4 * n);
-  HIP_TRY(c, hipMemcpyAsync(st.data(), b->d_status.p, 16ull * n, hipMemcpyDeviceToHost, s));
+  bool changed = false;
+  if (int r = batch_prepare1(b, k, t_solid, &changed)) return r;
+  if (int r = run_graph(b, b->g1, b->g1_def, k, changed, [&] { return batch_enqueue1(b, k, t_solid); })) return r;
+  std::vector<uint64_t> occ(n, 0);
+  for (uint32_t i = 0; i < n; ++i)
+    HIP_TRY(c, hipMemcpyAsync(&occ[i], b->iso[i]->occ.p, 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
-  // isolates whose table overflowed: double the table and re-run passes 2-3 alone
+  bool changed2 = false;
+  if (int r = batch_prepare2(b, k, gate, occ, &changed2)) return r;
+  if (!b->lds_idx.empty())
+    if (int r = run_graph(b, b->g2, b->g2_def, k, changed2, [&] { return batch_enqueue2(b, k); })) return r;
+  HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  // results of the LDS isolates; the others, and any that overflowed, take the global route
+  std::vector<uint32_t> st(16 * (size_t)n, 0), p3c(4 * (size_t)n, 0);
+  for (uint32_t i : b->lds_idx) {
+    HIP_TRY(c, hipMemcpyAsync(st.data() + 16 * i, b->iso[i]->st2.p, 64, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(p3c.data() + 4 * i, b->iso[i]->p3.p, 16, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(c, hipStreamSynchronize(s));
   HIP_TRY(c, hipEventRecord(c->ev[2], s));
-  const uint32_t kw = key_words_for_k(k), stride = table_stride(kw);
   for (uint32_t i = 0; i < n; ++i) {
     BatchIso& it = *b->iso[i];
-    it.overflow = st[4 * i] != 0;
-    for (int attempt = 0; it.overflow; ++attempt) {
-      if (attempt >= 6) return set_err(c, FB_ERR_OVERFLOW, "isolate %u: solid table overflow", i);
-      it.tab_cap *= 2;
-      if (int r = it.slots.get(c, it.tab_cap * stride * 8)) return r;
-      if (int r = it.keys.get(c, it.tab_cap * ref_words(k) * 8)) return r;
-      if (int r = it.cov.get(c, it.tab_cap * 2)) return r;
-      if (int r = it.mask.get(c, it.tab_cap)) return r;
-      DevSolidTable t;
-      t.slots = it.slots.as<uint64_t>();
-      t.cap = it.tab_cap;
-      t.kw = kw;
-      t.stride = stride;
-      t.max_probe = 4096;
-      t.status = b->d_status.as<uint32_t>() + 4 * i;
-      HIP_TRY(c, launch_table_clear(t, s));
-      Pass2Args a2{};
-      a2.reads = it.bank->view();
-      a2.k = k;
-      a2.split = it.split;
-      a2.solid_bits = it.bits.as<uint32_t>();
-      a2.log2w = it.cms.log2w;
-      a2.table = t;
-      a2.min_trans_cov = gate;
-      HIP_TRY(c, launch_pass2(a2, s));
-      HIP_TRY(c, launch_pass3(t, k, ref_words(k), gate, it.keys.as<uint64_t>(), it.cov.as<uint16_t>(),
-                              it.mask.as<uint8_t>(), it.p3.p, s));
-      uint32_t one[4];
-      HIP_TRY(c, hipMemcpyAsync(one, t.status, 16, hipMemcpyDeviceToHost, s));
-      HIP_TRY(c, hipStreamSynchronize(s));
-      it.overflow = one[0] != 0;
-      it.tab_stride = stride;
+    if (it.lds) {
+      const uint32_t* q = st.data() + 16 * i;
+      const uint64_t n_out = (uint64_t)q[2] + p3c[4 * i];
+      if (!q[0] && !q[8] && n_out <= it.out_cap) {
+        it.n_out = n_out;
+        it.route = 1;
+        continue;
+      }
+      it.route = q[0] ? 2 : (q[8] ? 3 : 4);
+    } else {
+      it.route = 0;
     }
+    if (it.windows == 0 && !it.lds) {
+      it.n_out = 0;
+      continue;
+    }
+    if (int r = batch_global_route(c, it, k, gate)) return r;
   }
   HIP_TRY(c, hipEventRecord(c->ev[3], s));
-  // per-isolate record counts, then sorting
-  std::vector<uint32_t> counts(n);
-  for (uint32_t i = 0; i < n; ++i)
-    HIP_TRY(c, hipMemcpyAsync(&counts[i], b->iso[i]->p3.p, 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
   HIP_TRY(c, hipEventRecord(c->ev[4], s));
   if (b->prm.sort) {
     size_t need = 0;
-    for (uint32_t i = 0; i < n; ++i) {
+    for (auto& it : b->iso) {
       size_t tb = 0;
-      HIP_TRY(c, sort_solid(nullptr, nullptr, nullptr, counts[i], ref_words(k), k, nullptr, &tb, s));
+      HIP_TRY(c, sort_solid(nullptr, nullptr, nullptr, (uint32_t)it->n_out, ref_words(k), k, nullptr, &tb, s));
       need = std::max(need, tb);
     }
     DBuf tmp;
     if (int r = tmp.get(c, need)) return r;
-    for (uint32_t i = 0; i < n; ++i) {
-      BatchIso& it = *b->iso[i];
-      if (counts[i] > 1) {
+    for (auto& it : b->iso) {
+      if (it->n_out > 1) {
         size_t tb = need;
-        HIP_TRY(c, sort_solid(it.keys.as<uint64_t>(), it.cov.as<uint16_t>(), it.mask.as<uint8_t>(), counts[i],
-                              ref_words(k), k, tmp.p, &tb, s));
+        HIP_TRY(c, sort_solid(it->keys.as<uint64_t>(), it->cov.as<uint16_t>(), it->mask.as<uint8_t>(),
+                              (uint32_t)it->n_out, ref_words(k), k, tmp.p, &tb, s));
       }
     }
   }
   HIP_TRY(c, hipEventRecord(c->ev[5], s));
   HIP_TRY(c, hipEventSynchronize(c->ev[5]));
-  for (uint32_t i = 0; i < n; ++i) b->iso[i]->n_out = counts[i];
   b->k = k;
+  b->nref = ref_words(k);
   (void)hipEventElapsedTime(&b->ms[0], c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&b->ms[1], c->ev[2], c->ev[3]);
   (void)hipEventElapsedTime(&b->ms[2], c->ev[4], c->ev[5]);
@@ -1520,5 +1736,10 @@ extern "C" int fb_batch_hll_registers(const fb_batch* b, uint32_t i, uint8_t* ou
 extern "C" int fb_batch_timings(const fb_batch* b, float out[4]) {
   if (!b || !out) return FB_ERR_INVALID;
   memcpy(out, b->ms, sizeof b->ms);
+  return FB_OK;
+}
+extern "C" int fb_batch_routes(const fb_batch* b, uint32_t* out) {
+  if (!b || !out) return FB_ERR_INVALID;
+  for (size_t i = 0; i < b->iso.size(); ++i) out[i] = b->iso[i]->route;
   return FB_OK;
 }

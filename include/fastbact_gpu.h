@@ -166,7 +166,8 @@ int fb_solid_kernel_timings(const fb_solid* s, float out[5]);
 /* Work counts of the build: [0] k-mer windows of pass 1, [1] pass-2 records
  * (sum of raw cov), [2] keys in the solid table, [3] table slots, [4] pass-2
  * route (0 global table, 1 LDS bins, 2 LDS bins abandoned for a full record
- * buffer, 3 abandoned for a full spill table), [5] pass-2 LDS bins over all
+ * buffer, 3 abandoned for a full spill table, 4 abandoned for a full output
+ * buffer; the abandoned routes finish on the global table), [5] pass-2 LDS bins over all
  * rounds, [6] keys per LDS bin table, [7] keys estimated from the pass-1
  * gate bits, [8] records the LDS route wrote (one per window with an edge,
  * 0 on another route), [9] reads of length >= k, [10] pass-2 rounds (the
@@ -179,10 +180,13 @@ int fb_solid_stats(const fb_solid* s, uint64_t out[11]);
  * holds n isolates' banks and per-isolate sketches and tables; one
  * fb_batch_run executes passes 1-3 for every isolate with one launch per
  * kernel (blocks map to isolates through a prefix table, no tile crosses
- * isolates).  The pass is captured as a HIP graph on its first run for a
- * given k and replayed afterwards.  Per-isolate results equal
- * fb_build_solid's on the same isolate.  genome_sizes[i] sizes isolate i's CMS
- * when params->cms_width is 0 (build.rs:74-84); params->k is ignored. */
+ * isolates).  A run is two HIP graphs, captured on the first run at a k and
+ * replayed while the launch shapes hold: pass 1 with the gate occupancy of
+ * every isolate, then (each isolate's pass 2 sized from its occupancy) passes
+ * 2 and 3 on LDS bins.  An isolate whose LDS route overflows re-runs pass 2
+ * alone on a global table.  Per-isolate results equal fb_build_solid's on
+ * the same isolate.  genome_sizes[i] sizes isolate i's CMS when
+ * params->cms_width is 0 (build.rs:74-84); params->k is ignored. */
 typedef struct fb_batch fb_batch;
 int fb_batch_create(fb_ctx* ctx, const fb_bank* const* banks, const uint64_t* genome_sizes, uint32_t n,
                     const fb_build_params* params, fb_batch** out);
@@ -199,9 +203,12 @@ int fb_batch_download(const fb_batch* b, uint32_t iso, uint64_t* keys, uint16_t*
 uint64_t fb_batch_cms_bytes(const fb_batch* b, uint32_t iso);
 int fb_batch_cms_download(const fb_batch* b, uint32_t iso, void* out, uint64_t bytes);
 int fb_batch_hll_registers(const fb_batch* b, uint32_t iso, uint8_t* out);
-/* Last run, milliseconds: [0] graph (passes 1-3, all isolates), [1] growth
- * re-runs, [2] sort, [3] whole run. */
+/* Last run, milliseconds: [0] graphs (passes 1-3, all isolates), [1]
+ * global-table re-runs, [2] sort, [3] whole run. */
 int fb_batch_timings(const fb_batch* b, float out[4]);
+/* Pass-2 route of every isolate in the last run (n entries, codes as
+ * fb_solid_stats[4]).  Diagnostics, no counterpart in the reference. */
+int fb_batch_routes(const fb_batch* b, uint32_t* out);
 
 /* ---- Genome estimator (estimator.rs) -------------------------------------- */
 typedef struct {

@@ -65,3 +65,45 @@ def test_batch_small_table_grows_exactly(ctx, orc, isolates):
         ref = orc.build_solid(oracle_arenas(b), 31, target_solid_cov=1, width=1 << 15, counter_bits=16)
         got = batch.result(i, 31)
         assert np.array_equal(got.keys, ref["keys"]) and np.array_equal(got.cov, ref["cov"])
+
+
+@pytest.mark.parametrize("k,bins2", [(21, 0), (55, 1 << 15), (99, 1 << 16)])
+def test_batch_lds_route_in_rounds_equals_oracle(ctx, orc, isolates, k, bins2):
+    """Passes 2-3 of every isolate on LDS bins in one launch per kernel,
+    including pass-2 rounds; the replayed graphs give the same sets."""
+    banks = [_bank(x) for x in isolates]
+    dev = [F.DeviceBank(ctx, b) for b in banks]
+    width = 1 << 16
+    batch = F.Batch(ctx, dev, [50_000] * len(banks), cms_width=width, counter_bits=16, hll_p=12, placement=2,
+                    bins2=bins2)
+    refs = [orc.build_solid(oracle_arenas(b), k, target_solid_cov=2, width=width, counter_bits=16, hll_p=12)
+            for b in banks]
+    for rep in range(3):
+        batch.run(k, 2)
+        routes = batch.routes()
+        for i, b in enumerate(banks):
+            got = batch.result(i, k)
+            assert np.array_equal(got.keys, refs[i]["keys"]), (rep, i)
+            assert np.array_equal(got.cov, refs[i]["cov"]) and np.array_equal(got.mask, refs[i]["mask"])
+            assert np.array_equal(got.hll_registers, refs[i]["hll"])
+            assert routes[i] == ("lds" if b.total_bases() else "global"), (i, routes)
+
+
+def test_batch_matches_plain_path_on_larger_isolates(ctx):
+    """Uneven isolates whose keys need several pass-2 rounds at a small
+    sketch: batch results equal fb_build_solid's, isolate by isolate."""
+    isos = [isolate(seed=300 + i, genome_len=g, n_pairs=n, gc=gc)
+            for i, (g, n, gc) in enumerate([(200_000, 40_000, 0.4), (90_000, 12_000, 0.55), (300_000, 60_000, 0.62)])]
+    dev = [F.DeviceBank(ctx, x.bank()) for x in isos]
+    batch = F.Batch(ctx, dev, [100_000] * 3, cms_width=1 << 17, counter_bits=16, hll_p=14)
+    for k, t in ((31, None), (77, 4)):
+        batch.run(k, t)
+        for i, d in enumerate(dev):
+            single = F.build_solid_kmers(ctx, d, k, target_solid_cov=t, cms_width=1 << 17, counter_bits=16,
+                                         hll_p=14)
+            got = batch.result(i, k)
+            assert np.array_equal(got.keys, single.keys) and np.array_equal(got.cov, single.cov)
+            assert np.array_equal(got.mask, single.mask)
+            assert np.array_equal(got.cms_table, single.cms_table)
+            assert np.array_equal(got.hll_registers, single.hll_registers)
+        assert batch.routes() == ["lds"] * 3

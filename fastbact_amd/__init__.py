@@ -27,4 +27,6 @@ from .api import (  # noqa: F401
     synth_params,
 )
 
+from .serve import Admission, Server, ServeOutcome, serve_params  # noqa: F401,E402
+
 __version__ = "0.1.0"

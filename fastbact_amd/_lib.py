@@ -18,6 +18,7 @@ FB_ERR_HIP = -4
 FB_ERR_EMPTY = -5
 FB_ERR_NODEV = -6
 FB_ERR_BUSY = -7
+FB_ERR_TIMEOUT = -8
 
 ERR_NAMES = {
     FB_ERR_INVALID: "FB_ERR_INVALID",
@@ -27,6 +28,7 @@ ERR_NAMES = {
     FB_ERR_EMPTY: "FB_ERR_EMPTY",
     FB_ERR_NODEV: "FB_ERR_NODEV",
     FB_ERR_BUSY: "FB_ERR_BUSY",
+    FB_ERR_TIMEOUT: "FB_ERR_TIMEOUT",
 }
 
 u8p = C.POINTER(C.c_uint8)
@@ -115,6 +117,60 @@ class SynthParams(C.Structure):
     ]
 
 
+class ServeParams(C.Structure):
+    _fields_ = [
+        ("n_passes", C.c_uint32),
+        ("k", C.c_uint32 * 4),
+        ("target_solid_cov", C.c_int32 * 4),
+        ("cms_width", C.c_uint64),
+        ("counter_bits", C.c_int32),
+        ("hll_p", C.c_int32),
+        ("estimate", C.c_uint32),
+        ("est_max_pairs", C.c_uint64),
+        ("target_cov", C.c_double),
+        ("hbm_budget", C.c_uint64),
+        ("max_inflight", C.c_uint32),
+        ("max_queue", C.c_uint32),
+        ("keep_sketches", C.c_uint32),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("error", C.c_char * 160),
+        ("n_passes", C.c_uint32),
+        ("genome_size", C.c_uint64),
+        ("has_estimate", C.c_uint32),
+        ("estimate", GenomeEstimateC),
+        ("n_solid", C.c_uint64 * 4),
+        ("route", C.c_uint32 * 4),
+        ("bases", C.c_uint64),
+        ("footprint", C.c_uint64),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("queued", C.c_uint64),
+        ("inflight", C.c_uint64),
+        ("iterations", C.c_uint64),
+        ("pass_runs", C.c_uint64),
+        ("max_batch", C.c_uint64),
+        ("max_groups", C.c_uint64),
+        ("budget", C.c_uint64),
+        ("inflight_bytes", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("busy_s", C.c_double),
+    ]
+
+
 # (name, restype, argtypes): every symbol include/fastbact_gpu.h and
 # include/fastbact_synth.h declare.
 SIGNATURES = [
@@ -173,6 +229,24 @@ SIGNATURES = [
     ("fb_batch_hll_registers", C.c_int, [vp, C.c_uint32, u8p]),
     ("fb_batch_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_batch_routes", C.c_int, [vp, u32p]),
+    ("fb_serve_params_default", None, [C.POINTER(ServeParams)]),
+    ("fb_server_create", C.c_int, [C.c_int, C.POINTER(ServeParams), C.POINTER(vp)]),
+    ("fb_server_destroy", None, [vp]),
+    ("fb_server_last_error", C.c_char_p, [vp]),
+    ("fb_server_submit", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.c_uint64, u64p]),
+    ("fb_server_wait", C.c_int, [vp, C.c_uint64, C.c_double, C.POINTER(ServeResult)]),
+    ("fb_server_download", C.c_int, [vp, C.c_uint64, C.c_uint32, u64p, u16p, u8p]),
+    ("fb_server_sketches", C.c_int, [vp, C.c_uint64, C.c_uint32, vp, C.c_uint64, u8p]),
+    ("fb_server_release", C.c_int, [vp, C.c_uint64]),
+    ("fb_server_stats", C.c_int, [vp, C.POINTER(ServeStats)]),
+    ("fb_admission_create", C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(vp)]),
+    ("fb_admission_destroy", None, [vp]),
+    ("fb_admission_submit", C.c_int, [vp, C.c_uint64, C.c_uint64]),
+    ("fb_admission_admit", C.c_uint32, [vp, u64p, C.c_uint32]),
+    ("fb_admission_retire", C.c_int, [vp, C.c_uint64]),
+    ("fb_admission_used", C.c_uint64, [vp]),
+    ("fb_admission_queued", C.c_uint32, [vp]),
+    ("fb_admission_inflight", C.c_uint32, [vp]),
     ("fb_count_stats_from_lengths", C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(CountStats)]),
     ("fb_compute_dynamic_kmers", C.c_uint32, [C.c_uint64, C.c_int64, u32p]),
     ("fb_estimate_genome", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.POINTER(CountStats),

@@ -32,6 +32,63 @@ uint8_t code_of(uint8_t c) {
 
 extern "C" uint64_t fb_splitmix64(uint64_t z) { return sm64(z); }
 
+// ---------------------------------------------------------------------------
+// Admission by footprint (serving)
+
+namespace fb {
+
+int Admission::submit(uint64_t id, uint64_t bytes) {
+  if (bytes > budget_) return FB_ERR_OOM;
+  queue_.emplace_back(id, bytes);
+  return FB_OK;
+}
+
+void Admission::admit(std::vector<uint64_t>& out) {
+  while (!queue_.empty() && inflight_.size() < max_inflight_ && used_ + queue_.front().second <= budget_) {
+    const auto q = queue_.front();
+    queue_.pop_front();
+    inflight_[q.first] = q.second;
+    used_ += q.second;
+    out.push_back(q.first);
+  }
+}
+
+int Admission::retire(uint64_t id) {
+  auto it = inflight_.find(id);
+  if (it == inflight_.end()) return FB_ERR_INVALID;
+  used_ -= it->second;
+  inflight_.erase(it);
+  return FB_OK;
+}
+
+}  // namespace fb
+
+struct fb_admission {
+  fb::Admission a;
+};
+
+extern "C" int fb_admission_create(uint64_t budget, uint32_t max_inflight, fb_admission** out) {
+  if (!out || max_inflight == 0) return FB_ERR_INVALID;
+  *out = new fb_admission{fb::Admission(budget, max_inflight)};
+  return FB_OK;
+}
+extern "C" void fb_admission_destroy(fb_admission* q) { delete q; }
+extern "C" int fb_admission_submit(fb_admission* q, uint64_t id, uint64_t bytes) {
+  return q ? q->a.submit(id, bytes) : FB_ERR_INVALID;
+}
+extern "C" uint32_t fb_admission_admit(fb_admission* q, uint64_t* ids, uint32_t max_ids) {
+  if (!q) return 0;
+  std::vector<uint64_t> got;
+  q->a.admit(got);
+  const uint32_t n = (uint32_t)got.size();
+  for (uint32_t i = 0; i < n && i < max_ids; ++i) ids[i] = got[i];
+  return n;
+}
+extern "C" int fb_admission_retire(fb_admission* q, uint64_t id) { return q ? q->a.retire(id) : FB_ERR_INVALID; }
+extern "C" uint64_t fb_admission_used(const fb_admission* q) { return q ? q->a.used() : 0; }
+extern "C" uint32_t fb_admission_queued(const fb_admission* q) { return q ? (uint32_t)q->a.queued() : 0; }
+extern "C" uint32_t fb_admission_inflight(const fb_admission* q) { return q ? (uint32_t)q->a.inflight() : 0; }
+
 extern "C" const char* fb_version(void) { return "fastbact-mi355x 0.1.0 (gfx950)"; }
 
 // sketch.rs:165-168

@@ -9,7 +9,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <map>
+#include <thread>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1144,6 +1147,8 @@ struct fb_batch {
   std::map<uint32_t, hipGraphExec_t> g1, g2;
   std::map<uint32_t, hipGraph_t> g1_def, g2_def;
   std::map<uint32_t, std::vector<uint64_t>> sig1, sig2;
+  bool graphs = true;   // false: launch directly (membership changes every run)
+  size_t cap_iso = 0;   // isolates the argument arrays hold
   float ms[4] = {0, 0, 0, 0};
   void drop(std::map<uint32_t, hipGraphExec_t>& ge, std::map<uint32_t, hipGraph_t>& gd, uint32_t kk) {
     auto e = ge.find(kk);
@@ -1175,15 +1180,15 @@ struct P2Plan {
   uint32_t log2_rounds = 0, split2 = 1, chunks2 = 1, lds_cap = 0;
   bool ok = false;
 };
+// lds_cap: LDS table entries of one bin; round_bins: bins one round takes.
 static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint32_t max_len, uint32_t k,
-                         uint32_t split, uint32_t bins2) {
+                         uint32_t split, uint32_t bins2, uint32_t lds_cap, uint32_t round_bins) {
   P2Plan q;
-  const uint32_t kw = key_words_for_k(k);
-  q.lds_cap = p2_lds_cap(kw);
+  q.lds_cap = lds_cap;
   while ((double)q.nb2 * 0.5 * (double)q.lds_cap < n_est) q.nb2 <<= 1;
-  const uint64_t max_bins = (uint64_t)p2_max_bins() << kP2MaxLog2Rounds;
+  const uint64_t max_bins = (uint64_t)round_bins << kP2MaxLog2Rounds;
   if (bins2) q.nb2 = bins2;
-  while ((q.nb2 >> q.log2_rounds) > p2_max_bins()) ++q.log2_rounds;
+  while ((q.nb2 >> q.log2_rounds) > round_bins) ++q.log2_rounds;
   q.nbr = q.nb2 >> q.log2_rounds;
   const uint32_t pairs_max = max_len > k ? max_len - k : 1;
   q.split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
@@ -1218,7 +1223,10 @@ extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uin
   // keys of pass 2: at most the windows, and the linear-counting estimate
   // stays below 8 keys per row-0 cell for the sketches in use
   const double n_est = (double)std::min<uint64_t>(windows, 8 * W);
-  const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2);
+  // host-only sizing: the LDS of a workgroup with the full 160 KB granted
+  const uint32_t kw = key_words_for_k(k);
+  const uint32_t lds_cap = (kP2LdsBytes - 1024u) / (kw * 8u + 20u);
+  const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u);
   const uint32_t rw = (2 * k + 8 + 63) / 64;
   uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;            // exact counts + gate bits
   b += p->hll_p ? (4ull << p->hll_p) : 0;                     // HLL registers
@@ -1229,6 +1237,54 @@ extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uin
   b += q.out_cap * (ref_words(k) * 8 + 3);                    // solid set
   b += bases / 4 + reads * 12 + 4096;                         // the bank itself
   return b;
+}
+
+// Sketches and fixed buffers of one isolate of a batch.
+static int batch_iso_init(fb_ctx* c, BatchIso* it, const fb_bank* bank, uint64_t genome_size,
+                          const fb_build_params* prm) {
+  it->bank = bank;
+  it->est_g = std::max<uint64_t>(genome_size, 50000);
+  if (int r = cms_init(c, &it->cms, batch_cms_width(prm, it->est_g), prm->counter_bits)) return r;
+  if (prm->hll_p) {
+    if (int r = hll_init(c, &it->hll, prm->hll_p)) return r;
+    it->has_hll = true;
+  }
+  if (int r = it->bits.get(c, ((it->cms.width * 4 + 31) / 32) * 4)) return r;
+  if (int r = it->occ.get(c, 8)) return r;
+  if (int r = it->p3.get(c, pass3_scratch_bytes())) return r;
+  if (int r = it->st2.get(c, 64)) return r;
+  return FB_OK;
+}
+
+// Argument arrays for up to n isolates.
+static int batch_reserve(fb_batch* b, size_t n) {
+  if (n <= b->cap_iso) return FB_OK;
+  fb_ctx* c = b->ctx;
+  const size_t nn = std::max<size_t>(n, 2 * b->cap_iso);
+  if (int r = b->d_p1.get(c, nn * sizeof(Pass1Args))) return r;
+  if (int r = b->d_bin.get(c, nn * sizeof(BinArgs))) return r;
+  if (int r = b->d_fill1.get(c, nn * 4 * sizeof(FillRange))) return r;
+  if (int r = b->d_blk1.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blkc.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_p2.get(c, nn * sizeof(P2BinArgs))) return r;
+  if (int r = b->d_p3.get(c, nn * sizeof(P3Args))) return r;
+  if (int r = b->d_tabs.get(c, nn * sizeof(DevSolidTable))) return r;
+  if (int r = b->d_fill2.get(c, nn * sizeof(FillRange))) return r;
+  if (int r = b->d_runs_fill.get(c, nn * sizeof(FillRange))) return r;
+  if (int r = b->d_blkg.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blks.get(c, (nn + 1) * 4)) return r;
+  if (int r = b->d_blkb.get(c, (nn + 1) * 4)) return r;
+  b->cap_iso = nn;
+  // the arrays moved: every captured graph is stale
+  for (auto& g : b->g1) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : b->g2) (void)hipGraphExecDestroy(g.second);
+  for (auto& g : b->g1_def) (void)hipGraphDestroy(g.second);
+  for (auto& g : b->g2_def) (void)hipGraphDestroy(g.second);
+  b->g1.clear();
+  b->g2.clear();
+  b->g1_def.clear();
+  b->g2_def.clear();
+  return FB_OK;
 }
 
 extern "C" int fb_batch_create(fb_ctx* c, const fb_bank* const* banks, const uint64_t* genome_sizes, uint32_t n,
@@ -1248,33 +1304,10 @@ extern "C" int fb_batch_create(fb_ctx* c, const fb_bank* const* banks, const uin
   for (uint32_t i = 0; i < n; ++i) {
     if (!banks[i]) return set_err(c, FB_ERR_INVALID, "bank %u is null", i);
     auto it = std::make_unique<BatchIso>();
-    it->bank = banks[i];
-    it->est_g = std::max<uint64_t>(genome_sizes[i], 50000);
-    if (int r = cms_init(c, &it->cms, batch_cms_width(prm, it->est_g), prm->counter_bits)) return r;
-    if (prm->hll_p) {
-      if (int r = hll_init(c, &it->hll, prm->hll_p)) return r;
-      it->has_hll = true;
-    }
-    if (int r = it->bits.get(c, ((it->cms.width * 4 + 31) / 32) * 4)) return r;
-    if (int r = it->occ.get(c, 8)) return r;
-    if (int r = it->p3.get(c, pass3_scratch_bytes())) return r;
-    if (int r = it->st2.get(c, 64)) return r;
+    if (int r = batch_iso_init(c, it.get(), banks[i], genome_sizes[i], prm)) return r;
     b->iso.push_back(std::move(it));
   }
-  const size_t nn = n;
-  if (int r = b->d_p1.get(c, nn * sizeof(Pass1Args))) return r;
-  if (int r = b->d_bin.get(c, nn * sizeof(BinArgs))) return r;
-  if (int r = b->d_fill1.get(c, nn * 4 * sizeof(FillRange))) return r;
-  if (int r = b->d_blk1.get(c, (nn + 1) * 4)) return r;
-  if (int r = b->d_blkc.get(c, (nn + 1) * 4)) return r;
-  if (int r = b->d_p2.get(c, nn * sizeof(P2BinArgs))) return r;
-  if (int r = b->d_p3.get(c, nn * sizeof(P3Args))) return r;
-  if (int r = b->d_tabs.get(c, nn * sizeof(DevSolidTable))) return r;
-  if (int r = b->d_fill2.get(c, nn * sizeof(FillRange))) return r;
-  if (int r = b->d_runs_fill.get(c, nn * sizeof(FillRange))) return r;
-  if (int r = b->d_blkg.get(c, (nn + 1) * 4)) return r;
-  if (int r = b->d_blks.get(c, (nn + 1) * 4)) return r;
-  if (int r = b->d_blkb.get(c, (nn + 1) * 4)) return r;
+  if (int r = batch_reserve(b.get(), n)) return r;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   *out = b.release();
   return FB_OK;
@@ -1423,7 +1456,7 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     BatchIso& it = *b->iso[i];
     it.est_keys = est_keys_from_occ(occ[i], it.cms.width, it.windows);
     const P2Plan q = plan_pass2(it.est_keys, it.windows, it.bank->n_reads, it.bank->max_len, k, it.split,
-                                b->prm.bins2);
+                                b->prm.bins2, p2_lds_cap(kw), p2_max_bins());
     const bool lds = q.ok && b->prm.placement != 1;
     it.lds = lds;
     if (!lds) continue;
@@ -1550,6 +1583,7 @@ static int run_graph(fb_batch* b, std::map<uint32_t, hipGraphExec_t>& ge, std::m
                      uint32_t k, bool changed, F&& enqueue) {
   fb_ctx* c = b->ctx;
   hipStream_t s = c->stream;
+  if (!b->graphs) return enqueue();
   if (changed) b->drop(ge, gd, k);
   if (ge.find(k) == ge.end()) {
     hipGraph_t graph = nullptr;
@@ -1635,6 +1669,7 @@ extern "C" int fb_batch_run(fb_batch* b, uint32_t k, int32_t target) {
   const uint32_t n = (uint32_t)b->iso.size();
   for (auto& it : b->iso)
     if (int r = check_add_budget(c, fb_bank_bases_for_k(it->bank, k))) return r;
+  if (int r = batch_reserve(b, n)) return r;
   HIP_TRY(c, hipEventRecord(c->ev[0], s));
   bool changed = false;
   if (int r = batch_prepare1(b, k, t_solid, &changed)) return r;
@@ -1741,5 +1776,450 @@ extern "C" int fb_batch_timings(const fb_batch* b, float out[4]) {
 extern "C" int fb_batch_routes(const fb_batch* b, uint32_t* out) {
   if (!b || !out) return FB_ERR_INVALID;
   for (size_t i = 0; i < b->iso.size(); ++i) out[i] = b->iso[i]->route;
+  return FB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Serving: a request loop with continuous batching.  One worker thread owns
+// the server's context.  Every iteration it admits queued requests whose
+// device footprint fits the budget (fb::Admission), uploads their reads and
+// sizes their sketches (from the genome estimate when asked), then runs one
+// batched pass for every k that in-flight requests are at: requests at
+// different stages of their k ladder share the iteration, new requests join
+// at the next one, and a request that finishes its last pass leaves at once,
+// freeing its memory for the queue.  A request's sketches and buffers stay on
+// the device across its passes.
+
+struct ServePass {
+  std::vector<uint64_t> keys;
+  std::vector<uint16_t> cov;
+  std::vector<uint8_t> mask;
+  std::vector<uint8_t> cms, hll;  // with keep_sketches
+  uint32_t route = 0;
+};
+
+struct ServeReq {
+  uint64_t id = 0;
+  // host copies of R1 / R2 until the reads are on the device
+  std::vector<uint64_t> data[2], offs[2];
+  std::vector<uint32_t> lens[2];
+  uint64_t genome_hint = 0, bases = 0, reads = 0, footprint = 0;
+  uint32_t max_len = 0;
+  fb_bank* bank = nullptr;
+  std::unique_ptr<BatchIso> iso;
+  uint32_t stage = 0;
+  int state = 0;  // 0 queued, 1 in flight, 2 done, 3 failed
+  int status = FB_OK;
+  std::string err;
+  fb_genome_estimate est{};
+  bool has_est = false;
+  uint64_t genome_size = 0;
+  std::vector<ServePass> passes;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  ~ServeReq() {
+    iso.reset();
+    if (bank) fb_bank_destroy(bank);
+  }
+};
+
+struct fb_server {
+  fb_ctx* ctx = nullptr;
+  fb_serve_params prm{};
+  fb_build_params bprm{};
+  std::unique_ptr<fb::Admission> adm;
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::map<uint64_t, std::unique_ptr<ServeReq>> reqs;
+  std::vector<ServeReq*> running;  // worker-owned
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::string err;
+  std::chrono::steady_clock::time_point t0;
+  std::map<uint32_t, std::unique_ptr<fb_batch>> batches;  // per pass index, membership set every iteration
+  fb_serve_stats st{};
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+extern "C" void fb_serve_params_default(fb_serve_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->n_passes = 3;
+  p->k[0] = 21;
+  p->k[1] = 55;
+  p->k[2] = 77;
+  p->target_solid_cov[0] = -1;
+  p->target_solid_cov[1] = 4;
+  p->target_solid_cov[2] = 4;
+  p->counter_bits = 16;
+  p->hll_p = 14;
+  p->estimate = 1;
+  p->est_max_pairs = 250000;
+  p->target_cov = 100.0;
+}
+
+// Genome size a request is sized for before its estimate exists.
+static uint64_t serve_size_hint(const fb_server* sv, const ServeReq& r) {
+  if (r.genome_hint) return r.genome_hint;
+  // at >= 20x coverage the genome is at most bases / 20
+  return std::max<uint64_t>(r.bases / (sv->prm.estimate ? 20 : 60), 50000);
+}
+
+static uint64_t serve_footprint(const fb_server* sv, const ServeReq& r) {
+  uint64_t f = 0;
+  for (uint32_t j = 0; j < sv->prm.n_passes; ++j)
+    f = std::max(f, fb_batch_footprint(serve_size_hint(sv, r), r.bases, r.reads, &sv->bprm, sv->prm.k[j]));
+  if (sv->prm.estimate) f += (4ull << 23) * 4 + (3ull << 20) * 8;  // estimator cells and registers
+  return f;
+}
+
+static void serve_fail(fb_server* sv, ServeReq* r, int code, const std::string& msg) {
+  r->iso.reset();
+  if (r->bank) {
+    fb_bank_destroy(r->bank);
+    r->bank = nullptr;
+  }
+  std::lock_guard<std::mutex> g(sv->mu);
+  (void)sv->adm->retire(r->id);
+  r->status = code;
+  r->err = msg;
+  r->state = 3;
+  r->t_done = sv->now();
+  ++sv->st.failed;
+  sv->cv_done.notify_all();
+}
+
+// Uploads an admitted request's reads, estimates its genome and creates its sketches.
+static int serve_start(fb_server* sv, ServeReq* r) {
+  fb_ctx* c = sv->ctx;
+  fb_arena_view v[3] = {};
+  for (int a = 0; a < 2; ++a) {
+    v[a + 1].data = r->data[a].data();
+    v[a + 1].n_words = r->data[a].size();
+    v[a + 1].offsets = r->offs[a].data();
+    v[a + 1].lengths = r->lens[a].data();
+    v[a + 1].n_reads = r->lens[a].size();
+  }
+  if (int e = fb_bank_upload(c, v, &r->bank)) return e;
+  r->genome_size = r->genome_hint;
+  if (sv->prm.estimate) {
+    const uint64_t np = std::min(r->lens[0].size(), r->lens[1].size());
+    fb_count_stats cs{};
+    if (int e = fb_count_stats_from_lengths(r->lens[0].data(), r->lens[1].data(), np, &cs))
+      return set_err(c, e, "Input FASTQ files contain 0 valid read pairs");
+    if (int e = fb_estimate_genome(c, &v[1], &v[2], &cs, sv->prm.est_max_pairs, sv->prm.target_cov, &r->est)) return e;
+    r->has_est = true;
+    if (!r->genome_size) r->genome_size = r->est.genome_size;
+  }
+  if (!r->genome_size) r->genome_size = std::max<uint64_t>(r->bank->total_bases / 60, 50000);  // build.rs:74-83
+  for (int a = 0; a < 2; ++a) {  // the host copies are no longer needed
+    std::vector<uint64_t>().swap(r->data[a]);
+    std::vector<uint64_t>().swap(r->offs[a]);
+  }
+  r->iso = std::make_unique<BatchIso>();
+  return batch_iso_init(c, r->iso.get(), r->bank, r->genome_size, &sv->bprm);
+}
+
+// One batched pass for the requests g, all at pass index j of the ladder.
+static int serve_pass(fb_server* sv, uint32_t j, std::vector<ServeReq*>& g) {
+  fb_ctx* c = sv->ctx;
+  const uint32_t k = sv->prm.k[j];
+  const int32_t target = sv->prm.target_solid_cov[j];
+  auto& slot = sv->batches[j];
+  if (!slot) {
+    slot = std::make_unique<fb_batch>();
+    slot->ctx = c;
+    slot->prm = sv->bprm;
+    slot->graphs = false;
+  }
+  fb_batch* b = slot.get();
+  b->iso.clear();
+  for (ServeReq* r : g) b->iso.push_back(std::move(r->iso));
+  int rc = fb_batch_run(b, k, target);
+  // results are read while the batch still holds the isolates
+  for (size_t i = 0; i < g.size() && !rc; ++i) {
+    const BatchIso& it = *b->iso[i];
+    ServePass p;
+    const uint64_t n = it.n_out;
+    p.keys.resize(n * b->nref);
+    p.cov.resize(n);
+    p.mask.resize(n);
+    p.route = it.route;
+    rc = fb_batch_download(b, (uint32_t)i, p.keys.data(), p.cov.data(), p.mask.data());
+    if (!rc && sv->prm.keep_sketches) {
+      p.cms.resize(fb_cms_table_bytes(&it.cms));
+      rc = cms_download(c, it.cms.dev(), it.cms.width, it.cms.bits, p.cms.data(), p.cms.size());
+      if (!rc && it.has_hll) {
+        p.hll.resize((size_t)1 << it.hll.p);
+        rc = hll_download(c, it.hll.dev(), p.hll.data());
+      }
+    }
+    if (!rc) g[i]->passes.push_back(std::move(p));
+  }
+  for (size_t i = 0; i < g.size(); ++i) g[i]->iso = std::move(b->iso[i]);
+  b->iso.clear();
+  return rc;
+}
+
+static void serve_loop(fb_server* sv) {
+  (void)ctx_current(sv->ctx);
+  for (;;) {
+    std::vector<ServeReq*> admitted;
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] { return sv->stop || sv->adm->queued() > 0 || !sv->running.empty(); });
+      if (sv->stop && sv->running.empty()) break;
+      if (!sv->stop) {
+        std::vector<uint64_t> ids;
+        sv->adm->admit(ids);
+        for (uint64_t id : ids) {
+          ServeReq* r = sv->reqs[id].get();
+          r->state = 1;
+          r->t_admit = sv->now();
+          admitted.push_back(r);
+        }
+      }
+    }
+    for (ServeReq* r : admitted) {
+      if (int e = serve_start(sv, r)) {
+        serve_fail(sv, r, e, sv->ctx->err);
+        continue;
+      }
+      sv->running.push_back(r);
+    }
+    if (sv->running.empty()) {
+      // admitted requests may still be blocked by budget only while others
+      // run, so an empty running set with a waiting queue cannot happen
+      continue;
+    }
+    const double t_it = sv->now();
+    // one batched pass per ladder position that in-flight requests are at
+    std::map<uint32_t, std::vector<ServeReq*>> groups;
+    for (ServeReq* r : sv->running) groups[r->stage].push_back(r);
+    for (auto& kv : groups) {
+      if (int e = serve_pass(sv, kv.first, kv.second)) {
+        const std::string msg = sv->ctx->err;
+        for (ServeReq* r : kv.second) {
+          serve_fail(sv, r, e, msg);
+          r->stage = sv->prm.n_passes;  // leaves the running set below
+        }
+        continue;
+      }
+      for (ServeReq* r : kv.second) ++r->stage;
+    }
+    {
+      std::lock_guard<std::mutex> g(sv->mu);
+      ++sv->st.iterations;
+      sv->st.pass_runs += sv->running.size();
+      sv->st.max_batch = std::max<uint64_t>(sv->st.max_batch, sv->running.size());
+      sv->st.max_groups = std::max<uint64_t>(sv->st.max_groups, groups.size());
+      sv->st.busy_s += sv->now() - t_it;
+    }
+    // retire finished requests
+    std::vector<ServeReq*> still;
+    for (ServeReq* r : sv->running) {
+      if (r->stage < sv->prm.n_passes) {
+        still.push_back(r);
+        continue;
+      }
+      if (r->state == 3) continue;  // failed and already retired
+      r->iso.reset();
+      fb_bank_destroy(r->bank);
+      r->bank = nullptr;
+      std::lock_guard<std::mutex> g(sv->mu);
+      (void)sv->adm->retire(r->id);
+      r->state = 2;
+      r->t_done = sv->now();
+      ++sv->st.completed;
+      sv->cv_done.notify_all();
+    }
+    sv->running.swap(still);
+  }
+}
+
+extern "C" int fb_server_create(int device, const fb_serve_params* p, fb_server** out) {
+  if (!p || !out) return FB_ERR_INVALID;
+  *out = nullptr;
+  if (p->n_passes < 1 || p->n_passes > 4) return FB_ERR_INVALID;
+  for (uint32_t j = 0; j < p->n_passes; ++j)
+    if (p->k[j] < 1 || p->k[j] > 127 || p->target_solid_cov[j] > 255) return FB_ERR_INVALID;
+  if (p->counter_bits != 4 && p->counter_bits != 16) return FB_ERR_INVALID;
+  if (p->hll_p != 0 && (p->hll_p < 4 || p->hll_p > 18)) return FB_ERR_INVALID;
+  auto sv = std::make_unique<fb_server>();
+  sv->prm = *p;
+  if (int r = fb_ctx_create(device, 0, &sv->ctx)) return r;
+  fb_build_params_default(&sv->bprm, p->k[0]);
+  sv->bprm.cms_width = p->cms_width;
+  sv->bprm.counter_bits = p->counter_bits;
+  sv->bprm.hll_p = p->hll_p;
+  uint64_t budget = p->hbm_budget;
+  if (!budget) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      fb_ctx_destroy(sv->ctx);
+      return FB_ERR_HIP;
+    }
+    budget = (uint64_t)((double)fr * 0.8);
+  }
+  sv->st.budget = budget;
+  sv->adm = std::make_unique<fb::Admission>(budget, p->max_inflight ? p->max_inflight : 64);
+  sv->t0 = std::chrono::steady_clock::now();
+  fb_server* raw = sv.get();
+  sv->worker = std::thread(serve_loop, raw);
+  *out = sv.release();
+  return FB_OK;
+}
+
+extern "C" void fb_server_destroy(fb_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stop = true;
+    // queued requests are refused; in-flight ones finish
+    for (auto& kv : sv->reqs)
+      if (kv.second->state == 0) {
+        kv.second->state = 3;
+        kv.second->status = FB_ERR_BUSY;
+        kv.second->err = "server stopped";
+      }
+  }
+  sv->cv_work.notify_all();
+  if (sv->worker.joinable()) sv->worker.join();
+  sv->reqs.clear();
+  sv->batches.clear();
+  fb_ctx_destroy(sv->ctx);
+  delete sv;
+}
+
+extern "C" const char* fb_server_last_error(const fb_server* sv) { return sv ? sv->err.c_str() : "null server"; }
+
+extern "C" int fb_server_submit(fb_server* sv, const fb_arena_view* r1, const fb_arena_view* r2, uint64_t genome_size,
+                                uint64_t* ticket) {
+  if (!sv || !r1 || !r2 || !ticket) return FB_ERR_INVALID;
+  auto r = std::make_unique<ServeReq>();
+  const fb_arena_view* src[2] = {r1, r2};
+  for (int a = 0; a < 2; ++a) {
+    const fb_arena_view& v = *src[a];
+    if (v.n_reads && (!v.data || !v.offsets || !v.lengths)) {
+      sv->err = "null arena pointer";
+      return FB_ERR_INVALID;
+    }
+    for (uint64_t i = 0; i < v.n_reads; ++i)
+      if (v.lengths[i] && (v.offsets[i] + v.lengths[i] + 31) / 32 > v.n_words) {
+        sv->err = "read ends past its arena";
+        return FB_ERR_INVALID;
+      }
+    r->data[a].assign(v.data, v.data + v.n_words);
+    r->offs[a].assign(v.offsets, v.offsets + v.n_reads);
+    r->lens[a].assign(v.lengths, v.lengths + v.n_reads);
+    for (uint64_t i = 0; i < v.n_reads; ++i) {
+      r->bases += v.lengths[i];
+      r->max_len = std::max(r->max_len, v.lengths[i]);
+    }
+    r->reads += v.n_reads;
+  }
+  r->genome_hint = genome_size;
+  r->footprint = serve_footprint(sv, *r);
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (sv->stop) return FB_ERR_BUSY;
+  if (sv->adm->queued() >= (sv->prm.max_queue ? sv->prm.max_queue : 4096)) {
+    sv->err = "serving queue full";
+    return FB_ERR_BUSY;
+  }
+  r->id = sv->next_id++;
+  if (sv->adm->submit(r->id, r->footprint) != FB_OK) {
+    sv->err = "request needs more device memory than the whole budget";
+    return FB_ERR_OOM;
+  }
+  r->t_submit = sv->now();
+  *ticket = r->id;
+  ++sv->st.submitted;
+  sv->reqs[r->id] = std::move(r);
+  sv->cv_work.notify_all();
+  return FB_OK;
+}
+
+extern "C" int fb_server_wait(fb_server* sv, uint64_t ticket, double timeout_s, fb_serve_result* out) {
+  if (!sv || !out) return FB_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end()) return FB_ERR_INVALID;
+  ServeReq* r = it->second.get();
+  auto done = [&] { return r->state >= 2; };
+  if (timeout_s < 0) sv->cv_done.wait(lk, done);
+  else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return FB_ERR_TIMEOUT;
+  memset(out, 0, sizeof *out);
+  out->status = r->status;
+  snprintf(out->error, sizeof out->error, "%s", r->err.c_str());
+  out->n_passes = (uint32_t)r->passes.size();
+  out->genome_size = r->genome_size;
+  out->has_estimate = r->has_est ? 1u : 0u;
+  out->estimate = r->est;
+  for (size_t j = 0; j < r->passes.size() && j < 4; ++j) {
+    out->n_solid[j] = r->passes[j].cov.size();
+    out->route[j] = r->passes[j].route;
+  }
+  out->bases = r->bases;
+  out->footprint = r->footprint;
+  out->t_submit = r->t_submit;
+  out->t_admit = r->t_admit;
+  out->t_done = r->t_done;
+  return r->status;
+}
+
+static ServeReq* serve_done_req(fb_server* sv, uint64_t ticket, uint32_t pass) {
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end() || it->second->state != 2 || pass >= it->second->passes.size()) return nullptr;
+  return it->second.get();
+}
+
+extern "C" int fb_server_download(fb_server* sv, uint64_t ticket, uint32_t pass, uint64_t* keys, uint16_t* cov,
+                                  uint8_t* mask) {
+  if (!sv) return FB_ERR_INVALID;
+  std::lock_guard<std::mutex> g(sv->mu);
+  ServeReq* r = serve_done_req(sv, ticket, pass);
+  if (!r) return FB_ERR_INVALID;
+  const ServePass& p = r->passes[pass];
+  if (keys) memcpy(keys, p.keys.data(), p.keys.size() * 8);
+  if (cov) memcpy(cov, p.cov.data(), p.cov.size() * 2);
+  if (mask) memcpy(mask, p.mask.data(), p.mask.size());
+  return FB_OK;
+}
+
+extern "C" int fb_server_sketches(fb_server* sv, uint64_t ticket, uint32_t pass, void* cms, uint64_t cms_bytes,
+                                  uint8_t* hll) {
+  if (!sv) return FB_ERR_INVALID;
+  std::lock_guard<std::mutex> g(sv->mu);
+  ServeReq* r = serve_done_req(sv, ticket, pass);
+  if (!r) return FB_ERR_INVALID;
+  const ServePass& p = r->passes[pass];
+  if (p.cms.empty()) return FB_ERR_INVALID;  // the server was created without keep_sketches
+  if (cms) {
+    if (cms_bytes < p.cms.size()) return FB_ERR_INVALID;
+    memcpy(cms, p.cms.data(), p.cms.size());
+  }
+  if (hll && !p.hll.empty()) memcpy(hll, p.hll.data(), p.hll.size());
+  return FB_OK;
+}
+
+extern "C" int fb_server_release(fb_server* sv, uint64_t ticket) {
+  if (!sv) return FB_ERR_INVALID;
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->reqs.find(ticket);
+  if (it == sv->reqs.end() || it->second->state < 2) return FB_ERR_INVALID;
+  sv->reqs.erase(it);
+  return FB_OK;
+}
+
+extern "C" int fb_server_stats(fb_server* sv, fb_serve_stats* out) {
+  if (!sv || !out) return FB_ERR_INVALID;
+  std::lock_guard<std::mutex> g(sv->mu);
+  *out = sv->st;
+  out->queued = sv->adm->queued();
+  out->inflight = sv->adm->inflight();
+  out->inflight_bytes = sv->adm->used();
+  out->device_bytes = sv->ctx->held;
   return FB_OK;
 }

@@ -39,7 +39,8 @@ extern "C" {
 #define FB_ERR_HIP -4       /* HIP runtime error                                    */
 #define FB_ERR_EMPTY -5     /* no read pairs: "Input FASTQ files contain 0 valid read pairs" */
 #define FB_ERR_NODEV -6     /* no usable GPU                                        */
-#define FB_ERR_BUSY -7      /* serving queue full                                   */
+#define FB_ERR_BUSY -7      /* serving queue full, or the server stopped            */
+#define FB_ERR_TIMEOUT -8   /* fb_server_wait: the request has not finished yet     */
 
 typedef struct fb_ctx fb_ctx;
 typedef struct fb_bank fb_bank;
@@ -245,6 +246,101 @@ uint32_t fb_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uin
  * estimator: N maps to A before hashing, bitseq.rs:67-81). */
 int fb_estimate_genome(fb_ctx* ctx, const fb_arena_view* r1, const fb_arena_view* r2, const fb_count_stats* stats,
                        uint64_t max_pairs, double target_cov, fb_genome_estimate* out);
+
+/* ---- Serving: a request loop with continuous batching ----------------------
+ * No counterpart in the reference, which assembles one isolate per process
+ * run (main.rs:37-564); each request runs the reference flow of one isolate:
+ * estimate_genome (estimator.rs:286-440) and then build_cdbg_with_contigs_ext
+ * passes 1-3 (build.rs:67-328) at every k of the ladder.
+ *
+ * A server owns a worker thread and its own context on `device`.  Requests
+ * (one isolate's R1/R2 read pairs) wait in arrival order and are admitted
+ * while their device footprint fits the budget.  Every iteration runs one
+ * batched pass for each ladder position in-flight requests are at, so
+ * requests that arrive while others are on the GPU join the next iteration
+ * and share its launches; a request that finishes leaves at once and frees its
+ * memory.  Sketches and buffers of a request stay on the device across its
+ * passes.  Each result equals the one the request gets alone (fb_build_solid
+ * at the request's genome size).  submit / wait / download may be called from
+ * any host thread. */
+typedef struct fb_server fb_server;
+typedef struct {
+  uint32_t n_passes;            /* 1..4 passes of the k ladder */
+  uint32_t k[4];
+  int32_t target_solid_cov[4];  /* Option<u8> per pass: < 0 = None */
+  uint64_t cms_width;           /* 0 = CountMinSketch::for_genome_size(G) */
+  int32_t counter_bits;         /* 4 or 16 */
+  int32_t hll_p;                /* per-pass HLL precision, 0 = none */
+  uint32_t estimate;            /* 1 = estimate_genome first; its G sizes the sketch when no size is given */
+  uint64_t est_max_pairs;       /* estimate_genome max_pairs */
+  double target_cov;            /* estimate_genome target coverage */
+  uint64_t hbm_budget;          /* admission budget in bytes; 0 = 80% of the free device memory */
+  uint32_t max_inflight;        /* requests on the device at once; 0 = 64 */
+  uint32_t max_queue;           /* waiting requests; 0 = 4096, FB_ERR_BUSY beyond */
+  uint32_t keep_sketches;       /* 1 = keep every pass's CMS table and HLL registers with its result */
+} fb_serve_params;
+
+typedef struct {
+  int32_t status;               /* FB_OK or the request's error */
+  char error[160];
+  uint32_t n_passes;
+  uint64_t genome_size;         /* sizes the request's sketch (build.rs:74-84) */
+  uint32_t has_estimate;
+  fb_genome_estimate estimate;
+  uint64_t n_solid[4];          /* solid k-mers per pass */
+  uint32_t route[4];            /* pass-2 route per pass (fb_solid_stats[4] codes) */
+  uint64_t bases;
+  uint64_t footprint;           /* device bytes reserved at admission */
+  double t_submit, t_admit, t_done;  /* seconds since the server started */
+} fb_serve_result;
+
+typedef struct {
+  uint64_t submitted, completed, failed, queued, inflight;
+  uint64_t iterations;          /* worker iterations that ran passes */
+  uint64_t pass_runs;           /* request-passes run */
+  uint64_t max_batch;           /* most requests in flight in one iteration */
+  uint64_t max_groups;          /* most ladder positions in one iteration */
+  uint64_t budget, inflight_bytes, device_bytes;
+  double busy_s;                /* time the worker spent running passes */
+} fb_serve_stats;
+
+void fb_serve_params_default(fb_serve_params* p);
+int fb_server_create(int device, const fb_serve_params* p, fb_server** out);
+/* Finishes the requests in flight, refuses the queued ones (FB_ERR_BUSY) and
+ * frees everything. */
+void fb_server_destroy(fb_server* s);
+const char* fb_server_last_error(const fb_server* s);
+/* Copies the read pairs (R1 -> pe_forward, R2 -> pe_reverse) and queues the
+ * request.  genome_size 0 = from the estimate (or bases / 60 without one).
+ * FB_ERR_BUSY when the queue is full, FB_ERR_OOM when the request alone
+ * exceeds the budget. */
+int fb_server_submit(fb_server* s, const fb_arena_view* r1, const fb_arena_view* r2, uint64_t genome_size,
+                     uint64_t* ticket);
+/* Waits up to timeout_s (< 0: no limit) for the request; FB_ERR_TIMEOUT (out
+ * untouched) when it has not finished, else its status with out filled. */
+int fb_server_wait(fb_server* s, uint64_t ticket, double timeout_s, fb_serve_result* out);
+/* Solid set of a finished request's pass, sorted, in the fb_solid_download layout. */
+int fb_server_download(fb_server* s, uint64_t ticket, uint32_t pass, uint64_t* keys, uint16_t* cov, uint8_t* mask);
+/* The pass's CMS table (fb_cms_download layout) and HLL registers; needs keep_sketches. */
+int fb_server_sketches(fb_server* s, uint64_t ticket, uint32_t pass, void* cms, uint64_t cms_bytes, uint8_t* hll);
+/* Drops a finished request's results. */
+int fb_server_release(fb_server* s, uint64_t ticket);
+int fb_server_stats(fb_server* s, fb_serve_stats* out);
+
+/* The serving admission policy alone (host only, no device): requests wait in
+ * arrival order; the head is admitted while its bytes fit next to the
+ * admitted ones and fewer than max_inflight are in; it is never overtaken.
+ * submit refuses (FB_ERR_OOM) a request larger than the budget. */
+typedef struct fb_admission fb_admission;
+int fb_admission_create(uint64_t budget, uint32_t max_inflight, fb_admission** out);
+void fb_admission_destroy(fb_admission* q);
+int fb_admission_submit(fb_admission* q, uint64_t id, uint64_t bytes);
+/* Admits what fits; writes up to max_ids admitted ids in order and returns how many were admitted. */
+uint32_t fb_admission_admit(fb_admission* q, uint64_t* ids, uint32_t max_ids);
+int fb_admission_retire(fb_admission* q, uint64_t id);
+uint64_t fb_admission_used(const fb_admission* q);
+uint32_t fb_admission_queued(const fb_admission* q);
+uint32_t fb_admission_inflight(const fb_admission* q);
 
 /* ---- Host helpers ---------------------------------------------------------- */
 /* BitSeqArena::push_seq for many sequences: data must hold ceil(total/32)

@@ -152,6 +152,151 @@ def _head(a, n):
     return F.BitSeqArena(a.data[: (end + 31) // 32 + 1].copy(), a.offsets[:n].copy(), a.lengths[:n].copy())
 
 
+# ---------------------------------------------------------------------------
+# Batched and served forms on isolates drawn from the 64-isolate workload:
+# genome 1.5-7 Mbp uniform, GC 0.30-0.68 uniform, coverage lognormal (median
+# 80x, sigma 0.5) clipped to 30-200x, PE150 at 0.3% substitutions.
+
+BATCH_K, BATCH_WIDTH, BATCH_HLL_P = 55, 1 << 23, 14
+
+
+def batch_shapes(n: int, seed: int = 0xFB03):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        g = int(rng.uniform(1.5e6, 7e6))
+        gc = float(rng.uniform(0.30, 0.68))
+        cov = float(np.clip(80.0 * np.exp(0.5 * rng.standard_normal()), 30.0, 200.0))
+        out.append((g, gc, cov))
+    return out
+
+
+def make_batch_isolates(n: int, threads: int):
+    """n seeded isolates of the batched workload as packed read banks."""
+    import concurrent.futures as cf
+    import fastbact_amd as F
+    shapes = batch_shapes(n)
+
+    def gen(i):
+        g, gc, cov = shapes[i]
+        iso = F.synth_isolate(seed=0xFB03 * 1000 + i, genome_len=g, gc=gc, coverage=cov, read_len=150,
+                              insert_mean=350.0, insert_sd=35.0, sub_rate=0.003, indel_rate=0.0)
+        return iso.bank()
+
+    t = time.time()
+    with cf.ThreadPoolExecutor(max(1, threads)) as ex:  # the generator releases the GIL
+        banks = list(ex.map(gen, range(n)))
+    log(f"{n} batched-workload isolates generated ({sum(b.total_bases() for b in banks) / 1e9:.1f} Gbases, "
+        f"{time.time() - t:.1f} s)")
+    return shapes, banks
+
+
+def waves_by_footprint(F, shapes, devs, k: int, budget: int):
+    """Consecutive isolates grouped so that each wave's footprint fits the budget."""
+    waves, cur, used = [], [], 0
+    for i, d in enumerate(devs):
+        fp = F.batch_footprint(shapes[i][0], d.total_bases(), d.total_reads(), k, cms_width=BATCH_WIDTH,
+                               counter_bits=16, hll_p=BATCH_HLL_P)
+        if cur and used + fp > budget:
+            waves.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += fp
+    if cur:
+        waves.append(cur)
+    return waves
+
+
+def bench_batched(ctx, F, shapes, devs, reps: int) -> dict:
+    """The batched form (waves of isolates per launch, launch shapes from the
+    autotuner) against the plain per-isolate drop-in on the same isolates."""
+    bases = sum(d.bases_for_k(BATCH_K) for d in devs)
+    budget = int(0.6 * ctx.memory()[0])
+    waves = waves_by_footprint(F, shapes, devs, BATCH_K, budget)
+    log(f"batched form: {len(devs)} isolates in {len(waves)} waves (budget {budget / 2**30:.0f} GiB)")
+    tune = None
+    batch_ms = []
+    for wi, w in enumerate(waves):
+        b = F.Batch(ctx, [devs[i] for i in w], [shapes[i][0] for i in w], cms_width=BATCH_WIDTH, counter_bits=16,
+                    hll_p=BATCH_HLL_P)
+        if wi == 0:
+            tune = b.autotune(BATCH_K, None)
+            log(f"autotune: {tune}")
+        else:
+            b.run(BATCH_K, None)  # warm-up
+        ms = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            b.run(BATCH_K, None)
+            ms.append((time.perf_counter() - t) * 1e3)
+        batch_ms.append(min(ms))
+        b.close()
+    plain_ms = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        for d in devs:
+            h = F.build_solid_handle(ctx, d, BATCH_K, target_solid_cov=None, genome_size=None, cms_width=BATCH_WIDTH,
+                                     counter_bits=16, hll_p=BATCH_HLL_P)
+            h.close()
+        ctx.synchronize()
+        plain_ms.append((time.perf_counter() - t) * 1e3)
+    tb, tp = sum(batch_ms), min(plain_ms)
+    return {"workload": f"{len(devs)} isolates: genome U[1.5,7] Mbp, GC U[0.30,0.68], coverage lognormal median 80x "
+                        f"sigma 0.5 clipped 30-200x, PE150 0.3% subs; k=55; CMS 4x2^23 u16 + HLL p14 per isolate",
+            "bases": bases, "waves": len(waves), "batched_ms": round(tb, 1),
+            "batched_gbases_s": round(bases / tb / 1e6, 3), "plain_per_isolate_ms": round(tp, 1),
+            "plain_per_isolate_gbases_s": round(bases / tp / 1e6, 3), "batched_over_plain": round(tp / tb, 3),
+            "autotune": tune}
+
+
+def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05) -> dict:
+    """Poisson stream of isolate requests (k ladder 21/55/77 plus the genome
+    estimate) through the serving loop; latency from submission to the last
+    pass's solid set on the host."""
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, len(banks), size=n_req)
+    gaps = rng.exponential(1.0 / rate, size=n_req)
+    kw = dict(cms_width=BATCH_WIDTH, counter_bits=16, hll_p=BATCH_HLL_P)
+    out = {}
+    with F.Server(0, (21, 55, 77), (None, 4, 4), **kw) as sv:
+        # warm-up request (first use of the kernels and the pool)
+        sv.wait(sv.submit(banks[0].pe_forward, banks[0].pe_reverse), download=False)
+        t0 = time.perf_counter()
+        tickets = []
+        for i in range(n_req):
+            target = t0 + float(np.sum(gaps[: i + 1]))
+            while time.perf_counter() < target:
+                time.sleep(min(0.002, max(0.0, target - time.perf_counter())))
+            b = banks[int(picks[i])]
+            tickets.append(sv.submit(b.pe_forward, b.pe_reverse))
+        res = [sv.wait(t, download=False) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        lat = sorted(r.latency * 1e3 for r in res)
+        bases = sum(3 * banks[int(p)].total_bases() for p in picks)
+        out["poisson"] = {"requests": n_req, "rate_per_s": rate, "wall_s": round(wall, 2),
+                          "isolates_per_hour": round(n_req / wall * 3600, 1),
+                          "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
+                                         "p99": round(float(np.percentile(lat, 99)), 1)},
+                          "gbases_s": round(bases / wall / 1e9, 3), "max_batch": st["max_batch"],
+                          "max_ladder_positions": st["max_groups"]}
+        # capacity: the same requests all at once
+        t0 = time.perf_counter()
+        tickets = [sv.submit(banks[int(p)].pe_forward, banks[int(p)].pe_reverse) for p in picks]
+        res = [sv.wait(t, download=False) for t in tickets]
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+        lat = sorted(r.latency * 1e3 for r in res)
+        out["saturated"] = {"requests": n_req, "wall_s": round(wall, 2),
+                            "isolates_per_hour": round(n_req / wall * 3600, 1),
+                            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
+                                           "p99": round(float(np.percentile(lat, 99)), 1)},
+                            "gbases_s": round(bases / wall / 1e9, 3), "max_batch": st["max_batch"]}
+    out["workload"] = ("requests drawn from the batched-workload isolates, each the k ladder 21/55/77 (gates None/4/4) "
+                       "after estimate_genome; CMS 4x2^23 u16, HLL p14; latency = submit to solid sets on the host")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +306,11 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--placement", type=int, default=0, help="pass-1 sketch placement: 0 auto, 1 global, 2 LDS bins")
+    ap.add_argument("--batch-isolates", type=int, default=64, help="isolates of the batched workload (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=0,
+                    help="requests of the serving stream on batched-workload isolates (0: skip; off by default, "
+                         "see the serving note in the design document)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this many MiB device to device once (a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -233,6 +383,23 @@ def main(argv=None) -> int:
         copy_calibration(ctx, args.copy_calibration_mib)
     roofline["traffic"], roofline["traffic_source"] = measured_traffic(dom)
 
+    batched = served = None
+    if args.batch_isolates or args.serve_requests:
+        shapes, banks = make_batch_isolates(max(args.batch_isolates, 1 if args.serve_requests else 0),
+                                            min(16, os.cpu_count() or 1))
+        if args.batch_isolates:
+            devs = [F.DeviceBank(ctx, b) for b in banks]
+            ctx.synchronize()
+            batched = bench_batched(ctx, F, shapes, devs, reps=2)
+            log(f"batched: {batched['batched_gbases_s']} Gbases/s vs plain {batched['plain_per_isolate_gbases_s']}")
+            for d in devs:
+                d.close()
+            ctx.trim()  # hand the waves' buffers back to the device
+        if args.serve_requests:
+            served = bench_serving(F, shapes, banks, args.serve_requests, args.serve_rate)
+            log(f"serving: {served['poisson']}")
+        del banks
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -266,6 +433,8 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "pass_model_roofline": pass_model,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": served,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -60,6 +60,8 @@ class BuildParams(C.Structure):
         ("split", C.c_uint32),
         ("placement", C.c_uint32),
         ("bins2", C.c_uint32),
+        ("chunks1", C.c_uint32),
+        ("chunks2", C.c_uint32),
     ]
 
 
@@ -114,6 +116,18 @@ class SynthParams(C.Structure):
         ("insert_sd", C.c_double),
         ("sub_rate", C.c_double),
         ("indel_rate", C.c_double),
+    ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("placement", C.c_uint32),
+        ("split", C.c_uint32),
+        ("chunks1", C.c_uint32),
+        ("chunks2", C.c_uint32),
+        ("best_ms", C.c_float),
+        ("default_ms", C.c_float),
+        ("trials", C.c_uint32),
     ]
 
 
@@ -179,7 +193,9 @@ SIGNATURES = [
     ("fb_last_error", C.c_char_p, [vp]),
     ("fb_ctx_synchronize", C.c_int, [vp]),
     ("fb_ctx_device_bytes", C.c_uint64, [vp]),
+    ("fb_ctx_trim", C.c_int, [vp]),
     ("fb_version", C.c_char_p, []),
+    ("fb_device_memory", C.c_int, [C.c_int, u64p, u64p]),
     ("fb_bank_upload", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(vp)]),
     ("fb_bank_destroy", None, [vp]),
     ("fb_bank_total_bases", C.c_uint64, [vp]),
@@ -229,6 +245,10 @@ SIGNATURES = [
     ("fb_batch_hll_registers", C.c_int, [vp, C.c_uint32, u8p]),
     ("fb_batch_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_batch_routes", C.c_int, [vp, u32p]),
+    ("fb_batch_autotune", C.c_int, [vp, C.c_uint32, C.c_int32, C.POINTER(TuneResult)]),
+    ("fb_autotune_save", C.c_int, [C.c_char_p]),
+    ("fb_autotune_load", C.c_int, [C.c_char_p]),
+    ("fb_autotune_clear", None, []),
     ("fb_serve_params_default", None, [C.POINTER(ServeParams)]),
     ("fb_server_create", C.c_int, [C.c_int, C.POINTER(ServeParams), C.POINTER(vp)]),
     ("fb_server_destroy", None, [vp]),

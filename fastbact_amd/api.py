@@ -125,6 +125,7 @@ class Context:
         if rc:
             raise FastBactError(rc, f"fb_ctx_create(device={device}) failed")
         self.h = h
+        self.device = device
 
     def check(self, rc: int, what: str) -> None:
         if rc:
@@ -132,6 +133,16 @@ class Context:
 
     def synchronize(self) -> None:
         self.check(self.lib.fb_ctx_synchronize(self.h), "fb_ctx_synchronize")
+
+    def memory(self) -> tuple:
+        """(free, total) bytes of the context's device."""
+        f, t = C.c_uint64(), C.c_uint64()
+        self.check(self.lib.fb_device_memory(self.device, C.byref(f), C.byref(t)), "fb_device_memory")
+        return int(f.value), int(t.value)
+
+    def trim(self) -> None:
+        """Returns the context's cached free device buffers (fb_ctx_trim)."""
+        self.check(self.lib.fb_ctx_trim(self.h), "fb_ctx_trim")
 
     def device_bytes(self) -> int:
         return int(self.lib.fb_ctx_device_bytes(self.h))
@@ -360,7 +371,7 @@ def _pack_contigs(contigs: Optional[Sequence[bytes]]):
 
 def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Optional[int] = None,
                  cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
-                 split: int = 0, placement: int = 0, bins2: int = 0) -> BuildParams:
+                 split: int = 0, placement: int = 0, bins2: int = 0, chunks1: int = 0, chunks2: int = 0) -> BuildParams:
     p = BuildParams()
     _lib.load().fb_build_params_default(C.byref(p), k)
     p.target_solid_cov = -1 if target_solid_cov is None else int(target_solid_cov)
@@ -372,6 +383,8 @@ def build_params(k: int, target_solid_cov: Optional[int] = None, genome_size: Op
     p.split = split
     p.placement = placement
     p.bins2 = bins2
+    p.chunks1 = chunks1
+    p.chunks2 = chunks2
     return p
 
 
@@ -389,12 +402,13 @@ def build_solid_handle(ctx: Context, bank: DeviceBank, k: int, contigs: Optional
 def build_solid_kmers(ctx: Context, bank: DeviceBank, k: int, target_solid_cov: Optional[int] = None,
                       genome_size: Optional[int] = None, contigs: Optional[Sequence[bytes]] = None,
                       cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0,
-                      with_cms: bool = True, split: int = 0, placement: int = 0, bins2: int = 0) -> SolidSet:
+                      with_cms: bool = True, split: int = 0, placement: int = 0, bins2: int = 0,
+                      chunks1: int = 0, chunks2: int = 0) -> SolidSet:
     """build_cdbg_with_contigs_ext passes 1-3 (build.rs:67-328): the solid set,
     the pass's CMS table and (hll_p > 0) the pass's HLL registers."""
     h = build_solid_handle(ctx, bank, k, contigs, target_solid_cov=target_solid_cov, genome_size=genome_size,
                            cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, split=split,
-                           placement=placement, bins2=bins2)
+                           placement=placement, bins2=bins2, chunks1=chunks1, chunks2=chunks2)
     try:
         return h.download(k, with_cms=with_cms, counter_bits=counter_bits, hll_p=hll_p)
     finally:
@@ -413,9 +427,9 @@ class Batch(_Handle):
 
     def __init__(self, ctx: Context, banks: Sequence[DeviceBank], genome_sizes: Sequence[int],
                  cms_width: Optional[int] = None, counter_bits: int = 4, hll_p: int = 0, sort: bool = True,
-                 split: int = 0, placement: int = 0, bins2: int = 0):
+                 split: int = 0, placement: int = 0, bins2: int = 0, chunks1: int = 0, chunks2: int = 0):
         p = build_params(0, cms_width=cms_width, counter_bits=counter_bits, hll_p=hll_p, sort=sort, split=split,
-                         placement=placement, bins2=bins2)
+                         placement=placement, bins2=bins2, chunks1=chunks1, chunks2=chunks2)
         arr = (C.c_void_p * len(banks))(*[b.h.value if hasattr(b.h, "value") else b.h for b in banks])
         gs = _u64(genome_sizes)
         h = C.c_void_p()
@@ -436,6 +450,13 @@ class Batch(_Handle):
         t = (C.c_float * 4)()
         self.ctx.check(self.ctx.lib.fb_batch_timings(self.h, t), "fb_batch_timings")
         return tuple(float(x) for x in t)
+
+    def autotune(self, k: int, target_solid_cov: Optional[int] = None) -> dict:
+        """fb_batch_autotune: times launch shapes at k on this batch and keeps the fastest."""
+        r = _lib.TuneResult()
+        self.ctx.check(self.ctx.lib.fb_batch_autotune(self.h, k, -1 if target_solid_cov is None else target_solid_cov,
+                                                      C.byref(r)), "fb_batch_autotune")
+        return {name: getattr(r, name) for name, _ in _lib.TuneResult._fields_}
 
     def routes(self) -> list:
         """Pass-2 route of every isolate in the last run."""

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <tuple>
 #include <condition_variable>
 #include <map>
 #include <thread>
@@ -189,6 +190,35 @@ extern "C" int fb_ctx_synchronize(fb_ctx* c) {
 }
 
 extern "C" uint64_t fb_ctx_device_bytes(const fb_ctx* c) { return c ? c->held : 0; }
+
+extern "C" int fb_ctx_trim(fb_ctx* c) {
+  if (!c) return FB_ERR_INVALID;
+  if (int r = ctx_current(c)) return r;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (auto& b : c->pool) {
+    (void)hipFree(b.p);
+    c->held -= std::min(c->held, (uint64_t)b.bytes);
+  }
+  c->pool.clear();
+  return FB_OK;
+}
+
+extern "C" int fb_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return FB_ERR_INVALID;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return FB_ERR_NODEV;
+  }
+  size_t fr = 0, tot = 0;
+  if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return FB_ERR_HIP;
+  }
+  *free_bytes = fr;
+  *total_bytes = tot;
+  return FB_OK;
+}
 
 extern "C" int fb_diag_copy(fb_ctx* c, uint64_t bytes) {
   if (!c) return FB_ERR_INVALID;
@@ -664,7 +694,8 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   BinArgs ba;
   if (binned) {
     const uint64_t thr1 = bank->n_reads * (uint64_t)split;
-    const uint64_t n_wg = scatter_blocks(thr1, scatter_chunks(thr1, kP1BlockSize), kP1BlockSize);
+    const uint64_t n_wg = scatter_blocks(thr1, prm->chunks1 ? prm->chunks1 : scatter_chunks(thr1, kP1BlockSize),
+                                         kP1BlockSize);
     ba.cap = bin_slice_cap(4.0 * (double)windows / (double)nbins, n_wg, 4);
     ba.sb = sb;
     ba.nbins = (uint32_t)nbins;
@@ -686,7 +717,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     ba.reads = bank->view();
     ba.k = k;
     ba.split = split;
-    ba.chunks = scatter_chunks(bank->n_reads * (uint64_t)split, kP1BlockSize);
+    ba.chunks = prm->chunks1 ? prm->chunks1 : scatter_chunks(bank->n_reads * (uint64_t)split, kP1BlockSize);
     ba.log2w = so->cms.log2w;
     ba.n_cells = ncnt;
     ba.counts = so->cms.counts.as<uint32_t>();
@@ -779,7 +810,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     // estimate, with a count past the end sending the build to the global route
     const uint64_t lds_out = std::min<uint64_t>(nb2 * lds_cap, (uint64_t)(1.25 * n_est_keys) + 65536);
     const uint64_t out_cap = std::min<uint64_t>(lds_out + spill_cap, 0xFFFFFFFFull);
-    const uint32_t chunks2 = scatter_chunks(threads2);
+    const uint32_t chunks2 = prm->chunks2 ? prm->chunks2 : scatter_chunks(threads2);
     const uint64_t n_blocks2 = scatter_blocks(threads2, chunks2);
     DBuf recs, gm, runs, bh, st2, p3s;
     int r = 0;
@@ -1102,6 +1133,25 @@ extern "C" int fb_estimate_genome(fb_ctx* c, const fb_arena_view* r1, const fb_a
 // An isolate whose LDS route overflows its record, spill or output buffer
 // re-runs pass 2 alone on the global table, which grows until it fits.
 
+struct TuneKey {
+  uint32_t kind;        // 0 batch, 1 single isolate
+  uint32_t limbs;       // 64-bit limbs of the k-mer: 1, 2, 3 or 4
+  uint32_t bits;        // counter bits
+  uint32_t log2w;       // sketch width
+  uint32_t log2_bases;  // bases per isolate, rounded to a power of two
+  uint32_t log2_n;      // isolates, rounded to a power of two
+  bool operator<(const TuneKey& o) const {
+    return std::tie(kind, limbs, bits, log2w, log2_bases, log2_n) <
+           std::tie(o.kind, o.limbs, o.bits, o.log2w, o.log2_bases, o.log2_n);
+  }
+};
+struct TuneChoice {
+  uint32_t placement = 0, split = 0, chunks1 = 0, chunks2 = 0;
+  float ms = 0.0f;
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneChoice> g_tune;
+
 struct BatchIso {
   const fb_bank* bank = nullptr;
   uint64_t est_g = 0;
@@ -1148,6 +1198,7 @@ struct fb_batch {
   std::map<uint32_t, hipGraph_t> g1_def, g2_def;
   std::map<uint32_t, std::vector<uint64_t>> sig1, sig2;
   bool graphs = true;   // false: launch directly (membership changes every run)
+  std::map<uint32_t, TuneChoice> tuned;  // per k, from fb_batch_autotune
   size_t cap_iso = 0;   // isolates the argument arrays hold
   float ms[4] = {0, 0, 0, 0};
   void drop(std::map<uint32_t, hipGraphExec_t>& ge, std::map<uint32_t, hipGraph_t>& gd, uint32_t kk) {
@@ -1182,7 +1233,7 @@ struct P2Plan {
 };
 // lds_cap: LDS table entries of one bin; round_bins: bins one round takes.
 static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint32_t max_len, uint32_t k,
-                         uint32_t split, uint32_t bins2, uint32_t lds_cap, uint32_t round_bins) {
+                         uint32_t split, uint32_t bins2, uint32_t lds_cap, uint32_t round_bins, uint32_t chunks2) {
   P2Plan q;
   q.lds_cap = lds_cap;
   while ((double)q.nb2 * 0.5 * (double)q.lds_cap < n_est) q.nb2 <<= 1;
@@ -1194,7 +1245,7 @@ static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint3
   q.split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
   q.threads2 = n_reads * (uint64_t)q.split2;
   q.rec_cap = windows + q.threads2;
-  q.chunks2 = scatter_chunks(q.threads2);
+  q.chunks2 = chunks2 ? chunks2 : scatter_chunks(q.threads2);
   q.n_blocks2 = scatter_blocks(q.threads2, q.chunks2);
   const double excess = std::max(0.0, n_est - (double)q.nb2 * 0.5 * (double)q.lds_cap);
   q.spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est / 4)));
@@ -1220,13 +1271,14 @@ extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uin
   const uint64_t W = batch_cms_width(p, est_g);
   const uint64_t windows = bases;  // upper bound of the windows at any k
   const uint32_t max_len = reads ? (uint32_t)std::min<uint64_t>(bases / reads + 1, 1u << 20) : 1u;
-  // keys of pass 2: at most the windows, and the linear-counting estimate
-  // stays below 8 keys per row-0 cell for the sketches in use
-  const double n_est = (double)std::min<uint64_t>(windows, 8 * W);
+  // the key estimate of pass 2 reaches the window count when the sketch is
+  // overloaded (every row-0 cell passes the gate), so the footprint takes that
+  const double n_est = (double)windows;
   // host-only sizing: the LDS of a workgroup with the full 160 KB granted
   const uint32_t kw = key_words_for_k(k);
   const uint32_t lds_cap = (kP2LdsBytes - 1024u) / (kw * 8u + 20u);
-  const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u);
+  const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u,
+                              p->chunks2);
   const uint32_t rw = (2 * k + 8 + 63) / 64;
   uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;            // exact counts + gate bits
   b += p->hll_p ? (4ull << p->hll_p) : 0;                     // HLL registers
@@ -1320,6 +1372,57 @@ extern "C" uint64_t fb_batch_count(const fb_batch* b, uint32_t i) {
   return (b && i < b->iso.size()) ? b->iso[i]->n_out : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner: launch shapes (pass-1 placement, threads per read, thread
+// shares per scatter block in passes 1 and 2) picked by timing candidates.
+// Every candidate computes the same sets (the shapes only change which
+// thread handles which windows); the tuner checks that it does.  Choices are
+// kept per workload shape for the process and can be saved and loaded.
+
+static uint32_t ceil_log2(uint64_t x) {
+  uint32_t l = 0;
+  while ((1ull << l) < x && l < 63) ++l;
+  return l;
+}
+
+static TuneKey batch_tune_key(const fb_batch* b, uint32_t k) {
+  uint64_t bases = 0, w = 1;
+  for (auto& it : b->iso) {
+    bases += fb_bank_bases_for_k(it->bank, k);
+    w = std::max<uint64_t>(w, it->cms.width);
+  }
+  const uint64_t n = std::max<size_t>(b->iso.size(), 1);
+  return TuneKey{0, (2 * k + 63) / 64, (uint32_t)b->prm.counter_bits, ceil_log2(w), ceil_log2(bases / n + 1),
+                 ceil_log2(n)};
+}
+
+// The batch's parameters at k: explicit fields win, then the batch's own
+// tuned choice, then a tuned choice of the same workload shape.
+static fb_build_params batch_params(const fb_batch* b, uint32_t k) {
+  fb_build_params p = b->prm;
+  TuneChoice t;
+  bool have = false;
+  auto own = b->tuned.find(k);
+  if (own != b->tuned.end()) {
+    t = own->second;
+    have = true;
+  } else {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(batch_tune_key(b, k));
+    if (it != g_tune.end()) {
+      t = it->second;
+      have = true;
+    }
+  }
+  if (have) {
+    if (!p.placement) p.placement = t.placement;
+    if (!p.split) p.split = t.split;
+    if (!p.chunks1) p.chunks1 = t.chunks1;
+    if (!p.chunks2) p.chunks2 = t.chunks2;
+  }
+  return p;
+}
+
 // Pass-1 layout and argument arrays; *changed when a launch shape differs
 // from the one the graph of this k was captured with.
 static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* changed) {
@@ -1331,22 +1434,23 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
   std::vector<uint32_t> blk1(n + 1, 0), blkc(n + 1, 0);
   b->max_nbins = 1;
   b->max_sb = 6;
+  const fb_build_params prm = batch_params(b, k);
   // one pass-1 placement for the whole batch: LDS bins unless asked for
   // global atomics or a sketch has more bins than the scatter kernel holds
-  bool binned = b->prm.placement != 1;
+  bool binned = prm.placement != 1;
   for (auto& it : b->iso) {
     uint32_t sb = kBinLog2Cells;
     while (sb > 6 && (1ull << sb) > it->cms.width * 4) --sb;
     if (((it->cms.width * 4) >> sb) > 8192) binned = false;
   }
-  if (b->prm.placement == 2 && !binned)
+  if (prm.placement == 2 && !binned)
     return set_err(c, FB_ERR_INVALID, "LDS placement needs at most 8192 pass-1 bins per isolate");
   b->binned = binned;
   for (uint32_t i = 0; i < n; ++i) {
     BatchIso& it = *b->iso[i];
     const uint64_t ncnt = it.cms.width * 4;
     it.windows = bank_windows(it.bank, k);
-    it.split = b->prm.split ? b->prm.split : default_split(k, it.bank->max_len);
+    it.split = prm.split ? prm.split : default_split(k, it.bank->max_len);
     it.threads1 = it.bank->n_reads * (uint64_t)it.split;
     uint32_t sb = kBinLog2Cells;
     while (sb > 6 && (1ull << sb) > ncnt) --sb;
@@ -1354,7 +1458,7 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
     it.sb = sb;
     it.nbins = (uint32_t)nbins;
     it.binned = binned;
-    it.chunks1 = scatter_chunks(it.threads1, kP1BlockSize);
+    it.chunks1 = prm.chunks1 ? prm.chunks1 : scatter_chunks(it.threads1, kP1BlockSize);
     if (binned) {
       const uint64_t n_wg = scatter_blocks(it.threads1, it.chunks1, kP1BlockSize);
       const uint64_t cap = bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg, 4);
@@ -1441,6 +1545,7 @@ static int batch_enqueue1(fb_batch* b, uint32_t k, uint32_t t_solid) {
 // Pass-2 layout from the gate occupancies; *changed as for pass 1.
 static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vector<uint64_t>& occ, bool* changed) {
   fb_ctx* c = b->ctx;
+  const fb_build_params prm = batch_params(b, k);
   const uint32_t n = (uint32_t)b->iso.size();
   const uint32_t kw = key_words_for_k(k), stride = table_stride(kw), nref = ref_words(k);
   const uint32_t rw = (2 * k + 8 + 63) / 64;
@@ -1456,8 +1561,8 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     BatchIso& it = *b->iso[i];
     it.est_keys = est_keys_from_occ(occ[i], it.cms.width, it.windows);
     const P2Plan q = plan_pass2(it.est_keys, it.windows, it.bank->n_reads, it.bank->max_len, k, it.split,
-                                b->prm.bins2, p2_lds_cap(kw), p2_max_bins());
-    const bool lds = q.ok && b->prm.placement != 1;
+                                prm.bins2, p2_lds_cap(kw), p2_max_bins(), prm.chunks2);
+    const bool lds = q.ok && prm.placement != 1;
     it.lds = lds;
     if (!lds) continue;
     it.nb2 = q.nb2;
@@ -1475,13 +1580,12 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     if (!r) r = grow(it.runs, (q.nbr * kBinSlices + 1) * 4);
     if (!r) r = grow(it.slots, q.spill_cap * stride * 8);
     it.spill_cap = q.spill_cap;
-    if (!r && q.out_cap > it.out_cap) {
-      r = it.keys.get(c, q.out_cap * nref * 8);
-      if (!r) r = it.cov.get(c, q.out_cap * 2);
-      if (!r) r = it.mask.get(c, q.out_cap);
-      it.out_cap = q.out_cap;
-    }
+    // entries are nref words wide, so the key buffer is checked in bytes at this k
+    if (!r) r = grow(it.keys, q.out_cap * nref * 8);
+    if (!r) r = grow(it.cov, q.out_cap * 2);
+    if (!r) r = grow(it.mask, q.out_cap);
     if (r) return r;
+    it.out_cap = q.out_cap;
     DevSolidTable t;
     t.slots = it.slots.as<uint64_t>();
     t.cap = it.spill_cap;
@@ -1642,12 +1746,13 @@ static int batch_global_route(fb_ctx* c, BatchIso& it, uint32_t k, uint32_t gate
     cap *= 2;
   }
   const uint64_t maxn = std::max<uint64_t>(st[1], 1);
-  if (maxn > it.out_cap || !it.keys.p) {
+  if (!it.keys.p || it.keys.bytes < maxn * nref * 8)
     if (int r = it.keys.get(c, maxn * nref * 8)) return r;
+  if (!it.cov.p || it.cov.bytes < maxn * 2)
     if (int r = it.cov.get(c, maxn * 2)) return r;
+  if (!it.mask.p || it.mask.bytes < maxn)
     if (int r = it.mask.get(c, maxn)) return r;
-    it.out_cap = maxn;
-  }
+  it.out_cap = maxn;
   HIP_TRY(c, launch_pass3(t, k, nref, gate, it.keys.as<uint64_t>(), it.cov.as<uint16_t>(), it.mask.as<uint8_t>(),
                           it.p3.p, s));
   uint32_t cnt[4] = {0, 0, 0, 0};
@@ -2222,4 +2327,119 @@ extern "C" int fb_server_stats(fb_server* sv, fb_serve_stats* out) {
   out->inflight_bytes = sv->adm->used();
   out->device_bytes = sv->ctx->held;
   return FB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner entry points
+
+extern "C" int fb_batch_autotune(fb_batch* b, uint32_t k, int32_t target, fb_tune_result* out) {
+  if (!b || !out) return FB_ERR_INVALID;
+  fb_ctx* c = b->ctx;
+  memset(out, 0, sizeof *out);
+  const size_t n = b->iso.size();
+  // one candidate: a warm-up run (allocations, graph capture) then a timed
+  // run; a warm-up already twice as slow as the best ends the candidate
+  std::vector<uint64_t> ref_counts;
+  auto trial = [&](const TuneChoice& t, float best, float* ms) -> int {
+    b->tuned[k] = t;
+    if (int r = fb_batch_run(b, k, target)) return r;
+    float warm = b->ms[3];
+    *ms = warm;
+    if (best <= 0.0f || warm < 2.0f * best) {
+      if (int r = fb_batch_run(b, k, target)) return r;
+      *ms = b->ms[3];
+    }
+    ++out->trials;
+    std::vector<uint64_t> counts(n);
+    for (size_t i = 0; i < n; ++i) counts[i] = b->iso[i]->n_out;
+    if (ref_counts.empty()) ref_counts = counts;
+    else if (counts != ref_counts)
+      return set_err(c, FB_ERR_INVALID, "autotune: launch shapes gave different solid sets at k=%u", k);
+    return FB_OK;
+  };
+  TuneChoice best;
+  float best_ms = 0.0f;
+  if (int r = trial(best, 0.0f, &best_ms)) return r;
+  out->default_ms = best_ms;
+  auto sweep = [&](uint32_t TuneChoice::*field, std::initializer_list<uint32_t> values) -> int {
+    for (uint32_t v : values) {
+      TuneChoice t = best;
+      t.*field = v;
+      float ms = 0.0f;
+      if (int r = trial(t, best_ms, &ms)) return r;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+    return FB_OK;
+  };
+  const fb_build_params& p = b->prm;
+  if (!p.placement)
+    if (int r = sweep(&TuneChoice::placement, {1u, 2u})) return r;
+  if (!p.split)
+    if (int r = sweep(&TuneChoice::split, {1u, 2u, 3u, 4u, 6u})) return r;
+  if (!p.chunks1 && best.placement != 1)
+    if (int r = sweep(&TuneChoice::chunks1, {1u, 2u, 4u, 8u, 16u})) return r;
+  if (!p.chunks2 && best.placement != 1)
+    if (int r = sweep(&TuneChoice::chunks2, {1u, 2u, 4u, 8u, 16u})) return r;
+  best.ms = best_ms;
+  b->tuned[k] = best;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[batch_tune_key(b, k)] = best;
+  }
+  out->placement = best.placement;
+  out->split = best.split;
+  out->chunks1 = best.chunks1;
+  out->chunks2 = best.chunks2;
+  out->best_ms = best_ms;
+  // leave the batch's results those of the choice
+  return fb_batch_run(b, k, target);
+}
+
+extern "C" int fb_autotune_save(const char* path) {
+  if (!path) return FB_ERR_INVALID;
+  FILE* f = fopen(path, "w");
+  if (!f) return FB_ERR_INVALID;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  fprintf(f, "# kind limbs bits log2w log2_bases log2_n placement split chunks1 chunks2 ms\n");
+  for (auto& kv : g_tune)
+    fprintf(f, "%u %u %u %u %u %u %u %u %u %u %.4f\n", kv.first.kind, kv.first.limbs, kv.first.bits, kv.first.log2w,
+            kv.first.log2_bases, kv.first.log2_n, kv.second.placement, kv.second.split, kv.second.chunks1,
+            kv.second.chunks2, (double)kv.second.ms);
+  return fclose(f) == 0 ? FB_OK : FB_ERR_INVALID;
+}
+
+extern "C" int fb_autotune_load(const char* path) {
+  if (!path) return FB_ERR_INVALID;
+  FILE* f = fopen(path, "r");
+  if (!f) return FB_ERR_INVALID;
+  char line[256];
+  std::map<TuneKey, TuneChoice> got;
+  int rc = FB_OK;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    TuneKey key{};
+    TuneChoice t;
+    double ms = 0.0;
+    if (sscanf(line, "%u %u %u %u %u %u %u %u %u %u %lf", &key.kind, &key.limbs, &key.bits, &key.log2w,
+               &key.log2_bases, &key.log2_n, &t.placement, &t.split, &t.chunks1, &t.chunks2, &ms) != 11 ||
+        t.placement > 2 || t.split > 64 || t.chunks1 > 64 || t.chunks2 > 64) {
+      rc = FB_ERR_INVALID;
+      break;
+    }
+    t.ms = (float)ms;
+    got[key] = t;
+  }
+  fclose(f);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  for (auto& kv : got) g_tune[kv.first] = kv.second;
+  return FB_OK;
+}
+
+extern "C" void fb_autotune_clear(void) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune.clear();
 }

@@ -66,10 +66,14 @@ const char* fb_last_error(const fb_ctx* ctx);
 int fb_ctx_synchronize(fb_ctx* ctx);
 /* Device bytes currently held by the context's objects and caches. */
 uint64_t fb_ctx_device_bytes(const fb_ctx* ctx);
+/* Returns the context's cached free buffers to the device (objects keep theirs). */
+int fb_ctx_trim(fb_ctx* ctx);
 /* Diagnostics, no counterpart in the reference: one plain device-to-device
  * copy of `bytes` (kernel k_copy_calibration reads and writes bytes rounded
  * up to 16), for scaling hardware traffic counters against a known size. */
 int fb_diag_copy(fb_ctx* ctx, uint64_t bytes);
+/* Free and total device memory of `device` in bytes. */
+int fb_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Library version string. */
 const char* fb_version(void);
 
@@ -134,6 +138,8 @@ typedef struct {
   uint32_t split;             /* threads per read; 0 = tuned default */
   uint32_t placement;         /* sketch and solid-table placement: 0 auto, 1 global tables, 2 LDS bins */
   uint32_t bins2;             /* pass-2 LDS bins over all rounds (power of two <= 2^20); 0 = sized from the sketch */
+  uint32_t chunks1;           /* thread shares per pass-1 scatter thread; 0 = tuned default */
+  uint32_t chunks2;           /* thread shares per pass-2 count/scatter thread; 0 = tuned default */
 } fb_build_params;
 
 void fb_build_params_default(fb_build_params* p, uint32_t k);
@@ -210,6 +216,27 @@ int fb_batch_timings(const fb_batch* b, float out[4]);
 /* Pass-2 route of every isolate in the last run (n entries, codes as
  * fb_solid_stats[4]).  Diagnostics, no counterpart in the reference. */
 int fb_batch_routes(const fb_batch* b, uint32_t* out);
+
+/* ---- Autotuner (no counterpart in the reference) ----------------------------
+ * Picks the launch shapes of a batch at k by timing: pass-1 placement (LDS
+ * bins or global atomics), threads per read, and thread shares per scatter
+ * thread of passes 1 and 2, by coordinate descent from the defaults.  Fields
+ * the batch's params set explicitly are not tuned.  Every candidate must give
+ * the same solid-set sizes as the defaults (FB_ERR_INVALID otherwise).  The
+ * choice is kept for the batch at k and, for the process, for every batch of
+ * the same shape (k-mer width, counter bits, sketch width, bases per
+ * isolate and isolate count, each rounded to a power of two); batches and
+ * serving iterations of that shape use it unless their params say otherwise. */
+typedef struct {
+  uint32_t placement, split, chunks1, chunks2;  /* the choice */
+  float best_ms, default_ms;                    /* whole-run times of the choice and of the defaults */
+  uint32_t trials;                              /* candidates timed */
+} fb_tune_result;
+int fb_batch_autotune(fb_batch* b, uint32_t k, int32_t target_solid_cov, fb_tune_result* out);
+/* The process's tuned choices as text (one line per shape), read back by load. */
+int fb_autotune_save(const char* path);
+int fb_autotune_load(const char* path);
+void fb_autotune_clear(void);
 
 /* ---- Genome estimator (estimator.rs) -------------------------------------- */
 typedef struct {

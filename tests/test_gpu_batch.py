@@ -107,3 +107,37 @@ def test_batch_matches_plain_path_on_larger_isolates(ctx):
             assert np.array_equal(got.cms_table, single.cms_table)
             assert np.array_equal(got.hll_registers, single.hll_registers)
         assert batch.routes() == ["lds"] * 3
+
+
+def test_autotune_keeps_results_and_carries_to_same_shape(ctx, orc, isolates):
+    """The tuner times launch shapes on the batch; every shape computes the
+    same sets, the choice is kept, and a new batch of the same shape uses it."""
+    F._lib.load().fb_autotune_clear()
+    banks = [_bank(x) for x in isolates[:4]]
+    dev = [F.DeviceBank(ctx, b) for b in banks]
+    width = 1 << 16
+    batch = F.Batch(ctx, dev, [50_000] * 4, cms_width=width, counter_bits=16, hll_p=12)
+    r = batch.autotune(55, 4)
+    assert r["trials"] >= 10 and r["placement"] in (1, 2) and r["best_ms"] <= r["default_ms"] * 1.5
+    for i, b in enumerate(banks):
+        ref = orc.build_solid(oracle_arenas(b), 55, target_solid_cov=4, width=width, counter_bits=16, hll_p=12)
+        got = batch.result(i, 55)
+        assert np.array_equal(got.keys, ref["keys"]) and np.array_equal(got.cov, ref["cov"])
+    again = F.Batch(ctx, dev, [50_000] * 4, cms_width=width, counter_bits=16, hll_p=12)
+    again.run(55, 4)
+    for i in range(4):
+        assert np.array_equal(again.result(i, 55).keys, batch.result(i, 55).keys)
+    F._lib.load().fb_autotune_clear()
+
+
+@pytest.mark.parametrize("chunks1,chunks2,split", [(1, 1, 1), (16, 16, 6), (3, 5, 2)])
+def test_launch_shapes_do_not_change_results(ctx, orc, chunks1, chunks2, split):
+    iso = isolate(seed=41, genome_len=150_000, n_pairs=30_000)
+    bank = iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    ref = orc.build_solid(oracle_arenas(bank), 31, target_solid_cov=3, width=1 << 18, counter_bits=16, hll_p=14)
+    g = F.build_solid_kmers(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 18, counter_bits=16, hll_p=14,
+                            placement=2, chunks1=chunks1, chunks2=chunks2, split=split)
+    assert np.array_equal(g.keys, ref["keys"]) and np.array_equal(g.cov, ref["cov"])
+    assert np.array_equal(g.mask, ref["mask"]) and np.array_equal(g.cms_table, ref["cms"].table())
+    assert np.array_equal(g.hll_registers, ref["hll"])

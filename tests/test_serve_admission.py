@@ -64,3 +64,24 @@ def test_server_needs_a_gpu_or_refuses_bad_params():
     assert _lib.load().fb_server_create(0, C.byref(p), C.byref(h)) == _lib.FB_ERR_INVALID
     with pytest.raises(ValueError):
         F.serve_params((21, 55), (None,))
+
+
+def test_autotune_choices_save_and_load(tmp_path):
+    lib = _lib.load()
+    lib.fb_autotune_clear()
+    good = tmp_path / "tune.txt"
+    good.write_text("# kind limbs bits log2w log2_bases log2_n placement split chunks1 chunks2 ms\n"
+                    "0 2 16 23 29 4 2 3 8 4 12.5\n0 1 16 24 29 0 2 2 16 8 25.0\n")
+    assert lib.fb_autotune_load(str(good).encode()) == _lib.FB_OK
+    out = tmp_path / "saved.txt"
+    assert lib.fb_autotune_save(str(out).encode()) == _lib.FB_OK
+    rows = [l.split() for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert sorted(r[:10] for r in rows) == sorted([["0", "1", "16", "24", "29", "0", "2", "2", "16", "8"],
+                                                  ["0", "2", "16", "23", "29", "4", "2", "3", "8", "4"]])
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 2 16 23 29 4 9 3 8 4 1.0\n")  # placement 9 does not exist
+    assert lib.fb_autotune_load(str(bad).encode()) == _lib.FB_ERR_INVALID
+    assert lib.fb_autotune_load(str(tmp_path / "absent.txt").encode()) == _lib.FB_ERR_INVALID
+    lib.fb_autotune_clear()
+    assert lib.fb_autotune_save(str(out).encode()) == _lib.FB_OK
+    assert [l for l in out.read_text().splitlines() if not l.startswith("#")] == []

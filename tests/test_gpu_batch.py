@@ -118,7 +118,8 @@ def test_autotune_keeps_results_and_carries_to_same_shape(ctx, orc, isolates):
     width = 1 << 16
     batch = F.Batch(ctx, dev, [50_000] * 4, cms_width=width, counter_bits=16, hll_p=12)
     r = batch.autotune(55, 4)
-    assert r["trials"] >= 10 and r["placement"] in (1, 2) and r["best_ms"] <= r["default_ms"] * 1.5
+    # the defaults (all 0) may stay the choice; a choice is never slower than them
+    assert r["trials"] >= 10 and r["placement"] in (0, 1, 2) and r["best_ms"] <= r["default_ms"]
     for i, b in enumerate(banks):
         ref = orc.build_solid(oracle_arenas(b), 55, target_solid_cov=4, width=width, counter_bits=16, hll_p=12)
         got = batch.result(i, 55)

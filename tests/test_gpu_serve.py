@@ -90,12 +90,13 @@ def test_serve_sketch_sized_from_the_estimate(ctx, requests_):
     assert np.array_equal(out.passes[0].keys, plain.keys) and np.array_equal(out.passes[0].cms_table, plain.cms_table)
 
 
-def test_serve_budget_admits_one_at_a_time(requests_):
+def test_serve_budget_admits_one_at_a_time():
     """A budget that holds any one request but no two: the next is admitted
     only after the one before it has finished and left; results are the same
     as with room for all."""
     kw = dict(cms_width=1 << 17, counter_bits=16, hll_p=12)
-    banks = requests_[:3]
+    # equal-sized requests: any one fits the budget below, no two do
+    banks = [isolate(seed=600 + i, genome_len=50_000, n_pairs=6000, gc=0.4 + 0.1 * i).bank() for i in range(3)]
     with F.Server(0, KS, TARGETS, **kw) as sv:
         free = [sv.wait(sv.submit(b.pe_forward, b.pe_reverse), timeout=60) for b in banks]
     fps = [o.footprint for o in free]

@@ -1254,3 +1254,389 @@ int orc_build_solid_mt(const orc_arena* arenas, int n_arenas, const uint8_t* con
     else orc_cms_free(c.cms);
     return 0;
 }
+
+/* ==========================================================================
+ * Unitig compaction: src/graph/build.rs:330-597, on a sorted solid set.
+ * ========================================================================== */
+
+typedef struct {
+    uint32_t target;   /* DirUnitigId */
+    uint8_t ov;
+    uint16_t support;
+} cg_edge;
+typedef struct {
+    cg_edge* e;
+    uint32_t n, cap;
+} cg_elist;
+
+typedef struct {
+    const uint64_t* keys;
+    const uint16_t* cov;
+    const uint8_t* mask;
+    uint8_t* visited;
+    uint64_t n;
+    uint32_t k;
+    int nw;
+    /* unitigs */
+    uint8_t* seq;
+    uint64_t seq_len, seq_cap;
+    uint64_t* off;
+    uint32_t* len;
+    float* ucov;
+    uint8_t* circ;
+    cg_elist* efwd;
+    cg_elist* erev;
+    uint64_t nu, ucap;
+} cg_t;
+
+/* the k-mer's index in the sorted set, or -1 (hash-map lookup of the reference) */
+static int64_t cg_find(const cg_t* g, const km_t* x) {
+    uint64_t lo = 0, hi = g->n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        const km_t m = km_load(g->keys + mid * (uint64_t)g->nw, g->nw);
+        if (km_lt(&m, x, g->nw)) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < g->n) {
+        const km_t m = km_load(g->keys + lo * (uint64_t)g->nw, g->nw);
+        if (km_eq(&m, x, g->nw)) return (int64_t)lo;
+    }
+    return -1;
+}
+
+/* base i (0 = first, most significant) of a k-mer: KmerInt::to_ascii */
+static uint8_t km_base(const km_t* x, uint32_t k, int nw, uint32_t i) {
+    const uint32_t bit = 2 * (k - 1 - i); /* from the least significant end */
+    const uint32_t word = (uint32_t)nw - 1 - bit / 64;
+    return (uint8_t)((x->w[word] >> (bit % 64)) & 3);
+}
+static const uint8_t CG_ASCII[4] = {'A', 'C', 'G', 'T'};
+static uint8_t cg_comp(uint8_t c) {
+    switch (c) {
+        case 'A': return 'T';
+        case 'C': return 'G';
+        case 'G': return 'C';
+        default: return 'A';
+    }
+}
+
+static void cg_push_bytes(cg_t* g, const uint8_t* s, uint64_t n) {
+    if (g->seq_len + n > g->seq_cap) {
+        g->seq_cap = (g->seq_len + n) * 2 + 1024;
+        g->seq = (uint8_t*)realloc(g->seq, g->seq_cap);
+    }
+    memcpy(g->seq + g->seq_len, s, n);
+    g->seq_len += n;
+}
+
+static uint64_t cg_new_unitig(cg_t* g, const uint8_t* s, uint64_t n, float cov, int circular) {
+    if (g->nu == g->ucap) {
+        g->ucap = g->ucap ? g->ucap * 2 : 1024;
+        g->off = (uint64_t*)realloc(g->off, g->ucap * 8);
+        g->len = (uint32_t*)realloc(g->len, g->ucap * 4);
+        g->ucov = (float*)realloc(g->ucov, g->ucap * 4);
+        g->circ = (uint8_t*)realloc(g->circ, g->ucap);
+        g->efwd = (cg_elist*)realloc(g->efwd, g->ucap * sizeof(cg_elist));
+        g->erev = (cg_elist*)realloc(g->erev, g->ucap * sizeof(cg_elist));
+    }
+    const uint64_t id = g->nu++;
+    g->off[id] = g->seq_len;
+    g->len[id] = (uint32_t)n;
+    g->ucov[id] = cov;
+    g->circ[id] = (uint8_t)circular;
+    memset(&g->efwd[id], 0, sizeof(cg_elist));
+    memset(&g->erev[id], 0, sizeof(cg_elist));
+    cg_push_bytes(g, s, n);
+    return id;
+}
+
+/* UnitigNode::add_edge (cdbg.rs): an edge to the same target and overlap adds
+ * its support (saturating u16). */
+static void cg_add_edge(cg_elist* l, uint32_t target, uint8_t ov, uint16_t support) {
+    for (uint32_t i = 0; i < l->n; i++) {
+        if (l->e[i].target == target && l->e[i].ov == ov) {
+            const uint32_t s = (uint32_t)l->e[i].support + support;
+            l->e[i].support = (uint16_t)(s > 65535 ? 65535 : s);
+            return;
+        }
+    }
+    if (l->n == l->cap) {
+        l->cap = l->cap ? l->cap * 2 : 4;
+        l->e = (cg_edge*)realloc(l->e, l->cap * sizeof(cg_edge));
+    }
+    l->e[l->n].target = target;
+    l->e[l->n].ov = ov;
+    l->e[l->n].support = support;
+    l->n++;
+}
+
+/* the walk's oriented (fwd, rev) pair of a directed k-mer */
+static void cg_oriented(const cg_t* g, uint64_t i, int is_rc, km_t* fwd, km_t* rev) {
+    const km_t x = km_load(g->keys + i * (uint64_t)g->nw, g->nw);
+    const km_t r = km_rev_comp(x, g->k, g->nw);
+    if (!is_rc) {
+        *fwd = x;
+        *rev = r;
+    } else {
+        *fwd = r;
+        *rev = x;
+    }
+}
+
+/* One step of the walk from the oriented (fwd, rev) by base b: the next
+ * directed k-mer and its index (-1 outside the set); the twin edge on the
+ * next k-mer is marked visited (build.rs:377-390, 425-432). */
+static int64_t cg_step(cg_t* g, const km_t* fwd, const km_t* rev, uint8_t b, int* next_rc) {
+    km_t nf = *fwd, nr = *rev;
+    km_roll(&nf, &nr, b, g->k, g->nw);
+    const int is_rc = km_lt(&nr, &nf, g->nw);
+    const km_t can = is_rc ? nr : nf;
+    const uint8_t rc_first = (uint8_t)(3 - km_base(fwd, g->k, g->nw, 0));
+    const int twin_edge = !is_rc ? rc_first + 4 : rc_first;
+    const int64_t j = cg_find(g, &can);
+    if (j >= 0) g->visited[j] |= (uint8_t)(1u << twin_edge);
+    *next_rc = is_rc;
+    return j;
+}
+
+/* The walk loop over 1-in/1-out k-mers (build.rs:393-439 and 486-523).
+ * stop_at >= 0: also ends when the canonical k-mer returns to stop_at. */
+static int64_t cg_walk(cg_t* g, int64_t cur, int cur_rc, int64_t stop_at, uint8_t** path, uint64_t* plen,
+                       uint64_t* pcap, uint64_t* total_cov, uint64_t* points, int* out_rc) {
+    while (cur >= 0 && cur != stop_at) {
+        const uint8_t m = g->mask[cur];
+        const uint8_t out_m = !cur_rc ? (m & 0x0F) : ((m >> 4) & 0x0F);
+        const uint8_t in_m = !cur_rc ? ((m >> 4) & 0x0F) : (m & 0x0F);
+        if (__builtin_popcount(in_m) != 1 || __builtin_popcount(out_m) != 1) break;
+        const uint8_t nb = (uint8_t)__builtin_ctz(out_m);
+        const int edge_bit = !cur_rc ? nb : nb + 4;
+        if (g->visited[cur] & (1u << edge_bit)) break;
+        g->visited[cur] |= (uint8_t)(1u << edge_bit);
+        km_t cf, cr;
+        cg_oriented(g, (uint64_t)cur, cur_rc, &cf, &cr);
+        int nrc = 0;
+        const int64_t nxt = cg_step(g, &cf, &cr, nb, &nrc);
+        if (*plen == *pcap) {
+            *pcap *= 2;
+            *path = (uint8_t*)realloc(*path, *pcap);
+        }
+        (*path)[(*plen)++] = CG_ASCII[nb];
+        *total_cov += g->cov[cur];
+        *points += 1;
+        cur = nxt;
+        cur_rc = nrc;
+        if (cur < 0) break;
+    }
+    *out_rc = cur_rc;
+    return cur;
+}
+
+int orc_compact(const uint64_t* keys, const uint16_t* cov, const uint8_t* mask, uint64_t n, uint32_t k,
+                orc_graph* out) {
+    memset(out, 0, sizeof *out);
+    if (!check_k(k)) return -1;
+    cg_t g;
+    memset(&g, 0, sizeof g);
+    g.keys = keys;
+    g.cov = cov;
+    g.mask = mask;
+    g.n = n;
+    g.k = k;
+    g.nw = orc_kmer_words_for_k(k);
+    g.visited = (uint8_t*)calloc(n ? n : 1, 1);
+    uint64_t pcap = 1024, plen = 0;
+    uint8_t* path = (uint8_t*)malloc(pcap);
+    uint8_t* rcbuf = NULL;
+    uint64_t rccap = 0;
+
+    /* 4. walks from boundary directed k-mers (build.rs:332-449) */
+    for (uint64_t i = 0; i < n; i++) {
+        for (int urc = 0; urc < 2; urc++) {
+            const uint8_t m0 = mask[i];
+            const int fo = __builtin_popcount(m0 & 0x0F), fi = __builtin_popcount((m0 >> 4) & 0x0F);
+            const int boundary = !urc ? (fi != 1 || fo != 1) : (fo != 1 || fi != 1);
+            if (!boundary) continue;
+            for (uint8_t b = 0; b < 4; b++) {
+                const uint8_t m = mask[i];
+                const uint8_t u_out = !urc ? (m & 0x0F) : ((m >> 4) & 0x0F);
+                const uint8_t u_vis = !urc ? (g.visited[i] & 0x0F) : ((g.visited[i] >> 4) & 0x0F);
+                if (!(u_out & (1u << b)) || (u_vis & (1u << b))) continue;
+                g.visited[i] |= (uint8_t)(1u << (!urc ? b : b + 4));
+                km_t fwd, rev;
+                cg_oriented(&g, i, urc, &fwd, &rev);
+                plen = 0;
+                for (uint32_t j = 0; j < k; j++) path[plen++] = CG_ASCII[km_base(&fwd, k, g.nw, j)];
+                int crc = 0;
+                int64_t cur = cg_step(&g, &fwd, &rev, b, &crc);
+                if (plen == pcap) {
+                    pcap *= 2;
+                    path = (uint8_t*)realloc(path, pcap);
+                }
+                path[plen++] = CG_ASCII[b];
+                uint64_t total = cov[i], points = 1;
+                int end_rc = 0;
+                (void)cg_walk(&g, cur, crc, -1, &path, &plen, &pcap, &total, &points, &end_rc);
+                if (plen > rccap) {
+                    rccap = plen * 2;
+                    rcbuf = (uint8_t*)realloc(rcbuf, rccap);
+                }
+                for (uint64_t j = 0; j < plen; j++) rcbuf[j] = cg_comp(path[plen - 1 - j]);
+                const uint8_t* fin = memcmp(rcbuf, path, plen) < 0 ? rcbuf : path;
+                float c = (float)total / (float)points;
+                if (!(c >= 1.0f)) c = 1.0f;
+                cg_new_unitig(&g, fin, plen, c, 0);
+            }
+        }
+    }
+
+    /* 5. isolated circular components (build.rs:452-549) */
+    for (uint64_t i = 0; i < n; i++) {
+        const uint8_t m = mask[i];
+        const uint8_t fo = m & 0x0F, fi = (m >> 4) & 0x0F;
+        if (__builtin_popcount(fo) != 1 || __builtin_popcount(fi) != 1) continue;
+        const uint8_t b = (uint8_t)__builtin_ctz(fo);
+        if (g.visited[i] & (1u << b)) continue;
+        g.visited[i] |= (uint8_t)(1u << b);
+        km_t fwd, rev;
+        cg_oriented(&g, i, 0, &fwd, &rev);
+        plen = 0;
+        for (uint32_t j = 0; j < k; j++) path[plen++] = CG_ASCII[km_base(&fwd, k, g.nw, j)];
+        int crc = 0;
+        const int64_t cur = cg_step(&g, &fwd, &rev, b, &crc);
+        if (plen == pcap) {
+            pcap *= 2;
+            path = (uint8_t*)realloc(path, pcap);
+        }
+        path[plen++] = CG_ASCII[b];
+        uint64_t total = cov[i], points = 1;
+        int end_rc = 0;
+        const int64_t end = cg_walk(&g, cur, crc, (int64_t)i, &path, &plen, &pcap, &total, &points, &end_rc);
+        if (end == (int64_t)i && plen > k) {
+            if (memcmp(path + plen - k, path, k) == 0) plen -= k;
+            else if (plen > k - 1 && memcmp(path + plen - (k - 1), path, k - 1) == 0) plen -= k - 1;
+            float c = (float)total / (float)points;
+            if (!(c >= 1.0f)) c = 1.0f;
+            const uint64_t id = cg_new_unitig(&g, path, plen, c, 1);
+            cg_add_edge(&g.efwd[id], (uint32_t)(id << 1), (uint8_t)(k - 1), 10);
+        }
+    }
+
+    /* 6. edges by exact k and k-1 overlaps (build.rs:552-594) */
+    const uint64_t nu = g.nu;
+    for (int pass = 0; pass < 2; pass++) {
+        const uint32_t ov = pass == 0 ? k : k - 1;
+        if (ov == 0) continue;
+        /* prefix map: (prefix, DirUnitigId) sorted by prefix then insertion order */
+        uint64_t np = 0;
+        uint32_t* ent = (uint32_t*)malloc((nu ? nu : 1) * 2 * sizeof(uint32_t));
+        uint8_t* rcs = (uint8_t*)malloc((size_t)ov * (nu ? nu : 1));
+        for (uint64_t id = 0; id < nu; id++) {
+            if (g.len[id] < ov) continue;
+            const uint8_t* s = g.seq + g.off[id];
+            const uint32_t L = g.len[id];
+            for (uint32_t j = 0; j < ov; j++) rcs[id * ov + j] = cg_comp(s[L - 1 - j]);
+            ent[np++] = (uint32_t)(id << 1);
+            ent[np++] = (uint32_t)(id << 1 | 1);
+        }
+        /* insertion sort is too slow for large graphs: sort by (prefix, dir id) */
+        typedef struct { const uint8_t* p; uint32_t d; } pe_t;
+        pe_t* pe = (pe_t*)malloc((np ? np : 1) * sizeof(pe_t));
+        for (uint64_t q = 0; q < np; q++) {
+            const uint32_t d = ent[q], id = d >> 1;
+            pe[q].p = (d & 1) ? rcs + (uint64_t)id * ov : g.seq + g.off[id];
+            pe[q].d = d;
+        }
+        /* merge sort on (prefix bytes, d) */
+        for (uint64_t width = 1; width < np; width *= 2) {
+            pe_t* tmp = (pe_t*)malloc(np * sizeof(pe_t));
+            for (uint64_t lo = 0; lo < np; lo += 2 * width) {
+                uint64_t mid = lo + width < np ? lo + width : np, hi = lo + 2 * width < np ? lo + 2 * width : np;
+                uint64_t a = lo, bb = mid, o = lo;
+                while (a < mid && bb < hi) {
+                    const int c = memcmp(pe[a].p, pe[bb].p, ov);
+                    if (c < 0 || (c == 0 && pe[a].d <= pe[bb].d)) tmp[o++] = pe[a++];
+                    else tmp[o++] = pe[bb++];
+                }
+                while (a < mid) tmp[o++] = pe[a++];
+                while (bb < hi) tmp[o++] = pe[bb++];
+            }
+            memcpy(pe, tmp, np * sizeof(pe_t));
+            free(tmp);
+        }
+        for (uint64_t id = 0; id < nu; id++) {
+            const uint32_t L = g.len[id];
+            if (L < ov) continue;
+            const uint8_t* s = g.seq + g.off[id];
+            /* fwd suffix = s[L-ov..]; rc suffix = rc(s)[L-ov..] = rc of s[0..ov) */
+            uint8_t* rcsuf = (uint8_t*)malloc(ov);
+            for (uint32_t j = 0; j < ov; j++) rcsuf[j] = cg_comp(s[ov - 1 - j]);
+            for (int side = 0; side < 2; side++) {
+                const uint8_t* suf = side == 0 ? s + (L - ov) : rcsuf;
+                uint64_t lo = 0, hi = np;
+                while (lo < hi) {
+                    const uint64_t mid = (lo + hi) >> 1;
+                    if (memcmp(pe[mid].p, suf, ov) < 0) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (uint64_t q = lo; q < np && memcmp(pe[q].p, suf, ov) == 0; q++) {
+                    const uint32_t t = pe[q].d;
+                    if ((t >> 1) != id || g.circ[id])
+                        cg_add_edge(side == 0 ? &g.efwd[id] : &g.erev[id], t, (uint8_t)ov, 10);
+                }
+            }
+            free(rcsuf);
+        }
+        free(pe);
+        free(rcs);
+        free(ent);
+    }
+
+    /* flatten */
+    out->n_unitigs = nu;
+    out->seq = g.seq ? g.seq : (uint8_t*)malloc(1);
+    out->offsets = g.off;
+    out->lengths = g.len;
+    out->cov = g.ucov;
+    out->circular = g.circ;
+    uint64_t ne = 0;
+    for (uint64_t id = 0; id < nu; id++) ne += g.efwd[id].n + g.erev[id].n;
+    out->n_edges = ne;
+    out->e_src = (uint32_t*)malloc((ne ? ne : 1) * 4);
+    out->e_dst = (uint32_t*)malloc((ne ? ne : 1) * 4);
+    out->e_ov = (uint8_t*)malloc(ne ? ne : 1);
+    out->e_support = (uint16_t*)malloc((ne ? ne : 1) * 2);
+    uint64_t q = 0;
+    for (uint64_t id = 0; id < nu; id++) {
+        for (int side = 0; side < 2; side++) {
+            const cg_elist* l = side == 0 ? &g.efwd[id] : &g.erev[id];
+            for (uint32_t j = 0; j < l->n; j++, q++) {
+                out->e_src[q] = (uint32_t)(id << 1 | (uint32_t)side);
+                out->e_dst[q] = l->e[j].target;
+                out->e_ov[q] = l->e[j].ov;
+                out->e_support[q] = l->e[j].support;
+            }
+            free(l->e);
+        }
+    }
+    free(g.efwd);
+    free(g.erev);
+    free(g.visited);
+    free(path);
+    free(rcbuf);
+    return 0;
+}
+
+void orc_graph_free(orc_graph* g) {
+    if (!g) return;
+    free(g->seq);
+    free(g->offsets);
+    free(g->lengths);
+    free(g->cov);
+    free(g->circular);
+    free(g->e_src);
+    free(g->e_dst);
+    free(g->e_ov);
+    free(g->e_support);
+    memset(g, 0, sizeof *g);
+}

@@ -169,6 +169,33 @@ int orc_build_solid_mt(const orc_arena* arenas, int n_arenas,
                        orc_cms** cms_out, uint8_t* hll_out, orc_solid_set* out);
 void orc_solid_free(orc_solid_set* s);
 
+/* ---- Unitig compaction (src/graph/build.rs:330-597) -------------------------
+ * Input: a solid set sorted by k-mer (keys nw words MSB first, cov, mask).
+ * The reference walks its hash-map shards in their iteration order, which
+ * decides the direction each linear unitig is walked in (its cov skips the
+ * node it ends on), where a circular unitig starts, and the node numbering.
+ * This restatement takes the k-mers in ascending order: boundary directed
+ * k-mers forward then reverse per k-mer, cycles by their smallest k-mer.
+ * Unitig i: seq[offsets[i] .. + lengths[i]) (ASCII), cov (f32), circular.
+ * Edges in each node's insertion order, forward side then reverse side:
+ * src = id << 1 | side, dst = DirUnitigId (id << 1 | is_rc), ov, support. */
+typedef struct {
+    uint64_t n_unitigs;
+    uint8_t* seq;
+    uint64_t* offsets;
+    uint32_t* lengths;
+    float* cov;
+    uint8_t* circular;
+    uint64_t n_edges;
+    uint32_t* e_src;
+    uint32_t* e_dst;
+    uint8_t* e_ov;
+    uint16_t* e_support;
+} orc_graph;
+int orc_compact(const uint64_t* keys, const uint16_t* cov, const uint8_t* mask, uint64_t n, uint32_t k,
+                orc_graph* out);
+void orc_graph_free(orc_graph* g);
+
 #ifdef __cplusplus
 }
 #endif

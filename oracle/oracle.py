@@ -68,6 +68,12 @@ class SolidSetC(C.Structure):
     _fields_ = [("n", C.c_uint64), ("nw", C.c_int), ("keys", u64p), ("cov", u16p), ("mask", u8p)]
 
 
+class GraphC(C.Structure):
+    _fields_ = [("n_unitigs", C.c_uint64), ("seq", u8p), ("offsets", u64p), ("lengths", u32p),
+                ("cov", C.POINTER(C.c_float)), ("circular", u8p), ("n_edges", C.c_uint64), ("e_src", u32p),
+                ("e_dst", u32p), ("e_ov", u8p), ("e_support", u16p)]
+
+
 _SIGS = [
     ("orc_splitmix64", C.c_uint64, [C.c_uint64]),
     ("orc_rev_comp_word", C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -110,6 +116,8 @@ _SIGS = [
     ("orc_build_solid_mt", C.c_int, [C.POINTER(Arena), C.c_int, u8p, u32p, C.c_uint64, C.c_uint32,
                                      C.POINTER(BuildParams), C.c_int, C.POINTER(vp), u8p, C.POINTER(SolidSetC)]),
     ("orc_solid_free", None, [C.POINTER(SolidSetC)]),
+    ("orc_compact", C.c_int, [u64p, u16p, u8p, C.c_uint64, C.c_uint32, C.POINTER(GraphC)]),
+    ("orc_graph_free", None, [C.POINTER(GraphC)]),
 ]
 
 _LIB = None
@@ -372,3 +380,35 @@ def build_solid(arenas: list, k: int, target_solid_cov: int | None = None, width
     lib().orc_solid_free(C.byref(ss))
     cms = Cms(width, counter_bits, handle=cms_h.value)
     return {"keys": keys, "cov": cov, "mask": mask, "cms": cms, "hll": regs}
+
+
+def compact(keys, cov, mask, k: int) -> dict:
+    """Unitig compaction (build.rs:330-597) of a sorted solid set -> dict with
+    seqs (list of bytes), cov (float32), circular (uint8) and the edge arrays
+    src, dst, ov, support (each node's edges in insertion order, forward side
+    then reverse side)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    cov = np.ascontiguousarray(cov, dtype=np.uint16)
+    mask = np.ascontiguousarray(mask, dtype=np.uint8)
+    n = int(cov.size)
+    g = GraphC()
+    rc = lib().orc_compact(_p(keys, C.c_uint64), _p(cov, C.c_uint16), _p(mask, C.c_uint8), n, k, C.byref(g))
+    if rc:
+        raise ValueError(f"orc_compact failed ({rc})")
+    nu, ne = int(g.n_unitigs), int(g.n_edges)
+
+    def arr(ptr, count, dt):
+        if count == 0:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).astype(dt, copy=True)
+
+    offs = arr(g.offsets, nu, np.uint64)
+    lens = arr(g.lengths, nu, np.uint32)
+    total = int(offs[-1] + lens[-1]) if nu else 0
+    seq = arr(g.seq, total, np.uint8).tobytes()
+    out = {"seqs": [seq[int(o):int(o) + int(ln)] for o, ln in zip(offs, lens)],
+           "cov": arr(g.cov, nu, np.float32), "circular": arr(g.circular, nu, np.uint8),
+           "src": arr(g.e_src, ne, np.uint32), "dst": arr(g.e_dst, ne, np.uint32),
+           "ov": arr(g.e_ov, ne, np.uint8), "support": arr(g.e_support, ne, np.uint16)}
+    lib().orc_graph_free(C.byref(g))
+    return out

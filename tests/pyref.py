@@ -295,3 +295,152 @@ def count_unitigs(solid: list, k: int) -> int:
                 if end == key:
                     count += 1
     return count
+
+
+def compact(solid: list, k: int) -> dict:
+    """The reference's unitig compaction (build.rs:330-597) on a solid set
+    [(kmer_int, cov, mask)] taken in list order (the boundary walks, then the
+    cycles), written with strings: every k-mer is a string, a step appends a
+    letter and drops the first, orientation is by comparing the integer of the
+    string with that of its reverse complement.  Returns seqs, cov (as float32
+    values), circular flags and edges (src, dst, ov, support) per node in
+    insertion order, forward side then reverse side."""
+    import numpy as np
+
+    def to_str(x):
+        return "".join("ACGT"[(x >> (2 * (k - 1 - i))) & 3] for i in range(k))
+
+    nodes = {key: [mask, 0, c] for key, c, mask in solid}  # mask, visited, cov
+    order = [key for key, _, _ in solid]
+
+    def oriented(key, rc):
+        s = to_str(key)
+        return revcomp(s) if rc else s
+
+    def step(s, b):
+        """next directed k-mer after string s by base b, marking its twin edge"""
+        t = s[1:] + "ACGT"[b]
+        key, rc = canonical(t)
+        rc_first = 3 - "ACGT".index(s[0])
+        if key in nodes:
+            nodes[key][1] |= 1 << (rc_first + 4 if not rc else rc_first)
+        return key, rc
+
+    def walk(key, rc, stop):
+        path, total, points = [], 0, 0
+        while key in nodes and key != stop:
+            m, vis, c = nodes[key]
+            out_m = (m & 15) if not rc else (m >> 4)
+            in_m = (m >> 4) if not rc else (m & 15)
+            if bin(in_m).count("1") != 1 or bin(out_m).count("1") != 1:
+                break
+            nb = (out_m & -out_m).bit_length() - 1
+            eb = nb if not rc else nb + 4
+            if vis & (1 << eb):
+                break
+            nodes[key][1] |= 1 << eb
+            key, rc = step(oriented(key, rc), nb)
+            path.append("ACGT"[nb])
+            total += c
+            points += 1
+        return key, path, total, points
+
+    unitigs = []  # [seq, cov, circular, edges_fwd, edges_rev]
+
+    def f32_mean(total, points):
+        v = np.float32(np.float32(total) / np.float32(points))
+        return max(v, np.float32(1.0))
+
+    for key in order:
+        for urc in (False, True):
+            m = nodes[key][0]
+            fo, fi = bin(m & 15).count("1"), bin(m >> 4).count("1")
+            if not (fi != 1 or fo != 1):
+                continue
+            for b in range(4):
+                m, vis, c = nodes[key]
+                out_m = (m & 15) if not urc else (m >> 4)
+                v = (vis & 15) if not urc else (vis >> 4)
+                if not (out_m & (1 << b)) or v & (1 << b):
+                    continue
+                nodes[key][1] |= 1 << (b if not urc else b + 4)
+                s = oriented(key, urc)
+                nkey, nrc = step(s, b)
+                _, tail, total, points = walk(nkey, nrc, None)
+                seq = s + "ACGT"[b] + "".join(tail)
+                seq = min(seq, revcomp(seq))
+                unitigs.append([seq, f32_mean(c + total, 1 + points), False, [], []])
+    for key in order:
+        m, vis, c = nodes[key]
+        if bin(m & 15).count("1") != 1 or bin(m >> 4).count("1") != 1:
+            continue
+        b = (m & 15).bit_length() - 1
+        if vis & (1 << b):
+            continue
+        nodes[key][1] |= 1 << b
+        s = oriented(key, False)
+        nkey, nrc = step(s, b)
+        end, tail, total, points = walk(nkey, nrc, key)
+        seq = s + "ACGT"[b] + "".join(tail)
+        if end == key and len(seq) > k:
+            if seq[-k:] == seq[:k]:
+                seq = seq[:-k]
+            elif seq[-(k - 1):] == seq[:k - 1]:
+                seq = seq[:-(k - 1)]
+            u = [seq, f32_mean(c + total, 1 + points), True, [], []]
+            u[3].append([len(unitigs) << 1, k - 1, 10])
+            unitigs.append(u)
+
+    def add(lst, target, ov):
+        for e in lst:
+            if e[0] == target and e[1] == ov:
+                e[2] = min(e[2] + 10, 65535)
+                return
+        lst.append([target, ov, 10])
+
+    for ov in (k, k - 1):
+        if ov == 0:
+            continue
+        pm = {}
+        for i, u in enumerate(unitigs):
+            if len(u[0]) >= ov:
+                pm.setdefault(u[0][:ov], []).append(i << 1)
+                pm.setdefault(revcomp(u[0])[:ov], []).append(i << 1 | 1)
+        for i, u in enumerate(unitigs):
+            if len(u[0]) < ov:
+                continue
+            for side, strand in ((0, u[0]), (1, revcomp(u[0]))):
+                for t in pm.get(strand[-ov:], []):
+                    if (t >> 1) != i or u[2]:
+                        add(u[3 + side], t, ov)
+    src, dst, ovs, sup = [], [], [], []
+    for i, u in enumerate(unitigs):
+        for side in (0, 1):
+            for t, ov, s in u[3 + side]:
+                src.append(i << 1 | side)
+                dst.append(t)
+                ovs.append(ov)
+                sup.append(s)
+    return {"seqs": [u[0].encode() for u in unitigs], "cov": np.array([u[1] for u in unitigs], np.float32),
+            "circular": np.array([1 if u[2] else 0 for u in unitigs], np.uint8), "src": np.array(src, np.uint32),
+            "dst": np.array(dst, np.uint32), "ov": np.array(ovs, np.uint8), "support": np.array(sup, np.uint16)}
+
+
+def solid_from_sequences(seqs: list, k: int, cov_of=None, circular: bool = False) -> list:
+    """An exact solid set of the k-mers of error-free sequences: every
+    adjacent pair of k-mers sets the out-edge of the first and the in-edge of
+    the second (build.rs:268-275 edge encoding); cov from cov_of(key) or 10."""
+    acc = {}
+    for s in seqs:
+        t = s + s[:k] if circular else s
+        kms = [t[i:i + k] for i in range(len(t) - k + 1)]
+        for i in range(len(kms) - 1):
+            u, v = kms[i], kms[i + 1]
+            b, p = "ACGT".index(v[-1]), "ACGT".index(u[0])
+            uk, urc = canonical(u)
+            vk, vrc = canonical(v)
+            acc.setdefault(uk, 0)
+            acc.setdefault(vk, 0)
+            acc[uk] |= 1 << (b + 4 if urc else b)
+            acc[vk] |= 1 << ((3 - p) if vrc else (3 - p) + 4)
+    return sorted((key, (cov_of(key) if cov_of else 10), m) for key, m in acc.items())

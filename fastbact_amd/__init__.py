@@ -18,6 +18,8 @@ from .api import (  # noqa: F401
     batch_footprint,
     build_solid_handle,
     build_solid_kmers,
+    compact_arrays,
+    compact_unitigs,
     compute_dynamic_kmers,
     count_stats,
     estimate_genome,

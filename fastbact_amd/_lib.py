@@ -232,6 +232,15 @@ SIGNATURES = [
     ("fb_solid_kernel_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_diag_copy", C.c_int, [vp, C.c_uint64]),
     ("fb_solid_stats", C.c_int, [vp, u64p]),
+    ("fb_compact_solid", C.c_int, [vp, C.POINTER(vp)]),
+    ("fb_compact_arrays", C.c_int, [vp, u64p, u16p, u8p, C.c_uint64, C.c_uint32, C.POINTER(vp)]),
+    ("fb_graph_destroy", None, [vp]),
+    ("fb_graph_unitig_count", C.c_uint64, [vp]),
+    ("fb_graph_base_count", C.c_uint64, [vp]),
+    ("fb_graph_edge_count", C.c_uint64, [vp]),
+    ("fb_graph_unitigs", C.c_int, [vp, u8p, u64p, u32p, C.POINTER(C.c_float), u8p]),
+    ("fb_graph_edges", C.c_int, [vp, u32p, u32p, u8p, u16p]),
+    ("fb_graph_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_batch_create", C.c_int, [vp, C.POINTER(vp), u64p, C.c_uint32, C.POINTER(BuildParams), C.POINTER(vp)]),
     ("fb_batch_destroy", None, [vp]),
     ("fb_batch_footprint", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(BuildParams), C.c_uint32]),

@@ -416,6 +416,61 @@ def build_solid_kmers(ctx: Context, bank: DeviceBank, k: int, target_solid_cov: 
 
 
 # ---------------------------------------------------------------------------
+# Unitig compaction
+
+
+def _graph_result(ctx: Context, h) -> dict:
+    lib = ctx.lib
+    try:
+        nu = int(lib.fb_graph_unitig_count(h))
+        nb = int(lib.fb_graph_base_count(h))
+        ne = int(lib.fb_graph_edge_count(h))
+        seq = np.zeros(max(nb, 1), np.uint8)
+        off = np.zeros(max(nu, 1), np.uint64)
+        ln = np.zeros(max(nu, 1), np.uint32)
+        cov = np.zeros(max(nu, 1), np.float32)
+        circ = np.zeros(max(nu, 1), np.uint8)
+        ctx.check(lib.fb_graph_unitigs(h, _ptr(seq, C.c_uint8), _ptr(off, C.c_uint64), _ptr(ln, C.c_uint32),
+                                       cov.ctypes.data_as(C.POINTER(C.c_float)), _ptr(circ, C.c_uint8)),
+                  "fb_graph_unitigs")
+        src = np.zeros(max(ne, 1), np.uint32)
+        dst = np.zeros(max(ne, 1), np.uint32)
+        ov = np.zeros(max(ne, 1), np.uint8)
+        sup = np.zeros(max(ne, 1), np.uint16)
+        ctx.check(lib.fb_graph_edges(h, _ptr(src, C.c_uint32), _ptr(dst, C.c_uint32), _ptr(ov, C.c_uint8),
+                                     _ptr(sup, C.c_uint16)), "fb_graph_edges")
+        t = (C.c_float * 3)()
+        ctx.check(lib.fb_graph_timings(h, t), "fb_graph_timings")
+        raw = seq[:nb].tobytes()
+        return {"seqs": [raw[int(o):int(o) + int(n)] for o, n in zip(off[:nu], ln[:nu])], "cov": cov[:nu],
+                "circular": circ[:nu], "src": src[:ne], "dst": dst[:ne], "ov": ov[:ne], "support": sup[:ne],
+                "timings_ms": tuple(float(x) for x in t)}
+    finally:
+        lib.fb_graph_destroy(h)
+
+
+def compact_unitigs(ctx: Context, solid: SolidHandle) -> dict:
+    """Unitig compaction (build.rs:330-597) of a sorted build's solid set on
+    the device: seqs, cov (float32), circular, and the edges (src, dst, ov,
+    support) in each node's insertion order."""
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_compact_solid(solid.h, C.byref(h)), "fb_compact_solid")
+    return _graph_result(ctx, h)
+
+
+def compact_arrays(ctx: Context, keys, cov, mask, k: int) -> dict:
+    """The same compaction of a solid set given as host arrays (keys
+    strictly ascending, fb_solid_download layout)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    cov = np.ascontiguousarray(cov, dtype=np.uint16)
+    mask = np.ascontiguousarray(mask, dtype=np.uint8)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_compact_arrays(ctx.h, _ptr(keys, C.c_uint64), _ptr(cov, C.c_uint16), _ptr(mask, C.c_uint8),
+                                        int(cov.size), k, C.byref(h)), "fb_compact_arrays")
+    return _graph_result(ctx, h)
+
+
+# ---------------------------------------------------------------------------
 # Batched form
 
 

@@ -246,4 +246,51 @@ struct EstArgs {
 hipError_t launch_est_first_insert(const EstArgs& a, hipStream_t s);
 hipError_t launch_est_solid(const EstArgs& a, hipStream_t s);
 
+// Unitig compaction of a sorted solid set (fb_compact.hip, build.rs:330-597).
+// Directed k-mer d = 2i + o (o = 1: reverse strand of k-mer i); start edges
+// are the out-edges of boundary strands, numbered in (d, base) order.
+struct CgArgs {
+  const uint64_t* keys = nullptr;  // n * nref words, most significant first, sorted
+  const uint16_t* cov = nullptr;
+  const uint8_t* mask = nullptr;
+  uint32_t n = 0, k = 0, nref = 1;
+  uint8_t *kind = nullptr, *fbase = nullptr, *lbase = nullptr, *obase = nullptr;  // 2n each
+  uint32_t *nstart = nullptr, *sbase = nullptr, *succ = nullptr;                // 2n each
+  uint32_t* bnext = nullptr;                                                    // 8n
+  uint32_t* up[4] = {};    // jump buffers 0/1 (paths) and 2/3 (cycles), 2n each
+  uint32_t* head[4] = {};
+  uint32_t* rank[4] = {};
+  uint32_t* lab[4] = {};
+  uint64_t n_se = 0;       // start edges
+  unsigned long long* csum = nullptr;  // n_se: cov sum of the path's counted nodes
+  unsigned long long* last = nullptr;  // n_se: rank << 32 | last counted node
+  uint32_t *walked = nullptr, *slen = nullptr, *uid = nullptr;  // n_se
+  uint64_t* soff = nullptr;                                      // n_se
+  unsigned long long* ccov = nullptr;  // 2n: cov sum per cycle label
+  uint32_t *clen = nullptr, *cyc_len = nullptr, *cid = nullptr;  // 2n
+  uint64_t* coff = nullptr;                                      // 2n, relative to circ_base
+  uint64_t circ_base = 0;                                        // bases of the linear unitigs
+  uint8_t* seq = nullptr;
+  uint32_t n_lin = 0;
+  uint64_t* u_off = nullptr;
+  uint32_t* u_len = nullptr;
+  float* u_cov = nullptr;
+  uint8_t* u_circ = nullptr;
+};
+hipError_t launch_cg_links(const CgArgs& a, hipStream_t s);
+hipError_t launch_cg_pred(const CgArgs& a, hipStream_t s);
+uint32_t cg_jump_rounds(uint64_t nodes);
+hipError_t launch_cg_jump(const CgArgs& a, uint32_t s, uint32_t t, uint32_t* out_s, hipStream_t st);
+hipError_t launch_cg_accumulate(const CgArgs& a, uint32_t s, hipStream_t st);
+hipError_t launch_cg_cycle_ranks(const CgArgs& a, uint32_t s, uint32_t* out_s, hipStream_t st);
+hipError_t launch_cg_cycle_flags(const CgArgs& a, uint32_t* flag, hipStream_t s);
+hipError_t launch_cg_table(const CgArgs& a, hipStream_t s);
+hipError_t launch_cg_write(const CgArgs& a, uint32_t sp, uint32_t sc, hipStream_t s);
+hipError_t launch_cg_flip(uint8_t* seq, const uint64_t* off, const uint32_t* len, uint32_t n_lin, uint8_t* flip,
+                          uint64_t lin_bases, hipStream_t s);
+hipError_t cg_exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
+                                hipStream_t s);
+hipError_t cg_exclusive_sum_u64(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
+                                hipStream_t s);
+
 }  // namespace fb

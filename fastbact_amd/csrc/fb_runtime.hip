@@ -597,6 +597,7 @@ struct fb_solid {
   // keys per LDS bin table, estimated keys, LDS-route records, reads >= k,
   // pass-2 rounds
   uint64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool sorted = false;  // records sorted by k-mer (the parity form)
 };
 
 extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
@@ -945,6 +946,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   for (size_t len = k; len < bank->len_hist.size(); ++len) so->stats[9] += bank->len_hist[len] / len;
   tbuf.release();
   so->n = n_out;
+  so->sorted = prm->sort != 0;
   if (prm->sort && n_out > 1) {
     size_t tb = 0;
     HIP_TRY(c, sort_solid(nullptr, nullptr, nullptr, n_out, so->nref, k, nullptr, &tb, s));
@@ -2442,4 +2444,342 @@ extern "C" int fb_autotune_load(const char* path) {
 extern "C" void fb_autotune_clear(void) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   g_tune.clear();
+}
+
+
+// ---------------------------------------------------------------------------
+// Unitig compaction (build.rs:330-597) of a sorted solid set: walks on the
+// device (fb_compact.hip), overlap edges (build.rs:552-594) on the host from
+// the unitig sequences.
+
+struct fb_graph {
+  fb_ctx* ctx = nullptr;
+  uint32_t k = 0;
+  uint64_t n_lin = 0;
+  std::vector<uint8_t> seq;
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> len;
+  std::vector<float> cov;
+  std::vector<uint8_t> circ;
+  std::vector<uint32_t> e_src, e_dst;
+  std::vector<uint8_t> e_ov;
+  std::vector<uint16_t> e_sup;
+  float ms[3] = {0, 0, 0};  // device walks, sequence download, edges
+};
+
+static uint8_t comp_ascii(uint8_t c) { return c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A'; }
+
+// build.rs:552-594 with UnitigNode::add_edge (cdbg.rs): for ov = k, then k-1,
+// every strand's ov-suffix meets the ov-prefixes of all strands (prefixes of
+// one key in unitig-id order, forward before reverse); self edges only on
+// circular unitigs; an edge to the same target and overlap adds its support.
+static void graph_edges(fb_graph* g) {
+  struct E {
+    uint32_t t;
+    uint8_t ov;
+    uint16_t sup;
+  };
+  const uint64_t U = g->len.size();
+  const uint32_t k = g->k;
+  std::vector<std::vector<E>> ef(U), er(U);
+  auto add = [](std::vector<E>& l, uint32_t t, uint8_t ov) {
+    for (E& e : l)
+      if (e.t == t && e.ov == ov) {
+        e.sup = (uint16_t)std::min<uint32_t>(65535u, (uint32_t)e.sup + 10u);
+        return;
+      }
+    l.push_back(E{t, ov, 10});
+  };
+  for (uint64_t u = 0; u < U; ++u)
+    if (g->circ[u]) ef[u].push_back(E{(uint32_t)(u << 1), (uint8_t)(k - 1), 10});
+  for (int pass = 0; pass < 2; ++pass) {
+    const uint32_t ov = pass == 0 ? k : k - 1;
+    if (ov == 0) continue;
+    std::vector<uint8_t> rcpre((size_t)U * ov);  // reverse strands' prefixes
+    struct P {
+      const uint8_t* p;
+      uint32_t d;
+    };
+    std::vector<P> pe;
+    pe.reserve(2 * U);
+    for (uint64_t u = 0; u < U; ++u) {
+      const uint32_t L = g->len[u];
+      if (L < ov) continue;
+      const uint8_t* s = g->seq.data() + g->off[u];
+      uint8_t* r = rcpre.data() + u * ov;
+      for (uint32_t j = 0; j < ov; ++j) r[j] = comp_ascii(s[L - 1 - j]);
+      pe.push_back(P{s, (uint32_t)(u << 1)});
+      pe.push_back(P{r, (uint32_t)(u << 1 | 1)});
+    }
+    std::sort(pe.begin(), pe.end(), [ov](const P& a, const P& b) {
+      const int c = memcmp(a.p, b.p, ov);
+      return c < 0 || (c == 0 && a.d < b.d);
+    });
+    std::vector<uint8_t> suf(ov);
+    for (uint64_t u = 0; u < U; ++u) {
+      const uint32_t L = g->len[u];
+      if (L < ov) continue;
+      const uint8_t* s = g->seq.data() + g->off[u];
+      for (int side = 0; side < 2; ++side) {
+        const uint8_t* q = s + (L - ov);
+        if (side) {
+          for (uint32_t j = 0; j < ov; ++j) suf[j] = comp_ascii(s[ov - 1 - j]);
+          q = suf.data();
+        }
+        auto lo = std::lower_bound(pe.begin(), pe.end(), q,
+                                   [ov](const P& a, const uint8_t* key) { return memcmp(a.p, key, ov) < 0; });
+        for (auto it = lo; it != pe.end() && memcmp(it->p, q, ov) == 0; ++it)
+          if ((it->d >> 1) != u || g->circ[u]) add(side ? er[u] : ef[u], it->d, (uint8_t)ov);
+      }
+    }
+  }
+  for (uint64_t u = 0; u < U; ++u)
+    for (int side = 0; side < 2; ++side)
+      for (const E& e : side ? er[u] : ef[u]) {
+        g->e_src.push_back((uint32_t)(u << 1 | (uint32_t)side));
+        g->e_dst.push_back(e.t);
+        g->e_ov.push_back(e.ov);
+        g->e_sup.push_back(e.sup);
+      }
+}
+
+static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, const uint8_t* mask, uint64_t n,
+                          uint32_t k, fb_graph* g) {
+  hipStream_t s = c->stream;
+  g->ctx = c;
+  g->k = k;
+  if (n == 0) return FB_OK;
+  if (n >= (1ull << 30)) return set_err(c, FB_ERR_INVALID, "compaction takes at most 2^30 solid k-mers");
+  const uint64_t N = 2 * n;
+  CgArgs a;
+  a.keys = keys;
+  a.cov = cov;
+  a.mask = mask;
+  a.n = (uint32_t)n;
+  a.k = k;
+  a.nref = ref_words(k);
+  DBuf kind, fbase, lbase, obase, nstart, sbase, succ, bnext, jb[16], ccov, clen, cyc_len, cid, coff, tmp;
+  int r = 0;
+  if (!r) r = kind.get(c, N);
+  if (!r) r = fbase.get(c, N);
+  if (!r) r = lbase.get(c, N);
+  if (!r) r = obase.get(c, N);
+  if (!r) r = nstart.get(c, N * 4);
+  if (!r) r = sbase.get(c, N * 4);
+  if (!r) r = succ.get(c, N * 4);
+  if (!r) r = bnext.get(c, N * 16);
+  for (int j = 0; j < 16 && !r; ++j) r = jb[j].get(c, N * 4);
+  if (!r) r = ccov.get(c, N * 8);
+  if (!r) r = clen.get(c, N * 4);
+  if (!r) r = cyc_len.get(c, N * 4);
+  if (!r) r = cid.get(c, N * 4);
+  if (!r) r = coff.get(c, N * 8);
+  if (r) return r;
+  a.kind = kind.as<uint8_t>();
+  a.fbase = fbase.as<uint8_t>();
+  a.lbase = lbase.as<uint8_t>();
+  a.obase = obase.as<uint8_t>();
+  a.nstart = nstart.as<uint32_t>();
+  a.sbase = sbase.as<uint32_t>();
+  a.succ = succ.as<uint32_t>();
+  a.bnext = bnext.as<uint32_t>();
+  for (int j = 0; j < 4; ++j) {
+    a.up[j] = jb[4 * j].as<uint32_t>();
+    a.head[j] = jb[4 * j + 1].as<uint32_t>();
+    a.rank[j] = jb[4 * j + 2].as<uint32_t>();
+    a.lab[j] = jb[4 * j + 3].as<uint32_t>();
+  }
+  a.ccov = ccov.as<unsigned long long>();
+  a.clen = clen.as<uint32_t>();
+  a.cyc_len = cyc_len.as<uint32_t>();
+  a.cid = cid.as<uint32_t>();
+  a.coff = coff.as<uint64_t>();
+  HIP_TRY(c, hipEventRecord(c->ev[0], s));
+  HIP_TRY(c, launch_cg_links(a, s));
+  // scan scratch: the largest of the scans below
+  size_t tb = 0, t1 = 0;
+  HIP_TRY(c, cg_exclusive_sum_u32(nullptr, nullptr, N, nullptr, &t1, s));
+  tb = std::max(tb, t1);
+  HIP_TRY(c, cg_exclusive_sum_u64(nullptr, nullptr, N, nullptr, &t1, s));
+  tb = std::max(tb, t1);
+  if (int e = tmp.get(c, tb + 256)) return e;
+  size_t tbs = tb;
+  HIP_TRY(c, cg_exclusive_sum_u32(a.nstart, a.sbase, N, tmp.p, &tbs, s));
+  uint32_t tail[2] = {0, 0};
+  HIP_TRY(c, hipMemcpyAsync(&tail[0], a.sbase + N - 1, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&tail[1], a.nstart + N - 1, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  a.n_se = (uint64_t)tail[0] + tail[1];
+  const uint64_t nse = std::max<uint64_t>(a.n_se, 1);
+  DBuf csum, last, walked, slen, uid, soff;
+  if (!r) r = csum.get(c, nse * 8);
+  if (!r) r = last.get(c, nse * 8);
+  if (!r) r = walked.get(c, nse * 4);
+  if (!r) r = slen.get(c, nse * 4);
+  if (!r) r = uid.get(c, nse * 4);
+  if (!r) r = soff.get(c, nse * 8);
+  if (r) return r;
+  a.csum = csum.as<unsigned long long>();
+  a.last = last.as<unsigned long long>();
+  a.walked = walked.as<uint32_t>();
+  a.slen = slen.as<uint32_t>();
+  a.uid = uid.as<uint32_t>();
+  a.soff = soff.as<uint64_t>();
+  HIP_TRY(c, launch_cg_pred(a, s));
+  uint32_t sp = 0, sc = 2;
+  HIP_TRY(c, launch_cg_jump(a, 0u, 1u, &sp, s));
+  HIP_TRY(c, launch_fill_u32(a.clen, N, 0u, s));
+  HIP_TRY(c, launch_fill_u32(reinterpret_cast<uint32_t*>(a.ccov), 2 * N, 0u, s));
+  HIP_TRY(c, launch_cg_accumulate(a, sp, s));
+  HIP_TRY(c, launch_cg_cycle_ranks(a, sp, &sc, s));
+  // unitig ids and offsets
+  uint32_t lin[4] = {0, 0, 0, 0};
+  uint64_t lb[2] = {0, 0};
+  if (a.n_se) {
+    size_t t = tb;
+    HIP_TRY(c, cg_exclusive_sum_u32(a.walked, a.uid, a.n_se, tmp.p, &t, s));
+    t = tb;
+    HIP_TRY(c, cg_exclusive_sum_u64(a.slen, a.soff, a.n_se, tmp.p, &t, s));
+    HIP_TRY(c, hipMemcpyAsync(&lin[0], a.uid + a.n_se - 1, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&lin[1], a.walked + a.n_se - 1, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&lb[0], a.soff + a.n_se - 1, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&lin[2], a.slen + a.n_se - 1, 4, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(c, launch_cg_cycle_flags(a, a.nstart, s));  // nstart is free again: cycle flags
+  {
+    size_t t = tb;
+    HIP_TRY(c, cg_exclusive_sum_u32(a.nstart, a.cid, N, tmp.p, &t, s));
+    t = tb;
+    HIP_TRY(c, cg_exclusive_sum_u64(a.cyc_len, a.coff, N, tmp.p, &t, s));
+  }
+  uint32_t ct[3] = {0, 0, 0};
+  uint64_t cb = 0;
+  HIP_TRY(c, hipMemcpyAsync(&ct[0], a.cid + N - 1, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&ct[1], a.nstart + N - 1, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&ct[2], a.cyc_len + N - 1, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&cb, a.coff + N - 1, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const uint64_t n_lin = (uint64_t)lin[0] + lin[1];
+  const uint64_t lin_bases = lb[0] + lin[2];
+  const uint64_t n_circ = (uint64_t)ct[0] + ct[1];
+  const uint64_t circ_bases = cb + ct[2];
+  const uint64_t U = n_lin + n_circ, B = lin_bases + circ_bases;
+  a.n_lin = (uint32_t)n_lin;
+  a.circ_base = lin_bases;
+  DBuf seq, uoff, ulen, ucov, ucirc, flip;
+  if (!r) r = seq.get(c, std::max<uint64_t>(B, 1));
+  if (!r) r = uoff.get(c, std::max<uint64_t>(U, 1) * 8);
+  if (!r) r = ulen.get(c, std::max<uint64_t>(U, 1) * 4);
+  if (!r) r = ucov.get(c, std::max<uint64_t>(U, 1) * 4);
+  if (!r) r = ucirc.get(c, std::max<uint64_t>(U, 1));
+  if (!r) r = flip.get(c, std::max<uint64_t>(n_lin, 1));
+  if (r) return r;
+  a.seq = seq.as<uint8_t>();
+  a.u_off = uoff.as<uint64_t>();
+  a.u_len = ulen.as<uint32_t>();
+  a.u_cov = ucov.as<float>();
+  a.u_circ = ucirc.as<uint8_t>();
+  HIP_TRY(c, launch_cg_table(a, s));
+  HIP_TRY(c, launch_cg_write(a, sp, sc, s));
+  HIP_TRY(c, launch_cg_flip(a.seq, a.u_off, a.u_len, a.n_lin, flip.as<uint8_t>(), lin_bases, s));
+  HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  g->n_lin = n_lin;
+  g->seq.resize(B);
+  g->off.resize(U);
+  g->len.resize(U);
+  g->cov.resize(U);
+  g->circ.resize(U);
+  if (B) HIP_TRY(c, hipMemcpyAsync(g->seq.data(), a.seq, B, hipMemcpyDeviceToHost, s));
+  if (U) {
+    HIP_TRY(c, hipMemcpyAsync(g->off.data(), a.u_off, U * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(g->len.data(), a.u_len, U * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(g->cov.data(), a.u_cov, U * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(g->circ.data(), a.u_circ, U, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(c, hipEventRecord(c->ev[2], s));
+  HIP_TRY(c, hipEventSynchronize(c->ev[2]));
+  (void)hipEventElapsedTime(&g->ms[0], c->ev[0], c->ev[1]);
+  (void)hipEventElapsedTime(&g->ms[1], c->ev[1], c->ev[2]);
+  return FB_OK;
+}
+
+static int compact_finish(fb_graph* g) {
+  const auto t0 = std::chrono::steady_clock::now();
+  graph_edges(g);
+  g->ms[2] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return FB_OK;
+}
+
+extern "C" int fb_compact_solid(const fb_solid* so, fb_graph** out) {
+  if (!so || !out) return FB_ERR_INVALID;
+  *out = nullptr;
+  fb_ctx* c = so->ctx;
+  if (!so->sorted) return set_err(c, FB_ERR_INVALID, "compaction needs the solid set sorted (params sort = 1)");
+  if (int r = ctx_current(c)) return r;
+  auto g = std::make_unique<fb_graph>();
+  if (int r = compact_device(c, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(), so->mask.as<uint8_t>(), so->n,
+                             so->k, g.get()))
+    return r;
+  compact_finish(g.get());
+  *out = g.release();
+  return FB_OK;
+}
+
+extern "C" int fb_compact_arrays(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, const uint8_t* mask,
+                                 uint64_t n, uint32_t k, fb_graph** out) {
+  if (!c || !out || (n && (!keys || !cov || !mask))) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (k < 1 || k > 127) return set_err(c, FB_ERR_INVALID, "k=%u outside 1..127", k);
+  const uint32_t nref = ref_words(k);
+  // sorted, distinct and within 2k bits: the lookups assume it
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t* x = keys + i * nref;
+    for (uint32_t w = 0; w < nref; ++w) {
+      const uint32_t bit = 64u * (nref - 1 - w);  // lowest bit of word w
+      const uint64_t lim = bit >= 2 * k ? 0ull : (2 * k - bit >= 64 ? ~0ull : (1ull << (2 * k - bit)) - 1ull);
+      if (x[w] & ~lim) return set_err(c, FB_ERR_INVALID, "key %llu has bits beyond 2k", (unsigned long long)i);
+    }
+    if (i && !std::lexicographical_compare(x - nref, x, x, x + nref))
+      return set_err(c, FB_ERR_INVALID, "keys must be strictly ascending (key %llu)", (unsigned long long)i);
+  }
+  if (int r = ctx_current(c)) return r;
+  DBuf dk, dc, dm;
+  if (int r = to_device(c, dk, keys, n * nref * 8)) return r;
+  if (int r = to_device(c, dc, cov, n * 2)) return r;
+  if (int r = to_device(c, dm, mask, n)) return r;
+  auto g = std::make_unique<fb_graph>();
+  if (int r = compact_device(c, dk.as<uint64_t>(), dc.as<uint16_t>(), dm.as<uint8_t>(), n, k, g.get())) return r;
+  compact_finish(g.get());
+  *out = g.release();
+  return FB_OK;
+}
+
+extern "C" void fb_graph_destroy(fb_graph* g) { delete g; }
+extern "C" uint64_t fb_graph_unitig_count(const fb_graph* g) { return g ? g->len.size() : 0; }
+extern "C" uint64_t fb_graph_base_count(const fb_graph* g) { return g ? g->seq.size() : 0; }
+extern "C" uint64_t fb_graph_edge_count(const fb_graph* g) { return g ? g->e_src.size() : 0; }
+extern "C" int fb_graph_unitigs(const fb_graph* g, uint8_t* seq, uint64_t* offsets, uint32_t* lengths, float* cov,
+                                uint8_t* circular) {
+  if (!g) return FB_ERR_INVALID;
+  if (seq && !g->seq.empty()) memcpy(seq, g->seq.data(), g->seq.size());
+  if (offsets && !g->off.empty()) memcpy(offsets, g->off.data(), g->off.size() * 8);
+  if (lengths && !g->len.empty()) memcpy(lengths, g->len.data(), g->len.size() * 4);
+  if (cov && !g->cov.empty()) memcpy(cov, g->cov.data(), g->cov.size() * 4);
+  if (circular && !g->circ.empty()) memcpy(circular, g->circ.data(), g->circ.size());
+  return FB_OK;
+}
+extern "C" int fb_graph_edges(const fb_graph* g, uint32_t* src, uint32_t* dst, uint8_t* ov, uint16_t* support) {
+  if (!g) return FB_ERR_INVALID;
+  const size_t n = g->e_src.size();
+  if (!n) return FB_OK;
+  if (src) memcpy(src, g->e_src.data(), n * 4);
+  if (dst) memcpy(dst, g->e_dst.data(), n * 4);
+  if (ov) memcpy(ov, g->e_ov.data(), n);
+  if (support) memcpy(support, g->e_sup.data(), n * 2);
+  return FB_OK;
+}
+extern "C" int fb_graph_timings(const fb_graph* g, float out[3]) {
+  if (!g || !out) return FB_ERR_INVALID;
+  memcpy(out, g->ms, sizeof g->ms);
+  return FB_OK;
 }

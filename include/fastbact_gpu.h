@@ -18,6 +18,7 @@
  *   fb_hll_*            HyperLogLog                      src/common/sketch.rs:24-115
  *   fb_build_solid      build_cdbg_with_contigs_ext<K>   src/graph/build.rs:67-328
  *                       (passes 1-3; unitig compaction, build.rs:330-597, is downstream)
+ *   fb_compact_*        unitig compaction (steps 4-6)    src/graph/build.rs:330-597
  *   fb_estimate_genome  estimate_genome                  src/ingest/estimator.rs:286-440
  *   fb_count_stats_from_lengths  count_fastx_exact (length part)  src/ingest/estimator.rs:167-283
  *   fb_compute_dynamic_kmers                             src/ingest/estimator.rs:104-161
@@ -181,6 +182,36 @@ int fb_solid_kernel_timings(const fb_solid* s, float out[5]);
  * LDS route splits the bins into rounds of at most 16384; 0 on another
  * route). */
 int fb_solid_stats(const fb_solid* s, uint64_t out[11]);
+
+/* ---- Unitig compaction: build_cdbg_with_contigs_ext step 4-6 (build.rs:330-597)
+ * Walks from boundary directed k-mers through 1-in/1-out k-mers, isolated
+ * cycles, then overlap edges (k and k-1).  The reference walks its hash-map
+ * shards in their iteration order; here the k-mers are taken in ascending
+ * order (boundary strands forward then reverse, bases 0..3; cycles by their
+ * smallest k-mer), which fixes the direction each linear unitig is walked in
+ * (its cov leaves out the node it ends on), where a circular unitig starts,
+ * and the node numbering: linear unitigs in walk order, then circular ones.
+ * The walks run on the device as list ranking; the edges on the host. */
+typedef struct fb_graph fb_graph;
+/* The solid set of a build (built with params sort = 1). */
+int fb_compact_solid(const fb_solid* s, fb_graph** out);
+/* A solid set given on the host: keys strictly ascending in the
+ * fb_solid_download layout, with cov and mask per key. */
+int fb_compact_arrays(fb_ctx* ctx, const uint64_t* keys, const uint16_t* cov, const uint8_t* mask, uint64_t n,
+                      uint32_t k, fb_graph** out);
+void fb_graph_destroy(fb_graph* g);
+uint64_t fb_graph_unitig_count(const fb_graph* g);
+uint64_t fb_graph_base_count(const fb_graph* g);
+uint64_t fb_graph_edge_count(const fb_graph* g);
+/* Unitig i: seq[offsets[i] .. + lengths[i]) as ASCII (CdbgGraph::get_unitig_seq),
+ * UnitigNode::kmer_cov and is_circular. */
+int fb_graph_unitigs(const fb_graph* g, uint8_t* seq, uint64_t* offsets, uint32_t* lengths, float* cov,
+                     uint8_t* circular);
+/* Edges of every node in insertion order, forward side then reverse side:
+ * src = id << 1 | side, dst = DirUnitigId (id << 1 | is_rc), overlap, support. */
+int fb_graph_edges(const fb_graph* g, uint32_t* src, uint32_t* dst, uint8_t* ov, uint16_t* support);
+/* Milliseconds: [0] device walks, [1] sequence download, [2] edges on the host. */
+int fb_graph_timings(const fb_graph* g, float out[3]);
 
 /* ---- Batched form: many isolates per launch --------------------------------
  * The reference builds one isolate per process run (main.rs:262).  A batch

@@ -297,6 +297,38 @@ def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05)
     return out
 
 
+def bench_compaction(ctx, dev, F, cpu: bool) -> dict:
+    """Unitig compaction (build.rs:330-597) of the first pass's solid set:
+    device walks, sequence download and host edges, with the oracle's
+    single-threaded compaction of the same set timed beside it."""
+    k, tgt = PASSES[0]
+    h = F.build_solid_handle(ctx, dev, k, target_solid_cov=tgt, genome_size=GENOME["genome_len"], cms_width=WIDTH,
+                             counter_bits=COUNTER_BITS, hll_p=HLL_P)
+    try:
+        F.compact_unitigs(ctx, h)  # warm-up
+        n_solid = h.count()
+        t = time.perf_counter()
+        g = F.compact_unitigs(ctx, h)
+        wall = (time.perf_counter() - t) * 1e3
+        s = h.download(k, with_cms=False) if cpu else None
+    finally:
+        h.close()
+    out = {"k": k, "solid_kmers": n_solid, "unitigs": len(g["seqs"]), "circular": int(g["circular"].sum()),
+           "bases": int(sum(len(x) for x in g["seqs"])), "edges": int(g["src"].size),
+           "device_walk_ms": round(g["timings_ms"][0], 3), "download_ms": round(g["timings_ms"][1], 3),
+           "host_edges_ms": round(g["timings_ms"][2], 3), "wall_ms": round(wall, 3),
+           "solid_kmers_per_s_device": round(n_solid / max(g["timings_ms"][0], 1e-6) * 1e3, 1)}
+    if s is not None:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        r = O.compact(s.keys, s.cov, s.mask, k)
+        out["cpu_baseline_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        out["cpu_baseline"] = "oracle compaction (single thread) of the same solid set"
+        out["equal_to_oracle"] = bool(r["seqs"] == g["seqs"] and np.array_equal(r["cov"], g["cov"])
+                                      and np.array_equal(r["src"], g["src"]) and np.array_equal(r["dst"], g["dst"]))
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +415,9 @@ def main(argv=None) -> int:
         copy_calibration(ctx, args.copy_calibration_mib)
     roofline["traffic"], roofline["traffic_source"] = measured_traffic(dom)
 
+    compaction = bench_compaction(ctx, dev, F, cpu=not args.no_cpu_baseline)
+    log(f"compaction: {compaction}")
+
     batched = served = None
     if args.batch_isolates or args.serve_requests:
         shapes, banks = make_batch_isolates(max(args.batch_isolates, 1 if args.serve_requests else 0),
@@ -433,6 +468,7 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "pass_model_roofline": pass_model,
         "cpu_baseline": cpu,
+        "compaction": compaction,
         "batched": batched,
         "serving": served,
     }

@@ -1640,3 +1640,89 @@ void orc_graph_free(orc_graph* g) {
     free(g->e_support);
     memset(g, 0, sizeof *g);
 }
+
+/* ------------------------------------------------------------------------- */
+/* filter_active_reads, local_multik.rs:259-333.  The boundary set is a sorted
+ * array of ntHash values (the reference uses an FxHashSet; membership is all
+ * that matters). */
+static int u64_cmp(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+static int set_has(const uint64_t* s, uint64_t n, uint64_t h) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (s[mid] < h) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && s[lo] == h;
+}
+/* is_active closure, local_multik.rs:297-309 */
+static int read_active(const uint8_t* seq, uint32_t len, uint32_t k, const uint64_t* set, uint64_t n) {
+    if (len < k) return 0;
+    uint32_t offs[3] = {0, (len - k) / 2, len - k};
+    for (int i = 0; i < 3; i++)
+        if (set_has(set, n, orc_nthash_canonical(seq + offs[i], k))) return 1;
+    return 0;
+}
+int orc_filter_active_reads(const orc_graph_view* g, const orc_arena arenas[3], uint8_t* keep_merged,
+                            uint8_t* keep_pairs) {
+    const uint32_t k = g->k < 31 ? g->k : 31; /* local_multik.rs:261 */
+    const uint64_t win = 350;                 /* boundary_window */
+    if (k == 0) return -1;
+    uint64_t cap = 0;
+    for (uint64_t i = 0; i < g->n_nodes; i++)
+        if (g->lengths[i] >= k) cap += g->lengths[i] - k + 1;
+    uint64_t* set = (uint64_t*)malloc((cap ? cap : 1) * sizeof(uint64_t));
+    if (!set) return -1;
+    uint64_t n = 0;
+    for (uint64_t i = 0; i < g->n_nodes; i++) {
+        const uint8_t fl = g->flags ? g->flags[i] : 0;
+        if (fl & 1) continue; /* is_deleted */
+        const uint64_t L = g->lengths[i];
+        if (L < k) continue;
+        const uint8_t* s = g->seq + g->offsets[i];
+        const uint32_t nf = g->n_fwd ? g->n_fwd[i] : 0, nr = g->n_rev ? g->n_rev[i] : 0;
+        if (L < 2000 || (fl & 2) || nf > 1 || nr > 1) {
+            for (uint64_t p = 0; p + k <= L; p++) set[n++] = orc_nthash_canonical(s + p, k);
+        } else {
+            uint64_t left_end = win < L - k + 1 ? win : L - k + 1;
+            for (uint64_t p = 0; p < left_end; p++) set[n++] = orc_nthash_canonical(s + p, k);
+            uint64_t rs = L > win ? L - win : 0;
+            if (rs > L - k) rs = L - k;
+            for (uint64_t p = rs; p <= L - k; p++) set[n++] = orc_nthash_canonical(s + p, k);
+        }
+    }
+    qsort(set, n, sizeof(uint64_t), u64_cmp);
+    uint64_t u = 0;
+    for (uint64_t i = 0; i < n; i++)
+        if (u == 0 || set[i] != set[u - 1]) set[u++] = set[i];
+    uint32_t maxlen = 0;
+    for (int a = 0; a < 3; a++)
+        for (uint64_t i = 0; i < arenas[a].n_reads; i++)
+            if (arenas[a].lengths[i] > maxlen) maxlen = arenas[a].lengths[i];
+    uint8_t* b1 = (uint8_t*)malloc(maxlen + 1);
+    uint8_t* b2 = (uint8_t*)malloc(maxlen + 1);
+    if (!b1 || !b2) {
+        free(set);
+        free(b1);
+        free(b2);
+        return -1;
+    }
+    for (uint64_t i = 0; i < arenas[0].n_reads; i++) {
+        orc_arena_extract_ascii(&arenas[0], i, b1);
+        keep_merged[i] = (uint8_t)read_active(b1, arenas[0].lengths[i], k, set, u);
+    }
+    const uint64_t np = arenas[1].n_reads < arenas[2].n_reads ? arenas[1].n_reads : arenas[2].n_reads;
+    for (uint64_t i = 0; i < np; i++) {
+        orc_arena_extract_ascii(&arenas[1], i, b1);
+        orc_arena_extract_ascii(&arenas[2], i, b2);
+        keep_pairs[i] = (uint8_t)(read_active(b1, arenas[1].lengths[i], k, set, u) ||
+                                  read_active(b2, arenas[2].lengths[i], k, set, u));
+    }
+    free(set);
+    free(b1);
+    free(b2);
+    return 0;
+}

@@ -196,6 +196,28 @@ int orc_compact(const uint64_t* keys, const uint16_t* cov, const uint8_t* mask, 
                 orc_graph* out);
 void orc_graph_free(orc_graph* g);
 
+/* ---- filter_active_reads (src/graph/local_multik.rs:259-333) ----------------
+ * A CdbgGraph given as arrays: node i has ASCII seq[offsets[i] .. + lengths[i]),
+ * flags bit 0 = is_deleted, bit 1 = is_circular, and edges_fwd.len() /
+ * edges_rev.len() in n_fwd / n_rev.  The boundary set holds canonical ntHash
+ * values at k' = min(k, 31) of every window of short (< 2000 bp), circular or
+ * branching unitigs and of the 350-window ends of the others; a read is active
+ * when its window at offset 0, (len-k')/2 or len-k' is in the set.  keep_merged
+ * gets one byte per read of arenas[0]; keep_pairs one byte per pair
+ * i < min(n(arenas[1]), n(arenas[2])), set when either mate is active. */
+typedef struct {
+    uint32_t k;
+    uint64_t n_nodes;
+    const uint8_t* seq;
+    const uint64_t* offsets;
+    const uint32_t* lengths;
+    const uint8_t* flags;
+    const uint32_t* n_fwd;
+    const uint32_t* n_rev;
+} orc_graph_view;
+int orc_filter_active_reads(const orc_graph_view* g, const orc_arena arenas[3], uint8_t* keep_merged,
+                            uint8_t* keep_pairs);
+
 #ifdef __cplusplus
 }
 #endif

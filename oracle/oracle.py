@@ -74,6 +74,11 @@ class GraphC(C.Structure):
                 ("e_dst", u32p), ("e_ov", u8p), ("e_support", u16p)]
 
 
+class GraphView(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("n_nodes", C.c_uint64), ("seq", u8p), ("offsets", u64p), ("lengths", u32p),
+                ("flags", u8p), ("n_fwd", u32p), ("n_rev", u32p)]
+
+
 _SIGS = [
     ("orc_splitmix64", C.c_uint64, [C.c_uint64]),
     ("orc_rev_comp_word", C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -118,6 +123,7 @@ _SIGS = [
     ("orc_solid_free", None, [C.POINTER(SolidSetC)]),
     ("orc_compact", C.c_int, [u64p, u16p, u8p, C.c_uint64, C.c_uint32, C.POINTER(GraphC)]),
     ("orc_graph_free", None, [C.POINTER(GraphC)]),
+    ("orc_filter_active_reads", C.c_int, [C.POINTER(GraphView), C.POINTER(Arena), u8p, u8p]),
 ]
 
 _LIB = None
@@ -412,3 +418,60 @@ def compact(keys, cov, mask, k: int) -> dict:
            "ov": arr(g.e_ov, ne, np.uint8), "support": arr(g.e_support, ne, np.uint16)}
     lib().orc_graph_free(C.byref(g))
     return out
+
+
+# ---- filter_active_reads (local_multik.rs:259-333) ----------------------------
+
+def edge_counts(graph: dict) -> tuple:
+    """edges_fwd.len() and edges_rev.len() per node from the edge arrays
+    (src = id << 1 | side)."""
+    n = len(graph["seqs"])
+    src = np.asarray(graph["src"], np.int64)
+    nf = np.bincount(src[(src & 1) == 0] >> 1, minlength=n).astype(np.uint32)
+    nr = np.bincount(src[(src & 1) == 1] >> 1, minlength=n).astype(np.uint32)
+    return nf, nr
+
+
+def unpack(data, offsets, lengths, ids=None) -> list:
+    """BitSeqArena::extract_ascii for the reads `ids` (all by default)."""
+    ids = np.arange(len(lengths)) if ids is None else np.asarray(ids, np.int64)
+    out = []
+    for i in ids:
+        g = int(offsets[i]) + np.arange(int(lengths[i]), dtype=np.uint64)
+        codes = (data[(g >> np.uint64(5)).astype(np.int64)] >> ((g & np.uint64(31)) << np.uint64(1))) & np.uint64(3)
+        out.append(np.frombuffer(b"ACGT", np.uint8)[codes.astype(np.int64)].tobytes())
+    return out
+
+
+def filter_active_reads(graph: dict, k: int, arenas: list) -> dict:
+    """filter_active_reads on a graph dict (seqs, circular, src; optional
+    deleted) at graph k and a bank of three arenas (merged, pe_forward,
+    pe_reverse as (data, offsets, lengths)) -> keep_merged, keep_pairs and the
+    filtered bank's three arenas, packed as BitSeqArena::push_seq does."""
+    seqs = [bytes(x) for x in graph["seqs"]]
+    n = len(seqs)
+    lens = np.array([len(x) for x in seqs], np.uint32)
+    offs = np.zeros(n, np.uint64)
+    if n:
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    buf = _bytes(b"".join(seqs))
+    flags = (np.asarray(graph["circular"], np.uint8) << 1).astype(np.uint8)
+    if "deleted" in graph:
+        flags |= np.asarray(graph["deleted"], np.uint8)
+    nf, nr = edge_counts(graph)
+    gv = GraphView(k, n, _p(buf, C.c_uint8), _p(offs, C.c_uint64), _p(lens, C.c_uint32), _p(flags, C.c_uint8),
+                   _p(nf, C.c_uint32), _p(nr, C.c_uint32))
+    ars = [tuple(np.ascontiguousarray(x) for x in a) for a in arenas]
+    arr = (Arena * 3)(*[arena(*a) for a in ars])
+    km = np.zeros(max(1, ars[0][2].size), np.uint8)
+    npairs = min(ars[1][2].size, ars[2][2].size)
+    kp = np.zeros(max(1, npairs), np.uint8)
+    rc = lib().orc_filter_active_reads(C.byref(gv), arr, _p(km, C.c_uint8), _p(kp, C.c_uint8))
+    if rc:
+        raise ValueError(f"orc_filter_active_reads failed ({rc})")
+    km, kp = km[:ars[0][2].size], kp[:npairs]
+    merged = unpack(*ars[0], np.flatnonzero(km))
+    r1 = unpack(*ars[1], np.flatnonzero(kp))
+    r2 = unpack(*ars[2], np.flatnonzero(kp))
+    return {"keep_merged": km, "keep_pairs": kp, "arenas": [pack(merged), pack(r1), pack(r2)],
+            "reads": [merged, r1, r2]}

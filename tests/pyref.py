@@ -444,3 +444,29 @@ def solid_from_sequences(seqs: list, k: int, cov_of=None, circular: bool = False
             acc[uk] |= 1 << (b + 4 if urc else b)
             acc[vk] |= 1 << ((3 - p) if vrc else (3 - p) + 4)
     return sorted((key, (cov_of(key) if cov_of else 10), m) for key, m in acc.items())
+
+
+def filter_active_reads(nodes: list, k: int, merged: list, pairs: list) -> tuple:
+    """local_multik.rs:259-333 on strings.  nodes: (seq, circular, deleted,
+    n_edges_fwd, n_edges_rev); merged: read strings; pairs: (r1, r2).
+    Returns (keep_merged, keep_pairs) as lists of bools."""
+    kk = min(k, 31)
+    boundary = set()
+    for seq, circ, deleted, nf, nr in nodes:
+        if deleted or len(seq) < kk:
+            continue
+        starts = range(len(seq) - kk + 1)
+        if not (len(seq) < 2000 or circ or nf > 1 or nr > 1):
+            # the 350 windows at each end (both ranges clipped to the unitig)
+            left = set(range(min(350, len(seq) - kk + 1)))
+            right = set(range(min(max(len(seq) - 350, 0), len(seq) - kk), len(seq) - kk + 1))
+            starts = sorted(left | right)
+        for p in starts:
+            boundary.add(nthash_canonical(seq[p:p + kk]))
+
+    def active(r: str) -> bool:
+        if len(r) < kk:
+            return False
+        return any(nthash_canonical(r[o:o + kk]) in boundary for o in (0, (len(r) - kk) // 2, len(r) - kk))
+
+    return [active(r) for r in merged], [active(a) or active(b) for a, b in pairs]

@@ -23,10 +23,12 @@ from .api import (  # noqa: F401
     compute_dynamic_kmers,
     count_stats,
     estimate_genome,
+    filter_active_reads,
     hll_estimate_registers,
     splitmix64,
     synth_isolate,
     synth_params,
+    upgrade_multik,
 )
 
 from .serve import Admission, Server, ServeOutcome, serve_params  # noqa: F401,E402

@@ -48,6 +48,22 @@ class ArenaView(C.Structure):
     ]
 
 
+class GraphView(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("n_nodes", C.c_uint64), ("seq", C.POINTER(C.c_uint8)),
+                ("offsets", C.POINTER(C.c_uint64)), ("lengths", C.POINTER(C.c_uint32)),
+                ("cov", C.POINTER(C.c_float)), ("flags", C.POINTER(C.c_uint8)),
+                ("n_edges_fwd", C.POINTER(C.c_uint32)), ("n_edges_rev", C.POINTER(C.c_uint32))]
+
+
+class MultiKResult(C.Structure):
+    _fields_ = [("executed", C.c_int32), ("early_stop", C.c_int32), ("genome_size_source", C.c_int32),
+                ("reserved", C.c_int32), ("genome_size_used", C.c_uint64), ("cms_width", C.c_uint64),
+                ("cms_allocated_bytes", C.c_uint64), ("active_unitigs_before", C.c_uint64),
+                ("active_edges_before", C.c_uint64), ("active_unitigs_after", C.c_uint64),
+                ("active_edges_after", C.c_uint64), ("kept_merged", C.c_uint64), ("kept_pairs", C.c_uint64),
+                ("prior_contigs", C.c_uint64), ("transferred_circular", C.c_uint64)]
+
+
 class BuildParams(C.Structure):
     _fields_ = [
         ("k", C.c_uint32),
@@ -201,6 +217,8 @@ SIGNATURES = [
     ("fb_bank_total_bases", C.c_uint64, [vp]),
     ("fb_bank_total_reads", C.c_uint64, [vp]),
     ("fb_bank_bases_for_k", C.c_uint64, [vp, C.c_uint32]),
+    ("fb_bank_arena_info", C.c_int, [vp, C.c_int, u64p, u64p, u64p]),
+    ("fb_bank_download_arena", C.c_int, [vp, C.c_int, u64p, u64p, u32p]),
     ("fb_cms_create", C.c_int, [vp, C.c_uint64, C.c_int, C.POINTER(vp)]),
     ("fb_cms_destroy", None, [vp]),
     ("fb_cms_width", C.c_uint64, [vp]),
@@ -241,6 +259,10 @@ SIGNATURES = [
     ("fb_graph_unitigs", C.c_int, [vp, u8p, u64p, u32p, C.POINTER(C.c_float), u8p]),
     ("fb_graph_edges", C.c_int, [vp, u32p, u32p, u8p, u16p]),
     ("fb_graph_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("fb_graph_view_of", C.c_int, [vp, C.POINTER(GraphView)]),
+    ("fb_filter_active_reads", C.c_int, [vp, vp, C.POINTER(GraphView), C.POINTER(vp), u8p, u8p]),
+    ("fb_upgrade_multik", C.c_int, [vp, vp, C.POINTER(GraphView), C.c_uint32, C.c_uint64, C.POINTER(BuildParams),
+                                    C.POINTER(MultiKResult), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
     ("fb_batch_create", C.c_int, [vp, C.POINTER(vp), u64p, C.c_uint32, C.POINTER(BuildParams), C.POINTER(vp)]),
     ("fb_batch_destroy", None, [vp]),
     ("fb_batch_footprint", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(BuildParams), C.c_uint32]),

@@ -203,6 +203,28 @@ class DeviceBank(_Handle):
     def bases_for_k(self, k: int) -> int:
         return int(self.ctx.lib.fb_bank_bases_for_k(self.h, k))
 
+    @classmethod
+    def _adopt(cls, ctx: Context, h) -> "DeviceBank":
+        obj = cls.__new__(cls)
+        _Handle.__init__(obj, ctx, h)
+        return obj
+
+    def download(self) -> ReadBank:
+        """The three arenas back on the host (BitSeqArena layout)."""
+        out = []
+        for a in range(3):
+            nr, nb, nw = C.c_uint64(), C.c_uint64(), C.c_uint64()
+            self.ctx.check(self.ctx.lib.fb_bank_arena_info(self.h, a, C.byref(nr), C.byref(nb), C.byref(nw)),
+                           "fb_bank_arena_info")
+            data = np.zeros(max(nw.value, 1), np.uint64)
+            offs = np.zeros(max(nr.value, 1), np.uint64)
+            lens = np.zeros(max(nr.value, 1), np.uint32)
+            self.ctx.check(self.ctx.lib.fb_bank_download_arena(self.h, a, _ptr(data, C.c_uint64),
+                                                               _ptr(offs, C.c_uint64), _ptr(lens, C.c_uint32)),
+                           "fb_bank_download_arena")
+            out.append(BitSeqArena(data[:nw.value], offs[:nr.value], lens[:nr.value]))
+        return ReadBank(*out)
+
 
 class CountMinSketch(_Handle):
     """CountMinSketch (sketch.rs:121-254); counter_bits 4 = reference nibbles, 16 = u16."""
@@ -468,6 +490,89 @@ def compact_arrays(ctx: Context, keys, cov, mask, k: int) -> dict:
     ctx.check(ctx.lib.fb_compact_arrays(ctx.h, _ptr(keys, C.c_uint64), _ptr(cov, C.c_uint16), _ptr(mask, C.c_uint8),
                                         int(cov.size), k, C.byref(h)), "fb_compact_arrays")
     return _graph_result(ctx, h)
+
+
+# ---------------------------------------------------------------------------
+# Multi-k flow: filter_active_reads and upgrade_multik_ext (local_multik.rs)
+
+
+class _GraphArrays:
+    """fb_graph_view over a graph dict (seqs, circular, src; optional cov,
+    deleted), keeping the arrays alive."""
+
+    def __init__(self, graph: dict, k: int):
+        seqs = [bytes(x) for x in graph["seqs"]]
+        n = len(seqs)
+        self.lens = np.array([len(x) for x in seqs], np.uint32)
+        self.offs = np.zeros(n, np.uint64)
+        if n:
+            self.offs[1:] = np.cumsum(self.lens[:-1], dtype=np.uint64)
+        self.seq = np.frombuffer(b"".join(seqs), np.uint8).copy()
+        self.flags = (np.asarray(graph.get("circular", np.zeros(n)), np.uint8) << 1).astype(np.uint8)
+        if "deleted" in graph:
+            self.flags |= np.asarray(graph["deleted"], np.uint8)
+        src = np.asarray(graph.get("src", np.zeros(0)), np.int64)
+        self.nf = np.bincount(src[(src & 1) == 0] >> 1, minlength=n).astype(np.uint32)
+        self.nr = np.bincount(src[(src & 1) == 1] >> 1, minlength=n).astype(np.uint32)
+        self.cov = np.asarray(graph.get("cov", np.zeros(n)), np.float32).copy()
+        self.view = _lib.GraphView(k, n, _ptr(self.seq, C.c_uint8), _ptr(self.offs, C.c_uint64),
+                                   _ptr(self.lens, C.c_uint32),
+                                   self.cov.ctypes.data_as(C.POINTER(C.c_float)) if n else None,
+                                   _ptr(self.flags, C.c_uint8), _ptr(self.nf, C.c_uint32), _ptr(self.nr, C.c_uint32))
+
+
+def filter_active_reads(ctx: Context, bank: DeviceBank, graph: dict, k: int) -> tuple:
+    """filter_active_reads (local_multik.rs:259-333) of a device bank against a
+    graph dict at graph k -> (DeviceBank of the kept reads, keep_merged,
+    keep_pairs)."""
+    ga = _GraphArrays(graph, k)
+    nm, npair = _arena_reads(ctx, bank)
+    km = np.zeros(max(nm, 1), np.uint8)
+    kp = np.zeros(max(npair, 1), np.uint8)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_filter_active_reads(ctx.h, bank.h, C.byref(ga.view), C.byref(h), _ptr(km, C.c_uint8),
+                                             _ptr(kp, C.c_uint8)), "fb_filter_active_reads")
+    return DeviceBank._adopt(ctx, h), km[:nm], kp[:npair]
+
+
+def _arena_reads(ctx: Context, bank: DeviceBank) -> tuple:
+    n = []
+    for a in range(3):
+        nr = C.c_uint64()
+        ctx.check(ctx.lib.fb_bank_arena_info(bank.h, a, C.byref(nr), None, None), "fb_bank_arena_info")
+        n.append(int(nr.value))
+    return n[0], min(n[1], n[2])
+
+
+MULTIK_EARLY_STOP = {0: None, 1: "K2_LE_PREV_K", 2: "NO_BRANCHING_REGIONS"}
+GENOME_SIZE_SOURCE = {0: "SKIPPED", 1: "EXPLICIT_PASSED", 2: "FALLBACK_READ_BASES_DIV_60"}
+
+
+def upgrade_multik(ctx: Context, bank: DeviceBank, graph: dict, k: int, k2: int, genome_size: Optional[int] = None,
+                   keep_solid: bool = False, keep_active: bool = False, **params) -> dict:
+    """upgrade_multik_ext (local_multik.rs:38-147) without simplify_graph:
+    result fields of MultiKUpgradeResult, the new graph dict ("graph", None on
+    an early stop), and optionally the k2 solid set ("solid", a SolidHandle)
+    and the filtered bank ("active", a DeviceBank).  params: counter_bits,
+    cms_width, hll_p and launch-shape fields of build_params."""
+    ga = _GraphArrays(graph, k)
+    p = build_params(k2, **params) if params else None
+    res = _lib.MultiKResult()
+    hg, hs, hb = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.fb_upgrade_multik(ctx.h, bank.h, C.byref(ga.view), k2, genome_size or 0,
+                                        C.byref(p) if p is not None else None, C.byref(res), C.byref(hg),
+                                        C.byref(hs) if keep_solid else None, C.byref(hb) if keep_active else None),
+              "fb_upgrade_multik")
+    out = {f: getattr(res, f) for f, _ in _lib.MultiKResult._fields_ if f != "reserved"}
+    out["executed"] = bool(out["executed"])
+    out["early_stop_reason"] = MULTIK_EARLY_STOP[out.pop("early_stop")]
+    out["genome_size_source"] = GENOME_SIZE_SOURCE[out["genome_size_source"]]
+    out["graph"] = _graph_result(ctx, hg) if hg.value else None
+    if keep_solid:
+        out["solid"] = SolidHandle(ctx, hs) if hs.value else None
+    if keep_active:
+        out["active"] = DeviceBank._adopt(ctx, hb) if hb.value else None
+    return out
 
 
 # ---------------------------------------------------------------------------

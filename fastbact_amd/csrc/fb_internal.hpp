@@ -288,6 +288,44 @@ hipError_t launch_cg_table(const CgArgs& a, hipStream_t s);
 hipError_t launch_cg_write(const CgArgs& a, uint32_t sp, uint32_t sc, hipStream_t s);
 hipError_t launch_cg_flip(uint8_t* seq, const uint64_t* off, const uint32_t* len, uint32_t n_lin, uint8_t* flip,
                           uint64_t lin_bases, hipStream_t s);
+// filter_active_reads (local_multik.rs:259-333), fb_filter.hip.
+struct FaArgs {
+  // boundary set: 2^log2cap u64 slots (kEmpty = empty) + set_flag[0] for the key kEmpty
+  uint64_t* set = nullptr;
+  uint32_t* set_flag = nullptr;
+  uint32_t log2cap = 0;
+  uint32_t k = 0;  // k' = min(graph k, 31)
+  // unitig windows: range r holds windows rpre[r] .. rpre[r+1]-1 starting at useq[rstart[r]]
+  const uint8_t* useq = nullptr;
+  const uint64_t* rstart = nullptr;
+  const uint64_t* rpre = nullptr;
+  uint32_t n_ranges = 0;
+  uint64_t n_windows = 0;
+  // source reads: merged reads, pe_forward, pe_reverse start at first[0..2]
+  DevReads reads;
+  uint64_t first[3] = {0, 0, 0};
+  uint64_t n_merged = 0, n_pairs = 0;
+  uint32_t* keep = nullptr;  // n_merged + n_pairs + 1
+  uint32_t* pos = nullptr;   // exclusive scan of keep
+  uint32_t* src = nullptr;   // kept reads' source index, output order
+  uint32_t* olen = nullptr;  // their lengths (+ one trailing 0)
+  uint64_t* opos = nullptr;  // exclusive scan of olen
+  // output list: arena a holds reads rfirst[a] .. rfirst[a+1]-1 and words word0[a] .. word0[a+1]-1
+  uint64_t rfirst[4] = {0, 0, 0, 0};
+  uint64_t word0[4] = {0, 0, 0, 0};
+  uint64_t* odata = nullptr;
+  uint64_t* ooffs = nullptr;
+  uint32_t* olens = nullptr;
+  uint32_t* hist = nullptr;  // read count per length (hist_n bins, last open)
+  uint32_t hist_n = 1;
+};
+hipError_t launch_fa_set_clear(const FaArgs& a, hipStream_t s);
+hipError_t launch_fa_set_insert(const FaArgs& a, hipStream_t s);
+hipError_t launch_fa_keep(const FaArgs& a, hipStream_t s);
+hipError_t launch_fa_list(const FaArgs& a, hipStream_t s);
+hipError_t launch_fa_offsets(const FaArgs& a, hipStream_t s);
+hipError_t launch_fa_gather(const FaArgs& a, hipStream_t s);
+
 hipError_t cg_exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
                                 hipStream_t s);
 hipError_t cg_exclusive_sum_u64(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,

@@ -246,6 +246,11 @@ struct fb_bank {
   uint64_t total_bases = 0;
   std::vector<uint64_t> len_hist;  // bases summed per read length (index = length), last bin open
   uint32_t max_len = 0;
+  // the three arenas (merged, pe_forward, pe_reverse) inside the device list:
+  // first read, read count, first word, words, and BitSeqArena::total_bases
+  // (bitseq.rs:185-192: last offset + last length, arena-local)
+  uint64_t ar_first[3] = {0, 0, 0}, ar_reads[3] = {0, 0, 0};
+  uint64_t ar_word0[3] = {0, 0, 0}, ar_words[3] = {0, 0, 0}, ar_bases[3] = {0, 0, 0};
   DevReads view() const {
     DevReads r;
     r.data = data.as<uint64_t>();
@@ -321,7 +326,51 @@ extern "C" int fb_bank_upload(fb_ctx* c, const fb_arena_view arenas[3], fb_bank*
     delete b;
     return r;
   }
+  uint64_t rpos = 0, wpos = 0;
+  for (int a = 0; a < 3; ++a) {
+    const fb_arena_view& v = arenas[a];
+    b->ar_first[a] = rpos;
+    b->ar_reads[a] = v.n_reads;
+    b->ar_word0[a] = wpos;
+    b->ar_words[a] = v.n_words;
+    b->ar_bases[a] = v.n_reads ? v.offsets[v.n_reads - 1] + v.lengths[v.n_reads - 1] : 0;
+    rpos += v.n_reads;
+    wpos += v.n_words;
+  }
+  // ReadBank::total_bases (bitseq.rs:373-375) sums the arenas' last offset +
+  // last length, which equals the sum of lengths for push_seq-built arenas
+  b->total_bases = b->ar_bases[0] + b->ar_bases[1] + b->ar_bases[2];
   *out = b;
+  return FB_OK;
+}
+
+extern "C" int fb_bank_arena_info(const fb_bank* b, int arena, uint64_t* n_reads, uint64_t* total_bases,
+                                  uint64_t* n_words) {
+  if (!b || arena < 0 || arena > 2) return FB_ERR_INVALID;
+  if (n_reads) *n_reads = b->ar_reads[arena];
+  if (total_bases) *total_bases = b->ar_bases[arena];
+  if (n_words) *n_words = b->ar_words[arena];
+  return FB_OK;
+}
+
+extern "C" int fb_bank_download_arena(const fb_bank* b, int arena, uint64_t* data, uint64_t* offsets,
+                                      uint32_t* lengths) {
+  if (!b || arena < 0 || arena > 2) return FB_ERR_INVALID;
+  fb_ctx* c = b->ctx;
+  if (int r = ctx_current(c)) return r;
+  const uint64_t nr = b->ar_reads[arena], nw = b->ar_words[arena];
+  if (data && nw)
+    HIP_TRY(c, hipMemcpyAsync(data, b->data.as<uint64_t>() + b->ar_word0[arena], nw * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+  if (lengths && nr)
+    HIP_TRY(c, hipMemcpyAsync(lengths, b->lens.as<uint32_t>() + b->ar_first[arena], nr * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+  if (offsets && nr)
+    HIP_TRY(c, hipMemcpyAsync(offsets, b->offs.as<uint64_t>() + b->ar_first[arena], nr * 8, hipMemcpyDeviceToHost,
+                              c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (offsets)  // device offsets are global base indices of the list
+    for (uint64_t i = 0; i < nr; ++i) offsets[i] -= b->ar_word0[arena] * 32;
   return FB_OK;
 }
 
@@ -2464,6 +2513,9 @@ struct fb_graph {
   std::vector<uint32_t> e_src, e_dst;
   std::vector<uint8_t> e_ov;
   std::vector<uint16_t> e_sup;
+  // per node: flags (bit 0 is_deleted, bit 1 is_circular), edges_fwd.len(), edges_rev.len()
+  std::vector<uint8_t> flags;
+  std::vector<uint32_t> nf, nr;
   float ms[3] = {0, 0, 0};  // device walks, sequence download, edges
 };
 
@@ -2532,6 +2584,14 @@ static void graph_edges(fb_graph* g) {
           if ((it->d >> 1) != u || g->circ[u]) add(side ? er[u] : ef[u], it->d, (uint8_t)ov);
       }
     }
+  }
+  g->flags.assign(U, 0);
+  g->nf.assign(U, 0);
+  g->nr.assign(U, 0);
+  for (uint64_t u = 0; u < U; ++u) {
+    g->flags[u] = g->circ[u] ? 2 : 0;
+    g->nf[u] = (uint32_t)ef[u].size();
+    g->nr[u] = (uint32_t)er[u].size();
   }
   for (uint64_t u = 0; u < U; ++u)
     for (int side = 0; side < 2; ++side)
@@ -2781,5 +2841,303 @@ extern "C" int fb_graph_edges(const fb_graph* g, uint32_t* src, uint32_t* dst, u
 extern "C" int fb_graph_timings(const fb_graph* g, float out[3]) {
   if (!g || !out) return FB_ERR_INVALID;
   memcpy(out, g->ms, sizeof g->ms);
+  return FB_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// filter_active_reads (local_multik.rs:259-333) on the device (fb_filter.hip)
+// and the multi-k upgrade driver (upgrade_multik_ext, local_multik.rs:38-147).
+
+extern "C" int fb_graph_view_of(const fb_graph* g, fb_graph_view* v) {
+  if (!g || !v) return FB_ERR_INVALID;
+  v->k = g->k;
+  v->n_nodes = g->len.size();
+  v->seq = g->seq.data();
+  v->offsets = g->off.data();
+  v->lengths = g->len.data();
+  v->cov = g->cov.data();
+  v->flags = g->flags.data();
+  v->n_edges_fwd = g->nf.data();
+  v->n_edges_rev = g->nr.data();
+  return FB_OK;
+}
+
+static uint8_t view_flags(const fb_graph_view* g, uint64_t i) { return g->flags ? g->flags[i] : 0; }
+static uint32_t view_nf(const fb_graph_view* g, uint64_t i) { return g->n_edges_fwd ? g->n_edges_fwd[i] : 0; }
+static uint32_t view_nr(const fb_graph_view* g, uint64_t i) { return g->n_edges_rev ? g->n_edges_rev[i] : 0; }
+
+static int check_view(fb_ctx* c, const fb_graph_view* g) {
+  if (!g) return set_err(c, FB_ERR_INVALID, "null graph");
+  if (g->k < 1) return set_err(c, FB_ERR_INVALID, "graph k must be >= 1");
+  if (g->n_nodes && (!g->seq || !g->offsets || !g->lengths)) return set_err(c, FB_ERR_INVALID, "graph: null arrays");
+  return FB_OK;
+}
+
+// Kept reads, filtered list: kept[0] merged reads, kept[1] pairs.
+static int filter_active(fb_ctx* c, const fb_bank* b, const fb_graph_view* g, fb_bank** out, uint8_t* keep_m,
+                         uint8_t* keep_p, uint64_t kept[2]) {
+  if (int r = check_view(c, g)) return r;
+  if (b->n_reads >= (1ull << 31)) return set_err(c, FB_ERR_INVALID, "filter_active_reads takes < 2^31 reads");
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+  const uint32_t k = std::min<uint32_t>(g->k, 31);  // local_multik.rs:261
+  const uint64_t win = 350;                          // boundary_window
+
+  // windows of the boundary set, as ranges over the kept unitig sequences
+  std::vector<uint8_t> useq;
+  std::vector<uint64_t> rstart, rpre(1, 0);
+  auto range = [&](uint64_t at, uint64_t n) {
+    if (!n) return;
+    rstart.push_back(at);
+    rpre.push_back(rpre.back() + n);
+  };
+  for (uint64_t i = 0; i < g->n_nodes; ++i) {
+    const uint8_t fl = view_flags(g, i);
+    const uint64_t L = g->lengths[i];
+    if ((fl & 1) || L < k) continue;
+    const uint64_t base = useq.size();
+    useq.insert(useq.end(), g->seq + g->offsets[i], g->seq + g->offsets[i] + L);
+    if (L < 2000 || (fl & 2) || view_nf(g, i) > 1 || view_nr(g, i) > 1) {
+      range(base, L - k + 1);
+    } else {
+      range(base, std::min<uint64_t>(win, L - k + 1));
+      const uint64_t rs = std::min<uint64_t>(L > win ? L - win : 0, L - k);
+      range(base + rs, L - k - rs + 1);
+    }
+  }
+  if (rstart.size() >= (1ull << 32)) return set_err(c, FB_ERR_INVALID, "too many unitigs");
+  FaArgs a;
+  a.k = k;
+  a.n_ranges = (uint32_t)rstart.size();
+  a.n_windows = rpre.back();
+  uint32_t log2cap = 10;
+  while ((1ull << log2cap) < 2 * a.n_windows) ++log2cap;
+  a.log2cap = log2cap;
+  const uint64_t nm = b->ar_reads[0], np = std::min(b->ar_reads[1], b->ar_reads[2]);
+  a.reads = b->view();
+  for (int i = 0; i < 3; ++i) a.first[i] = b->ar_first[i];
+  a.n_merged = nm;
+  a.n_pairs = np;
+  const uint64_t n = nm + np;
+
+  DBuf dseq, drs, drp, dset, dflag, dkeep, dpos, dsrc, dolen, dopos, dtmp, dhist;
+  if (int r = to_device(c, dseq, useq.data(), useq.size())) return r;
+  if (int r = to_device(c, drs, rstart.data(), rstart.size() * 8)) return r;
+  if (int r = to_device(c, drp, rpre.data(), rpre.size() * 8)) return r;
+  if (int r = dset.get(c, (1ull << log2cap) * 8)) return r;
+  if (int r = dflag.get(c, 256)) return r;
+  if (int r = dkeep.get(c, (n + 1) * 4)) return r;
+  if (int r = dpos.get(c, (n + 1) * 4)) return r;
+  a.useq = dseq.as<uint8_t>();
+  a.rstart = drs.as<uint64_t>();
+  a.rpre = drp.as<uint64_t>();
+  a.set = dset.as<uint64_t>();
+  a.set_flag = dflag.as<uint32_t>();
+  a.keep = dkeep.as<uint32_t>();
+  a.pos = dpos.as<uint32_t>();
+  HIP_TRY(c, launch_fa_set_clear(a, s));
+  HIP_TRY(c, launch_fa_set_insert(a, s));
+  HIP_TRY(c, launch_fa_keep(a, s));
+  size_t tb = 0, tb2 = 0;
+  HIP_TRY(c, cg_exclusive_sum_u32(nullptr, nullptr, n + 1, nullptr, &tb, s));
+  HIP_TRY(c, cg_exclusive_sum_u64(nullptr, nullptr, n + 1 + np, nullptr, &tb2, s));
+  if (int r = dtmp.get(c, std::max<size_t>(std::max(tb, tb2), 16))) return r;
+  tb = dtmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u32(a.keep, a.pos, n + 1, dtmp.p, &tb, s));
+  uint32_t cnt[2] = {0, 0};
+  HIP_TRY(c, hipMemcpyAsync(&cnt[0], a.pos + nm, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&cnt[1], a.pos + n, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const uint64_t nkm = cnt[0], nkp = cnt[1] - cnt[0], n_out = nkm + 2 * nkp;
+  kept[0] = nkm;
+  kept[1] = nkp;
+
+  if (int r = dsrc.get(c, std::max<uint64_t>(n_out, 1) * 4)) return r;
+  if (int r = dolen.get(c, (n_out + 1) * 4)) return r;
+  if (int r = dopos.get(c, (n_out + 1) * 8)) return r;
+  a.src = dsrc.as<uint32_t>();
+  a.olen = dolen.as<uint32_t>();
+  a.opos = dopos.as<uint64_t>();
+  HIP_TRY(c, launch_fill_u32(a.olen, n_out + 1, 0, s));
+  HIP_TRY(c, launch_fa_list(a, s));
+  tb2 = dtmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u64(a.olen, a.opos, n_out + 1, dtmp.p, &tb2, s));
+  uint64_t bnd[3] = {0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(&bnd[0], a.opos + nkm, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&bnd[1], a.opos + nkm + nkp, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&bnd[2], a.opos + n_out, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const uint64_t B[3] = {bnd[0], bnd[1] - bnd[0], bnd[2] - bnd[1]};
+  a.rfirst[0] = 0;
+  a.rfirst[1] = nkm;
+  a.rfirst[2] = nkm + nkp;
+  a.rfirst[3] = n_out;
+  a.word0[0] = 0;
+  for (int i = 0; i < 3; ++i) a.word0[i + 1] = a.word0[i] + (B[i] + 31) / 32;
+
+  auto nb = std::make_unique<fb_bank>();
+  nb->ctx = c;
+  if (int r = nb->data.get(c, (a.word0[3] + kArenaPadWords) * 8)) return r;
+  if (int r = nb->offs.get(c, std::max<uint64_t>(n_out, 1) * 8)) return r;
+  if (int r = nb->lens.get(c, std::max<uint64_t>(n_out, 1) * 4)) return r;
+  a.hist_n = b->max_len + 1;
+  if (int r = dhist.get(c, (uint64_t)a.hist_n * 4)) return r;
+  a.odata = nb->data.as<uint64_t>();
+  a.ooffs = nb->offs.as<uint64_t>();
+  a.olens = nb->lens.as<uint32_t>();
+  a.hist = dhist.as<uint32_t>();
+  HIP_TRY(c, launch_fill_u32(a.hist, a.hist_n, 0, s));
+  HIP_TRY(c, launch_fa_offsets(a, s));
+  HIP_TRY(c, launch_fa_gather(a, s));
+  HIP_TRY(c, hipMemsetAsync(a.odata + a.word0[3], 0, kArenaPadWords * 8, s));
+  std::vector<uint32_t> hist(a.hist_n), hk;
+  HIP_TRY(c, hipMemcpyAsync(hist.data(), a.hist, (uint64_t)a.hist_n * 4, hipMemcpyDeviceToHost, s));
+  if (keep_m || keep_p) {
+    hk.resize(n);
+    if (n) HIP_TRY(c, hipMemcpyAsync(hk.data(), a.keep, n * 4, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(c, hipStreamSynchronize(s));
+  for (uint64_t i = 0; keep_m && i < nm; ++i) keep_m[i] = (uint8_t)hk[i];
+  for (uint64_t i = 0; keep_p && i < np; ++i) keep_p[i] = (uint8_t)hk[nm + i];
+
+  nb->n_reads = n_out;
+  nb->len_hist.assign(a.hist_n, 0);
+  nb->max_len = 0;
+  for (uint32_t L = 0; L < a.hist_n; ++L)
+    if (hist[L]) {
+      nb->len_hist[L] = (uint64_t)hist[L] * L;
+      nb->max_len = L;
+    }
+  const uint64_t nr_out[3] = {nkm, nkp, nkp};
+  for (int i = 0; i < 3; ++i) {
+    nb->ar_first[i] = a.rfirst[i];
+    nb->ar_reads[i] = nr_out[i];
+    nb->ar_word0[i] = a.word0[i];
+    nb->ar_words[i] = a.word0[i + 1] - a.word0[i];
+    nb->ar_bases[i] = B[i];
+  }
+  nb->total_bases = B[0] + B[1] + B[2];
+  *out = nb.release();
+  return FB_OK;
+}
+
+extern "C" int fb_filter_active_reads(fb_ctx* c, const fb_bank* bank, const fb_graph_view* g, fb_bank** out,
+                                      uint8_t* keep_merged, uint8_t* keep_pairs) {
+  if (!c || !bank || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (bank->ctx != c) return set_err(c, FB_ERR_INVALID, "the bank belongs to another context");
+  uint64_t kept[2];
+  return filter_active(c, bank, g, out, keep_merged, keep_pairs, kept);
+}
+
+// Appends an old circular unitig to the new graph as upgrade_multik_ext
+// does (local_multik.rs:117-128): circular, its cov, one forward self edge
+// of overlap k2 - 1 and support 10.
+static void graph_push_circular(fb_graph* g, const uint8_t* seq, uint32_t len, float cov, uint32_t k2) {
+  const uint32_t id = (uint32_t)g->len.size();
+  g->off.push_back(g->seq.size());
+  g->seq.insert(g->seq.end(), seq, seq + len);
+  g->len.push_back(len);
+  g->cov.push_back(cov);
+  g->circ.push_back(1);
+  g->flags.push_back(2);
+  g->nf.push_back(1);
+  g->nr.push_back(0);
+  g->e_src.push_back(id << 1);
+  g->e_dst.push_back(id << 1);
+  g->e_ov.push_back((uint8_t)(k2 - 1));
+  g->e_sup.push_back(10);
+}
+
+extern "C" int fb_upgrade_multik(fb_ctx* c, const fb_bank* bank, const fb_graph_view* prev, uint32_t k2,
+                                 uint64_t genome_size, const fb_build_params* params, fb_multik_result* res,
+                                 fb_graph** out_graph, fb_solid** out_solid, fb_bank** out_active) {
+  if (!c || !bank || !res || !out_graph) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out_graph = nullptr;
+  if (out_solid) *out_solid = nullptr;
+  if (out_active) *out_active = nullptr;
+  if (bank->ctx != c) return set_err(c, FB_ERR_INVALID, "the bank belongs to another context");
+  if (int r = check_view(c, prev)) return r;
+  if (k2 > 127) return set_err(c, FB_ERR_INVALID, "k2=%u above 127", k2);
+  memset(res, 0, sizeof *res);
+  // inspect_graph (diag.rs:112-133): active unitigs and edges
+  for (uint64_t i = 0; i < prev->n_nodes; ++i)
+    if (!(view_flags(prev, i) & 1)) {
+      res->active_unitigs_before++;
+      res->active_edges_before += view_nf(prev, i) + view_nr(prev, i);
+    }
+  res->active_unitigs_after = res->active_unitigs_before;
+  res->active_edges_after = res->active_edges_before;
+  if (k2 <= prev->k) {
+    res->early_stop = FB_MULTIK_K2_LE_PREV_K;
+    return FB_OK;
+  }
+  bool branching = false;  // local_multik.rs:60-68
+  for (uint64_t i = 0; i < prev->n_nodes && !branching; ++i) {
+    const uint8_t fl = view_flags(prev, i);
+    if (!(fl & 1) && !(fl & 2) && (view_nf(prev, i) > 1 || view_nr(prev, i) > 1)) branching = true;
+  }
+  if (!branching) {
+    res->early_stop = FB_MULTIK_NO_BRANCHING_REGIONS;
+    return FB_OK;
+  }
+  // prior contigs: every active linear unitig (local_multik.rs:84-90)
+  std::vector<uint8_t> contigs;
+  std::vector<uint32_t> clens;
+  for (uint64_t i = 0; i < prev->n_nodes; ++i)
+    if (!(view_flags(prev, i) & 3)) {
+      contigs.insert(contigs.end(), prev->seq + prev->offsets[i], prev->seq + prev->offsets[i] + prev->lengths[i]);
+      clens.push_back(prev->lengths[i]);
+    }
+  res->prior_contigs = clens.size();
+  fb_bank* active = nullptr;
+  uint64_t kept[2] = {0, 0};
+  if (int r = filter_active(c, bank, prev, &active, nullptr, nullptr, kept)) return r;
+  std::unique_ptr<fb_bank> active_own(active);
+  res->kept_merged = kept[0];
+  res->kept_pairs = kept[1];
+  // local_multik.rs:95-102: genome size of the original bank
+  if (genome_size) {
+    res->genome_size_used = std::max<uint64_t>(genome_size, 50000);
+    res->genome_size_source = FB_GENOME_EXPLICIT_PASSED;
+  } else {
+    res->genome_size_used = std::max<uint64_t>(bank->total_bases / 60, 50000);
+    res->genome_size_source = FB_GENOME_FALLBACK_READ_BASES_DIV_60;
+  }
+  fb_build_params p;
+  if (params) p = *params;
+  else fb_build_params_default(&p, k2);
+  p.k = k2;
+  p.target_solid_cov = 4;
+  p.genome_size = res->genome_size_used;
+  p.sort = 1;
+  res->cms_width = p.cms_width ? next_pow2(p.cms_width) : fb_cms_width_for_genome_size(res->genome_size_used);
+  res->cms_allocated_bytes = 4 * res->cms_width / 2;  // sketch.rs:171-174
+  fb_solid* solid = nullptr;
+  if (int r = fb_build_solid(c, active, contigs.data(), clens.data(), clens.size(), &p, &solid)) return r;
+  std::unique_ptr<fb_solid> solid_own(solid);
+  fb_graph* g = nullptr;
+  if (int r = fb_compact_solid(solid, &g)) return r;
+  std::unique_ptr<fb_graph> g_own(g);
+  // local_multik.rs:107-129: keep the old circular unitigs the new graph lacks
+  for (uint64_t i = 0; i < prev->n_nodes; ++i) {
+    const uint8_t fl = view_flags(prev, i);
+    if ((fl & 1) || !(fl & 2)) continue;
+    const int64_t old_len = prev->lengths[i];
+    bool exists = false;
+    for (uint64_t j = 0; j < g->len.size() && !exists; ++j)
+      exists = g->circ[j] && std::llabs((int64_t)g->len[j] - old_len) <= (int64_t)k2 + 10;
+    if (!exists) {
+      graph_push_circular(g, prev->seq + prev->offsets[i], prev->lengths[i], prev->cov ? prev->cov[i] : 0.0f, k2);
+      res->transferred_circular++;
+    }
+  }
+  res->executed = 1;
+  res->active_unitigs_after = g->len.size();
+  res->active_edges_after = g->e_src.size();
+  *out_graph = g_own.release();
+  if (out_solid) *out_solid = solid_own.release();
+  if (out_active) *out_active = active_own.release();
   return FB_OK;
 }

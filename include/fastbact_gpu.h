@@ -19,6 +19,8 @@
  *   fb_build_solid      build_cdbg_with_contigs_ext<K>   src/graph/build.rs:67-328
  *                       (passes 1-3; unitig compaction, build.rs:330-597, is downstream)
  *   fb_compact_*        unitig compaction (steps 4-6)    src/graph/build.rs:330-597
+ *   fb_filter_active_reads  filter_active_reads          src/graph/local_multik.rs:259-333
+ *   fb_upgrade_multik   upgrade_multik_ext<K2>           src/graph/local_multik.rs:38-147
  *   fb_estimate_genome  estimate_genome                  src/ingest/estimator.rs:286-440
  *   fb_count_stats_from_lengths  count_fastx_exact (length part)  src/ingest/estimator.rs:167-283
  *   fb_compute_dynamic_kmers                             src/ingest/estimator.rs:104-161
@@ -88,6 +90,12 @@ uint64_t fb_bank_total_bases(const fb_bank* bank);
 uint64_t fb_bank_total_reads(const fb_bank* bank);
 /* Sum of lengths of reads with len >= k (the bases a k pass counts). */
 uint64_t fb_bank_bases_for_k(const fb_bank* bank, uint32_t k);
+/* Arena a (0 merged, 1 pe_forward, 2 pe_reverse): BitSeqArena::len(),
+ * total_bases() (bitseq.rs:185-192) and data words. */
+int fb_bank_arena_info(const fb_bank* bank, int arena, uint64_t* n_reads, uint64_t* total_bases, uint64_t* n_words);
+/* Copies arena a back in the BitSeqArena layout (arena-local offsets);
+ * sizes from fb_bank_arena_info, any pointer may be null. */
+int fb_bank_download_arena(const fb_bank* bank, int arena, uint64_t* data, uint64_t* offsets, uint32_t* lengths);
 
 /* ---- CountMinSketch (sketch.rs:121-254) -----------------------------------
  * width is rounded up to a power of two (sketch.rs:141-143).  counter_bits 4
@@ -212,6 +220,77 @@ int fb_graph_unitigs(const fb_graph* g, uint8_t* seq, uint64_t* offsets, uint32_
 int fb_graph_edges(const fb_graph* g, uint32_t* src, uint32_t* dst, uint8_t* ov, uint16_t* support);
 /* Milliseconds: [0] device walks, [1] sequence download, [2] edges on the host. */
 int fb_graph_timings(const fb_graph* g, float out[3]);
+
+/* ---- Multi-k flow: filter_active_reads and upgrade_multik_ext ---------------
+ * (src/graph/local_multik.rs:38-147,259-333)
+ * A CdbgGraph as arrays, so that any graph (this library's compaction, or a
+ * caller's simplified graph with deleted nodes) can drive the flow.  Node i:
+ * ASCII seq[offsets[i] .. + lengths[i]), kmer_cov, flags (bit 0 is_deleted,
+ * bit 1 is_circular), edges_fwd.len() and edges_rev.len().  cov, flags and
+ * the edge counts may be null (read as 0). */
+typedef struct {
+  uint32_t k;
+  uint64_t n_nodes;
+  const uint8_t* seq;
+  const uint64_t* offsets;
+  const uint32_t* lengths;
+  const float* cov;
+  const uint8_t* flags;
+  const uint32_t* n_edges_fwd;
+  const uint32_t* n_edges_rev;
+} fb_graph_view;
+/* A view of a compacted graph (valid while g lives). */
+int fb_graph_view_of(const fb_graph* g, fb_graph_view* out);
+/* filter_active_reads: the reads of `bank` near unitig ends, branches,
+ * short or circular unitigs, as a new device bank (pushed in order, as
+ * BitSeqArena::push_seq packs them).  The boundary set holds the canonical
+ * ntHash (k' = min(k, 31)) of every window of unitigs < 2000 bp, circular or
+ * with > 1 edge on a side, and of the 350 windows at each end of the others;
+ * a read is active when its window at 0, (len-k')/2 or len-k' is in the set;
+ * a pair (i < min(pe_forward, pe_reverse reads)) is kept when either mate is.
+ * keep_merged (one byte per merged read) and keep_pairs (one per pair) may
+ * be null. */
+int fb_filter_active_reads(fb_ctx* ctx, const fb_bank* bank, const fb_graph_view* g, fb_bank** out,
+                           uint8_t* keep_merged, uint8_t* keep_pairs);
+
+#define FB_MULTIK_K2_LE_PREV_K 1          /* early_stop "K2_LE_PREV_K" */
+#define FB_MULTIK_NO_BRANCHING_REGIONS 2  /* early_stop "NO_BRANCHING_REGIONS" */
+#define FB_GENOME_EXPLICIT_PASSED 1       /* genome_size_source "EXPLICIT_PASSED" */
+#define FB_GENOME_FALLBACK_READ_BASES_DIV_60 2
+/* MultiKUpgradeResult (local_multik.rs:12-23) plus the filter's counts. */
+typedef struct {
+  int32_t executed;
+  int32_t early_stop;          /* 0 none, else FB_MULTIK_* */
+  int32_t genome_size_source;  /* 0 skipped, else FB_GENOME_* */
+  int32_t reserved;
+  uint64_t genome_size_used;
+  uint64_t cms_width;
+  uint64_t cms_allocated_bytes;  /* CountMinSketch::allocated_bytes_for_genome_size (nibbles) */
+  uint64_t active_unitigs_before;
+  uint64_t active_edges_before;
+  uint64_t active_unitigs_after;
+  uint64_t active_edges_after;
+  uint64_t kept_merged;           /* merged reads kept by the filter */
+  uint64_t kept_pairs;            /* pairs kept by the filter */
+  uint64_t prior_contigs;         /* active linear unitigs passed as contigs */
+  uint64_t transferred_circular;  /* old circular unitigs added to the new graph */
+} fb_multik_result;
+/* upgrade_multik_ext<K2>: on a graph with branching, builds the solid set at
+ * k2 from the filtered bank plus every active linear unitig as a prior contig
+ * (x10 CMS adds, +100 cov, +1000 edges), target_solid_cov Some(4), genome
+ * size max(genome_size, 50000) or, with genome_size 0, max(bank bases / 60,
+ * 50000); compacts it; then appends each old circular unitig that no new
+ * circular unitig matches within k2 + 10 bases.  simplify_graph (before and
+ * after the transfer in the reference) is not part of this library: the
+ * returned graph is the compaction itself.  params (may be null: the
+ * reference's nibble sketch) sets counter_bits, cms_width, hll_p and the
+ * launch shape; k, target, genome size and sort are set here.  On an early
+ * stop (res->executed = 0) nothing is built and *out_graph is null.
+ * out_solid and out_active (may be null) receive the k2 solid set and the
+ * filtered bank, both on the device. */
+int fb_upgrade_multik(fb_ctx* ctx, const fb_bank* bank, const fb_graph_view* prev, uint32_t k2, uint64_t genome_size,
+                      const fb_build_params* params, fb_multik_result* res, fb_graph** out_graph, fb_solid** out_solid,
+                      fb_bank** out_active);
 
 /* ---- Batched form: many isolates per launch --------------------------------
  * The reference builds one isolate per process run (main.rs:262).  A batch

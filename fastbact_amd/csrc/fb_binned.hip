@@ -2,19 +2,21 @@
 //
 // The Count-Min table (4 x width cells) is cut into bins of 2^sb cells, each
 // small enough to count in one workgroup's LDS.
-//   Phase A (k_bin_scatter): each workgroup walks its reads twice.  The first
-//     walk counts the workgroup's updates per bin in LDS (and feeds the HLL).
-//     One global atomic per bin then reserves a run in that bin's region,
-//     in the slice that belongs to the workgroup's XCD.  The second walk goes
-//     in rounds of kRound windows per thread: the round's cells are counted
-//     per bin, sorted by bin into an LDS stage (a counting sort), and written
-//     out in stage order, so a wave's store covers a few contiguous runs
-//     instead of 64 scattered lines.  An update that finds its slice full is
-//     added to the table directly with a global atomic, so any input
-//     (low-complexity reads included) stays exact.
+//   Phase A (k_bin_scatter): each workgroup walks its reads once, in rounds
+//     of kRound windows per thread.  A round's cells are counted per bin,
+//     sorted by bin into an LDS stage (a counting sort) and written out in
+//     stage order, so a wave's store covers a few contiguous runs instead of
+//     64 scattered lines.  Each bin's offsets go into chunks of kChunk
+//     entries (one 128-byte line) that the workgroup reserves in the bin's
+//     region with one global atomic per chunk, in the slice that belongs to
+//     the workgroup's XCD; a workgroup fills its own lines only, and marks
+//     the unused tail of its last chunk per bin with kNoCell.  An update that
+//     finds its slice full is added to the table directly with a global
+//     atomic, so any input (low-complexity reads included) stays exact.
 //   Phase B (k_bin_count): one workgroup per bin counts the offsets of the
-//     bin's slices with LDS atomics, adds them to the table (the bin's cells belong to this
-//     workgroup alone) and writes the pass-2 threshold bits of those cells.
+//     bin's slices with LDS atomics, adds them to the table (the bin's cells
+//     belong to this workgroup alone) and writes the pass-2 threshold bits of
+//     those cells.
 // Every update is +1, so the final counters equal the direct-atomic ones.
 #include "fb_device.hpp"
 #include "fb_internal.hpp"
@@ -24,9 +26,12 @@ using namespace fbk;
 
 constexpr int kScatterBlock = (int)kP1BlockSize;
 constexpr int kCountBlock = 1024;
-constexpr int kRound = 7;                                     // windows per thread per staging round
+constexpr int kRound = 6;                                // windows per thread per staging round
 constexpr uint32_t kStage = 4u * kRound * kP1BlockSize;  // staged cells per round
+constexpr uint32_t kChunk = kP1Chunk;                    // offsets per reserved chunk (one line)
+constexpr uint16_t kNoCell = 0xFFFFu;                    // unused slot (bins hold at most 2^15 cells)
 static_assert(kSlices == kBinSlices, "slice count of device and host code");
+static_assert(kBinLog2Cells <= 15, "kNoCell must not be a cell offset");
 
 __device__ __forceinline__ bool share_of(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
                                          uint32_t& w0, uint32_t& w1) {
@@ -41,59 +46,24 @@ __device__ __forceinline__ bool share_of(const DevReads& R, uint64_t gid, uint32
   return w0 < w1;
 }
 
-// Calls f(cell) for the 4 cells of every window of this thread's share and,
-// when HLL, updates the HLL registers.
-template <int L, bool HLL, class F>
-__device__ __forceinline__ void walk_cells(const BinArgs& a, uint64_t r, uint32_t w0, uint32_t w1, F&& f) {
-  const uint32_t k = a.k;
-  const uint64_t* __restrict__ data = a.reads.data;
-  const uint64_t g = a.reads.offs[r] + w0;
-  Kmer<L> win = window_bits<L>(data, g, k);
-  Kmer<L> fwd = fwd_from_window<L>(win, k);
-  Kmer<L> rev = rc_from_window<L>(win, k);
-  BaseStream in;
-  in.init(data, g + k);
-  NtHash nh;
-  BaseStream outs;
-  if (HLL) {
-    nh.f = 0;
-    nh.r = 0;
-    BaseStream s0;
-    s0.init(data, g);
-    for (uint32_t j = 0; j < k; ++j) nh.push(s0.next(), k);
-    outs.init(data, g);
-  }
-  const uint32_t lw = a.log2w;
-  const uint64_t wmask = (1ull << lw) - 1ull;
-  for (uint32_t w = w0;;) {
-    const uint64_t key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
-#pragma unroll
-    for (int row = 0; row < 4; ++row) f(((uint64_t)row << lw) | (splitmix64(key ^ cms_seed(row)) & wmask));
-    if (HLL) {
-      uint32_t idx, rho;
-      hll_index_rho(splitmix64(nh.canonical()), a.hll.p, idx, rho);
-      if (rho > a.hll.regs[idx]) atomicMax(a.hll.regs + idx, rho);
-    }
-    if (++w >= w1) break;
-    const uint32_t b = in.next();
-    kmer_roll<L>(fwd, rev, b, k);
-    if (HLL) nh.roll(outs.next(), b, k);
-  }
-}
-
-// The windows of a thread's shares gid0, gid0 + blockDim, ... (chunks of
-// them), one share at a time.
-template <int L>
+// The windows of a thread's shares gid0, gid0 + step, ... (count of them),
+// one share at a time.  With HLL every window also feeds the HLL
+// registers (add_hash of splitmix64(canonical ntHash)).
+template <int L, bool HLL>
 struct CellWalker {
   uint64_t gid;     // next share to open
+  uint64_t step;    // distance between the thread's shares
   uint32_t left;    // shares not yet opened
   uint32_t w, w1;   // next window of the open share and its end; w == w1: none open
   Kmer<L> fwd, rev;
   BaseStream in;
+  NtHash nh;        // HLL only
+  BaseStream outs;  // HLL only: the base leaving the ntHash window
 
-  __device__ __forceinline__ void init(uint64_t gid0, uint32_t chunks) {
+  __device__ __forceinline__ void init(uint64_t gid0, uint64_t stride, uint32_t count) {
     gid = gid0;
-    left = chunks;
+    step = stride;
+    left = count;
     w = w1 = 0;
   }
   // Opens the next share with windows, if any.
@@ -102,7 +72,7 @@ struct CellWalker {
       uint64_t r = 0;
       uint32_t w0 = 0, e = 0;
       const bool ok = share_of(a.reads, gid, a.split, a.k, r, w0, e);
-      gid += blockDim.x;
+      gid += step;
       --left;
       if (ok) {
         const uint64_t g = a.reads.offs[r] + w0;
@@ -110,6 +80,13 @@ struct CellWalker {
         fwd = fwd_from_window<L>(win, a.k);
         rev = rc_from_window<L>(win, a.k);
         in.init(a.reads.data, g + a.k);
+        if (HLL) {
+          nh.f = 0;
+          nh.r = 0;
+          outs.init(a.reads.data, g);
+          BaseStream s0 = outs;
+          for (uint32_t j = 0; j < a.k; ++j) nh.push(s0.next(), a.k);
+        }
         w = w0;
         w1 = e;
         return;
@@ -120,7 +97,16 @@ struct CellWalker {
   // Canonical fold of the next window of the open share (has() must hold).
   __device__ __forceinline__ uint64_t take(const BinArgs& a) {
     const uint64_t key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
-    if (++w < w1) kmer_roll<L>(fwd, rev, in.next(), a.k);
+    if (HLL) {
+      uint32_t idx, rho;
+      hll_index_rho(splitmix64(nh.canonical()), a.hll.p, idx, rho);
+      if (rho > a.hll.regs[idx]) atomicMax(a.hll.regs + idx, rho);
+    }
+    if (++w < w1) {
+      const uint32_t b = in.next();
+      kmer_roll<L>(fwd, rev, b, a.k);
+      if (HLL) nh.roll(outs.next(), b, a.k);
+    }
     return key;
   }
 };
@@ -129,13 +115,34 @@ __device__ __forceinline__ uint64_t cell_of(const BinArgs& a, uint64_t key, int 
   return ((uint64_t)row << a.log2w) | (splitmix64(key ^ cms_seed(row)) & ((1ull << a.log2w) - 1ull));
 }
 
-// Exclusive scan of v[0..n) in place; one block.
-__device__ __forceinline__ void block_exclusive_scan(uint32_t* v, uint32_t n, uint32_t* s_wave) {
+// A bin's current chunk in one LDS word: next free entry (region position,
+// clamped to kPosMax, which is past every region's capacity) << 6 | entries
+// left in the chunk (0..63).
+constexpr uint32_t kPosMax = (1u << 26) - 1u;
+static_assert(kChunk == 64, "entries left in a chunk take 6 bits");
+__device__ __forceinline__ uint32_t chunk_word(uint64_t pos, uint32_t left) {
+  return ((pos < kPosMax ? (uint32_t)pos : kPosMax) << 6) | left;
+}
+
+// Exclusive scan of the round counts v[0..n) in place (one block).  Before
+// the scan each thread reserves, for the bins it scans, the chunks a count
+// needs beyond what is left of the bin's current chunk: nbase[b] is the
+// first entry of the new chunks.
+__device__ __forceinline__ void scan_and_reserve(uint32_t* v, uint32_t n, uint32_t* s_wave, const uint32_t* cur,
+                                                 uint32_t* nbase, uint32_t* __restrict__ cursor, uint32_t sl) {
   const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
   const uint32_t lo = threadIdx.x * per < n ? threadIdx.x * per : n;
   const uint32_t hi = lo + per < n ? lo + per : n;
   uint32_t sum = 0;
-  for (uint32_t i = lo; i < hi; ++i) sum += v[i];
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t need = v[i];
+    const uint32_t lft = cur[i] & 63u;
+    sum += need;
+    if (need > lft) {
+      const uint32_t more = (need - lft + kChunk - 1) / kChunk * kChunk;
+      nbase[i] = atomicAdd(cursor + (uint64_t)i * kSlices + sl, more);
+    }
+  }
   uint32_t incl = sum;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -153,39 +160,41 @@ __device__ __forceinline__ void block_exclusive_scan(uint32_t* v, uint32_t n, ui
   }
 }
 
-// Block blk covers thread shares [blk * chunks * blockDim, (blk + 1) * chunks * blockDim).
-// LDS: gcur[nbins] (the block's next position per bin), rcur[nbins] (round
-// counts, then run ends), stage[kStage].
+// Block blk covers thread shares [blk * chunks * blockDim, (blk + 1) * chunks * blockDim),
+// or, with a.grid, every a.grid-th block-sized group of shares from blk on.
+// LDS: rcur[nbins] (round counts, then run ends), cur[nbins] (the bin's
+// current chunk, chunk_word), nbase[nbins] (first entry of the round's new
+// chunks), stage[kStage].
 template <int L, bool HLL>
 __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
   __shared__ uint32_t s_wave[kScatterBlock / 64];
   const uint32_t nb = a.nbins;
-  uint32_t* gcur = sh;
-  uint32_t* rcur = sh + nb;
-  uint32_t* stage = sh + 2 * nb;
-  for (uint32_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) sh[i] = 0;
-  __syncthreads();
-  const uint64_t gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
+  uint32_t* rcur = sh;
+  uint32_t* cur = sh + nb;
+  uint32_t* nbase = sh + 2 * nb;
+  uint32_t* stage = sh + 3 * nb;
+  for (uint32_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) sh[i] = 0;  // rcur, cur (no chunk yet)
+  uint64_t gid0, stride;
+  uint32_t count;
+  if (a.grid) {
+    const uint64_t total = a.reads.n_reads * (uint64_t)a.split;
+    gid0 = blk * blockDim.x + threadIdx.x;
+    stride = (uint64_t)a.grid * blockDim.x;
+    count = gid0 < total ? (uint32_t)((total - 1 - gid0) / stride + 1) : 0u;
+  } else {
+    gid0 = blk * a.chunks * blockDim.x + threadIdx.x;
+    stride = blockDim.x;
+    count = a.chunks;
+  }
   const uint32_t sb = a.sb;
-  for (uint32_t ch = 0; ch < a.chunks; ++ch) {
-    uint64_t r = 0;
-    uint32_t w0 = 0, w1 = 0;
-    if (share_of(a.reads, gid0 + (uint64_t)ch * blockDim.x, a.split, a.k, r, w0, w1))
-      walk_cells<L, HLL>(a, r, w0, w1, [&](uint64_t cell) { atomicAdd(gcur + (cell >> sb), 1u); });
-  }
-  __syncthreads();
   const uint32_t sl = xcc_slice();
-  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
-    const uint32_t c = gcur[b];
-    gcur[b] = c ? atomicAdd(a.cursor + (uint64_t)b * kSlices + sl, c) : 0u;
-  }
-  // (the first round's barrier orders these writes before their use)
   const uint64_t cap = a.cap;
   const uint32_t smask = (1u << sb) - 1u;
   uint16_t* __restrict__ out = a.data + sl * cap;
   uint32_t* __restrict__ counts = a.counts;
-  CellWalker<L> wk;
-  wk.init(gid0, a.chunks);
+  CellWalker<L, HLL> wk;
+  wk.init(gid0, stride, count);
+  __syncthreads();
   for (;;) {
     // a round takes up to kRound windows of one share; its cells stay in registers
     if (!wk.has()) wk.open(a);
@@ -204,7 +213,7 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
       }
     }
     if (!__syncthreads_or(n)) break;
-    block_exclusive_scan(rcur, nb, s_wave);
+    scan_and_reserve(rcur, nb, s_wave, cur, nbase, a.cursor, sl);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kRound; ++j) {
@@ -217,31 +226,53 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
       }
     }
     __syncthreads();
-    // rcur[b] is now the end of bin b's run in the stage
+    // rcur[b] is now the end of bin b's run in the stage; rank q of the run
+    // goes to the current chunk while it lasts, then to the new chunks
     const uint32_t total = rcur[nb - 1];
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
       const uint32_t cell = stage[i];
       const uint32_t b = cell >> sb;
-      const uint32_t pos = gcur[b] + i - (b ? rcur[b - 1] : 0u);
+      const uint32_t q = i - (b ? rcur[b - 1] : 0u);
+      const uint32_t cw = cur[b], lft = cw & 63u;
+      const uint64_t pos = q < lft ? (uint64_t)(cw >> 6) + q : (uint64_t)nbase[b] + (q - lft);
       if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = (uint16_t)(cell & smask);
       else atomicAdd(counts + cell, 1u);
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) gcur[b] += rcur[b] - (b ? rcur[b - 1] : 0u);
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+      const uint32_t need = rcur[b] - (b ? rcur[b - 1] : 0u);
+      const uint32_t cw = cur[b], lft = cw & 63u;
+      if (need <= lft) {
+        cur[b] = chunk_word((uint64_t)(cw >> 6) + need, lft - need);
+      } else {
+        const uint32_t rest = need - lft;
+        const uint32_t more = (rest + kChunk - 1) / kChunk * kChunk;
+        cur[b] = chunk_word((uint64_t)nbase[b] + rest, more - rest);
+      }
+    }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) rcur[b] = 0;
     __syncthreads();
   }
+  // mark what is left of each bin's last chunk (a wave per bin)
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t b = wave; b < nb; b += blockDim.x >> 6) {
+    const uint32_t cw = cur[b], lft = cw & 63u;
+    for (uint32_t j = lane; j < lft; j += 64u) {
+      const uint64_t pos = (uint64_t)(cw >> 6) + j;
+      if (pos < cap) out[(uint64_t)b * (kSlices * cap) + pos] = kNoCell;
+    }
+  }
 }
 
 template <int L, bool HLL>
-__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinArgs a) {
+__global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter(BinArgs a) {
   extern __shared__ uint32_t sh[];
   bin_scatter_body<L, HLL>(a, blockIdx.x, sh);
 }
 
 template <int L, bool HLL>
-__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter_m(const BinArgs* __restrict__ args,
+__global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter_m(const BinArgs* __restrict__ args,
                                                                  const uint32_t* __restrict__ blk, uint32_t n_iso) {
   extern __shared__ uint32_t sh[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
@@ -261,12 +292,13 @@ __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uin
     const uint32_t n4 = n & ~3u;
     for (uint32_t i = threadIdx.x * 4u; i < n4; i += blockDim.x * 4u) {
       const uint2 v = *reinterpret_cast<const uint2*>(d + i);
-      atomicAdd(lds + (v.x & 0xFFFFu), 1u);
-      atomicAdd(lds + (v.x >> 16), 1u);
-      atomicAdd(lds + (v.y & 0xFFFFu), 1u);
-      atomicAdd(lds + (v.y >> 16), 1u);
+      const uint32_t e[4] = {v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (e[j] != kNoCell) atomicAdd(lds + e[j], 1u);
     }
-    for (uint32_t i = n4 + threadIdx.x; i < n; i += blockDim.x) atomicAdd(lds + d[i], 1u);
+    for (uint32_t i = n4 + threadIdx.x; i < n; i += blockDim.x)
+      if (d[i] != kNoCell) atomicAdd(lds + d[i], 1u);
   }
   __syncthreads();
   // add into the table and emit the threshold bits, 64 cells per wave step
@@ -303,7 +335,7 @@ __global__ __launch_bounds__(kCountBlock) void k_bin_count_m(const BinArgs* __re
   bin_count_body(args[iso], blockIdx.x - blk[iso], lds);
 }
 
-size_t bin_scatter_lds(uint32_t nbins) { return ((size_t)nbins * 2 + kStage) * sizeof(uint32_t); }
+size_t bin_scatter_lds(uint32_t nbins) { return ((size_t)nbins * 3 + kStage) * sizeof(uint32_t); }
 
 // Dynamic LDS beyond the 64 KB default for the scatter (bins + stage) and
 // count (2^sb counters) kernels; the limit less 1 KB of static LDS.
@@ -344,12 +376,34 @@ hipError_t check_bin_pass1(uint32_t nbins, uint32_t sb) {
   return hipSuccess;
 }
 
+uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
+  if (pass1_lds_attrs() != hipSuccess) return 1;
+  const uint32_t L = (2 * k + 63) / 64;
+  const void* f = nullptr;
+#define BR(LL) f = hll ? reinterpret_cast<const void*>(&k_bin_scatter<LL, true>) \
+                       : reinterpret_cast<const void*>(&k_bin_scatter<LL, false>)
+  switch (L) {
+    case 1: BR(1); break;
+    case 2: BR(2); break;
+    case 3: BR(3); break;
+    default: BR(4); break;
+  }
+#undef BR
+  int per = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, kScatterBlock, bin_scatter_lds(nbins)) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1;
+  return (uint32_t)(per > 0 ? per : 1) * (uint32_t)(cus > 0 ? cus : 1);
+}
+
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
   if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
   if (a.reads.n_reads) {
     const uint32_t L = (2 * a.k + 63) / 64;
-    const unsigned grid = (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks, kP1BlockSize);
+    const unsigned grid =
+        a.grid ? a.grid : (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks, kP1BlockSize);
     const size_t lds = bin_scatter_lds(a.nbins);
     const bool h = a.hll.regs != nullptr;
 #define BS(LL)                                                                                              \

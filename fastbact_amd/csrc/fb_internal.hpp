@@ -84,6 +84,7 @@ struct BinArgs {
   uint32_t k = 0;
   uint32_t split = 1;
   uint32_t chunks = 1;         // thread shares per scatter-block thread (blocks = threads / (block * chunks))
+  uint32_t grid = 0;           // nonzero: a grid of this many blocks strides over all thread shares
   uint32_t log2w = 0;
   uint64_t n_cells = 0;        // 4 * width
   uint32_t* counts = nullptr;  // exact u32 cells
@@ -98,17 +99,27 @@ struct BinArgs {
   uint32_t cap_value = 65535;  // counter saturation (15 or 65535)
 };
 size_t bin_scatter_lds(uint32_t nbins);
+// Pass-1 scatter blocks resident on the whole device at once (the grid of
+// the striding form).
+uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins);
 // mid (optional) is recorded between the scatter and count kernels.
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid = nullptr);
 constexpr uint32_t kScatterBlockSize = 512;  // block size of the pass-2 count and scatter kernels
 constexpr uint32_t kP1BlockSize = 512;       // block size of the pass-1 bin scatter kernel
 constexpr uint32_t kBinLog2Cells = 15;       // pass-1 bin: 2^15 Count-Min cells (128 KB of LDS counters)
+constexpr uint32_t kP1Chunk = 64;            // pass-1 offsets per reserved chunk: one 128-byte line
 constexpr uint32_t kP2GateBlockSize = 256;   // block size of the pass-2 gate kernel
 // Thread shares per scatter-block thread: enough blocks to fill the chip a few
 // times over, each flushing its bin histogram once for more work.
 inline uint32_t scatter_chunks(uint64_t threads, uint32_t block = kScatterBlockSize) {
   const uint64_t c = threads / ((uint64_t)block * 2048u);
   return c < 1 ? 1u : (c > 16 ? 16u : (uint32_t)c);
+}
+// Pass 1: about 1024 workgroups (two rounds of two per CU), since each
+// leaves a partly used line per bin behind.
+inline uint32_t p1_chunks(uint64_t threads) {
+  const uint64_t c = threads / ((uint64_t)kP1BlockSize * 1024u);
+  return c < 1 ? 1u : (c > 64 ? 64u : (uint32_t)c);
 }
 inline uint64_t scatter_blocks(uint64_t threads, uint32_t chunks, uint32_t block = kScatterBlockSize) {
   const uint64_t per = (uint64_t)block * chunks;

@@ -667,11 +667,17 @@ static uint32_t key_words_for_k(uint32_t k) { return (k + 30) / 31; }
 // for two more workgroups' shares (few workgroups split unevenly over the
 // XCDs).  A full region spills (pass 1) or falls back (pass 2), so the size
 // only matters for speed.
-static uint64_t bin_slice_cap(double per_bin, uint64_t n_wg, uint64_t align) {
+// Pass-1 regions are filled in line-sized chunks, each workgroup leaving at
+// most one partly used chunk per bin: add one chunk per workgroup of the
+// slice, and keep regions a whole number of chunks (lines).
+static uint64_t bin_slice_cap(double per_bin, uint64_t n_wg) {
   const double wgs = (double)std::max<uint64_t>(n_wg, 1);
-  const uint64_t c = (uint64_t)(per_bin * (1.1 / kBinSlices + 2.0 / wgs)) + 64;
-  return (c + align - 1) / align * align;
+  const uint64_t c = (uint64_t)(per_bin * (1.1 / kBinSlices + 2.0 / wgs)) + 64 +
+                     (n_wg / kBinSlices + 2) * kP1Chunk;
+  return (c + kP1Chunk - 1) / kP1Chunk * kP1Chunk;
 }
+// Region positions are kept in 26 bits by the scatter kernel.
+constexpr uint64_t kP1MaxCap = (1ull << 26) - 128;
 static uint32_t table_stride(uint32_t kw) { return kw + 5; }
 
 extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
@@ -744,9 +750,10 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   BinArgs ba;
   if (binned) {
     const uint64_t thr1 = bank->n_reads * (uint64_t)split;
-    const uint64_t n_wg = scatter_blocks(thr1, prm->chunks1 ? prm->chunks1 : scatter_chunks(thr1, kP1BlockSize),
-                                         kP1BlockSize);
-    ba.cap = bin_slice_cap(4.0 * (double)windows / (double)nbins, n_wg, 4);
+    // without an explicit chunk count the scatter grid is the resident blocks, striding over the shares
+    ba.grid = prm->chunks1 ? 0u : bin_scatter_resident(k, so->has_hll, (uint32_t)nbins);
+    const uint64_t n_wg = ba.grid ? ba.grid : scatter_blocks(thr1, prm->chunks1, kP1BlockSize);
+    ba.cap = std::min<uint64_t>(bin_slice_cap(4.0 * (double)windows / (double)nbins, n_wg), kP1MaxCap);
     ba.sb = sb;
     ba.nbins = (uint32_t)nbins;
     if (int r = bdata.get(c, nbins * kBinSlices * ba.cap * 2)) return fail(r);
@@ -767,7 +774,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     ba.reads = bank->view();
     ba.k = k;
     ba.split = split;
-    ba.chunks = prm->chunks1 ? prm->chunks1 : scatter_chunks(bank->n_reads * (uint64_t)split, kP1BlockSize);
+    ba.chunks = prm->chunks1 ? prm->chunks1 : 1u;
     ba.log2w = so->cms.log2w;
     ba.n_cells = ncnt;
     ba.counts = so->cms.counts.as<uint32_t>();
@@ -1333,7 +1340,10 @@ extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uin
   const uint32_t rw = (2 * k + 8 + 63) / 64;
   uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;            // exact counts + gate bits
   b += p->hll_p ? (4ull << p->hll_p) : 0;                     // HLL registers
-  const uint64_t bins = (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536;
+  const uint64_t thr1 = reads * (uint64_t)default_split(k, max_len);
+  const uint64_t n_wg1 = scatter_blocks(thr1, p->chunks1 ? p->chunks1 : p1_chunks(thr1), kP1BlockSize);
+  const uint64_t nbins1 = std::max<uint64_t>(1, (W * 4) >> kBinLog2Cells);
+  const uint64_t bins = (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536 + nbins1 * (n_wg1 + 16) * kP1Chunk * 2;
   b += std::max<uint64_t>(bins, q.rec_cap * rw * 8);          // pass-1 bins, then pass-2 records
   b += q.threads2 * 8 + q.n_blocks2 * q.nbr * 4;              // gate masks, run matrix
   b += q.spill_cap * table_stride(key_words_for_k(k)) * 8;    // spill table
@@ -1509,10 +1519,11 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
     it.sb = sb;
     it.nbins = (uint32_t)nbins;
     it.binned = binned;
-    it.chunks1 = prm.chunks1 ? prm.chunks1 : scatter_chunks(it.threads1, kP1BlockSize);
+    it.chunks1 = prm.chunks1 ? prm.chunks1 : p1_chunks(it.threads1);
     if (binned) {
       const uint64_t n_wg = scatter_blocks(it.threads1, it.chunks1, kP1BlockSize);
-      const uint64_t cap = bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg, 4);
+      const uint64_t cap =
+          std::min<uint64_t>(bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg), kP1MaxCap);
       const uint64_t need = nbins * kBinSlices * cap * 2;
       if (need > it.scratch.bytes || !it.scratch.p)
         if (int r = it.scratch.get(c, need)) return r;
@@ -2431,7 +2442,7 @@ extern "C" int fb_batch_autotune(fb_batch* b, uint32_t k, int32_t target, fb_tun
   if (!p.split)
     if (int r = sweep(&TuneChoice::split, {1u, 2u, 3u, 4u, 6u})) return r;
   if (!p.chunks1 && best.placement != 1)
-    if (int r = sweep(&TuneChoice::chunks1, {1u, 2u, 4u, 8u, 16u})) return r;
+    if (int r = sweep(&TuneChoice::chunks1, {2u, 4u, 8u, 16u, 32u, 64u})) return r;
   if (!p.chunks2 && best.placement != 1)
     if (int r = sweep(&TuneChoice::chunks2, {1u, 2u, 4u, 8u, 16u})) return r;
   best.ms = best_ms;

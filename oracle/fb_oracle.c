@@ -1726,3 +1726,204 @@ int orc_filter_active_reads(const orc_graph_view* g, const orc_arena arenas[3], 
     free(b2);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Ingest, trim_merge.rs:17-417                                               */
+static uint8_t up_ascii(uint8_t c) { return (c >= 'a' && c <= 'z') ? (uint8_t)(c - 32) : c; }
+static int eq_nocase(uint8_t a, uint8_t b) { return up_ascii(a) == up_ascii(b); }
+static const uint8_t AD_TRUSEQ[] = "AGATCGGAAGAGC";
+static const uint8_t AD_NEXTERA[] = "AGATGTGTATAAGAGACAG";
+static const uint8_t AD_NEXTERA_RC[] = "CTGTCTCTTATACACATCT";
+
+/* at most one mismatch over the first 10 bases of the adapter */
+static int adapter_at(const uint8_t* w, const uint8_t* ad) {
+    int mm = 0;
+    for (int j = 0; j < 10; j++)
+        if (!eq_nocase(w[j], ad[j]) && ++mm > 1) return 0;
+    return 1;
+}
+/* trim_merge.rs:17-75 */
+uint32_t orc_trim_adapter(const uint8_t* seq, uint32_t len) {
+    if (len < 10) return len;
+    uint32_t cut = len;
+    const uint32_t start = len > 100 ? len - 100 : 0;
+    for (uint32_t i = start; i + 10 <= len; i++) {
+        const uint8_t b0 = up_ascii(seq[i]), b1 = up_ascii(seq[i + 1]);
+        const int agat = (b0 == 'A' && b1 == 'G') || b0 == 'A' || b1 == 'G';
+        const int ctgt = (b0 == 'C' && b1 == 'T') || b0 == 'C' || b1 == 'T';
+        if (!agat && !ctgt) continue;
+        if (agat && (adapter_at(seq + i, AD_TRUSEQ) || adapter_at(seq + i, AD_NEXTERA)))
+            if (i < cut) cut = i;
+        if (ctgt && cut > i && adapter_at(seq + i, AD_NEXTERA_RC)) cut = i;
+    }
+    return cut;
+}
+/* trim_merge.rs:77-113 */
+int64_t orc_trim_quality(const uint8_t* qual, uint32_t len, uint32_t qual_len, uint8_t min_qual, uint32_t min_len) {
+    if (len < min_len || qual_len != len) return -1;
+    uint32_t cut = len;
+    if (qual_len >= 4) {
+        uint64_t sum = 0;
+        for (uint32_t j = cut - 4; j < cut; j++) sum += qual[j] > 33 ? qual[j] - 33 : 0;
+        if (sum >= (uint64_t)min_qual * 4) return len;
+        while (cut >= 4) {
+            sum = 0;
+            for (uint32_t j = cut - 4; j < cut; j++) sum += qual[j] > 33 ? qual[j] - 33 : 0;
+            if (sum / 4 >= min_qual) break;
+            cut--;
+        }
+    }
+    return cut < min_len ? -1 : (int64_t)cut;
+}
+static uint8_t rc_ascii(uint8_t c) {
+    switch (c) {
+        case 'A': case 'a': return 'T';
+        case 'C': case 'c': return 'G';
+        case 'G': case 'g': return 'C';
+        case 'T': case 't': return 'A';
+        default: return 'A';
+    }
+}
+/* trim_merge.rs:127-231 */
+uint32_t orc_merge_read_pair(const uint8_t* r1, const uint8_t* q1, uint32_t l1, uint32_t ql1, const uint8_t* r2,
+                             const uint8_t* q2, uint32_t l2, uint32_t ql2, uint32_t min_overlap,
+                             double max_mismatch_ratio, uint8_t* out, uint32_t* mismatches) {
+    if (l1 < min_overlap || l2 < min_overlap) return 0;
+    uint8_t* rc2 = (uint8_t*)malloc(l2 ? l2 : 1);
+    for (uint32_t i = 0; i < l2; i++) rc2[i] = rc_ascii(r2[l2 - 1 - i]);
+    const uint32_t max_d = l1 < l2 ? l1 : l2;
+    uint32_t best_d = 0, best_m = 0;
+    double best_score = -INFINITY;
+    for (uint32_t d = max_d; d >= min_overlap && d > 0; d--) {
+        const uint8_t* s1 = r1 + (l1 - d);
+        const uint64_t max_allowed = (uint64_t)((double)d * max_mismatch_ratio);
+        uint64_t mm = 0;
+        for (uint32_t i = 0; i < d; i++)
+            if (!eq_nocase(s1[i], rc2[i]) && ++mm > max_allowed) break;
+        if (mm <= max_allowed) {
+            const double score = (double)d - 2.5 * (double)mm;
+            if (score > best_score) {
+                best_score = score;
+                best_d = d;
+                best_m = (uint32_t)mm;
+            }
+        }
+        if ((double)d <= best_score) break;
+    }
+    if (best_d < min_overlap || best_d == 0) {
+        free(rc2);
+        return 0;
+    }
+    uint32_t o = 0;
+    for (uint32_t i = 0; i < l1 - best_d; i++) out[o++] = r1[i];
+    for (uint32_t i = 0; i < best_d; i++) {
+        const uint8_t b1 = r1[l1 - best_d + i], b2 = rc2[i];
+        const uint32_t i1 = l1 - best_d + i;
+        const uint8_t qa = i1 < ql1 ? q1[i1] : 'I';
+        const uint8_t qb = (i < l2 && l2 - 1 - i < ql2) ? q2[l2 - 1 - i] : 'I';
+        if (eq_nocase(b1, b2) || qa >= qb) out[o++] = up_ascii(b1);
+        else out[o++] = up_ascii(b2);
+    }
+    for (uint32_t i = best_d; i < l2; i++) out[o++] = rc2[i];
+    if (mismatches) *mismatches = best_m;
+    free(rc2);
+    return best_d;
+}
+
+typedef struct { uint8_t* p; uint64_t n, cap; } bytes_t;
+typedef struct { uint32_t* p; uint64_t n, cap; } u32s_t;
+static void bytes_push(bytes_t* b, const uint8_t* s, uint64_t n) {
+    if (b->n + n > b->cap) {
+        b->cap = (b->n + n) * 2 + 1024;
+        b->p = (uint8_t*)realloc(b->p, b->cap);
+    }
+    memcpy(b->p + b->n, s, n);
+    b->n += n;
+}
+static void u32_push(u32s_t* v, uint32_t x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap * 2 + 1024;
+        v->p = (uint32_t*)realloc(v->p, v->cap * 4);
+    }
+    v->p[v->n++] = x;
+}
+/* trim_merge.rs:299-405 */
+int orc_ingest_records(const orc_records* r1, const orc_records* r2, const orc_ingest_params* p, orc_ingest_out* out) {
+    memset(out, 0, sizeof *out);
+    if (r1->n != r2->n) return -1;
+    bytes_t seq[3] = {{0}};
+    u32s_t len[3] = {{0}};
+    const int subsample = p->subsample_factor < 0.999;
+    const double thr_f = p->subsample_factor * 18446744073709551616.0;
+    const uint64_t keep_thr = thr_f <= 0.0 ? 0 : (thr_f >= 18446744073709551616.0 ? UINT64_MAX : (uint64_t)thr_f);
+    uint8_t* mbuf = NULL;
+    uint64_t mcap = 0;
+    int rc = 0;
+    for (uint64_t i = 0; i < r1->n; i++) {
+        const uint64_t pair_idx = i + 1;
+        const uint8_t* s1 = r1->seq + r1->seq_off[i];
+        const uint8_t* s2 = r2->seq + r2->seq_off[i];
+        const uint8_t* q1 = r1->qual + r1->qual_off[i];
+        const uint8_t* q2 = r2->qual + r2->qual_off[i];
+        uint32_t l1 = r1->seq_len[i], l2 = r2->seq_len[i];
+        uint32_t ql1 = r1->qual_len[i], ql2 = r2->qual_len[i];
+        out->stats.input_read_pairs++;
+        out->stats.bases_before_trim += (uint64_t)l1 + l2;
+        if (subsample && orc_splitmix64(pair_idx ^ p->seed) > keep_thr) continue;
+        if (p->enable_trim) {
+            const uint32_t a1 = orc_trim_adapter(s1, l1), a2 = orc_trim_adapter(s2, l2);
+            if (ql1 < a1 || ql2 < a2) {
+                rc = -3;
+                break;
+            }
+            const int64_t t1 = orc_trim_quality(q1, a1, a1, p->min_qual, p->min_len);
+            const int64_t t2 = orc_trim_quality(q2, a2, a2, p->min_qual, p->min_len);
+            if (t1 < 0 || t2 < 0) continue;
+            l1 = (uint32_t)t1;
+            l2 = (uint32_t)t2;
+            ql1 = l1;
+            ql2 = l2;
+        } else if (l1 < p->min_len || l2 < p->min_len) {
+            continue;
+        }
+        out->stats.retained_read_pairs++;
+        out->stats.bases_after_trim += (uint64_t)l1 + l2;
+        uint32_t d = 0;
+        if (p->enable_merge) {
+            if ((uint64_t)l1 + l2 > mcap) {
+                mcap = ((uint64_t)l1 + l2) * 2;
+                mbuf = (uint8_t*)realloc(mbuf, mcap);
+            }
+            d = orc_merge_read_pair(s1, q1, l1, ql1, s2, q2, l2, ql2, 15, 0.10, mbuf, NULL);
+        }
+        if (d) {
+            const uint32_t ml = l1 + l2 - d;
+            out->stats.merged_pairs_count++;
+            out->stats.merged_bases += ml;
+            bytes_push(&seq[0], mbuf, ml);
+            u32_push(&len[0], ml);
+        } else {
+            out->stats.unmerged_pairs_count++;
+            bytes_push(&seq[1], s1, l1);
+            u32_push(&len[1], l1);
+            bytes_push(&seq[2], s2, l2);
+            u32_push(&len[2], l2);
+        }
+    }
+    free(mbuf);
+    for (int a = 0; a < 3; a++) {
+        out->seq[a] = seq[a].p;
+        out->len[a] = len[a].p;
+        out->n[a] = len[a].n;
+    }
+    if (rc) return rc;
+    if (out->n[0] + out->n[1] + out->n[2] == 0) return -2;
+    return 0;
+}
+void orc_ingest_free(orc_ingest_out* out) {
+    for (int a = 0; a < 3; a++) {
+        free(out->seq[a]);
+        free(out->len[a]);
+    }
+    memset(out, 0, sizeof *out);
+}

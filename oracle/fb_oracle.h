@@ -218,6 +218,54 @@ typedef struct {
 int orc_filter_active_reads(const orc_graph_view* g, const orc_arena arenas[3], uint8_t* keep_merged,
                             uint8_t* keep_pairs);
 
+/* ---- Ingest: trimming, merging, subsampling (src/ingest/trim_merge.rs:17-417)
+ * Records are given as parsed FASTQ/FASTA: seq and qual bytes (qual_len 0
+ * for FASTA). */
+/* trim_adapter (trim_merge.rs:17-75): the kept length of seq. */
+uint32_t orc_trim_adapter(const uint8_t* seq, uint32_t len);
+/* trim_quality (trim_merge.rs:77-113): the kept length, or -1 for None. */
+int64_t orc_trim_quality(const uint8_t* qual, uint32_t len, uint32_t qual_len, uint8_t min_qual, uint32_t min_len);
+/* merge_read_pair (trim_merge.rs:127-231): the overlap of the merge, or 0 when
+ * the pair stays unmerged; out receives l1 + l2 - overlap bytes. */
+uint32_t orc_merge_read_pair(const uint8_t* r1, const uint8_t* q1, uint32_t l1, uint32_t ql1, const uint8_t* r2,
+                             const uint8_t* q2, uint32_t l2, uint32_t ql2, uint32_t min_overlap,
+                             double max_mismatch_ratio, uint8_t* out, uint32_t* mismatches);
+typedef struct {
+    const uint8_t* seq;
+    const uint64_t* seq_off;
+    const uint32_t* seq_len;
+    const uint8_t* qual;
+    const uint64_t* qual_off;
+    const uint32_t* qual_len;
+    uint64_t n;
+} orc_records;
+typedef struct {
+    double subsample_factor;
+    uint32_t min_len;
+    uint8_t min_qual;
+    int enable_trim;
+    int enable_merge;
+    uint64_t seed;
+} orc_ingest_params;
+/* IngestStats (trim_merge.rs:256-265) */
+typedef struct {
+    uint64_t input_read_pairs, retained_read_pairs, bases_before_trim, bases_after_trim;
+    uint64_t merged_pairs_count, merged_bases, unmerged_pairs_count;
+} orc_ingest_stats;
+/* The three read lists of the bank as ASCII: list a holds n[a] reads of
+ * lengths len[a], concatenated in seq[a] (malloc'd; orc_ingest_free). */
+typedef struct {
+    uint8_t* seq[3];
+    uint32_t* len[3];
+    uint64_t n[3];
+    orc_ingest_stats stats;
+} orc_ingest_out;
+/* ingest_reads_ext_with_seed's record loop (trim_merge.rs:299-405).
+ * 0 ok; -1 mate counts differ; -2 no read survived; -3 a quality string is
+ * shorter than its adapter-trimmed read (the reference's slice panics). */
+int orc_ingest_records(const orc_records* r1, const orc_records* r2, const orc_ingest_params* p, orc_ingest_out* out);
+void orc_ingest_free(orc_ingest_out* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,26 @@ class GraphView(C.Structure):
                 ("flags", u8p), ("n_fwd", u32p), ("n_rev", u32p)]
 
 
+class Records(C.Structure):
+    _fields_ = [("seq", u8p), ("seq_off", u64p), ("seq_len", u32p), ("qual", u8p), ("qual_off", u64p),
+                ("qual_len", u32p), ("n", C.c_uint64)]
+
+
+class IngestParams(C.Structure):
+    _fields_ = [("subsample_factor", C.c_double), ("min_len", C.c_uint32), ("min_qual", C.c_uint8),
+                ("enable_trim", C.c_int), ("enable_merge", C.c_int), ("seed", C.c_uint64)]
+
+
+class IngestStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("input_read_pairs", "retained_read_pairs", "bases_before_trim",
+                                           "bases_after_trim", "merged_pairs_count", "merged_bases",
+                                           "unmerged_pairs_count")]
+
+
+class IngestOut(C.Structure):
+    _fields_ = [("seq", u8p * 3), ("len", u32p * 3), ("n", C.c_uint64 * 3), ("stats", IngestStats)]
+
+
 _SIGS = [
     ("orc_splitmix64", C.c_uint64, [C.c_uint64]),
     ("orc_rev_comp_word", C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -124,6 +144,13 @@ _SIGS = [
     ("orc_compact", C.c_int, [u64p, u16p, u8p, C.c_uint64, C.c_uint32, C.POINTER(GraphC)]),
     ("orc_graph_free", None, [C.POINTER(GraphC)]),
     ("orc_filter_active_reads", C.c_int, [C.POINTER(GraphView), C.POINTER(Arena), u8p, u8p]),
+    ("orc_trim_adapter", C.c_uint32, [u8p, C.c_uint32]),
+    ("orc_trim_quality", C.c_int64, [u8p, C.c_uint32, C.c_uint32, C.c_uint8, C.c_uint32]),
+    ("orc_merge_read_pair", C.c_uint32, [u8p, u8p, C.c_uint32, C.c_uint32, u8p, u8p, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, C.c_double, u8p, u32p]),
+    ("orc_ingest_records", C.c_int, [C.POINTER(Records), C.POINTER(Records), C.POINTER(IngestParams),
+                                     C.POINTER(IngestOut)]),
+    ("orc_ingest_free", None, [C.POINTER(IngestOut)]),
 ]
 
 _LIB = None
@@ -475,3 +502,83 @@ def filter_active_reads(graph: dict, k: int, arenas: list) -> dict:
     r2 = unpack(*ars[2], np.flatnonzero(kp))
     return {"keep_merged": km, "keep_pairs": kp, "arenas": [pack(merged), pack(r1), pack(r2)],
             "reads": [merged, r1, r2]}
+
+
+# ---- ingest (trim_merge.rs:17-417) -----------------------------------------
+
+def trim_adapter(seq: bytes) -> int:
+    b = _bytes(seq)
+    return int(lib().orc_trim_adapter(_p(b, C.c_uint8), b.size))
+
+
+def trim_quality(qual: bytes, min_qual: int, min_len: int, seq_len: int | None = None):
+    q = _bytes(qual)
+    n = q.size if seq_len is None else seq_len
+    r = int(lib().orc_trim_quality(_p(q, C.c_uint8), n, q.size, min_qual, min_len))
+    return None if r < 0 else r
+
+
+def merge_read_pair(r1: bytes, q1: bytes, r2: bytes, q2: bytes, min_overlap: int = 15, ratio: float = 0.10):
+    """-> (merged bytes, overlap, mismatches) or None when unmerged."""
+    a, qa, b, qb = (_bytes(x) for x in (r1, q1, r2, q2))
+    out = np.zeros(a.size + b.size + 1, np.uint8)
+    mm = np.zeros(1, np.uint32)
+    d = int(lib().orc_merge_read_pair(_p(a, C.c_uint8), _p(qa, C.c_uint8), a.size, qa.size, _p(b, C.c_uint8),
+                                      _p(qb, C.c_uint8), b.size, qb.size, min_overlap, ratio, _p(out, C.c_uint8),
+                                      _p(mm, C.c_uint32)))
+    if d == 0:
+        return None
+    return out[:a.size + b.size - d].tobytes(), d, int(mm[0])
+
+
+def records(seqs, quals=None) -> tuple:
+    """Parsed records as (seq bytes, seq offsets, seq lengths, qual bytes, qual offsets, qual lengths)."""
+    quals = quals if quals is not None else [b""] * len(seqs)
+    sl = np.array([len(x) for x in seqs], np.uint32)
+    ql = np.array([len(x) for x in quals], np.uint32)
+    so = np.zeros(len(seqs), np.uint64)
+    qo = np.zeros(len(seqs), np.uint64)
+    if len(seqs):
+        so[1:] = np.cumsum(sl[:-1], dtype=np.uint64)
+        qo[1:] = np.cumsum(ql[:-1], dtype=np.uint64)
+    return (_bytes(b"".join(bytes(x) for x in seqs)), so, sl, _bytes(b"".join(bytes(x) for x in quals)), qo, ql)
+
+
+def _records_c(rec) -> Records:
+    s, so, sl, q, qo, ql = rec
+    r = Records(_p(s, C.c_uint8), _p(so, C.c_uint64), _p(sl, C.c_uint32), _p(q, C.c_uint8), _p(qo, C.c_uint64),
+                _p(ql, C.c_uint32), sl.size)
+    r._keep = rec
+    return r
+
+
+INGEST_ERRORS = {-1: "Mismatched paired-end FASTQ mate counts between R1 and R2",
+                 -2: "No valid reads survived trimming and quality filtering",
+                 -3: "quality string shorter than its adapter-trimmed read"}
+
+
+def ingest_records(r1, r2, subsample_factor: float = 1.0, min_len: int = 50, min_qual: int = 20,
+                   enable_trim: bool = True, enable_merge: bool = True, seed: int = 0) -> dict:
+    """ingest_reads_ext_with_seed's record loop on parsed records (records()):
+    -> {"reads": [merged, pe_forward, pe_reverse] as lists of bytes, "arenas":
+    the three arenas packed as push_seq does, "stats": IngestStats fields}.
+    Raises ValueError with the reference's message on its error paths."""
+    a, b = _records_c(r1), _records_c(r2)
+    prm = IngestParams(subsample_factor, min_len, min_qual, int(enable_trim), int(enable_merge), seed)
+    out = IngestOut()
+    rc = lib().orc_ingest_records(C.byref(a), C.byref(b), C.byref(prm), C.byref(out))
+    try:
+        if rc:
+            raise ValueError(INGEST_ERRORS.get(rc, f"orc_ingest_records failed ({rc})"))
+        reads = []
+        for i in range(3):
+            n = int(out.n[i])
+            lens = np.ctypeslib.as_array(out.len[i], shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+            tot = int(lens.sum(dtype=np.uint64))
+            raw = np.ctypeslib.as_array(out.seq[i], shape=(tot,)).tobytes() if tot else b""
+            offs = np.concatenate([[0], np.cumsum(lens, dtype=np.uint64)]).astype(np.int64)
+            reads.append([raw[offs[j]:offs[j + 1]] for j in range(n)])
+        st = {f: int(getattr(out.stats, f)) for f, _ in IngestStats._fields_}
+    finally:
+        lib().orc_ingest_free(C.byref(out))
+    return {"reads": reads, "arenas": [pack(r) for r in reads], "stats": st}

@@ -470,3 +470,76 @@ def filter_active_reads(nodes: list, k: int, merged: list, pairs: list) -> tuple
         return any(nthash_canonical(r[o:o + kk]) in boundary for o in (0, (len(r) - kk) // 2, len(r) - kk))
 
     return [active(r) for r in merged], [active(a) or active(b) for a, b in pairs]
+
+
+# ---- ingest (trim_merge.rs:17-417), written on Python strings ---------------
+
+ADAPTERS_AGAT = ("AGATCGGAAGAGC", "AGATGTGTATAAGAGACAG")
+ADAPTER_CTGT = "CTGTCTCTTATACACATCT"
+
+
+def _close(window: str, adapter: str) -> bool:
+    return sum(1 for x, y in zip(window.upper(), adapter[:10]) if x != y) <= 1
+
+
+def trim_adapter(seq: str) -> int:
+    """Kept length: the first position (within the last 100, at least 10
+    bases from the end) where an adapter's first 10 bases match with at most
+    one mismatch, behind the AG / CT prefix test."""
+    n = len(seq)
+    if n < 10:
+        return n
+    for i in range(max(n - 100, 0), n - 9):
+        b0, b1 = seq[i].upper(), seq[i + 1].upper()
+        agat = b0 == "A" or b1 == "G"
+        ctgt = b0 == "C" or b1 == "T"
+        w = seq[i:i + 10]
+        if agat and any(_close(w, ad) for ad in ADAPTERS_AGAT):
+            return i
+        if ctgt and _close(w, ADAPTER_CTGT):
+            return i
+    return n
+
+
+def trim_quality(qual: str, min_qual: int, min_len: int):
+    """Kept length or None: drop the tail until a 4-base window averages at
+    least min_qual (integer mean of Phred = ord - 33, floored at 0)."""
+    q = [max(ord(c) - 33, 0) for c in qual]
+    if len(q) < min_len:
+        return None
+    cut = len(q)
+    if cut >= 4 and sum(q[-4:]) < 4 * min_qual:
+        while cut >= 4 and sum(q[cut - 4:cut]) // 4 < min_qual:
+            cut -= 1
+    return None if cut < min_len else cut
+
+
+def merge_pair(r1: str, q1: str, r2: str, q2: str, min_overlap: int = 15, ratio: float = 0.10):
+    """-> (merged, overlap, mismatches) or None.  Overlaps are tried from the
+    longest down; score = d - 2.5 m over those with m <= floor(d * ratio);
+    the search stops once d cannot beat the best score."""
+    if len(r1) < min_overlap or len(r2) < min_overlap:
+        return None
+    rc2 = "".join({"A": "T", "C": "G", "G": "C", "T": "A"}.get(c.upper(), "A") for c in reversed(r2))
+    best = None
+    for d in range(min(len(r1), len(r2)), min_overlap - 1, -1):
+        m = sum(1 for x, y in zip(r1[len(r1) - d:].upper(), rc2[:d]) if x != y)
+        if m <= int(d * ratio):
+            score = d - 2.5 * m
+            if best is None or score > best[0]:
+                best = (score, d, m)
+        if best is not None and d <= best[0]:
+            break
+    if best is None:
+        return None
+    _, d, m = best
+    out = [r1[:len(r1) - d]]
+    for i in range(d):
+        j = len(r1) - d + i
+        b1, b2 = r1[j], rc2[i]
+        qa = q1[j] if j < len(q1) else "I"
+        k2 = len(r2) - 1 - i
+        qb = q2[k2] if k2 < len(q2) else "I"
+        out.append(b1.upper() if (b1.upper() == b2 or qa >= qb) else b2)
+    out.append(rc2[d:])
+    return "".join(out), d, m

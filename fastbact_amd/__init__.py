@@ -25,8 +25,11 @@ from .api import (  # noqa: F401
     estimate_genome,
     filter_active_reads,
     hll_estimate_registers,
+    ingest_files,
+    ingest_records,
     splitmix64,
     synth_isolate,
+    read_fastx,
     synth_params,
     upgrade_multik,
 )

@@ -64,6 +64,23 @@ class MultiKResult(C.Structure):
                 ("prior_contigs", C.c_uint64), ("transferred_circular", C.c_uint64)]
 
 
+class RecordsView(C.Structure):
+    _fields_ = [("seq", C.POINTER(C.c_uint8)), ("seq_offsets", C.POINTER(C.c_uint64)),
+                ("seq_lengths", C.POINTER(C.c_uint32)), ("qual", C.POINTER(C.c_uint8)),
+                ("qual_offsets", C.POINTER(C.c_uint64)), ("qual_lengths", C.POINTER(C.c_uint32)), ("n", C.c_uint64)]
+
+
+class IngestParams(C.Structure):
+    _fields_ = [("subsample_factor", C.c_double), ("min_len", C.c_uint32), ("min_qual", C.c_uint32),
+                ("enable_trim", C.c_int32), ("enable_merge", C.c_int32), ("seed", C.c_uint64)]
+
+
+class IngestStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("input_read_pairs", "retained_read_pairs", "bases_before_trim",
+                                           "bases_after_trim", "merged_pairs_count", "merged_bases",
+                                           "unmerged_pairs_count")] + [("parse_ms", C.c_float), ("device_ms", C.c_float)]
+
+
 class BuildParams(C.Structure):
     _fields_ = [
         ("k", C.c_uint32),
@@ -259,6 +276,16 @@ SIGNATURES = [
     ("fb_graph_unitigs", C.c_int, [vp, u8p, u64p, u32p, C.POINTER(C.c_float), u8p]),
     ("fb_graph_edges", C.c_int, [vp, u32p, u32p, u8p, u16p]),
     ("fb_graph_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
+    ("fb_fastx_read", C.c_int, [C.c_char_p, C.POINTER(vp)]),
+    ("fb_fastx_last_error", C.c_char_p, []),
+    ("fb_fastx_count", C.c_uint64, [vp]),
+    ("fb_fastx_view", C.c_int, [vp, C.POINTER(RecordsView)]),
+    ("fb_fastx_destroy", None, [vp]),
+    ("fb_ingest_params_default", None, [C.POINTER(IngestParams)]),
+    ("fb_ingest_records", C.c_int, [vp, C.POINTER(RecordsView), C.POINTER(RecordsView), C.POINTER(IngestParams),
+                                    C.POINTER(vp), C.POINTER(IngestStats)]),
+    ("fb_ingest_files", C.c_int, [vp, C.c_char_p, C.c_char_p, C.POINTER(IngestParams), C.POINTER(vp),
+                                  C.POINTER(IngestStats)]),
     ("fb_graph_view_of", C.c_int, [vp, C.POINTER(GraphView)]),
     ("fb_filter_active_reads", C.c_int, [vp, vp, C.POINTER(GraphView), C.POINTER(vp), u8p, u8p]),
     ("fb_upgrade_multik", C.c_int, [vp, vp, C.POINTER(GraphView), C.c_uint32, C.c_uint64, C.POINTER(BuildParams),

@@ -576,6 +576,93 @@ def upgrade_multik(ctx: Context, bank: DeviceBank, graph: dict, k: int, k2: int,
 
 
 # ---------------------------------------------------------------------------
+# Ingest front end (trim_merge.rs:17-417)
+
+
+def read_fastx(path) -> list:
+    """open_fastx + the record reader: [(seq bytes, qual bytes), ...] (qual
+    b"" for FASTA); gzip when the name ends in ".gz"."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    rc = lib.fb_fastx_read(str(path).encode(), C.byref(h))
+    if rc:
+        raise FastBactError(rc, lib.fb_fastx_last_error().decode(errors="replace"))
+    try:
+        v = _lib.RecordsView()
+        lib.fb_fastx_view(h, C.byref(v))
+        n = int(v.n)
+        if n == 0:
+            return []
+        so = np.ctypeslib.as_array(v.seq_offsets, shape=(n,))
+        sl = np.ctypeslib.as_array(v.seq_lengths, shape=(n,))
+        qo = np.ctypeslib.as_array(v.qual_offsets, shape=(n,))
+        ql = np.ctypeslib.as_array(v.qual_lengths, shape=(n,))
+        sb = int((so + sl).max())
+        qb = int((qo + ql).max())
+        seq = C.string_at(v.seq, sb) if sb else b""
+        qual = C.string_at(v.qual, qb) if qb else b""
+        return [(seq[int(o):int(o) + int(a)], qual[int(p):int(p) + int(b)]) for o, a, p, b in zip(so, sl, qo, ql)]
+    finally:
+        lib.fb_fastx_destroy(h)
+
+
+def ingest_params(subsample_factor: float = 1.0, min_len: int = 30, min_qual: int = 20, trim: bool = True,
+                  merge: bool = True, seed: int = 0) -> _lib.IngestParams:
+    p = _lib.IngestParams()
+    _lib.load().fb_ingest_params_default(C.byref(p))
+    p.subsample_factor, p.min_len, p.min_qual = float(subsample_factor), int(min_len), int(min_qual)
+    p.enable_trim, p.enable_merge, p.seed = int(bool(trim)), int(bool(merge)), int(seed)
+    return p
+
+
+class _Records:
+    """fb_records_view over [(seq, qual), ...], keeping the arrays alive."""
+
+    def __init__(self, recs):
+        seqs = [bytes(s) for s, _ in recs]
+        quals = [bytes(q) for _, q in recs]
+        self.sl = np.array([len(x) for x in seqs], np.uint32)
+        self.ql = np.array([len(x) for x in quals], np.uint32)
+        self.so = np.zeros(len(seqs), np.uint64)
+        self.qo = np.zeros(len(seqs), np.uint64)
+        if len(seqs):
+            self.so[1:] = np.cumsum(self.sl[:-1], dtype=np.uint64)
+            self.qo[1:] = np.cumsum(self.ql[:-1], dtype=np.uint64)
+        self.s = np.frombuffer(b"".join(seqs) or b"\0", np.uint8).copy()
+        self.q = np.frombuffer(b"".join(quals) or b"\0", np.uint8).copy()
+        self.view = _lib.RecordsView(_ptr(self.s, C.c_uint8), _ptr(self.so, C.c_uint64), _ptr(self.sl, C.c_uint32),
+                                     _ptr(self.q, C.c_uint8), _ptr(self.qo, C.c_uint64), _ptr(self.ql, C.c_uint32),
+                                     len(seqs))
+
+
+def _ingest_stats(st) -> dict:
+    return {f: (float(getattr(st, f)) if f.endswith("_ms") else int(getattr(st, f))) for f, _ in st._fields_}
+
+
+def ingest_records(ctx: Context, r1, r2, **params) -> tuple:
+    """ingest_reads_ext_with_seed's record loop on the device over parsed
+    mates [(seq, qual), ...] -> (DeviceBank, stats dict)."""
+    a, b = _Records(r1), _Records(r2)
+    p = ingest_params(**params)
+    st = _lib.IngestStats()
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_ingest_records(ctx.h, C.byref(a.view), C.byref(b.view), C.byref(p), C.byref(h),
+                                        C.byref(st)), "fb_ingest_records")
+    return DeviceBank._adopt(ctx, h), _ingest_stats(st)
+
+
+def ingest_files(ctx: Context, r1_path, r2_path, **params) -> tuple:
+    """ingest_reads_ext_with_seed: FASTQ/FASTA (.gz) mates to a device bank
+    -> (DeviceBank, stats dict with parse_ms and device_ms)."""
+    p = ingest_params(**params)
+    st = _lib.IngestStats()
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_ingest_files(ctx.h, str(r1_path).encode(), str(r2_path).encode(), C.byref(p), C.byref(h),
+                                      C.byref(st)), "fb_ingest_files")
+    return DeviceBank._adopt(ctx, h), _ingest_stats(st)
+
+
+# ---------------------------------------------------------------------------
 # Batched form
 
 

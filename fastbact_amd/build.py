@@ -63,7 +63,7 @@ def build(jobs: int = 4, verbose: bool = False) -> Path:
             f.result()
     objs = [str(OBJ / (Path(s[s.index("-c") + 1]).stem + ".o")) for s in steps]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

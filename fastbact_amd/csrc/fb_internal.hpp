@@ -337,6 +337,42 @@ hipError_t launch_fa_list(const FaArgs& a, hipStream_t s);
 hipError_t launch_fa_offsets(const FaArgs& a, hipStream_t s);
 hipError_t launch_fa_gather(const FaArgs& a, hipStream_t s);
 
+// Ingest record loop (trim_merge.rs:17-417), fb_ingest.hip.
+struct DevRecords {
+  const uint8_t* seq = nullptr;
+  const uint64_t* seq_off = nullptr;
+  const uint32_t* seq_len = nullptr;
+  const uint8_t* qual = nullptr;
+  const uint64_t* qual_off = nullptr;
+  const uint32_t* qual_len = nullptr;
+};
+struct IngArgs {
+  DevRecords r1, r2;
+  uint64_t n = 0;  // pairs
+  uint32_t subsample = 0;
+  uint64_t keep_threshold = 0, seed = 0;
+  uint32_t enable_trim = 1, enable_merge = 1, min_len = 0, min_qual = 0;
+  uint32_t* status = nullptr;            // [0] = 1: a quality string shorter than its clipped read
+  unsigned long long* stats = nullptr;   // retained, bases after trim, merged, merged bases, unmerged
+  uint32_t *merged = nullptr, *unmerged = nullptr;  // n + 1 flags
+  uint32_t *pos_m = nullptr, *pos_u = nullptr;      // their exclusive scans
+  uint32_t *t1 = nullptr, *t2 = nullptr, *ov = nullptr;
+  uint32_t* src = nullptr;   // pair of every output read
+  uint32_t* olen = nullptr;  // its length (+ one trailing 0)
+  uint64_t* opos = nullptr;  // exclusive scan of olen
+  uint64_t rfirst[4] = {0, 0, 0, 0};
+  uint64_t word0[4] = {0, 0, 0, 0};
+  uint64_t* odata = nullptr;
+  uint64_t* ooffs = nullptr;
+  uint32_t* olens = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t hist_n = 1;
+};
+hipError_t launch_ing_pairs(const IngArgs& a, hipStream_t s);
+hipError_t launch_ing_list(const IngArgs& a, hipStream_t s);
+hipError_t launch_ing_offsets(const IngArgs& a, hipStream_t s);
+hipError_t launch_ing_gather(const IngArgs& a, hipStream_t s);
+
 hipError_t cg_exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
                                 hipStream_t s);
 hipError_t cg_exclusive_sum_u64(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,

@@ -3152,3 +3152,213 @@ extern "C" int fb_upgrade_multik(fb_ctx* c, const fb_bank* bank, const fb_graph_
   if (out_active) *out_active = active_own.release();
   return FB_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Ingest front end (trim_merge.rs:17-417): records to a device bank
+// (fb_ingest.hip), and the FASTQ reader on two host threads.
+
+extern "C" void fb_ingest_params_default(fb_ingest_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->subsample_factor = 1.0;
+  p->min_len = 30;  // main.rs:215-224
+  p->min_qual = 20;
+  p->enable_trim = 1;
+  p->enable_merge = 1;
+}
+
+// Uploads one mate file's records; returns the device view.
+static int upload_records(fb_ctx* c, const fb_records_view* v, DBuf* bufs, DevRecords* d) {
+  uint64_t sb = 0, qb = 0;
+  for (uint64_t i = 0; i < v->n; ++i) {
+    sb = std::max<uint64_t>(sb, v->seq_offsets[i] + v->seq_lengths[i]);
+    qb = std::max<uint64_t>(qb, v->qual_offsets[i] + v->qual_lengths[i]);
+  }
+  if (int r = to_device(c, bufs[0], v->seq, sb)) return r;
+  if (int r = to_device(c, bufs[1], v->seq_offsets, v->n * 8)) return r;
+  if (int r = to_device(c, bufs[2], v->seq_lengths, v->n * 4)) return r;
+  if (int r = to_device(c, bufs[3], v->qual, qb)) return r;
+  if (int r = to_device(c, bufs[4], v->qual_offsets, v->n * 8)) return r;
+  if (int r = to_device(c, bufs[5], v->qual_lengths, v->n * 4)) return r;
+  d->seq = bufs[0].as<uint8_t>();
+  d->seq_off = bufs[1].as<uint64_t>();
+  d->seq_len = bufs[2].as<uint32_t>();
+  d->qual = bufs[3].as<uint8_t>();
+  d->qual_off = bufs[4].as<uint64_t>();
+  d->qual_len = bufs[5].as<uint32_t>();
+  return FB_OK;
+}
+
+static bool records_ok(const fb_records_view* v) {
+  return v && (v->n == 0 || (v->seq_offsets && v->seq_lengths && v->qual_offsets && v->qual_lengths));
+}
+
+extern "C" int fb_ingest_records(fb_ctx* c, const fb_records_view* r1, const fb_records_view* r2,
+                                 const fb_ingest_params* p, fb_bank** out, fb_ingest_stats* st) {
+  if (!c || !p || !out || !records_ok(r1) || !records_ok(r2)) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (r1->n != r2->n) return set_err(c, FB_ERR_INVALID, "Mismatched paired-end FASTQ mate counts between R1 and R2");
+  const uint64_t n = r1->n;
+  if (n >= (1ull << 31)) return set_err(c, FB_ERR_INVALID, "ingest takes < 2^31 pairs per call");
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+  fb_ingest_stats hs;
+  memset(&hs, 0, sizeof hs);
+  hs.input_read_pairs = n;
+  for (uint64_t i = 0; i < n; ++i) hs.bases_before_trim += (uint64_t)r1->seq_lengths[i] + r2->seq_lengths[i];
+  HIP_TRY(c, hipEventRecord(c->ev[0], s));
+
+  IngArgs a;
+  DBuf b1[6], b2[6];
+  if (int r = upload_records(c, r1, b1, &a.r1)) return r;
+  if (int r = upload_records(c, r2, b2, &a.r2)) return r;
+  a.n = n;
+  a.subsample = p->subsample_factor < 0.999 ? 1u : 0u;
+  const double thr = p->subsample_factor * 18446744073709551616.0;  // (factor * u64::MAX as f64) as u64
+  a.keep_threshold = thr <= 0.0 ? 0ull : (thr >= 18446744073709551616.0 ? ~0ull : (uint64_t)thr);
+  a.seed = p->seed;
+  a.enable_trim = p->enable_trim ? 1u : 0u;
+  a.enable_merge = p->enable_merge ? 1u : 0u;
+  a.min_len = p->min_len;
+  a.min_qual = p->min_qual;
+  DBuf dstat, dsum, dm, du, dpm, dpu, dt1, dt2, dov, dtmp, dsrc, dolen, dopos, dhist;
+  if (int r = dstat.get(c, 256)) return r;
+  if (int r = dsum.get(c, 64)) return r;
+  for (DBuf* d : {&dm, &du, &dpm, &dpu})
+    if (int r = d->get(c, (n + 1) * 4)) return r;
+  for (DBuf* d : {&dt1, &dt2, &dov})
+    if (int r = d->get(c, std::max<uint64_t>(n, 1) * 4)) return r;
+  a.status = dstat.as<uint32_t>();
+  a.stats = dsum.as<unsigned long long>();
+  a.merged = dm.as<uint32_t>();
+  a.unmerged = du.as<uint32_t>();
+  a.pos_m = dpm.as<uint32_t>();
+  a.pos_u = dpu.as<uint32_t>();
+  a.t1 = dt1.as<uint32_t>();
+  a.t2 = dt2.as<uint32_t>();
+  a.ov = dov.as<uint32_t>();
+  HIP_TRY(c, launch_fill_u32(a.status, 1, 0, s));
+  HIP_TRY(c, launch_fill_u32(dsum.as<uint32_t>(), 10, 0, s));
+  HIP_TRY(c, launch_ing_pairs(a, s));
+  size_t tb = 0, tb2 = 0;
+  HIP_TRY(c, cg_exclusive_sum_u32(nullptr, nullptr, n + 1, nullptr, &tb, s));
+  HIP_TRY(c, cg_exclusive_sum_u64(nullptr, nullptr, 2 * n + 1, nullptr, &tb2, s));
+  if (int r = dtmp.get(c, std::max<size_t>(std::max(tb, tb2), 16))) return r;
+  tb = dtmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u32(a.merged, a.pos_m, n + 1, dtmp.p, &tb, s));
+  tb = dtmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u32(a.unmerged, a.pos_u, n + 1, dtmp.p, &tb, s));
+  uint32_t cnt[3] = {0, 0, 0};
+  unsigned long long sums[5] = {0, 0, 0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(&cnt[0], a.pos_m + n, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&cnt[1], a.pos_u + n, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&cnt[2], a.status, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(sums, a.stats, sizeof sums, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (cnt[2]) return set_err(c, FB_ERR_INVALID, "a quality string is shorter than its adapter-clipped read");
+  hs.retained_read_pairs = sums[0];
+  hs.bases_after_trim = sums[1];
+  hs.merged_pairs_count = sums[2];
+  hs.merged_bases = sums[3];
+  hs.unmerged_pairs_count = sums[4];
+  if (st) *st = hs;
+  const uint64_t nm = cnt[0], nu = cnt[1], n_out = nm + 2 * nu;
+  if (n_out == 0)
+    return set_err(c, FB_ERR_EMPTY, "No valid reads survived trimming and quality filtering (check input FASTQ files)");
+
+  if (int r = dsrc.get(c, n_out * 4)) return r;
+  if (int r = dolen.get(c, (n_out + 1) * 4)) return r;
+  if (int r = dopos.get(c, (n_out + 1) * 8)) return r;
+  a.src = dsrc.as<uint32_t>();
+  a.olen = dolen.as<uint32_t>();
+  a.opos = dopos.as<uint64_t>();
+  HIP_TRY(c, launch_fill_u32(a.olen, n_out + 1, 0, s));
+  HIP_TRY(c, launch_ing_list(a, s));
+  tb2 = dtmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u64(a.olen, a.opos, n_out + 1, dtmp.p, &tb2, s));
+  uint64_t bnd[3] = {0, 0, 0};
+  HIP_TRY(c, hipMemcpyAsync(&bnd[0], a.opos + nm, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&bnd[1], a.opos + nm + nu, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&bnd[2], a.opos + n_out, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const uint64_t B[3] = {bnd[0], bnd[1] - bnd[0], bnd[2] - bnd[1]};
+  a.rfirst[0] = 0;
+  a.rfirst[1] = nm;
+  a.rfirst[2] = nm + nu;
+  a.rfirst[3] = n_out;
+  a.word0[0] = 0;
+  for (int i = 0; i < 3; ++i) a.word0[i + 1] = a.word0[i] + (B[i] + 31) / 32;
+  uint32_t max_len = 0;
+  for (uint64_t i = 0; i < n; ++i) max_len = std::max(max_len, r1->seq_lengths[i] + r2->seq_lengths[i]);
+
+  auto nb = std::make_unique<fb_bank>();
+  nb->ctx = c;
+  if (int r = nb->data.get(c, (a.word0[3] + kArenaPadWords) * 8)) return r;
+  if (int r = nb->offs.get(c, n_out * 8)) return r;
+  if (int r = nb->lens.get(c, n_out * 4)) return r;
+  a.hist_n = max_len + 1;
+  if (int r = dhist.get(c, (uint64_t)a.hist_n * 4)) return r;
+  a.odata = nb->data.as<uint64_t>();
+  a.ooffs = nb->offs.as<uint64_t>();
+  a.olens = nb->lens.as<uint32_t>();
+  a.hist = dhist.as<uint32_t>();
+  HIP_TRY(c, launch_fill_u32(a.hist, a.hist_n, 0, s));
+  HIP_TRY(c, launch_ing_offsets(a, s));
+  HIP_TRY(c, launch_ing_gather(a, s));
+  HIP_TRY(c, hipMemsetAsync(a.odata + a.word0[3], 0, kArenaPadWords * 8, s));
+  std::vector<uint32_t> hist(a.hist_n);
+  HIP_TRY(c, hipMemcpyAsync(hist.data(), a.hist, (uint64_t)a.hist_n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  (void)hipEventElapsedTime(&hs.device_ms, c->ev[0], c->ev[1]);
+  if (st) *st = hs;
+  nb->n_reads = n_out;
+  nb->len_hist.assign(a.hist_n, 0);
+  for (uint32_t L = 0; L < a.hist_n; ++L)
+    if (hist[L]) {
+      nb->len_hist[L] = (uint64_t)hist[L] * L;
+      nb->max_len = L;
+    }
+  const uint64_t nr_out[3] = {nm, nu, nu};
+  for (int i = 0; i < 3; ++i) {
+    nb->ar_first[i] = a.rfirst[i];
+    nb->ar_reads[i] = nr_out[i];
+    nb->ar_word0[i] = a.word0[i];
+    nb->ar_words[i] = a.word0[i + 1] - a.word0[i];
+    nb->ar_bases[i] = B[i];
+  }
+  nb->total_bases = B[0] + B[1] + B[2];
+  *out = nb.release();
+  return FB_OK;
+}
+
+extern "C" int fb_ingest_files(fb_ctx* c, const char* r1_path, const char* r2_path, const fb_ingest_params* p,
+                               fb_bank** out, fb_ingest_stats* st) {
+  if (!c || !r1_path || !r2_path || !p || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
+  fb_fastx* x[2] = {nullptr, nullptr};
+  int rc[2] = {FB_OK, FB_OK};
+  std::string err[2];
+  std::thread t([&] {
+    rc[1] = fb_fastx_read(r2_path, &x[1]);
+    if (rc[1]) err[1] = fb_fastx_last_error();
+  });
+  rc[0] = fb_fastx_read(r1_path, &x[0]);
+  if (rc[0]) err[0] = fb_fastx_last_error();
+  t.join();
+  std::unique_ptr<fb_fastx, void (*)(fb_fastx*)> own0(x[0], fb_fastx_destroy), own1(x[1], fb_fastx_destroy);
+  for (int i = 0; i < 2; ++i)
+    if (rc[i]) return set_err(c, rc[i], "%s", err[i].c_str());
+  const float parse_ms =
+      (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  fb_ingest_stats hs;
+  memset(&hs, 0, sizeof hs);
+  fb_records_view v[2];
+  fb_fastx_view(x[0], &v[0]);
+  fb_fastx_view(x[1], &v[1]);
+  const int r = fb_ingest_records(c, &v[0], &v[1], p, out, &hs);
+  hs.parse_ms = parse_ms;
+  if (st) *st = hs;
+  return r;
+}

@@ -21,6 +21,8 @@
  *   fb_compact_*        unitig compaction (steps 4-6)    src/graph/build.rs:330-597
  *   fb_filter_active_reads  filter_active_reads          src/graph/local_multik.rs:259-333
  *   fb_upgrade_multik   upgrade_multik_ext<K2>           src/graph/local_multik.rs:38-147
+ *   fb_ingest_*         ingest_reads_ext_with_seed       src/ingest/trim_merge.rs:17-417
+ *   fb_fastx_*          open_fastx + record parsing      src/ingest/trim_merge.rs:233-252
  *   fb_estimate_genome  estimate_genome                  src/ingest/estimator.rs:286-440
  *   fb_count_stats_from_lengths  count_fastx_exact (length part)  src/ingest/estimator.rs:167-283
  *   fb_compute_dynamic_kmers                             src/ingest/estimator.rs:104-161
@@ -291,6 +293,68 @@ typedef struct {
 int fb_upgrade_multik(fb_ctx* ctx, const fb_bank* bank, const fb_graph_view* prev, uint32_t k2, uint64_t genome_size,
                       const fb_build_params* params, fb_multik_result* res, fb_graph** out_graph, fb_solid** out_solid,
                       fb_bank** out_active);
+
+/* ---- Ingest front end (src/ingest/trim_merge.rs:17-417) ---------------------
+ * Parsed records: record i has seq[seq_offsets[i] .. + seq_lengths[i]) and
+ * qual[qual_offsets[i] .. + qual_lengths[i]) (qual_lengths 0 for FASTA). */
+typedef struct {
+  const uint8_t* seq;
+  const uint64_t* seq_offsets;
+  const uint32_t* seq_lengths;
+  const uint8_t* qual;
+  const uint64_t* qual_offsets;
+  const uint32_t* qual_lengths;
+  uint64_t n;
+} fb_records_view;
+
+/* open_fastx + needletail's record reader (trim_merge.rs:233-252): FASTQ
+ * (4-line records) or FASTA (multi-line), gzip when the name ends in ".gz".
+ * Errors: FB_ERR_INVALID with fb_fastx_last_error() (per thread). */
+typedef struct fb_fastx fb_fastx;
+int fb_fastx_read(const char* path, fb_fastx** out);
+const char* fb_fastx_last_error(void);
+uint64_t fb_fastx_count(const fb_fastx* x);
+/* Pointers into x (valid while x lives). */
+int fb_fastx_view(const fb_fastx* x, fb_records_view* out);
+void fb_fastx_destroy(fb_fastx* x);
+
+/* ingest_reads_ext_with_seed's arguments; main.rs:215-224 passes min_len 30,
+ * min_qual 20 (fb_ingest_params_default). */
+typedef struct {
+  double subsample_factor; /* < 0.999: keep pair i (1-based) iff splitmix64(i ^ seed) <= factor * 2^64 */
+  uint32_t min_len;
+  uint32_t min_qual;
+  int32_t enable_trim;
+  int32_t enable_merge;
+  uint64_t seed;
+} fb_ingest_params;
+void fb_ingest_params_default(fb_ingest_params* p);
+/* IngestStats (trim_merge.rs:256-265), then timings with no counterpart in
+ * the reference (milliseconds: parsing on the host, the device steps). */
+typedef struct {
+  uint64_t input_read_pairs;
+  uint64_t retained_read_pairs;
+  uint64_t bases_before_trim;
+  uint64_t bases_after_trim;
+  uint64_t merged_pairs_count;
+  uint64_t merged_bases;
+  uint64_t unmerged_pairs_count;
+  float parse_ms;
+  float device_ms;
+} fb_ingest_stats;
+/* The record loop on the device: subsample keep-test, trim_adapter,
+ * trim_quality, merge_read_pair (min overlap 15, 10% mismatches), then the
+ * reads pushed into a new device bank as the reference pushes them (merged
+ * reads, and the unmerged mates in pe_forward / pe_reverse, in pair order).
+ * Errors as the reference's: FB_ERR_INVALID for unequal mate counts (and,
+ * where the reference's slice would panic, a quality string shorter than its
+ * adapter-clipped read), FB_ERR_EMPTY when no read survives. */
+int fb_ingest_records(fb_ctx* ctx, const fb_records_view* r1, const fb_records_view* r2, const fb_ingest_params* p,
+                      fb_bank** out, fb_ingest_stats* stats);
+/* ingest_reads_ext_with_seed: both files parsed (on two host threads), then
+ * fb_ingest_records. */
+int fb_ingest_files(fb_ctx* ctx, const char* r1_path, const char* r2_path, const fb_ingest_params* p, fb_bank** out,
+                    fb_ingest_stats* stats);
 
 /* ---- Batched form: many isolates per launch --------------------------------
  * The reference builds one isolate per process run (main.rs:262).  A batch

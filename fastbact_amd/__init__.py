@@ -21,8 +21,10 @@ from .api import (  # noqa: F401
     compact_arrays,
     compact_unitigs,
     compute_dynamic_kmers,
+    count_fastx,
     count_stats,
     estimate_genome,
+    estimate_genome_files,
     filter_active_reads,
     hll_estimate_registers,
     ingest_files,

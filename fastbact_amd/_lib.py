@@ -327,6 +327,9 @@ SIGNATURES = [
     ("fb_admission_inflight", C.c_uint32, [vp]),
     ("fb_count_stats_from_lengths", C.c_int, [u32p, u32p, C.c_uint64, C.POINTER(CountStats)]),
     ("fb_compute_dynamic_kmers", C.c_uint32, [C.c_uint64, C.c_int64, u32p]),
+    ("fb_count_fastx", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(CountStats)]),
+    ("fb_estimate_genome_files", C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_uint64, C.c_double,
+                                           C.POINTER(GenomeEstimateC)]),
     ("fb_estimate_genome", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.POINTER(CountStats),
                                      C.c_uint64, C.c_double, C.POINTER(GenomeEstimateC)]),
     ("fb_pack_ascii", C.c_int, [u8p, u32p, C.c_uint64, u64p, u64p]),

@@ -795,6 +795,32 @@ def estimate_genome(ctx: Context, r1: BitSeqArena, r2: BitSeqArena, max_pairs: i
                           out.u_ge_1, out.u_ge_2, out.sample_kmer_depth, out.pairs_sampled, out.m_sample)
 
 
+def _estimate_result(out) -> GenomeEstimate:
+    return GenomeEstimate(out.genome_size, out.exact_read_pairs, out.total_bases, out.median_read_len,
+                          out.max_read_len, out.estimated_cov, out.subsample_factor,
+                          [int(out.kmers[i]) for i in range(out.n_kmers)], CONFIDENCE[out.confidence],
+                          out.u_ge_1, out.u_ge_2, out.sample_kmer_depth, out.pairs_sampled, out.m_sample)
+
+
+def count_fastx(r1_path, r2_path) -> CountStats:
+    """count_fastx_exact (estimator.rs:167-283) on two read files."""
+    lib = _lib.load()
+    st = CountStats()
+    rc = lib.fb_count_fastx(str(r1_path).encode(), str(r2_path).encode(), C.byref(st))
+    if rc:
+        raise FastBactError(rc, lib.fb_fastx_last_error().decode(errors="replace"))
+    return st
+
+
+def estimate_genome_files(ctx: Context, r1_path, r2_path, max_pairs: int = 150_000,
+                          target_cov: float = 100.0) -> GenomeEstimate:
+    """estimate_genome (estimator.rs:286-440) on two FASTQ/FASTA (.gz) files."""
+    out = GenomeEstimateC()
+    ctx.check(ctx.lib.fb_estimate_genome_files(ctx.h, str(r1_path).encode(), str(r2_path).encode(), max_pairs,
+                                               float(target_cov), C.byref(out)), "fb_estimate_genome_files")
+    return _estimate_result(out)
+
+
 # ---------------------------------------------------------------------------
 # Synthetic isolates
 

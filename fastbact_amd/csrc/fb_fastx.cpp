@@ -14,14 +14,21 @@
 #include <zlib.h>
 
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fastbact_gpu.h"
+#include "fb_host.hpp"
 
 struct fb_fastx {
-  std::vector<uint8_t> seq, qual;
-  std::vector<uint64_t> soff, qoff;
-  std::vector<uint32_t> slen, qlen;
+  std::vector<uint8_t> seq, qual, ids;
+  std::vector<uint64_t> soff, qoff, ioff;
+  std::vector<uint32_t> slen, qlen, ilen;
+  void add_id(const uint8_t* b, const uint8_t* e) {  // the header line after '@' / '>'
+    ioff.push_back(ids.size());
+    ilen.push_back((uint32_t)(e - b));
+    ids.insert(ids.end(), b, e);
+  }
 };
 
 static thread_local std::string g_fastx_err;
@@ -102,6 +109,7 @@ static int parse(const std::vector<uint8_t>& buf, fb_fastx* x, const std::string
       if (se - sb != qe - qb)
         return fastx_fail("Corrupted record " + std::to_string(rec + 1) + " in \"" + path +
                           "\": sequence and quality lengths differ");
+      x->add_id(b + 1, e);
       x->soff.push_back(x->seq.size());
       x->slen.push_back((uint32_t)(se - sb));
       x->seq.insert(x->seq.end(), sb, se);
@@ -114,6 +122,7 @@ static int parse(const std::vector<uint8_t>& buf, fb_fastx* x, const std::string
     bool open = false;
     while (lr.next(&b, &e)) {
       if (b < e && *b == '>') {
+        x->add_id(b + 1, e);
         x->soff.push_back(x->seq.size());
         x->slen.push_back(0);
         x->qoff.push_back(x->qual.size());
@@ -157,4 +166,79 @@ extern "C" int fb_fastx_view(const fb_fastx* x, fb_records_view* v) {
   v->qual_lengths = x->qlen.data();
   v->n = x->slen.size();
   return FB_OK;
+}
+
+namespace fb {
+
+int fastx_read_pair(const char* r1, const char* r2, fb_fastx** a, fb_fastx** b, std::string* err) {
+  *a = *b = nullptr;
+  int rc[2] = {FB_OK, FB_OK};
+  std::string e[2];
+  std::thread t([&] {
+    rc[1] = fb_fastx_read(r2, b);
+    if (rc[1]) e[1] = fb_fastx_last_error();
+  });
+  rc[0] = fb_fastx_read(r1, a);
+  if (rc[0]) e[0] = fb_fastx_last_error();
+  t.join();
+  for (int i = 0; i < 2; ++i)
+    if (rc[i]) {
+      fb_fastx_destroy(*a);
+      fb_fastx_destroy(*b);
+      *a = *b = nullptr;
+      *err = e[i];
+      return rc[i];
+    }
+  return FB_OK;
+}
+
+// extract_read_id_prefix (estimator.rs:89-99): up to the first ASCII
+// whitespace, less a trailing "/1" or "/2".
+static std::string id_prefix(const uint8_t* p, uint32_t n) {
+  uint32_t e = 0;
+  while (e < n && p[e] != ' ' && p[e] != '\t' && p[e] != '\n' && p[e] != '\r' && p[e] != '\f') ++e;
+  if (e >= 2 && p[e - 2] == '/' && (p[e - 1] == '1' || p[e - 1] == '2')) e -= 2;
+  return std::string(reinterpret_cast<const char*>(p), e);
+}
+
+int fastx_count(const fb_fastx* a, const fb_fastx* b, fb_count_stats* out, std::string* err) {
+  const uint64_t n1 = a->slen.size(), n2 = b->slen.size(), n = n1 < n2 ? n1 : n2;
+  for (uint64_t i = 0; i < n; ++i) {
+    const std::string p1 = id_prefix(a->ids.data() + a->ioff[i], a->ilen[i]);
+    const std::string p2 = id_prefix(b->ids.data() + b->ioff[i], b->ilen[i]);
+    if (p1 != p2) {
+      *err = "FASTQ paired-end synchronization error at pair #" + std::to_string(i + 1) + ": R1 ID '" +
+             std::string(reinterpret_cast<const char*>(a->ids.data() + a->ioff[i]), a->ilen[i]) +
+             "' does not match R2 ID '" +
+             std::string(reinterpret_cast<const char*>(b->ids.data() + b->ioff[i]), b->ilen[i]) + "'";
+      return FB_ERR_INVALID;
+    }
+  }
+  if (n1 > n2) {
+    *err = "FASTQ pairing mismatch: R1 has more reads than R2 (R2 ended prematurely after " + std::to_string(n) +
+           " pairs)";
+    return FB_ERR_INVALID;
+  }
+  if (n2 > n1) {
+    *err = "FASTQ pairing mismatch: R2 has more reads than R1 (R1 ended prematurely after " + std::to_string(n) +
+           " pairs)";
+    return FB_ERR_INVALID;
+  }
+  const int r = fb_count_stats_from_lengths(a->slen.data(), b->slen.data(), n, out);
+  if (r) *err = "Input FASTQ files contain 0 valid read pairs";
+  return r;
+}
+
+}  // namespace fb
+
+extern "C" int fb_count_fastx(const char* r1_path, const char* r2_path, fb_count_stats* out) {
+  if (!r1_path || !r2_path || !out) return fastx_fail("null argument");
+  fb_fastx *a = nullptr, *b = nullptr;
+  std::string err;
+  if (int r = fb::fastx_read_pair(r1_path, r2_path, &a, &b, &err)) return fastx_fail(err), r;
+  const int r = fb::fastx_count(a, b, out, &err);
+  fb_fastx_destroy(a);
+  fb_fastx_destroy(b);
+  if (r) fastx_fail(err);
+  return r;
 }

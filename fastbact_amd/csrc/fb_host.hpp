@@ -6,6 +6,7 @@
 
 #include <deque>
 #include <map>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -42,4 +43,8 @@ bool checkpoint_stop(uint64_t u1, uint64_t u2, uint64_t m_sample);
 // subsample factor and k ladder.
 void estimate_finish(uint64_t u_ge_1, uint64_t u_ge_2, uint64_t m_sample, const fb_count_stats* st,
                      double target_cov, fb_genome_estimate* o);
+// Both mate files read on two host threads; *err names the failing file's error.
+int fastx_read_pair(const char* r1, const char* r2, fb_fastx** a, fb_fastx** b, std::string* err);
+// count_fastx_exact (estimator.rs:167-283) on two read files.
+int fastx_count(const fb_fastx* a, const fb_fastx* b, fb_count_stats* out, std::string* err);
 }  // namespace fb

@@ -3362,3 +3362,38 @@ extern "C" int fb_ingest_files(fb_ctx* c, const char* r1_path, const char* r2_pa
   if (st) *st = hs;
   return r;
 }
+
+// estimate_genome on two read files (estimator.rs:286-440): count_fastx_exact
+// over every pair, then the sketch loop over the first pairs, packed to 2 bits
+// (lossless for the estimator: its ntHash maps non-ACGT bytes as the packing does).
+extern "C" int fb_estimate_genome_files(fb_ctx* c, const char* r1_path, const char* r2_path, uint64_t max_pairs,
+                                        double target_cov, fb_genome_estimate* out) {
+  if (!c || !r1_path || !r2_path || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  fb_fastx *a = nullptr, *b = nullptr;
+  std::string err;
+  if (int r = fb::fastx_read_pair(r1_path, r2_path, &a, &b, &err)) return set_err(c, r, "%s", err.c_str());
+  std::unique_ptr<fb_fastx, void (*)(fb_fastx*)> own0(a, fb_fastx_destroy), own1(b, fb_fastx_destroy);
+  fb_count_stats cs;
+  if (int r = fb::fastx_count(a, b, &cs, &err)) return set_err(c, r, "%s", err.c_str());
+  const uint64_t n = std::min<uint64_t>(std::min<uint64_t>(max_pairs, 150000), cs.exact_read_pairs);
+  fb_records_view v[2];
+  fb_fastx_view(a, &v[0]);
+  fb_fastx_view(b, &v[1]);
+  std::vector<uint64_t> words[2], offs[2];
+  std::vector<uint32_t> lens[2];
+  fb_arena_view av[2];
+  for (int m = 0; m < 2; ++m) {
+    uint64_t bases = 0;
+    lens[m].assign(v[m].seq_lengths, v[m].seq_lengths + n);
+    for (uint64_t i = 0; i < n; ++i) bases += lens[m][i];
+    std::vector<uint8_t> cat;
+    cat.reserve(bases);
+    for (uint64_t i = 0; i < n; ++i)
+      cat.insert(cat.end(), v[m].seq + v[m].seq_offsets[i], v[m].seq + v[m].seq_offsets[i] + lens[m][i]);
+    words[m].assign((bases + 31) / 32 + 1, 0);
+    offs[m].assign(n, 0);
+    fb_pack_ascii(cat.data(), lens[m].data(), n, words[m].data(), offs[m].data());
+    av[m] = fb_arena_view{words[m].data(), (uint64_t)words[m].size(), offs[m].data(), lens[m].data(), n};
+  }
+  return fb_estimate_genome(c, &av[0], &av[1], &cs, max_pairs, target_cov, out);
+}

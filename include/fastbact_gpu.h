@@ -441,6 +441,14 @@ typedef struct {
 /* count_fastx_exact's statistics from per-read lengths of the pairs.
  * FB_ERR_EMPTY when n_pairs == 0. */
 int fb_count_stats_from_lengths(const uint32_t* len_r1, const uint32_t* len_r2, uint64_t n_pairs, fb_count_stats* out);
+/* count_fastx_exact (estimator.rs:167-283) on two FASTQ/FASTA (.gz) files:
+ * the pairing and read-ID prefix checks (extract_read_id_prefix,
+ * estimator.rs:89-99) and the length statistics.  Errors: FB_ERR_INVALID or
+ * FB_ERR_EMPTY with fb_fastx_last_error(). */
+int fb_count_fastx(const char* r1_path, const char* r2_path, fb_count_stats* out);
+/* estimate_genome (estimator.rs:286-440) on two read files. */
+int fb_estimate_genome_files(fb_ctx* ctx, const char* r1_path, const char* r2_path, uint64_t max_pairs,
+                             double target_cov, fb_genome_estimate* out);
 /* compute_dynamic_kmers; target_max_k < 0 = None.  Returns the count (<= 8). */
 uint32_t fb_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uint32_t* out8);
 /* estimate_genome over R1[i]/R2[i] pairs (ASCII->2-bit is lossless for the

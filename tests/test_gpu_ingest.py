@@ -145,3 +145,25 @@ def test_errors(ctx, tmp_path):
     assert st["retained_read_pairs"] == 10
     with pytest.raises(F.FastBactError, match="Failed to open"):
         F.ingest_files(ctx, tmp_path / "none.fq", fa2)
+
+
+@pytest.mark.parametrize("max_pairs,gz", [(150_000, True), (7_000, False)])
+def test_estimate_genome_from_files_matches_oracle(ctx, orc, tmp_path, max_pairs, gz):
+    """estimate_genome on FASTQ files (count_fastx_exact, then the sketch
+    loop over the first pairs) equals the oracle's on the same reads."""
+    from tests.helpers import isolate
+    from tests.test_gpu_parity import _est_equal
+    iso = isolate(seed=41, genome_len=200_000, n_pairs=20_000, sub_rate=0.004)
+    n, L = iso.r1.shape
+    ext = ".fq.gz" if gz else ".fq"
+    p1, p2 = tmp_path / ("e1" + ext), tmp_path / ("e2" + ext)
+    q = b"I" * L
+    _write_fastq(p1, [bytes(r) for r in iso.r1], [q] * n, gz)
+    _write_fastq(p2, [bytes(r) for r in iso.r2], [q] * n, gz)
+    g = F.estimate_genome_files(ctx, p1, p2, max_pairs, 100.0)
+    bank = iso.bank()
+    r1, r2 = bank.pe_forward, bank.pe_reverse
+    st = orc.count_stats(r1.lengths, r2.lengths)
+    o = orc.estimate_genome((r1.data, r1.offsets, r1.lengths), (r2.data, r2.offsets, r2.lengths), st, max_pairs,
+                            100.0)
+    _est_equal(g, o)

@@ -1927,3 +1927,171 @@ void orc_ingest_free(orc_ingest_out* out) {
     }
     memset(out, 0, sizeof *out);
 }
+
+/* ------------------------------------------------------------------------- */
+/* PE mapper, mapper.rs:89-279.  The index is a sorted array of (key, value);
+ * the reference's FxHashMap only answers membership and the unique value. */
+typedef struct { uint64_t key; uint32_t dir, pos; } tk_t;
+static int tk_cmp(const void* a, const void* b) {
+    const tk_t *x = (const tk_t*)a, *y = (const tk_t*)b;
+    return x->key < y->key ? -1 : x->key > y->key;
+}
+static const tk_t* tk_find(const tk_t* t, uint64_t n, uint64_t key) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (t[mid].key < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && t[lo].key == key ? t + lo : NULL;
+}
+/* canonical k-mer of ascii[0..k): to_u64 of the canonical and whether it is the rc */
+static uint64_t can_key(const uint8_t* ascii, uint32_t k, int nw, int* is_rc) {
+    km_t f = km_from_ascii(ascii, k, nw);
+    km_t r = km_rev_comp(f, k, nw);
+    *is_rc = km_lt(&r, &f, nw);
+    return *is_rc ? km_fold(&r, nw) : km_fold(&f, nw);
+}
+static int d_cmp(const void* a, const void* b) {
+    const double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : x > y;
+}
+static int u32_cmp(const void* a, const void* b) {
+    const uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : x > y;
+}
+void orc_insert_dist_of(const uint32_t* sizes, uint64_t n, orc_insert_dist* o) {
+    if (n < 20) {
+        o->median = 450.0;
+        o->mad = 50.0;
+        o->min_insert = 250;
+        o->max_insert = 700;
+        return;
+    }
+    uint32_t* s = (uint32_t*)malloc(n * 4);
+    double* d = (double*)malloc(n * sizeof(double));
+    memcpy(s, sizes, n * 4);
+    qsort(s, n, 4, u32_cmp);
+    const double med = (double)s[n / 2];
+    for (uint64_t i = 0; i < n; i++) d[i] = fabs((double)s[i] - med);
+    qsort(d, n, sizeof(double), d_cmp);
+    double mad = d[n / 2];
+    if (mad < 20.0) mad = 20.0;
+    double lo = med - 3.5 * mad;
+    if (lo < 50.0) lo = 50.0;
+    o->median = med;
+    o->mad = mad;
+    o->min_insert = (uint64_t)lo;
+    o->max_insert = (uint64_t)(med + 3.5 * mad);
+    free(s);
+    free(d);
+}
+/* map_read closure, mapper.rs:150-162 */
+static int map_read(const uint8_t* r, uint32_t len, uint32_t k, int nw, const tk_t* idx, uint64_t n,
+                    uint32_t* dir, uint32_t* pos, uint32_t* step) {
+    for (uint32_t st = 0; st <= 100; st += 20) {
+        if (st + k > len) continue;
+        int is_rc = 0;
+        const tk_t* h = tk_find(idx, n, can_key(r + st, k, nw, &is_rc));
+        if (h) {
+            *dir = is_rc ? (h->dir ^ 1u) : h->dir;
+            *pos = h->pos;
+            *step = st;
+            return 1;
+        }
+    }
+    return 0;
+}
+int orc_map_paired_reads(const orc_graph_view* g, const orc_arena* pf, const orc_arena* pr, uint32_t k,
+                         orc_pair_map* out) {
+    memset(out, 0, sizeof *out);
+    if (k < 1 || k > 127) return -1;
+    const int nw = orc_kmer_words_for_k(k);
+    /* index_terminal_kmers, mapper.rs:89-135 */
+    uint64_t cap = 0;
+    for (uint64_t i = 0; i < g->n_nodes; i++)
+        if (!(g->flags && (g->flags[i] & 1)) && g->lengths[i] >= k) {
+            const uint64_t t = g->lengths[i] - k + 1;
+            cap += t <= 2000 ? t : 2000;
+        }
+    tk_t* all = (tk_t*)malloc((cap ? cap : 1) * sizeof(tk_t));
+    uint64_t n = 0;
+    for (uint64_t i = 0; i < g->n_nodes; i++) {
+        if ((g->flags && (g->flags[i] & 1)) || g->lengths[i] < k) continue;
+        const uint8_t* s = g->seq + g->offsets[i];
+        const uint64_t t = g->lengths[i] - k + 1;
+        for (uint64_t p = 0; p < t; p++) {
+            if (t > 2000 && p >= 1000 && p < t - 1000) continue;
+            int is_rc = 0;
+            all[n].key = can_key(s + p, k, nw, &is_rc);
+            all[n].dir = ((uint32_t)i << 1) | (uint32_t)is_rc;
+            all[n].pos = (uint32_t)p;
+            n++;
+        }
+    }
+    qsort(all, n, sizeof(tk_t), tk_cmp);
+    uint64_t u = 0;
+    for (uint64_t i = 0; i < n;) {
+        uint64_t j = i + 1;
+        while (j < n && all[j].key == all[i].key) j++;
+        if (j - i == 1) all[u++] = all[i];
+        i = j;
+    }
+    /* map_paired_reads, mapper.rs:138-279 */
+    const uint64_t np = pf->n_reads < pr->n_reads ? pf->n_reads : pr->n_reads;
+    uint32_t maxlen = 1;
+    for (uint64_t i = 0; i < np; i++) {
+        if (pf->lengths[i] > maxlen) maxlen = pf->lengths[i];
+        if (pr->lengths[i] > maxlen) maxlen = pr->lengths[i];
+    }
+    uint8_t* b1 = (uint8_t*)malloc(maxlen);
+    uint8_t* b2 = (uint8_t*)malloc(maxlen);
+    uint64_t lcap = 1024, scap = 1024;
+    out->links = (orc_pair_link*)malloc(lcap * sizeof(orc_pair_link));
+    out->sizes = (uint32_t*)malloc(scap * 4);
+    for (uint64_t i = 0; i < np; i++) {
+        const uint32_t l1 = pf->lengths[i], l2 = pr->lengths[i];
+        if (l1 < k || l2 < k) continue;
+        orc_arena_extract_ascii(pf, i, b1);
+        orc_arena_extract_ascii(pr, i, b2);
+        uint32_t u1, p1, s1, u2, p2, s2;
+        if (!map_read(b1, l1, k, nw, all, u, &u1, &p1, &s1) || !map_read(b2, l2, k, nw, all, u, &u2, &p2, &s2))
+            continue;
+        const uint32_t u2f = u2 ^ 1u;
+        if ((u1 >> 1) == (u2 >> 1)) {
+            const uint32_t node_len = g->lengths[u1 >> 1];
+            if (node_len < 1500) continue;
+            int64_t span = -1;
+            if (!(u1 & 1) && (u2 & 1)) span = ((int64_t)p2 + k + s2) - ((int64_t)p1 - s1);
+            else if ((u1 & 1) && !(u2 & 1)) span = ((int64_t)p1 + k + s1) - ((int64_t)p2 - s2);
+            int64_t min_len = l1 > l2 ? l1 : l2;
+            if (min_len < 100) min_len = 100;
+            if (span >= min_len && span <= 1500) {
+                if (out->n_sizes == scap) out->sizes = (uint32_t*)realloc(out->sizes, (scap *= 2) * 4);
+                out->sizes[out->n_sizes++] = (uint32_t)span;
+            }
+        } else {
+            const int64_t n1 = g->lengths[u1 >> 1], n2 = g->lengths[u2 >> 1];
+            int64_t d1 = (u1 & 1) ? (int64_t)p1 + k + s1 : n1 - ((int64_t)p1 - s1);
+            int64_t d2 = (u2f & 1) ? n2 - ((int64_t)p2 - s2) : (int64_t)p2 + k + s2;
+            if (d1 < 0) d1 = 0;
+            if (d2 < 0) d2 = 0;
+            if (d1 > UINT32_MAX || d2 > UINT32_MAX || l1 > 65535 || l2 > 65535) continue;
+            if (out->n_links + 2 > lcap) out->links = (orc_pair_link*)realloc(out->links, (lcap *= 2) * sizeof(orc_pair_link));
+            orc_pair_link a = {u1, u2f, (uint32_t)i, (uint32_t)d1, (uint32_t)d2, (uint16_t)l1, (uint16_t)l2};
+            orc_pair_link b = {u2f ^ 1u, u1 ^ 1u, (uint32_t)i, (uint32_t)d2, (uint32_t)d1, (uint16_t)l2, (uint16_t)l1};
+            out->links[out->n_links++] = a;
+            out->links[out->n_links++] = b;
+        }
+    }
+    orc_insert_dist_of(out->sizes, out->n_sizes, &out->dist);
+    free(all);
+    free(b1);
+    free(b2);
+    return 0;
+}
+void orc_pair_map_free(orc_pair_map* m) {
+    free(m->links);
+    free(m->sizes);
+    memset(m, 0, sizeof *m);
+}

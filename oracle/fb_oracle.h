@@ -266,6 +266,33 @@ typedef struct {
 int orc_ingest_records(const orc_records* r1, const orc_records* r2, const orc_ingest_params* p, orc_ingest_out* out);
 void orc_ingest_free(orc_ingest_out* out);
 
+/* ---- PE mapper (src/resolve/mapper.rs:89-279) -------------------------------
+ * index_terminal_kmers over the graph's active unitigs (flags bit 0 deleted;
+ * every window when a unitig has <= 2000, else the first and last 1000),
+ * keeping keys (KmerInt::to_u64 of the canonical k-mer) seen once; then
+ * map_paired_reads over pe_forward / pe_reverse.  Evidence records come in
+ * pair order, two per linked pair: (u1, twin(u2)) then (u2, twin(u1)) as
+ * DirUnitigIds; insert sizes in pair order. */
+typedef struct {
+    uint32_t u1, u2, pair_id, dist_u1_exit, dist_u2_entry;
+    uint16_t r1_len, r2_len;
+} orc_pair_link;
+typedef struct {
+    double median, mad;
+    uint64_t min_insert, max_insert;
+} orc_insert_dist;
+typedef struct {
+    uint64_t n_links, n_sizes;
+    orc_pair_link* links;
+    uint32_t* sizes;
+    orc_insert_dist dist;
+} orc_pair_map;
+int orc_map_paired_reads(const orc_graph_view* g, const orc_arena* pe_forward, const orc_arena* pe_reverse,
+                         uint32_t k, orc_pair_map* out);
+void orc_pair_map_free(orc_pair_map* m);
+/* InsertSizeDist from sizes (mapper.rs:255-276); fewer than 20: the default. */
+void orc_insert_dist_of(const uint32_t* sizes, uint64_t n, orc_insert_dist* out);
+
 #ifdef __cplusplus
 }
 #endif

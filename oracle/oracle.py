@@ -99,6 +99,20 @@ class IngestOut(C.Structure):
     _fields_ = [("seq", u8p * 3), ("len", u32p * 3), ("n", C.c_uint64 * 3), ("stats", IngestStats)]
 
 
+class PairLink(C.Structure):
+    _fields_ = [("u1", C.c_uint32), ("u2", C.c_uint32), ("pair_id", C.c_uint32), ("dist_u1_exit", C.c_uint32),
+                ("dist_u2_entry", C.c_uint32), ("r1_len", C.c_uint16), ("r2_len", C.c_uint16)]
+
+
+class InsertDist(C.Structure):
+    _fields_ = [("median", C.c_double), ("mad", C.c_double), ("min_insert", C.c_uint64), ("max_insert", C.c_uint64)]
+
+
+class PairMap(C.Structure):
+    _fields_ = [("n_links", C.c_uint64), ("n_sizes", C.c_uint64), ("links", C.POINTER(PairLink)), ("sizes", u32p),
+                ("dist", InsertDist)]
+
+
 _SIGS = [
     ("orc_splitmix64", C.c_uint64, [C.c_uint64]),
     ("orc_rev_comp_word", C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -151,6 +165,10 @@ _SIGS = [
     ("orc_ingest_records", C.c_int, [C.POINTER(Records), C.POINTER(Records), C.POINTER(IngestParams),
                                      C.POINTER(IngestOut)]),
     ("orc_ingest_free", None, [C.POINTER(IngestOut)]),
+    ("orc_map_paired_reads", C.c_int, [C.POINTER(GraphView), C.POINTER(Arena), C.POINTER(Arena), C.c_uint32,
+                                       C.POINTER(PairMap)]),
+    ("orc_pair_map_free", None, [C.POINTER(PairMap)]),
+    ("orc_insert_dist_of", None, [u32p, C.c_uint64, C.POINTER(InsertDist)]),
 ]
 
 _LIB = None
@@ -582,3 +600,55 @@ def ingest_records(r1, r2, subsample_factor: float = 1.0, min_len: int = 50, min
     finally:
         lib().orc_ingest_free(C.byref(out))
     return {"reads": reads, "arenas": [pack(r) for r in reads], "stats": st}
+
+
+# ---- PE mapper (mapper.rs:89-279) --------------------------------------------
+
+LINK_FIELDS = ("u1", "u2", "pair_id", "dist_u1_exit", "dist_u2_entry", "r1_len", "r2_len")
+
+
+def _graph_view(graph: dict, k: int):
+    seqs = [bytes(x) for x in graph["seqs"]]
+    n = len(seqs)
+    lens = np.array([len(x) for x in seqs], np.uint32)
+    offs = np.zeros(n, np.uint64)
+    if n:
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    buf = _bytes(b"".join(seqs))
+    flags = (np.asarray(graph.get("circular", np.zeros(n)), np.uint8) << 1).astype(np.uint8)
+    if "deleted" in graph:
+        flags |= np.asarray(graph["deleted"], np.uint8)
+    gv = GraphView(k, n, _p(buf, C.c_uint8), _p(offs, C.c_uint64), _p(lens, C.c_uint32), _p(flags, C.c_uint8),
+                   None, None)
+    gv._keep = (buf, offs, lens, flags)
+    return gv
+
+
+def map_paired_reads(graph: dict, pe_forward, pe_reverse, k: int) -> dict:
+    """map_paired_reads over a graph dict and the two mate arenas ((data,
+    offsets, lengths)) -> {"links": structured array of the evidence records
+    in pair order, "sizes": insert sizes in pair order, "dist": InsertSizeDist}."""
+    gv = _graph_view(graph, k)
+    a, b = arena(*pe_forward), arena(*pe_reverse)
+    m = PairMap()
+    rc = lib().orc_map_paired_reads(C.byref(gv), C.byref(a), C.byref(b), k, C.byref(m))
+    if rc:
+        raise ValueError(f"orc_map_paired_reads failed ({rc})")
+    try:
+        dt = np.dtype([(f, np.uint32) for f in LINK_FIELDS[:5]] + [("r1_len", np.uint16), ("r2_len", np.uint16)])
+        nl = int(m.n_links)
+        links = np.frombuffer(C.string_at(m.links, nl * dt.itemsize), dt).copy() if nl else np.zeros(0, dt)
+        ns = int(m.n_sizes)
+        sizes = np.ctypeslib.as_array(m.sizes, shape=(ns,)).copy() if ns else np.zeros(0, np.uint32)
+        dist = {"median": m.dist.median, "mad": m.dist.mad, "min_insert": int(m.dist.min_insert),
+                "max_insert": int(m.dist.max_insert)}
+    finally:
+        lib().orc_pair_map_free(C.byref(m))
+    return {"links": links, "sizes": sizes, "dist": dist}
+
+
+def insert_dist(sizes) -> dict:
+    s = np.ascontiguousarray(sizes, np.uint32)
+    d = InsertDist()
+    lib().orc_insert_dist_of(_p(s, C.c_uint32), s.size, C.byref(d))
+    return {"median": d.median, "mad": d.mad, "min_insert": int(d.min_insert), "max_insert": int(d.max_insert)}

@@ -543,3 +543,69 @@ def merge_pair(r1: str, q1: str, r2: str, q2: str, min_overlap: int = 15, ratio:
         out.append(b1.upper() if (b1.upper() == b2 or qa >= qb) else b2)
     out.append(rc2[d:])
     return "".join(out), d, m
+
+
+# ---- PE mapper (mapper.rs:89-279), with dictionaries of Python ints ---------
+
+def _fold_canonical(s: str, k: int) -> tuple:
+    """(to_u64 of the canonical k-mer, is_rc): the fold XORs the 64-bit words
+    of the reference's integer type for k."""
+    x, is_rc = canonical(s)
+    nw = 1 if k <= 31 else (2 if k <= 63 else 4)
+    f = 0
+    for w in ref_words(x, k)[:nw]:
+        f ^= w
+    return f, is_rc
+
+
+def map_paired_reads(unitigs: list, pairs: list, k: int):
+    """unitigs: sequences (active, linear); pairs: (r1, r2) strings.
+    -> (links as tuples in pair order, insert sizes in pair order)."""
+    counts, where = {}, {}
+    for uid, seq in enumerate(unitigs):
+        t = len(seq) - k + 1
+        if t <= 0:
+            continue
+        starts = list(range(t)) if t <= 2000 else list(range(1000)) + list(range(t - 1000, t))
+        for p in starts:
+            h, rc = _fold_canonical(seq[p:p + k], k)
+            counts[h] = counts.get(h, 0) + 1
+            where[h] = (uid * 2 + int(rc), p)
+    index = {h: v for h, v in where.items() if counts[h] == 1}
+
+    def hit(r):
+        for st in (0, 20, 40, 60, 80, 100):
+            if st + k <= len(r):
+                h, rc = _fold_canonical(r[st:st + k], k)
+                if h in index:
+                    d, p = index[h]
+                    return (d ^ 1 if rc else d), p, st
+        return None
+
+    links, sizes = [], []
+    for i, (r1, r2) in enumerate(pairs):
+        if len(r1) < k or len(r2) < k:
+            continue
+        a, b = hit(r1), hit(r2)
+        if a is None or b is None:
+            continue
+        (u1, p1, s1), (u2, p2, s2) = a, b
+        if u1 // 2 == u2 // 2:
+            if len(unitigs[u1 // 2]) < 1500:
+                continue
+            if u1 % 2 == 0 and u2 % 2 == 1:
+                span = (p2 + k + s2) - (p1 - s1)
+            elif u1 % 2 == 1 and u2 % 2 == 0:
+                span = (p1 + k + s1) - (p2 - s2)
+            else:
+                span = -1
+            if max(len(r1), len(r2), 100) <= span <= 1500:
+                sizes.append(span)
+        else:
+            u2f = u2 ^ 1
+            d1 = p1 + k + s1 if u1 % 2 else len(unitigs[u1 // 2]) - (p1 - s1)
+            d2 = len(unitigs[u2 // 2]) - (p2 - s2) if u2f % 2 else p2 + k + s2
+            d1, d2 = max(d1, 0), max(d2, 0)
+            links.append((u1, u2f, i, d1, d2, len(r1), len(r2)))
+            links.append((u2f ^ 1, u1 ^ 1, i, d2, d1, len(r2), len(r1)))
+    return links, sizes

@@ -29,6 +29,7 @@ from .api import (  # noqa: F401
     hll_estimate_registers,
     ingest_files,
     ingest_records,
+    map_paired_reads,
     splitmix64,
     synth_isolate,
     read_fastx,

@@ -81,6 +81,16 @@ class IngestStats(C.Structure):
                                            "unmerged_pairs_count")] + [("parse_ms", C.c_float), ("device_ms", C.c_float)]
 
 
+class PairLink(C.Structure):
+    _fields_ = [("u1", C.c_uint32), ("u2", C.c_uint32), ("pair_id", C.c_uint32), ("dist_u1_exit", C.c_uint32),
+                ("dist_u2_entry", C.c_uint32), ("r1_len", C.c_uint16), ("r2_len", C.c_uint16)]
+
+
+class InsertSizeDist(C.Structure):
+    _fields_ = [("median", C.c_double), ("mad", C.c_double), ("min_insert", C.c_uint64), ("max_insert", C.c_uint64),
+                ("n_sizes", C.c_uint64)]
+
+
 class BuildParams(C.Structure):
     _fields_ = [
         ("k", C.c_uint32),
@@ -287,6 +297,12 @@ SIGNATURES = [
     ("fb_ingest_files", C.c_int, [vp, C.c_char_p, C.c_char_p, C.POINTER(IngestParams), C.POINTER(vp),
                                   C.POINTER(IngestStats)]),
     ("fb_graph_view_of", C.c_int, [vp, C.POINTER(GraphView)]),
+    ("fb_map_paired_reads", C.c_int, [vp, C.POINTER(GraphView), vp, C.c_uint32, C.POINTER(vp)]),
+    ("fb_pair_map_link_count", C.c_uint64, [vp]),
+    ("fb_pair_map_links", C.c_int, [vp, C.POINTER(PairLink)]),
+    ("fb_pair_map_sizes", C.c_int, [vp, C.POINTER(C.c_uint32)]),
+    ("fb_pair_map_dist", C.c_int, [vp, C.POINTER(InsertSizeDist)]),
+    ("fb_pair_map_destroy", None, [vp]),
     ("fb_filter_active_reads", C.c_int, [vp, vp, C.POINTER(GraphView), C.POINTER(vp), u8p, u8p]),
     ("fb_upgrade_multik", C.c_int, [vp, vp, C.POINTER(GraphView), C.c_uint32, C.c_uint64, C.POINTER(BuildParams),
                                     C.POINTER(MultiKResult), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),

@@ -576,6 +576,35 @@ def upgrade_multik(ctx: Context, bank: DeviceBank, graph: dict, k: int, k2: int,
 
 
 # ---------------------------------------------------------------------------
+# PE mapper (mapper.rs:89-279)
+
+LINK_DTYPE = np.dtype([("u1", np.uint32), ("u2", np.uint32), ("pair_id", np.uint32), ("dist_u1_exit", np.uint32),
+                       ("dist_u2_entry", np.uint32), ("r1_len", np.uint16), ("r2_len", np.uint16)])
+
+
+def map_paired_reads(ctx: Context, bank: DeviceBank, graph: dict, k: int) -> dict:
+    """map_paired_reads<K> on the device: {"links": evidence records in pair
+    order (LINK_DTYPE), "sizes": intra-unitig insert sizes in pair order,
+    "dist": InsertSizeDist fields}."""
+    ga = _GraphArrays(graph, k)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_map_paired_reads(ctx.h, C.byref(ga.view), bank.h, k, C.byref(h)), "fb_map_paired_reads")
+    try:
+        n = int(ctx.lib.fb_pair_map_link_count(h))
+        links = np.zeros(max(n, 1), LINK_DTYPE)
+        ctx.check(ctx.lib.fb_pair_map_links(h, links.ctypes.data_as(C.POINTER(_lib.PairLink))), "fb_pair_map_links")
+        d = _lib.InsertSizeDist()
+        ctx.check(ctx.lib.fb_pair_map_dist(h, C.byref(d)), "fb_pair_map_dist")
+        sizes = np.zeros(max(int(d.n_sizes), 1), np.uint32)
+        ctx.check(ctx.lib.fb_pair_map_sizes(h, _ptr(sizes, C.c_uint32)), "fb_pair_map_sizes")
+        return {"links": links[:n], "sizes": sizes[:int(d.n_sizes)],
+                "dist": {"median": d.median, "mad": d.mad, "min_insert": int(d.min_insert),
+                         "max_insert": int(d.max_insert)}}
+    finally:
+        ctx.lib.fb_pair_map_destroy(h)
+
+
+# ---------------------------------------------------------------------------
 # Ingest front end (trim_merge.rs:17-417)
 
 

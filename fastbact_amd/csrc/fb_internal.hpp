@@ -373,6 +373,43 @@ hipError_t launch_ing_list(const IngArgs& a, hipStream_t s);
 hipError_t launch_ing_offsets(const IngArgs& a, hipStream_t s);
 hipError_t launch_ing_gather(const IngArgs& a, hipStream_t s);
 
+// PE mapper (mapper.rs:89-279), fb_mapper.hip.
+struct PairLinkRec {  // the layout of fb_pair_link
+  uint32_t u1, u2, pair_id, dist_u1_exit, dist_u2_entry;
+  uint16_t r1_len, r2_len;
+};
+struct MapArgs {
+  uint32_t k = 0;
+  // unitigs as a packed read list (one read per unitig) and their lengths
+  DevReads useq;
+  const uint32_t* ulen = nullptr;
+  // indexed windows: range r holds windows rpre[r] .. rpre[r+1]-1 of unitig rng_u[r] from rng_p0[r]
+  const uint32_t* rng_u = nullptr;
+  const uint32_t* rng_p0 = nullptr;
+  const uint64_t* rpre = nullptr;
+  uint32_t n_ranges = 0;
+  uint64_t n_windows = 0;
+  // index table: 2^log2cap slots; the key kEmpty is counted in special_*
+  uint64_t* keys = nullptr;
+  uint32_t* cnt = nullptr;
+  uint64_t* val = nullptr;
+  uint32_t log2cap = 0;
+  uint32_t* special_cnt = nullptr;
+  uint64_t* special_val = nullptr;
+  // pairs: pe_forward read first_f + i and pe_reverse read first_r + i
+  DevReads reads;
+  uint64_t first_f = 0, first_r = 0, n_pairs = 0;
+  uint32_t *f_size = nullptr, *f_link = nullptr;      // n_pairs + 1 flags
+  uint32_t *pos_size = nullptr, *pos_link = nullptr;  // their scans
+  uint32_t *span = nullptr, *lu1 = nullptr, *lu2 = nullptr, *ld1 = nullptr, *ld2 = nullptr;
+  uint32_t* sizes = nullptr;
+  PairLinkRec* links = nullptr;
+};
+hipError_t launch_map_clear(const MapArgs& a, hipStream_t s);
+hipError_t launch_map_insert(const MapArgs& a, hipStream_t s);
+hipError_t launch_map_pairs(const MapArgs& a, hipStream_t s);
+hipError_t launch_map_compact(const MapArgs& a, hipStream_t s);
+
 hipError_t cg_exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
                                 hipStream_t s);
 hipError_t cg_exclusive_sum_u64(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,

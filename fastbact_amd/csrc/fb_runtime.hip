@@ -3397,3 +3397,179 @@ extern "C" int fb_estimate_genome_files(fb_ctx* c, const char* r1_path, const ch
   }
   return fb_estimate_genome(c, &av[0], &av[1], &cs, max_pairs, target_cov, out);
 }
+
+// ---------------------------------------------------------------------------
+// PE mapper (mapper.rs:89-279): index and pair mapping on the device
+// (fb_mapper.hip), the insert-size distribution on the host.
+
+struct fb_pair_map {
+  std::vector<fb_pair_link> links;
+  std::vector<uint32_t> sizes;
+  fb_insert_size_dist dist;
+};
+
+// InsertSizeDist (mapper.rs:255-276): median and MAD by selection, which
+// picks the same elements as the reference's full sorts.
+static void insert_dist(const std::vector<uint32_t>& sizes, fb_insert_size_dist* o) {
+  o->n_sizes = sizes.size();
+  if (sizes.size() < 20) {
+    o->median = 450.0;
+    o->mad = 50.0;
+    o->min_insert = 250;
+    o->max_insert = 700;
+    return;
+  }
+  std::vector<uint32_t> s(sizes);
+  const size_t h = s.size() / 2;
+  std::nth_element(s.begin(), s.begin() + h, s.end());
+  const double med = (double)s[h];
+  std::vector<double> d(s.size());
+  for (size_t i = 0; i < s.size(); ++i) d[i] = fabs((double)s[i] - med);
+  std::nth_element(d.begin(), d.begin() + h, d.end());
+  const double mad = std::max(d[h], 20.0);
+  o->median = med;
+  o->mad = mad;
+  o->min_insert = (uint64_t)std::max(med - 3.5 * mad, 50.0);
+  o->max_insert = (uint64_t)(med + 3.5 * mad);
+}
+
+extern "C" int fb_map_paired_reads(fb_ctx* c, const fb_graph_view* g, const fb_bank* b, uint32_t k,
+                                   fb_pair_map** out) {
+  if (!c || !b || !out) return set_err(c, FB_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (int r = check_view(c, g)) return r;
+  if (b->ctx != c) return set_err(c, FB_ERR_INVALID, "the bank belongs to another context");
+  if (k < 1 || k > 127) return set_err(c, FB_ERR_INVALID, "k=%u outside 1..127", k);
+  if (g->n_nodes >= (1ull << 31)) return set_err(c, FB_ERR_INVALID, "too many unitigs");
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+  // every unitig packed as one read (inactive ones too, so ids stay unitig ids)
+  const uint64_t U = g->n_nodes;
+  std::vector<uint8_t> cat;
+  std::vector<uint32_t> lens(U);
+  uint64_t bases = 0;
+  for (uint64_t i = 0; i < U; ++i) bases += lens[i] = g->lengths[i];
+  cat.reserve(bases);
+  for (uint64_t i = 0; i < U; ++i) cat.insert(cat.end(), g->seq + g->offsets[i], g->seq + g->offsets[i] + lens[i]);
+  std::vector<uint64_t> words((bases + 31) / 32 + 1, 0), offs(U);
+  if (U) fb_pack_ascii(cat.data(), lens.data(), U, words.data(), offs.data());
+  // index_terminal_kmers windows (mapper.rs:99-131)
+  std::vector<uint32_t> ru, rp0;
+  std::vector<uint64_t> rpre(1, 0);
+  auto range = [&](uint32_t u, uint32_t p0, uint64_t n) {
+    if (!n) return;
+    ru.push_back(u);
+    rp0.push_back(p0);
+    rpre.push_back(rpre.back() + n);
+  };
+  for (uint64_t i = 0; i < U; ++i) {
+    if ((view_flags(g, i) & 1) || lens[i] < k) continue;
+    const uint64_t t = lens[i] - k + 1;
+    if (t <= 2000) {
+      range((uint32_t)i, 0, t);
+    } else {
+      range((uint32_t)i, 0, 1000);
+      range((uint32_t)i, (uint32_t)(t - 1000), 1000);
+    }
+  }
+  MapArgs a;
+  a.k = k;
+  DBuf ud, uo, ul, dru, drp, dpre, dkeys, dcnt, dval, dspec;
+  fb_arena_view uv{words.data(), (uint64_t)words.size(), offs.data(), lens.data(), U};
+  uint64_t unr = 0;
+  if (int r = upload_reads(c, &uv, 1, ud, uo, ul, &unr, nullptr, nullptr, nullptr)) return r;
+  a.useq.data = ud.as<uint64_t>();
+  a.useq.offs = uo.as<uint64_t>();
+  a.useq.lens = ul.as<uint32_t>();
+  a.useq.n_reads = U;
+  a.ulen = ul.as<uint32_t>();
+  if (int r = to_device(c, dru, ru.data(), ru.size() * 4)) return r;
+  if (int r = to_device(c, drp, rp0.data(), rp0.size() * 4)) return r;
+  if (int r = to_device(c, dpre, rpre.data(), rpre.size() * 8)) return r;
+  a.rng_u = dru.as<uint32_t>();
+  a.rng_p0 = drp.as<uint32_t>();
+  a.rpre = dpre.as<uint64_t>();
+  a.n_ranges = (uint32_t)ru.size();
+  a.n_windows = rpre.back();
+  uint32_t log2cap = 10;
+  while ((1ull << log2cap) < 2 * a.n_windows) ++log2cap;
+  a.log2cap = log2cap;
+  if (int r = dkeys.get(c, (1ull << log2cap) * 8)) return r;
+  if (int r = dcnt.get(c, (1ull << log2cap) * 4)) return r;
+  if (int r = dval.get(c, (1ull << log2cap) * 8)) return r;
+  if (int r = dspec.get(c, 256)) return r;
+  a.keys = dkeys.as<uint64_t>();
+  a.cnt = dcnt.as<uint32_t>();
+  a.val = dval.as<uint64_t>();
+  a.special_cnt = dspec.as<uint32_t>();
+  a.special_val = reinterpret_cast<uint64_t*>(dspec.as<uint8_t>() + 64);
+  HIP_TRY(c, launch_map_clear(a, s));
+  HIP_TRY(c, launch_map_insert(a, s));
+  // map_paired_reads (mapper.rs:138-253)
+  const uint64_t n = std::min(b->ar_reads[1], b->ar_reads[2]);
+  if (n >= (1ull << 31)) return set_err(c, FB_ERR_INVALID, "mapping takes < 2^31 pairs");
+  a.reads = b->view();
+  a.first_f = b->ar_first[1];
+  a.first_r = b->ar_first[2];
+  a.n_pairs = n;
+  DBuf fs, fl, ps, pl, sp, u1, u2, d1, d2, tmp, dsz, dln;
+  for (DBuf* d : {&fs, &fl, &ps, &pl})
+    if (int r = d->get(c, (n + 1) * 4)) return r;
+  for (DBuf* d : {&sp, &u1, &u2, &d1, &d2})
+    if (int r = d->get(c, std::max<uint64_t>(n, 1) * 4)) return r;
+  a.f_size = fs.as<uint32_t>();
+  a.f_link = fl.as<uint32_t>();
+  a.pos_size = ps.as<uint32_t>();
+  a.pos_link = pl.as<uint32_t>();
+  a.span = sp.as<uint32_t>();
+  a.lu1 = u1.as<uint32_t>();
+  a.lu2 = u2.as<uint32_t>();
+  a.ld1 = d1.as<uint32_t>();
+  a.ld2 = d2.as<uint32_t>();
+  HIP_TRY(c, launch_map_pairs(a, s));
+  size_t tb = 0;
+  HIP_TRY(c, cg_exclusive_sum_u32(nullptr, nullptr, n + 1, nullptr, &tb, s));
+  if (int r = tmp.get(c, std::max<size_t>(tb, 16))) return r;
+  tb = tmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u32(a.f_size, a.pos_size, n + 1, tmp.p, &tb, s));
+  tb = tmp.bytes;
+  HIP_TRY(c, cg_exclusive_sum_u32(a.f_link, a.pos_link, n + 1, tmp.p, &tb, s));
+  uint32_t cnt[2] = {0, 0};
+  HIP_TRY(c, hipMemcpyAsync(&cnt[0], a.pos_size + n, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&cnt[1], a.pos_link + n, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (int r = dsz.get(c, std::max<uint64_t>(cnt[0], 1) * 4)) return r;
+  if (int r = dln.get(c, std::max<uint64_t>(cnt[1], 1) * 2 * sizeof(PairLinkRec))) return r;
+  a.sizes = dsz.as<uint32_t>();
+  a.links = dln.as<PairLinkRec>();
+  HIP_TRY(c, launch_map_compact(a, s));
+  auto m = std::make_unique<fb_pair_map>();
+  m->sizes.resize(cnt[0]);
+  m->links.resize(2ull * cnt[1]);
+  static_assert(sizeof(PairLinkRec) == sizeof(fb_pair_link), "link record layout");
+  if (cnt[0]) HIP_TRY(c, hipMemcpyAsync(m->sizes.data(), a.sizes, cnt[0] * 4ull, hipMemcpyDeviceToHost, s));
+  if (cnt[1])
+    HIP_TRY(c, hipMemcpyAsync(m->links.data(), a.links, 2ull * cnt[1] * sizeof(PairLinkRec), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  insert_dist(m->sizes, &m->dist);
+  *out = m.release();
+  return FB_OK;
+}
+
+extern "C" uint64_t fb_pair_map_link_count(const fb_pair_map* m) { return m ? m->links.size() : 0; }
+extern "C" int fb_pair_map_links(const fb_pair_map* m, fb_pair_link* out) {
+  if (!m || (!out && !m->links.empty())) return FB_ERR_INVALID;
+  if (!m->links.empty()) memcpy(out, m->links.data(), m->links.size() * sizeof(fb_pair_link));
+  return FB_OK;
+}
+extern "C" int fb_pair_map_sizes(const fb_pair_map* m, uint32_t* out) {
+  if (!m || (!out && !m->sizes.empty())) return FB_ERR_INVALID;
+  if (!m->sizes.empty()) memcpy(out, m->sizes.data(), m->sizes.size() * 4);
+  return FB_OK;
+}
+extern "C" int fb_pair_map_dist(const fb_pair_map* m, fb_insert_size_dist* out) {
+  if (!m || !out) return FB_ERR_INVALID;
+  *out = m->dist;
+  return FB_OK;
+}
+extern "C" void fb_pair_map_destroy(fb_pair_map* m) { delete m; }

@@ -22,6 +22,7 @@
  *   fb_filter_active_reads  filter_active_reads          src/graph/local_multik.rs:259-333
  *   fb_upgrade_multik   upgrade_multik_ext<K2>           src/graph/local_multik.rs:38-147
  *   fb_ingest_*         ingest_reads_ext_with_seed       src/ingest/trim_merge.rs:17-417
+ *   fb_map_paired_reads map_paired_reads<K>              src/resolve/mapper.rs:89-279
  *   fb_fastx_*          open_fastx + record parsing      src/ingest/trim_merge.rs:233-252
  *   fb_estimate_genome  estimate_genome                  src/ingest/estimator.rs:286-440
  *   fb_count_stats_from_lengths  count_fastx_exact (length part)  src/ingest/estimator.rs:167-283
@@ -293,6 +294,37 @@ typedef struct {
 int fb_upgrade_multik(fb_ctx* ctx, const fb_bank* bank, const fb_graph_view* prev, uint32_t k2, uint64_t genome_size,
                       const fb_build_params* params, fb_multik_result* res, fb_graph** out_graph, fb_solid** out_solid,
                       fb_bank** out_active);
+
+/* ---- PE mapper (src/resolve/mapper.rs:89-279) --------------------------------
+ * map_paired_reads<K>: index_terminal_kmers over the graph's active unitigs
+ * (every window of a unitig with <= 2000, else its first and last 1000;
+ * keys seen once), then each pair of pe_forward / pe_reverse mapped by the
+ * first unique window among steps 0, 20, .., 100 of each mate.  The
+ * reference's PairLinkMatrix is a hash map from (u1, u2) to a vector of
+ * ReadPairEvidence; here it is the flat list of evidence records in pair
+ * order (two per linked pair: (u1, twin(u2)) then (u2, twin(u1)), as
+ * DirUnitigIds id << 1 | is_rc), which regroups into the same map. */
+typedef struct {
+  uint32_t u1, u2;            /* DirUnitigIds of the link */
+  uint32_t pair_id;           /* ReadPairEvidence */
+  uint32_t dist_u1_exit;
+  uint32_t dist_u2_entry;
+  uint16_t r1_len, r2_len;
+} fb_pair_link;
+/* InsertSizeDist (mapper.rs:12-30,255-276) and the sizes it came from. */
+typedef struct {
+  double median, mad;
+  uint64_t min_insert, max_insert;
+  uint64_t n_sizes;
+} fb_insert_size_dist;
+typedef struct fb_pair_map fb_pair_map;
+int fb_map_paired_reads(fb_ctx* ctx, const fb_graph_view* g, const fb_bank* bank, uint32_t k, fb_pair_map** out);
+uint64_t fb_pair_map_link_count(const fb_pair_map* m);
+int fb_pair_map_links(const fb_pair_map* m, fb_pair_link* out);
+/* Intra-unitig insert sizes in pair order. */
+int fb_pair_map_sizes(const fb_pair_map* m, uint32_t* out);
+int fb_pair_map_dist(const fb_pair_map* m, fb_insert_size_dist* out);
+void fb_pair_map_destroy(fb_pair_map* m);
 
 /* ---- Ingest front end (src/ingest/trim_merge.rs:17-417) ---------------------
  * Parsed records: record i has seq[seq_offsets[i] .. + seq_lengths[i]) and

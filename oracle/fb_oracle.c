@@ -2095,3 +2095,175 @@ void orc_pair_map_free(orc_pair_map* m) {
     free(m->sizes);
     memset(m, 0, sizeof *m);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Consensus polishing, consensus.rs:64-271.  Seeds are a sorted array of
+ * (hash, node, pos, count); the reference's FxHashMaps answer the same
+ * membership and unique-value queries. */
+typedef struct { uint64_t h; uint32_t id, pos; uint32_t cnt; } seed_t;
+static int seed_cmp(const void* a, const void* b) {
+    const seed_t *x = (const seed_t*)a, *y = (const seed_t*)b;
+    return x->h < y->h ? -1 : x->h > y->h;
+}
+static seed_t* seed_find(seed_t* t, uint64_t n, uint64_t h) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (t[mid].h < h) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && t[lo].h == h ? t + lo : NULL;
+}
+static int rc_exact(const uint8_t* anchor, const uint8_t* rk, uint32_t k) {
+    for (uint32_t j = 0; j < k; j++) {
+        uint8_t c;
+        switch (rk[k - 1 - j]) {
+            case 'A': case 'a': c = 'T'; break;
+            case 'T': case 't': c = 'A'; break;
+            case 'C': case 'c': c = 'G'; break;
+            case 'G': case 'g': c = 'C'; break;
+            default: return 0;
+        }
+        if (!eq_nocase(anchor[j], c)) return 0;
+    }
+    return 1;
+}
+int orc_polish_contigs(const orc_graph_view* g, const orc_arena* reads, int n_arenas, uint32_t min_output_len,
+                       orc_polish_out* out) {
+    memset(out, 0, sizeof *out);
+    const uint32_t k = 31;
+    const uint64_t U = g->n_nodes;
+    uint64_t total = 0, nsamp = 0;
+    for (uint64_t i = 0; i < U; i++) total += g->lengths[i];
+#define ELIG(i) (!(g->flags && (g->flags[i] & 1)) && g->lengths[i] >= min_output_len)
+    for (uint64_t i = 0; i < U; i++)
+        if (ELIG(i) && g->lengths[i] >= k) nsamp += (g->lengths[i] - k) / 25 + 1;
+    seed_t* sd = (seed_t*)malloc((nsamp ? nsamp : 1) * sizeof(seed_t));
+    uint64_t n = 0;
+    for (uint64_t i = 0; i < U; i++) {
+        if (!ELIG(i) || g->lengths[i] < k) continue;
+        const uint8_t* s = g->seq + g->offsets[i];
+        for (uint32_t p = 0; p + k <= g->lengths[i]; p += 25) {
+            sd[n].h = orc_nthash_canonical(s + p, k);
+            sd[n].id = (uint32_t)i;
+            sd[n].pos = p;
+            sd[n].cnt = 0;
+            n++;
+        }
+    }
+    /* insert order: the last sampled (id, pos) of a hash is the stored one */
+    for (uint64_t i = 0; i < n; i++) sd[i].cnt = (uint32_t)i;
+    qsort(sd, n, sizeof(seed_t), seed_cmp);
+    uint64_t u = 0;
+    for (uint64_t i = 0; i < n;) {
+        uint64_t j = i, last = i;
+        while (j < n && sd[j].h == sd[i].h) {
+            if (sd[j].cnt > sd[last].cnt) last = j;
+            j++;
+        }
+        sd[u] = sd[last];
+        sd[u].cnt = 0;
+        u++;
+        i = j;
+    }
+    for (uint64_t i = 0; i < U; i++) {
+        if (!ELIG(i) || g->lengths[i] < k) continue;
+        const uint8_t* s = g->seq + g->offsets[i];
+        for (uint32_t p = 0; p + k <= g->lengths[i]; p++) {
+            seed_t* e = seed_find(sd, u, orc_nthash_canonical(s + p, k));
+            if (e && e->cnt < UINT32_MAX) e->cnt++;
+        }
+    }
+    uint64_t v = 0;
+    for (uint64_t i = 0; i < u; i++)
+        if (sd[i].cnt == 1) sd[v++] = sd[i];
+    /* pileups */
+    uint64_t* poff = (uint64_t*)calloc(U + 1, 8);
+    for (uint64_t i = 0; i < U; i++) poff[i + 1] = poff[i] + (ELIG(i) ? g->lengths[i] : 0);
+    uint16_t* pile = (uint16_t*)calloc(poff[U] * 4 + 4, 2);
+    uint32_t maxlen = 1;
+    for (int a = 0; a < n_arenas; a++)
+        for (uint64_t i = 0; i < reads[a].n_reads; i++)
+            if (reads[a].lengths[i] > maxlen) maxlen = reads[a].lengths[i];
+    uint8_t* r = (uint8_t*)malloc(maxlen);
+    for (int a = 0; a < n_arenas; a++)
+        for (uint64_t ri = 0; ri < reads[a].n_reads; ri++) {
+            const uint32_t len = reads[a].lengths[ri];
+            if (len < k) continue;
+            orc_arena_extract_ascii(&reads[a], ri, r);
+            int had = 0, orient = -1;
+            uint32_t cid = 0, cpos = 0, rpos = 0;
+            for (uint32_t p = 0; p + k <= len; p += 30) {
+                const seed_t* e = seed_find(sd, v, orc_nthash_canonical(r + p, k));
+                if (!e) continue;
+                had = 1;
+                const uint8_t* anc = g->seq + g->offsets[e->id] + e->pos;
+                int fwd = 1;
+                for (uint32_t j = 0; j < k && fwd; j++) fwd = eq_nocase(anc[j], r[p + j]);
+                if (fwd || rc_exact(anc, r + p, k)) {
+                    orient = fwd ? 0 : 1;
+                    cid = e->id;
+                    cpos = e->pos;
+                    rpos = p;
+                    break;
+                }
+            }
+            if (orient < 0) {
+                if (had) out->stats.rejected_anchors++;
+                continue;
+            }
+            uint16_t* pl = pile + poff[cid] * 4;
+            const int64_t plen = g->lengths[cid];
+            int voted = 0;
+            for (uint32_t idx = 0; idx < len; idx++) {
+                const int64_t t = orient == 0 ? (int64_t)cpos - rpos + idx : (int64_t)cpos + k - 1 + rpos - idx;
+                if (t < 0 || t >= plen) continue;
+                uint32_t code = A2B[r[idx]];
+                if (orient == 1) code = 3 - code;
+                if (pl[t * 4 + code] < 65535) pl[t * 4 + code]++;
+                if (orient == 0) out->stats.fwd_pileup_bases++;
+                else out->stats.rev_pileup_bases++;
+                voted = 1;
+            }
+            if (voted) {
+                if (orient == 0) out->stats.fwd_mapped_reads++;
+                else out->stats.rev_mapped_reads++;
+            }
+        }
+    /* consensus */
+    out->seq = (uint8_t*)malloc(total ? total : 1);
+    uint64_t o = 0;
+    for (uint64_t i = 0; i < U; i++) {
+        const uint8_t* s = g->seq + g->offsets[i];
+        memcpy(out->seq + o, s, g->lengths[i]);
+        if (ELIG(i)) {
+            const uint16_t* pl = pile + poff[i] * 4;
+            for (uint32_t p = 0; p < g->lengths[i]; p++) {
+                const uint16_t* c = pl + (uint64_t)p * 4;
+                const uint16_t tot = (uint16_t)(c[0] + c[1] + c[2] + c[3]); /* u16 sum wraps */
+                if (tot < 5) continue;
+                uint32_t mi = 0;
+                for (uint32_t b = 1; b < 4; b++)
+                    if (c[b] > c[mi]) mi = b;
+                if ((double)c[mi] / (double)tot > 0.75) {
+                    const uint8_t nb = TWO_BIT_TO_ASCII[mi];
+                    if (!eq_nocase(out->seq[o + p], nb)) {
+                        out->seq[o + p] = nb;
+                        out->stats.bases_modified++;
+                    }
+                }
+            }
+        }
+        o += g->lengths[i];
+    }
+#undef ELIG
+    free(sd);
+    free(poff);
+    free(pile);
+    free(r);
+    return 0;
+}
+void orc_polish_free(orc_polish_out* o) {
+    free(o->seq);
+    memset(o, 0, sizeof *o);
+}

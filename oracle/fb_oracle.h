@@ -293,6 +293,27 @@ void orc_pair_map_free(orc_pair_map* m);
 /* InsertSizeDist from sizes (mapper.rs:255-276); fewer than 20: the default. */
 void orc_insert_dist_of(const uint32_t* sizes, uint64_t n, orc_insert_dist* out);
 
+/* ---- Consensus polishing (src/polish/consensus.rs:64-271) -------------------
+ * Seeds: canonical ntHash (k 31) of every 25th window of the output-eligible
+ * unitigs (not deleted, >= min_output_len), kept when the hash occurs once
+ * over all their windows.  Reads (the bank's merged reads, then pe_forward)
+ * anchor at the first seed among windows 0, 30, 60, .. that equals the
+ * contig k-mer forward or as its reverse complement; each anchored read votes
+ * its bases into u16 saturating pileups; a position with coverage >= 5 takes
+ * the dominant base when it has > 75%.  out->seq receives every node's
+ * sequence after polishing (the node's own bytes when unchanged). */
+typedef struct {
+    uint64_t bases_modified, fwd_mapped_reads, rev_mapped_reads, fwd_pileup_bases, rev_pileup_bases,
+        rejected_anchors;
+} orc_polish_stats;
+typedef struct {
+    uint8_t* seq; /* concatenated in node order, lengths unchanged */
+    orc_polish_stats stats;
+} orc_polish_out;
+int orc_polish_contigs(const orc_graph_view* g, const orc_arena* reads, int n_arenas, uint32_t min_output_len,
+                       orc_polish_out* out);
+void orc_polish_free(orc_polish_out* o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,15 @@ class PairMap(C.Structure):
                 ("dist", InsertDist)]
 
 
+class PolishStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("bases_modified", "fwd_mapped_reads", "rev_mapped_reads",
+                                           "fwd_pileup_bases", "rev_pileup_bases", "rejected_anchors")]
+
+
+class PolishOut(C.Structure):
+    _fields_ = [("seq", u8p), ("stats", PolishStats)]
+
+
 _SIGS = [
     ("orc_splitmix64", C.c_uint64, [C.c_uint64]),
     ("orc_rev_comp_word", C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -169,6 +178,9 @@ _SIGS = [
                                        C.POINTER(PairMap)]),
     ("orc_pair_map_free", None, [C.POINTER(PairMap)]),
     ("orc_insert_dist_of", None, [u32p, C.c_uint64, C.POINTER(InsertDist)]),
+    ("orc_polish_contigs", C.c_int, [C.POINTER(GraphView), C.POINTER(Arena), C.c_int, C.c_uint32,
+                                     C.POINTER(PolishOut)]),
+    ("orc_polish_free", None, [C.POINTER(PolishOut)]),
 ]
 
 _LIB = None
@@ -652,3 +664,27 @@ def insert_dist(sizes) -> dict:
     d = InsertDist()
     lib().orc_insert_dist_of(_p(s, C.c_uint32), s.size, C.byref(d))
     return {"median": d.median, "mad": d.mad, "min_insert": int(d.min_insert), "max_insert": int(d.max_insert)}
+
+
+# ---- consensus polishing (consensus.rs:64-271) --------------------------------
+
+def polish_contigs(graph: dict, read_arenas: list, min_output_len: int) -> dict:
+    """polish_contigs over a graph dict (seqs; optional deleted) and read
+    arenas (the reference passes merged reads then pe_forward) -> {"seqs":
+    every node's sequence after polishing, "stats": PolishStats fields}."""
+    gv = _graph_view(graph, 31)
+    ars = [arena(*a) for a in read_arenas]
+    arr = (Arena * max(1, len(ars)))(*ars)
+    out = PolishOut()
+    rc = lib().orc_polish_contigs(C.byref(gv), arr, len(ars), min_output_len, C.byref(out))
+    if rc:
+        raise ValueError(f"orc_polish_contigs failed ({rc})")
+    try:
+        lens = [len(bytes(x)) for x in graph["seqs"]]
+        raw = C.string_at(out.seq, sum(lens)) if sum(lens) else b""
+        offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        seqs = [raw[offs[i]:offs[i + 1]] for i in range(len(lens))]
+        st = {f: int(getattr(out.stats, f)) for f, _ in PolishStats._fields_}
+    finally:
+        lib().orc_polish_free(C.byref(out))
+    return {"seqs": seqs, "stats": st}

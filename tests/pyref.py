@@ -609,3 +609,73 @@ def map_paired_reads(unitigs: list, pairs: list, k: int):
             links.append((u1, u2f, i, d1, d2, len(r1), len(r2)))
             links.append((u2f ^ 1, u1 ^ 1, i, d2, d1, len(r2), len(r1)))
     return links, sizes
+
+
+# ---- consensus polishing (consensus.rs:64-271), on strings -----------------
+
+def polish(seqs: list, eligible: list, reads: list):
+    """seqs: node sequences; eligible: bools; reads: strings in order.
+    -> (polished sequences, stats dict)."""
+    k = 31
+    sampled, last = set(), {}
+    for i, s in enumerate(seqs):
+        if eligible[i]:
+            for p in range(0, len(s) - k + 1, 25):
+                h = nthash_canonical(s[p:p + k])
+                sampled.add(h)
+                last[h] = (i, p)
+    occ = {h: 0 for h in sampled}
+    for i, s in enumerate(seqs):
+        if eligible[i]:
+            for p in range(len(s) - k + 1):
+                h = nthash_canonical(s[p:p + k])
+                if h in occ:
+                    occ[h] += 1
+    seeds = {h: v for h, v in last.items() if occ[h] == 1}
+    piles = {i: [[0, 0, 0, 0] for _ in s] for i, s in enumerate(seqs) if eligible[i]}
+    st = dict(bases_modified=0, fwd_mapped_reads=0, rev_mapped_reads=0, fwd_pileup_bases=0, rev_pileup_bases=0,
+              rejected_anchors=0)
+    for r in reads:
+        if len(r) < k:
+            continue
+        hit, anchor = False, None
+        for p in range(0, len(r) - k + 1, 30):
+            h = nthash_canonical(r[p:p + k])
+            if h in seeds:
+                hit = True
+                cid, cp = seeds[h]
+                a = seqs[cid][cp:cp + k]
+                if a.upper() == r[p:p + k].upper():
+                    anchor = (cid, cp, p, "f")
+                    break
+                if a.upper() == revcomp(r[p:p + k].upper()):
+                    anchor = (cid, cp, p, "r")
+                    break
+        if anchor is None:
+            st["rejected_anchors"] += int(hit)
+            continue
+        cid, cp, rp, o = anchor
+        pile = piles[cid]
+        voted = False
+        for idx, b in enumerate(r):
+            t = cp - rp + idx if o == "f" else cp + k - 1 + rp - idx
+            if 0 <= t < len(pile):
+                code = "ACGT".index(b) if o == "f" else 3 - "ACGT".index(b)
+                pile[t][code] = min(pile[t][code] + 1, 65535)
+                st["fwd_pileup_bases" if o == "f" else "rev_pileup_bases"] += 1
+                voted = True
+        if voted:
+            st["fwd_mapped_reads" if o == "f" else "rev_mapped_reads"] += 1
+    out = []
+    for i, s in enumerate(seqs):
+        s = list(s)
+        if eligible[i]:
+            for p, c in enumerate(piles[i]):
+                tot = sum(c) % 65536
+                if tot >= 5:
+                    m = max(range(4), key=lambda b: (c[b], -b))
+                    if c[m] / tot > 0.75 and s[p].upper() != "ACGT"[m]:
+                        s[p] = "ACGT"[m]
+                        st["bases_modified"] += 1
+        out.append("".join(s))
+    return out, st

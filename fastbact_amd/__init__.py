@@ -30,6 +30,7 @@ from .api import (  # noqa: F401
     ingest_files,
     ingest_records,
     map_paired_reads,
+    polish_contigs,
     splitmix64,
     synth_isolate,
     read_fastx,

@@ -91,6 +91,11 @@ class InsertSizeDist(C.Structure):
                 ("n_sizes", C.c_uint64)]
 
 
+class PolishStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("bases_modified", "fwd_mapped_reads", "rev_mapped_reads",
+                                           "fwd_pileup_bases", "rev_pileup_bases", "rejected_anchors")]
+
+
 class BuildParams(C.Structure):
     _fields_ = [
         ("k", C.c_uint32),
@@ -298,6 +303,8 @@ SIGNATURES = [
                                   C.POINTER(IngestStats)]),
     ("fb_graph_view_of", C.c_int, [vp, C.POINTER(GraphView)]),
     ("fb_map_paired_reads", C.c_int, [vp, C.POINTER(GraphView), vp, C.c_uint32, C.POINTER(vp)]),
+    ("fb_polish_contigs", C.c_int, [vp, C.POINTER(GraphView), vp, C.c_uint32, C.POINTER(C.c_uint8),
+                                    C.POINTER(PolishStats)]),
     ("fb_pair_map_link_count", C.c_uint64, [vp]),
     ("fb_pair_map_links", C.c_int, [vp, C.POINTER(PairLink)]),
     ("fb_pair_map_sizes", C.c_int, [vp, C.POINTER(C.c_uint32)]),

@@ -604,6 +604,21 @@ def map_paired_reads(ctx: Context, bank: DeviceBank, graph: dict, k: int) -> dic
         ctx.lib.fb_pair_map_destroy(h)
 
 
+def polish_contigs(ctx: Context, bank: DeviceBank, graph: dict, min_output_len: int) -> tuple:
+    """polish_contigs (consensus.rs:64-271) on the device with the bank's
+    merged and pe_forward reads -> (every node's sequence after polishing,
+    PolishStats dict)."""
+    ga = _GraphArrays(graph, 31)
+    total = int(ga.lens.sum(dtype=np.uint64))
+    out = np.zeros(max(total, 1), np.uint8)
+    st = _lib.PolishStats()
+    ctx.check(ctx.lib.fb_polish_contigs(ctx.h, C.byref(ga.view), bank.h, min_output_len, _ptr(out, C.c_uint8),
+                                        C.byref(st)), "fb_polish_contigs")
+    raw = out[:total].tobytes()
+    offs = np.concatenate([[0], np.cumsum(ga.lens, dtype=np.uint64)]).astype(np.int64)
+    return [raw[offs[i]:offs[i + 1]] for i in range(len(ga.lens))], {f: int(getattr(st, f)) for f, _ in st._fields_}
+
+
 # ---------------------------------------------------------------------------
 # Ingest front end (trim_merge.rs:17-417)
 

@@ -410,6 +410,42 @@ hipError_t launch_map_insert(const MapArgs& a, hipStream_t s);
 hipError_t launch_map_pairs(const MapArgs& a, hipStream_t s);
 hipError_t launch_map_compact(const MapArgs& a, hipStream_t s);
 
+// Consensus polishing (consensus.rs:64-271), fb_polish.hip.
+struct PolArgs {
+  // node sequences (ASCII, concatenated; soff per node) and lengths
+  const uint8_t* seq = nullptr;
+  const uint64_t* soff = nullptr;
+  const uint32_t* ulen = nullptr;
+  // windows: range r covers windows rpre[r] .. rpre[r+1]-1 of unitig rng_u[r], every rng_step bases from 0
+  const uint32_t* rng_u = nullptr;
+  const uint64_t* rpre = nullptr;
+  uint32_t n_ranges = 0, rng_step = 1;
+  uint64_t n_windows = 0;
+  // seed table; the hash kEmpty lives in special (sampled flag, count) and special_val
+  uint64_t* keys = nullptr;
+  uint32_t* cnt = nullptr;
+  uint64_t* val = nullptr;
+  uint32_t log2cap = 0;
+  uint32_t* special = nullptr;
+  uint64_t* special_val = nullptr;
+  // reads: n_first of them from first0, the rest from first1
+  DevReads reads;
+  uint64_t n_reads = 0, n_first = 0, first0 = 0, first1 = 0;
+  // pileups of the eligible nodes: 4 u32 per base from poff[node]
+  uint32_t* pile = nullptr;
+  const uint64_t* poff = nullptr;
+  const uint32_t* elig = nullptr;  // eligible node ids, ascending
+  uint32_t n_elig = 0;
+  uint64_t n_pile = 0;
+  uint8_t* out = nullptr;            // polished sequences, laid out as seq
+  unsigned long long* stats = nullptr;  // fwd reads, rev reads, fwd bases, rev bases, rejected, modified
+};
+hipError_t launch_pol_clear(const PolArgs& a, hipStream_t s);
+hipError_t launch_pol_sample(const PolArgs& a, hipStream_t s);
+hipError_t launch_pol_count(const PolArgs& a, hipStream_t s);
+hipError_t launch_pol_reads(const PolArgs& a, hipStream_t s);
+hipError_t launch_pol_consensus(const PolArgs& a, hipStream_t s);
+
 hipError_t cg_exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
                                 hipStream_t s);
 hipError_t cg_exclusive_sum_u64(const uint32_t* in, uint64_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,

@@ -3573,3 +3573,122 @@ extern "C" int fb_pair_map_dist(const fb_pair_map* m, fb_insert_size_dist* out) 
   return FB_OK;
 }
 extern "C" void fb_pair_map_destroy(fb_pair_map* m) { delete m; }
+
+// ---------------------------------------------------------------------------
+// Consensus polishing (consensus.rs:64-271): seeds, anchors, pileups and the
+// consensus on the device (fb_polish.hip).
+
+extern "C" int fb_polish_contigs(fb_ctx* c, const fb_graph_view* g, const fb_bank* b, uint32_t min_output_len,
+                                 uint8_t* out_seq, fb_polish_stats* st) {
+  if (!c || !b || !st) return set_err(c, FB_ERR_INVALID, "null argument");
+  if (int r = check_view(c, g)) return r;
+  if (b->ctx != c) return set_err(c, FB_ERR_INVALID, "the bank belongs to another context");
+  if (g->n_nodes >= (1ull << 31)) return set_err(c, FB_ERR_INVALID, "too many unitigs");
+  const uint64_t U = g->n_nodes;
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < U; ++i) total += g->lengths[i];
+  if (total && !out_seq) return set_err(c, FB_ERR_INVALID, "null output");
+  memset(st, 0, sizeof *st);
+  if (int r = ctx_current(c)) return r;
+  hipStream_t s = c->stream;
+  const uint32_t K = 31;
+  // nodes as one contiguous ASCII buffer in node order
+  std::vector<uint8_t> cat;
+  cat.reserve(total);
+  std::vector<uint64_t> soff(U);
+  std::vector<uint32_t> lens(U);
+  for (uint64_t i = 0; i < U; ++i) {
+    soff[i] = cat.size();
+    lens[i] = g->lengths[i];
+    cat.insert(cat.end(), g->seq + g->offsets[i], g->seq + g->offsets[i] + lens[i]);
+  }
+  std::vector<uint32_t> elig;
+  std::vector<uint64_t> poff(U + 1, 0);
+  for (uint64_t i = 0; i < U; ++i) {
+    const bool e = !(view_flags(g, i) & 1) && lens[i] >= min_output_len;
+    poff[i + 1] = poff[i] + (e ? lens[i] : 0);
+    if (e) elig.push_back((uint32_t)i);
+  }
+  auto ranges = [&](uint32_t step, std::vector<uint32_t>* ru, std::vector<uint64_t>* pre) {
+    pre->assign(1, 0);
+    for (uint32_t u : elig)
+      if (lens[u] >= K) {
+        ru->push_back(u);
+        pre->push_back(pre->back() + (lens[u] - K) / step + 1);
+      }
+  };
+  std::vector<uint32_t> su, au;
+  std::vector<uint64_t> spre, apre;
+  ranges(25, &su, &spre);
+  ranges(1, &au, &apre);
+  PolArgs a;
+  DBuf dseq, dsoff, dlen, dsu, dspre, dau, dapre, dkeys, dcnt, dval, dspec, dpile, dpoff, delig, dout, dstat;
+  if (int r = to_device(c, dseq, cat.data(), cat.size())) return r;
+  if (int r = to_device(c, dsoff, soff.data(), U * 8)) return r;
+  if (int r = to_device(c, dlen, lens.data(), U * 4)) return r;
+  if (int r = to_device(c, dsu, su.data(), su.size() * 4)) return r;
+  if (int r = to_device(c, dspre, spre.data(), spre.size() * 8)) return r;
+  if (int r = to_device(c, dau, au.data(), au.size() * 4)) return r;
+  if (int r = to_device(c, dapre, apre.data(), apre.size() * 8)) return r;
+  a.seq = dseq.as<uint8_t>();
+  a.soff = dsoff.as<uint64_t>();
+  a.ulen = dlen.as<uint32_t>();
+  uint32_t log2cap = 10;
+  while ((1ull << log2cap) < 2 * spre.back()) ++log2cap;
+  a.log2cap = log2cap;
+  if (int r = dkeys.get(c, (1ull << log2cap) * 8)) return r;
+  if (int r = dcnt.get(c, (1ull << log2cap) * 4)) return r;
+  if (int r = dval.get(c, (1ull << log2cap) * 8)) return r;
+  if (int r = dspec.get(c, 256)) return r;
+  a.keys = dkeys.as<uint64_t>();
+  a.cnt = dcnt.as<uint32_t>();
+  a.val = dval.as<uint64_t>();
+  a.special = dspec.as<uint32_t>();
+  a.special_val = reinterpret_cast<uint64_t*>(dspec.as<uint8_t>() + 64);
+  HIP_TRY(c, launch_pol_clear(a, s));
+  a.rng_u = dsu.as<uint32_t>();
+  a.rpre = dspre.as<uint64_t>();
+  a.n_ranges = (uint32_t)su.size();
+  a.rng_step = 25;
+  a.n_windows = spre.back();
+  HIP_TRY(c, launch_pol_sample(a, s));
+  a.rng_u = dau.as<uint32_t>();
+  a.rpre = dapre.as<uint64_t>();
+  a.n_ranges = (uint32_t)au.size();
+  a.rng_step = 1;
+  a.n_windows = apre.back();
+  HIP_TRY(c, launch_pol_count(a, s));
+  // anchors and pileups: merged reads, then pe_forward (main.rs:466-469)
+  if (int r = dpile.get(c, std::max<uint64_t>(poff[U], 1) * 16)) return r;
+  if (int r = to_device(c, dpoff, poff.data(), (U + 1) * 8)) return r;
+  if (int r = to_device(c, delig, elig.data(), elig.size() * 4)) return r;
+  if (int r = dstat.get(c, 64)) return r;
+  a.pile = dpile.as<uint32_t>();
+  a.poff = dpoff.as<uint64_t>();
+  a.elig = delig.as<uint32_t>();
+  a.n_elig = (uint32_t)elig.size();
+  a.n_pile = poff[U];
+  a.stats = dstat.as<unsigned long long>();
+  HIP_TRY(c, launch_fill_u32(a.pile, std::max<uint64_t>(poff[U], 1) * 4, 0, s));
+  HIP_TRY(c, launch_fill_u32(dstat.as<uint32_t>(), 16, 0, s));
+  a.reads = b->view();
+  a.first0 = b->ar_first[0];
+  a.n_first = b->ar_reads[0];
+  a.first1 = b->ar_first[1];
+  a.n_reads = b->ar_reads[0] + b->ar_reads[1];
+  HIP_TRY(c, launch_pol_reads(a, s));
+  if (int r = to_device(c, dout, cat.data(), cat.size())) return r;
+  a.out = dout.as<uint8_t>();
+  HIP_TRY(c, launch_pol_consensus(a, s));
+  unsigned long long sums[6];
+  HIP_TRY(c, hipMemcpyAsync(sums, a.stats, sizeof sums, hipMemcpyDeviceToHost, s));
+  if (total) HIP_TRY(c, hipMemcpyAsync(out_seq, a.out, total, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  st->fwd_mapped_reads = sums[0];
+  st->rev_mapped_reads = sums[1];
+  st->fwd_pileup_bases = sums[2];
+  st->rev_pileup_bases = sums[3];
+  st->rejected_anchors = sums[4];
+  st->bases_modified = sums[5];
+  return FB_OK;
+}

@@ -23,6 +23,7 @@
  *   fb_upgrade_multik   upgrade_multik_ext<K2>           src/graph/local_multik.rs:38-147
  *   fb_ingest_*         ingest_reads_ext_with_seed       src/ingest/trim_merge.rs:17-417
  *   fb_map_paired_reads map_paired_reads<K>              src/resolve/mapper.rs:89-279
+ *   fb_polish_contigs   polish_contigs                   src/polish/consensus.rs:64-271
  *   fb_fastx_*          open_fastx + record parsing      src/ingest/trim_merge.rs:233-252
  *   fb_estimate_genome  estimate_genome                  src/ingest/estimator.rs:286-440
  *   fb_count_stats_from_lengths  count_fastx_exact (length part)  src/ingest/estimator.rs:167-283
@@ -325,6 +326,25 @@ int fb_pair_map_links(const fb_pair_map* m, fb_pair_link* out);
 int fb_pair_map_sizes(const fb_pair_map* m, uint32_t* out);
 int fb_pair_map_dist(const fb_pair_map* m, fb_insert_size_dist* out);
 void fb_pair_map_destroy(fb_pair_map* m);
+
+/* ---- Consensus polishing (src/polish/consensus.rs:64-271) -------------------
+ * polish_contigs on the graph's output-eligible nodes (not deleted, at least
+ * min_output_len bases) with the bank's merged reads and then pe_forward
+ * reads, as main.rs:466-470 passes them: unique 31-mer ntHash seeds every 25
+ * bp, read anchors at windows 0, 30, 60, .., saturating u16 pileups, and the
+ * dominant base where coverage >= 5 and it has > 75%.  out_seq receives every
+ * node's sequence in node order (the sum of g->lengths bytes; nodes that are
+ * not eligible are copied unchanged). */
+typedef struct {
+  uint64_t bases_modified;
+  uint64_t fwd_mapped_reads;
+  uint64_t rev_mapped_reads;
+  uint64_t fwd_pileup_bases;
+  uint64_t rev_pileup_bases;
+  uint64_t rejected_anchors;
+} fb_polish_stats;
+int fb_polish_contigs(fb_ctx* ctx, const fb_graph_view* g, const fb_bank* bank, uint32_t min_output_len,
+                      uint8_t* out_seq, fb_polish_stats* stats);
 
 /* ---- Ingest front end (src/ingest/trim_merge.rs:17-417) ---------------------
  * Parsed records: record i has seq[seq_offsets[i] .. + seq_lengths[i]) and

@@ -326,6 +326,81 @@ def bench_compaction(ctx, dev, F, cpu: bool) -> dict:
         out["cpu_baseline"] = "oracle compaction (single thread) of the same solid set"
         out["equal_to_oracle"] = bool(r["seqs"] == g["seqs"] and np.array_equal(r["cov"], g["cov"])
                                       and np.array_equal(r["src"], g["src"]) and np.array_equal(r["dst"], g["dst"]))
+    return out, g
+
+
+def bench_next_rows(ctx, dev, F, iso, bank, g21, cpu: bool, cpu_pairs: int = 200_000) -> dict:
+    """The rows either side of the path on the benchmark isolate, each timed
+    on the device with the oracle's single-threaded form beside it:
+    ingest (trim / merge / subsample of the isolate's pairs as parsed FASTQ
+    records), the multi-k upgrade from the first pass's graph to k 55, the PE
+    mapper and consensus polishing on the k 55 graph."""
+    out = {}
+    n, L = iso.r1.shape
+    rng = np.random.default_rng(0xFB06)
+    q1 = rng.integers(63, 74, size=(n, L), dtype=np.uint8)  # Phred 30-40
+    q2 = rng.integers(63, 74, size=(n, L), dtype=np.uint8)
+    F.ingest_arrays(ctx, iso.r1[:1000], q1[:1000], iso.r2[:1000], q2[:1000])[0].close()  # warm-up
+    t = time.perf_counter()
+    ib, st = F.ingest_arrays(ctx, iso.r1, q1, iso.r2, q2)
+    wall = (time.perf_counter() - t) * 1e3
+    out["ingest"] = {"pairs": n, "retained_pairs": st["retained_read_pairs"], "merged_pairs": st["merged_pairs_count"],
+                     "device_ms": round(st["device_ms"], 2), "wall_ms": round(wall, 2),
+                     "pairs_per_s_device": round(n / max(st["device_ms"], 1e-6) * 1e3)}
+    ib.close()
+    if cpu:
+        from oracle import oracle as O
+        m = min(cpu_pairs, n)
+        recs1 = O.records([bytes(r) for r in iso.r1[:m]], [bytes(q) for q in q1[:m]])
+        recs2 = O.records([bytes(r) for r in iso.r2[:m]], [bytes(q) for q in q2[:m]])
+        t = time.perf_counter()
+        ref = O.ingest_records(recs1, recs2, min_len=30, min_qual=20)
+        cpu_ms = (time.perf_counter() - t) * 1e3
+        sb, sst = F.ingest_arrays(ctx, iso.r1[:m], q1[:m], iso.r2[:m], q2[:m])
+        got = sb.download()
+        same = all(np.array_equal(a.data[:len(d)], d[:len(a.data)]) and np.array_equal(a.lengths, ln)
+                   for a, (d, _, ln) in zip(got.arenas(), ref["arenas"]))
+        sb.close()
+        out["ingest"].update({"cpu_baseline_pairs_per_s": round(m / cpu_ms * 1e3), "cpu_sample_pairs": m,
+                              "equal_to_oracle_on_sample": bool(same and {k: sst[k] for k in ref["stats"]} == ref["stats"])})
+    # multi-k upgrade 21 -> 55 (filtered bank + prior contigs), then map and polish on its graph
+    kw = dict(genome_size=GENOME["genome_len"], cms_width=WIDTH, counter_bits=COUNTER_BITS)
+    t = time.perf_counter()
+    up = F.upgrade_multik(ctx, dev, g21, PASSES[0][0], 55, keep_active=True, **kw)
+    up_ms = (time.perf_counter() - t) * 1e3
+    g55 = up["graph"]
+    out["multik"] = {"k1": PASSES[0][0], "k2": 55, "wall_ms": round(up_ms, 2), "executed": up["executed"],
+                     "kept_pairs": up["kept_pairs"], "pairs": n, "prior_contigs": up["prior_contigs"],
+                     "unitigs_before": up["active_unitigs_before"], "unitigs_after": up["active_unitigs_after"]}
+    up["active"].close()
+    t = time.perf_counter()
+    fb, km, kp = F.filter_active_reads(ctx, dev, g21, PASSES[0][0])
+    out["multik"]["filter_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+    fb.close()
+    t = time.perf_counter()
+    mp = F.map_paired_reads(ctx, dev, g55, 55)
+    out["mapper"] = {"k": 55, "wall_ms": round((time.perf_counter() - t) * 1e3, 2), "links": int(mp["links"].size),
+                     "insert_sizes": int(mp["sizes"].size), "dist": mp["dist"]}
+    t = time.perf_counter()
+    pseqs, pst = F.polish_contigs(ctx, dev, g55, 500)
+    out["polish"] = {"wall_ms": round((time.perf_counter() - t) * 1e3, 2), **pst}
+    if cpu:
+        from oracle import oracle as O
+        arenas = [(a.data, a.offsets, a.lengths) for a in bank.arenas()]
+        t = time.perf_counter()
+        fr = O.filter_active_reads(g21, PASSES[0][0], arenas, pack_reads=False)
+        out["multik"]["filter_cpu_baseline_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        out["multik"]["filter_equal_to_oracle"] = bool(np.array_equal(fr["keep_pairs"], kp))
+        t = time.perf_counter()
+        mr = O.map_paired_reads(g55, arenas[1], arenas[2], 55)
+        out["mapper"]["cpu_baseline_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        out["mapper"]["equal_to_oracle"] = bool(mr["links"].size == mp["links"].size and
+                                                np.array_equal(mr["sizes"], mp["sizes"]) and mr["dist"] == mp["dist"])
+        t = time.perf_counter()
+        pr = O.polish_contigs(g55, arenas[:2], 500)
+        out["polish"]["cpu_baseline_ms"] = round((time.perf_counter() - t) * 1e3, 1)
+        out["polish"]["equal_to_oracle"] = bool(pr["seqs"] == pseqs and pr["stats"] == pst)
+        out["cpu_baseline"] = "oracle single-threaded forms; ingest on the first pairs (cpu_sample_pairs)"
     return out
 
 
@@ -415,8 +490,11 @@ def main(argv=None) -> int:
         copy_calibration(ctx, args.copy_calibration_mib)
     roofline["traffic"], roofline["traffic_source"] = measured_traffic(dom)
 
-    compaction = bench_compaction(ctx, dev, F, cpu=not args.no_cpu_baseline)
+    compaction, g21 = bench_compaction(ctx, dev, F, cpu=not args.no_cpu_baseline)
     log(f"compaction: {compaction}")
+    next_rows = bench_next_rows(ctx, dev, F, iso, bank, g21, cpu=not args.no_cpu_baseline)
+    log(f"next rows: {next_rows}")
+    del g21
 
     batched = served = None
     if args.batch_isolates or args.serve_requests:
@@ -469,6 +547,7 @@ def main(argv=None) -> int:
         "pass_model_roofline": pass_model,
         "cpu_baseline": cpu,
         "compaction": compaction,
+        "next_rows": next_rows,
         "batched": batched,
         "serving": served,
     }

@@ -27,6 +27,7 @@ from .api import (  # noqa: F401
     estimate_genome_files,
     filter_active_reads,
     hll_estimate_registers,
+    ingest_arrays,
     ingest_files,
     ingest_records,
     map_paired_reads,

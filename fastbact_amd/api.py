@@ -679,6 +679,33 @@ class _Records:
                                      len(seqs))
 
 
+class _FixedRecords:
+    """fb_records_view over equal-length records given as 2-D uint8 arrays."""
+
+    def __init__(self, seq2d: np.ndarray, qual2d: np.ndarray):
+        self.s = np.ascontiguousarray(seq2d, np.uint8)
+        self.q = np.ascontiguousarray(qual2d, np.uint8)
+        n, L = self.s.shape
+        self.so = np.arange(n, dtype=np.uint64) * np.uint64(L)
+        self.sl = np.full(n, L, np.uint32)
+        self.qo = np.arange(n, dtype=np.uint64) * np.uint64(self.q.shape[1])
+        self.ql = np.full(n, self.q.shape[1], np.uint32)
+        self.view = _lib.RecordsView(_ptr(self.s, C.c_uint8), _ptr(self.so, C.c_uint64), _ptr(self.sl, C.c_uint32),
+                                     _ptr(self.q, C.c_uint8), _ptr(self.qo, C.c_uint64), _ptr(self.ql, C.c_uint32), n)
+
+
+def ingest_arrays(ctx: Context, r1, q1, r2, q2, **params) -> tuple:
+    """ingest_records for equal-length mates given as (pairs, length) uint8
+    arrays of bases and qualities -> (DeviceBank, stats dict)."""
+    a, b = _FixedRecords(r1, q1), _FixedRecords(r2, q2)
+    p = ingest_params(**params)
+    st = _lib.IngestStats()
+    h = C.c_void_p()
+    ctx.check(ctx.lib.fb_ingest_records(ctx.h, C.byref(a.view), C.byref(b.view), C.byref(p), C.byref(h),
+                                        C.byref(st)), "fb_ingest_records")
+    return DeviceBank._adopt(ctx, h), _ingest_stats(st)
+
+
 def _ingest_stats(st) -> dict:
     return {f: (float(getattr(st, f)) if f.endswith("_ms") else int(getattr(st, f))) for f, _ in st._fields_}
 

@@ -500,7 +500,7 @@ def unpack(data, offsets, lengths, ids=None) -> list:
     return out
 
 
-def filter_active_reads(graph: dict, k: int, arenas: list) -> dict:
+def filter_active_reads(graph: dict, k: int, arenas: list, pack_reads: bool = True) -> dict:
     """filter_active_reads on a graph dict (seqs, circular, src; optional
     deleted) at graph k and a bank of three arenas (merged, pe_forward,
     pe_reverse as (data, offsets, lengths)) -> keep_merged, keep_pairs and the
@@ -527,6 +527,8 @@ def filter_active_reads(graph: dict, k: int, arenas: list) -> dict:
     if rc:
         raise ValueError(f"orc_filter_active_reads failed ({rc})")
     km, kp = km[:ars[0][2].size], kp[:npairs]
+    if not pack_reads:
+        return {"keep_merged": km, "keep_pairs": kp}
     merged = unpack(*ars[0], np.flatnonzero(km))
     r1 = unpack(*ars[1], np.flatnonzero(kp))
     r2 = unpack(*ars[2], np.flatnonzero(kp))

@@ -155,15 +155,28 @@ static __device__ __forceinline__ uint32_t out_arena(const uint64_t* first, uint
 }
 
 // Output offsets (global base index of the new list, arenas word-aligned),
-// lengths and the length histogram.
-__global__ void k_fa_offsets(FaArgs a) {
+// lengths and the length histogram (counted per block in LDS for lengths
+// below kLdsHist: the lengths of a read set take few values, and one global
+// counter per length would serialise every thread on it).
+constexpr uint32_t kLdsHist = 4096;
+__global__ __launch_bounds__(256) void k_fa_offsets(FaArgs a) {
+  __shared__ uint32_t h[kLdsHist];
+  const uint32_t nh = a.hist_n < kLdsHist ? a.hist_n : kLdsHist;
+  for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.rfirst[3]) return;
-  const uint32_t ar = out_arena(a.rfirst, j);
-  a.ooffs[j] = a.opos[j] - a.opos[a.rfirst[ar]] + a.word0[ar] * 32ull;
-  const uint32_t len = a.olen[j];
-  a.olens[j] = len;
-  atomicAdd(&a.hist[len < a.hist_n ? len : a.hist_n - 1u], 1u);
+  if (j < a.rfirst[3]) {
+    const uint32_t ar = out_arena(a.rfirst, j);
+    a.ooffs[j] = a.opos[j] - a.opos[a.rfirst[ar]] + a.word0[ar] * 32ull;
+    const uint32_t len = a.olen[j];
+    a.olens[j] = len;
+    const uint32_t bin = len < a.hist_n ? len : a.hist_n - 1u;
+    if (bin < nh) atomicAdd(&h[bin], 1u);
+    else atomicAdd(&a.hist[bin], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x)
+    if (h[i]) atomicAdd(&a.hist[i], h[i]);
 }
 
 // Output word w: bases 32w .. 32w+31 of the new list, taken from the kept

@@ -1,7 +1,7 @@
 // fb_ingest.hip -- the ingest record loop (src/ingest/trim_merge.rs:17-417)
 // on the device, from parsed FASTQ/FASTA records to a device read bank.
 //
-// 1. k_ing_pairs: one thread per pair applies the subsample keep-test
+// 1. k_ing_pairs: one wave per pair applies the subsample keep-test
 //    (splitmix64 of the 1-based pair index), 3' adapter clipping, the
 //    sliding-window quality trim and the overlap merge, and records the
 //    outcome (dropped / merged / unmerged), the trimmed lengths and the
@@ -51,18 +51,30 @@ static __device__ __forceinline__ bool close10(const uint8_t* w, const char* ad)
   return true;
 }
 
+// The record loop runs one wave per pair: the 64 lanes take consecutive
+// positions of each scan (adapter starts, quality-trim cuts, overlap bases),
+// so every load of a step is coalesced and a step ends with one ballot.
+// The three functions below are called by all lanes of the wave with the
+// same arguments and return the same value on every lane.
+
 // trim_adapter (trim_merge.rs:17-75): the first position, among the last 100
 // that leave 10 bases, where TruSeq or Nextera (behind A/G) or the Nextera
 // reverse complement (behind C/T) match their first 10 bases with at most one
 // mismatch.
-static __device__ uint32_t trim_adapter(const uint8_t* s, uint32_t len) {
+static __device__ uint32_t wave_trim_adapter(const uint8_t* s, uint32_t len, uint32_t lane) {
   if (len < 10) return len;
-  for (uint32_t i = len > 100 ? len - 100 : 0; i + 10 <= len; ++i) {
-    const uint8_t b0 = up_ascii(s[i]), b1 = up_ascii(s[i + 1]);
-    const bool agat = b0 == 'A' || b1 == 'G';
-    const bool ctgt = b0 == 'C' || b1 == 'T';
-    if (agat && (close10(s + i, "AGATCGGAAG") || close10(s + i, "AGATGTGTAT"))) return i;
-    if (ctgt && close10(s + i, "CTGTCTCTTA")) return i;
+  for (uint32_t base = len > 100 ? len - 100 : 0; base + 10 <= len; base += 64) {
+    const uint32_t i = base + lane;
+    bool hit = false;
+    if (i + 10 <= len) {
+      const uint8_t b0 = up_ascii(s[i]), b1 = up_ascii(s[i + 1]);
+      const bool agat = b0 == 'A' || b1 == 'G';
+      const bool ctgt = b0 == 'C' || b1 == 'T';
+      hit = (agat && (close10(s + i, "AGATCGGAAG") || close10(s + i, "AGATGTGTAT"))) ||
+            (ctgt && close10(s + i, "CTGTCTCTTA"));
+    }
+    const uint64_t m = __ballot(hit);
+    if (m) return base + (uint32_t)__ffsll((long long)m) - 1u;
   }
   return len;
 }
@@ -70,38 +82,56 @@ static __device__ uint32_t trim_adapter(const uint8_t* s, uint32_t len) {
 static __device__ __forceinline__ uint32_t phred(uint8_t q) { return q > 33 ? q - 33u : 0u; }
 
 // trim_quality (trim_merge.rs:77-113) on a read whose qualities are as long
-// as its bases: the kept length, or -1.
-static __device__ int64_t trim_quality(const uint8_t* q, uint32_t len, uint32_t min_qual, uint32_t min_len) {
+// as its bases: the kept length, or -1.  The reference keeps the whole read
+// when its last 4 qualities average min_qual (sum >= 4 min_qual, the same
+// test as sum / 4 >= min_qual), else drops bases until the window ending at
+// the cut passes, stopping at cut 3: the cut is len - j for the first j whose
+// window passes.
+static __device__ int64_t wave_trim_quality(const uint8_t* q, uint32_t len, uint32_t min_qual, uint32_t min_len,
+                                            uint32_t lane) {
   if (len < min_len) return -1;
   uint32_t cut = len;
   if (len >= 4) {
-    uint32_t sum = phred(q[cut - 4]) + phred(q[cut - 3]) + phred(q[cut - 2]) + phred(q[cut - 1]);
-    if (sum < min_qual * 4u) {
-      while (cut >= 4) {
-        sum = phred(q[cut - 4]) + phred(q[cut - 3]) + phred(q[cut - 2]) + phred(q[cut - 1]);
-        if (sum / 4u >= min_qual) break;
-        --cut;
+    cut = 3;
+    for (uint32_t j0 = 0; j0 + 4 <= len; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      bool pass = false;
+      if (j + 4 <= len) {
+        const uint32_t c = len - j;
+        pass = (phred(q[c - 4]) + phred(q[c - 3]) + phred(q[c - 2]) + phred(q[c - 1])) / 4u >= min_qual;
+      }
+      const uint64_t m = __ballot(pass);
+      if (m) {
+        cut = len - (j0 + (uint32_t)__ffsll((long long)m) - 1u);
+        break;
       }
     }
   }
   return cut < min_len ? -1 : (int64_t)cut;
 }
 
-// merge_read_pair's overlap search (trim_merge.rs:127-196): the best overlap
-// or 0.  rc2[i] = rc_ascii(r2[l2 - 1 - i]).  The scores d - 2.5 m are kept
+// merge_read_pair's overlap search (trim_merge.rs:127-196) with min overlap
+// 15 and 10% mismatches: the best overlap or 0.  rc2[i] = rc_ascii(r2[l2-1-i]).
+// A step counts 64 positions of an overlap; counting stops once the mismatches
+// pass the limit, where the reference's stop too (an overlap over the limit
+// is not used, so its exact count does not matter).  Scores d - 2.5 m are kept
 // doubled (2d - 5m) in integers, which orders them as the f64 scores do.
-static __device__ uint32_t merge_overlap(const uint8_t* r1, uint32_t l1, const uint8_t* r2, uint32_t l2,
-                                         uint32_t min_overlap, double ratio) {
+static __device__ uint32_t wave_merge(const uint8_t* r1, uint32_t l1, const uint8_t* r2, uint32_t l2,
+                                      uint32_t lane) {
+  const uint32_t min_overlap = 15;
   if (l1 < min_overlap || l2 < min_overlap) return 0;
   const uint32_t max_d = l1 < l2 ? l1 : l2;
   uint32_t best_d = 0;
-  int64_t best2 = INT64_MIN;  // doubled best score
-  for (uint32_t d = max_d; d >= min_overlap && d > 0; --d) {
+  int64_t best2 = INT64_MIN;
+  for (uint32_t d = max_d; d >= min_overlap; --d) {
     const uint8_t* s1 = r1 + (l1 - d);
-    const uint64_t max_allowed = (uint64_t)((double)d * ratio);
+    const uint64_t max_allowed = (uint64_t)((double)d * 0.10);
     uint64_t mm = 0;
-    for (uint32_t i = 0; i < d; ++i)
-      if (up_ascii(s1[i]) != rc_ascii(r2[l2 - 1 - i]) && ++mm > max_allowed) break;
+    for (uint32_t i0 = 0; i0 < d && mm <= max_allowed; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      const bool mis = i < d && up_ascii(s1[i]) != rc_ascii(r2[l2 - 1 - i]);
+      mm += (uint64_t)__popcll(__ballot(mis));
+    }
     if (mm <= max_allowed) {
       const int64_t sc2 = 2 * (int64_t)d - 5 * (int64_t)mm;
       if (sc2 > best2) {
@@ -117,13 +147,14 @@ static __device__ uint32_t merge_overlap(const uint8_t* r1, uint32_t l1, const u
 // Block sums of the stats counters: [0] retained pairs, [1] bases after
 // trimming, [2] merged pairs, [3] merged bases, [4] unmerged pairs.
 constexpr int kIngBlock = 256;
+constexpr int kIngWaves = kIngBlock / 64;  // pairs per block in k_ing_pairs
 
 __global__ __launch_bounds__(kIngBlock) void k_ing_pairs(IngArgs a) {
   __shared__ unsigned long long s_sum[5];
   if (threadIdx.x < 5) s_sum[threadIdx.x] = 0ull;
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t st = 0, t1 = 0, t2 = 0, d = 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t i = (uint64_t)blockIdx.x * kIngWaves + (threadIdx.x >> 6);
   if (i < a.n) {
     const uint8_t* s1 = a.r1.seq + a.r1.seq_off[i];
     const uint8_t* s2 = a.r2.seq + a.r2.seq_off[i];
@@ -132,13 +163,13 @@ __global__ __launch_bounds__(kIngBlock) void k_ing_pairs(IngArgs a) {
     uint32_t l1 = a.r1.seq_len[i], l2 = a.r2.seq_len[i];
     bool keep = !a.subsample || splitmix64((i + 1) ^ a.seed) <= a.keep_threshold;
     if (keep && a.enable_trim) {
-      const uint32_t c1 = trim_adapter(s1, l1), c2 = trim_adapter(s2, l2);
+      const uint32_t c1 = wave_trim_adapter(s1, l1, lane), c2 = wave_trim_adapter(s2, l2, lane);
       if (a.r1.qual_len[i] < c1 || a.r2.qual_len[i] < c2) {
-        a.status[0] = 1u;  // the reference's quality slice would panic: the host reports an error
+        if (lane == 0) a.status[0] = 1u;  // the reference's quality slice would panic: the host reports an error
         keep = false;
       } else {
-        const int64_t x1 = trim_quality(q1, c1, a.min_qual, a.min_len);
-        const int64_t x2 = trim_quality(q2, c2, a.min_qual, a.min_len);
+        const int64_t x1 = wave_trim_quality(q1, c1, a.min_qual, a.min_len, lane);
+        const int64_t x2 = wave_trim_quality(q2, c2, a.min_qual, a.min_len, lane);
         keep = x1 >= 0 && x2 >= 0;
         l1 = (uint32_t)x1;
         l2 = (uint32_t)x2;
@@ -146,26 +177,31 @@ __global__ __launch_bounds__(kIngBlock) void k_ing_pairs(IngArgs a) {
     } else if (keep) {
       keep = l1 >= a.min_len && l2 >= a.min_len;
     }
+    uint32_t st = 0, t1 = 0, t2 = 0, d = 0;
     if (keep) {
       t1 = l1;
       t2 = l2;
-      d = a.enable_merge ? merge_overlap(s1, l1, s2, l2, 15u, 0.10) : 0u;
+      d = a.enable_merge ? wave_merge(s1, l1, s2, l2, lane) : 0u;
       st = d ? 1u : 2u;
-      atomicAdd(&s_sum[0], 1ull);
-      atomicAdd(&s_sum[1], (unsigned long long)l1 + l2);
-      if (d) {
-        atomicAdd(&s_sum[2], 1ull);
-        atomicAdd(&s_sum[3], (unsigned long long)(l1 + l2 - d));
-      } else {
-        atomicAdd(&s_sum[4], 1ull);
-      }
     }
-    a.merged[i] = st == 1u ? 1u : 0u;
-    a.unmerged[i] = st == 2u ? 1u : 0u;
-    a.t1[i] = t1;
-    a.t2[i] = t2;
-    a.ov[i] = d;
-  } else if (i == a.n) {  // the scans' totals
+    if (lane == 0) {
+      if (keep) {
+        atomicAdd(&s_sum[0], 1ull);
+        atomicAdd(&s_sum[1], (unsigned long long)t1 + t2);
+        if (d) {
+          atomicAdd(&s_sum[2], 1ull);
+          atomicAdd(&s_sum[3], (unsigned long long)(t1 + t2 - d));
+        } else {
+          atomicAdd(&s_sum[4], 1ull);
+        }
+      }
+      a.merged[i] = st == 1u ? 1u : 0u;
+      a.unmerged[i] = st == 2u ? 1u : 0u;
+      a.t1[i] = t1;
+      a.t2[i] = t2;
+      a.ov[i] = d;
+    }
+  } else if (i == a.n && lane == 0) {  // the scans' totals
     a.merged[i] = 0u;
     a.unmerged[i] = 0u;
   }
@@ -196,14 +232,29 @@ static __device__ __forceinline__ uint32_t arena_of(const uint64_t* first, uint6
   return j < first[1] ? 0u : (j < first[2] ? 1u : 2u);
 }
 
+// Output offsets (global base index of the new list, arenas word-aligned),
+// lengths and the length histogram (counted per block in LDS for lengths
+// below kLdsHist: the lengths of a read set take few values, and one global
+// counter per length would serialise every thread on it).
+constexpr uint32_t kLdsHist = 4096;
 __global__ __launch_bounds__(kIngBlock) void k_ing_offsets(IngArgs a) {
+  __shared__ uint32_t h[kLdsHist];
+  const uint32_t nh = a.hist_n < kLdsHist ? a.hist_n : kLdsHist;
+  for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
   const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.rfirst[3]) return;
-  const uint32_t ar = arena_of(a.rfirst, j);
-  a.ooffs[j] = a.opos[j] - a.opos[a.rfirst[ar]] + a.word0[ar] * 32ull;
-  const uint32_t len = a.olen[j];
-  a.olens[j] = len;
-  atomicAdd(&a.hist[len < a.hist_n ? len : a.hist_n - 1u], 1u);
+  if (j < a.rfirst[3]) {
+    const uint32_t ar = arena_of(a.rfirst, j);
+    a.ooffs[j] = a.opos[j] - a.opos[a.rfirst[ar]] + a.word0[ar] * 32ull;
+    const uint32_t len = a.olen[j];
+    a.olens[j] = len;
+    const uint32_t bin = len < a.hist_n ? len : a.hist_n - 1u;
+    if (bin < nh) atomicAdd(&h[bin], 1u);
+    else atomicAdd(&a.hist[bin], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nh; i += blockDim.x)
+    if (h[i]) atomicAdd(&a.hist[i], h[i]);
 }
 
 // Base p of output read j (of arena ar) as a 2-bit code.
@@ -256,7 +307,8 @@ static uint32_t ing_grid(uint64_t n) {
 }
 
 hipError_t launch_ing_pairs(const IngArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_ing_pairs, dim3(ing_grid(a.n + 1)), dim3(kIngBlock), 0, s, a);
+  const uint64_t blocks = (a.n + 1 + kIngWaves - 1) / kIngWaves;  // one wave per pair, and one for the totals
+  hipLaunchKernelGGL(k_ing_pairs, dim3((uint32_t)blocks), dim3(kIngBlock), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_ing_list(const IngArgs& a, hipStream_t s) {

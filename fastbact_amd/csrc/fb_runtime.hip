@@ -703,7 +703,13 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   const uint32_t gate = prm->target_solid_cov >= 0 ? (uint32_t)prm->target_solid_cov : 3u;
 
   uint64_t contig_bases = 0;
-  for (uint64_t i = 0; i < n_contigs; ++i) contig_bases += contig_lens[i];
+  uint32_t contig_max = 0;
+  for (uint64_t i = 0; i < n_contigs; ++i) {
+    contig_bases += contig_lens[i];
+    contig_max = std::max(contig_max, contig_lens[i]);
+  }
+  // contigs run to many kb: about 64 windows per thread share
+  const uint32_t contig_split = std::max<uint32_t>(1u, std::min<uint32_t>(4096u, (contig_max + 63) / 64));
   if (int r = check_add_budget(c, fb_bank_bases_for_k(bank, k) + 10 * contig_bases)) return r;
 
   fb_solid* so = new fb_solid();
@@ -765,7 +771,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     Pass1Args pc{};
     pc.reads = cview;
     pc.k = k;
-    pc.split = 1;
+    pc.split = contig_split;
     pc.weight = 10;
     pc.cms = so->cms.dev();
     HIP_TRY(c, launch_pass1(pc, s));
@@ -964,7 +970,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
       if (c_reads) {
         Pass2Args pc = p2;
         pc.reads = cview;
-        pc.split = 1;
+        pc.split = contig_split;
         pc.contig_mode = 1;
         HIP_TRY(c, launch_pass2(pc, s));
       }

@@ -212,21 +212,71 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_start_init(CgArgs a) {
 }
 
 // Counted 1-in/1-out nodes add their cov to their path and bid for its last
-// node; cycle nodes add to their cycle's label.
-__global__ __launch_bounds__(kCgBlock) void k_cg_accumulate(CgArgs a, uint32_t s) {
-  const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= 2ull * a.n) return;
-  const uint32_t u = a.up[s][d];
-  if (u == kStHead) {
-    const uint32_t e = a.head[s][d];
-    atomicAdd((unsigned long long*)a.csum + e, (unsigned long long)a.cov[d >> 1]);
-    atomicMax((unsigned long long*)a.last + e, ((unsigned long long)a.rank[s][d] << 32) | (unsigned long long)d);
-  } else if (u < kStHead) {
-    // on a cycle: lab = smallest node of the cycle
-    const uint32_t c = a.lab[s][d];
-    atomicAdd((unsigned long long*)a.ccov + c, (unsigned long long)a.cov[d >> 1]);
-    atomicAdd(a.clen + c, 1u);
+// node; cycle nodes add to their cycle's label.  A graph of few long unitigs
+// sends most nodes of a block to the same few paths, so each block first
+// combines its nodes per path (or cycle) in an LDS table and then adds one
+// total per entry; a node that finds no free LDS entry adds its own.
+constexpr int kAccBlock = 1024;
+constexpr uint32_t kAccSlots = 2048;
+constexpr uint32_t kCycleTag = 0x80000000u;  // marks a cycle label in the LDS table
+
+__device__ __forceinline__ void acc_global(const CgArgs& a, uint32_t tag, unsigned long long v, unsigned long long m) {
+  if (tag & kCycleTag) {
+    const uint32_t c = tag & ~kCycleTag;
+    atomicAdd((unsigned long long*)a.ccov + c, v);
+    atomicAdd(a.clen + c, (uint32_t)m);
+  } else {
+    atomicAdd((unsigned long long*)a.csum + tag, v);
+    atomicMax((unsigned long long*)a.last + tag, m);
   }
+}
+
+__global__ __launch_bounds__(kAccBlock) void k_cg_accumulate(CgArgs a, uint32_t s) {
+  __shared__ uint32_t key[kAccSlots];
+  __shared__ unsigned long long sum[kAccSlots];  // cov total
+  __shared__ unsigned long long agg[kAccSlots];  // paths: max (rank << 32 | node); cycles: node count
+  for (uint32_t i = threadIdx.x; i < kAccSlots; i += blockDim.x) {
+    key[i] = kNone;
+    sum[i] = 0ull;
+    agg[i] = 0ull;
+  }
+  __syncthreads();
+  const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < 2ull * a.n) {
+    const uint32_t u = a.up[s][d];
+    uint32_t tag = kNone;
+    unsigned long long v = 0, m = 0;
+    if (u == kStHead) {
+      tag = a.head[s][d];
+      v = a.cov[d >> 1];
+      m = ((unsigned long long)a.rank[s][d] << 32) | (unsigned long long)d;
+    } else if (u < kStHead) {  // on a cycle: lab = smallest node of the cycle
+      tag = a.lab[s][d] | kCycleTag;
+      v = a.cov[d >> 1];
+      m = 1ull;
+    }
+    if (tag != kNone) {
+      uint32_t slot = (tag * 0x9E3779B1u) >> (32 - 11);  // 11 = log2(kAccSlots)
+      bool done = false;
+      for (int probe = 0; probe < 16 && !done; ++probe, slot = (slot + 1) & (kAccSlots - 1)) {
+        uint32_t k = key[slot];
+        if (k == kNone) {
+          const uint32_t old = atomicCAS(&key[slot], kNone, tag);
+          k = old == kNone ? tag : old;
+        }
+        if (k == tag) {
+          atomicAdd(&sum[slot], v);
+          if (tag & kCycleTag) atomicAdd(&agg[slot], m);
+          else atomicMax(&agg[slot], m);
+          done = true;
+        }
+      }
+      if (!done) acc_global(a, tag, v, m);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kAccSlots; i += blockDim.x)
+    if (key[i] != kNone) acc_global(a, key[i], sum[i], agg[i]);
 }
 
 // Which start edges are walked, and the length of their unitigs.
@@ -484,7 +534,8 @@ hipError_t launch_cg_jump(const CgArgs& a, uint32_t s, uint32_t t, uint32_t* out
 
 hipError_t launch_cg_accumulate(const CgArgs& a, uint32_t s, hipStream_t st) {
   hipLaunchKernelGGL(k_cg_start_init, dim3(cg_grid(2ull * a.n)), dim3(kCgBlock), 0, st, a);
-  hipLaunchKernelGGL(k_cg_accumulate, dim3(cg_grid(2ull * a.n)), dim3(kCgBlock), 0, st, a, s);
+  hipLaunchKernelGGL(k_cg_accumulate, dim3((unsigned)((2ull * a.n + kAccBlock - 1) / kAccBlock)), dim3(kAccBlock), 0,
+                     st, a, s);
   hipLaunchKernelGGL(k_cg_paths, dim3(cg_grid(2ull * a.n)), dim3(kCgBlock), 0, st, a);
   hipLaunchKernelGGL(k_cg_cycles, dim3(cg_grid(2ull * a.n)), dim3(kCgBlock), 0, st, a, s);
   return hipGetLastError();

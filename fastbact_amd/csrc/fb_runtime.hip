@@ -2626,7 +2626,8 @@ static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, 
   g->ctx = c;
   g->k = k;
   if (n == 0) return FB_OK;
-  if (n >= (1ull << 30)) return set_err(c, FB_ERR_INVALID, "compaction takes at most 2^30 solid k-mers");
+  // start edges (up to 4 per directed k-mer) and cycle labels share 31-bit ids
+  if (n >= (1ull << 28)) return set_err(c, FB_ERR_INVALID, "compaction takes fewer than 2^28 solid k-mers");
   const uint64_t N = 2 * n;
   CgArgs a;
   a.keys = keys;

@@ -67,7 +67,7 @@ def kernel_bytes(k: int, st: dict, bases: int, solid: int) -> dict:
             "p2_bins": recs + solid * (8.0 * ref_words(k) + 3.0)}
 
 
-KERNEL_NAMES = {"bin_scatter": "k_bin_scatter (pass-1 Count-Min bin scatter + HLL)",
+KERNEL_NAMES = {"bin_scatter": "k_bin_buffer (pass-1 Count-Min scatter through per-bin LDS buffers + HLL)",
                 "bin_count": "k_bin_count (pass-1 LDS counts + gate bits)",
                 "p2_scatter": "k_p2_scatter (pass-2 adjacency record scatter)",
                 "p2_bins": "k_p2_bins (pass-2 LDS tables + pass 3)"}

@@ -13,6 +13,10 @@
 //     the unused tail of its last chunk per bin with kNoCell.  An update that
 //     finds its slice full is added to the table directly with a global
 //     atomic, so any input (low-complexity reads included) stays exact.
+//     With a resident grid and at most kBufMaxBins bins, phase A is
+//     k_bin_buffer instead: every bin has a 64-byte buffer in LDS, and a
+//     buffer is written out only when full, so each store to a bin region
+//     covers 64 whole bytes (no line is written back partly filled).
 //   Phase B (k_bin_count): one workgroup per bin counts the offsets of the
 //     bin's slices with LDS atomics, adds them to the table (the bin's cells
 //     belong to this workgroup alone) and writes the pass-2 threshold bits of
@@ -279,6 +283,147 @@ __global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter_m(const BinArg
   bin_scatter_body<L, HLL>(args[iso], blockIdx.x - blk[iso], sh);
 }
 
+// Phase A with a buffer per bin (the resident-grid form, up to kBufMaxBins
+// bins).  Each bin holds kBufE offsets (64 bytes) in LDS.  A thread appends
+// its round's cells to their bins' buffers with one LDS atomic each.  The
+// thread whose cell fills a buffer puts the bin on the round's full list,
+// and after a barrier half a wave per full bin writes the buffer into the
+// bin's current chunk (64 offsets, one line, reserved with one global atomic
+// per chunk).  Cells that found their buffer full try again after the flush.
+// Every store to the bin regions thus writes 64 whole bytes, once; a line
+// is written by one workgroup only, in two halves.  After kBufTries
+// placement steps in one round, the cells still waiting are added to the
+// table with global atomics (heavily repeated k-mers: exact, and no
+// workgroup waits on a bin for long).
+// LDS: cnt[nbins] (entries in the buffer; may pass kBufE while cells are
+// placed), cur[nbins] (chunk_word of the current chunk), full[nbins],
+// buf[nbins * kBufE] (u16).
+constexpr int kBufBlock = 1024;
+constexpr uint32_t kBufE = 32;
+constexpr uint32_t kBufMaxBins = 2048;
+constexpr int kBufTries = 4;
+static_assert(2 * kBufE == kChunk, "a chunk takes two buffer flushes");
+
+// A bin's current chunk in the buffered kernel: its next free entry (a
+// multiple of kBufE) / kBufE << 2 | the halves of the chunk still free (0..2).
+__device__ __forceinline__ uint32_t half_word(uint32_t pos, uint32_t halves) { return (pos / kBufE) << 2 | halves; }
+__device__ __forceinline__ uint32_t half_pos(uint32_t w) { return (w >> 2) * kBufE; }
+
+size_t bin_buffer_lds(uint32_t nbins) { return (size_t)nbins * (3 * sizeof(uint32_t) + kBufE * sizeof(uint16_t)); }
+
+template <int L, bool HLL>
+__device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
+  __shared__ uint32_t s_nfull[2];
+  const uint32_t nb = a.nbins;
+  uint32_t* cnt = sh;
+  uint32_t* cur = sh + nb;
+  uint32_t* full = sh + 2 * nb;
+  uint16_t* buf = reinterpret_cast<uint16_t*>(sh + 3 * nb);
+  for (uint32_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) sh[i] = 0;  // cnt, cur (no chunk yet)
+  if (threadIdx.x < 2) s_nfull[threadIdx.x] = 0;
+  const uint64_t total = a.reads.n_reads * (uint64_t)a.split;
+  const uint64_t gid0 = blk * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)a.grid * blockDim.x;
+  const uint32_t count = gid0 < total ? (uint32_t)((total - 1 - gid0) / stride + 1) : 0u;
+  const uint32_t sb = a.sb;
+  const uint32_t sl = xcc_slice();
+  const uint64_t cap = a.cap;
+  const uint32_t smask = (1u << sb) - 1u;
+  uint16_t* __restrict__ out = a.data + sl * cap;
+  uint32_t* __restrict__ cursor = a.cursor + sl;
+  const uint32_t h = threadIdx.x & 31u;
+  CellWalker<L, HLL> wk;
+  wk.init(gid0, stride, count);
+  uint32_t step = 0;  // placement steps so far; its parity picks the full-list counter
+  __syncthreads();
+  for (;;) {
+    if (!wk.has()) wk.open(a);
+    uint32_t cells[4 * kRound];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int j = 0; j < kRound; ++j) {
+      if (wk.has()) {
+        const uint64_t key = wk.take(a);
+#pragma unroll
+        for (int row = 0; row < 4; ++row) cells[4 * j + row] = (uint32_t)cell_of(a, key, row);
+        pend |= 0xFu << (4 * j);
+      }
+    }
+    if (!__syncthreads_or(pend)) break;
+    for (int tries = 0;; ++tries) {
+      // values derived from the cells are recomputed in every step rather
+      // than kept live across the loop (which would spill)
+#pragma unroll
+      for (int i = 0; i < 4 * kRound; ++i) asm volatile("" : "+v"(cells[i]));
+      uint32_t* nfull = s_nfull + (step & 1u);
+      if (tries == kBufTries) {
+#pragma unroll
+        for (int i = 0; i < 4 * kRound; ++i)
+          if ((pend >> i) & 1u) atomicAdd(a.counts + cells[i], 1u);
+        pend = 0;
+      }
+#pragma unroll
+      for (int i = 0; i < 4 * kRound; ++i) {
+        if ((pend >> i) & 1u) {
+          const uint32_t cell = cells[i], b = cell >> sb;
+          const uint32_t slot = atomicAdd(cnt + b, 1u);
+          if (slot < kBufE) {
+            buf[b * kBufE + slot] = (uint16_t)(cell & smask);
+            pend &= ~(1u << i);
+            if (slot == kBufE - 1u) full[atomicAdd(nfull, 1u)] = b;
+          }
+        }
+      }
+      __syncthreads();
+      const uint32_t nf = *nfull;
+      if (threadIdx.x == 0) s_nfull[(step + 1u) & 1u] = 0;  // read by no thread until the next barrier
+      ++step;
+      // a new chunk for each full bin that has none left, all reserved at once
+      for (uint32_t f = threadIdx.x; f < nf; f += blockDim.x) {
+        const uint32_t b = full[f];
+        if ((cur[b] & 3u) == 0) cur[b] = half_word(atomicAdd(cursor + (uint64_t)b * kSlices, kChunk), 2u);
+      }
+      __syncthreads();
+      for (uint32_t f = threadIdx.x >> 5; f < nf; f += blockDim.x >> 5) {
+        const uint32_t b = full[f];
+        const uint32_t cw = cur[b];
+        const uint64_t pos = half_pos(cw);
+        const uint16_t e = buf[b * kBufE + h];
+        if (pos + h < cap) out[(uint64_t)b * (kSlices * cap) + pos + h] = e;
+        else atomicAdd(a.counts + ((uint64_t)b << sb) + e, 1u);
+        if (h == 0) {
+          cur[b] = cw + (1u << 2) - 1u;  // next half: position + kBufE, one half fewer
+          cnt[b] = 0;
+        }
+      }
+      if (!__syncthreads_or(pend)) break;
+    }
+  }
+  // each bin's buffer (fewer than kBufE entries: a full one was flushed in
+  // its step), then kNoCell to the end of its chunk (a wave per bin)
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    if (cnt[b] && (cur[b] & 3u) == 0) cur[b] = half_word(atomicAdd(cursor + (uint64_t)b * kSlices, kChunk), 2u);
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  for (uint32_t b = wave; b < nb; b += blockDim.x >> 6) {
+    const uint32_t c = cnt[b];
+    const uint32_t cw = cur[b];
+    const uint32_t left = (cw & 3u) * kBufE;
+    const uint64_t pos = half_pos(cw);
+    if (lane < left) {
+      const uint16_t e = lane < c ? buf[b * kBufE + lane] : kNoCell;
+      if (pos + lane < cap) out[(uint64_t)b * (kSlices * cap) + pos + lane] = e;
+      else if (e != kNoCell) atomicAdd(a.counts + ((uint64_t)b << sb) + e, 1u);
+    }
+  }
+}
+
+template <int L, bool HLL>
+__global__ __launch_bounds__(kBufBlock) void k_bin_buffer(BinArgs a) {
+  extern __shared__ uint32_t sh[];
+  bin_buffer_body<L, HLL>(a, blockIdx.x, sh);
+}
+
 __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uint32_t* lds) {
   const uint32_t S = 1u << a.sb;
   const uint64_t first = (uint64_t)b << a.sb;
@@ -359,6 +504,14 @@ static hipError_t pass1_lds_attrs() {
                        reinterpret_cast<const void*>(&k_bin_scatter_m<3, false>),
                        reinterpret_cast<const void*>(&k_bin_scatter_m<4, true>),
                        reinterpret_cast<const void*>(&k_bin_scatter_m<4, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<1, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<1, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<2, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<2, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<3, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<3, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<4, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer<4, false>),
                        reinterpret_cast<const void*>(&k_bin_count),
                        reinterpret_cast<const void*>(&k_bin_count_m)};
     hipError_t e = hipSuccess;
@@ -379,9 +532,13 @@ hipError_t check_bin_pass1(uint32_t nbins, uint32_t sb) {
 uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
   if (pass1_lds_attrs() != hipSuccess) return 1;
   const uint32_t L = (2 * k + 63) / 64;
+  const bool buffered = nbins <= kBufMaxBins;
   const void* f = nullptr;
-#define BR(LL) f = hll ? reinterpret_cast<const void*>(&k_bin_scatter<LL, true>) \
-                       : reinterpret_cast<const void*>(&k_bin_scatter<LL, false>)
+#define BR(LL)                                                                                 \
+  if (buffered) f = hll ? reinterpret_cast<const void*>(&k_bin_buffer<LL, true>)              \
+                        : reinterpret_cast<const void*>(&k_bin_buffer<LL, false>);            \
+  else f = hll ? reinterpret_cast<const void*>(&k_bin_scatter<LL, true>)                     \
+               : reinterpret_cast<const void*>(&k_bin_scatter<LL, false>)
   switch (L) {
     case 1: BR(1); break;
     case 2: BR(2); break;
@@ -390,7 +547,9 @@ uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
   }
 #undef BR
   int per = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, kScatterBlock, bin_scatter_lds(nbins)) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, buffered ? kBufBlock : kScatterBlock,
+                                                   buffered ? bin_buffer_lds(nbins) : bin_scatter_lds(nbins)) !=
+          hipSuccess ||
       hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 1;
@@ -400,7 +559,24 @@ uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
   if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
-  if (a.reads.n_reads) {
+  if (a.reads.n_reads && a.grid && a.nbins <= kBufMaxBins) {
+    if (bin_buffer_lds(a.nbins) + 1024 > kP2LdsBytes) return hipErrorInvalidConfiguration;
+    const uint32_t L = (2 * a.k + 63) / 64;
+    const size_t lds = bin_buffer_lds(a.nbins);
+    const bool h = a.hll.regs != nullptr;
+#define BB(LL)                                                                                           \
+  if (h) hipLaunchKernelGGL((k_bin_buffer<LL, true>), dim3(a.grid), dim3(kBufBlock), lds, s, a);        \
+  else hipLaunchKernelGGL((k_bin_buffer<LL, false>), dim3(a.grid), dim3(kBufBlock), lds, s, a)
+    switch (L) {
+      case 1: BB(1); break;
+      case 2: BB(2); break;
+      case 3: BB(3); break;
+      default: BB(4); break;
+    }
+#undef BB
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  } else if (a.reads.n_reads) {
     const uint32_t L = (2 * a.k + 63) / 64;
     const unsigned grid =
         a.grid ? a.grid : (unsigned)scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks, kP1BlockSize);

@@ -144,6 +144,88 @@ static __device__ uint32_t wave_merge(const uint8_t* r1, uint32_t l1, const uint
   return best_d >= min_overlap ? best_d : 0u;
 }
 
+// The same overlap search with every overlap counted at once, for mates of
+// at most kPlaneBases bases.  The trimmed R1 and the reverse complement of
+// the trimmed R2 are held as bit planes in LDS (bit j of word w is base
+// 64 w + j): R1's two code bits and an ACGT flag, rc(R2)'s two code bits
+// (rc_ascii gives ACGT only).  Lane t takes overlaps d = 15 + t, 79 + t, ..;
+// the mismatches of overlap d are the set bits of
+// ~ok1 | (lo1 ^ lo2) | (hi1 ^ hi2) over R1 from l1 - d against rc(R2) from
+// 0, 64 positions per step.  The best overlap maximises 2d - 5m among those
+// within the limit, ties to the longer overlap: the overlap the descending
+// search keeps (it replaces its best only on a strictly higher score).
+constexpr uint32_t kPlaneBases = 1024;
+constexpr uint32_t kPlaneWords = kPlaneBases / 64 + 1;  // one zero word past the end for the funnel shifts
+struct MergePlanes {
+  uint64_t lo1[kPlaneWords], hi1[kPlaneWords], ok1[kPlaneWords], lo2[kPlaneWords], hi2[kPlaneWords];
+};
+
+static __device__ __forceinline__ uint64_t plane_bits(const uint64_t* p, uint32_t pos) {
+  const uint32_t w = pos >> 6, s = pos & 63u;
+  return s ? (p[w] >> s) | (p[w + 1] << (64u - s)) : p[w];
+}
+
+static __device__ uint32_t wave_merge_planes(const uint8_t* r1, uint32_t l1, const uint8_t* r2, uint32_t l2,
+                                             uint32_t lane, MergePlanes& P) {
+  const uint32_t min_overlap = 15;
+  if (l1 < min_overlap || l2 < min_overlap) return 0;
+  const uint32_t n1 = (l1 + 63) / 64, n2 = (l2 + 63) / 64;
+  for (uint32_t w = 0; w < n1; ++w) {
+    const uint32_t i = 64 * w + lane;
+    const uint8_t u = i < l1 ? up_ascii(r1[i]) : (uint8_t)0;
+    const uint32_t c = code_of(u);
+    const uint64_t lo = __ballot(c & 1u), hi = __ballot(c >> 1), ok = __ballot(u == 'A' || u == 'C' || u == 'G' || u == 'T');
+    if (lane == 0) {
+      P.lo1[w] = lo;
+      P.hi1[w] = hi;
+      P.ok1[w] = ok;
+    }
+  }
+  for (uint32_t w = 0; w < n2; ++w) {
+    const uint32_t i = 64 * w + lane;
+    const uint32_t c = i < l2 ? code_of(rc_ascii(r2[l2 - 1 - i])) : 0u;
+    const uint64_t lo = __ballot(c & 1u), hi = __ballot(c >> 1);
+    if (lane == 0) {
+      P.lo2[w] = lo;
+      P.hi2[w] = hi;
+    }
+  }
+  if (lane == 0) {
+    P.lo1[n1] = P.hi1[n1] = P.ok1[n1] = 0ull;
+    P.lo2[n2] = P.hi2[n2] = 0ull;
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  const uint32_t max_d = l1 < l2 ? l1 : l2;
+  uint64_t best = 0;  // (2d - 5m) << 32 | d; every score within the limit is positive
+  for (uint32_t d0 = min_overlap; d0 <= max_d; d0 += 64) {
+    const uint32_t d = d0 + lane;
+    if (d > max_d) continue;
+    const uint32_t o = l1 - d;
+    uint32_t mm = 0;
+    for (uint32_t q = 0; 64 * q < d; ++q) {
+      const uint32_t p = o + 64 * q;
+      uint64_t m = ~plane_bits(P.ok1, p) | (plane_bits(P.lo1, p) ^ P.lo2[q]) | (plane_bits(P.hi1, p) ^ P.hi2[q]);
+      const uint32_t n = d - 64 * q;
+      if (n < 64) m &= (1ull << n) - 1ull;
+      mm += (uint32_t)__popcll(m);
+    }
+    const uint32_t max_allowed = (uint32_t)((double)d * 0.10);
+    if (mm <= max_allowed) {
+      const uint64_t key = (uint64_t)(2 * d - 5 * mm) << 32 | d;
+      best = key > best ? key : best;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)best, off), hi = __shfl_xor((uint32_t)(best >> 32), off);
+    const uint64_t o = (uint64_t)hi << 32 | lo;
+    best = o > best ? o : best;
+  }
+  __builtin_amdgcn_wave_barrier();  // the planes are rewritten for the next read only after every lane is done
+  return (uint32_t)best;
+}
+
 // Block sums of the stats counters: [0] retained pairs, [1] bases after
 // trimming, [2] merged pairs, [3] merged bases, [4] unmerged pairs.
 constexpr int kIngBlock = 256;
@@ -151,6 +233,7 @@ constexpr int kIngWaves = kIngBlock / 64;  // pairs per block in k_ing_pairs
 
 __global__ __launch_bounds__(kIngBlock) void k_ing_pairs(IngArgs a) {
   __shared__ unsigned long long s_sum[5];
+  __shared__ MergePlanes s_planes[kIngWaves];
   if (threadIdx.x < 5) s_sum[threadIdx.x] = 0ull;
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63u;
@@ -181,7 +264,9 @@ __global__ __launch_bounds__(kIngBlock) void k_ing_pairs(IngArgs a) {
     if (keep) {
       t1 = l1;
       t2 = l2;
-      d = a.enable_merge ? wave_merge(s1, l1, s2, l2, lane) : 0u;
+      if (a.enable_merge)
+        d = l1 <= kPlaneBases && l2 <= kPlaneBases ? wave_merge_planes(s1, l1, s2, l2, lane, s_planes[threadIdx.x >> 6])
+                                                   : wave_merge(s1, l1, s2, l2, lane);
       st = d ? 1u : 2u;
     }
     if (lane == 0) {

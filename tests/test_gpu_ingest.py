@@ -106,6 +106,16 @@ def test_files_match_records_and_oracle(ctx, orc, tmp_path, gz, crlf):
     assert st["parse_ms"] > 0 and st["device_ms"] > 0
 
 
+@pytest.mark.parametrize("L,insert,n", [(250, (180, 700), 2000), (1020, (700, 2400), 300), (1100, (800, 2600), 300)])
+def test_long_mates_match_oracle(ctx, orc, L, insert, n):
+    """Mates up to 1024 bases are merged with every overlap counted at once
+    from bit planes; longer ones take the overlap-by-overlap search.  Both
+    give the oracle's bank, with overlaps of 15 up to the whole mate."""
+    r1s, q1s, r2s, q2s = _pairs(31 + L, n, L=L, insert=insert)
+    _, st, _ = _both(ctx, orc, r1s, q1s, r2s, q2s)
+    assert st["merged_pairs_count"] > n // 10 and st["unmerged_pairs_count"] > n // 10
+
+
 def test_scale_100k_pairs(ctx, orc):
     rng = np.random.default_rng(77)
     genome = rng.choice(np.frombuffer(b"ACGT", np.uint8), size=300_000)

@@ -13,7 +13,8 @@
 //     the unused tail of its last chunk per bin with kNoCell.  An update that
 //     finds its slice full is added to the table directly with a global
 //     atomic, so any input (low-complexity reads included) stays exact.
-//     With a resident grid and at most kBufMaxBins bins, phase A is
+//     With a resident grid (or the batch's per-isolate grids) and
+//     kBufMinBins to kBufMaxBins bins, phase A is
 //     k_bin_buffer instead: every bin has a 64-byte buffer in LDS, and a
 //     buffer is written out only when full, so each store to a bin region
 //     covers 64 whole bytes (no line is written back partly filled).
@@ -283,8 +284,7 @@ __global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter_m(const BinArg
   bin_scatter_body<L, HLL>(args[iso], blockIdx.x - blk[iso], sh);
 }
 
-// Phase A with a buffer per bin (the resident-grid form, up to kBufMaxBins
-// bins).  Each bin holds kBufE offsets (64 bytes) in LDS.  A thread appends
+// Phase A with a buffer per bin (from kBufMinBins to kBufMaxBins bins).  Each bin holds kBufE offsets (64 bytes) in LDS.  A thread appends
 // its round's cells to their bins' buffers with one LDS atomic each.  The
 // thread whose cell fills a buffer puts the bin on the round's full list,
 // and after a barrier half a wave per full bin writes the buffer into the
@@ -298,9 +298,10 @@ __global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter_m(const BinArg
 // LDS: cnt[nbins] (entries in the buffer; may pass kBufE while cells are
 // placed), cur[nbins] (chunk_word of the current chunk), full[nbins],
 // buf[nbins * kBufE] (u16).
-constexpr int kBufBlock = 1024;
+constexpr int kBufBlock = (int)kP1BufBlock;
 constexpr uint32_t kBufE = 32;
-constexpr uint32_t kBufMaxBins = 2048;
+constexpr uint32_t kBufMaxBins = kP1BufMaxBins;
+constexpr uint32_t kBufMinBins = kP1BufMinBins;
 constexpr int kBufTries = 4;
 static_assert(2 * kBufE == kChunk, "a chunk takes two buffer flushes");
 
@@ -308,6 +309,11 @@ static_assert(2 * kBufE == kChunk, "a chunk takes two buffer flushes");
 // multiple of kBufE) / kBufE << 2 | the halves of the chunk still free (0..2).
 __device__ __forceinline__ uint32_t half_word(uint32_t pos, uint32_t halves) { return (pos / kBufE) << 2 | halves; }
 __device__ __forceinline__ uint32_t half_pos(uint32_t w) { return (w >> 2) * kBufE; }
+
+// With fewer bins the staged scatter's open lines fit the caches better and
+// its two blocks per CU hide each other's barriers: it is the faster form
+// there (1024 bins: 18.3 against 18.0 Gbases/s for the whole build).
+bool bin_buffered(uint32_t nbins) { return nbins >= kBufMinBins && nbins <= kBufMaxBins; }
 
 size_t bin_buffer_lds(uint32_t nbins) { return (size_t)nbins * (3 * sizeof(uint32_t) + kBufE * sizeof(uint16_t)); }
 
@@ -424,6 +430,14 @@ __global__ __launch_bounds__(kBufBlock) void k_bin_buffer(BinArgs a) {
   bin_buffer_body<L, HLL>(a, blockIdx.x, sh);
 }
 
+template <int L, bool HLL>
+__global__ __launch_bounds__(kBufBlock) void k_bin_buffer_m(const BinArgs* __restrict__ args,
+                                                           const uint32_t* __restrict__ blk, uint32_t n_iso) {
+  extern __shared__ uint32_t sh[];
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  bin_buffer_body<L, HLL>(args[iso], blockIdx.x - blk[iso], sh);
+}
+
 __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uint32_t* lds) {
   const uint32_t S = 1u << a.sb;
   const uint64_t first = (uint64_t)b << a.sb;
@@ -512,6 +526,14 @@ static hipError_t pass1_lds_attrs() {
                        reinterpret_cast<const void*>(&k_bin_buffer<3, false>),
                        reinterpret_cast<const void*>(&k_bin_buffer<4, true>),
                        reinterpret_cast<const void*>(&k_bin_buffer<4, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<1, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<1, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<2, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<2, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<3, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<3, false>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<4, true>),
+                       reinterpret_cast<const void*>(&k_bin_buffer_m<4, false>),
                        reinterpret_cast<const void*>(&k_bin_count),
                        reinterpret_cast<const void*>(&k_bin_count_m)};
     hipError_t e = hipSuccess;
@@ -532,7 +554,7 @@ hipError_t check_bin_pass1(uint32_t nbins, uint32_t sb) {
 uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
   if (pass1_lds_attrs() != hipSuccess) return 1;
   const uint32_t L = (2 * k + 63) / 64;
-  const bool buffered = nbins <= kBufMaxBins;
+  const bool buffered = bin_buffered(nbins);
   const void* f = nullptr;
 #define BR(LL)                                                                                 \
   if (buffered) f = hll ? reinterpret_cast<const void*>(&k_bin_buffer<LL, true>)              \
@@ -559,7 +581,7 @@ uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
   if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
-  if (a.reads.n_reads && a.grid && a.nbins <= kBufMaxBins) {
+  if (a.reads.n_reads && a.grid && bin_buffered(a.nbins)) {
     if (bin_buffer_lds(a.nbins) + 1024 > kP2LdsBytes) return hipErrorInvalidConfiguration;
     const uint32_t L = (2 * a.k + 63) / 64;
     const size_t lds = bin_buffer_lds(a.nbins);
@@ -601,11 +623,28 @@ hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid) {
   return hipGetLastError();
 }
 
-hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s) {
+hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, bool buffered,
+                                    hipStream_t s) {
   if (!m.total_blocks) return hipSuccess;
   if (hipError_t e = check_bin_pass1(max_nbins, 6)) return e;  // count LDS checked by its launcher
   const uint32_t L = (2 * k + 63) / 64;
   const BinArgs* a = static_cast<const BinArgs*>(m.args);
+  if (buffered) {
+    if (max_nbins > kBufMaxBins || bin_buffer_lds(max_nbins) + 1024 > kP2LdsBytes) return hipErrorInvalidConfiguration;
+    const size_t lds = bin_buffer_lds(max_nbins);
+    const dim3 g(m.total_blocks), bl(kBufBlock);
+#define BBM(LL)                                                                                    \
+  if (hll) hipLaunchKernelGGL((k_bin_buffer_m<LL, true>), g, bl, lds, s, a, m.blk, m.n_iso);      \
+  else hipLaunchKernelGGL((k_bin_buffer_m<LL, false>), g, bl, lds, s, a, m.blk, m.n_iso)
+    switch (L) {
+      case 1: BBM(1); break;
+      case 2: BBM(2); break;
+      case 3: BBM(3); break;
+      default: BBM(4); break;
+    }
+#undef BBM
+    return hipGetLastError();
+  }
   const size_t lds = bin_scatter_lds(max_nbins);
 #define BSM(LL)                                                                                                   \
   if (hll) hipLaunchKernelGGL((k_bin_scatter_m<LL, true>), dim3(m.total_blocks), dim3(kScatterBlock), lds, s, a, m.blk, \

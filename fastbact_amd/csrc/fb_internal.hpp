@@ -99,6 +99,8 @@ struct BinArgs {
   uint32_t cap_value = 65535;  // counter saturation (15 or 65535)
 };
 size_t bin_scatter_lds(uint32_t nbins);
+// Whether pass 1 with nbins bins uses the per-bin buffer kernel.
+bool bin_buffered(uint32_t nbins);
 // Pass-1 scatter blocks resident on the whole device at once (the grid of
 // the striding form).
 uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins);
@@ -108,6 +110,9 @@ constexpr uint32_t kScatterBlockSize = 512;  // block size of the pass-2 count a
 constexpr uint32_t kP1BlockSize = 512;       // block size of the pass-1 bin scatter kernel
 constexpr uint32_t kBinLog2Cells = 15;       // pass-1 bin: 2^15 Count-Min cells (128 KB of LDS counters)
 constexpr uint32_t kP1Chunk = 64;            // pass-1 offsets per reserved chunk: one 128-byte line
+constexpr uint32_t kP1BufBlock = 1024;       // block size of the pass-1 per-bin buffer kernel
+constexpr uint32_t kP1BufMaxBins = 2048;     // most bins the per-bin buffer kernel holds in LDS
+constexpr uint32_t kP1BufMinBins = 1025;     // fewest bins it is used for
 constexpr uint32_t kP2GateBlockSize = 256;   // block size of the pass-2 gate kernel
 // Thread shares per scatter-block thread: enough blocks to fill the chip a few
 // times over, each flushing its bin histogram once for more work.
@@ -125,7 +130,11 @@ inline uint64_t scatter_blocks(uint64_t threads, uint32_t chunks, uint32_t block
   const uint64_t per = (uint64_t)block * chunks;
   return (threads + per - 1) / per;
 }
-hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, hipStream_t s);
+// buffered: the per-isolate BinArgs carry grid = the isolate's block count
+// and every isolate has at most kP1BufMaxBins bins (the per-bin LDS buffer
+// kernel); otherwise the staged scatter over chunked blocks.
+hipError_t launch_bin_scatter_multi(const MultiLaunch& m, uint32_t k, bool hll, uint32_t max_nbins, bool buffered,
+                                    hipStream_t s);
 // m.blk counts bins: one block per bin.
 hipError_t launch_bin_count_multi(const MultiLaunch& m, uint32_t max_sb, hipStream_t s);
 // Bits of count >= t (saturating-counter semantics: cap 15 or 65535).

@@ -1250,6 +1250,7 @@ struct fb_batch {
   MultiLaunch m1, mc;
   uint32_t n_fill1 = 0, max_nbins = 1, max_sb = 6;
   bool binned = false;
+  bool buffered = false;  // binned with the per-bin buffer kernel
   // pass 2 (LDS isolates only, in lds_idx order)
   std::vector<uint32_t> lds_idx;
   DBuf d_p2, d_p3, d_tabs, d_fill2, d_runs_fill, d_blkg, d_blks, d_blkb;
@@ -1513,6 +1514,20 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
   if (prm.placement == 2 && !binned)
     return set_err(c, FB_ERR_INVALID, "LDS placement needs at most 8192 pass-1 bins per isolate");
   b->binned = binned;
+  // the per-bin buffer kernel when every isolate's bins fit it: each isolate
+  // gets threads / (block * chunks1) blocks striding over its thread shares
+  bool buffered = binned;
+  uint32_t most_bins = 1;
+  for (auto& it : b->iso) {
+    uint32_t sb = kBinLog2Cells;
+    while (sb > 6 && (1ull << sb) > it->cms.width * 4) --sb;
+    most_bins = std::max<uint32_t>(most_bins, (uint32_t)std::max<uint64_t>(1, (it->cms.width * 4) >> sb));
+  }
+  // one kernel serves the whole batch: buffered when the most bins of any
+  // isolate take it (an isolate with fewer bins runs it with its own count)
+  if (!bin_buffered(most_bins)) buffered = false;
+  b->buffered = buffered;
+  const uint32_t blk_threads = buffered ? kP1BufBlock : kP1BlockSize;
   for (uint32_t i = 0; i < n; ++i) {
     BatchIso& it = *b->iso[i];
     const uint64_t ncnt = it.cms.width * 4;
@@ -1527,7 +1542,7 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
     it.binned = binned;
     it.chunks1 = prm.chunks1 ? prm.chunks1 : p1_chunks(it.threads1);
     if (binned) {
-      const uint64_t n_wg = scatter_blocks(it.threads1, it.chunks1, kP1BlockSize);
+      const uint64_t n_wg = scatter_blocks(it.threads1, it.chunks1, blk_threads);
       const uint64_t cap =
           std::min<uint64_t>(bin_slice_cap(4.0 * (double)it.windows / (double)nbins, n_wg), kP1MaxCap);
       const uint64_t need = nbins * kBinSlices * cap * 2;
@@ -1568,7 +1583,8 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
     bi.bits = it.bits.as<uint32_t>();
     bi.t_solid = t_solid;
     bi.cap_value = it.cms.cap();
-    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(it.threads1, it.chunks1, kP1BlockSize)
+    bi.grid = buffered ? (uint32_t)scatter_blocks(it.threads1, it.chunks1, blk_threads) : 0u;
+    blk1[i + 1] = blk1[i] + (uint32_t)(binned ? scatter_blocks(it.threads1, it.chunks1, blk_threads)
                                               : (it.threads1 + kPassBlock - 1) / kPassBlock);
     blkc[i + 1] = blkc[i] + (binned ? it.nbins : 0u);
   }
@@ -1583,7 +1599,7 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
   b->m1 = MultiLaunch{binned ? b->d_bin.p : b->d_p1.p, b->d_blk1.as<uint32_t>(), n, blk1[n]};
   b->mc = MultiLaunch{b->d_bin.p, b->d_blkc.as<uint32_t>(), n, blkc[n]};
   b->n_fill1 = (uint32_t)fills.size();
-  const std::vector<uint64_t> sig = {binned, b->n_fill1, blk1[n], blkc[n], b->max_nbins, b->max_sb};
+  const std::vector<uint64_t> sig = {binned, buffered, b->n_fill1, blk1[n], blkc[n], b->max_nbins, b->max_sb};
   if (b->sig1[k] != sig) *changed = true;
   b->sig1[k] = sig;
   return FB_OK;
@@ -1596,7 +1612,7 @@ static int batch_enqueue1(fb_batch* b, uint32_t k, uint32_t t_solid) {
   const bool hll = b->prm.hll_p != 0;
   HIP_TRY(c, launch_fill_ranges(b->d_fill1.as<FillRange>(), b->n_fill1, s));
   if (b->binned) {
-    HIP_TRY(c, launch_bin_scatter_multi(b->m1, k, hll, b->max_nbins, s));
+    HIP_TRY(c, launch_bin_scatter_multi(b->m1, k, hll, b->max_nbins, b->buffered, s));
     HIP_TRY(c, launch_bin_count_multi(b->mc, b->max_sb, s));
   } else {
     HIP_TRY(c, launch_pass1_multi(b->m1, k, true, hll, s));

@@ -42,6 +42,26 @@ def test_batch_equals_oracle_per_isolate(ctx, orc, isolates, k, target, placemen
             assert np.array_equal(got.hll_registers, ref["hll"])
 
 
+@pytest.mark.parametrize("width", [1 << 22, 1 << 24, 1 << 25])
+def test_batch_pass1_forms_equal_oracle(ctx, orc, isolates, width):
+    """Pass 1 of a batch runs the per-bin LDS buffer kernel when the isolates
+    have 1025 to 2048 bins (width 2^24: 2048 bins) and the staged scatter
+    otherwise (2^22: 512 bins, 2^25: 4096 bins); both give each isolate the
+    oracle's table, registers and solid set, the empty isolate included."""
+    banks = [_bank(x) for x in isolates]
+    dev = [F.DeviceBank(ctx, b) for b in banks]
+    batch = F.Batch(ctx, dev, [50_000] * len(banks), cms_width=width, counter_bits=16, hll_p=12, placement=2)
+    for rep in range(2):
+        batch.run(31, 3)
+        for i, b in enumerate(banks):
+            got = batch.result(i, 31)
+            ref = orc.build_solid(oracle_arenas(b), 31, target_solid_cov=3, width=width, counter_bits=16, hll_p=12)
+            assert np.array_equal(got.keys, ref["keys"]), (rep, i)
+            assert np.array_equal(got.cov, ref["cov"]) and np.array_equal(got.mask, ref["mask"])
+            assert np.array_equal(got.cms_table, ref["cms"].table())
+            assert np.array_equal(got.hll_registers, ref["hll"])
+
+
 def test_batch_multi_k_reuses_buffers_and_matches_single(ctx, isolates):
     banks = [_bank(x) for x in isolates[:4]]
     dev = [F.DeviceBank(ctx, b) for b in banks]

@@ -114,15 +114,16 @@ def test_saturation_nibble_and_u16(ctx, orc):
 @pytest.mark.parametrize("k", [21, 55, 127])
 def test_pass1_scatter_forms_agree(ctx, orc, small_iso, k):
     """Both pass-1 scatter kernels give the oracle's table and solid set: the
-    per-bin LDS buffers (resident grid, up to 2048 bins: widths 2^18, 2^22)
-    and the staged counting sort (a chunked grid, or 4096 bins at 2^25).
-    Low-complexity reads pile their cells into a few bins in every round, so
-    the buffers' retries and their direct-add fallback run too."""
+    per-bin LDS buffers (resident grid, 1025 to 2048 bins: width 2^24) and
+    the staged counting sort (32 and 512 bins at widths 2^18 and 2^22, a
+    chunked grid, 4096 bins at 2^25).  Low-complexity reads pile their cells
+    into a few bins in every round, so the buffers' retries and their
+    direct-add fallback run too."""
     base = small_iso.bank()
     low = [b"A" * 150] * 300 + [b"AC" * 75] * 300 + [b"AAC" * 50] * 300
     bank = F.ReadBank(F.BitSeqArena.from_ascii(low), base.pe_forward, base.pe_reverse)
     dev = F.DeviceBank(ctx, bank)
-    for width, chunks1 in ((1 << 18, 0), (1 << 18, 2), (1 << 22, 0), (1 << 25, 0)):
+    for width, chunks1 in ((1 << 18, 0), (1 << 18, 2), (1 << 22, 0), (1 << 24, 0), (1 << 25, 0)):
         g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=3, cms_width=width, hll_p=14, placement=2,
                                 chunks1=chunks1)
         r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=3, width=width, hll_p=14)

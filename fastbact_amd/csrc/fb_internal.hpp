@@ -185,7 +185,14 @@ struct P2BinArgs {
   uint32_t log2_rounds = 0;
   uint32_t round = 0;            // round of this launch sequence
   uint64_t* gate_masks = nullptr;  // n_reads * split, gate bit of window p0+i at bit i
-  uint32_t rw = 1;               // 64-bit words per record: ceil((2k + 8) / 64)
+  uint32_t rw = 1;               // 64-bit words per record: ceil((2k + 8 + cflag) / 64)
+  // Prior contigs (the multi-k upgrade): their windows follow the reads' in
+  // the share space, csplit shares of at most 63 pairs per contig, ungated;
+  // their records carry a ninth info bit (cflag = 1) and add the contig
+  // weights (cov +100 a side, edge +1000) in the bin kernel.
+  DevReads contigs;
+  uint32_t csplit = 1;
+  uint32_t cflag = 0;
   // nbins * kBinSlices + 1 run counters, zeroed before every round: record
   // counts per (bin, slice), then (after the scan) region starts; the last
   // entry holds the round's record total.

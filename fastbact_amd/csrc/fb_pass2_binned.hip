@@ -55,7 +55,8 @@ __device__ __forceinline__ void pack62b(const Kmer<L>& x, uint64_t (&w)[KW]) {
 }
 
 // Record layout: bits [0, 2k) hold the k-mer integer (limbs least significant
-// first), bits [2k, 2k+8) the edge byte; rw = ceil((2k+8)/64) words.
+// first), bits [2k, 2k+8) the edge byte, and with prior contigs bit 2k+8 the
+// contig flag; rw = ceil((2k+8+cflag)/64) words.
 template <int L>
 __device__ __forceinline__ void pack_record(const Kmer<L>& key, uint32_t info, uint32_t k, uint64_t (&w)[L + 1]) {
   const uint32_t q = (2u * k) >> 6, s = (2u * k) & 63u;
@@ -63,7 +64,7 @@ __device__ __forceinline__ void pack_record(const Kmer<L>& key, uint32_t info, u
   for (int j = 0; j <= L; ++j) {
     uint64_t v = j < L ? key.a[j] : 0ull;
     if ((uint32_t)j == q) v |= (uint64_t)info << s;
-    if (s > 56u && (uint32_t)j == q + 1u) v |= (uint64_t)info >> (64u - s);
+    if (s > 55u && (uint32_t)j == q + 1u) v |= (uint64_t)info >> (64u - s);
     w[j] = v;
   }
 }
@@ -81,8 +82,8 @@ __device__ __forceinline__ void unpack_record(const uint64_t* __restrict__ src, 
     if ((uint32_t)j == q + 1u) hi = w[j];
   }
   uint64_t v = lo >> s;
-  if (s > 56u) v |= hi << (64u - s);
-  info = (uint32_t)v & 0xffu;
+  if (s > 55u) v |= hi << (64u - s);
+  info = (uint32_t)v & 0x1ffu;
 #pragma unroll
   for (int j = 0; j < L; ++j) key.a[j] = w[j];
   key.a[L - 1] &= top_mask<L>(k);
@@ -99,6 +100,17 @@ __device__ __forceinline__ bool p2_share(const DevReads& R, uint64_t gid, uint32
   p0 = (uint32_t)((uint64_t)part * units / split);
   p1 = (uint32_t)((uint64_t)(part + 1) * units / split);
   return p0 < p1;
+}
+
+// Share gid of the pass-2 share space: the reads' shares (n_reads * split),
+// then the prior contigs' (contigs.n_reads * csplit).  src is the list the
+// share's sequence r belongs to.
+__device__ __forceinline__ bool p2_share_any(const P2BinArgs& a, uint64_t gid, const DevReads*& src, uint64_t& r,
+                                             uint32_t& p0, uint32_t& p1, bool& contig) {
+  const uint64_t nr = a.reads.n_reads * (uint64_t)a.split;
+  contig = gid >= nr;
+  src = contig ? &a.contigs : &a.reads;
+  return contig ? p2_share(a.contigs, gid - nr, a.csplit, a.k, r, p0, p1) : p2_share(a.reads, gid, a.split, a.k, r, p0, p1);
 }
 
 // Gate of one Count-Min row over the windows p0..p1 of read r (p1 - p0 <=
@@ -158,11 +170,11 @@ __global__ __launch_bounds__(kP2GateBlock) void k_p2_gate_m(const P2BinArgs* __r
 // window p0+i at bit i of gates.  cb(key, fold, left, e_left, right, e_right)
 // is called for every window with a record (build.rs:253-285).
 template <int L, class CB>
-__device__ __forceinline__ void p2_walk(const P2BinArgs& a, uint64_t r, uint32_t p0, uint32_t p1, uint64_t gates,
-                                        CB&& cb) {
+__device__ __forceinline__ void p2_walk(const P2BinArgs& a, const DevReads& R, uint64_t r, uint32_t p0, uint32_t p1,
+                                        uint64_t gates, CB&& cb) {
   const uint32_t k = a.k;
-  const uint64_t* __restrict__ data = a.reads.data;
-  const uint64_t g = a.reads.offs[r] + p0;
+  const uint64_t* __restrict__ data = R.data;
+  const uint64_t g = R.offs[r] + p0;
   Kmer<L> win = window_bits<L>(data, g, k);
   Kmer<L> fwd = fwd_from_window<L>(win, k);
   Kmer<L> rev = rc_from_window<L>(win, k);
@@ -226,14 +238,22 @@ __device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, 
     const uint64_t gid = gid0 + (uint64_t)ch * blockDim.x;
     uint64_t r = 0;
     uint32_t p0 = 0, p1 = 0;
-    if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
-    uint64_t m = a.gate_masks[gid];
-    if (round == 0) {
-      if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
-      a.gate_masks[gid] = m;
+    const DevReads* src = nullptr;
+    bool contig = false;
+    if (!p2_share_any(a, gid, src, r, p0, p1, contig)) continue;
+    uint64_t m;
+    if (contig) {  // every window of a prior contig (build.rs:198-239)
+      m = (p1 - p0 >= 63u) ? ~0ull : (2ull << (p1 - p0)) - 1ull;
+      if (round == 0) a.gate_masks[gid] = m;
+    } else {
+      m = a.gate_masks[gid];
+      if (round == 0) {
+        if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
+        a.gate_masks[gid] = m;
+      }
     }
     if (m)
-      p2_walk<L>(a, r, p0, p1, m, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
+      p2_walk<L>(a, *src, r, p0, p1, m, [&](const Kmer<L>&, uint64_t fold, bool, uint32_t, bool, uint32_t) {
         if (p2_in_round(a, fold, round)) atomicAdd(hist + p2_bin_of(a, fold), 1u);
       });
   }
@@ -365,10 +385,13 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk
     const uint64_t gid = gid0 + (uint64_t)ch * blockDim.x;
     uint64_t r = 0;
     uint32_t p0 = 0, p1 = 0;
-    if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) continue;
+    const DevReads* src = nullptr;
+    bool contig = false;
+    if (!p2_share_any(a, gid, src, r, p0, p1, contig)) continue;
     const uint64_t gates = a.gate_masks[gid];
     if (!gates) continue;
-    p2_walk<L>(a, r, p0, p1, gates,
+    const uint32_t cbit = contig ? 256u : 0u;
+    p2_walk<L>(a, *src, r, p0, p1, gates,
                [&](const Kmer<L>& key, uint64_t fold, bool left, uint32_t el, bool right, uint32_t er) {
                  if (!p2_in_round(a, fold, round)) return;
                  const uint32_t pos = atomicAdd(cur + p2_bin_of(a, fold), 1u);
@@ -377,7 +400,7 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk
                    return;
                  }
                  uint64_t w[L + 1];
-                 pack_record<L>(key, (left ? 1u : 0u) | (el << 1) | (right ? 16u : 0u) | (er << 5), k, w);
+                 pack_record<L>(key, (left ? 1u : 0u) | (el << 1) | (right ? 16u : 0u) | (er << 5) | cbit, k, w);
                  uint64_t* dst = a.recs + (uint64_t)pos * rw;
 #pragma unroll
                  for (int j = 0; j <= L; ++j)
@@ -464,28 +487,45 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
     pack62b<L, KW>(key, w);
     const bool left = info & 1u, right = (info >> 4) & 1u;
     const uint32_t el = (info >> 1) & 7u, er = (info >> 5) & 7u;
+    // a prior contig's side adds cov 100 and edge 1000 (build.rs:219-231)
+    const bool contig = (info >> 8) & 1u;
+    const uint32_t cw = contig ? 100u : 1u;
     const int e = lds_find<KW>(keys, cap, w);
     if (e >= 0) {
-      atomicAdd(cov + e, (uint32_t)left + (uint32_t)right);
-      // u16 edge counters, two per word; an increment happens only while the
-      // counter is below the gate, so it stays below gate + blockDim < 2^16
-      if (left) {
-        uint32_t* ew = edges + 4u * (uint32_t)e + (el >> 1);
-        const uint32_t sh = (el & 1u) * 16u;
-        if (((*reinterpret_cast<volatile uint32_t*>(ew) >> sh) & 0xffffu) < gate) atomicAdd(ew, 1u << sh);
-      }
-      if (right) {
-        uint32_t* ew = edges + 4u * (uint32_t)e + (er >> 1);
-        const uint32_t sh = (er & 1u) * 16u;
-        if (((*reinterpret_cast<volatile uint32_t*>(ew) >> sh) & 0xffffu) < gate) atomicAdd(ew, 1u << sh);
+      atomicAdd(cov + e, cw * ((uint32_t)left + (uint32_t)right));
+      // u16 edge counters, two per word; a read's increment happens only
+      // while the counter is below the gate, so it stays below gate +
+      // blockDim < 2^16; a contig's +1000 saturates at 65535 (compare and
+      // swap), and from there on reads no longer add
+      for (int side = 0; side < 2; ++side) {
+        if (!(side ? right : left)) continue;
+        const uint32_t ed = side ? er : el;
+        uint32_t* ew = edges + 4u * (uint32_t)e + (ed >> 1);
+        const uint32_t sh = (ed & 1u) * 16u;
+        if (contig) {
+          uint32_t old = *reinterpret_cast<volatile uint32_t*>(ew), seen;
+          do {
+            seen = old;
+            const uint32_t hv = (seen >> sh) & 0xffffu;
+            const uint32_t nv = hv + 1000u < 65535u ? hv + 1000u : 65535u;
+            old = atomicCAS(ew, seen, (seen & ~(0xffffu << sh)) | (nv << sh));
+          } while (old != seen);
+        } else if (((*reinterpret_cast<volatile uint32_t*>(ew) >> sh) & 0xffffu) < gate) {
+          atomicAdd(ew, 1u << sh);
+        }
       }
     } else {
       // LDS table full: this key lives in the global table for the whole pass
       uint32_t* c = table_find_global<KW>(a.table, w, slot_hash_words<KW>(w));
       if (c) {
-        atomicAdd(c, (uint32_t)left + (uint32_t)right);
-        if (left && c[2 + el] < gate) atomicAdd(c + 2 + el, 1u);
-        if (right && c[2 + er] < gate) atomicAdd(c + 2 + er, 1u);
+        atomicAdd(c, cw * ((uint32_t)left + (uint32_t)right));
+        if (contig) {
+          if (left) atomicAdd(c + 2 + el, 1000u);
+          if (right) atomicAdd(c + 2 + er, 1000u);
+        } else {
+          if (left && c[2 + el] < gate) atomicAdd(c + 2 + el, 1u);
+          if (right && c[2 + er] < gate) atomicAdd(c + 2 + er, 1u);
+        }
       }
     }
   }
@@ -646,14 +686,15 @@ static hipError_t dispatch_lkw(uint32_t k, F&& f) {
 
 struct P2Launch {
   const P2BinArgs& a;
-  uint64_t threads;
+  uint64_t threads;       // all shares: reads, then prior contigs
+  uint64_t read_threads;  // the reads' shares (the gate kernels' grid)
   size_t scat_lds, lds;
   hipStream_t s;
   hipEvent_t gated, scattered;
   template <int L, int KW>
   hipError_t operator()() {
-    if (threads) {
-      const unsigned ggrid = (unsigned)((threads + kP2GateBlock - 1) / kP2GateBlock);
+    if (read_threads) {
+      const unsigned ggrid = (unsigned)((read_threads + kP2GateBlock - 1) / kP2GateBlock);
       for (uint32_t row = 0; row < 3; ++row)
         hipLaunchKernelGGL((k_p2_gate<L>), dim3(ggrid), dim3(kP2GateBlock), 0, s, a, row);
     }
@@ -693,10 +734,13 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated,
   const size_t scat_lds = p2_scatter_lds(a.nbins);
   const size_t lds = p2_lds_bytes(KW, a.lds_cap);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
-  if (a.rw != (2 * a.k + 8 + 63) / 64) return hipErrorInvalidValue;
-  if (a.n_blocks != scatter_blocks(a.reads.n_reads * (uint64_t)a.split, a.chunks)) return hipErrorInvalidValue;
+  if (a.cflag > 1u || (a.contigs.n_reads && !a.cflag)) return hipErrorInvalidValue;
+  if (a.rw != (2 * a.k + 8 + a.cflag + 63) / 64) return hipErrorInvalidValue;
+  const uint64_t read_threads = a.reads.n_reads * (uint64_t)a.split;
+  const uint64_t threads = read_threads + a.contigs.n_reads * (uint64_t)a.csplit;
+  if (a.n_blocks != scatter_blocks(threads, a.chunks)) return hipErrorInvalidValue;
   if (a.log2_rounds > kP2MaxLog2Rounds || a.nbins != (1u << a.log2_nbins)) return hipErrorInvalidValue;
-  return dispatch_lkw(a.k, P2Launch{a, a.reads.n_reads * (uint64_t)a.split, scat_lds, lds, s, gated, scattered});
+  return dispatch_lkw(a.k, P2Launch{a, threads, read_threads, scat_lds, lds, s, gated, scattered});
 }
 
 struct P2LaunchMulti {

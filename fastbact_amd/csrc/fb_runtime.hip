@@ -839,7 +839,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   uint64_t rec_sum = 0, claimed = 0;
   uint64_t route = 0;
   bool done = false;
-  // LDS-privatised pass 2+3 (reads only; prior contigs use the global table)
+  // LDS-privatised pass 2+3 (reads, then the prior contigs' windows)
   const uint32_t kw = tab.kw;
   const uint32_t lds_cap = p2_lds_cap(kw);
   // bins over all rounds: the estimated keys at half the LDS table capacity
@@ -861,11 +861,19 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   so->stats[7] = (uint64_t)n_est_keys;
   const uint32_t pairs_max = bank->max_len > k ? bank->max_len - k : 1;
   const uint32_t split2 = std::max<uint32_t>(split, (pairs_max + 62) / 63);
-  const uint64_t threads2 = bank->n_reads * (uint64_t)split2;
+  // prior contigs follow the reads in the share space, at most 63 pairs a share
+  const uint32_t cpairs_max = contig_max > k ? contig_max - k : 1;
+  const uint32_t csplit2 = (cpairs_max + 62) / 63;
+  uint64_t contig_windows = 0;
+  for (uint64_t i = 0; i < n_contigs; ++i)
+    if (contig_lens[i] >= k) contig_windows += contig_lens[i] - k + 1;
+  const uint64_t threads2 = bank->n_reads * (uint64_t)split2 + c_reads * (uint64_t)csplit2;
   // a window at a share boundary can give one record to each of two threads
-  const uint64_t rec_cap = windows + threads2;
-  if (!c_reads && prm->placement != 1 && nb2 <= max_bins && windows > 0 && rec_cap < (1ull << 32)) {
-    const uint32_t rw = (2 * k + 8 + 63) / 64;
+  const uint64_t rec_cap = windows + contig_windows + threads2;
+  if (prm->placement != 1 && nb2 <= max_bins && windows > 0 && rec_cap < (1ull << 32) &&
+      c_reads * (uint64_t)csplit2 <= (1ull << 26)) {
+    const uint32_t cflag = c_reads ? 1u : 0u;
+    const uint32_t rw = (2 * k + 8 + cflag + 63) / 64;
     // keys beyond what the LDS tables are expected to hold, plus slack
     const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
     const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est_keys / 4)));
@@ -894,6 +902,9 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     HIP_TRY(c, launch_fill_u32(st2.as<uint32_t>(), 16, 0, s));
     P2BinArgs pa;
     pa.reads = bank->view();
+    pa.contigs = cview;
+    pa.csplit = csplit2;
+    pa.cflag = cflag;
     pa.k = k;
     pa.split = split2;
     pa.chunks = chunks2;

@@ -86,6 +86,32 @@ def test_prior_contigs_weights(ctx, orc, small_iso, k):
     assert np.array_equal(g.cms_table, r["cms"].table())
 
 
+@pytest.mark.parametrize("k", [21, 31, 55, 77, 127])
+def test_prior_contigs_both_pass2_routes(ctx, orc, small_iso, k):
+    """Prior contigs through the LDS pass 2 (their windows in shares of at
+    most 63 pairs after the reads', records flagged as contig) and through
+    the global table: a contig over many shares, contigs of k and k + 1 bases,
+    contigs repeated until their edge counters (+1000 each) and cov (+100 a
+    side) saturate, and few bins, which sends keys past the LDS tables into
+    the global table.  Every form gives the oracle's solid set."""
+    bank = small_iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    gen = small_iso.genome.tobytes()
+    contigs = ([gen[0:9000], gen[9000:9000 + k], gen[9100:9100 + k + 1], gen[15000:15000 + 3 * k]]
+               + [gen[20000:20600]] * 70 + [gen[22000:22050 + k]] * 400)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=4, width=1 << 18, contigs=contigs)
+    for placement, bins2 in ((1, 0), (2, 0), (2, 2)):
+        g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=4, cms_width=1 << 18, contigs=contigs,
+                                placement=placement, bins2=bins2)
+        _solid_equal(g, r)
+        h = F.build_solid_handle(ctx, dev, k, target_solid_cov=4, cms_width=1 << 18, contigs=contigs,
+                                 placement=placement, bins2=bins2)
+        st = h.stats()
+        h.close()
+        assert st["pass2_route"] == ("lds" if placement == 2 else "global"), st["pass2_route"]
+    assert (r["cov"] == 65535 // 2).any()  # saturated cov of the contig repeated 400 times
+
+
 def test_empty_bank(ctx, orc):
     bank = F.ReadBank()
     dev = F.DeviceBank(ctx, bank)

@@ -167,33 +167,28 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_pred(CgArgs a) {
       up = (tp != kNone && (tp ^ 1u) == (p ^ 1u)) ? kStDead : p;
     }
   }
-  a.up[0][d] = up;
-  a.head[0][d] = head;
-  a.rank[0][d] = (up == kStOff || up == kStDead) ? 0u : 1u;
-  a.lab[0][d] = (uint32_t)d;
+  a.js[0][d] = make_uint4(up, head, (up == kStOff || up == kStDead) ? 0u : 1u, (uint32_t)d);
 }
 
 // One pointer-jumping round: up/head/rank/lab from buffer s into buffer t.
 __global__ __launch_bounds__(kCgBlock) void k_cg_jump(CgArgs a, uint32_t s, uint32_t t) {
   const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= 2ull * a.n) return;
-  const uint32_t u = a.up[s][d];
-  uint32_t nu = u, nh = a.head[s][d], nr = a.rank[s][d], nl = a.lab[s][d];
+  const uint4 me = a.js[s][d];
+  const uint32_t u = me.x;
+  uint32_t nu = u, nh = me.y, nr = me.z, nl = me.w;
   if (u < kStHead) {
-    const uint32_t uu = a.up[s][u];
-    nl = min(nl, a.lab[s][u]);
-    if (uu == kStDead) {
+    const uint4 o = a.js[s][u];
+    nl = min(nl, o.w);
+    if (o.x == kStDead) {
       nu = kStDead;
     } else {
-      nu = uu;
-      nr += a.rank[s][u];
-      nh = a.head[s][u];
+      nu = o.x;
+      nr += o.z;
+      nh = o.y;
     }
   }
-  a.up[t][d] = nu;
-  a.head[t][d] = nh;
-  a.rank[t][d] = nr;
-  a.lab[t][d] = nl;
+  a.js[t][d] = make_uint4(nu, nh, nr, nl);
 }
 
 // Start edges: the start node's cov and itself as the last node so far.
@@ -243,15 +238,16 @@ __global__ __launch_bounds__(kAccBlock) void k_cg_accumulate(CgArgs a, uint32_t 
   __syncthreads();
   const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d < 2ull * a.n) {
-    const uint32_t u = a.up[s][d];
+    const uint4 me = a.js[s][d];
+    const uint32_t u = me.x;
     uint32_t tag = kNone;
     unsigned long long v = 0, m = 0;
     if (u == kStHead) {
-      tag = a.head[s][d];
+      tag = me.y;
       v = a.cov[d >> 1];
-      m = ((unsigned long long)a.rank[s][d] << 32) | (unsigned long long)d;
+      m = ((unsigned long long)me.z << 32) | (unsigned long long)d;
     } else if (u < kStHead) {  // on a cycle: lab = smallest node of the cycle
-      tag = a.lab[s][d] | kCycleTag;
+      tag = me.w | kCycleTag;
       v = a.cov[d >> 1];
       m = 1ull;
     }
@@ -314,9 +310,10 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_cycles(CgArgs a, uint32_t s) {
   const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= 2ull * a.n) return;
   uint32_t len = 0;
-  if (a.up[s][d] < kStHead && a.lab[s][d] == (uint32_t)d && !(d & 1u)) {
-    const uint32_t tw = (uint32_t)d ^ 1u;
-    const bool self_twin = a.up[s][tw] < kStHead && a.lab[s][tw] == (uint32_t)d;
+  const uint4 me = a.js[s][d];
+  if (me.x < kStHead && me.w == (uint32_t)d && !(d & 1u)) {
+    const uint4 tw = a.js[s][d ^ 1u];
+    const bool self_twin = tw.x < kStHead && tw.w == (uint32_t)d;
     if (!self_twin) len = a.clen[d];
   }
   a.cyc_len[d] = len;  // bases of the circular unitig (one per node), 0 = none
@@ -327,10 +324,11 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_cycles(CgArgs a, uint32_t s) {
 __global__ __launch_bounds__(kCgBlock) void k_cg_cycle_init(CgArgs a, uint32_t s, uint32_t t) {
   const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= 2ull * a.n) return;
-  const uint32_t u = a.up[s][d];
+  const uint4 me = a.js[s][d];
+  const uint32_t u = me.x;
   uint32_t nu = kStOff, nr = 0;
   if (u < kStHead) {
-    const uint32_t c = a.lab[s][d];
+    const uint32_t c = me.w;
     if (a.cyc_len[c]) {
       if (c == (uint32_t)d) {
         nu = kStHead;
@@ -341,10 +339,7 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_cycle_init(CgArgs a, uint32_t s
       }
     }
   }
-  a.up[t][d] = nu;
-  a.rank[t][d] = nr;
-  a.head[t][d] = kNone;
-  a.lab[t][d] = (uint32_t)d;
+  a.js[t][d] = make_uint4(nu, kNone, nr, (uint32_t)d);
 }
 
 // The strand of k-mer d written as ASCII at out[0..cnt).
@@ -379,17 +374,19 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_write(CgArgs a, uint32_t sp, ui
     return;
   }
   // counted node of a path (sp: the buffer of the first jump)
-  if (a.up[sp][d] == kStHead) {
-    const uint32_t e = a.head[sp][d];
-    if (a.walked[e]) a.seq[a.soff[e] + k + a.rank[sp][d]] = "ACGT"[a.obase[d]];
+  const uint4 jp = a.js[sp][d];
+  if (jp.x == kStHead) {
+    const uint32_t e = jp.y;
+    if (a.walked[e]) a.seq[a.soff[e] + k + jp.z] = "ACGT"[a.obase[d]];
     return;
   }
   // node of a walked cycle (sc: the buffer of the cycle jump)
-  if (a.up[sc][d] == kStHead) {
-    const uint32_t c = a.lab[sp][d];  // cycle start (label of the first jump)
+  const uint4 jc = a.js[sc][d];
+  if (jc.x == kStHead) {
+    const uint32_t c = jp.w;  // cycle start (label of the first jump)
     const uint32_t L_ = a.cyc_len[c];
     const uint64_t off = a.circ_base + a.coff[c];
-    const uint32_t r = a.rank[sc][d];
+    const uint32_t r = jc.z;
     if (r == 0) cg_write_kmer<L>(a, (uint32_t)d, a.seq + off, k < L_ ? k : L_);
     if (k + r < L_) a.seq[off + k + r] = "ACGT"[a.obase[d]];
   }

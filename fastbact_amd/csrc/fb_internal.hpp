@@ -284,10 +284,10 @@ struct CgArgs {
   uint8_t *kind = nullptr, *fbase = nullptr, *lbase = nullptr, *obase = nullptr;  // 2n each
   uint32_t *nstart = nullptr, *sbase = nullptr, *succ = nullptr;                // 2n each
   uint32_t* bnext = nullptr;                                                    // 8n
-  uint32_t* up[4] = {};    // jump buffers 0/1 (paths) and 2/3 (cycles), 2n each
-  uint32_t* head[4] = {};
-  uint32_t* rank[4] = {};
-  uint32_t* lab[4] = {};
+  // jump buffers 0/1 (paths) and 2/3 (cycles), 2n each: per directed node
+  // x = up (next node towards the head, or a state), y = head (start edge),
+  // z = rank, w = lab (smallest node seen), one 16-byte gather per jump
+  uint4* js[4] = {};
   uint64_t n_se = 0;       // start edges
   unsigned long long* csum = nullptr;  // n_se: cov sum of the path's counted nodes
   unsigned long long* last = nullptr;  // n_se: rank << 32 | last counted node

@@ -170,10 +170,19 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_pred(CgArgs a) {
   a.js[0][d] = make_uint4(up, head, (up == kStOff || up == kStDead) ? 0u : 1u, (uint32_t)d);
 }
 
-// One pointer-jumping round: up/head/rank/lab from buffer s into buffer t.
-__global__ __launch_bounds__(kCgBlock) void k_cg_jump(CgArgs a, uint32_t s, uint32_t t) {
+// One pointer-jumping round r: up/head/rank/lab from buffer s into buffer t.
+// A round that turns no node terminal and lowers no label ends the jump:
+// round r turns terminal the nodes 2^r < D <= 2^(r+1) links behind a head
+// (D = 1 starts terminal), and gives a cycle's smallest node as label to the
+// cycle nodes 2^r <= D < 2^(r+1) links after it, so a round with neither
+// means every path is resolved and every cycle labelled.
+// The later rounds return at once; both buffers then agree on everything the
+// later kernels read (the state of path nodes, and lab and up < kStHead of
+// cycle nodes, whose up and rank keep moving).
+__global__ __launch_bounds__(kCgBlock) void k_cg_jump(CgArgs a, uint32_t s, uint32_t t, uint32_t* flag, uint32_t r) {
   const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= 2ull * a.n) return;
+  if (r > 0 && !flag[r - 1]) return;
   const uint4 me = a.js[s][d];
   const uint32_t u = me.x;
   uint32_t nu = u, nh = me.y, nr = me.z, nl = me.w;
@@ -189,6 +198,7 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_jump(CgArgs a, uint32_t s, uint
     }
   }
   a.js[t][d] = make_uint4(nu, nh, nr, nl);
+  if ((u < kStHead && nu >= kStHead) || nl != me.w) flag[r] = 1u;
 }
 
 // Start edges: the start node's cov and itself as the last node so far.
@@ -519,8 +529,11 @@ uint32_t cg_jump_rounds(uint64_t nodes) {
 // buffer holding the result.  Every chain of at most 2n links resolves.
 hipError_t launch_cg_jump(const CgArgs& a, uint32_t s, uint32_t t, uint32_t* out_s, hipStream_t st) {
   const uint32_t rounds = cg_jump_rounds(2ull * a.n);
+  if (rounds > 64) return hipErrorInvalidValue;
+  uint32_t* flag = a.jflag + (s == 0 ? 0 : 64);  // the path jump starts in buffer 0, the cycle jump in 2
+  if (hipError_t e = launch_fill_u32(flag, 64, 0u, st)) return e;
   for (uint32_t r = 0; r < rounds; ++r) {
-    hipLaunchKernelGGL(k_cg_jump, dim3(cg_grid(2ull * a.n)), dim3(kCgBlock), 0, st, a, s, t);
+    hipLaunchKernelGGL(k_cg_jump, dim3(cg_grid(2ull * a.n)), dim3(kCgBlock), 0, st, a, s, t, flag, r);
     const uint32_t x = s;
     s = t;
     t = x;

@@ -288,6 +288,9 @@ struct CgArgs {
   // x = up (next node towards the head, or a state), y = head (start edge),
   // z = rank, w = lab (smallest node seen), one 16-byte gather per jump
   uint4* js[4] = {};
+  // 2 x 64 progress flags, zeroed before each jump (paths, then cycles):
+  // flag r is set when round r turned a node terminal or lowered a label
+  uint32_t* jflag = nullptr;
   uint64_t n_se = 0;       // start edges
   unsigned long long* csum = nullptr;  // n_se: cov sum of the path's counted nodes
   unsigned long long* last = nullptr;  // n_se: rank << 32 | last counted node

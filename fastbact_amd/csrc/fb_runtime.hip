@@ -2663,7 +2663,7 @@ static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, 
   a.n = (uint32_t)n;
   a.k = k;
   a.nref = ref_words(k);
-  DBuf kind, fbase, lbase, obase, nstart, sbase, succ, bnext, jb[4], ccov, clen, cyc_len, cid, coff, tmp;
+  DBuf kind, fbase, lbase, obase, nstart, sbase, succ, bnext, jb[4], jflag, ccov, clen, cyc_len, cid, coff, tmp;
   int r = 0;
   if (!r) r = kind.get(c, N);
   if (!r) r = fbase.get(c, N);
@@ -2674,6 +2674,7 @@ static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, 
   if (!r) r = succ.get(c, N * 4);
   if (!r) r = bnext.get(c, N * 16);
   for (int j = 0; j < 4 && !r; ++j) r = jb[j].get(c, N * 16);
+  if (!r) r = jflag.get(c, 128 * 4);
   if (!r) r = ccov.get(c, N * 8);
   if (!r) r = clen.get(c, N * 4);
   if (!r) r = cyc_len.get(c, N * 4);
@@ -2689,6 +2690,7 @@ static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, 
   a.succ = succ.as<uint32_t>();
   a.bnext = bnext.as<uint32_t>();
   for (int j = 0; j < 4; ++j) a.js[j] = jb[j].as<uint4>();
+  a.jflag = jflag.as<uint32_t>();
   a.ccov = ccov.as<unsigned long long>();
   a.clen = clen.as<uint32_t>();
   a.cyc_len = cyc_len.as<uint32_t>();

@@ -19,14 +19,18 @@ inline uint64_t sm64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-uint8_t code_of(uint8_t c) {
-  switch (c) {
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return 0;  // A and every non-ACGT byte (bitseq.rs:67-81)
+// ASCII -> 2-bit code: C/c 1, G/g 2, T/t 3, A and every other byte 0
+// (bitseq.rs:67-81), as a table so that packing does not branch per base.
+struct CodeTable {
+  uint8_t v[256];
+  constexpr CodeTable() : v() {
+    v['C'] = v['c'] = 1;
+    v['G'] = v['g'] = 2;
+    v['T'] = v['t'] = 3;
   }
-}
+};
+constexpr CodeTable kCodes;
+inline uint8_t code_of(uint8_t c) { return kCodes.v[c]; }
 
 }  // namespace
 
@@ -266,6 +270,14 @@ extern "C" int fb_pack_ascii(const uint8_t* ascii, const uint32_t* lengths, uint
         for (int b = 0; b < 32; ++b) w |= (uint64_t)code_of(ascii[j + b]) << (2 * b);
         data[g >> 5] = w;
         j += 32;
+        // then whole words for as long as they last
+        while (j + 32 <= len) {
+          uint64_t x = 0;
+          const uint8_t* a = ascii + j;
+          for (int b = 0; b < 32; ++b) x |= (uint64_t)kCodes.v[a[b]] << (2 * b);
+          data[(pos + j) >> 5] = x;
+          j += 32;
+        }
       } else {
         data[g >> 5] |= (uint64_t)code_of(ascii[j]) << sh;
         ++j;

@@ -319,6 +319,9 @@ size_t bin_buffer_lds(uint32_t nbins) { return (size_t)nbins * (3 * sizeof(uint3
 
 template <int L, bool HLL>
 __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
+  // windows per thread per round: 8 where the walk state leaves room for
+  // their cells in registers (k <= 64), else 6
+  constexpr int kBufRound = L <= 2 ? 8 : 6;
   __shared__ uint32_t s_nfull[2];
   const uint32_t nb = a.nbins;
   uint32_t* cnt = sh;
@@ -344,10 +347,10 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   __syncthreads();
   for (;;) {
     if (!wk.has()) wk.open(a);
-    uint32_t cells[4 * kRound];
+    uint32_t cells[4 * kBufRound];
     uint32_t pend = 0;
 #pragma unroll
-    for (int j = 0; j < kRound; ++j) {
+    for (int j = 0; j < kBufRound; ++j) {
       if (wk.has()) {
         const uint64_t key = wk.take(a);
 #pragma unroll
@@ -360,16 +363,16 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
       // values derived from the cells are recomputed in every step rather
       // than kept live across the loop (which would spill)
 #pragma unroll
-      for (int i = 0; i < 4 * kRound; ++i) asm volatile("" : "+v"(cells[i]));
+      for (int i = 0; i < 4 * kBufRound; ++i) asm volatile("" : "+v"(cells[i]));
       uint32_t* nfull = s_nfull + (step & 1u);
       if (tries == kBufTries) {
 #pragma unroll
-        for (int i = 0; i < 4 * kRound; ++i)
+        for (int i = 0; i < 4 * kBufRound; ++i)
           if ((pend >> i) & 1u) atomicAdd(a.counts + cells[i], 1u);
         pend = 0;
       }
 #pragma unroll
-      for (int i = 0; i < 4 * kRound; ++i) {
+      for (int i = 0; i < 4 * kBufRound; ++i) {
         if ((pend >> i) & 1u) {
           const uint32_t cell = cells[i], b = cell >> sb;
           const uint32_t slot = atomicAdd(cnt + b, 1u);

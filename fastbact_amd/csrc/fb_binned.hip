@@ -284,19 +284,20 @@ __global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter_m(const BinArg
   bin_scatter_body<L, HLL>(args[iso], blockIdx.x - blk[iso], sh);
 }
 
-// Phase A with a buffer per bin (from kBufMinBins to kBufMaxBins bins).  Each bin holds kBufE offsets (64 bytes) in LDS.  A thread appends
-// its round's cells to their bins' buffers with one LDS atomic each.  The
-// thread whose cell fills a buffer puts the bin on the round's full list,
-// and after a barrier half a wave per full bin writes the buffer into the
-// bin's current chunk (64 offsets, one line, reserved with one global atomic
-// per chunk).  Cells that found their buffer full try again after the flush.
-// Every store to the bin regions thus writes 64 whole bytes, once; a line
-// is written by one workgroup only, in two halves.  After kBufTries
-// placement steps in one round, the cells still waiting are added to the
-// table with global atomics (heavily repeated k-mers: exact, and no
-// workgroup waits on a bin for long).
+// Phase A with a buffer per bin (from kBufMinBins to kBufMaxBins bins).
+// Each bin holds kBufE offsets (64 bytes) in LDS.  A thread appends its
+// round's cells to their bins' buffers with one LDS atomic each.  After a
+// barrier a thread per bin lists the full buffers (reserving a new chunk of
+// 64 offsets, one line, with one global atomic where the bin has none
+// left), and after another half a wave per full bin writes the buffer into
+// the bin's current chunk.  Cells that found their buffer full try again
+// after the flush.  Every store to the bin regions thus writes 64 whole
+// bytes, once; a line is written by one workgroup only, in two halves.
+// After kBufTries placement steps in one round, the cells still waiting are
+// added to the table with global atomics (heavily repeated k-mers: exact,
+// and no workgroup waits on a bin for long).
 // LDS: cnt[nbins] (entries in the buffer; may pass kBufE while cells are
-// placed), cur[nbins] (chunk_word of the current chunk), full[nbins],
+// placed), cur[nbins] (half_word of the current chunk), full[nbins],
 // buf[nbins * kBufE] (u16).
 constexpr int kBufBlock = (int)kP1BufBlock;
 constexpr uint32_t kBufE = 32;
@@ -322,14 +323,14 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   // windows per thread per round: 8 where the walk state leaves room for
   // their cells in registers (k <= 64), else 6
   constexpr int kBufRound = L <= 2 ? 8 : 6;
-  __shared__ uint32_t s_nfull[2];
+  __shared__ uint32_t s_nfull;
   const uint32_t nb = a.nbins;
   uint32_t* cnt = sh;
   uint32_t* cur = sh + nb;
   uint32_t* full = sh + 2 * nb;
   uint16_t* buf = reinterpret_cast<uint16_t*>(sh + 3 * nb);
   for (uint32_t i = threadIdx.x; i < 2 * nb; i += blockDim.x) sh[i] = 0;  // cnt, cur (no chunk yet)
-  if (threadIdx.x < 2) s_nfull[threadIdx.x] = 0;
+
   const uint64_t total = a.reads.n_reads * (uint64_t)a.split;
   const uint64_t gid0 = blk * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)a.grid * blockDim.x;
@@ -343,7 +344,6 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   const uint32_t h = threadIdx.x & 31u;
   CellWalker<L, HLL> wk;
   wk.init(gid0, stride, count);
-  uint32_t step = 0;  // placement steps so far; its parity picks the full-list counter
   __syncthreads();
   for (;;) {
     if (!wk.has()) wk.open(a);
@@ -364,7 +364,7 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
       // than kept live across the loop (which would spill)
 #pragma unroll
       for (int i = 0; i < 4 * kBufRound; ++i) asm volatile("" : "+v"(cells[i]));
-      uint32_t* nfull = s_nfull + (step & 1u);
+      if (threadIdx.x == 0) s_nfull = 0;  // every thread read it before the last barrier
       if (tries == kBufTries) {
 #pragma unroll
         for (int i = 0; i < 4 * kBufRound; ++i)
@@ -379,20 +379,19 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
           if (slot < kBufE) {
             buf[b * kBufE + slot] = (uint16_t)(cell & smask);
             pend &= ~(1u << i);
-            if (slot == kBufE - 1u) full[atomicAdd(nfull, 1u)] = b;
           }
         }
       }
       __syncthreads();
-      const uint32_t nf = *nfull;
-      if (threadIdx.x == 0) s_nfull[(step + 1u) & 1u] = 0;  // read by no thread until the next barrier
-      ++step;
-      // a new chunk for each full bin that has none left, all reserved at once
-      for (uint32_t f = threadIdx.x; f < nf; f += blockDim.x) {
-        const uint32_t b = full[f];
+      // the full bins, a thread per bin: a new chunk where none is left (all
+      // reserved at once), then onto the list
+      for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        if (cnt[b] < kBufE) continue;
         if ((cur[b] & 3u) == 0) cur[b] = half_word(atomicAdd(cursor + (uint64_t)b * kSlices, kChunk), 2u);
+        full[atomicAdd(&s_nfull, 1u)] = b;
       }
       __syncthreads();
+      const uint32_t nf = s_nfull;
       for (uint32_t f = threadIdx.x >> 5; f < nf; f += blockDim.x >> 5) {
         const uint32_t b = full[f];
         const uint32_t cw = cur[b];

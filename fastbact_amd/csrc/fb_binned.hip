@@ -347,7 +347,7 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   __syncthreads();
   for (;;) {
     if (!wk.has()) wk.open(a);
-    uint32_t cells[4 * kBufRound];
+    uint32_t cells[4 * kBufRound] = {};  // zero for windows not taken: the asm below reads every cell
     uint32_t pend = 0;
 #pragma unroll
     for (int j = 0; j < kBufRound; ++j) {

@@ -449,16 +449,22 @@ __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uin
   for (uint32_t sl = 0; sl < kSlices; ++sl) {
     const uint32_t filled = a.cursor[(uint64_t)b * kSlices + sl];
     const uint32_t n = filled < a.cap ? filled : (uint32_t)a.cap;
-    const uint16_t* __restrict__ d = a.data + ((uint64_t)b * kSlices + sl) * a.cap;  // cap is a multiple of 4
-    const uint32_t n4 = n & ~3u;
-    for (uint32_t i = threadIdx.x * 4u; i < n4; i += blockDim.x * 4u) {
-      const uint2 v = *reinterpret_cast<const uint2*>(d + i);
-      const uint32_t e[4] = {v.x & 0xFFFFu, v.x >> 16, v.y & 0xFFFFu, v.y >> 16};
+    // cap is a multiple of 64, so every region starts on a 128-byte line;
+    // a thread takes 16 offsets (two 16-byte loads in flight) at a time
+    const uint16_t* __restrict__ d = a.data + ((uint64_t)b * kSlices + sl) * a.cap;
+    const uint32_t n16 = n & ~15u;
+    for (uint32_t i = threadIdx.x * 16u; i < n16; i += blockDim.x * 16u) {
+      const uint4 v0 = *reinterpret_cast<const uint4*>(d + i);
+      const uint4 v1 = *reinterpret_cast<const uint4*>(d + i + 8);
+      const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (e[j] != kNoCell) atomicAdd(lds + e[j], 1u);
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t lo = w[j] & 0xFFFFu, hi = w[j] >> 16;
+        if (lo != kNoCell) atomicAdd(lds + lo, 1u);
+        if (hi != kNoCell) atomicAdd(lds + hi, 1u);
+      }
     }
-    for (uint32_t i = n4 + threadIdx.x; i < n; i += blockDim.x)
+    for (uint32_t i = n16 + threadIdx.x; i < n; i += blockDim.x)
       if (d[i] != kNoCell) atomicAdd(lds + d[i], 1u);
   }
   __syncthreads();
@@ -582,6 +588,7 @@ uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins) {
 
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid) {
   if (a.nbins == 0 || (a.nbins << a.sb) < a.n_cells) return hipErrorInvalidValue;
+  if (a.cap % kChunk) return hipErrorInvalidValue;  // regions start on lines (16-byte loads in the count kernel)
   if (hipError_t e = check_bin_pass1(a.nbins, a.sb)) return e;
   if (a.reads.n_reads && a.grid && bin_buffered(a.nbins)) {
     if (bin_buffer_lds(a.nbins) + 1024 > kP2LdsBytes) return hipErrorInvalidConfiguration;

@@ -69,11 +69,12 @@ __device__ __forceinline__ void pack_record(const Kmer<L>& key, uint32_t info, u
   }
 }
 template <int L>
-__device__ __forceinline__ void unpack_record(const uint64_t* __restrict__ src, uint32_t rw, uint32_t k, Kmer<L>& key,
-                                              uint32_t& info) {
-  uint64_t w[L + 1];
+__device__ __forceinline__ void load_record(const uint64_t* __restrict__ src, uint32_t rw, uint64_t (&w)[L + 1]) {
 #pragma unroll
   for (int j = 0; j <= L; ++j) w[j] = (uint32_t)j < rw ? src[j] : 0ull;
+}
+template <int L>
+__device__ __forceinline__ void decode_record(const uint64_t (&w)[L + 1], uint32_t k, Kmer<L>& key, uint32_t& info) {
   const uint32_t q = (2u * k) >> 6, s = (2u * k) & 63u;
   uint64_t lo = 0, hi = 0;
 #pragma unroll
@@ -479,10 +480,17 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   const uint32_t n = e1 > e0 ? (uint32_t)(e1 - e0) : 0u;
   const uint32_t rw = a.rw;
   const uint32_t gate = a.min_trans_cov;
+  // each thread loads its next record while it tables the current one
+  uint64_t nxt[L + 1];
+  if (threadIdx.x < n) load_record<L>(a.recs + (e0 + threadIdx.x) * rw, rw, nxt);
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    uint64_t cur[L + 1];
+#pragma unroll
+    for (int j = 0; j <= L; ++j) cur[j] = nxt[j];
+    if (i + blockDim.x < n) load_record<L>(a.recs + (e0 + i + blockDim.x) * rw, rw, nxt);
     Kmer<L> key;
     uint32_t info;
-    unpack_record<L>(a.recs + (e0 + i) * rw, rw, a.k, key, info);
+    decode_record<L>(cur, a.k, key, info);
     uint64_t w[KW];
     pack62b<L, KW>(key, w);
     const bool left = info & 1u, right = (info >> 4) & 1u;

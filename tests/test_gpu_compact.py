@@ -73,7 +73,7 @@ def test_reference_repeat_case_on_gpu(ctx, orc):
     _same(g, orc.compact(s.keys, s.cov, s.mask, k))
 
 
-def test_cfg1_scale_compaction(ctx, orc):
+def test_1mbp_isolate_compaction(ctx, orc):
     iso = F.synth_isolate(seed=0xFB01, genome_len=1_000_000, gc=0.5, coverage=30.0, insert_mean=350,
                           insert_sd=30, sub_rate=0.005)
     dev = F.DeviceBank(ctx, iso.bank())

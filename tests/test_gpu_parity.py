@@ -257,7 +257,7 @@ def test_estimate_genome_ragged_and_empty(ctx, orc):
     assert "0 valid read pairs" in str(e.value)
 
 
-def test_cfg1_scale_plain_isolate(ctx, orc):
+def test_1mbp_isolate_100k_pairs(ctx, orc):
     """1 Mbp at 50% GC, 100k PE150 pairs (30x), 0.5% errors, k=31, CMS 4 x 2^22
     u16 counters, HLL p=14, solid threshold 3: bit-exact against the oracle."""
     iso = F.synth_isolate(seed=0xFB01, genome_len=1_000_000, gc=0.5, coverage=30.0, insert_mean=350,

@@ -99,8 +99,9 @@ struct CellWalker {
     }
   }
   __device__ __forceinline__ bool has() const { return w < w1; }
-  // Canonical fold of the next window of the open share (has() must hold).
-  __device__ __forceinline__ uint64_t take(const BinArgs& a) {
+  // Canonical fold of the next window of the open share (has() must hold);
+  // nt holds the ntHash roll terms of this k (in LDS).
+  __device__ __forceinline__ uint64_t take(const BinArgs& a, const NtRollTab& nt) {
     const uint64_t key = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
     if (HLL) {
       uint32_t idx, rho;
@@ -110,7 +111,7 @@ struct CellWalker {
     if (++w < w1) {
       const uint32_t b = in.next();
       kmer_roll<L>(fwd, rev, b, a.k);
-      if (HLL) nh.roll(outs.next(), b, a.k);
+      if (HLL) nt_roll(nh, outs.next(), b, nt);
     }
     return key;
   }
@@ -197,6 +198,8 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
   const uint32_t smask = (1u << sb) - 1u;
   uint16_t* __restrict__ out = a.data + sl * cap;
   uint32_t* __restrict__ counts = a.counts;
+  __shared__ NtRollTab s_nt;
+  if (HLL && threadIdx.x < 4) s_nt.fill(threadIdx.x, a.k);
   CellWalker<L, HLL> wk;
   wk.init(gid0, stride, count);
   __syncthreads();
@@ -208,7 +211,7 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
 #pragma unroll
     for (int j = 0; j < kRound; ++j) {
       if (wk.has()) {
-        const uint64_t key = wk.take(a);
+        const uint64_t key = wk.take(a, s_nt);
 #pragma unroll
         for (int row = 0; row < 4; ++row) {
           cells[4 * j + row] = (uint32_t)cell_of(a, key, row);
@@ -320,9 +323,9 @@ size_t bin_buffer_lds(uint32_t nbins) { return (size_t)nbins * (3 * sizeof(uint3
 
 template <int L, bool HLL>
 __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
-  // windows per thread per round: 8 where the walk state leaves room for
-  // their cells in registers (k <= 64), else 6
-  constexpr int kBufRound = L <= 2 ? 8 : 6;
+  // windows per thread per round: as many as the walk state leaves room
+  // for in registers (8 for k <= 64, 6 to k 96, then 5)
+  constexpr int kBufRound = L <= 2 ? 8 : (L == 3 ? 6 : 5);
   __shared__ uint32_t s_nfull;
   const uint32_t nb = a.nbins;
   uint32_t* cnt = sh;
@@ -342,6 +345,8 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   uint16_t* __restrict__ out = a.data + sl * cap;
   uint32_t* __restrict__ cursor = a.cursor + sl;
   const uint32_t h = threadIdx.x & 31u;
+  __shared__ NtRollTab s_nt;
+  if (HLL && threadIdx.x < 4) s_nt.fill(threadIdx.x, a.k);
   CellWalker<L, HLL> wk;
   wk.init(gid0, stride, count);
   __syncthreads();
@@ -352,7 +357,7 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
 #pragma unroll
     for (int j = 0; j < kBufRound; ++j) {
       if (wk.has()) {
-        const uint64_t key = wk.take(a);
+        const uint64_t key = wk.take(a, s_nt);
 #pragma unroll
         for (int row = 0; row < 4; ++row) cells[4 * j + row] = (uint32_t)cell_of(a, key, row);
         pend |= 0xFu << (4 * j);

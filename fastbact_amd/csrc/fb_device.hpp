@@ -193,6 +193,26 @@ struct NtHash {
   __device__ __forceinline__ uint64_t canonical() const { return f < r ? f : r; }
 };
 
+// The four per-base terms of NtHash::roll for a fixed k, looked up (from
+// LDS) instead of selected among the seeds and rotated at every window.
+struct NtRollTab {
+  uint64_t out_f[4];  // rol(nt_fwd(b), k)
+  uint64_t in_f[4];   // nt_fwd(b)
+  uint64_t out_r[4];  // ror(nt_rev(b), 1)
+  uint64_t in_r[4];   // rol(nt_rev(b), k - 1)
+  // Entry b; threads 0..3 of a block fill the table together.
+  __device__ __forceinline__ void fill(uint32_t b, uint32_t k) {
+    out_f[b] = rol64(nt_fwd(b), k & 63u);
+    in_f[b] = nt_fwd(b);
+    out_r[b] = ror64(nt_rev(b), 1);
+    in_r[b] = rol64(nt_rev(b), (k - 1u) & 63u);
+  }
+};
+__device__ __forceinline__ void nt_roll(NtHash& h, uint32_t out, uint32_t in, const NtRollTab& t) {
+  h.f = rol64(h.f, 1) ^ t.out_f[out] ^ t.in_f[in];
+  h.r = ror64(h.r, 1) ^ t.out_r[out] ^ t.in_r[in];
+}
+
 // HyperLogLog register index and rank (sketch.rs:49-63) at precision p.
 __device__ __forceinline__ void hll_index_rho(uint64_t h, uint32_t p, uint32_t& idx, uint32_t& rho) {
   idx = (uint32_t)(h >> (64u - p));

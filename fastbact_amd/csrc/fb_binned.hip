@@ -63,7 +63,6 @@ struct CellWalker {
   Kmer<L> fwd, rev;
   BaseStream in;
   NtHash nh;        // HLL only
-  BaseStream outs;  // HLL only: the base leaving the ntHash window
 
   __device__ __forceinline__ void init(uint64_t gid0, uint64_t stride, uint32_t count) {
     gid = gid0;
@@ -88,8 +87,8 @@ struct CellWalker {
         if (HLL) {
           nh.f = 0;
           nh.r = 0;
-          outs.init(a.reads.data, g);
-          BaseStream s0 = outs;
+          BaseStream s0;
+          s0.init(a.reads.data, g);
           for (uint32_t j = 0; j < a.k; ++j) nh.push(s0.next(), a.k);
         }
         w = w0;
@@ -110,8 +109,11 @@ struct CellWalker {
     }
     if (++w < w1) {
       const uint32_t b = in.next();
+      // the base leaving the window is the forward k-mer's first (most
+      // significant) base, in its top limb since 2k > 64 (L - 1)
+      const uint32_t out = (uint32_t)(fwd.a[L - 1] >> ((2u * a.k - 2u) & 63u)) & 3u;
       kmer_roll<L>(fwd, rev, b, a.k);
-      if (HLL) nt_roll(nh, outs.next(), b, nt);
+      if (HLL) nt_roll(nh, out, b, nt);
     }
     return key;
   }

@@ -70,8 +70,8 @@ struct CellWalker {
     left = count;
     w = w1 = 0;
   }
-  // Opens the next share with windows, if any.
-  __device__ __forceinline__ void open(const BinArgs& a) {
+  // Opens the next share with windows, if any (nt: as for take()).
+  __device__ __forceinline__ void open(const BinArgs& a, const NtRollTab& nt) {
     while (left) {
       uint64_t r = 0;
       uint32_t w0 = 0, e = 0;
@@ -89,7 +89,7 @@ struct CellWalker {
           nh.r = 0;
           BaseStream s0;
           s0.init(a.reads.data, g);
-          for (uint32_t j = 0; j < a.k; ++j) nh.push(s0.next(), a.k);
+          for (uint32_t j = 0; j < a.k; ++j) nt_push(nh, s0.next(), nt);
         }
         w = w0;
         w1 = e;
@@ -207,7 +207,7 @@ __device__ __forceinline__ void bin_scatter_body(const BinArgs& a, uint64_t blk,
   __syncthreads();
   for (;;) {
     // a round takes up to kRound windows of one share; its cells stay in registers
-    if (!wk.has()) wk.open(a);
+    if (!wk.has()) wk.open(a, s_nt);
     uint32_t cells[4 * kRound];
     uint32_t n = 0;
 #pragma unroll
@@ -353,7 +353,7 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   wk.init(gid0, stride, count);
   __syncthreads();
   for (;;) {
-    if (!wk.has()) wk.open(a);
+    if (!wk.has()) wk.open(a, s_nt);
     uint32_t cells[4 * kBufRound] = {};  // zero for windows not taken: the asm below reads every cell
     uint32_t pend = 0;
 #pragma unroll

@@ -208,6 +208,11 @@ struct NtRollTab {
     in_r[b] = rol64(nt_rev(b), (k - 1u) & 63u);
   }
 };
+// NtHash::push with the table's "in" terms (they are the same two terms).
+__device__ __forceinline__ void nt_push(NtHash& h, uint32_t b, const NtRollTab& t) {
+  h.f = rol64(h.f, 1) ^ t.in_f[b];
+  h.r = ror64(h.r, 1) ^ t.in_r[b];
+}
 __device__ __forceinline__ void nt_roll(NtHash& h, uint32_t out, uint32_t in, const NtRollTab& t) {
   h.f = rol64(h.f, 1) ^ t.out_f[out] ^ t.in_f[in];
   h.r = ror64(h.r, 1) ^ t.out_r[out] ^ t.in_r[in];

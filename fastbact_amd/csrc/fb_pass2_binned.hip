@@ -68,6 +68,35 @@ __device__ __forceinline__ void pack_record(const Kmer<L>& key, uint32_t info, u
     w[j] = v;
   }
 }
+// Record pos of rw words at dst = recs + pos * rw.  The lanes of a store
+// write to scattered lines, so the number of stores per lane, not their
+// width, sets the cost: 16-byte stores wherever they stay 16-byte aligned
+// (every word pair for even rw; for odd rw the pairs from word 0 or from
+// word 1, by the parity of pos) and one 8-byte store for the word left over.
+template <int L>
+__device__ __forceinline__ void store_record(uint64_t* dst, const uint64_t (&w)[L + 1], uint32_t rw, uint64_t pos) {
+  if (rw == 1u) {
+    dst[0] = w[0];
+    return;
+  }
+  if (!(rw & 1u)) {
+#pragma unroll
+    for (int j = 0; j + 1 <= L; j += 2)
+      if ((uint32_t)j < rw) *reinterpret_cast<ulonglong2*>(dst + j) = make_ulonglong2(w[j], w[j + 1]);
+    return;
+  }
+  const uint32_t odd = (uint32_t)(pos & 1u);
+  uint64_t last = w[0];
+#pragma unroll
+  for (int j = 0; j + 2 <= L; j += 2) {
+    if ((uint32_t)j + 2u < rw)
+      *reinterpret_cast<ulonglong2*>(dst + j + odd) =
+          odd ? make_ulonglong2(w[j + 1], w[j + 2]) : make_ulonglong2(w[j], w[j + 1]);
+    if ((uint32_t)j + 2u == rw - 1u) last = w[j + 2];
+  }
+  if (odd) last = w[0];
+  dst[odd ? 0u : rw - 1u] = last;
+}
 template <int L>
 __device__ __forceinline__ void load_record(const uint64_t* __restrict__ src, uint32_t rw, uint64_t (&w)[L + 1]) {
 #pragma unroll
@@ -402,10 +431,7 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk
                  }
                  uint64_t w[L + 1];
                  pack_record<L>(key, (left ? 1u : 0u) | (el << 1) | (right ? 16u : 0u) | (er << 5) | cbit, k, w);
-                 uint64_t* dst = a.recs + (uint64_t)pos * rw;
-#pragma unroll
-                 for (int j = 0; j <= L; ++j)
-                   if ((uint32_t)j < rw) dst[j] = w[j];
+                 store_record<L>(a.recs + (uint64_t)pos * rw, w, rw, pos);
                });
   }
 }

@@ -164,9 +164,11 @@ hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t n_ref_words
                         uint64_t* out_keys, uint16_t* out_cov, uint8_t* out_mask, void* scratch, hipStream_t s,
                         const uint32_t* out_base = nullptr, uint32_t out_cap = 0xFFFFFFFFu);
 // Sort the compacted records by k-mer (integer order).  Uses scratch from
-// the caller: tmp_bytes queried with keys == nullptr.
+// the caller: tmp_bytes queried with keys == nullptr.  top_first: for keys
+// over 64 bits, first one sort by the top 64 bits and a check (synchronising
+// the stream) that no two keys share them, else the limb-by-limb sort.
 hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t n_ref_words,
-                      uint32_t k, void* tmp, size_t* tmp_bytes, hipStream_t s);
+                      uint32_t k, void* tmp, size_t* tmp_bytes, hipStream_t s, bool top_first = false);
 
 // Pass 2 + 3 with LDS-privatised solid tables (fb_pass2_binned.hip).
 constexpr uint32_t kP2LdsBytes = 160u * 1024u;  // workgroup LDS of the bin kernel

@@ -736,8 +736,60 @@ __global__ __launch_bounds__(kBlock) void k_gather_records(const uint64_t* __res
 }
 
 // Scratch layout: [limb keys x2][perm x2][record copy][cub temp].
+// Most significant 64 bits of every key of more than 64 bits: bits
+// [2k - 64, 2k) of the integer (words most significant first).
+__global__ __launch_bounds__(kBlock) void k_top64(const uint64_t* __restrict__ keys, uint32_t n, uint32_t nref,
+                                                  uint32_t k, uint64_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t sh = 2 * k - 64, a = sh >> 6, off = sh & 63u;
+  const uint64_t* key = keys + (uint64_t)i * nref;
+  uint64_t v = key[nref - 1 - a] >> off;
+  if (off) v |= key[nref - 2 - a] << (64 - off);
+  out[i] = v;
+}
+// Runs of equal values in the sorted v: the first element of a run of two
+// or more walks it, up to kTieRunMax + 1 long.
+constexpr uint32_t kTieRunMax = 64;
+__device__ __forceinline__ uint32_t tie_run(const uint64_t* __restrict__ v, uint32_t n, uint32_t i) {
+  if (i + 1 >= n || v[i] != v[i + 1] || (i > 0 && v[i - 1] == v[i])) return 0u;
+  uint32_t j = i + 2;
+  while (j < n && j - i <= kTieRunMax && v[j] == v[i]) ++j;
+  return j - i;
+}
+// *maxrun = the longest run (capped at kTieRunMax + 1; *maxrun zeroed before).
+__global__ __launch_bounds__(kBlock) void k_tie_runs(const uint64_t* __restrict__ v, uint32_t n,
+                                                     uint32_t* __restrict__ maxrun) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = tie_run(v, n, i);
+  if (m) atomicMax(maxrun, m);
+}
+// Sorts the permutation within every run (at most kTieRunMax long) by the
+// whole key, words most significant first.
+__global__ __launch_bounds__(kBlock) void k_fix_ties(const uint64_t* __restrict__ v, uint32_t n,
+                                                     const uint64_t* __restrict__ keys, uint32_t nref,
+                                                     uint32_t* __restrict__ perm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = tie_run(v, n, i);
+  auto less = [&](uint32_t a, uint32_t b) {
+    for (uint32_t w = 0; w < nref; ++w) {
+      const uint64_t x = keys[(uint64_t)a * nref + w], y = keys[(uint64_t)b * nref + w];
+      if (x != y) return x < y;
+    }
+    return false;
+  };
+  for (uint32_t x = i + 1; x < i + m; ++x) {
+    const uint32_t p = perm[x];
+    uint32_t y = x;
+    for (; y > i && less(p, perm[y - 1]); --y) perm[y] = perm[y - 1];
+    perm[y] = p;
+  }
+}
+
 hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t nref, uint32_t k, void* tmp,
-                      size_t* tmp_bytes, hipStream_t s) {
+                      size_t* tmp_bytes, hipStream_t s, bool top_first) {
   size_t cub_bytes = 0;
   hipcub::DoubleBuffer<uint64_t> dk(nullptr, nullptr);
   hipcub::DoubleBuffer<uint32_t> dv(nullptr, nullptr);
@@ -745,7 +797,7 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   if (e != hipSuccess) return e;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nn = n ? n : 1;
-  const size_t need = al(nn * 8) * 2 + al(nn * 4) * 2 + al(nn * nref * 8) + al(nn * 2) + al(nn) + al(cub_bytes);
+  const size_t need = al(nn * 8) * 2 + al(nn * 4) * 2 + al(nn * nref * 8) + al(nn * 2) + al(nn) + al(4) + al(cub_bytes);
   if (keys == nullptr) {
     *tmp_bytes = need;
     return hipSuccess;
@@ -760,6 +812,7 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   uint64_t* ck = (uint64_t*)p; p += al(nn * nref * 8);
   uint16_t* cc = (uint16_t*)p; p += al(nn * 2);
   uint8_t* cm = (uint8_t*)p; p += al(nn);
+  uint32_t* ties = (uint32_t*)p; p += al(4);
   void* ctmp = p;
   const unsigned grid = grid_for(n);
   hipLaunchKernelGGL(k_iota, dim3(grid), dim3(kBlock), 0, s, v0, n);
@@ -767,7 +820,32 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   const uint32_t L = (2 * k + 63) / 64;
   uint32_t* perm = v0;
   uint32_t* perm_alt = v1;
-  for (uint32_t limb = 0; limb < L; ++limb) {
+  uint32_t first = 0;  // limbs to sort, from the least significant
+  if (top_first && L > 1) {
+    // One sort by the top 64 bits gives the full order when no two keys
+    // share them; short runs of keys that do are then sorted in place, and
+    // the limb-by-limb sort runs only for a run longer than kTieRunMax.
+    hipLaunchKernelGGL(k_top64, dim3(grid), dim3(kBlock), 0, s, keys, n, nref, k, k0);
+    hipcub::DoubleBuffer<uint64_t> bk(k0, k1);
+    hipcub::DoubleBuffer<uint32_t> bv(v0, v1);
+    size_t cb = cub_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(ctmp, cb, bk, bv, (int)n, 0, 64, s);
+    if (e != hipSuccess) return e;
+    if ((e = launch_fill_u32(ties, 1, 0u, s))) return e;
+    hipLaunchKernelGGL(k_tie_runs, dim3(grid), dim3(kBlock), 0, s, bk.Current(), n, ties);
+    uint32_t longest = kTieRunMax + 1;
+    if ((e = hipMemcpyAsync(&longest, ties, 4, hipMemcpyDeviceToHost, s))) return e;
+    if ((e = hipStreamSynchronize(s))) return e;
+    if (longest > kTieRunMax) {
+      hipLaunchKernelGGL(k_iota, dim3(grid), dim3(kBlock), 0, s, v0, n);
+    } else {
+      perm = bv.Current();
+      first = L;
+      if (longest)
+        hipLaunchKernelGGL(k_fix_ties, dim3(grid), dim3(kBlock), 0, s, bk.Current(), n, keys, nref, perm);
+    }
+  }
+  for (uint32_t limb = first; limb < L; ++limb) {
     const uint32_t word = nref - 1 - limb;  // reference words are most significant first
     hipLaunchKernelGGL(k_gather_limb, dim3(grid), dim3(kBlock), 0, s, keys, perm, n, nref, word, k0);
     const int end_bit = (limb == L - 1) ? (int)(2 * k - 64 * (L - 1)) : 64;

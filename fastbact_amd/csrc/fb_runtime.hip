@@ -1026,7 +1026,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     DBuf tmp;
     if (int r = tmp.get(c, tb)) return fail(r);
     HIP_TRY(c, sort_solid(so->keys.as<uint64_t>(), so->cov.as<uint16_t>(), so->mask.as<uint8_t>(), n_out, so->nref, k,
-                          tmp.p, &tb, s));
+                          tmp.p, &tb, s, true));
   }
   HIP_TRY(c, hipEventRecord(c->ev[5], s));
   HIP_TRY(c, hipEventSynchronize(c->ev[5]));

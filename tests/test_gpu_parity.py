@@ -278,6 +278,36 @@ def test_1mbp_isolate_100k_pairs(ctx, orc):
     assert est > len(g)
 
 
+@pytest.mark.parametrize("k", [33, 55, 64, 77, 127])
+def test_sort_with_shared_top_bits_matches_oracle(ctx, orc, k):
+    """Keys over 64 bits are sorted by their top 64 bits first; runs of keys
+    that share those bits are then sorted in place when at most 64 long,
+    else the whole set is sorted limb by limb.  Reads that start with the
+    same 40 bases give solid k-mers sharing their first 32 bases: in runs
+    of up to 20 (20 distinct reads) or of 300 (300 reads); a random isolate
+    gives no runs.  Every order equals the oracle's."""
+    rng = np.random.default_rng(k)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    head = np.frombuffer(b"AC" * 20, np.uint8)
+    banks = []
+    for n_tails in (6, 300):
+        tails = [rng.choice(acgt, size=160) for _ in range(n_tails)]
+        seqs = [np.concatenate([head, t]).tobytes() for t in tails for _ in range(3)]
+        banks.append(F.ReadBank(F.BitSeqArena.from_ascii(seqs)))
+    runs = []
+    for b in banks + [isolate(seed=k, genome_len=20_000, n_pairs=1500, sub_rate=0.0).bank()]:
+        r = orc.build_solid(oracle_arenas(b), k, target_solid_cov=2, width=1 << 18)
+        g = F.build_solid_kmers(ctx, F.DeviceBank(ctx, b), k, target_solid_cov=2, cms_width=1 << 18)
+        _solid_equal(g, r)
+        nref = r["keys"].shape[1]
+        top = [sum(int(x) << (64 * (nref - 1 - j)) for j, x in enumerate(row)) >> (2 * k - 64) for row in r["keys"]]
+        runs.append(max(np.unique(np.array(top, dtype=object), return_counts=True)[1]))
+    # the three paths: short runs sorted in place, a long run, no run (an
+    # error-free isolate)
+    assert 2 <= runs[0] <= 64 and runs[2] == 1, runs
+    assert runs[1] > 64 or k == 33, runs
+
+
 @pytest.mark.parametrize("k,bins2", [(31, 1), (55, 2), (77, 1), (127, 4)])
 def test_lds_table_full_spills_to_global_table(ctx, orc, k, bins2):
     """Few pass-2 bins force keys past the LDS table into the global table;

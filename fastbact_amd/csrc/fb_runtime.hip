@@ -1926,7 +1926,7 @@ extern "C" int fb_batch_run(fb_batch* b, uint32_t k, int32_t target) {
       if (it->n_out > 1) {
         size_t tb = need;
         HIP_TRY(c, sort_solid(it->keys.as<uint64_t>(), it->cov.as<uint16_t>(), it->mask.as<uint8_t>(),
-                              (uint32_t)it->n_out, ref_words(k), k, tmp.p, &tb, s));
+                              (uint32_t)it->n_out, ref_words(k), k, tmp.p, &tb, s, true));
       }
     }
   }

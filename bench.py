@@ -127,21 +127,63 @@ def measured_traffic(kernel: str):
         return None, None
 
 
-def cpu_baseline(bank, pairs: int, threads: int) -> dict:
+def host_info() -> dict:
+    """Cores of this host: nproc, the CPUs this process may use, the model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "allowed_cpus": allowed, "model": model}
+
+
+def cpu_baseline(ctx, F, bank, pairs: int, thread_counts) -> dict:
+    """The oracle's threaded form (the reference's CPU structures: CAS
+    Count-Min, 64 mutex-sharded maps fed by 2048-entry buffers, 4096-read
+    pass-2 chunks) on the first `pairs` pairs of the isolate, at each thread
+    count; the GPU's passes on the same sample are checked equal to its
+    output.  Gbases/s is the rate on that sample: the per-window work does not
+    depend on the sample size, while the sample's maps are smaller than the
+    whole isolate's (fewer distinct error k-mers), so the whole isolate would
+    run a little slower per base, not faster."""
     from oracle import oracle as O
-    import fastbact_amd as F
     n = min(pairs, len(bank.pe_forward))
     sub = F.ReadBank(F.BitSeqArena(), _head(bank.pe_forward, n), _head(bank.pe_reverse, n))
     arenas = [(a.data, a.offsets, a.lengths) for a in sub.arenas()]
     bases = sum(int(a.lengths[a.lengths >= k].sum(dtype=np.uint64)) for a in sub.arenas() for k, _ in PASSES)
-    t = time.time()
-    for k, tgt in PASSES:
-        O.build_solid(arenas, k, target_solid_cov=tgt, width=WIDTH, counter_bits=COUNTER_BITS, hll_p=HLL_P,
-                      threads=threads)
-    dt = time.time() - t
-    return {"value": bases / dt / 1e9, "unit": "Gbases/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} pairs of the same isolate ({sub.total_bases()} bases), the same 3 passes "
-                      f"(k=21/55/77, CMS 4x2^24 u16, HLL p16), oracle threaded form, {dt:.1f} s"}
+    runs, ref = [], None
+    for threads in thread_counts:
+        t = time.time()
+        res = [O.build_solid(arenas, k, target_solid_cov=tgt, width=WIDTH, counter_bits=COUNTER_BITS, hll_p=HLL_P,
+                             threads=threads) for k, tgt in PASSES]
+        dt = time.time() - t
+        ref = ref or res
+        runs.append({"value": bases / dt / 1e9, "unit": "Gbases/s", "cores": threads, "seconds": round(dt, 2)})
+        log(f"CPU baseline: {threads} threads, {dt:.1f} s")
+    dev = F.DeviceBank(ctx, sub)
+    equal = True
+    for (k, tgt), r in zip(PASSES, ref):
+        g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=tgt, genome_size=GENOME["genome_len"], cms_width=WIDTH,
+                                counter_bits=COUNTER_BITS, hll_p=HLL_P)
+        equal = equal and bool(np.array_equal(g.keys, r["keys"]) and np.array_equal(g.cov, r["cov"])
+                               and np.array_equal(g.mask, r["mask"]) and np.array_equal(g.cms_table, r["cms"].table())
+                               and np.array_equal(g.hll_registers, r["hll"]))
+    dev.close()
+    out = dict(runs[0])
+    out.update({"kind": "port",
+                "sample": f"first {n} pairs of the same isolate ({sub.total_bases()} bases, "
+                          f"{100.0 * n / len(bank.pe_forward):.0f}% of its pairs), the same 3 passes (k=21/55/77, "
+                          f"CMS 4x2^24 u16, HLL p16), oracle threaded form",
+                "runs": runs, "host": host_info(), "gpu_equal_to_oracle_on_sample": equal})
+    return out
 
 
 def _head(a, n):
@@ -414,9 +456,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--placement", type=int, default=0, help="pass-1 sketch placement: 0 auto, 1 global, 2 LDS bins")
     ap.add_argument("--batch-isolates", type=int, default=64, help="isolates of the batched workload (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=0,
-                    help="requests of the serving stream on batched-workload isolates (0: skip; off by default, "
-                         "see the serving note in the design document)")
+    ap.add_argument("--serve-requests", type=int, default=96,
+                    help="requests of the Poisson serving stream on batched-workload isolates (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this many MiB device to device once (a known-size "
@@ -515,9 +556,12 @@ def main(argv=None) -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"CPU baseline: oracle threaded form, {threads} threads, {args.cpu_pairs} pairs")
-        cpu = cpu_baseline(bank, args.cpu_pairs, threads)
+        # the host's share of cores (16 per GPU on these boxes), then the
+        # reference's default --threads 8 (cli.rs:26-27)
+        share = args.cpu_threads or min(16, host_info()["allowed_cpus"])
+        counts = [share] + ([8] if share != 8 else [])
+        log(f"CPU baseline: oracle threaded form at {counts} threads, {args.cpu_pairs} pairs")
+        cpu = cpu_baseline(ctx, F, bank, args.cpu_pairs, counts)
 
     sm = sorted(step_ms)
     out = {
@@ -538,8 +582,11 @@ def main(argv=None) -> int:
                         "0.3% subs + 0.01% indels; k=21/55/77 passes; CMS 4x2^24 u16; HLL p16; "
                         "solid gates None/4/4; sorted solid set",
             "bases_per_step": bases_step,
-            "isolate_latency_ms": {"p50": round(float(np.percentile(sm, 50)), 3),
-                                   "p99": round(float(np.percentile(sm, 99)), 3)},
+            # per-isolate latency of the served form (Poisson arrivals, submit to
+            # solid sets on the host), when the serving stream ran
+            "isolate_latency_ms": served["poisson"]["latency_ms"] if served else None,
+            "plain_step_ms": {"p50": round(float(np.percentile(sm, 50)), 3),
+                              "p99": round(float(np.percentile(sm, 99)), 3)},
             "h2d_upload_s": round(h2d_s, 3),
             "passes": per_pass,
         },

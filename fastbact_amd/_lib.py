@@ -194,6 +194,24 @@ class ServeParams(C.Structure):
         ("max_inflight", C.c_uint32),
         ("max_queue", C.c_uint32),
         ("keep_sketches", C.c_uint32),
+        ("ctx_budget", C.c_uint64),
+    ]
+
+
+class Pass2Plan(C.Structure):
+    _fields_ = [
+        ("nb2", C.c_uint64),
+        ("nbr", C.c_uint64),
+        ("spill_cap", C.c_uint64),
+        ("out_cap", C.c_uint64),
+        ("rec_cap", C.c_uint64),
+        ("n_blocks2", C.c_uint64),
+        ("threads2", C.c_uint64),
+        ("log2_rounds", C.c_uint32),
+        ("split2", C.c_uint32),
+        ("chunks2", C.c_uint32),
+        ("lds_cap", C.c_uint32),
+        ("ok", C.c_uint32),
     ]
 
 
@@ -316,6 +334,9 @@ SIGNATURES = [
     ("fb_batch_create", C.c_int, [vp, C.POINTER(vp), u64p, C.c_uint32, C.POINTER(BuildParams), C.POINTER(vp)]),
     ("fb_batch_destroy", None, [vp]),
     ("fb_batch_footprint", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(BuildParams), C.c_uint32]),
+    ("fb_solid_out_bytes", None, [C.c_uint64, C.c_uint32, u64p]),
+    ("fb_solid_out_fits", C.c_int, [C.c_uint64, C.c_uint32, u64p]),
+    ("fb_plan_pass2", C.c_int, [C.c_double, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(Pass2Plan)]),
     ("fb_batch_run", C.c_int, [vp, C.c_uint32, C.c_int32]),
     ("fb_batch_size", C.c_uint32, [vp]),
     ("fb_batch_count", C.c_uint64, [vp, C.c_uint32]),

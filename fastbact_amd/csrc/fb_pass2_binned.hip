@@ -527,21 +527,28 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
     const int e = lds_find<KW>(keys, cap, w);
     if (e >= 0) {
       atomicAdd(cov + e, cw * ((uint32_t)left + (uint32_t)right));
-      // u16 edge counters, two per word; a read's increment happens only
-      // while the counter is below the gate, so it stays below gate +
-      // blockDim < 2^16; a contig's +1000 saturates at 65535 (compare and
-      // swap), and from there on reads no longer add
+      // u16 edge counters, two per word.  Only `value >= gate` (gate <= 255)
+      // is ever read from them.  A read adds 1 after it has loaded a value
+      // below the gate; the load may be stale, so adds land past the gate,
+      // at most two per thread of the block (the two sides of its record;
+      // the LDS serves a wave's operations in issue order, so its next load
+      // sees its earlier adds).  A contig's +1000 saturates by compare and
+      // swap at kEdgeCeil = 65535 - 4 * blockDim and never lowers a value,
+      // so every half stays <= max(kEdgeCeil, gate) + 2 * blockDim < 65535
+      // and a read add can neither carry into the other half nor wrap.
       for (int side = 0; side < 2; ++side) {
         if (!(side ? right : left)) continue;
         const uint32_t ed = side ? er : el;
         uint32_t* ew = edges + 4u * (uint32_t)e + (ed >> 1);
         const uint32_t sh = (ed & 1u) * 16u;
         if (contig) {
+          constexpr uint32_t kEdgeCeil = 65535u - 4u * (uint32_t)kP2BinBlock;
           uint32_t old = *reinterpret_cast<volatile uint32_t*>(ew), seen;
           do {
             seen = old;
             const uint32_t hv = (seen >> sh) & 0xffffu;
-            const uint32_t nv = hv + 1000u < 65535u ? hv + 1000u : 65535u;
+            if (hv >= kEdgeCeil) break;
+            const uint32_t nv = hv + 1000u < kEdgeCeil ? hv + 1000u : kEdgeCeil;
             old = atomicCAS(ew, seen, (seen & ~(0xffffu << sh)) | (nv << sh));
           } while (old != seen);
         } else if (((*reinterpret_cast<volatile uint32_t*>(ew) >> sh) & 0xffffu) < gate) {

@@ -680,6 +680,40 @@ static uint64_t bin_slice_cap(double per_bin, uint64_t n_wg) {
 constexpr uint64_t kP1MaxCap = (1ull << 26) - 128;
 static uint32_t table_stride(uint32_t kw) { return kw + 5; }
 
+extern "C" void fb_solid_out_bytes(uint64_t cap, uint32_t nref, uint64_t out[3]) {
+  if (!out) return;
+  out[0] = cap * nref * 8;
+  out[1] = cap * 2;
+  out[2] = cap;
+}
+extern "C" int fb_solid_out_fits(uint64_t cap, uint32_t nref, const uint64_t have[3]) {
+  if (!have) return 0;
+  uint64_t need[3];
+  fb_solid_out_bytes(cap, nref, need);
+  return have[0] >= need[0] && have[1] >= need[1] && have[2] >= need[2];
+}
+// Solid outputs of an isolate that keeps them across the k ladder: grown by
+// bytes at this k (an entry is nref words wide), never by entry count.
+static int solid_out_grow(fb_ctx* c, DBuf& keys, DBuf& cov, DBuf& mask, uint64_t cap, uint32_t nref) {
+  uint64_t need[3];
+  fb_solid_out_bytes(std::max<uint64_t>(cap, 1), nref, need);
+  DBuf* b[3] = {&keys, &cov, &mask};
+  for (int i = 0; i < 3; ++i)
+    if (!b[i]->p || b[i]->bytes < need[i])
+      if (int r = b[i]->get(c, need[i])) return r;
+  return FB_OK;
+}
+// The guard before every launch that writes cap solid entries of nref words.
+static int solid_out_check(fb_ctx* c, const DBuf& keys, const DBuf& cov, const DBuf& mask, uint64_t cap,
+                           uint32_t nref) {
+  const uint64_t have[3] = {keys.p ? keys.bytes : 0, cov.p ? cov.bytes : 0, mask.p ? mask.bytes : 0};
+  if (fb_solid_out_fits(cap, nref, have)) return FB_OK;
+  return set_err(c, FB_ERR_INVALID,
+                 "internal: solid output buffers (%llu/%llu/%llu bytes) cannot hold %llu entries of %u key words",
+                 (unsigned long long)have[0], (unsigned long long)have[1], (unsigned long long)have[2],
+                 (unsigned long long)cap, nref);
+}
+
 extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
                               uint64_t n_contigs, const fb_build_params* prm, fb_solid** out) {
   if (!c || !bank || !prm || !out) return set_err(c, FB_ERR_INVALID, "null argument");
@@ -895,6 +929,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     if (!r) r = so->keys.get(c, out_cap * so->nref * 8);
     if (!r) r = so->cov.get(c, out_cap * 2);
     if (!r) r = so->mask.get(c, out_cap);
+    if (!r) r = solid_out_check(c, so->keys, so->cov, so->mask, out_cap, so->nref);
     if (r) return fail(r);
     tab.cap = spill_cap;
     tab.slots = tbuf.as<uint64_t>();
@@ -999,6 +1034,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     if (int r = so->keys.get(c, maxn * so->nref * 8)) return fail(r);
     if (int r = so->cov.get(c, maxn * 2)) return fail(r);
     if (int r = so->mask.get(c, maxn)) return fail(r);
+    if (int r = solid_out_check(c, so->keys, so->cov, so->mask, maxn, so->nref)) return fail(r);
     DBuf cnt;
     if (int r = cnt.get(c, pass3_scratch_bytes())) return fail(r);
     HIP_TRY(c, hipEventRecord(c->ev[4], s));
@@ -1329,6 +1365,27 @@ static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint3
   q.out_cap = std::min<uint64_t>(lds_out + q.spill_cap, 0xFFFFFFFFull);
   q.ok = q.nb2 <= max_bins && windows > 0 && q.rec_cap < (1ull << 32);
   return q;
+}
+
+extern "C" int fb_plan_pass2(double est_keys, uint64_t windows, uint64_t n_reads, uint32_t max_len, uint32_t k,
+                             fb_pass2_plan* out) {
+  if (!out || k < 1 || k > 127) return FB_ERR_INVALID;
+  const uint32_t kw = key_words_for_k(k);
+  const uint32_t lds_cap = (kP2LdsBytes - 1024u) / (kw * 8u + 20u);
+  const P2Plan q = plan_pass2(est_keys, windows, n_reads, max_len, k, default_split(k, max_len), 0, lds_cap, 16384u, 0);
+  out->nb2 = q.nb2;
+  out->nbr = q.nbr;
+  out->spill_cap = q.spill_cap;
+  out->out_cap = q.out_cap;
+  out->rec_cap = q.rec_cap;
+  out->n_blocks2 = q.n_blocks2;
+  out->threads2 = q.threads2;
+  out->log2_rounds = q.log2_rounds;
+  out->split2 = q.split2;
+  out->chunks2 = q.chunks2;
+  out->lds_cap = q.lds_cap;
+  out->ok = q.ok ? 1u : 0u;
+  return FB_OK;
 }
 
 // Linear-counting estimate of the distinct keys landing in gate-passing
@@ -1675,10 +1732,8 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     if (!r) r = grow(it.runs, (q.nbr * kBinSlices + 1) * 4);
     if (!r) r = grow(it.slots, q.spill_cap * stride * 8);
     it.spill_cap = q.spill_cap;
-    // entries are nref words wide, so the key buffer is checked in bytes at this k
-    if (!r) r = grow(it.keys, q.out_cap * nref * 8);
-    if (!r) r = grow(it.cov, q.out_cap * 2);
-    if (!r) r = grow(it.mask, q.out_cap);
+    if (!r) r = solid_out_grow(c, it.keys, it.cov, it.mask, q.out_cap, nref);
+    if (!r) r = solid_out_check(c, it.keys, it.cov, it.mask, q.out_cap, nref);
     if (r) return r;
     it.out_cap = q.out_cap;
     DevSolidTable t;
@@ -1841,12 +1896,8 @@ static int batch_global_route(fb_ctx* c, BatchIso& it, uint32_t k, uint32_t gate
     cap *= 2;
   }
   const uint64_t maxn = std::max<uint64_t>(st[1], 1);
-  if (!it.keys.p || it.keys.bytes < maxn * nref * 8)
-    if (int r = it.keys.get(c, maxn * nref * 8)) return r;
-  if (!it.cov.p || it.cov.bytes < maxn * 2)
-    if (int r = it.cov.get(c, maxn * 2)) return r;
-  if (!it.mask.p || it.mask.bytes < maxn)
-    if (int r = it.mask.get(c, maxn)) return r;
+  if (int r = solid_out_grow(c, it.keys, it.cov, it.mask, maxn, nref)) return r;
+  if (int r = solid_out_check(c, it.keys, it.cov, it.mask, maxn, nref)) return r;
   it.out_cap = maxn;
   HIP_TRY(c, launch_pass3(t, k, nref, gate, it.keys.as<uint64_t>(), it.cov.as<uint16_t>(), it.mask.as<uint8_t>(),
                           it.p3.p, s));
@@ -2038,6 +2089,7 @@ struct fb_server {
   std::chrono::steady_clock::time_point t0;
   std::map<uint32_t, std::unique_ptr<fb_batch>> batches;  // per pass index, membership set every iteration
   fb_serve_stats st{};
+  uint32_t waiters = 0;  // fb_server_wait calls in progress (under mu)
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -2198,16 +2250,28 @@ static void serve_loop(fb_server* sv) {
     // one batched pass per ladder position that in-flight requests are at
     std::map<uint32_t, std::vector<ServeReq*>> groups;
     for (ServeReq* r : sv->running) groups[r->stage].push_back(r);
+    // A request published as finished (state 2 or 3) may be released, and so
+    // freed, by a client at once: the worker takes it out of `running` first
+    // and touches it no more after publishing it.
+    std::vector<std::pair<ServeReq*, int>> failed;
+    std::string fail_msg;
     for (auto& kv : groups) {
       if (int e = serve_pass(sv, kv.first, kv.second)) {
-        const std::string msg = sv->ctx->err;
-        for (ServeReq* r : kv.second) {
-          serve_fail(sv, r, e, msg);
-          r->stage = sv->prm.n_passes;  // leaves the running set below
-        }
+        fail_msg = sv->ctx->err;
+        for (ServeReq* r : kv.second) failed.emplace_back(r, e);
         continue;
       }
       for (ServeReq* r : kv.second) ++r->stage;
+    }
+    if (!failed.empty()) {
+      std::vector<ServeReq*> keep;
+      for (ServeReq* r : sv->running) {
+        bool f = false;
+        for (auto& fr : failed) f = f || fr.first == r;
+        if (!f) keep.push_back(r);
+      }
+      sv->running.swap(keep);
+      for (auto& fr : failed) serve_fail(sv, fr.first, fr.second, fail_msg);
     }
     {
       std::lock_guard<std::mutex> g(sv->mu);
@@ -2224,7 +2288,6 @@ static void serve_loop(fb_server* sv) {
         still.push_back(r);
         continue;
       }
-      if (r->state == 3) continue;  // failed and already retired
       r->iso.reset();
       fb_bank_destroy(r->bank);
       r->bank = nullptr;
@@ -2249,7 +2312,7 @@ extern "C" int fb_server_create(int device, const fb_serve_params* p, fb_server*
   if (p->hll_p != 0 && (p->hll_p < 4 || p->hll_p > 18)) return FB_ERR_INVALID;
   auto sv = std::make_unique<fb_server>();
   sv->prm = *p;
-  if (int r = fb_ctx_create(device, 0, &sv->ctx)) return r;
+  if (int r = fb_ctx_create(device, p->ctx_budget, &sv->ctx)) return r;
   fb_build_params_default(&sv->bprm, p->k[0]);
   sv->bprm.cms_width = p->cms_width;
   sv->bprm.counter_bits = p->counter_bits;
@@ -2285,8 +2348,14 @@ extern "C" void fb_server_destroy(fb_server* sv) {
         kv.second->err = "server stopped";
       }
   }
+  sv->cv_done.notify_all();
   sv->cv_work.notify_all();
   if (sv->worker.joinable()) sv->worker.join();
+  {
+    // the waiters of queued and in-flight requests have all been woken
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_done.wait(lk, [&] { return sv->waiters == 0; });
+  }
   sv->reqs.clear();
   sv->batches.clear();
   fb_ctx_destroy(sv->ctx);
@@ -2348,6 +2417,14 @@ extern "C" int fb_server_wait(fb_server* sv, uint64_t ticket, double timeout_s, 
   if (it == sv->reqs.end()) return FB_ERR_INVALID;
   ServeReq* r = it->second.get();
   auto done = [&] { return r->state >= 2; };
+  // counted while waiting, so that fb_server_destroy frees nothing under a waiter
+  struct Waiting {
+    fb_server* sv;
+    explicit Waiting(fb_server* s) : sv(s) { ++sv->waiters; }
+    ~Waiting() {
+      if (--sv->waiters == 0 && sv->stop) sv->cv_done.notify_all();
+    }
+  } waiting(sv);
   if (timeout_s < 0) sv->cv_done.wait(lk, done);
   else if (!sv->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return FB_ERR_TIMEOUT;
   memset(out, 0, sizeof *out);

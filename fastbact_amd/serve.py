@@ -87,6 +87,8 @@ class Server:
         if rc == _lib.FB_ERR_TIMEOUT:
             raise TimeoutError(f"request {ticket} not finished after {timeout} s")
         if rc:
+            if release:
+                self.lib.fb_server_release(self.h, ticket)
             raise FastBactError(rc, f"request {ticket}: {res.error.decode(errors='replace')}")
         est = None
         if res.has_estimate:

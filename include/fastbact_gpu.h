@@ -428,6 +428,25 @@ void fb_batch_destroy(fb_batch* b);
  * outputs, scratch) -- the admission footprint of the serving loop. */
 uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uint64_t reads, const fb_build_params* params,
                             uint32_t k);
+/* Output sizing, host arithmetic only (no device; no counterpart in the
+ * reference).  An isolate of the batched and served forms keeps its solid
+ * output buffers across the k ladder while an entry widens with k (1, 2 or 4
+ * key words), so buffers are sized and checked in bytes at the k of each
+ * launch.  fb_solid_out_bytes: out[0] keys = cap * nref * 8, out[1] cov =
+ * cap * 2, out[2] mask = cap.  fb_solid_out_fits: 1 when buffers of have[0..2]
+ * bytes hold cap entries of nref words; the library refuses (FB_ERR_INVALID)
+ * every launch that writes solid outputs unless this holds. */
+void fb_solid_out_bytes(uint64_t cap, uint32_t nref, uint64_t out[3]);
+int fb_solid_out_fits(uint64_t cap, uint32_t nref, const uint64_t have[3]);
+/* Pass-2 plan of the LDS route for an isolate with est_keys estimated keys
+ * (the sizing fb_batch_footprint uses; a workgroup granted 160 KB of LDS):
+ * bins, rounds, record/spill/output capacities in entries. */
+typedef struct {
+  uint64_t nb2, nbr, spill_cap, out_cap, rec_cap, n_blocks2, threads2;
+  uint32_t log2_rounds, split2, chunks2, lds_cap, ok;
+} fb_pass2_plan;
+int fb_plan_pass2(double est_keys, uint64_t windows, uint64_t n_reads, uint32_t max_len, uint32_t k,
+                  fb_pass2_plan* out);
 int fb_batch_run(fb_batch* b, uint32_t k, int32_t target_solid_cov);
 uint32_t fb_batch_size(const fb_batch* b);
 uint64_t fb_batch_count(const fb_batch* b, uint32_t iso);
@@ -539,6 +558,8 @@ typedef struct {
   uint32_t max_inflight;        /* requests on the device at once; 0 = 64 */
   uint32_t max_queue;           /* waiting requests; 0 = 4096, FB_ERR_BUSY beyond */
   uint32_t keep_sketches;       /* 1 = keep every pass's CMS table and HLL registers with its result */
+  uint64_t ctx_budget;          /* device bytes the server's context may hold; 0 = unlimited.  A
+                                   request whose upload or pass needs more fails with FB_ERR_OOM */
 } fb_serve_params;
 
 typedef struct {
@@ -568,7 +589,8 @@ typedef struct {
 void fb_serve_params_default(fb_serve_params* p);
 int fb_server_create(int device, const fb_serve_params* p, fb_server** out);
 /* Finishes the requests in flight, refuses the queued ones (FB_ERR_BUSY) and
- * frees everything. */
+ * wakes their waiters, waits until every fb_server_wait in progress has
+ * returned, then frees everything.  No other call may start once destroy has. */
 void fb_server_destroy(fb_server* s);
 const char* fb_server_last_error(const fb_server* s);
 /* Copies the read pairs (R1 -> pe_forward, R2 -> pe_reverse) and queues the
@@ -584,7 +606,9 @@ int fb_server_wait(fb_server* s, uint64_t ticket, double timeout_s, fb_serve_res
 int fb_server_download(fb_server* s, uint64_t ticket, uint32_t pass, uint64_t* keys, uint16_t* cov, uint8_t* mask);
 /* The pass's CMS table (fb_cms_download layout) and HLL registers; needs keep_sketches. */
 int fb_server_sketches(fb_server* s, uint64_t ticket, uint32_t pass, void* cms, uint64_t cms_bytes, uint8_t* hll);
-/* Drops a finished request's results. */
+/* Drops a finished (done or failed) request's results.  The worker touches
+ * no request once it has published it as finished, so this may follow
+ * fb_server_wait at once, whatever its status. */
 int fb_server_release(fb_server* s, uint64_t ticket);
 int fb_server_stats(fb_server* s, fb_serve_stats* out);
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import fastbact_amd as F
-from tests.helpers import isolate, oracle_arenas, ragged_bank
+from tests.helpers import distinct_canonical_kmers, isolate, oracle_arenas, ragged_bank
 
 pytestmark = pytest.mark.gpu
 
@@ -110,6 +110,35 @@ def test_prior_contigs_both_pass2_routes(ctx, orc, small_iso, k):
         h.close()
         assert st["pass2_route"] == ("lds" if placement == 2 else "global"), st["pass2_route"]
     assert (r["cov"] == 65535 // 2).any()  # saturated cov of the contig repeated 400 times
+
+
+@pytest.mark.parametrize("k", [21, 55, 77])
+def test_contig_and_read_edges_share_counters(ctx, orc, k):
+    """Reads and prior-contig records of the same keys and edges meet in one
+    bin at once: 2000 error-free reads tile a 700 bp sequence (every key seen
+    hundreds of times, edges counted by reads) while 400 copies of the same
+    sequence as prior contigs saturate the same edge counters (+1000 each)
+    and the interior keys' cov (+100 a side).
+    The LDS route (1 and 2 bins, so each bin holds many keys) and the global
+    route give the oracle's solid set."""
+    rng = np.random.default_rng(k)
+    seq = rng.choice(np.frombuffer(b"ACGT", np.uint8), size=700)
+    reads = []
+    for _ in range(2000):
+        s = int(rng.integers(0, 700 - 150 + 1))
+        r = seq[s:s + 150].tobytes()
+        if rng.random() < 0.5:
+            r = r[::-1].translate(bytes.maketrans(b"ACGT", b"TGCA"))
+        reads.append(r)
+    bank = F.ReadBank(F.BitSeqArena.from_ascii(reads[:1000]), F.BitSeqArena.from_ascii(reads[1000:]))
+    dev = F.DeviceBank(ctx, bank)
+    contigs = [seq.tobytes()] * 400
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=4, width=1 << 16, contigs=contigs)
+    assert len(r["cov"]) == 700 - k + 1 and (r["cov"] == 65535 // 2).sum() >= 600 - k
+    for placement, bins2 in ((2, 1), (2, 2), (2, 0), (1, 0)):
+        g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=4, cms_width=1 << 16, contigs=contigs,
+                                placement=placement, bins2=bins2)
+        _solid_equal(g, r)
 
 
 def test_empty_bank(ctx, orc):
@@ -272,10 +301,13 @@ def test_1mbp_isolate_100k_pairs(ctx, orc):
         _solid_equal(g, r)
         assert np.array_equal(g.cms_table, r["cms"].table())
         assert np.array_equal(g.hll_registers, r["hll"])
-    # HLL of distinct 31-mers within the reference's 3% gate of the exact count
-    est = F.hll_estimate_registers(g.hll_registers, 14)
     assert 0.5e6 < len(g) < 1.1e6
-    assert est > len(g)
+    # the pass's HLL (splitmix64 of the canonical ntHash) estimates the
+    # distinct canonical 31-mers within the reference's 3% gate
+    # (sketch.rs:262-284) of their exact count
+    est = F.hll_estimate_registers(g.hll_registers, 14)
+    exact = distinct_canonical_kmers([iso.r1, iso.r2], 31)
+    assert abs(est - exact) / exact < 0.03, (est, exact)
 
 
 @pytest.mark.parametrize("k", [33, 55, 64, 77, 127])

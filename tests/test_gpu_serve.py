@@ -127,3 +127,48 @@ def test_serve_errors():
         with pytest.raises(F.FastBactError) as e:
             sv.submit(bank.pe_forward, bank.pe_reverse)
         assert e.value.code == F._lib.FB_ERR_OOM
+
+
+def test_failed_pass_released_at_once(requests_):
+    """A pass that fails fails every request of its group.  Here the server's
+    context is capped (ctx_budget 8 MiB) below what pass 1 of these requests
+    needs, while upload and sketches fit: each request ends with FB_ERR_OOM and
+    is released the moment its wait returns (Server.wait releases on error),
+    while the worker goes on with the others; the server then closes."""
+    kw = dict(cms_width=1 << 17, counter_bits=16, hll_p=12, estimate=0, ctx_budget=8 << 20)
+    with F.Server(0, KS, TARGETS, **kw) as sv:
+        tickets = [sv.submit(b.pe_forward, b.pe_reverse) for b in requests_[:3]]
+        for t in tickets:
+            with pytest.raises(F.FastBactError) as e:
+                sv.wait(t, timeout=60)
+            assert e.value.code == F._lib.FB_ERR_OOM, e.value
+            # released: the ticket is gone
+            assert sv.lib.fb_server_release(sv.h, t) == F._lib.FB_ERR_INVALID
+        st = sv.stats()
+    assert st["failed"] == 3 and st["completed"] == 0, st
+
+
+def test_destroy_wakes_waiters(requests_):
+    """fb_server_destroy with a waiter blocked on a queued request: the
+    waiter returns (FB_ERR_BUSY when the request was still queued, its
+    result when it had been admitted) and destroy returns once it has."""
+    import ctypes as C
+    import threading
+
+    kw = dict(cms_width=1 << 17, counter_bits=16, hll_p=12, max_inflight=1)
+    sv = F.Server(0, KS, TARGETS, **kw)
+    ts = [sv.submit(b.pe_forward, b.pe_reverse) for b in requests_[:3]]
+    rcs = {}
+
+    def waiter():
+        res = F._lib.ServeResult()
+        rcs["last"] = sv.lib.fb_server_wait(sv.h, ts[-1], -1.0, C.byref(res))
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    time.sleep(0.05)
+    h, sv.h = sv.h, None
+    sv.lib.fb_server_destroy(h)  # returns only after the waiter has
+    th.join(timeout=60)
+    assert not th.is_alive()
+    assert rcs["last"] in (0, F._lib.FB_ERR_BUSY), rcs

@@ -321,7 +321,9 @@ def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05)
                           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
                                          "p99": round(float(np.percentile(lat, 99)), 1)},
                           "gbases_s": round(bases / wall / 1e9, 3), "max_batch": st["max_batch"],
-                          "max_ladder_positions": st["max_groups"]}
+                          "max_ladder_positions": st["max_groups"],
+                          "worker_s": {n: round(st[n], 2) for n in ("start_s", "pass_s", "fetch_s")}}
+        st0 = st
         # capacity: the same requests all at once
         t0 = time.perf_counter()
         tickets = [sv.submit(banks[int(p)].pe_forward, banks[int(p)].pe_reverse) for p in picks]
@@ -333,7 +335,8 @@ def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05)
                             "isolates_per_hour": round(n_req / wall * 3600, 1),
                             "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 1),
                                            "p99": round(float(np.percentile(lat, 99)), 1)},
-                            "gbases_s": round(bases / wall / 1e9, 3), "max_batch": st["max_batch"]}
+                            "gbases_s": round(bases / wall / 1e9, 3), "max_batch": st["max_batch"],
+                            "worker_s": {n: round(st[n] - st0[n], 2) for n in ("start_s", "pass_s", "fetch_s")}}
     out["workload"] = ("requests drawn from the batched-workload isolates, each the k ladder 21/55/77 (gates None/4/4) "
                        "after estimate_genome; CMS 4x2^23 u16, HLL p14; latency = submit to solid sets on the host")
     return out
@@ -459,6 +462,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests of the Poisson serving stream on batched-workload isolates (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
+    ap.add_argument("--no-next-rows", action="store_true",
+                    help="skip compaction and the next rows (a short run of the timed steps alone, for profiling)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this many MiB device to device once (a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -531,11 +536,13 @@ def main(argv=None) -> int:
         copy_calibration(ctx, args.copy_calibration_mib)
     roofline["traffic"], roofline["traffic_source"] = measured_traffic(dom)
 
-    compaction, g21 = bench_compaction(ctx, dev, F, cpu=not args.no_cpu_baseline)
-    log(f"compaction: {compaction}")
-    next_rows = bench_next_rows(ctx, dev, F, iso, bank, g21, cpu=not args.no_cpu_baseline)
-    log(f"next rows: {next_rows}")
-    del g21
+    compaction = next_rows = None
+    if not args.no_next_rows:
+        compaction, g21 = bench_compaction(ctx, dev, F, cpu=not args.no_cpu_baseline)
+        log(f"compaction: {compaction}")
+        next_rows = bench_next_rows(ctx, dev, F, iso, bank, g21, cpu=not args.no_cpu_baseline)
+        log(f"next rows: {next_rows}")
+        del g21
 
     batched = served = None
     if args.batch_isolates or args.serve_requests:

@@ -248,6 +248,9 @@ class ServeStats(C.Structure):
         ("inflight_bytes", C.c_uint64),
         ("device_bytes", C.c_uint64),
         ("busy_s", C.c_double),
+        ("start_s", C.c_double),
+        ("pass_s", C.c_double),
+        ("fetch_s", C.c_double),
     ]
 
 

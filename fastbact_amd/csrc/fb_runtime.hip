@@ -2189,7 +2189,9 @@ static int serve_pass(fb_server* sv, uint32_t j, std::vector<ServeReq*>& g) {
   fb_batch* b = slot.get();
   b->iso.clear();
   for (ServeReq* r : g) b->iso.push_back(std::move(r->iso));
+  const double t0 = sv->now();
   int rc = fb_batch_run(b, k, target);
+  const double t1 = sv->now();
   // results are read while the batch still holds the isolates
   for (size_t i = 0; i < g.size() && !rc; ++i) {
     const BatchIso& it = *b->iso[i];
@@ -2212,6 +2214,9 @@ static int serve_pass(fb_server* sv, uint32_t j, std::vector<ServeReq*>& g) {
   }
   for (size_t i = 0; i < g.size(); ++i) g[i]->iso = std::move(b->iso[i]);
   b->iso.clear();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->st.pass_s += t1 - t0;
+  sv->st.fetch_s += sv->now() - t1;
   return rc;
 }
 
@@ -2235,7 +2240,13 @@ static void serve_loop(fb_server* sv) {
       }
     }
     for (ServeReq* r : admitted) {
-      if (int e = serve_start(sv, r)) {
+      const double t0 = sv->now();
+      const int e = serve_start(sv, r);
+      {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->st.start_s += sv->now() - t0;
+      }
+      if (e) {
         serve_fail(sv, r, e, sv->ctx->err);
         continue;
       }

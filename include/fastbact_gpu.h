@@ -584,6 +584,9 @@ typedef struct {
   uint64_t max_groups;          /* most ladder positions in one iteration */
   uint64_t budget, inflight_bytes, device_bytes;
   double busy_s;                /* time the worker spent running passes */
+  double start_s;               /* worker time admitting requests: upload, estimate, sketches */
+  double pass_s;                /* worker time in the batched passes (device work and its sizing) */
+  double fetch_s;               /* worker time copying results to the host */
 } fb_serve_stats;
 
 void fb_serve_params_default(fb_serve_params* p);

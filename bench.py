@@ -111,6 +111,23 @@ def copy_calibration(ctx, mib: int) -> None:
     log(f"copy calibration: {mib} MiB device-to-device (k_copy_calibration)")
 
 
+def atomic_sweep(ctx) -> list:
+    """Random u32 atomicAdd rate (k_atomic_rate) at table sizes from
+    L2-resident (4 MiB) to HBM-resident (1 GiB): the device figures the
+    passes' atomics are compared with."""
+    import ctypes as C
+    out = []
+    n_updates = 1 << 28
+    for log2n in (20, 22, 24, 26, 28):
+        for ret in (0, 1):
+            ms = C.c_float()
+            ctx.check(ctx.lib.fb_diag_atomic_rate(ctx.h, log2n, n_updates, ret, C.byref(ms)), "fb_diag_atomic_rate")
+            out.append({"table_mib": (4 << log2n) >> 20, "with_return": bool(ret), "updates": n_updates,
+                        "ms": round(ms.value, 3), "g_atomics_s": round(n_updates / ms.value / 1e6, 1)})
+    log(f"atomic sweep: {[(o['table_mib'], o['with_return'], o['g_atomics_s']) for o in out]}")
+    return out
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_full_isolate_k21_55_77.json")
 
 
@@ -543,6 +560,7 @@ def main(argv=None) -> int:
         next_rows = bench_next_rows(ctx, dev, F, iso, bank, g21, cpu=not args.no_cpu_baseline)
         log(f"next rows: {next_rows}")
         del g21
+    atomics = atomic_sweep(ctx) if not args.no_next_rows else None
 
     batched = served = None
     if args.batch_isolates or args.serve_requests:
@@ -604,6 +622,7 @@ def main(argv=None) -> int:
         "next_rows": next_rows,
         "batched": batched,
         "serving": served,
+        "atomic_sweep": atomics,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

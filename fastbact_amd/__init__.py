@@ -26,7 +26,9 @@ from .api import (  # noqa: F401
     estimate_genome,
     estimate_genome_files,
     filter_active_reads,
+    auto_subsample_factor,
     hll_estimate_registers,
+    nthash_windows,
     ingest_arrays,
     ingest_files,
     ingest_records,

@@ -287,6 +287,10 @@ SIGNATURES = [
     ("fb_hll_clear", C.c_int, [vp]),
     ("fb_hll_add_hashes", C.c_int, [vp, u64p, C.c_uint64]),
     ("fb_hll_add_bank", C.c_int, [vp, vp, C.c_uint32]),
+    ("fb_hll_add_canonical_hashes", C.c_int, [vp, u64p, C.c_uint64]),
+    ("fb_hll_merge", C.c_int, [vp, vp]),
+    ("fb_cms_add_and_estimate", C.c_int, [vp, u64p, C.c_uint64, u32p]),
+    ("fb_nthash_windows", C.c_int, [vp, u8p, C.c_uint64, C.c_uint32, u64p]),
     ("fb_hll_registers", C.c_int, [vp, u8p]),
     ("fb_hll_estimate", C.c_int, [vp, C.POINTER(C.c_double)]),
     ("fb_hll_estimate_registers", C.c_double, [u8p, C.c_int]),
@@ -302,6 +306,7 @@ SIGNATURES = [
     ("fb_solid_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_solid_kernel_timings", C.c_int, [vp, C.POINTER(C.c_float)]),
     ("fb_diag_copy", C.c_int, [vp, C.c_uint64]),
+    ("fb_diag_atomic_rate", C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_float)]),
     ("fb_solid_stats", C.c_int, [vp, u64p]),
     ("fb_compact_solid", C.c_int, [vp, C.POINTER(vp)]),
     ("fb_compact_arrays", C.c_int, [vp, u64p, u16p, u8p, C.c_uint64, C.c_uint32, C.POINTER(vp)]),
@@ -377,6 +382,8 @@ SIGNATURES = [
     ("fb_count_fastx", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(CountStats)]),
     ("fb_estimate_genome_files", C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_uint64, C.c_double,
                                            C.POINTER(GenomeEstimateC)]),
+    ("fb_auto_subsample_factor", C.c_double, [C.POINTER(GenomeEstimateC), C.c_double, C.c_double, C.c_uint64,
+                                              C.POINTER(C.c_int32)]),
     ("fb_estimate_genome", C.c_int, [vp, C.POINTER(ArenaView), C.POINTER(ArenaView), C.POINTER(CountStats),
                                      C.c_uint64, C.c_double, C.POINTER(GenomeEstimateC)]),
     ("fb_pack_ascii", C.c_int, [u8p, u32p, C.c_uint64, u64p, u64p]),

@@ -256,6 +256,14 @@ class CountMinSketch(_Handle):
                        "fb_cms_query")
         return out
 
+    def add_and_estimate(self, keys) -> np.ndarray:
+        """add_and_estimate (sketch.rs:248-253) for each key in order."""
+        k = _u64(np.atleast_1d(keys))
+        out = np.zeros(k.size, np.uint32)
+        self.ctx.check(self.ctx.lib.fb_cms_add_and_estimate(self.h, _ptr(k, C.c_uint64), k.size,
+                                                            _ptr(out, C.c_uint32)), "fb_cms_add_and_estimate")
+        return out
+
     def add_bank(self, bank: DeviceBank, k: int) -> None:
         self.ctx.check(self.ctx.lib.fb_cms_add_bank(self.h, bank.h, k), "fb_cms_add_bank")
 
@@ -285,6 +293,16 @@ class HyperLogLog(_Handle):
         a = _u64(np.atleast_1d(hashes))
         self.ctx.check(self.ctx.lib.fb_hll_add_hashes(self.h, _ptr(a, C.c_uint64), a.size), "fb_hll_add_hashes")
 
+    def add_canonical_hash(self, hashes) -> None:
+        """add_canonical_hash (sketch.rs:66-70): add_hash(splitmix64(h))."""
+        a = _u64(np.atleast_1d(hashes))
+        self.ctx.check(self.ctx.lib.fb_hll_add_canonical_hashes(self.h, _ptr(a, C.c_uint64), a.size),
+                       "fb_hll_add_canonical_hashes")
+
+    def merge(self, other: "HyperLogLog") -> None:
+        """HyperLogLog::merge (sketch.rs:73-80)."""
+        self.ctx.check(self.ctx.lib.fb_hll_merge(self.h, other.h), "fb_hll_merge")
+
     def add_bank(self, bank: DeviceBank, k: int) -> None:
         self.ctx.check(self.ctx.lib.fb_hll_add_bank(self.h, bank.h, k), "fb_hll_add_bank")
 
@@ -300,6 +318,16 @@ class HyperLogLog(_Handle):
 
     def clear(self) -> None:
         self.ctx.check(self.ctx.lib.fb_hll_clear(self.h), "fb_hll_clear")
+
+
+def nthash_windows(ctx: Context, seq: bytes, k: int) -> np.ndarray:
+    """RollingNtHash (kmer.rs:433-477): the canonical ntHash of every window
+    of seq (new on the first k bases, then roll)."""
+    buf = np.frombuffer(bytes(seq), np.uint8).copy()
+    out = np.zeros(max(buf.size - k + 1, 0), np.uint64)
+    ctx.check(ctx.lib.fb_nthash_windows(ctx.h, _ptr(buf, C.c_uint8), buf.size, k, _ptr(out, C.c_uint64)),
+              "fb_nthash_windows")
+    return out
 
 
 def hll_estimate_registers(regs: np.ndarray, p: int) -> float:
@@ -864,6 +892,22 @@ def estimate_genome(ctx: Context, r1: BitSeqArena, r2: BitSeqArena, max_pairs: i
                           out.max_read_len, out.estimated_cov, out.subsample_factor,
                           [int(out.kmers[i]) for i in range(out.n_kmers)], CONFIDENCE[out.confidence],
                           out.u_ge_1, out.u_ge_2, out.sample_kmer_depth, out.pairs_sampled, out.m_sample)
+
+
+SUBSAMPLE_REASONS = ("genome_size", "estimate", "within_trigger", "estimate_not_high")
+
+
+def auto_subsample_factor(est: GenomeEstimate, depth: float, threshold_ratio: float = 1.5,
+                          genome_size: int = 0) -> tuple:
+    """The pipeline's auto-subsampling policy (main.rs:66-185): (factor, reason)."""
+    e = GenomeEstimateC()
+    e.total_bases = est.total_bases
+    e.estimated_cov = est.estimated_cov
+    e.confidence = {v: k for k, v in CONFIDENCE.items()}[est.confidence]
+    why = C.c_int32()
+    f = _lib.load().fb_auto_subsample_factor(C.byref(e), float(depth), float(threshold_ratio), int(genome_size),
+                                             C.byref(why))
+    return float(f), SUBSAMPLE_REASONS[why.value]
 
 
 def _estimate_result(out) -> GenomeEstimate:

@@ -181,6 +181,30 @@ void estimate_finish(uint64_t u_ge_1, uint64_t u_ge_2, uint64_t m_sample, const 
 
 }  // namespace fb
 
+// main.rs:66-185 with --enable-auto-subsample and without --no-subsample.
+extern "C" double fb_auto_subsample_factor(const fb_genome_estimate* est, double depth, double threshold_ratio,
+                                           uint64_t genome_size, int32_t* reason) {
+  int32_t why = 2;
+  double factor = 1.0;
+  double cov = 0.0;
+  bool have = false;
+  if (genome_size > 0 && est) {
+    cov = (double)est->total_bases / (double)genome_size;
+    have = true;
+  } else if (est && est->confidence == 0) {
+    cov = est->estimated_cov;
+    have = true;
+  } else {
+    why = 3;
+  }
+  if (have && depth > 0.0 && cov > threshold_ratio * depth) {
+    factor = std::min(1.0, std::max(0.01, depth / cov));
+    why = genome_size > 0 ? 0 : 1;
+  }
+  if (reason) *reason = why;
+  return factor;
+}
+
 // estimator.rs:104-161
 extern "C" uint32_t fb_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uint32_t* out) {
   uint64_t min_k = median_len < 75 ? 21 : 31;

@@ -150,6 +150,8 @@ hipError_t launch_pack_u8(const uint32_t* regs, uint64_t n, uint8_t* out, hipStr
 hipError_t launch_table_clear(const DevSolidTable& t, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_copy_calibration(const void* src, void* dst, uint64_t n16, hipStream_t s);
+hipError_t launch_atomic_rate(uint32_t* table, uint32_t log2n, uint64_t n_updates, bool ret, uint32_t* sink,
+                              hipStream_t s);
 // Pass 2 (build.rs:140-289).
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);
 hipError_t launch_pass2_multi(const MultiLaunch& m, uint32_t k, hipStream_t s);
@@ -261,6 +263,17 @@ hipError_t launch_cms_add_keys(DevCms c, const uint64_t* keys, uint64_t n, uint3
 hipError_t launch_cms_query_keys(DevCms c, const uint64_t* keys, uint64_t n, uint32_t cap, uint32_t* out,
                                  hipStream_t s);
 hipError_t launch_hll_add_hashes(DevHll h, const uint64_t* hashes, uint64_t n, hipStream_t s);
+hipError_t launch_hll_add_canonical(DevHll h, const uint64_t* hashes, uint64_t n, hipStream_t s);
+hipError_t launch_hll_merge(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s);
+hipError_t launch_cms_row_cells(DevCms c, const uint64_t* keys, uint64_t n, uint32_t row, uint32_t* cells,
+                                uint32_t* idx, hipStream_t s);
+hipError_t launch_cms_row_estimate(const uint32_t* counts, const uint32_t* cells, const uint32_t* idx, uint64_t n,
+                                   uint32_t row, uint32_t cap, uint32_t* out, hipStream_t s);
+// Stable radix sort of (u32 key, u32 value) pairs on key bits [0, end_bit);
+// tmp == nullptr sizes *tmp_bytes.
+hipError_t sort_u32_pairs(const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                          uint32_t end_bit, void* tmp, size_t* tmp_bytes, hipStream_t s);
+hipError_t launch_nthash_windows(const uint8_t* seq, uint64_t n_windows, uint32_t k, uint64_t* out, hipStream_t s);
 
 // Estimator (estimator.rs:296-379), first-insertion-time form.  Reads are
 // R1[0], R2[0], R1[1], R2[1], ... in the order the reference visits them.

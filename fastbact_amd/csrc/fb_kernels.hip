@@ -256,6 +256,37 @@ hipError_t launch_copy_calibration(const void* src, void* dst, uint64_t n16, hip
   return hipGetLastError();
 }
 
+// Random u32 atomicAdd over a table of 2^log2n counters: update i goes to
+// splitmix64(i ^ seed) mod 2^log2n, with (ret) or without a returned value.
+// It measures the device's atomic rate at a given table size (L2-resident,
+// infinity-cache-resident or HBM-resident), the figure the Count-Min and
+// bin-cursor atomics of the passes are compared with.
+template <bool RET>
+__global__ __launch_bounds__(kBlock) void k_atomic_rate(uint32_t* __restrict__ table, uint32_t log2n,
+                                                        uint64_t n_updates, uint64_t seed, uint32_t* __restrict__ sink) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t mask = (1ull << log2n) - 1ull;
+  uint32_t acc = 0;
+  for (; i < n_updates; i += stride) {
+    const uint64_t c = splitmix64(i ^ seed) & mask;
+    if (RET) acc += atomicAdd(table + c, 1u);
+    else atomicAdd(table + c, 1u);
+  }
+  if (RET && acc == 0xFFFFFFFFu) sink[0] = acc;  // keeps the returned values live
+}
+hipError_t launch_atomic_rate(uint32_t* table, uint32_t log2n, uint64_t n_updates, bool ret, uint32_t* sink,
+                              hipStream_t s) {
+  const uint64_t g = 256u * 64u;  // 64 blocks of 256 per CU: enough updates in flight
+  if (ret)
+    hipLaunchKernelGGL(k_atomic_rate<true>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates,
+                       0x6a09e667f3bcc909ull, sink);
+  else
+    hipLaunchKernelGGL(k_atomic_rate<false>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates,
+                       0x6a09e667f3bcc909ull, sink);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kBlock) void k_table_clear(uint64_t* __restrict__ slots, uint64_t cap,
                                                         uint32_t stride, uint32_t kw) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -917,6 +948,110 @@ __global__ __launch_bounds__(kBlock) void k_hll_add_hashes(DevHll h, const uint6
 hipError_t launch_hll_add_hashes(DevHll h, const uint64_t* hashes, uint64_t n, hipStream_t s) {
   if (!n) return hipSuccess;
   hipLaunchKernelGGL(k_hll_add_hashes, dim3(grid_for(n)), dim3(kBlock), 0, s, h, hashes, n);
+  return hipGetLastError();
+}
+
+// add_canonical_hash (sketch.rs:66-70): add_hash of the splitmix64-mixed hash.
+__global__ __launch_bounds__(kBlock) void k_hll_add_canonical(DevHll h, const uint64_t* __restrict__ hashes,
+                                                              uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t idx, rho;
+  hll_index_rho(splitmix64(hashes[i]), h.p, idx, rho);
+  if (rho > h.regs[idx]) atomicMax(h.regs + idx, rho);
+}
+hipError_t launch_hll_add_canonical(DevHll h, const uint64_t* hashes, uint64_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_hll_add_canonical, dim3(grid_for(n)), dim3(kBlock), 0, s, h, hashes, n);
+  return hipGetLastError();
+}
+
+// HyperLogLog::merge (sketch.rs:73-80): register-wise max.
+__global__ __launch_bounds__(kBlock) void k_hll_merge(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                                      uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && src[i] > dst[i]) dst[i] = src[i];
+}
+hipError_t launch_hll_merge(uint32_t* dst, const uint32_t* src, uint64_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_hll_merge, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+
+// add_and_estimate (sketch.rs:248-253) over a batch in index order.  For row
+// `row`: cells[i] = key i's cell, idx[i] = i, to be sorted by cell (a stable
+// sort keeps the indices of one cell increasing).
+__global__ __launch_bounds__(kBlock) void k_cms_row_cells(DevCms c, const uint64_t* __restrict__ keys, uint64_t n,
+                                                          uint32_t row, uint32_t* __restrict__ cells,
+                                                          uint32_t* __restrict__ idx) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t wmask = (1ull << c.log2w) - 1ull;
+  cells[i] = (uint32_t)(((uint64_t)row << c.log2w) | (splitmix64(keys[i] ^ cms_seed((int)row)) & wmask));
+  idx[i] = (uint32_t)i;
+}
+// Sorted position p holds the rank-th add to its cell within the batch (rank
+// = p minus the first position of its cell's run), so right after that add
+// the cell reads counts[cell] + rank + 1; the estimate is the minimum over
+// the rows, clamped at cap.  Row 0 writes out[], later rows lower it.
+__global__ __launch_bounds__(kBlock) void k_cms_row_estimate(const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ cells,
+                                                             const uint32_t* __restrict__ idx, uint64_t n, uint32_t row,
+                                                             uint32_t cap, uint32_t* __restrict__ out) {
+  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t cell = cells[p];
+  uint64_t lo = 0, hi = p;  // first position whose cell equals cell
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (cells[mid] < cell) lo = mid + 1;
+    else hi = mid;
+  }
+  const uint64_t v = (uint64_t)counts[cell] + (p - lo) + 1u;
+  const uint32_t e = v < cap ? (uint32_t)v : cap;
+  const uint32_t i = idx[p];
+  if (row == 0 || e < out[i]) out[i] = e;
+}
+hipError_t launch_cms_row_cells(DevCms c, const uint64_t* keys, uint64_t n, uint32_t row, uint32_t* cells,
+                                uint32_t* idx, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_cms_row_cells, dim3(grid_for(n)), dim3(kBlock), 0, s, c, keys, n, row, cells, idx);
+  return hipGetLastError();
+}
+hipError_t launch_cms_row_estimate(const uint32_t* counts, const uint32_t* cells, const uint32_t* idx, uint64_t n,
+                                   uint32_t row, uint32_t cap, uint32_t* out, hipStream_t s) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_cms_row_estimate, dim3(grid_for(n)), dim3(kBlock), 0, s, counts, cells, idx, n, row, cap, out);
+  return hipGetLastError();
+}
+hipError_t sort_u32_pairs(const uint32_t* kin, uint32_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                          uint32_t end_bit, void* tmp, size_t* tmp_bytes, hipStream_t s) {
+  return hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, kin, kout, vin, vout, (int)n, 0, (int)end_bit, s);
+}
+
+// RollingNtHash (kmer.rs:433-477) over an ASCII sequence: thread t takes
+// windows [64t, 64t + 64), pushing its first window's k bases (new) and
+// rolling from there (roll); out[i] = canonical hash of window i.
+__device__ __forceinline__ uint32_t ascii_code(uint8_t c) {
+  return (c == 'C' || c == 'c') ? 1u : (c == 'G' || c == 'g') ? 2u : (c == 'T' || c == 't') ? 3u : 0u;
+}
+__global__ __launch_bounds__(kBlock) void k_nthash_windows(const uint8_t* __restrict__ seq, uint64_t n_windows,
+                                                           uint32_t k, uint64_t* __restrict__ out) {
+  const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 64u;
+  if (w0 >= n_windows) return;
+  const uint64_t w1 = w0 + 64u < n_windows ? w0 + 64u : n_windows;
+  NtHash h{0, 0};
+  for (uint32_t j = 0; j < k; ++j) h.push(ascii_code(seq[w0 + j]), k);
+  out[w0] = h.canonical();
+  for (uint64_t w = w0 + 1; w < w1; ++w) {
+    h.roll(ascii_code(seq[w - 1]), ascii_code(seq[w + k - 1]), k);
+    out[w] = h.canonical();
+  }
+}
+hipError_t launch_nthash_windows(const uint8_t* seq, uint64_t n_windows, uint32_t k, uint64_t* out, hipStream_t s) {
+  if (!n_windows) return hipSuccess;
+  const uint64_t threads = (n_windows + 63) / 64;
+  hipLaunchKernelGGL(k_nthash_windows, dim3(grid_for(threads)), dim3(kBlock), 0, s, seq, n_windows, k, out);
   return hipGetLastError();
 }
 

@@ -43,6 +43,7 @@ struct fb_ctx {
   hipEvent_t ev[12] = {};
 };
 
+
 static int set_err(fb_ctx* c, int code, const char* fmt, ...) {
   if (c) {
     char buf[512];
@@ -233,6 +234,28 @@ extern "C" int fb_diag_copy(fb_ctx* c, uint64_t bytes) {
   HIP_TRY(c, hipStreamSynchronize(s));
   HIP_TRY(c, launch_copy_calibration(src.p, dst.p, n16, s));
   HIP_TRY(c, hipStreamSynchronize(s));
+  return FB_OK;
+}
+
+extern "C" int fb_diag_atomic_rate(fb_ctx* c, uint32_t log2_counters, uint64_t n_updates, int with_return,
+                                   float* ms) {
+  if (!c || !ms || log2_counters < 10 || log2_counters > 30) return set_err(c, FB_ERR_INVALID, "bad argument");
+  if (int r = ctx_current(c)) return r;
+  const uint64_t n = 1ull << log2_counters;
+  DBuf table, sink;
+  if (int r = table.get(c, n * 4)) return r;
+  if (int r = sink.get(c, 4)) return r;
+  hipStream_t s = c->stream;
+  HIP_TRY(c, launch_fill_u32(table.as<uint32_t>(), n, 0, s));
+  // one launch to bring the table into whatever cache level holds it, then the timed one
+  HIP_TRY(c, launch_atomic_rate(table.as<uint32_t>(), log2_counters, n_updates, with_return != 0,
+                                sink.as<uint32_t>(), s));
+  HIP_TRY(c, hipEventRecord(c->ev[0], s));
+  HIP_TRY(c, launch_atomic_rate(table.as<uint32_t>(), log2_counters, n_updates, with_return != 0,
+                                sink.as<uint32_t>(), s));
+  HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  HIP_TRY(c, hipEventSynchronize(c->ev[1]));
+  HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
   return FB_OK;
 }
 
@@ -489,6 +512,55 @@ extern "C" int fb_cms_query(fb_cms* m, const uint64_t* keys, uint64_t n, uint32_
   return FB_OK;
 }
 
+extern "C" int fb_cms_add_and_estimate(fb_cms* m, const uint64_t* keys, uint64_t n, uint32_t* out) {
+  if (!m || (n && (!keys || !out))) return FB_ERR_INVALID;
+  fb_ctx* c = m->ctx;
+  if (int r = ctx_current(c)) return r;
+  if (!n) return FB_OK;
+  if (n > 0xFFFFFFFFull) return set_err(c, FB_ERR_INVALID, "at most 2^32 - 1 keys per call");
+  if (int r = check_add_budget(c, n)) return r;
+  hipStream_t s = c->stream;
+  DBuf d, o, cells, idx, scells, sidx, tmp;
+  if (int r = to_device(c, d, keys, n * 8)) return r;
+  if (int r = o.get(c, n * 4)) return r;
+  if (int r = cells.get(c, n * 4)) return r;
+  if (int r = idx.get(c, n * 4)) return r;
+  if (int r = scells.get(c, n * 4)) return r;
+  if (int r = sidx.get(c, n * 4)) return r;
+  const uint32_t end_bit = m->log2w + 2;  // cells < 4 * width
+  size_t tb = 0;
+  HIP_TRY(c, sort_u32_pairs(nullptr, nullptr, nullptr, nullptr, n, end_bit, nullptr, &tb, s));
+  if (int r = tmp.get(c, tb)) return r;
+  // every row's estimates read the counts before the batch, then the batch is added
+  for (uint32_t row = 0; row < 4; ++row) {
+    HIP_TRY(c, launch_cms_row_cells(m->dev(), d.as<uint64_t>(), n, row, cells.as<uint32_t>(), idx.as<uint32_t>(), s));
+    size_t t2 = tb;
+    HIP_TRY(c, sort_u32_pairs(cells.as<uint32_t>(), scells.as<uint32_t>(), idx.as<uint32_t>(), sidx.as<uint32_t>(), n,
+                              end_bit, tmp.p, &t2, s));
+    HIP_TRY(c, launch_cms_row_estimate(m->counts.as<uint32_t>(), scells.as<uint32_t>(), sidx.as<uint32_t>(), n, row,
+                                       m->cap(), o.as<uint32_t>(), s));
+  }
+  HIP_TRY(c, launch_cms_add_keys(m->dev(), d.as<uint64_t>(), n, 1, s));
+  HIP_TRY(c, hipMemcpyAsync(out, o.p, n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return FB_OK;
+}
+
+extern "C" int fb_nthash_windows(fb_ctx* c, const uint8_t* seq, uint64_t len, uint32_t k, uint64_t* out) {
+  if (!c || k == 0 || (len && !seq)) return set_err(c, FB_ERR_INVALID, "bad argument");
+  if (len < k) return FB_OK;  // no window
+  if (!out) return set_err(c, FB_ERR_INVALID, "null output");
+  if (int r = ctx_current(c)) return r;
+  const uint64_t nw = len - k + 1;
+  DBuf d, o;
+  if (int r = to_device(c, d, seq, len)) return r;
+  if (int r = o.get(c, nw * 8)) return r;
+  HIP_TRY(c, launch_nthash_windows(d.as<uint8_t>(), nw, k, o.as<uint64_t>(), c->stream));
+  HIP_TRY(c, hipMemcpyAsync(out, o.p, nw * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+
 static uint32_t default_split(uint32_t k, uint32_t max_len) {
   // about 48 windows per thread
   uint32_t win = max_len >= k ? max_len - k + 1 : 1;
@@ -586,6 +658,27 @@ extern "C" int fb_hll_add_hashes(fb_hll* h, const uint64_t* hashes, uint64_t n) 
   DBuf d;
   if (int r = to_device(c, d, hashes, n * 8)) return r;
   HIP_TRY(c, launch_hll_add_hashes(h->dev(), d.as<uint64_t>(), n, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+extern "C" int fb_hll_add_canonical_hashes(fb_hll* h, const uint64_t* hashes, uint64_t n) {
+  if (!h || (n && !hashes)) return FB_ERR_INVALID;
+  fb_ctx* c = h->ctx;
+  if (int r = ctx_current(c)) return r;
+  DBuf d;
+  if (int r = to_device(c, d, hashes, n * 8)) return r;
+  HIP_TRY(c, launch_hll_add_canonical(h->dev(), d.as<uint64_t>(), n, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return FB_OK;
+}
+extern "C" int fb_hll_merge(fb_hll* dst, const fb_hll* src) {
+  if (!dst || !src) return FB_ERR_INVALID;
+  fb_ctx* c = dst->ctx;
+  if (dst->p != src->p)
+    return set_err(c, FB_ERR_INVALID, "HLL precisions differ (%d and %d)", dst->p, src->p);
+  if (int r = ctx_current(c)) return r;
+  if (src->ctx != c) HIP_TRY(c, hipStreamSynchronize(src->ctx->stream));
+  HIP_TRY(c, launch_hll_merge(dst->regs.as<uint32_t>(), src->regs.as<uint32_t>(), (uint64_t)1 << dst->p, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return FB_OK;
 }

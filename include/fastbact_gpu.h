@@ -79,6 +79,13 @@ int fb_ctx_trim(fb_ctx* ctx);
  * copy of `bytes` (kernel k_copy_calibration reads and writes bytes rounded
  * up to 16), for scaling hardware traffic counters against a known size. */
 int fb_diag_copy(fb_ctx* ctx, uint64_t bytes);
+/* Diagnostics, no counterpart in the reference: the device's rate of random
+ * 32-bit atomicAdd over a table of 2^log2_counters u32 (10..30), n_updates
+ * updates at splitmix64-spread cells, with (with_return 1) or without the
+ * returned value; *ms = the time of one launch (kernel k_atomic_rate) after
+ * a warm-up launch.  Small tables stay in L2, mid-size ones in the infinity
+ * cache, large ones reach HBM. */
+int fb_diag_atomic_rate(fb_ctx* ctx, uint32_t log2_counters, uint64_t n_updates, int with_return, float* ms);
 /* Free and total device memory of `device` in bytes. */
 int fb_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Library version string. */
@@ -116,6 +123,13 @@ int fb_cms_clear(fb_cms* cms);
 int fb_cms_add_keys(fb_cms* cms, const uint64_t* keys, uint64_t n);
 /* CountMinSketch::estimate for each key. */
 int fb_cms_query(fb_cms* cms, const uint64_t* keys, uint64_t n, uint32_t* out);
+/* CountMinSketch::add_and_estimate (sketch.rs:248-253) for keys[0..n) in
+ * order: out[i] is what add_and_estimate(keys[i]) returns when the keys are
+ * added one at a time, earlier keys of the batch included (each row's cell
+ * reads its count before the call plus the adds of keys 0..i landing in
+ * it).  On the device: per row a stable sort of the batch by cell gives each
+ * add its rank in its cell. */
+int fb_cms_add_and_estimate(fb_cms* cms, const uint64_t* keys, uint64_t n, uint32_t* out);
 /* Pass 1 of build_cdbg_with_contigs_ext (build.rs:93-116): every canonical
  * k-mer of every read with len >= k, key = KmerInt::to_u64. */
 int fb_cms_add_bank(fb_cms* cms, const fb_bank* bank, uint32_t k);
@@ -130,12 +144,26 @@ void fb_hll_destroy(fb_hll* hll);
 int fb_hll_clear(fb_hll* hll);
 /* HyperLogLog::add_hash for each hash. */
 int fb_hll_add_hashes(fb_hll* hll, const uint64_t* hashes, uint64_t n);
+/* HyperLogLog::add_canonical_hash (sketch.rs:66-70) for each hash:
+ * add_hash(splitmix64(h)). */
+int fb_hll_add_canonical_hashes(fb_hll* hll, const uint64_t* hashes, uint64_t n);
+/* HyperLogLog::merge (sketch.rs:73-80): dst's registers become the
+ * register-wise max of dst and src.  FB_ERR_INVALID when the precisions
+ * differ (the reference asserts equal register counts). */
+int fb_hll_merge(fb_hll* dst, const fb_hll* src);
 /* add_hash(splitmix64(canonical ntHash_k)) over every window of every read
  * with len >= k (HyperLogLog::add_canonical_hash, sketch.rs:68-70). */
 int fb_hll_add_bank(fb_hll* hll, const fb_bank* bank, uint32_t k);
 int fb_hll_registers(fb_hll* hll, uint8_t* out /* 2^p bytes */);
 /* HyperLogLog::estimate (sketch.rs:83-114), generalised to m = 2^p. */
 int fb_hll_estimate(fb_hll* hll, double* out);
+
+/* ---- RollingNtHash (kmer.rs:433-477) ---------------------------------------
+ * out[i] = canonical() of window i of the ASCII sequence (len - k + 1
+ * windows; none when len < k): RollingNtHash::new on the first k bases,
+ * then roll(out_base, in_base) once per window.  Non-ACGT bytes hash as A
+ * (ASCII_TO_2BIT).  On the device, a thread per 64 windows. */
+int fb_nthash_windows(fb_ctx* ctx, const uint8_t* seq, uint64_t len, uint32_t k, uint64_t* out);
 /* The same estimator on a host register array. */
 double fb_hll_estimate_registers(const uint8_t* regs, int p);
 
@@ -526,6 +554,17 @@ uint32_t fb_compute_dynamic_kmers(uint64_t median_len, int64_t target_max_k, uin
  * estimator: N maps to A before hashing, bitseq.rs:67-81). */
 int fb_estimate_genome(fb_ctx* ctx, const fb_arena_view* r1, const fb_arena_view* r2, const fb_count_stats* stats,
                        uint64_t max_pairs, double target_cov, fb_genome_estimate* out);
+/* The pipeline's subsampling policy with auto-subsampling enabled
+ * (main.rs:66-185; host arithmetic only): coverage = total_bases /
+ * genome_size when genome_size > 0, else the estimate's estimated_cov when its
+ * confidence is HIGH (LOW_COVERAGE_WARNING and UNRELIABLE keep every read);
+ * factor = clamp(depth / coverage, 0.01, 1) when depth > 0 and coverage >
+ * threshold_ratio * depth (the hidden --subsample-threshold-ratio, 1.5 by
+ * default), else 1.  The factor feeds fb_ingest_params.subsample_factor.
+ * *reason (may be null): 0 subsampled from the genome size, 1 subsampled
+ * from the estimate, 2 kept: within the trigger, 3 kept: estimate not HIGH. */
+double fb_auto_subsample_factor(const fb_genome_estimate* est, double depth, double threshold_ratio,
+                                uint64_t genome_size, int32_t* reason);
 
 /* ---- Serving: a request loop with continuous batching ----------------------
  * No counterpart in the reference, which assembles one isolate per process

@@ -331,6 +331,10 @@ def cms_width_for_genome_size(g: int) -> int:
     return int(lib().orc_cms_width_for_genome_size(g))
 
 
+def hll_merge(regs: np.ndarray, other: np.ndarray, p: int) -> None:
+    lib().orc_hll_merge(_p(regs, C.c_uint8), _p(other, C.c_uint8), p)
+
+
 def hll_add_hashes(regs: np.ndarray, p: int, hashes) -> None:
     for h in np.asarray(hashes, dtype=np.uint64).tolist():
         lib().orc_hll_add_hash(_p(regs, C.c_uint8), p, h)

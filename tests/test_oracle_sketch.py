@@ -161,3 +161,22 @@ def _sm64_np(z):
 def test_numpy_splitmix_matches(orc):
     xs = np.array([x for x, _ in PYV["splitmix64"]], np.uint64)
     assert _sm64_np(xs).tolist() == [y for _, y in PYV["splitmix64"]]
+
+
+@pytest.mark.parametrize("bits", [4, 16])
+def test_add_and_estimate_is_sequential_add_then_estimate(orc, bits):
+    """add_and_estimate (sketch.rs:248-253) = add, then estimate: against a
+    dictionary of exact per-cell counts updated one key at a time, on a
+    narrow sketch (shared cells) with repeated keys, saturating at 15 / 65535."""
+    width = 1 << 6
+    cap = 15 if bits == 4 else 65535
+    rng = np.random.default_rng(bits)
+    keys = rng.integers(0, 40, size=3000).tolist() + [7] * 200
+    c = orc.Cms(width, bits)
+    counts = {}
+    for key in keys:
+        for cell in P.cms_cells(key, width):
+            counts[cell] = counts.get(cell, 0) + 1
+        want = min(min(counts[cell], cap) for cell in P.cms_cells(key, width))
+        assert c.add_and_estimate(key) == want
+    assert any(v > 15 for v in counts.values())

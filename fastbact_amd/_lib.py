@@ -290,6 +290,7 @@ SIGNATURES = [
     ("fb_hll_add_canonical_hashes", C.c_int, [vp, u64p, C.c_uint64]),
     ("fb_hll_merge", C.c_int, [vp, vp]),
     ("fb_cms_add_and_estimate", C.c_int, [vp, u64p, C.c_uint64, u32p]),
+    ("fb_cms_add_and_estimate_static", C.c_int, [vp, u64p, C.c_uint64, u32p]),
     ("fb_nthash_windows", C.c_int, [vp, u8p, C.c_uint64, C.c_uint32, u64p]),
     ("fb_hll_registers", C.c_int, [vp, u8p]),
     ("fb_hll_estimate", C.c_int, [vp, C.POINTER(C.c_double)]),

@@ -41,8 +41,34 @@ struct fb_ctx {
   std::string err;
   std::vector<PoolBuf> pool;  // free buffers
   hipEvent_t ev[12] = {};
+  unsigned char* scratch = nullptr;  // this device's copy of `scratch` below
 };
 
+// Static device scratch of the entry points that allocate nothing
+// (fb_cms_add_and_estimate_static).  Each device holds its own copy until the
+// process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (static_scratch_of), never
+// through the array's name.  Every call on a device shares that copy.
+constexpr uint64_t kStaticMaxKeys = 1ull << 18;            // keys per call
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;   // 16 MiB: 8 MiB of arrays, 8 MiB of sort space
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char*> g_scratch;  // device -> address of its copy
+
+// The current device's copy of `scratch`, looked up once per device; called
+// with that device current (context creation).
+static hipError_t static_scratch_of(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  if (it == g_scratch.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = g_scratch.emplace(device, static_cast<unsigned char*>(p)).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
 
 static int set_err(fb_ctx* c, int code, const char* fmt, ...) {
   if (c) {
@@ -165,6 +191,11 @@ extern "C" int fb_ctx_create(int device, uint64_t budget, fb_ctx** out) {
       delete c;
       return FB_ERR_HIP;
     }
+  }
+  if (static_scratch_of(device, &c->scratch) != hipSuccess) {
+    (void)hipGetLastError();
+    fb_ctx_destroy(c);
+    return FB_ERR_HIP;
   }
   *out = c;
   return FB_OK;
@@ -542,6 +573,48 @@ extern "C" int fb_cms_add_and_estimate(fb_cms* m, const uint64_t* keys, uint64_t
   }
   HIP_TRY(c, launch_cms_add_keys(m->dev(), d.as<uint64_t>(), n, 1, s));
   HIP_TRY(c, hipMemcpyAsync(out, o.p, n * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return FB_OK;
+}
+
+// fb_cms_add_and_estimate with every buffer of the call in the device's
+// static scratch: keys, estimates, per-row cells and indices, their sorted
+// copies and the sort's temporary space.  No allocation happens.
+extern "C" int fb_cms_add_and_estimate_static(fb_cms* m, const uint64_t* keys, uint64_t n, uint32_t* out) {
+  if (!m || (n && (!keys || !out))) return FB_ERR_INVALID;
+  fb_ctx* c = m->ctx;
+  if (n > kStaticMaxKeys)
+    return set_err(c, FB_ERR_INVALID, "at most %llu keys per call (got %llu)", (unsigned long long)kStaticMaxKeys,
+                   (unsigned long long)n);
+  if (!c->scratch) return set_err(c, FB_ERR_INVALID, "no static scratch on this device");
+  if (int r = ctx_current(c)) return r;
+  if (!n) return FB_OK;
+  if (int r = check_add_budget(c, n)) return r;
+  hipStream_t s = c->stream;
+  // layout: keys | estimates | cells | idx | sorted cells | sorted idx | sort space
+  unsigned char* p = c->scratch;
+  uint64_t* d_keys = reinterpret_cast<uint64_t*>(p);
+  uint32_t* d_out = reinterpret_cast<uint32_t*>(p + kStaticMaxKeys * 8);
+  uint32_t* cells = d_out + kStaticMaxKeys;
+  uint32_t* idx = cells + kStaticMaxKeys;
+  uint32_t* scells = idx + kStaticMaxKeys;
+  uint32_t* sidx = scells + kStaticMaxKeys;
+  unsigned char* tmp = reinterpret_cast<unsigned char*>(sidx + kStaticMaxKeys);
+  const size_t tmp_room = kStaticScratchBytes - (size_t)(tmp - p);
+  const uint32_t end_bit = m->log2w + 2;
+  size_t tb = 0;
+  HIP_TRY(c, sort_u32_pairs(nullptr, nullptr, nullptr, nullptr, n, end_bit, nullptr, &tb, s));
+  if (tb > tmp_room)
+    return set_err(c, FB_ERR_INVALID, "sort space %zu bytes exceeds the static scratch (%zu)", tb, tmp_room);
+  HIP_TRY(c, hipMemcpyAsync(d_keys, keys, n * 8, hipMemcpyHostToDevice, s));
+  for (uint32_t row = 0; row < 4; ++row) {
+    HIP_TRY(c, launch_cms_row_cells(m->dev(), d_keys, n, row, cells, idx, s));
+    size_t t2 = tb;
+    HIP_TRY(c, sort_u32_pairs(cells, scells, idx, sidx, n, end_bit, tmp, &t2, s));
+    HIP_TRY(c, launch_cms_row_estimate(m->counts.as<uint32_t>(), scells, sidx, n, row, m->cap(), d_out, s));
+  }
+  HIP_TRY(c, launch_cms_add_keys(m->dev(), d_keys, n, 1, s));
+  HIP_TRY(c, hipMemcpyAsync(out, d_out, n * 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   return FB_OK;
 }

@@ -130,6 +130,16 @@ int fb_cms_query(fb_cms* cms, const uint64_t* keys, uint64_t n, uint32_t* out);
  * it).  On the device: per row a stable sort of the batch by cell gives each
  * add its rank in its cell. */
 int fb_cms_add_and_estimate(fb_cms* cms, const uint64_t* keys, uint64_t n, uint32_t* out);
+/* fb_cms_add_and_estimate without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid it).
+ * Every buffer of the call (keys, estimates, the per-row cells and indices,
+ * their sorted copies and the sort's space) lives in a static 16 MiB device
+ * array that each GPU holds from its first context until the process ends.
+ * At most 262144 keys per call (FB_ERR_INVALID beyond).  All calls on one
+ * GPU share that array: calls that overlap on one GPU race, and the array
+ * keeps the last call's contents.  Results equal fb_cms_add_and_estimate's. */
+int fb_cms_add_and_estimate_static(fb_cms* cms, const uint64_t* keys, uint64_t n, uint32_t* out);
 /* Pass 1 of build_cdbg_with_contigs_ext (build.rs:93-116): every canonical
  * k-mer of every read with len >= k, key = KmerInt::to_u64. */
 int fb_cms_add_bank(fb_cms* cms, const fb_bank* bank, uint32_t k);

@@ -112,19 +112,20 @@ def copy_calibration(ctx, mib: int) -> None:
 
 
 def atomic_sweep(ctx) -> list:
-    """Random u32 atomicAdd rate (k_atomic_rate) at table sizes from
-    L2-resident (4 MiB) to HBM-resident (1 GiB): the device figures the
-    passes' atomics are compared with."""
+    """Random 4-byte access rate (k_atomic_rate) at table sizes from
+    L2-resident (4 MiB) to HBM-resident (1 GiB): atomicAdd without and with
+    the returned value, and plain loads -- the device figures the passes'
+    atomics and random probes are compared with."""
     import ctypes as C
     out = []
     n_updates = 1 << 28
     for log2n in (20, 22, 24, 26, 28):
-        for ret in (0, 1):
+        for mode, name in ((0, "atomic"), (1, "atomic_ret"), (2, "load"), (3, "store")):
             ms = C.c_float()
-            ctx.check(ctx.lib.fb_diag_atomic_rate(ctx.h, log2n, n_updates, ret, C.byref(ms)), "fb_diag_atomic_rate")
-            out.append({"table_mib": (4 << log2n) >> 20, "with_return": bool(ret), "updates": n_updates,
-                        "ms": round(ms.value, 3), "g_atomics_s": round(n_updates / ms.value / 1e6, 1)})
-    log(f"atomic sweep: {[(o['table_mib'], o['with_return'], o['g_atomics_s']) for o in out]}")
+            ctx.check(ctx.lib.fb_diag_atomic_rate(ctx.h, log2n, n_updates, mode, C.byref(ms)), "fb_diag_atomic_rate")
+            out.append({"table_mib": (4 << log2n) >> 20, "op": name, "accesses": n_updates,
+                        "ms": round(ms.value, 3), "g_per_s": round(n_updates / ms.value / 1e6, 1)})
+    log(f"random access sweep: {[(o['table_mib'], o['op'], o['g_per_s']) for o in out]}")
     return out
 
 
@@ -308,6 +309,66 @@ def bench_batched(ctx, F, shapes, devs, reps: int) -> dict:
             "autotune": tune}
 
 
+def bench_small_mix(ctx, F, n: int, threads: int, reps: int = 2, seed: int = 0xFB07) -> dict:
+    """Where batching pays: n small isolates (genome U[50, 300] kbp, GC
+    U[0.30, 0.68], coverage U[30, 100]x, PE150 0.3% subs; k = 55, each CMS
+    sized by the reference's genome-size rule, HLL p14), which one at a time
+    cannot fill the device; batched in waves against plain per-isolate calls."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.uniform(5e4, 3e5)), float(rng.uniform(0.30, 0.68)), float(rng.uniform(30.0, 100.0)))
+              for _ in range(n)]
+
+    def gen(i):
+        g, gc, cov = shapes[i]
+        return F.synth_isolate(seed=seed * 1000 + i, genome_len=g, gc=gc, coverage=cov, read_len=150,
+                               insert_mean=350.0, insert_sd=35.0, sub_rate=0.003, indel_rate=0.0).bank()
+
+    with cf.ThreadPoolExecutor(max(1, threads)) as ex:
+        banks = list(ex.map(gen, range(n)))
+    devs = [F.DeviceBank(ctx, b) for b in banks]
+    del banks
+    ctx.synchronize()
+    bases = sum(d.bases_for_k(BATCH_K) for d in devs)
+    kw = dict(counter_bits=16, hll_p=BATCH_HLL_P)
+    budget = int(0.6 * ctx.memory()[0])
+    waves, cur, used = [], [], 0
+    for i, d in enumerate(devs):
+        fp = F.batch_footprint(shapes[i][0], d.total_bases(), d.total_reads(), BATCH_K, **kw)
+        if cur and used + fp > budget:
+            waves.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += fp
+    waves.append(cur)
+    batch_ms = []
+    for w in waves:
+        b = F.Batch(ctx, [devs[i] for i in w], [shapes[i][0] for i in w], **kw)
+        b.run(BATCH_K, None)  # warm-up, captures the graphs
+        ms = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            b.run(BATCH_K, None)
+            ms.append((time.perf_counter() - t) * 1e3)
+        batch_ms.append(min(ms))
+        b.close()
+    plain_ms = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        for i, d in enumerate(devs):
+            F.build_solid_handle(ctx, d, BATCH_K, target_solid_cov=None, genome_size=shapes[i][0], **kw).close()
+        ctx.synchronize()
+        plain_ms.append((time.perf_counter() - t) * 1e3)
+    for d in devs:
+        d.close()
+    tb, tp = sum(batch_ms), min(plain_ms)
+    return {"workload": f"{n} isolates: genome U[50,300] kbp, GC U[0.30,0.68], coverage U[30,100]x, PE150 0.3% subs; "
+                        f"k=55; CMS from the genome-size rule (u16) + HLL p14 per isolate",
+            "bases": bases, "waves": len(waves), "batched_ms": round(tb, 1),
+            "batched_gbases_s": round(bases / tb / 1e6, 3), "plain_per_isolate_ms": round(tp, 1),
+            "plain_per_isolate_gbases_s": round(bases / tp / 1e6, 3), "batched_over_plain": round(tp / tb, 3)}
+
+
 def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05) -> dict:
     """Poisson stream of isolate requests (k ladder 21/55/77 plus the genome
     estimate) through the serving loop; latency from submission to the last
@@ -481,6 +542,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-rate", type=float, default=2.0, help="mean request arrivals per second")
     ap.add_argument("--no-next-rows", action="store_true",
                     help="skip compaction and the next rows (a short run of the timed steps alone, for profiling)")
+    ap.add_argument("--no-atomic-sweep", action="store_true",
+                    help="skip the random access / atomic rate sweep (k_atomic_rate)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the steps, copy a buffer of this many MiB device to device once (a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -560,7 +623,7 @@ def main(argv=None) -> int:
         next_rows = bench_next_rows(ctx, dev, F, iso, bank, g21, cpu=not args.no_cpu_baseline)
         log(f"next rows: {next_rows}")
         del g21
-    atomics = atomic_sweep(ctx) if not args.no_next_rows else None
+    atomics = atomic_sweep(ctx) if not args.no_atomic_sweep else None
 
     batched = served = None
     if args.batch_isolates or args.serve_requests:
@@ -574,6 +637,9 @@ def main(argv=None) -> int:
             for d in devs:
                 d.close()
             ctx.trim()  # hand the waves' buffers back to the device
+            batched["small_isolate_mix"] = bench_small_mix(ctx, F, 256, min(16, os.cpu_count() or 1))
+            log(f"small-isolate mix: {batched['small_isolate_mix']}")
+            ctx.trim()
         if args.serve_requests:
             served = bench_serving(F, shapes, banks, args.serve_requests, args.serve_rate)
             log(f"serving: {served['poisson']}")

@@ -150,7 +150,7 @@ hipError_t launch_pack_u8(const uint32_t* regs, uint64_t n, uint8_t* out, hipStr
 hipError_t launch_table_clear(const DevSolidTable& t, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_copy_calibration(const void* src, void* dst, uint64_t n16, hipStream_t s);
-hipError_t launch_atomic_rate(uint32_t* table, uint32_t log2n, uint64_t n_updates, bool ret, uint32_t* sink,
+hipError_t launch_atomic_rate(uint32_t* table, uint32_t log2n, uint64_t n_updates, int mode, uint32_t* sink,
                               hipStream_t s);
 // Pass 2 (build.rs:140-289).
 hipError_t launch_pass2(const Pass2Args& a, hipStream_t s);

@@ -261,7 +261,10 @@ hipError_t launch_copy_calibration(const void* src, void* dst, uint64_t n16, hip
 // It measures the device's atomic rate at a given table size (L2-resident,
 // infinity-cache-resident or HBM-resident), the figure the Count-Min and
 // bin-cursor atomics of the passes are compared with.
-template <bool RET>
+// MODE 0: atomicAdd, 1: atomicAdd with its returned value used, 2: plain
+// 4-byte loads at the same cells (the random-read rate), 3: plain 4-byte
+// stores (the random-write rate).
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_atomic_rate(uint32_t* __restrict__ table, uint32_t log2n,
                                                         uint64_t n_updates, uint64_t seed, uint32_t* __restrict__ sink) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,20 +273,25 @@ __global__ __launch_bounds__(kBlock) void k_atomic_rate(uint32_t* __restrict__ t
   uint32_t acc = 0;
   for (; i < n_updates; i += stride) {
     const uint64_t c = splitmix64(i ^ seed) & mask;
-    if (RET) acc += atomicAdd(table + c, 1u);
+    if (MODE == 1) acc += atomicAdd(table + c, 1u);
+    else if (MODE == 2) acc += table[c];
+    else if (MODE == 3) table[c] = (uint32_t)i;
     else atomicAdd(table + c, 1u);
   }
-  if (RET && acc == 0xFFFFFFFFu) sink[0] = acc;  // keeps the returned values live
+  if ((MODE == 1 || MODE == 2) && acc == 0xFFFFFFFFu) sink[0] = acc;  // keeps the returned values live
 }
-hipError_t launch_atomic_rate(uint32_t* table, uint32_t log2n, uint64_t n_updates, bool ret, uint32_t* sink,
+hipError_t launch_atomic_rate(uint32_t* table, uint32_t log2n, uint64_t n_updates, int mode, uint32_t* sink,
                               hipStream_t s) {
   const uint64_t g = 256u * 64u;  // 64 blocks of 256 per CU: enough updates in flight
-  if (ret)
-    hipLaunchKernelGGL(k_atomic_rate<true>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates,
-                       0x6a09e667f3bcc909ull, sink);
+  const uint64_t seed = 0x6a09e667f3bcc909ull;
+  if (mode == 1)
+    hipLaunchKernelGGL(k_atomic_rate<1>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates, seed, sink);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_atomic_rate<2>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates, seed, sink);
+  else if (mode == 3)
+    hipLaunchKernelGGL(k_atomic_rate<3>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates, seed, sink);
   else
-    hipLaunchKernelGGL(k_atomic_rate<false>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates,
-                       0x6a09e667f3bcc909ull, sink);
+    hipLaunchKernelGGL(k_atomic_rate<0>, dim3((unsigned)g), dim3(kBlock), 0, s, table, log2n, n_updates, seed, sink);
   return hipGetLastError();
 }
 

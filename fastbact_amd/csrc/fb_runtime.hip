@@ -268,9 +268,9 @@ extern "C" int fb_diag_copy(fb_ctx* c, uint64_t bytes) {
   return FB_OK;
 }
 
-extern "C" int fb_diag_atomic_rate(fb_ctx* c, uint32_t log2_counters, uint64_t n_updates, int with_return,
-                                   float* ms) {
-  if (!c || !ms || log2_counters < 10 || log2_counters > 30) return set_err(c, FB_ERR_INVALID, "bad argument");
+extern "C" int fb_diag_atomic_rate(fb_ctx* c, uint32_t log2_counters, uint64_t n_updates, int mode, float* ms) {
+  if (!c || !ms || log2_counters < 10 || log2_counters > 30 || mode < 0 || mode > 3)
+    return set_err(c, FB_ERR_INVALID, "bad argument");
   if (int r = ctx_current(c)) return r;
   const uint64_t n = 1ull << log2_counters;
   DBuf table, sink;
@@ -279,11 +279,9 @@ extern "C" int fb_diag_atomic_rate(fb_ctx* c, uint32_t log2_counters, uint64_t n
   hipStream_t s = c->stream;
   HIP_TRY(c, launch_fill_u32(table.as<uint32_t>(), n, 0, s));
   // one launch to bring the table into whatever cache level holds it, then the timed one
-  HIP_TRY(c, launch_atomic_rate(table.as<uint32_t>(), log2_counters, n_updates, with_return != 0,
-                                sink.as<uint32_t>(), s));
+  HIP_TRY(c, launch_atomic_rate(table.as<uint32_t>(), log2_counters, n_updates, mode, sink.as<uint32_t>(), s));
   HIP_TRY(c, hipEventRecord(c->ev[0], s));
-  HIP_TRY(c, launch_atomic_rate(table.as<uint32_t>(), log2_counters, n_updates, with_return != 0,
-                                sink.as<uint32_t>(), s));
+  HIP_TRY(c, launch_atomic_rate(table.as<uint32_t>(), log2_counters, n_updates, mode, sink.as<uint32_t>(), s));
   HIP_TRY(c, hipEventRecord(c->ev[1], s));
   HIP_TRY(c, hipEventSynchronize(c->ev[1]));
   HIP_TRY(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));

@@ -80,12 +80,12 @@ int fb_ctx_trim(fb_ctx* ctx);
  * up to 16), for scaling hardware traffic counters against a known size. */
 int fb_diag_copy(fb_ctx* ctx, uint64_t bytes);
 /* Diagnostics, no counterpart in the reference: the device's rate of random
- * 32-bit atomicAdd over a table of 2^log2_counters u32 (10..30), n_updates
- * updates at splitmix64-spread cells, with (with_return 1) or without the
- * returned value; *ms = the time of one launch (kernel k_atomic_rate) after
- * a warm-up launch.  Small tables stay in L2, mid-size ones in the infinity
- * cache, large ones reach HBM. */
-int fb_diag_atomic_rate(fb_ctx* ctx, uint32_t log2_counters, uint64_t n_updates, int with_return, float* ms);
+ * 4-byte accesses over a table of 2^log2_counters u32 (10..30), n_updates
+ * accesses at splitmix64-spread cells.  mode 0: atomicAdd, 1: atomicAdd with
+ * its returned value used, 2: plain loads, 3: plain stores.  *ms = the time of one launch
+ * (kernel k_atomic_rate) after a warm-up launch.  Small tables stay in L2,
+ * mid-size ones in the infinity cache, large ones reach HBM. */
+int fb_diag_atomic_rate(fb_ctx* ctx, uint32_t log2_counters, uint64_t n_updates, int mode, float* ms);
 /* Free and total device memory of `device` in bytes. */
 int fb_device_memory(int device, uint64_t* free_bytes, uint64_t* total_bytes);
 /* Library version string. */

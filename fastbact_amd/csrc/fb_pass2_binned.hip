@@ -11,9 +11,10 @@
 //     launch; the four rows together do not fit one L2, and probing them all
 //     per window is bound by the misses.  The launch of the last row
 //     (k_p2_count, with the block layout of the scatter kernel) also counts
-//     the records per (bin, slice); a one-block scan (k_p2_scan) turns the
-//     counts into exact region starts, so no region can overflow whatever the
-//     skew of the k-mer spectrum.
+//     the records per (bin, slice) in the same walk (a window's record is
+//     settled one window behind its gate probe); a one-block scan
+//     (k_p2_scan) turns the counts into exact region starts, so no region
+//     can overflow whatever the skew of the k-mer spectrum.
 //   Phase A (k_p2_scatter): block b writes into the region of (bin, slice),
 //     slice = b mod 8.  Workgroups are handed to the 8 XCDs in turn, so each
 //     region is filled from one XCD and its partly written lines stay in one
@@ -26,9 +27,9 @@
 //   Phase B (k_p2_bins): one workgroup per bin builds the bin's table in LDS
 //     (write-once keys claimed word by word with compare-and-swap, as the
 //     global table does), then emits the solid k-mers (pass 3) directly.  A
-//     key that finds the LDS table full goes to the global table; each key
-//     is in exactly one of the two, so pass 3 over the global table appends
-//     the rest.
+//     key that finds no free LDS entry near its home slot goes to the global
+//     table; each key is in exactly one of the two, so pass 3 over the global
+//     table appends the rest.
 #include "fb_device.hpp"
 #include "fb_internal.hpp"
 #include "fb_table.hpp"
@@ -254,6 +255,56 @@ __device__ __forceinline__ bool p2_in_round(const P2BinArgs& a, uint64_t fold, u
   return a.log2_rounds == 0 || (uint32_t)(slot_hash2(fold) >> (64u - a.log2_rounds)) == round;
 }
 
+// Row 3 of the gate and the record count of round 0 in one walk over the
+// windows p0..p1 of read r (gate bits m of rows 0..2): window i's row-3 probe
+// settles its gate bit, and with it the pair (i-1, i), so window i-1's
+// record (it has one when either of its pairs joins two gate-passing
+// windows, as p2_walk decides) is counted one window behind.  Returns the
+// final mask.
+template <int L>
+__device__ __forceinline__ uint64_t gate3_count(const P2BinArgs& a, uint64_t r, uint32_t p0, uint32_t p1, uint64_t m,
+                                                uint32_t* hist) {
+  const uint32_t k = a.k;
+  const uint64_t* __restrict__ data = a.reads.data;
+  const uint64_t g = a.reads.offs[r] + p0;
+  Kmer<L> win = window_bits<L>(data, g, k);
+  Kmer<L> fwd = fwd_from_window<L>(win, k);
+  Kmer<L> rev = rc_from_window<L>(win, k);
+  BaseStream in;
+  in.init(data, g + k);
+  const uint64_t wmask = (1ull << a.log2w) - 1ull;
+  const uint64_t rbase = 3ull << a.log2w;
+  const uint64_t seed = cms_seed(3);
+  const uint32_t last = p1 - p0;
+  uint64_t prev_fold = 0;
+  bool prev_solid = false, prev_left = false;
+  auto count = [&](uint64_t fold) {
+    if (p2_in_round(a, fold, 0)) atomicAdd(hist + p2_bin_of(a, fold), 1u);
+  };
+  for (uint32_t i = 0;; ++i) {
+    const uint64_t fold = kmer_lt<L>(rev, fwd) ? kmer_fold<L>(rev) : kmer_fold<L>(fwd);
+    bool solid = (m >> i) & 1ull;
+    if (solid) {
+      const uint64_t c = rbase | (splitmix64(fold ^ seed) & wmask);
+      if (!((a.solid_bits[c >> 5] >> (c & 31u)) & 1u)) {
+        m &= ~(1ull << i);
+        solid = false;
+      }
+    }
+    if (i) {
+      const bool right = prev_solid && solid;  // pair (i-1, i)
+      if (prev_left || right) count(prev_fold);
+      prev_left = right;
+    }
+    prev_solid = solid;
+    prev_fold = fold;
+    if (i == last) break;
+    kmer_roll<L>(fwd, rev, in.next(), k);
+  }
+  if (prev_left) count(prev_fold);  // window p1: its left pair only
+  return m;
+}
+
 // Round 0 applies row 3 of the gate and stores the final mask; every round
 // then counts its records of the final mask per bin for slice sl.  Block blk
 // covers the thread shares of scatter block blk:
@@ -278,8 +329,9 @@ __device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, 
     } else {
       m = a.gate_masks[gid];
       if (round == 0) {
-        if (m) m = gate_row<L>(a, r, p0, p1, m, 3);
+        if (m) m = gate3_count<L>(a, r, p0, p1, m, hist);
         a.gate_masks[gid] = m;
+        continue;
       }
     }
     if (m)
@@ -461,11 +513,18 @@ __device__ __forceinline__ uint32_t lds_slot(const uint64_t (&w)[KW], uint32_t c
   return (uint32_t)(((h & 0xffffffffull) * (uint64_t)cap) >> 32);
 }
 
-// Finds or claims the LDS entry of key; -1 when the table is full.
+// Finds or claims the LDS entry of key within kLdsMaxProbe slots of its
+// home slot; -1 when all of them hold other keys.  Entries are written once,
+// so a key that finds its probe window full finds it full at every later
+// lookup too and lives in the global table for the whole pass, and a key
+// claimed in the window is found there by every later lookup.  The bound
+// keeps a crowded bin from scanning its whole table for every new key.
+constexpr uint32_t kLdsMaxProbe = 128;
 template <int KW>
 __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, const uint64_t (&w)[KW]) {
   uint32_t s = lds_slot<KW>(w, cap);
-  for (uint32_t probe = 0; probe < cap; ++probe) {
+  const uint32_t probes = cap < kLdsMaxProbe ? cap : kLdsMaxProbe;
+  for (uint32_t probe = 0; probe < probes; ++probe) {
     bool match = true;
 #pragma unroll
     for (int j = 0; j < KW; ++j) {
@@ -556,7 +615,7 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
         }
       }
     } else {
-      // LDS table full: this key lives in the global table for the whole pass
+      // no LDS entry: this key lives in the global table for the whole pass
       uint32_t* c = table_find_global<KW>(a.table, w, slot_hash_words<KW>(w));
       if (c) {
         atomicAdd(c, cw * ((uint32_t)left + (uint32_t)right));

@@ -2205,12 +2205,125 @@ extern "C" int fb_batch_routes(const fb_batch* b, uint32_t* out) {
 // freeing its memory for the queue.  A request's sketches and buffers stay on
 // the device across its passes.
 
+// Page-locked host blocks for served solid sets, kept and reused across
+// requests.  A download into page-locked memory runs at the host link's
+// rate, where one into pageable memory goes through the driver's staging
+// copies (and a std::vector would first zero what it is about to receive).
+// A block returns to the pool when its request is released; the pool keeps
+// at most kCacheBytes of free blocks and frees the rest.
+class PinnedPool {
+ public:
+  static constexpr size_t kGrain = 4u << 20;
+  static constexpr size_t kCacheBytes = 8ull << 30;
+  ~PinnedPool() {
+    for (auto& kv : free_) (void)hipHostFree(kv.second);
+  }
+  // A block of at least bytes (nullptr when none can be pinned); *got = its size.
+  void* get(size_t bytes, size_t* got) {
+    bytes = (bytes + kGrain - 1) / kGrain * kGrain;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = free_.lower_bound(bytes);
+      if (it != free_.end() && it->first <= bytes + bytes / 2) {
+        *got = it->first;
+        void* p = it->second;
+        cached_ -= it->first;
+        free_.erase(it);
+        return p;
+      }
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    *got = bytes;
+    return p;
+  }
+  void put(void* p, size_t bytes) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (cached_ + bytes <= kCacheBytes) {
+        free_.emplace(bytes, p);
+        cached_ += bytes;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
+  }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  size_t cached_ = 0;
+};
+
+// One pass's solid set on the host: n entries of nref key words, cov and
+// mask, in one pinned block (keys, then cov, then mask) or, when no block
+// could be pinned, in vectors.
 struct ServePass {
-  std::vector<uint64_t> keys;
-  std::vector<uint16_t> cov;
-  std::vector<uint8_t> mask;
+  PinnedPool* pool = nullptr;
+  void* blk = nullptr;
+  size_t blk_bytes = 0;
+  uint64_t n = 0;
+  uint32_t nref = 1;
+  std::vector<uint64_t> keys_v;
+  std::vector<uint16_t> cov_v;
+  std::vector<uint8_t> mask_v;
   std::vector<uint8_t> cms, hll;  // with keep_sketches
   uint32_t route = 0;
+
+  ServePass() = default;
+  ServePass(const ServePass&) = delete;
+  ServePass& operator=(const ServePass&) = delete;
+  ServePass(ServePass&& o) noexcept { *this = std::move(o); }
+  ServePass& operator=(ServePass&& o) noexcept {
+    if (this != &o) {
+      release();
+      pool = o.pool;
+      blk = o.blk;
+      blk_bytes = o.blk_bytes;
+      n = o.n;
+      nref = o.nref;
+      keys_v = std::move(o.keys_v);
+      cov_v = std::move(o.cov_v);
+      mask_v = std::move(o.mask_v);
+      cms = std::move(o.cms);
+      hll = std::move(o.hll);
+      route = o.route;
+      o.blk = nullptr;
+      o.blk_bytes = 0;
+    }
+    return *this;
+  }
+  ~ServePass() { release(); }
+  void release() {
+    if (blk) pool->put(blk, blk_bytes);
+    blk = nullptr;
+    blk_bytes = 0;
+  }
+  // Room for n entries of nref words.
+  void alloc(PinnedPool* pl, uint64_t entries, uint32_t words) {
+    n = entries;
+    nref = words;
+    const size_t kb = (size_t)n * nref * 8, bytes = kb + (size_t)n * 3;
+    if (pl && bytes) blk = pl->get(bytes, &blk_bytes);
+    if (blk) {
+      pool = pl;
+    } else {
+      keys_v.resize((size_t)n * nref);
+      cov_v.resize(n);
+      mask_v.resize(n);
+    }
+  }
+  uint64_t* keys() { return blk ? static_cast<uint64_t*>(blk) : keys_v.data(); }
+  uint16_t* cov() {
+    return blk ? reinterpret_cast<uint16_t*>(static_cast<char*>(blk) + (size_t)n * nref * 8) : cov_v.data();
+  }
+  uint8_t* mask() {
+    return blk ? reinterpret_cast<uint8_t*>(static_cast<char*>(blk) + (size_t)n * nref * 8 + (size_t)n * 2)
+               : mask_v.data();
+  }
 };
 
 struct ServeReq {
@@ -2238,6 +2351,7 @@ struct ServeReq {
 };
 
 struct fb_server {
+  PinnedPool pinned;  // declared first: destroyed after the requests that hold its blocks
   fb_ctx* ctx = nullptr;
   fb_serve_params prm{};
   fb_build_params bprm{};
@@ -2360,12 +2474,9 @@ static int serve_pass(fb_server* sv, uint32_t j, std::vector<ServeReq*>& g) {
   for (size_t i = 0; i < g.size() && !rc; ++i) {
     const BatchIso& it = *b->iso[i];
     ServePass p;
-    const uint64_t n = it.n_out;
-    p.keys.resize(n * b->nref);
-    p.cov.resize(n);
-    p.mask.resize(n);
+    p.alloc(&sv->pinned, it.n_out, b->nref);
     p.route = it.route;
-    rc = fb_batch_download(b, (uint32_t)i, p.keys.data(), p.cov.data(), p.mask.data());
+    rc = fb_batch_download(b, (uint32_t)i, p.keys(), p.cov(), p.mask());
     if (!rc && sv->prm.keep_sketches) {
       p.cms.resize(fb_cms_table_bytes(&it.cms));
       rc = cms_download(c, it.cms.dev(), it.cms.width, it.cms.bits, p.cms.data(), p.cms.size());
@@ -2610,7 +2721,7 @@ extern "C" int fb_server_wait(fb_server* sv, uint64_t ticket, double timeout_s, 
   out->has_estimate = r->has_est ? 1u : 0u;
   out->estimate = r->est;
   for (size_t j = 0; j < r->passes.size() && j < 4; ++j) {
-    out->n_solid[j] = r->passes[j].cov.size();
+    out->n_solid[j] = r->passes[j].n;
     out->route[j] = r->passes[j].route;
   }
   out->bases = r->bases;
@@ -2633,10 +2744,10 @@ extern "C" int fb_server_download(fb_server* sv, uint64_t ticket, uint32_t pass,
   std::lock_guard<std::mutex> g(sv->mu);
   ServeReq* r = serve_done_req(sv, ticket, pass);
   if (!r) return FB_ERR_INVALID;
-  const ServePass& p = r->passes[pass];
-  if (keys) memcpy(keys, p.keys.data(), p.keys.size() * 8);
-  if (cov) memcpy(cov, p.cov.data(), p.cov.size() * 2);
-  if (mask) memcpy(mask, p.mask.data(), p.mask.size());
+  ServePass& p = r->passes[pass];
+  if (keys) memcpy(keys, p.keys(), (size_t)p.n * p.nref * 8);
+  if (cov) memcpy(cov, p.cov(), (size_t)p.n * 2);
+  if (mask) memcpy(mask, p.mask(), (size_t)p.n);
   return FB_OK;
 }
 

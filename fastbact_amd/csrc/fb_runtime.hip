@@ -108,11 +108,18 @@ static int pool_get(fb_ctx* c, size_t bytes, void** out) {
     c->pool.erase(c->pool.begin() + best);
     return FB_OK;
   }
-  // free pooled buffers first if the budget would be exceeded
-  while (c->budget && c->held + bytes > c->budget && !c->pool.empty()) {
-    (void)hipFree(c->pool.back().p);
-    c->held -= c->pool.back().bytes;
-    c->pool.pop_back();
+  // free pooled buffers first, the longest-unused first, if the budget would
+  // be exceeded
+  size_t drop = 0;
+  uint64_t held = c->held;
+  while (c->budget && held + bytes > c->budget && drop < c->pool.size()) {
+    (void)hipFree(c->pool[drop].p);
+    held -= c->pool[drop].bytes;
+    ++drop;
+  }
+  if (drop) {
+    c->pool.erase(c->pool.begin(), c->pool.begin() + drop);
+    c->held = held;
   }
   if (c->budget && c->held + bytes > c->budget)
     return set_err(c, FB_ERR_OOM, "HBM budget exceeded: need %zu bytes, %llu of %llu held", bytes,
@@ -2603,15 +2610,18 @@ extern "C" int fb_server_create(int device, const fb_serve_params* p, fb_server*
   sv->bprm.cms_width = p->cms_width;
   sv->bprm.counter_bits = p->counter_bits;
   sv->bprm.hll_p = p->hll_p;
-  uint64_t budget = p->hbm_budget;
-  if (!budget) {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-      fb_ctx_destroy(sv->ctx);
-      return FB_ERR_HIP;
-    }
-    budget = (uint64_t)((double)fr * 0.8);
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    fb_ctx_destroy(sv->ctx);
+    return FB_ERR_HIP;
   }
+  // The context caches the buffers of finished requests for the next ones.
+  // Capped below the device, it frees cached buffers as it needs room;
+  // uncapped, it filled the device until an allocation failed and then freed
+  // its whole cache at once (seconds of hipFree and hipMalloc in one pass,
+  // the p99 of the served stream).
+  if (!p->ctx_budget) sv->ctx->budget = (uint64_t)((double)fr * 0.95);
+  const uint64_t budget = p->hbm_budget ? p->hbm_budget : (uint64_t)((double)fr * 0.8);
   sv->st.budget = budget;
   sv->adm = std::make_unique<fb::Admission>(budget, p->max_inflight ? p->max_inflight : 64);
   sv->t0 = std::chrono::steady_clock::now();

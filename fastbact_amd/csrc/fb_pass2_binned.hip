@@ -503,14 +503,20 @@ __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_m(const P2BinArg
   p2_scatter_body<L, KW>(args[iso], blockIdx.x - blk[iso], sh, round);
 }
 
-// LDS hash of a packed key (placement inside a bin; independent of the bin bits).
+// LDS hash of a packed key: its home slot inside the bin's table.  It sets
+// only where a key is placed (probe lengths), never a result, so one
+// multiply per key word and a fold of the high half suffice; the odd
+// multipliers differ per word, and the bin was chosen by an unrelated hash
+// (slot_hash2 of the k-mer's fold), so the keys of a bin spread evenly.
 template <int KW>
 __device__ __forceinline__ uint32_t lds_slot(const uint64_t (&w)[KW], uint32_t cap) {
-  uint64_t h = w[0];
+  constexpr uint64_t kMul[5] = {0x9e3779b97f4a7c15ull, 0xc2b2ae3d27d4eb4full, 0x165667b19e3779f9ull,
+                                0xd6e8feb86659fd93ull, 0xff51afd7ed558ccdull};
+  uint64_t h = 0;
 #pragma unroll
-  for (int j = 1; j < KW; ++j) h = splitmix64(h) ^ w[j];
-  h = splitmix64(h ^ 0x51ed270b27a3d9c5ull);
-  return (uint32_t)(((h & 0xffffffffull) * (uint64_t)cap) >> 32);
+  for (int j = 0; j < KW; ++j) h ^= (w[j] ^ (w[j] >> 31)) * kMul[j];
+  const uint32_t x = (uint32_t)(h >> 32) ^ (uint32_t)h;
+  return (uint32_t)(((uint64_t)x * (uint64_t)cap) >> 32);
 }
 
 // Finds or claims the LDS entry of key within kLdsMaxProbe slots of its

@@ -422,7 +422,7 @@ def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05)
 
 def bench_compaction(ctx, dev, F, cpu: bool) -> dict:
     """Unitig compaction (build.rs:330-597) of the first pass's solid set:
-    device walks, sequence download and host edges, with the oracle's
+    device walks, downloads and device overlap edges, with the oracle's
     single-threaded compaction of the same set timed beside it."""
     k, tgt = PASSES[0]
     h = F.build_solid_handle(ctx, dev, k, target_solid_cov=tgt, genome_size=GENOME["genome_len"], cms_width=WIDTH,
@@ -439,7 +439,7 @@ def bench_compaction(ctx, dev, F, cpu: bool) -> dict:
     out = {"k": k, "solid_kmers": n_solid, "unitigs": len(g["seqs"]), "circular": int(g["circular"].sum()),
            "bases": int(sum(len(x) for x in g["seqs"])), "edges": int(g["src"].size),
            "device_walk_ms": round(g["timings_ms"][0], 3), "download_ms": round(g["timings_ms"][1], 3),
-           "host_edges_ms": round(g["timings_ms"][2], 3), "wall_ms": round(wall, 3),
+           "device_edges_ms": round(g["timings_ms"][2], 3), "wall_ms": round(wall, 3),
            "solid_kmers_per_s_device": round(n_solid / max(g["timings_ms"][0], 1e-6) * 1e3, 1)}
     if s is not None:
         from oracle import oracle as O

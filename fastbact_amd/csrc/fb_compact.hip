@@ -581,6 +581,149 @@ hipError_t launch_cg_flip(uint8_t* seq, const uint64_t* off, const uint32_t* len
   return hipGetLastError();
 }
 
+// ---- overlap edges (build.rs:552-594) -------------------------------------
+// Strand d = 2u + o of unitig u (o = 1: its reverse complement).  The
+// ov-prefix of strand 2u is u's first ov bases, of 2u + 1 the reverse
+// complement of u's last ov bases.  End e = 2u + side: side 0 is u's last ov
+// bases, side 1 the reverse complement of its first ov bases.  For ov = k,
+// then k - 1, end (u, side) meets every strand whose ov-prefix equals it, in
+// ascending strand order (the order of build.rs's sorted prefix list for
+// equal prefixes); a strand of u itself only when u is circular.  A
+// circular unitig's forward end starts with a self edge of overlap k - 1
+// (support 10), and its own k - 1 match adds 10 to that edge instead of a
+// second one (UnitigNode::add_edge).  Prefixes are found by a 64-bit hash of
+// their bases, sorted with the strand ids, and confirmed base by base.
+
+__device__ __forceinline__ uint32_t cg_code(uint8_t c) { return c == 'A' ? 0u : c == 'C' ? 1u : c == 'G' ? 2u : 3u; }
+
+// Base j of strand d's ov-prefix.
+__device__ __forceinline__ uint32_t cg_pre_base(const CgEdgeArgs& a, uint32_t d, uint32_t j) {
+  const uint32_t u = d >> 1, L = a.len[u];
+  const uint8_t* s = a.seq + a.off[u];
+  return (d & 1u) ? 3u - cg_code(s[L - 1u - j]) : cg_code(s[j]);
+}
+// Base j of end e's ov bases.
+__device__ __forceinline__ uint32_t cg_end_base(const CgEdgeArgs& a, uint32_t e, uint32_t j) {
+  const uint32_t u = e >> 1, L = a.len[u];
+  const uint8_t* s = a.seq + a.off[u];
+  return (e & 1u) ? 3u - cg_code(s[a.ov - 1u - j]) : cg_code(s[L - a.ov + j]);
+}
+template <class B>
+__device__ __forceinline__ uint64_t cg_hash(uint32_t ov, B&& base) {
+  uint64_t h = 0x6a09e667f3bcc909ull, w = 0;
+  for (uint32_t j = 0; j < ov; ++j) {
+    w = (w << 2) | base(j);
+    if ((j & 31u) == 31u || j + 1u == ov) {
+      h = splitmix64(h ^ w);
+      w = 0;
+    }
+  }
+  return h;
+}
+
+__global__ __launch_bounds__(kCgBlock) void k_cg_prefix(CgEdgeArgs a) {
+  const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= 2ull * a.U) return;
+  const uint32_t dd = (uint32_t)d;
+  a.hkey[d] = a.len[dd >> 1] >= a.ov ? cg_hash(a.ov, [&](uint32_t j) { return cg_pre_base(a, dd, j); }) : ~0ull;
+  a.hval[d] = dd;
+}
+
+// Visits the strands whose ov-prefix equals end e's, ascending: f(d).
+template <class F>
+__device__ __forceinline__ void cg_matches(const CgEdgeArgs& a, uint32_t e, F&& f) {
+  const uint32_t u = e >> 1;
+  if (a.len[u] < a.ov) return;
+  const uint64_t h = cg_hash(a.ov, [&](uint32_t j) { return cg_end_base(a, e, j); });
+  uint32_t lo = 0, hi = 2u * a.U;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.skey[mid] < h) lo = mid + 1;
+    else hi = mid;
+  }
+  for (uint32_t i = lo; i < 2u * a.U && a.skey[i] == h; ++i) {
+    const uint32_t d = a.sval[i];
+    if (a.len[d >> 1] < a.ov) continue;
+    if ((d >> 1) == u && !a.circ[u]) continue;
+    bool eq = true;
+    for (uint32_t j = 0; j < a.ov && eq; ++j) eq = cg_pre_base(a, d, j) == cg_end_base(a, e, j);
+    if (eq) f(d);
+  }
+}
+
+// The self edge of a circular unitig's forward end absorbs its k - 1 match.
+__device__ __forceinline__ bool cg_merged(const CgEdgeArgs& a, uint32_t e, uint32_t d) {
+  return a.merge && (e & 1u) == 0u && a.circ[e >> 1] && d == e;
+}
+
+// cnt[e] += the end's matches of this pass (the merged one not counted; it
+// sets bit 31 instead).  With first, cnt[e] starts at the self edge.
+__global__ __launch_bounds__(kCgBlock) void k_cg_end_count(CgEdgeArgs a, uint32_t first) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2ull * a.U) return;
+  const uint32_t ee = (uint32_t)e;
+  uint32_t c = first ? (((ee & 1u) == 0u && a.circ[ee >> 1]) ? 1u : 0u) : a.cnt[e];
+  cg_matches(a, ee, [&](uint32_t d) {
+    if (cg_merged(a, ee, d)) c |= 0x80000000u;
+    else ++c;
+  });
+  a.cnt[e] = c;
+}
+
+// Writes end e's edges from base[e]: its self edge in the first pass, then
+// this pass's matches after the first pass's (pos[e] counts written edges).
+__global__ __launch_bounds__(kCgBlock) void k_cg_end_write(CgEdgeArgs a, uint32_t first) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 2ull * a.U) return;
+  const uint32_t ee = (uint32_t)e;
+  uint32_t p = first ? 0u : a.pos[e];
+  const uint64_t b = a.base[e];
+  if (first && (ee & 1u) == 0u && a.circ[ee >> 1]) {
+    a.e_src[b] = ee;
+    a.e_dst[b] = ee;
+    a.e_ov[b] = (uint8_t)(a.k - 1u);
+    a.e_sup[b] = (a.cnt[e] & 0x80000000u) ? 20 : 10;
+    p = 1;
+  }
+  cg_matches(a, ee, [&](uint32_t d) {
+    if (cg_merged(a, ee, d)) return;
+    a.e_src[b + p] = ee;
+    a.e_dst[b + p] = d;
+    a.e_ov[b + p] = (uint8_t)a.ov;
+    a.e_sup[b + p] = 10;
+    ++p;
+  });
+  a.pos[e] = p;
+}
+
+// tot[e] = end e's edge count (cnt without its flag).
+__global__ __launch_bounds__(kCgBlock) void k_cg_end_total(const uint32_t* cnt, uint32_t* tot, uint32_t n) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) tot[e] = cnt[e] & 0x7FFFFFFFu;
+}
+
+hipError_t launch_cg_prefix(const CgEdgeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_prefix, dim3(cg_grid(2ull * a.U)), dim3(kCgBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_cg_end_count(const CgEdgeArgs& a, bool first, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_end_count, dim3(cg_grid(2ull * a.U)), dim3(kCgBlock), 0, s, a, first ? 1u : 0u);
+  return hipGetLastError();
+}
+hipError_t launch_cg_end_total(const uint32_t* cnt, uint32_t* tot, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_end_total, dim3(cg_grid(n)), dim3(kCgBlock), 0, s, cnt, tot, n);
+  return hipGetLastError();
+}
+hipError_t launch_cg_end_write(const CgEdgeArgs& a, bool first, hipStream_t s) {
+  hipLaunchKernelGGL(k_cg_end_write, dim3(cg_grid(2ull * a.U)), dim3(kCgBlock), 0, s, a, first ? 1u : 0u);
+  return hipGetLastError();
+}
+// Stable radix sort of (u64 key, u32 value) pairs; tmp == nullptr sizes *tmp_bytes.
+hipError_t cg_sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                             void* tmp, size_t* tmp_bytes, hipStream_t s) {
+  return hipcub::DeviceRadixSort::SortPairs(tmp, *tmp_bytes, kin, kout, vin, vout, (int)n, 0, 64, s);
+}
+
 // Exclusive prefix sums (device-wide).
 hipError_t cg_exclusive_sum_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* tmp, size_t* tmp_bytes,
                                 hipStream_t s) {

@@ -331,6 +331,35 @@ hipError_t launch_cg_cycle_ranks(const CgArgs& a, uint32_t s, uint32_t* out_s, h
 hipError_t launch_cg_cycle_flags(const CgArgs& a, uint32_t* flag, hipStream_t s);
 hipError_t launch_cg_table(const CgArgs& a, hipStream_t s);
 hipError_t launch_cg_write(const CgArgs& a, uint32_t sp, uint32_t sc, hipStream_t s);
+// Overlap edges of the compacted unitigs (fb_compact.hip): per pass (ov = k,
+// then k - 1) every strand's ov-prefix hash (k_cg_prefix), sorted with the
+// strand ids (skey/sval), then per end e = 2u + side its edge count and,
+// after a scan into base, its edges.
+struct CgEdgeArgs {
+  const uint8_t* seq = nullptr;
+  const uint64_t* off = nullptr;
+  const uint32_t* len = nullptr;
+  const uint8_t* circ = nullptr;
+  uint32_t U = 0, k = 0, ov = 0;
+  uint32_t merge = 0;  // this pass is ov = k - 1: circular self edges absorb their match
+  uint64_t* hkey = nullptr;
+  uint32_t* hval = nullptr;
+  const uint64_t* skey = nullptr;
+  const uint32_t* sval = nullptr;
+  uint32_t* cnt = nullptr;   // per end: edges (bit 31: the self edge absorbed a match)
+  uint32_t* pos = nullptr;   // per end: edges written so far
+  const uint32_t* base = nullptr;
+  uint32_t* e_src = nullptr;
+  uint32_t* e_dst = nullptr;
+  uint8_t* e_ov = nullptr;
+  uint16_t* e_sup = nullptr;
+};
+hipError_t launch_cg_prefix(const CgEdgeArgs& a, hipStream_t s);
+hipError_t launch_cg_end_count(const CgEdgeArgs& a, bool first, hipStream_t s);
+hipError_t launch_cg_end_total(const uint32_t* cnt, uint32_t* tot, uint32_t n, hipStream_t s);
+hipError_t launch_cg_end_write(const CgEdgeArgs& a, bool first, hipStream_t s);
+hipError_t cg_sort_pairs_u64(const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, uint64_t n,
+                             void* tmp, size_t* tmp_bytes, hipStream_t s);
 hipError_t launch_cg_flip(uint8_t* seq, const uint64_t* off, const uint32_t* len, uint32_t n_lin, uint8_t* flip,
                           uint64_t lin_bases, hipStream_t s);
 // filter_active_reads (local_multik.rs:259-333), fb_filter.hip.

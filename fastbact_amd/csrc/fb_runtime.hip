@@ -2914,9 +2914,8 @@ extern "C" void fb_autotune_clear(void) {
 
 
 // ---------------------------------------------------------------------------
-// Unitig compaction (build.rs:330-597) of a sorted solid set: walks on the
-// device (fb_compact.hip), overlap edges (build.rs:552-594) on the host from
-// the unitig sequences.
+// Unitig compaction (build.rs:330-597) of a sorted solid set: walks and
+// overlap edges (build.rs:552-594) on the device (fb_compact.hip).
 
 struct fb_graph {
   fb_ctx* ctx = nullptr;
@@ -2933,91 +2932,99 @@ struct fb_graph {
   // per node: flags (bit 0 is_deleted, bit 1 is_circular), edges_fwd.len(), edges_rev.len()
   std::vector<uint8_t> flags;
   std::vector<uint32_t> nf, nr;
-  float ms[3] = {0, 0, 0};  // device walks, sequence download, edges
+  float ms[3] = {0, 0, 0};  // device walks, downloads, device edges
 };
 
-static uint8_t comp_ascii(uint8_t c) { return c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A'; }
-
-// build.rs:552-594 with UnitigNode::add_edge (cdbg.rs): for ov = k, then k-1,
-// every strand's ov-suffix meets the ov-prefixes of all strands (prefixes of
-// one key in unitig-id order, forward before reverse); self edges only on
-// circular unitigs; an edge to the same target and overlap adds its support.
-static void graph_edges(fb_graph* g) {
-  struct E {
-    uint32_t t;
-    uint8_t ov;
-    uint16_t sup;
-  };
-  const uint64_t U = g->len.size();
-  const uint32_t k = g->k;
-  std::vector<std::vector<E>> ef(U), er(U);
-  auto add = [](std::vector<E>& l, uint32_t t, uint8_t ov) {
-    for (E& e : l)
-      if (e.t == t && e.ov == ov) {
-        e.sup = (uint16_t)std::min<uint32_t>(65535u, (uint32_t)e.sup + 10u);
-        return;
-      }
-    l.push_back(E{t, ov, 10});
-  };
-  for (uint64_t u = 0; u < U; ++u)
-    if (g->circ[u]) ef[u].push_back(E{(uint32_t)(u << 1), (uint8_t)(k - 1), 10});
-  for (int pass = 0; pass < 2; ++pass) {
-    const uint32_t ov = pass == 0 ? k : k - 1;
-    if (ov == 0) continue;
-    std::vector<uint8_t> rcpre((size_t)U * ov);  // reverse strands' prefixes
-    struct P {
-      const uint8_t* p;
-      uint32_t d;
-    };
-    std::vector<P> pe;
-    pe.reserve(2 * U);
-    for (uint64_t u = 0; u < U; ++u) {
-      const uint32_t L = g->len[u];
-      if (L < ov) continue;
-      const uint8_t* s = g->seq.data() + g->off[u];
-      uint8_t* r = rcpre.data() + u * ov;
-      for (uint32_t j = 0; j < ov; ++j) r[j] = comp_ascii(s[L - 1 - j]);
-      pe.push_back(P{s, (uint32_t)(u << 1)});
-      pe.push_back(P{r, (uint32_t)(u << 1 | 1)});
-    }
-    std::sort(pe.begin(), pe.end(), [ov](const P& a, const P& b) {
-      const int c = memcmp(a.p, b.p, ov);
-      return c < 0 || (c == 0 && a.d < b.d);
-    });
-    std::vector<uint8_t> suf(ov);
-    for (uint64_t u = 0; u < U; ++u) {
-      const uint32_t L = g->len[u];
-      if (L < ov) continue;
-      const uint8_t* s = g->seq.data() + g->off[u];
-      for (int side = 0; side < 2; ++side) {
-        const uint8_t* q = s + (L - ov);
-        if (side) {
-          for (uint32_t j = 0; j < ov; ++j) suf[j] = comp_ascii(s[ov - 1 - j]);
-          q = suf.data();
-        }
-        auto lo = std::lower_bound(pe.begin(), pe.end(), q,
-                                   [ov](const P& a, const uint8_t* key) { return memcmp(a.p, key, ov) < 0; });
-        for (auto it = lo; it != pe.end() && memcmp(it->p, q, ov) == 0; ++it)
-          if ((it->d >> 1) != u || g->circ[u]) add(side ? er[u] : ef[u], it->d, (uint8_t)ov);
-      }
-    }
+// The overlap edges of the U unitigs of a (sequences, offsets, lengths and
+// circular flags on the device), into g's edge lists; tot: U * 2 edge counts
+// per end (node u: forward end 2u, reverse end 2u + 1) on the host.
+static int compact_edges(fb_ctx* c, const CgArgs& a, uint64_t U, uint32_t k, fb_graph* g,
+                         std::vector<uint32_t>& tot_h) {
+  hipStream_t s = c->stream;
+  tot_h.assign(2 * U, 0);
+  if (!U) return FB_OK;
+  if (U >= (1ull << 30)) return set_err(c, FB_ERR_INVALID, "edges take fewer than 2^30 unitigs");
+  const uint64_t N2 = 2 * U;
+  DBuf hk, hv, sk[2], sv[2], cnt, tot, base, pos, tmp;
+  int r = 0;
+  if (!r) r = hk.get(c, N2 * 8);
+  if (!r) r = hv.get(c, N2 * 4);
+  for (int p = 0; p < 2 && !r; ++p) {
+    r = sk[p].get(c, N2 * 8);
+    if (!r) r = sv[p].get(c, N2 * 4);
   }
-  g->flags.assign(U, 0);
-  g->nf.assign(U, 0);
-  g->nr.assign(U, 0);
-  for (uint64_t u = 0; u < U; ++u) {
-    g->flags[u] = g->circ[u] ? 2 : 0;
-    g->nf[u] = (uint32_t)ef[u].size();
-    g->nr[u] = (uint32_t)er[u].size();
+  if (!r) r = cnt.get(c, N2 * 4);
+  if (!r) r = tot.get(c, N2 * 4);
+  if (!r) r = base.get(c, N2 * 4);
+  if (!r) r = pos.get(c, N2 * 4);
+  if (r) return r;
+  size_t tb = 0, t1 = 0;
+  HIP_TRY(c, cg_sort_pairs_u64(nullptr, nullptr, nullptr, nullptr, N2, nullptr, &tb, s));
+  HIP_TRY(c, cg_exclusive_sum_u32(nullptr, nullptr, N2, nullptr, &t1, s));
+  tb = std::max(tb, t1);
+  if (int e = tmp.get(c, tb + 256)) return e;
+  CgEdgeArgs e;
+  e.seq = a.seq;
+  e.off = a.u_off;
+  e.len = a.u_len;
+  e.circ = a.u_circ;
+  e.U = (uint32_t)U;
+  e.k = k;
+  e.hkey = hk.as<uint64_t>();
+  e.hval = hv.as<uint32_t>();
+  e.cnt = cnt.as<uint32_t>();
+  e.pos = pos.as<uint32_t>();
+  const int npass = k > 1 ? 2 : 1;  // overlaps k and k - 1 (none of 0)
+  for (int p = 0; p < npass; ++p) {
+    e.ov = p == 0 ? k : k - 1;
+    e.merge = p == 1;
+    HIP_TRY(c, launch_cg_prefix(e, s));
+    size_t t = tb;
+    HIP_TRY(c, cg_sort_pairs_u64(hk.as<uint64_t>(), sk[p].as<uint64_t>(), hv.as<uint32_t>(), sv[p].as<uint32_t>(), N2,
+                                 tmp.p, &t, s));
+    e.skey = sk[p].as<uint64_t>();
+    e.sval = sv[p].as<uint32_t>();
+    HIP_TRY(c, launch_cg_end_count(e, p == 0, s));
   }
-  for (uint64_t u = 0; u < U; ++u)
-    for (int side = 0; side < 2; ++side)
-      for (const E& e : side ? er[u] : ef[u]) {
-        g->e_src.push_back((uint32_t)(u << 1 | (uint32_t)side));
-        g->e_dst.push_back(e.t);
-        g->e_ov.push_back(e.ov);
-        g->e_sup.push_back(e.sup);
-      }
+  HIP_TRY(c, launch_cg_end_total(cnt.as<uint32_t>(), tot.as<uint32_t>(), (uint32_t)N2, s));
+  {
+    size_t t = tb;
+    HIP_TRY(c, cg_exclusive_sum_u32(tot.as<uint32_t>(), base.as<uint32_t>(), N2, tmp.p, &t, s));
+  }
+  uint32_t last_base = 0;
+  HIP_TRY(c, hipMemcpyAsync(&last_base, base.as<uint32_t>() + N2 - 1, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(tot_h.data(), tot.p, N2 * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  const uint64_t E = (uint64_t)last_base + tot_h[N2 - 1];
+  g->e_src.resize(E);
+  g->e_dst.resize(E);
+  g->e_ov.resize(E);
+  g->e_sup.resize(E);
+  if (!E) return FB_OK;
+  DBuf es, ed, eo, eu;
+  if (!r) r = es.get(c, E * 4);
+  if (!r) r = ed.get(c, E * 4);
+  if (!r) r = eo.get(c, E);
+  if (!r) r = eu.get(c, E * 2);
+  if (r) return r;
+  e.base = base.as<uint32_t>();
+  e.e_src = es.as<uint32_t>();
+  e.e_dst = ed.as<uint32_t>();
+  e.e_ov = eo.as<uint8_t>();
+  e.e_sup = eu.as<uint16_t>();
+  for (int p = 0; p < npass; ++p) {
+    e.ov = p == 0 ? k : k - 1;
+    e.merge = p == 1;
+    e.skey = sk[p].as<uint64_t>();
+    e.sval = sv[p].as<uint32_t>();
+    HIP_TRY(c, launch_cg_end_write(e, p == 0, s));
+  }
+  HIP_TRY(c, hipMemcpyAsync(g->e_src.data(), es.p, E * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(g->e_dst.data(), ed.p, E * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(g->e_ov.data(), eo.p, E, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(g->e_sup.data(), eu.p, E * 2, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return FB_OK;
 }
 
 static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, const uint8_t* mask, uint64_t n,
@@ -3158,6 +3165,9 @@ static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, 
   HIP_TRY(c, launch_cg_write(a, sp, sc, s));
   HIP_TRY(c, launch_cg_flip(a.seq, a.u_off, a.u_len, a.n_lin, flip.as<uint8_t>(), lin_bases, s));
   HIP_TRY(c, hipEventRecord(c->ev[1], s));
+  std::vector<uint32_t> tot;
+  if (int e = compact_edges(c, a, U, k, g, tot)) return e;
+  HIP_TRY(c, hipEventRecord(c->ev[3], s));
   g->n_lin = n_lin;
   g->seq.resize(B);
   g->off.resize(U);
@@ -3174,14 +3184,16 @@ static int compact_device(fb_ctx* c, const uint64_t* keys, const uint16_t* cov, 
   HIP_TRY(c, hipEventRecord(c->ev[2], s));
   HIP_TRY(c, hipEventSynchronize(c->ev[2]));
   (void)hipEventElapsedTime(&g->ms[0], c->ev[0], c->ev[1]);
-  (void)hipEventElapsedTime(&g->ms[1], c->ev[1], c->ev[2]);
-  return FB_OK;
-}
-
-static int compact_finish(fb_graph* g) {
-  const auto t0 = std::chrono::steady_clock::now();
-  graph_edges(g);
-  g->ms[2] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  (void)hipEventElapsedTime(&g->ms[2], c->ev[1], c->ev[3]);
+  (void)hipEventElapsedTime(&g->ms[1], c->ev[3], c->ev[2]);
+  g->flags.assign(U, 0);
+  g->nf.assign(U, 0);
+  g->nr.assign(U, 0);
+  for (uint64_t u = 0; u < U; ++u) {
+    g->flags[u] = g->circ[u] ? 2 : 0;
+    g->nf[u] = tot[2 * u];
+    g->nr[u] = tot[2 * u + 1];
+  }
   return FB_OK;
 }
 
@@ -3195,7 +3207,6 @@ extern "C" int fb_compact_solid(const fb_solid* so, fb_graph** out) {
   if (int r = compact_device(c, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(), so->mask.as<uint8_t>(), so->n,
                              so->k, g.get()))
     return r;
-  compact_finish(g.get());
   *out = g.release();
   return FB_OK;
 }
@@ -3224,7 +3235,6 @@ extern "C" int fb_compact_arrays(fb_ctx* c, const uint64_t* keys, const uint16_t
   if (int r = to_device(c, dm, mask, n)) return r;
   auto g = std::make_unique<fb_graph>();
   if (int r = compact_device(c, dk.as<uint64_t>(), dc.as<uint16_t>(), dm.as<uint8_t>(), n, k, g.get())) return r;
-  compact_finish(g.get());
   *out = g.release();
   return FB_OK;
 }

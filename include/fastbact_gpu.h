@@ -241,7 +241,8 @@ int fb_solid_stats(const fb_solid* s, uint64_t out[11]);
  * smallest k-mer), which fixes the direction each linear unitig is walked in
  * (its cov leaves out the node it ends on), where a circular unitig starts,
  * and the node numbering: linear unitigs in walk order, then circular ones.
- * The walks run on the device as list ranking; the edges on the host. */
+ * The walks run on the device as list ranking, and so do the overlap edges
+ * (every strand end against the hashed, sorted strand prefixes). */
 typedef struct fb_graph fb_graph;
 /* The solid set of a build (built with params sort = 1). */
 int fb_compact_solid(const fb_solid* s, fb_graph** out);
@@ -260,7 +261,7 @@ int fb_graph_unitigs(const fb_graph* g, uint8_t* seq, uint64_t* offsets, uint32_
 /* Edges of every node in insertion order, forward side then reverse side:
  * src = id << 1 | side, dst = DirUnitigId (id << 1 | is_rc), overlap, support. */
 int fb_graph_edges(const fb_graph* g, uint32_t* src, uint32_t* dst, uint8_t* ov, uint16_t* support);
-/* Milliseconds: [0] device walks, [1] sequence download, [2] edges on the host. */
+/* Milliseconds: [0] device walks, [1] downloads (sequences and edges), [2] device overlap edges. */
 int fb_graph_timings(const fb_graph* g, float out[3]);
 
 /* ---- Multi-k flow: filter_active_reads and upgrade_multik_ext ---------------

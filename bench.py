@@ -389,10 +389,13 @@ def bench_serving(F, shapes, banks, n_req: int, rate: float, seed: int = 0xFB05)
                 time.sleep(min(0.002, max(0.0, target - time.perf_counter())))
             b = banks[int(picks[i])]
             tickets.append(sv.submit(b.pe_forward, b.pe_reverse))
+            if (i + 1) % 24 == 0:
+                log(f"serving: {i + 1}/{n_req} requests submitted")
         res = [sv.wait(t, download=False) for t in tickets]
         wall = time.perf_counter() - t0
         st = sv.stats()
         lat = sorted(r.latency * 1e3 for r in res)
+        log(f"serving: Poisson stream done in {wall:.1f} s")
         bases = sum(3 * banks[int(p)].total_bases() for p in picks)
         out["poisson"] = {"requests": n_req, "rate_per_s": rate, "wall_s": round(wall, 2),
                           "isolates_per_hour": round(n_req / wall * 3600, 1),

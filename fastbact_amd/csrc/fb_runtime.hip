@@ -38,6 +38,8 @@ struct fb_ctx {
   hipStream_t stream = nullptr;
   uint64_t budget = 0;  // 0 = unlimited
   uint64_t held = 0;    // bytes allocated from HIP (in use + pooled)
+  uint64_t pooled = 0;  // bytes of the free buffers in the pool
+  uint64_t pool_cap = 0;  // most free bytes the pool keeps (the longest-unused go first); 0 = no limit
   std::string err;
   std::vector<PoolBuf> pool;  // free buffers
   hipEvent_t ev[12] = {};
@@ -105,6 +107,7 @@ static int pool_get(fb_ctx* c, size_t bytes, void** out) {
   }
   if (best != (size_t)-1) {
     *out = c->pool[best].p;
+    c->pooled -= c->pool[best].bytes;
     c->pool.erase(c->pool.begin() + best);
     return FB_OK;
   }
@@ -118,6 +121,7 @@ static int pool_get(fb_ctx* c, size_t bytes, void** out) {
     ++drop;
   }
   if (drop) {
+    c->pooled -= c->held - held;
     c->pool.erase(c->pool.begin(), c->pool.begin() + drop);
     c->held = held;
   }
@@ -133,6 +137,7 @@ static int pool_get(fb_ctx* c, size_t bytes, void** out) {
       c->held -= b.bytes;
     }
     c->pool.clear();
+    c->pooled = 0;
     (void)hipGetLastError();
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
@@ -149,6 +154,15 @@ static void pool_put(fb_ctx* c, void* p, size_t bytes) {
   bytes = (bytes + 255) & ~(size_t)255;
   if (bytes == 0) bytes = 256;
   c->pool.push_back({p, bytes});
+  c->pooled += bytes;
+  size_t drop = 0;
+  while (c->pool_cap && c->pooled > c->pool_cap && drop + 1 < c->pool.size()) {
+    (void)hipFree(c->pool[drop].p);
+    c->held -= c->pool[drop].bytes;
+    c->pooled -= c->pool[drop].bytes;
+    ++drop;
+  }
+  if (drop) c->pool.erase(c->pool.begin(), c->pool.begin() + drop);
 }
 
 // RAII device buffer from the pool.
@@ -239,6 +253,7 @@ extern "C" int fb_ctx_trim(fb_ctx* c) {
     c->held -= std::min(c->held, (uint64_t)b.bytes);
   }
   c->pool.clear();
+  c->pooled = 0;
   return FB_OK;
 }
 
@@ -1567,9 +1582,30 @@ static double est_keys_from_occ(uint64_t occ, uint64_t width, uint64_t windows) 
   return std::min(n, (double)windows + 1.0);
 }
 
-extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uint64_t reads,
-                                       const fb_build_params* p, uint32_t k) {
-  if (!p) return 0;
+// Device bytes of one isolate of a batch at k, by buffer.  The buffers of an
+// isolate kept across a k ladder only grow, so the ladder holds, per buffer,
+// its largest size over the ladder's k (serve_footprint).
+struct IsoBytes {
+  uint64_t fixed = 0;    // counts, gate bits, HLL registers, the bank
+  uint64_t scratch = 0;  // pass-1 bins, then pass-2 records
+  uint64_t masks = 0;    // gate masks
+  uint64_t runs = 0;     // per-block run matrix
+  uint64_t spill = 0;    // spill table
+  uint64_t out = 0;      // solid set
+  uint64_t total() const { return fixed + scratch + masks + runs + spill + out; }
+  void take_max(const IsoBytes& o) {
+    fixed = std::max(fixed, o.fixed);
+    scratch = std::max(scratch, o.scratch);
+    masks = std::max(masks, o.masks);
+    runs = std::max(runs, o.runs);
+    spill = std::max(spill, o.spill);
+    out = std::max(out, o.out);
+  }
+};
+
+static IsoBytes iso_bytes(uint64_t genome_size, uint64_t bases, uint64_t reads, const fb_build_params* p,
+                          uint32_t k) {
+  IsoBytes b;
   k = std::max<uint32_t>(k, 1);
   const uint64_t est_g = std::max<uint64_t>(genome_size, 50000);
   const uint64_t W = batch_cms_width(p, est_g);
@@ -1584,18 +1620,25 @@ extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uin
   const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u,
                               p->chunks2);
   const uint32_t rw = (2 * k + 8 + 63) / 64;
-  uint64_t b = W * 4 * 4 + (W * 4 + 31) / 32 * 4;            // exact counts + gate bits
-  b += p->hll_p ? (4ull << p->hll_p) : 0;                     // HLL registers
+  b.fixed = W * 4 * 4 + (W * 4 + 31) / 32 * 4;               // exact counts + gate bits
+  b.fixed += p->hll_p ? (4ull << p->hll_p) : 0;               // HLL registers
+  b.fixed += bases / 4 + reads * 12 + 4096;                   // the bank itself
   const uint64_t thr1 = reads * (uint64_t)default_split(k, max_len);
   const uint64_t n_wg1 = scatter_blocks(thr1, p->chunks1 ? p->chunks1 : p1_chunks(thr1), kP1BlockSize);
   const uint64_t nbins1 = std::max<uint64_t>(1, (W * 4) >> kBinLog2Cells);
   const uint64_t bins = (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536 + nbins1 * (n_wg1 + 16) * kP1Chunk * 2;
-  b += std::max<uint64_t>(bins, q.rec_cap * rw * 8);          // pass-1 bins, then pass-2 records
-  b += q.threads2 * 8 + q.n_blocks2 * q.nbr * 4;              // gate masks, run matrix
-  b += q.spill_cap * table_stride(key_words_for_k(k)) * 8;    // spill table
-  b += q.out_cap * (ref_words(k) * 8 + 3);                    // solid set
-  b += bases / 4 + reads * 12 + 4096;                         // the bank itself
+  b.scratch = std::max<uint64_t>(bins, q.rec_cap * rw * 8);  // pass-1 bins, then pass-2 records
+  b.masks = q.threads2 * 8;
+  b.runs = q.n_blocks2 * q.nbr * 4;
+  b.spill = q.spill_cap * table_stride(key_words_for_k(k)) * 8;
+  b.out = q.out_cap * (ref_words(k) * 8 + 3);
   return b;
+}
+
+extern "C" uint64_t fb_batch_footprint(uint64_t genome_size, uint64_t bases, uint64_t reads,
+                                       const fb_build_params* p, uint32_t k) {
+  if (!p) return 0;
+  return iso_bytes(genome_size, bases, reads, p, k).total();
 }
 
 // Sketches and fixed buffers of one isolate of a batch.
@@ -2404,10 +2447,13 @@ static uint64_t serve_size_hint(const fb_server* sv, const ServeReq& r) {
   return std::max<uint64_t>(r.bases / (sv->prm.estimate ? 20 : 60), 50000);
 }
 
+// A request's buffers only grow along its k ladder: the footprint is, per
+// buffer, the largest size over the ladder.
 static uint64_t serve_footprint(const fb_server* sv, const ServeReq& r) {
-  uint64_t f = 0;
+  IsoBytes m;
   for (uint32_t j = 0; j < sv->prm.n_passes; ++j)
-    f = std::max(f, fb_batch_footprint(serve_size_hint(sv, r), r.bases, r.reads, &sv->bprm, sv->prm.k[j]));
+    m.take_max(iso_bytes(serve_size_hint(sv, r), r.bases, r.reads, &sv->bprm, sv->prm.k[j]));
+  uint64_t f = m.total();
   if (sv->prm.estimate) f += (4ull << 23) * 4 + (3ull << 20) * 8;  // estimator cells and registers
   return f;
 }
@@ -2615,12 +2661,12 @@ extern "C" int fb_server_create(int device, const fb_serve_params* p, fb_server*
     fb_ctx_destroy(sv->ctx);
     return FB_ERR_HIP;
   }
-  // The context caches the buffers of finished requests for the next ones.
-  // Capped below the device, it frees cached buffers as it needs room;
-  // uncapped, it filled the device until an allocation failed and then freed
-  // its whole cache at once (seconds of hipFree and hipMalloc in one pass,
-  // the p99 of the served stream).
-  if (!p->ctx_budget) sv->ctx->budget = (uint64_t)((double)fr * 0.95);
+  // The context caches the buffers of finished requests for the next ones,
+  // at most a fifth of the device's free memory, the longest-unused freed
+  // first.  Uncached beyond that, it filled the device until an allocation
+  // failed and then freed its whole cache at once (seconds of hipFree and
+  // hipMalloc in one pass, the p99 of the served stream).
+  sv->ctx->pool_cap = (uint64_t)((double)fr * 0.2);
   const uint64_t budget = p->hbm_budget ? p->hbm_budget : (uint64_t)((double)fr * 0.8);
   sv->st.budget = budget;
   sv->adm = std::make_unique<fb::Admission>(budget, p->max_inflight ? p->max_inflight : 64);

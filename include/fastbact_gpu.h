@@ -608,11 +608,10 @@ typedef struct {
   uint32_t max_inflight;        /* requests on the device at once; 0 = 64 */
   uint32_t max_queue;           /* waiting requests; 0 = 4096, FB_ERR_BUSY beyond */
   uint32_t keep_sketches;       /* 1 = keep every pass's CMS table and HLL registers with its result */
-  uint64_t ctx_budget;          /* device bytes the server's context may hold, its cached free
-                                   buffers included; 0 = 95% of the device memory free when the
-                                   server is created.  Cached buffers are freed before the cap
-                                   is passed; a request whose upload or pass needs more than the
-                                   cap fails with FB_ERR_OOM */
+  uint64_t ctx_budget;          /* device bytes the server's context may hold; 0 = unlimited.  A
+                                   request whose upload or pass needs more fails with FB_ERR_OOM.
+                                   The context keeps at most a fifth of the device memory free at
+                                   creation in cached free buffers, the longest-unused freed first */
 } fb_serve_params;
 
 typedef struct {

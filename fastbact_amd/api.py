@@ -390,6 +390,14 @@ class SolidHandle(_Handle):
                 "lds_bins": int(t[5]), "lds_bin_keys": int(t[6]), "est_keys": int(t[7]),
                 "pass2_records": int(t[8]), "reads_used": int(t[9]), "pass2_rounds": int(t[10])}
 
+    def pass2_gate(self) -> str:
+        """'window': the Count-Min gate applied per window pair; 'key': once
+        per kept key and low edge (the ungated LDS route)."""
+        r = int(self.ctx.lib.fb_solid_pass2_gate(self.h))
+        if r < 0:
+            self.ctx.check(r, "fb_solid_pass2_gate")
+        return ("window", "key")[r]
+
     def download(self, k: int, with_cms: bool = True, counter_bits: int = 4, hll_p: int = 0) -> SolidSet:
         n, nw = self.count(), self.key_words()
         keys = np.zeros((n, nw), np.uint64)

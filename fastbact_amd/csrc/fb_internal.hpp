@@ -142,6 +142,11 @@ hipError_t launch_solid_bits(const uint32_t* counts, uint64_t n_counters, uint32
                              uint32_t* bits, hipStream_t s);
 // Sum of set bits of n u32 words added to *out (zeroed by the caller).
 hipError_t launch_popcount(const uint32_t* words, uint64_t n, unsigned long long* out, hipStream_t s);
+// Histogram of min(count, 255) over n counters added to hist[256] (zeroed by the caller).
+hipError_t launch_count_hist(const uint32_t* counts, uint64_t n, uint32_t* hist, hipStream_t s);
+// Keys with at least f windows expected from the row-0 count histogram (see
+// fb_runtime.hip): the pass-2 tables' load in the ungated form.
+double est_kept_keys(const uint32_t* hist, uint64_t width, uint64_t windows, uint32_t t, uint32_t f);
 // Reference-layout packing of the exact counts.
 hipError_t launch_pack_nibble(const uint32_t* counts, uint64_t n_counters, uint8_t* out, hipStream_t s);
 hipError_t launch_pack_u16(const uint32_t* counts, uint64_t n_counters, uint16_t* out, hipStream_t s);
@@ -210,6 +215,13 @@ struct P2BinArgs {
   uint32_t* status = nullptr;    // zeroed: [0] record buffer overflow (never for exact runs), [1] records of all rounds
   uint32_t lds_cap = 0;          // LDS table entries
   uint32_t min_trans_cov = 0;
+  // Ungated form: records of every window pair, the Count-Min gate applied
+  // per kept key in the bin kernel (needs the counters' cap >= t_solid, and
+  // t_solid <= 10 with prior contigs); filt levels of the LDS record filter
+  // (0: none, else min(min_trans_cov, kP2MaxFilt)).
+  uint32_t ungated = 0;
+  uint32_t t_solid = 0;
+  uint32_t filt = 0;
   DevSolidTable table;           // keys that find the LDS table full
   uint32_t nref = 1;
   uint64_t* out_keys = nullptr;
@@ -219,9 +231,11 @@ struct P2BinArgs {
   uint32_t out_cap = 0xFFFFFFFFu; // output entries; positions past it are not written
   unsigned long long* stats = nullptr;  // zeroed; [0] records (sum of raw cov), [1] keys held in LDS
 };
-uint32_t p2_lds_cap(uint32_t kw);
+constexpr uint32_t kP2FiltWords = 2048;  // u32 words per level of the ungated bin kernel's record filter
+constexpr uint32_t kP2MaxFilt = 3;       // filter levels: records counted up to 3 per position
+uint32_t p2_lds_cap(uint32_t kw, uint32_t filt = 0);
 uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
-size_t p2_lds_bytes(uint32_t kw, uint32_t cap);
+size_t p2_lds_bytes(uint32_t kw, uint32_t cap, uint32_t filt = 0);
 // gated (optional) is recorded after the gate, count, scan and offset kernels,
 // scattered after the scatter kernel.
 hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated = nullptr,

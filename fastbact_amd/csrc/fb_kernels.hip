@@ -183,6 +183,28 @@ __global__ __launch_bounds__(kBlock) void k_popcount(const uint32_t* __restrict_
     if (t) atomicAdd(out, t);
   }
 }
+// Histogram of min(count, 255) over n counters, added to hist[256].
+__global__ __launch_bounds__(kBlock) void k_count_hist(const uint32_t* __restrict__ counts, uint64_t n,
+                                                       uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = counts[i];
+    atomicAdd(h + (c < 255u ? c : 255u), 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x)
+    if (h[i]) atomicAdd(hist + i, h[i]);
+}
+hipError_t launch_count_hist(const uint32_t* counts, uint64_t n, uint32_t* hist, hipStream_t s) {
+  uint64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 1024) g = 1024;
+  if (g == 0) g = 1;
+  hipLaunchKernelGGL(k_count_hist, dim3((unsigned)g), dim3(kBlock), 0, s, counts, n, hist);
+  return hipGetLastError();
+}
+
 hipError_t launch_popcount(const uint32_t* words, uint64_t n, unsigned long long* out, hipStream_t s) {
   uint64_t g = (n + kBlock - 1) / kBlock;
   if (g > 1024) g = 1024;

@@ -30,6 +30,32 @@
 //     key that finds no free LDS entry near its home slot goes to the global
 //     table; each key is in exactly one of the two, so pass 3 over the global
 //     table appends the rest.
+//
+// The ungated form (P2BinArgs::ungated) drops the per-window gate and its
+// walks.  build.rs:253-289 pushes the pair (u, v) when est(u) >= t and
+// est(v) >= t, and pass 3 keeps a key when one of its edge counters reaches
+// m = min_trans_cov >= t.  Each edge slot e of a canonical k-mer u names one
+// neighbour v_e, and every pair occurrence counted on slot e is a distinct
+// window of u and a distinct window of v_e.  So when a slot has occ >= t
+// pair occurrences, u and v_e both have >= t windows, and the Count-Min
+// estimate (never below the count, while the counters' cap is >= t) passes
+// both: the slot's gated count equals occ.  When occ < t the slot cannot
+// reach m either way.  Hence the mask, and which keys are kept, do not
+// depend on the gate at all.  Only cov does: cov(u) is the sum of the
+// gated counts over u's slots, and a slot with 0 < occ < t contributes
+// only if est(v_e) >= t.  The records therefore cover every window pair,
+// and the bin kernel, once per kept key and low slot, rebuilds v_e from
+// the key and the slot and probes its four Count-Min rows; failing slots
+// leave cov.  (A kept key whose own estimate fails would need every slot
+// dropped; it can only be a prior contig's key, whose ten adds per window
+// pass any gate t <= 10, the limit the host enforces with prior contigs.)
+// Ungated records include every error k-mer, so a bin kernel first counts
+// its records into an LDS filter: nf bit levels (level l set at a position
+// when l + 1 records hashed there), two positions per key.  A key whose
+// two positions do not both reach level nf - 1 has fewer than nf <= m
+// records, so no slot of it can reach m: its records are skipped, and the
+// tables hold the keys that can be kept (plus the filter's few false
+// positives, which pass 3 drops by their mask).
 #include "fb_device.hpp"
 #include "fb_internal.hpp"
 #include "fb_table.hpp"
@@ -120,6 +146,141 @@ __device__ __forceinline__ void decode_record(const uint64_t (&w)[L + 1], uint32
   key.a[L - 1] &= top_mask<L>(k);
 }
 
+// A fresh read of a workgroup-shared word that other threads update.  The
+// pointer must point into LDS: it is cast to the LDS address space, since a
+// volatile read through a generic pointer compiles to a flat load, which
+// also waits for every global load in flight.
+template <class T>
+__device__ __forceinline__ T lds_vload(const T* p) {
+  typedef __attribute__((address_space(3))) const volatile T* lds_ptr;
+  return *(lds_ptr)(p);
+}
+
+// Bits of every window of the share p0..p1 (p1 - p0 <= 63 pairs).
+__device__ __forceinline__ uint64_t all_windows(uint32_t p0, uint32_t p1) {
+  return (p1 - p0 >= 63u) ? ~0ull : (2ull << (p1 - p0)) - 1ull;
+}
+
+// Reverse complement of the 2k-bit k-mer y: the complement with its 2-bit
+// groups reversed within 2k bits.
+template <int L>
+__device__ __forceinline__ Kmer<L> kmer_revcomp(const Kmer<L>& y, uint32_t k) {
+  Kmer<L> c;
+#pragma unroll
+  for (int i = 0; i < L; ++i) c.a[i] = ~y.a[i];
+  c.a[L - 1] &= top_mask<L>(k);
+  return fwd_from_window<L>(c, k);
+}
+
+// Whether the neighbour of canonical k-mer x through edge slot q passes all
+// four Count-Min rows of the gate bitmap.  Slot q < 4 appends base q in x's
+// orientation (edge_u of a forward-canonical u, edge_v of a reverse-canonical
+// v); slot 4 + b prepends base 3 - b (build.rs:268-275).
+template <int L>
+__device__ __forceinline__ bool neighbour_passes(const P2BinArgs& a, const Kmer<L>& x, uint32_t q) {
+  const uint32_t k = a.k;
+  Kmer<L> y;
+  if (q < 4u) {
+#pragma unroll
+    for (int i = L - 1; i > 0; --i) y.a[i] = (x.a[i] << 2) | (x.a[i - 1] >> 62);
+    y.a[0] = (x.a[0] << 2) | q;
+    y.a[L - 1] &= top_mask<L>(k);
+  } else {
+#pragma unroll
+    for (int i = 0; i < L - 1; ++i) y.a[i] = (x.a[i] >> 2) | (x.a[i + 1] << 62);
+    y.a[L - 1] = x.a[L - 1] >> 2;
+    const uint32_t pos = 2u * (k - 1u);
+    const uint64_t b = (uint64_t)(7u - q) << (pos & 63u);
+#pragma unroll
+    for (int i = 0; i < L; ++i)
+      if ((pos >> 6) == (uint32_t)i) y.a[i] |= b;
+  }
+  const Kmer<L> r = kmer_revcomp<L>(y, k);
+  const uint64_t fold = kmer_lt<L>(r, y) ? kmer_fold<L>(r) : kmer_fold<L>(y);
+  const uint64_t wmask = (1ull << a.log2w) - 1ull;
+#pragma unroll
+  for (int row = 0; row < 4; ++row) {
+    const uint64_t c = ((uint64_t)row << a.log2w) | (splitmix64(fold ^ cms_seed(row)) & wmask);
+    if (!((a.solid_bits[c >> 5] >> (c & 31u)) & 1u)) return false;
+  }
+  return true;
+}
+
+// Key of 62-bit words wk as limbs.
+template <int L, int KW>
+__device__ __forceinline__ Kmer<L> unpack62b(const uint64_t (&wk)[KW]) {
+  uint64_t limb[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < KW; ++j) {
+    const int bit = 62 * j, li = bit / 64, off = bit % 64;
+    if (li < 4) limb[li] |= wk[j] << off;
+    if (off > 2 && li + 1 < 4) limb[li + 1] |= wk[j] >> (64 - off);
+  }
+  Kmer<L> x;
+#pragma unroll
+  for (int i = 0; i < L; ++i) x.a[i] = limb[i];
+  return x;
+}
+
+// What an ungated key's cov loses to the gate: the counts of its slots with
+// 0 < count < t whose neighbour fails the gate (edge(q): the slot's count,
+// exact below the edge gate m >= t; prior-contig slots hold >= 1000).
+template <int L, class Edge>
+__device__ __forceinline__ uint32_t gate_loss(const P2BinArgs& a, const Kmer<L>& x, Edge&& edge) {
+  uint32_t loss = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < 8; ++q) {
+    const uint32_t v = edge(q);
+    if (v != 0u && v < a.t_solid && !neighbour_passes<L>(a, x, q)) loss += v;
+  }
+  return loss;
+}
+
+// LDS record filter of the ungated bin kernel: kP2FiltWords words per level.
+constexpr uint32_t kFiltLog2Bits = 16;
+static_assert((1u << kFiltLog2Bits) == 32u * kP2FiltWords, "filter level size");
+template <int L>
+__device__ __forceinline__ void filt_positions(const Kmer<L>& key, uint32_t& p0, uint32_t& p1) {
+  uint64_t h = 0x8cb92ba72f3d8dd7ull;
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    h = (h ^ key.a[j]) * 0x9e3779b97f4a7c15ull;
+    h ^= h >> 29;
+  }
+  h *= 0xbf58476d1ce4e5b9ull;
+  p0 = (uint32_t)(h >> (64u - kFiltLog2Bits));
+  p1 = (uint32_t)(h >> 8) & ((1u << kFiltLog2Bits) - 1u);
+}
+// One record at both positions: it sets the lowest level not yet set there
+// (an atomicOr that finds the bit set moves on to the next level), so after
+// all records level l is set where at least l + 1 records landed.  A prior
+// contig's record sets the top level: its key (and any key sharing both
+// positions) always passes.
+template <int L>
+__device__ __forceinline__ void filt_add(uint32_t* f, uint32_t nf, const Kmer<L>& key, bool contig) {
+  uint32_t pos[2];
+  filt_positions<L>(key, pos[0], pos[1]);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t wd = pos[j] >> 5, bit = 1u << (pos[j] & 31u);
+    uint32_t* top = f + (nf - 1u) * kP2FiltWords + wd;
+    if (lds_vload(top) & bit) continue;
+    if (contig) {
+      atomicOr(top, bit);
+      continue;
+    }
+    for (uint32_t l = 0; l < nf; ++l)
+      if (!(atomicOr(f + l * kP2FiltWords + wd, bit) & bit)) break;
+  }
+}
+template <int L>
+__device__ __forceinline__ bool filt_pass(const uint32_t* f, uint32_t nf, const Kmer<L>& key) {
+  uint32_t p0, p1;
+  filt_positions<L>(key, p0, p1);
+  const uint32_t* top = f + (nf - 1u) * kP2FiltWords;
+  return ((lds_vload(top + (p0 >> 5)) >> (p0 & 31u)) & (lds_vload(top + (p1 >> 5)) >> (p1 & 31u)) & 1u) != 0u;
+}
+
 __device__ __forceinline__ bool p2_share(const DevReads& R, uint64_t gid, uint32_t split, uint32_t k, uint64_t& r,
                                          uint32_t& p0, uint32_t& p1) {
   r = gid / split;
@@ -179,7 +340,7 @@ template <int L>
 __device__ __forceinline__ void p2_gate_body(const P2BinArgs& a, uint64_t gid, uint32_t row) {
   uint64_t r = 0;
   uint32_t p0 = 0, p1 = 0;
-  if (!p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) return;
+  if (a.ungated || !p2_share(a.reads, gid, a.split, a.k, r, p0, p1)) return;
   const uint64_t m = row ? a.gate_masks[gid] : ~0ull;
   a.gate_masks[gid] = m ? gate_row<L>(a, r, p0, p1, m, row) : 0ull;
 }
@@ -323,8 +484,10 @@ __device__ __forceinline__ void p2_count_body(const P2BinArgs& a, uint64_t blk, 
     bool contig = false;
     if (!p2_share_any(a, gid, src, r, p0, p1, contig)) continue;
     uint64_t m;
-    if (contig) {  // every window of a prior contig (build.rs:198-239)
-      m = (p1 - p0 >= 63u) ? ~0ull : (2ull << (p1 - p0)) - 1ull;
+    if (a.ungated) {  // every window pair
+      m = all_windows(p0, p1);
+    } else if (contig) {  // every window of a prior contig (build.rs:198-239)
+      m = all_windows(p0, p1);
       if (round == 0) a.gate_masks[gid] = m;
     } else {
       m = a.gate_masks[gid];
@@ -470,7 +633,7 @@ __device__ __forceinline__ void p2_scatter_body(const P2BinArgs& a, uint64_t blk
     const DevReads* src = nullptr;
     bool contig = false;
     if (!p2_share_any(a, gid, src, r, p0, p1, contig)) continue;
-    const uint64_t gates = a.gate_masks[gid];
+    const uint64_t gates = a.ungated ? all_windows(p0, p1) : a.gate_masks[gid];
     if (!gates) continue;
     const uint32_t cbit = contig ? 256u : 0u;
     p2_walk<L>(a, *src, r, p0, p1, gates,
@@ -536,7 +699,7 @@ __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, 
     for (int j = 0; j < KW; ++j) {
       if (!match) break;
       unsigned long long* p = keys + (uint64_t)j * cap + s;
-      unsigned long long v = *reinterpret_cast<volatile unsigned long long*>(p);
+      unsigned long long v = lds_vload(p);
       if (v == kEmpty) {
         const unsigned long long old = atomicCAS(p, (unsigned long long)kEmpty, (unsigned long long)w[j]);
         v = old == kEmpty ? (unsigned long long)w[j] : old;
@@ -549,16 +712,103 @@ __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, 
   return -1;
 }
 
+// f(record) for records [0, n) of src, thread t taking t, t + blockDim, ...;
+// each thread has kRecInFlight independent record loads in flight (the
+// record loop is bound by their latency, not by the LDS work).
+constexpr int kRecInFlight = 4;
+template <int L, class F>
+__device__ __forceinline__ void for_records(const uint64_t* __restrict__ src, uint32_t n, uint32_t rw, F&& f) {
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kRecInFlight * blockDim.x) {
+    uint64_t r[kRecInFlight][L + 1];
+#pragma unroll
+    for (int u = 0; u < kRecInFlight; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * blockDim.x;
+      if (i < n) load_record<L>(src + (uint64_t)i * rw, rw, r[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kRecInFlight; ++u)
+      if (i0 + (uint32_t)u * blockDim.x < n) f(r[u]);
+  }
+}
+
+// One record's sides into entry e of the LDS table, or into the global table
+// when e < 0 (the key's probe window in LDS is full).
+template <int KW>
+__device__ __forceinline__ void p2_table_add(const P2BinArgs& a, uint32_t* cov, uint32_t* edges, int e,
+                                             const uint64_t (&w)[KW], uint32_t info) {
+  const uint32_t gate = a.min_trans_cov;
+  const bool left = info & 1u, right = (info >> 4) & 1u;
+  const uint32_t el = (info >> 1) & 7u, er = (info >> 5) & 7u;
+  // a prior contig's side adds cov 100 and edge 1000 (build.rs:219-231)
+  const bool contig = (info >> 8) & 1u;
+  const uint32_t cw = contig ? 100u : 1u;
+  if (e >= 0) {
+    atomicAdd(cov + e, cw * ((uint32_t)left + (uint32_t)right));
+    // u16 edge counters, two per word.  Only `value >= gate` (gate <= 255)
+    // is ever read from them, and values below the gate are exact.  A read
+    // adds 1 after it has loaded a value below the gate; the load may be
+    // stale, so adds land past the gate, at most two per thread of the block
+    // (the two sides of its record; the LDS serves a wave's operations in
+    // issue order, so its next record's loads see its earlier adds).  A
+    // contig's +1000 saturates by compare and swap at kEdgeCeil = 65535 -
+    // 4 * blockDim and never lowers a value, so every half stays <=
+    // max(kEdgeCeil, gate) + 2 * blockDim < 65535 and a read add can neither
+    // carry into the other half nor wrap.
+    for (int side = 0; side < 2; ++side) {
+      if (!(side ? right : left)) continue;
+      const uint32_t ed = side ? er : el;
+      uint32_t* ew = edges + 4u * (uint32_t)e + (ed >> 1);
+      const uint32_t sh = (ed & 1u) * 16u;
+      if (contig) {
+        constexpr uint32_t kEdgeCeil = 65535u - 4u * (uint32_t)kP2BinBlock;
+        uint32_t old = lds_vload(ew), seen;
+        do {
+          seen = old;
+          const uint32_t hv = (seen >> sh) & 0xffffu;
+          if (hv >= kEdgeCeil) break;
+          const uint32_t nv = hv + 1000u < kEdgeCeil ? hv + 1000u : kEdgeCeil;
+          old = atomicCAS(ew, seen, (seen & ~(0xffffu << sh)) | (nv << sh));
+        } while (old != seen);
+      } else if (((lds_vload(ew) >> sh) & 0xffffu) < gate) {
+        atomicAdd(ew, 1u << sh);
+      }
+    }
+  } else {
+    // no LDS entry: this key lives in the global table for the whole pass
+    uint32_t* c = table_find_global<KW>(a.table, w, slot_hash_words<KW>(w));
+    if (c) {
+      atomicAdd(c, cw * ((uint32_t)left + (uint32_t)right));
+      if (contig) {
+        if (left) atomicAdd(c + 2 + el, 1000u);
+        if (right) atomicAdd(c + 2 + er, 1000u);
+      } else {
+        if (left && c[2 + el] < gate) atomicAdd(c + 2 + el, 1u);
+        if (right && c[2 + er] < gate) atomicAdd(c + 2 + er, 1u);
+      }
+    }
+  }
+}
+
+// With the record filter (nf levels) a bin's records are read twice: the
+// first pass counts every record into the filter, the second tables the
+// records whose key passes it (and every prior contig's record).  Both
+// passes keep the lanes of a wave on the same short path; a single pass
+// that tables keys as their filter count completes was measured slower, its
+// per-record branches (new keys, parking) being taken by some lane of
+// nearly every wave.
 template <int L, int KW>
 __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, unsigned char* smem, uint32_t round) {
   if ((round >> a.log2_rounds) != 0u) return;  // this isolate has fewer rounds
   const uint32_t cap = a.lds_cap;
-  unsigned long long* keys = reinterpret_cast<unsigned long long*>(smem);      // KW * cap
+  const uint32_t nf = a.filt;                                                   // filter levels (0: none)
+  uint32_t* filt = reinterpret_cast<uint32_t*>(smem);                           // nf * kP2FiltWords
+  unsigned long long* keys = reinterpret_cast<unsigned long long*>(filt + nf * kP2FiltWords);  // KW * cap
   uint32_t* cov = reinterpret_cast<uint32_t*>(keys + (uint64_t)KW * cap);       // cap
   uint32_t* edges = cov + cap;                                                  // 4 * cap (u16 pairs)
   __shared__ uint32_t s_wave[kP2BinBlock / 64];
   __shared__ uint32_t s_base;
   __shared__ unsigned long long s_stat[2];
+  for (uint32_t i = threadIdx.x; i < nf * kP2FiltWords; i += blockDim.x) filt[i] = 0u;
   for (uint32_t i = threadIdx.x; i < KW * cap; i += blockDim.x) keys[i] = kEmpty;
   for (uint32_t i = threadIdx.x; i < 5 * cap; i += blockDim.x) cov[i] = 0u;
   if (threadIdx.x < 2) s_stat[threadIdx.x] = 0ull;
@@ -571,70 +821,24 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   const uint32_t n = e1 > e0 ? (uint32_t)(e1 - e0) : 0u;
   const uint32_t rw = a.rw;
   const uint32_t gate = a.min_trans_cov;
-  // each thread loads its next record while it tables the current one
-  uint64_t nxt[L + 1];
-  if (threadIdx.x < n) load_record<L>(a.recs + (e0 + threadIdx.x) * rw, rw, nxt);
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-    uint64_t cur[L + 1];
-#pragma unroll
-    for (int j = 0; j <= L; ++j) cur[j] = nxt[j];
-    if (i + blockDim.x < n) load_record<L>(a.recs + (e0 + i + blockDim.x) * rw, rw, nxt);
+  if (nf) {
+    for_records<L>(a.recs + e0 * rw, n, rw, [&](const uint64_t (&cur)[L + 1]) {
+      Kmer<L> key;
+      uint32_t info;
+      decode_record<L>(cur, a.k, key, info);
+      filt_add<L>(filt, nf, key, (info >> 8) & 1u);
+    });
+    __syncthreads();
+  }
+  for_records<L>(a.recs + e0 * rw, n, rw, [&](const uint64_t (&cur)[L + 1]) {
     Kmer<L> key;
     uint32_t info;
     decode_record<L>(cur, a.k, key, info);
+    if (nf && !((info >> 8) & 1u) && !filt_pass<L>(filt, nf, key)) return;  // fewer than nf records
     uint64_t w[KW];
     pack62b<L, KW>(key, w);
-    const bool left = info & 1u, right = (info >> 4) & 1u;
-    const uint32_t el = (info >> 1) & 7u, er = (info >> 5) & 7u;
-    // a prior contig's side adds cov 100 and edge 1000 (build.rs:219-231)
-    const bool contig = (info >> 8) & 1u;
-    const uint32_t cw = contig ? 100u : 1u;
-    const int e = lds_find<KW>(keys, cap, w);
-    if (e >= 0) {
-      atomicAdd(cov + e, cw * ((uint32_t)left + (uint32_t)right));
-      // u16 edge counters, two per word.  Only `value >= gate` (gate <= 255)
-      // is ever read from them.  A read adds 1 after it has loaded a value
-      // below the gate; the load may be stale, so adds land past the gate,
-      // at most two per thread of the block (the two sides of its record;
-      // the LDS serves a wave's operations in issue order, so its next load
-      // sees its earlier adds).  A contig's +1000 saturates by compare and
-      // swap at kEdgeCeil = 65535 - 4 * blockDim and never lowers a value,
-      // so every half stays <= max(kEdgeCeil, gate) + 2 * blockDim < 65535
-      // and a read add can neither carry into the other half nor wrap.
-      for (int side = 0; side < 2; ++side) {
-        if (!(side ? right : left)) continue;
-        const uint32_t ed = side ? er : el;
-        uint32_t* ew = edges + 4u * (uint32_t)e + (ed >> 1);
-        const uint32_t sh = (ed & 1u) * 16u;
-        if (contig) {
-          constexpr uint32_t kEdgeCeil = 65535u - 4u * (uint32_t)kP2BinBlock;
-          uint32_t old = *reinterpret_cast<volatile uint32_t*>(ew), seen;
-          do {
-            seen = old;
-            const uint32_t hv = (seen >> sh) & 0xffffu;
-            if (hv >= kEdgeCeil) break;
-            const uint32_t nv = hv + 1000u < kEdgeCeil ? hv + 1000u : kEdgeCeil;
-            old = atomicCAS(ew, seen, (seen & ~(0xffffu << sh)) | (nv << sh));
-          } while (old != seen);
-        } else if (((*reinterpret_cast<volatile uint32_t*>(ew) >> sh) & 0xffffu) < gate) {
-          atomicAdd(ew, 1u << sh);
-        }
-      }
-    } else {
-      // no LDS entry: this key lives in the global table for the whole pass
-      uint32_t* c = table_find_global<KW>(a.table, w, slot_hash_words<KW>(w));
-      if (c) {
-        atomicAdd(c, cw * ((uint32_t)left + (uint32_t)right));
-        if (contig) {
-          if (left) atomicAdd(c + 2 + el, 1000u);
-          if (right) atomicAdd(c + 2 + er, 1000u);
-        } else {
-          if (left && c[2 + el] < gate) atomicAdd(c + 2 + el, 1u);
-          if (right && c[2 + er] < gate) atomicAdd(c + 2 + er, 1u);
-        }
-      }
-    }
-  }
+    p2_table_add<KW>(a, cov, edges, lds_find<KW>(keys, cap, w), w, info);
+  });
   __syncthreads();
   // pass 3 on the LDS table: count, one atomic per block, write
   const uint32_t per = (cap + blockDim.x - 1) / blockDim.x;
@@ -655,12 +859,19 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   uint32_t mine = 0, held = 0;
   unsigned long long raw = 0;
   for (uint32_t e = lo; e < hi; ++e) {
+    if (keys[e] == kEmpty) continue;
+    ++held;
+    raw += cov[e];
     uint32_t m, cv;
-    mine += keep_of(e, m, cv) ? 1u : 0u;
-    if (keys[e] != kEmpty) {
-      ++held;
-      raw += cov[e];
+    if (!keep_of(e, m, cv)) continue;
+    if (a.ungated) {  // the gate's share of cov, once per kept key (this thread's entries only)
+      uint64_t wk[KW];
+#pragma unroll
+      for (int j = 0; j < KW; ++j) wk[j] = keys[(uint64_t)j * cap + e];
+      cov[e] -= gate_loss<L>(a, unpack62b<L, KW>(wk),
+                             [&](uint32_t q) { return (edges[4u * e + (q >> 1)] >> ((q & 1u) * 16u)) & 0xffffu; });
     }
+    ++mine;
   }
   if (held) {
     atomicAdd(s_stat, raw);
@@ -727,6 +938,36 @@ __global__ __launch_bounds__(kP2BinBlock) void k_p2_bins_m(const P2BinArgs* __re
   p2_bins_body<L, KW>(args[iso], blockIdx.x - blk[iso], smem, round);
 }
 
+// The ungated form's correction of the global (spill) table before its pass
+// 3: every kept key's cov loses its slots whose neighbour fails the gate.
+template <int L, int KW>
+__device__ __forceinline__ void p2_spill_gate_body(const P2BinArgs& a, uint64_t i0, uint64_t step) {
+  const DevSolidTable& t = a.table;
+  for (uint64_t i = i0; i < t.cap; i += step) {
+    uint64_t* s = t.slots + i * t.stride;
+    uint64_t wk[KW];
+#pragma unroll
+    for (int j = 0; j < KW; ++j) wk[j] = s[j];
+    if (wk[0] == kEmpty) continue;
+    uint32_t* c = reinterpret_cast<uint32_t*>(s + KW);  // cov, pad, 8 edges
+    bool keep = false;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) keep |= c[2 + q] >= a.min_trans_cov;
+    if (!keep) continue;
+    c[0] -= gate_loss<L>(a, unpack62b<L, KW>(wk), [&](uint32_t q) { return c[2 + q]; });
+  }
+}
+template <int L, int KW>
+__global__ __launch_bounds__(256) void k_p2_spill_gate(P2BinArgs a) {
+  p2_spill_gate_body<L, KW>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
+}
+template <int L, int KW>
+__global__ __launch_bounds__(256) void k_p2_spill_gate_m(const P2BinArgs* __restrict__ args) {
+  const P2BinArgs& a = args[blockIdx.y];
+  if (a.ungated)
+    p2_spill_gate_body<L, KW>(a, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
+}
+
 // LDS the bin kernels may use: the full 160 KB once the attribute is granted,
 // else the 64 KB every workgroup has.
 static uint32_t g_lds_bytes = 0;
@@ -768,12 +1009,14 @@ static uint32_t lds_budget() {
 
 uint32_t p2_max_bins() { return (lds_budget() - 1024u) / 4u >= 16384u ? 16384u : 8192u; }
 
-uint32_t p2_lds_cap(uint32_t kw) {
-  // keys + cov + 8 u16 edges per entry within the workgroup's LDS
+uint32_t p2_lds_cap(uint32_t kw, uint32_t filt) {
+  // keys + cov + 8 u16 edges per entry within the workgroup's LDS, after the filter
   const uint32_t entry = kw * 8u + 4u + 16u;
-  return (lds_budget() - 1024u) / entry;
+  return (lds_budget() - 1024u - filt * kP2FiltWords * 4u) / entry;
 }
-size_t p2_lds_bytes(uint32_t kw, uint32_t cap) { return (size_t)cap * (kw * 8u + 4u + 16u); }
+size_t p2_lds_bytes(uint32_t kw, uint32_t cap, uint32_t filt) {
+  return (size_t)filt * kP2FiltWords * 4u + (size_t)cap * (kw * 8u + 4u + 16u);
+}
 
 // Calls F<L, KW>() for the template pair of k.
 template <class F>
@@ -799,7 +1042,7 @@ struct P2Launch {
   hipEvent_t gated, scattered;
   template <int L, int KW>
   hipError_t operator()() {
-    if (read_threads) {
+    if (read_threads && !a.ungated) {
       const unsigned ggrid = (unsigned)((read_threads + kP2GateBlock - 1) / kP2GateBlock);
       for (uint32_t row = 0; row < 3; ++row)
         hipLaunchKernelGGL((k_p2_gate<L>), dim3(ggrid), dim3(kP2GateBlock), 0, s, a, row);
@@ -831,6 +1074,12 @@ struct P2Launch {
       hipLaunchKernelGGL((k_p2_bins<L, KW>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, ar);
       if (hipError_t e = hipGetLastError()) return e;
     }
+    if (a.ungated) {
+      const uint64_t g = (a.table.cap + 255) / 256;
+      hipLaunchKernelGGL((k_p2_spill_gate<L, KW>), dim3((unsigned)(g < 4096 ? (g ? g : 1) : 4096)), dim3(256), 0, s,
+                         a);
+      if (hipError_t e = hipGetLastError()) return e;
+    }
     return hipSuccess;
   }
 };
@@ -838,9 +1087,11 @@ struct P2Launch {
 hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated, hipEvent_t scattered) {
   const uint32_t KW = (a.k + 30) / 31;
   const size_t scat_lds = p2_scatter_lds(a.nbins);
-  const size_t lds = p2_lds_bytes(KW, a.lds_cap);
+  const size_t lds = p2_lds_bytes(KW, a.lds_cap, a.filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (a.cflag > 1u || (a.contigs.n_reads && !a.cflag)) return hipErrorInvalidValue;
+  if (a.filt > kP2MaxFilt || (a.ungated && (!a.solid_bits || a.t_solid > a.min_trans_cov)))
+    return hipErrorInvalidValue;
   if (a.rw != (2 * a.k + 8 + a.cflag + 63) / 64) return hipErrorInvalidValue;
   const uint64_t read_threads = a.reads.n_reads * (uint64_t)a.split;
   const uint64_t threads = read_threads + a.contigs.n_reads * (uint64_t)a.csplit;

@@ -833,6 +833,7 @@ struct fb_solid {
   // pass-2 rounds
   uint64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool sorted = false;  // records sorted by k-mer (the parity form)
+  int p2_gate = 0;      // as fb_solid_pass2_gate
 };
 
 extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
@@ -898,6 +899,44 @@ static int solid_out_check(fb_ctx* c, const DBuf& keys, const DBuf& cov, const D
                  "internal: solid output buffers (%llu/%llu/%llu bytes) cannot hold %llu entries of %u key words",
                  (unsigned long long)have[0], (unsigned long long)have[1], (unsigned long long)have[2],
                  (unsigned long long)cap, nref);
+}
+
+// Keys the ungated pass-2 tables hold: keys with at least f windows (the
+// kept keys, plus the record filter's few false positives).  Row-0 cells
+// whose count reaches T hold them, where T = max(f, a quarter of the mean
+// count of the cells that pass the gate) leaves out the cells that collect
+// only error k-mers (on the 5 Mbp isolate at k 21 the mean is about 36);
+// linear counting over those cells (at least a quarter of the count at f),
+// plus a tenth.  hist[c]: row-0 cells with
+// min(count, 255) = c.  The estimate only sizes the tables: keys beyond it
+// go to the global table or, past the output buffer, to the global route.
+double fb::est_kept_keys(const uint32_t* hist, uint64_t width, uint64_t windows, uint32_t t, uint32_t f) {
+  uint64_t occ_t = 0;
+  double sum_t = 0.0;
+  for (uint32_t c = std::max(std::min(t, 255u), 1u); c < 256; ++c) {
+    occ_t += hist[c];
+    sum_t += (double)c * hist[c];
+  }
+  const double mean = occ_t ? sum_t / (double)occ_t : 0.0;
+  const double W = (double)width;
+  auto linear = [&](uint32_t T) {  // distinct keys in cells with count >= T
+    uint64_t occ = 0;
+    for (uint32_t c = T; c < 256; ++c) occ += hist[c];
+    return occ >= width ? (double)windows : -W * log(1.0 - (double)occ / W);
+  };
+  // with f = 1 (edge gate 1) every key with a record is kept: all of them count
+  const uint32_t fl = std::max(f, 1u);
+  const uint32_t T = fl == 1u ? 1u : std::min<uint32_t>(255u, std::max<uint32_t>(fl, (uint32_t)(mean / 4.0)));
+  const double n = std::max(linear(T), 0.25 * linear(fl));
+  return std::min(1.1 * n, (double)windows + 1.0);
+}
+// Distinct keys of the pass (linear counting over the occupied row-0 cells).
+static double est_distinct_keys(const uint32_t* hist, uint64_t width, uint64_t windows) {
+  uint64_t occ = 0;
+  for (uint32_t c = 1; c < 256; ++c) occ += hist[c];
+  const double W = (double)width;
+  const double n = occ >= width ? (double)windows : -W * log(1.0 - (double)occ / W);
+  return std::min(1.1 * n, (double)windows + 1.0);
 }
 
 extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* contigs, const uint32_t* contig_lens,
@@ -967,8 +1006,11 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   uint32_t sb = kBinLog2Cells;
   while (sb > 6 && (1ull << sb) > ncnt) --sb;
   const uint64_t nbins = std::max<uint64_t>(1, ncnt >> sb);
-  const bool binned = prm->placement == 2 || (prm->placement == 0 && nbins <= 8192 && windows >= (1u << 20));
-  if (prm->placement == 2 && nbins > 8192)
+  // placement 3: the LDS route with the per-window pass-2 gate (the ungated
+  // form is the default wherever it applies)
+  const bool lds_placement = prm->placement == 2 || prm->placement == 3;
+  const bool binned = lds_placement || (prm->placement == 0 && nbins <= 8192 && windows >= (1u << 20));
+  if (lds_placement && nbins > 8192)
     return fail(set_err(c, FB_ERR_INVALID, "LDS placement needs at most 8192 bins (width %llu)",
                         (unsigned long long)so->cms.width));
   DBuf bits, bdata, bcur;
@@ -1035,22 +1077,32 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   // Capacity from the sketch itself: occ = row-0 cells whose count passes the
   // gate; linear counting N = -W ln(1 - occ/W) estimates the distinct k-mers
   // landing in such cells, an upper-side estimate of the keys pass 2 inserts.
+  // The ungated pass-2 form (records of every window pair, the gate applied
+  // per kept key) needs the counters' cap >= t_solid, so that a k-mer with
+  // t_solid windows passes the gate, and with prior contigs t_solid <= 10, so
+  // that every contig k-mer (ten adds per window) does.
+  const bool ungated = prm->placement != 3 && t_solid <= so->cms.cap() && (c_reads == 0 || t_solid <= 10);
+  const uint32_t filt = ungated && gate >= 2 ? std::min(gate, kP2MaxFilt) : 0u;
   uint64_t cap;
-  double n_est_keys = 0.0;
+  double n_est_keys = 0.0, n_distinct = 0.0;
   {
     DBuf pc;
-    if (int r = pc.get(c, 8)) return fail(r);
-    HIP_TRY(c, launch_fill_u32(pc.as<uint32_t>(), 2, 0, s));
+    if (int r = pc.get(c, 8 + 256 * 4)) return fail(r);
+    HIP_TRY(c, launch_fill_u32(pc.as<uint32_t>(), 2 + 256, 0, s));
     HIP_TRY(c, launch_popcount(bits.as<uint32_t>(), so->cms.width / 32 ? so->cms.width / 32 : 1,
                                pc.as<unsigned long long>(), s));
-    unsigned long long occ = 0;
-    HIP_TRY(c, hipMemcpyAsync(&occ, pc.p, 8, hipMemcpyDeviceToHost, s));
+    if (ungated) HIP_TRY(c, launch_count_hist(so->cms.counts.as<uint32_t>(), so->cms.width, pc.as<uint32_t>() + 2, s));
+    uint32_t hbuf[2 + 256];
+    HIP_TRY(c, hipMemcpyAsync(hbuf, pc.p, sizeof(hbuf), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
+    unsigned long long occ = 0;
+    memcpy(&occ, hbuf, 8);
     const double W = (double)so->cms.width;
     double n_est = occ >= so->cms.width ? (double)windows : -W * log(1.0 - (double)occ / W);
     n_est = std::min(n_est, (double)windows + 1.0);
     cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * n_est) + 1024));
-    n_est_keys = n_est;
+    n_est_keys = ungated ? est_kept_keys(hbuf + 2, so->cms.width, windows, t_solid, std::max(filt, 1u)) : n_est;
+    n_distinct = ungated ? est_distinct_keys(hbuf + 2, so->cms.width, windows) : n_est;
   }
   DBuf tbuf, status;
   if (int r = status.get(c, 16)) return fail(r);
@@ -1061,10 +1113,12 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   bool done = false;
   // LDS-privatised pass 2+3 (reads, then the prior contigs' windows)
   const uint32_t kw = tab.kw;
-  const uint32_t lds_cap = p2_lds_cap(kw);
+  const uint32_t lds_cap = p2_lds_cap(kw, filt);
   // bins over all rounds: the estimated keys at half the LDS table capacity
+  // (0.6 for the ungated form, whose estimate carries its own margin)
+  const double load = ungated ? 0.6 : 0.5;
   uint64_t nb2 = 1;
-  while ((double)nb2 * 0.5 * (double)lds_cap < n_est_keys) nb2 <<= 1;
+  while ((double)nb2 * load * (double)lds_cap < n_est_keys) nb2 <<= 1;
   const uint64_t max_bins = (uint64_t)p2_max_bins() << kP2MaxLog2Rounds;
   if (prm->bins2) {
     if (prm->bins2 > max_bins || (prm->bins2 & (prm->bins2 - 1)))
@@ -1095,8 +1149,12 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     const uint32_t cflag = c_reads ? 1u : 0u;
     const uint32_t rw = (2 * k + 8 + cflag + 63) / 64;
     // keys beyond what the LDS tables are expected to hold, plus slack
-    const double excess = std::max(0.0, n_est_keys - (double)nb2 * 0.5 * (double)lds_cap);
-    const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est_keys / 4)));
+    const double excess = std::max(0.0, n_est_keys - (double)nb2 * load * (double)lds_cap);
+    // fewer ungated bins than the estimate asks for (an explicit bins2) also
+    // overload their record filters, which then let most keys through: the
+    // spill table is sized for all of them
+    const double spilled = ungated && excess > 0.0 ? n_distinct : 2.0 * excess + n_est_keys / 4;
+    const uint64_t spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)spilled));
     // solid keys from the bins: at most the keys they hold; sized from the
     // estimate, with a count past the end sending the build to the global route
     const uint64_t lds_out = std::min<uint64_t>(nb2 * lds_cap, (uint64_t)(1.25 * n_est_keys) + 65536);
@@ -1107,7 +1165,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     int r = 0;
     if (!r) r = bh.get(c, std::max<uint64_t>(n_blocks2 * nbr, 1) * 4);
     if (!r) r = recs.get(c, rec_cap * rw * 8);
-    if (!r) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
+    if (!r && !ungated) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
     if (!r) r = runs.get(c, (nbr * kBinSlices + 1) * 4);
     if (!r) r = st2.get(c, 64);
     if (!r) r = p3s.get(c, pass3_scratch_bytes());
@@ -1147,6 +1205,9 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.stats = reinterpret_cast<unsigned long long*>(st2.as<uint32_t>() + 4);
     pa.lds_cap = lds_cap;
     pa.min_trans_cov = gate;
+    pa.ungated = ungated ? 1u : 0u;
+    pa.t_solid = t_solid;
+    pa.filt = filt;
     pa.table = tab;
     pa.nref = so->nref;
     pa.out_keys = so->keys.as<uint64_t>();
@@ -1176,6 +1237,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
       route = 1;
       so->stats[8] = hb[1];
       so->stats[10] = 1ull << log2_rounds;
+      so->p2_gate = ungated ? 1 : 0;
       done = true;
     } else {
       route = hb[0] ? 2 : (ts[0] ? 3 : 4);
@@ -1311,6 +1373,7 @@ extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[11]) {
   memcpy(out, s->stats, sizeof s->stats);
   return FB_OK;
 }
+extern "C" int fb_solid_pass2_gate(const fb_solid* s) { return s ? s->p2_gate : FB_ERR_INVALID; }
 
 // ---------------------------------------------------------------------------
 // Genome estimator

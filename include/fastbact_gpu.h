@@ -187,7 +187,9 @@ typedef struct {
   int32_t hll_p;              /* 0 = no per-pass HLL; else precision of the pass HLL */
   uint32_t sort;              /* 1 = records sorted by k-mer (parity form) */
   uint32_t split;             /* threads per read; 0 = tuned default */
-  uint32_t placement;         /* sketch and solid-table placement: 0 auto, 1 global tables, 2 LDS bins */
+  uint32_t placement;         /* sketch and solid-table placement: 0 auto, 1 global tables, 2 LDS bins,
+                                 3 LDS bins with the pass-2 gate applied per window (2 applies it per
+                                 kept key wherever the counters' cap allows: the same result) */
   uint32_t bins2;             /* pass-2 LDS bins over all rounds (power of two <= 2^20); 0 = sized from the sketch */
   uint32_t chunks1;           /* thread shares per pass-1 scatter thread; 0 = tuned default */
   uint32_t chunks2;           /* thread shares per pass-2 count/scatter thread; 0 = tuned default */
@@ -232,6 +234,13 @@ int fb_solid_kernel_timings(const fb_solid* s, float out[5]);
  * LDS route splits the bins into rounds of at most 16384; 0 on another
  * route). */
 int fb_solid_stats(const fb_solid* s, uint64_t out[11]);
+/* Where the pass-2 Count-Min gate was applied: 0 per window pair (the
+ * global route, or the LDS route with placement 3), 1 once per kept key and
+ * low edge (the ungated LDS route: records of every window pair, the mask
+ * and kept keys do not depend on the gate, cov's low edges are probed per
+ * key).  Both give the reference's set.  Diagnostics, no counterpart in
+ * the reference. */
+int fb_solid_pass2_gate(const fb_solid* s);
 
 /* ---- Unitig compaction: build_cdbg_with_contigs_ext step 4-6 (build.rs:330-597)
  * Walks from boundary directed k-mers through 1-in/1-out k-mers, isolated

@@ -6,8 +6,10 @@ u16 counters, HLL p = 16 -- the workload whose throughput bench.py reports.
 The oracle needs minutes per pass at this size, so the full isolate is
 checked through properties that do not depend on its size, and a 300k-pair
 prefix of the same reads against the oracle itself:
-- the LDS route (binned pass 1, LDS tables in pass 2) and the global route
-  (global atomics, global hash table) give the same CMS, HLL and solid set;
+- the LDS route (binned pass 1, LDS tables in pass 2; its pass-2 gate
+  applied once per kept key and low edge, and per window pair as placement
+  3) and the global route (global atomics, global hash table, gate per
+  window pair) give the same CMS, HLL and solid set;
 - the set is sorted by key and free of duplicates;
 - a 4-bit run's table equals min(u16 counter, 15) cell by cell in the
   reference's nibble layout, with the same solid set (gates <= 15);
@@ -67,6 +69,9 @@ def test_routes_agree_sorted_unique(ctx, full, k, tgt):
     assert 4_000_000 < len(lds) < 20_000_000
     assert np.array_equal(lds.keys, glb.keys)
     assert np.array_equal(lds.cov, glb.cov) and np.array_equal(lds.mask, glb.mask)
+    win = _build(ctx, dev, k, tgt, placement=3, with_cms=False)
+    assert np.array_equal(win.keys, lds.keys)
+    assert np.array_equal(win.cov, lds.cov) and np.array_equal(win.mask, lds.mask)
     assert np.array_equal(lds.cms_table, glb.cms_table)
     assert np.array_equal(lds.hll_registers, glb.hll_registers)
     assert _sorted_unique(lds.keys)

@@ -24,7 +24,7 @@ def small_iso():
     return isolate(seed=21, genome_len=30_000, n_pairs=2500)
 
 
-@pytest.mark.parametrize("placement", [1, 2])
+@pytest.mark.parametrize("placement", [1, 2, 3])
 @pytest.mark.parametrize("k", [21, 31, 32, 33, 55, 63, 64, 65, 77, 99, 127])
 def test_build_solid_matches_oracle_all_widths(ctx, orc, small_iso, k, placement):
     bank = small_iso.bank()
@@ -100,15 +100,17 @@ def test_prior_contigs_both_pass2_routes(ctx, orc, small_iso, k):
     contigs = ([gen[0:9000], gen[9000:9000 + k], gen[9100:9100 + k + 1], gen[15000:15000 + 3 * k]]
                + [gen[20000:20600]] * 70 + [gen[22000:22050 + k]] * 400)
     r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=4, width=1 << 18, contigs=contigs)
-    for placement, bins2 in ((1, 0), (2, 0), (2, 2)):
+    for placement, bins2 in ((1, 0), (2, 0), (2, 2), (3, 0), (3, 2)):
         g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=4, cms_width=1 << 18, contigs=contigs,
                                 placement=placement, bins2=bins2)
         _solid_equal(g, r)
         h = F.build_solid_handle(ctx, dev, k, target_solid_cov=4, cms_width=1 << 18, contigs=contigs,
                                  placement=placement, bins2=bins2)
         st = h.stats()
+        gate = h.pass2_gate()
         h.close()
-        assert st["pass2_route"] == ("lds" if placement == 2 else "global"), st["pass2_route"]
+        assert st["pass2_route"] == ("global" if placement == 1 else "lds"), st["pass2_route"]
+        assert gate == ("key" if placement == 2 else "window"), gate
     assert (r["cov"] == 65535 // 2).any()  # saturated cov of the contig repeated 400 times
 
 
@@ -356,8 +358,9 @@ def test_lds_table_full_spills_to_global_table(ctx, orc, k, bins2):
     assert st["keys"] > 6000 * bins2  # more keys than the LDS tables hold
 
 
-@pytest.mark.parametrize("placement", [1, 2])
+@pytest.mark.parametrize("placement", [1, 3])
 def test_pass2_placements_agree_on_stats(ctx, small_iso, placement):
+    """The per-window gate gives the same records and keys on both routes."""
     dev = F.DeviceBank(ctx, small_iso.bank())
     h = F.build_solid_handle(ctx, dev, 31, target_solid_cov=3, cms_width=1 << 18, placement=placement)
     st = h.stats()
@@ -366,6 +369,34 @@ def test_pass2_placements_agree_on_stats(ctx, small_iso, placement):
     st1 = h1.stats()
     h1.close()
     assert st["records"] == st1["records"] and st["keys"] == st1["keys"] and st["windows"] == st1["windows"]
+
+
+@pytest.mark.parametrize("k,target,bits,width", [(21, None, 16, 1 << 22), (21, 2, 4, 1 << 20), (31, 4, 16, 1 << 21),
+                                                  (55, 4, 4, 1 << 22), (77, None, 16, 1 << 20), (127, 3, 16, 1 << 22),
+                                                  (33, 1, 16, 1 << 20), (21, 16, 4, 1 << 18)])
+def test_gate_per_kept_key_equals_oracle(ctx, orc, k, target, bits, width):
+    """A sketch far wider than the input leaves most error k-mers below the
+    gate, so many kept keys have low edges whose neighbour fails it: the
+    ungated route (records of every pair, the gate probed per kept key and
+    low edge, error k-mers dropped by the bin kernel's record filter) must
+    still give the oracle's cov, mask and key set.  Target 16 with nibble
+    counters (cap 15 < t) takes the per-window gate (nothing is solid), and
+    target 1 has no record filter."""
+    iso = isolate(seed=100 + k, genome_len=50_000, n_pairs=4000, sub_rate=0.01)
+    bank = iso.bank()
+    dev = F.DeviceBank(ctx, bank)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=target, width=width, counter_bits=bits)
+    for placement in (2, 3):
+        h = F.build_solid_handle(ctx, dev, k, target_solid_cov=target, cms_width=width, counter_bits=bits,
+                                 placement=placement)
+        gate, st = h.pass2_gate(), h.stats()
+        g = h.download(k, counter_bits=bits)
+        h.close()
+        _solid_equal(g, r)
+        assert gate == ("key" if placement == 2 and (target is None or target <= (15 if bits == 4 else 255))
+                        else "window"), (placement, gate, st, len(r["cov"]))
+    if target != 16:
+        assert len(r["cov"]) > 4000
 
 
 @pytest.mark.parametrize("k,bins2", [(21, 1 << 15), (55, 1 << 16), (127, 1 << 18)])
@@ -387,4 +418,13 @@ def test_pass2_rounds_match_oracle(ctx, orc, small_iso, k, bins2):
     assert st["pass2_route"] == "lds" and st["pass2_rounds"] == bins2 // 16384
     assert st1["pass2_rounds"] == 1
     # every round writes its own records: together they are the one-round records
-    assert st["pass2_records"] == st1["pass2_records"] and st["records"] == st1["records"]
+    assert st["pass2_records"] == st1["pass2_records"]
+    # with the per-window gate the tables hold the same keys in any number of
+    # rounds (the ungated route's record filter lets through keys by how they
+    # share bins, so its table contents vary with the bin count)
+    raw = []
+    for b2 in (bins2, 0):
+        h3 = F.build_solid_handle(ctx, dev, k, target_solid_cov=2, cms_width=1 << 18, placement=3, bins2=b2)
+        raw.append(h3.stats()["records"])
+        h3.close()
+    assert raw[0] == raw[1]

@@ -246,9 +246,12 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated 
 // isolate's run counters before each of the max_rounds rounds (an isolate
 // with fewer rounds skips the later ones).
 struct FillRange;
+// lds_cap, filt: the isolates' LDS table entries and record-filter levels
+// (one value for all of them); the spill tables of ungated isolates get
+// their gate correction after the last round.
 hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
                                   const FillRange* runs_fill, uint32_t n_fill, uint32_t k, uint32_t max_nbins,
-                                  uint32_t max_rounds, uint32_t lds_cap, hipStream_t s);
+                                  uint32_t max_rounds, uint32_t lds_cap, uint32_t filt, hipStream_t s);
 constexpr uint32_t kP2MaxLog2Rounds = 6;  // at most 64 pass-2 rounds
 
 // Multi-table forms for the batched pass (blockIdx.y = isolate).

@@ -1133,18 +1133,23 @@ struct P2LaunchMulti {
                            static_cast<const P2BinArgs*>(bins.args), bins.blk, bins.n_iso, round);
       if (hipError_t e = hipGetLastError()) return e;
     }
+    if (bins.n_iso) {  // the spill tables' gate correction (each ungated isolate's)
+      hipLaunchKernelGGL((k_p2_spill_gate_m<L, KW>), dim3(256, bins.n_iso), dim3(256), 0, s,
+                         static_cast<const P2BinArgs*>(bins.args));
+      if (hipError_t e = hipGetLastError()) return e;
+    }
     return hipSuccess;
   }
 };
 
 hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& scatter, const MultiLaunch& bins,
                                   const FillRange* runs_fill, uint32_t n_fill, uint32_t k, uint32_t max_nbins,
-                                  uint32_t max_rounds, uint32_t lds_cap, hipStream_t s) {
+                                  uint32_t max_rounds, uint32_t lds_cap, uint32_t filt, hipStream_t s) {
   const uint32_t KW = (k + 30) / 31;
   const size_t scat_lds = p2_scatter_lds(max_nbins);
-  const size_t lds = p2_lds_bytes(KW, lds_cap);
+  const size_t lds = p2_lds_bytes(KW, lds_cap, filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
-  if (max_rounds == 0 || max_rounds > (1u << kP2MaxLog2Rounds)) return hipErrorInvalidValue;
+  if (max_rounds == 0 || max_rounds > (1u << kP2MaxLog2Rounds) || filt > kP2MaxFilt) return hipErrorInvalidValue;
   return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, runs_fill, n_fill, max_rounds, (size_t)max_nbins * 4,
                                        scat_lds, lds, s});
 }

@@ -1551,7 +1551,7 @@ struct fb_batch {
   std::vector<uint32_t> lds_idx;
   DBuf d_p2, d_p3, d_tabs, d_fill2, d_runs_fill, d_blkg, d_blks, d_blkb;
   MultiLaunch mg, ms2, mb;
-  uint32_t n_fill2 = 0, max_nbr = 1, max_rounds = 1, lds_cap = 0;
+  uint32_t n_fill2 = 0, max_nbr = 1, max_rounds = 1, lds_cap = 0, filt = 0;
   // A graph reads the per-isolate pointers from the argument arrays, which
   // are re-uploaded before every launch; what it holds fixed are the launch
   // shapes, recorded here per k.
@@ -1593,11 +1593,16 @@ struct P2Plan {
   bool ok = false;
 };
 // lds_cap: LDS table entries of one bin; round_bins: bins one round takes.
+// load: the LDS tables' target fill at the estimate (0.5; 0.6 for the
+// ungated form, whose estimate has a margin of its own); n_distinct (the
+// ungated form): the spill table's size when bins2 forces fewer bins than
+// the estimate asks for, since their record filters then let most keys in.
 static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint32_t max_len, uint32_t k,
-                         uint32_t split, uint32_t bins2, uint32_t lds_cap, uint32_t round_bins, uint32_t chunks2) {
+                         uint32_t split, uint32_t bins2, uint32_t lds_cap, uint32_t round_bins, uint32_t chunks2,
+                         double load = 0.5, double n_distinct = 0.0) {
   P2Plan q;
   q.lds_cap = lds_cap;
-  while ((double)q.nb2 * 0.5 * (double)q.lds_cap < n_est) q.nb2 <<= 1;
+  while ((double)q.nb2 * load * (double)q.lds_cap < n_est) q.nb2 <<= 1;
   const uint64_t max_bins = (uint64_t)round_bins << kP2MaxLog2Rounds;
   if (bins2) q.nb2 = bins2;
   while ((q.nb2 >> q.log2_rounds) > round_bins) ++q.log2_rounds;
@@ -1608,8 +1613,9 @@ static P2Plan plan_pass2(double n_est, uint64_t windows, uint64_t n_reads, uint3
   q.rec_cap = windows + q.threads2;
   q.chunks2 = chunks2 ? chunks2 : scatter_chunks(q.threads2);
   q.n_blocks2 = scatter_blocks(q.threads2, q.chunks2);
-  const double excess = std::max(0.0, n_est - (double)q.nb2 * 0.5 * (double)q.lds_cap);
-  q.spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * excess + n_est / 4)));
+  const double excess = std::max(0.0, n_est - (double)q.nb2 * load * (double)q.lds_cap);
+  const double spilled = n_distinct > 0.0 && excess > 0.0 ? n_distinct : 2.0 * excess + n_est / 4;
+  q.spill_cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)spilled));
   const uint64_t lds_out = std::min<uint64_t>(q.nb2 * q.lds_cap, (uint64_t)(1.25 * n_est) + 65536);
   q.out_cap = std::min<uint64_t>(lds_out + q.spill_cap, 0xFFFFFFFFull);
   q.ok = q.nb2 <= max_bins && windows > 0 && q.rec_cap < (1ull << 32);
@@ -1715,7 +1721,7 @@ static int batch_iso_init(fb_ctx* c, BatchIso* it, const fb_bank* bank, uint64_t
     it->has_hll = true;
   }
   if (int r = it->bits.get(c, ((it->cms.width * 4 + 31) / 32) * 4)) return r;
-  if (int r = it->occ.get(c, 8)) return r;
+  if (int r = it->occ.get(c, 8 + 256 * 4)) return r;  // gate occupancy (u64), row-0 count histogram
   if (int r = it->p3.get(c, pass3_scratch_bytes())) return r;
   if (int r = it->st2.get(c, 64)) return r;
   return FB_OK;
@@ -1901,7 +1907,7 @@ static int batch_prepare1(fb_batch* b, uint32_t k, uint32_t t_solid, bool* chang
     }
     DevReads rv = it.bank->view();
     fills.push_back({it.cms.counts.as<uint32_t>(), ncnt, 0u});
-    fills.push_back({it.occ.as<uint32_t>(), 2, 0u});
+    fills.push_back({it.occ.as<uint32_t>(), 2 + 256, 0u});
     if (it.has_hll) fills.push_back({it.hll.regs.as<uint32_t>(), (uint64_t)1 << it.hll.p, 0u});
     if (binned) fills.push_back({it.bcur.as<uint32_t>(), nbins * kBinSlices, 0u});
     Pass1Args& a1 = p1[i];
@@ -1965,14 +1971,19 @@ static int batch_enqueue1(fb_batch* b, uint32_t k, uint32_t t_solid) {
       HIP_TRY(c, launch_solid_bits(it->cms.counts.as<uint32_t>(), it->cms.width * 4, t_solid, it->cms.cap(),
                                    it->bits.as<uint32_t>(), s));
   }
-  for (auto& it : b->iso)
+  for (auto& it : b->iso) {
     HIP_TRY(c, launch_popcount(it->bits.as<uint32_t>(), it->cms.width / 32 ? it->cms.width / 32 : 1,
                                it->occ.as<unsigned long long>(), s));
+    HIP_TRY(c, launch_count_hist(it->cms.counts.as<uint32_t>(), it->cms.width, it->occ.as<uint32_t>() + 2, s));
+  }
   return FB_OK;
 }
 
 // Pass-2 layout from the gate occupancies; *changed as for pass 1.
-static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vector<uint64_t>& occ, bool* changed) {
+// occ: per isolate the gate occupancy (u64) and the row-0 count histogram
+// (256 u32), as graph 1 leaves them.
+static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t gate, const std::vector<uint32_t>& occ,
+                          bool* changed) {
   fb_ctx* c = b->ctx;
   const fb_build_params prm = batch_params(b, k);
   const uint32_t n = (uint32_t)b->iso.size();
@@ -1986,11 +1997,23 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
   std::vector<uint32_t> blkg(1, 0), blks(1, 0), blkb(1, 0);
   b->max_nbr = 1;
   b->max_rounds = 1;
+  // the ungated pass-2 form wherever it applies (as fb_build_solid; the
+  // counters' cap is the batch's)
+  const uint32_t cap_value = prm.counter_bits == 4 ? 15u : 65535u;
+  const bool ungated = prm.placement != 3 && t_solid <= cap_value;
+  const uint32_t filt = ungated && gate >= 2 ? std::min(gate, kP2MaxFilt) : 0u;
+  b->filt = filt;
   for (uint32_t i = 0; i < n; ++i) {
     BatchIso& it = *b->iso[i];
-    it.est_keys = est_keys_from_occ(occ[i], it.cms.width, it.windows);
+    const uint32_t* oi = occ.data() + (size_t)i * (2 + 256);
+    uint64_t occ_i = 0;
+    memcpy(&occ_i, oi, 8);
+    it.est_keys = ungated ? est_kept_keys(oi + 2, it.cms.width, it.windows, t_solid, std::max(filt, 1u))
+                          : est_keys_from_occ(occ_i, it.cms.width, it.windows);
+    const double n_distinct = ungated ? est_distinct_keys(oi + 2, it.cms.width, it.windows) : it.est_keys;
     const P2Plan q = plan_pass2(it.est_keys, it.windows, it.bank->n_reads, it.bank->max_len, k, it.split,
-                                prm.bins2, p2_lds_cap(kw), p2_max_bins(), prm.chunks2);
+                                prm.bins2, p2_lds_cap(kw, filt), p2_max_bins(), prm.chunks2, ungated ? 0.6 : 0.5,
+                                ungated ? n_distinct : 0.0);
     const bool lds = q.ok && prm.placement != 1;
     it.lds = lds;
     if (!lds) continue;
@@ -2004,7 +2027,7 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     b->lds_cap = q.lds_cap;
     auto grow = [&](DBuf& d, uint64_t bytes) -> int { return (d.p && d.bytes >= bytes) ? FB_OK : d.get(c, bytes); };
     int r = grow(it.scratch, q.rec_cap * rw * 8);
-    if (!r) r = grow(it.gm, std::max<uint64_t>(q.threads2, 1) * 8);
+    if (!r && !ungated) r = grow(it.gm, std::max<uint64_t>(q.threads2, 1) * 8);
     if (!r) r = grow(it.bh, std::max<uint64_t>(q.n_blocks2 * q.nbr, 1) * 4);
     if (!r) r = grow(it.runs, (q.nbr * kBinSlices + 1) * 4);
     if (!r) r = grow(it.slots, q.spill_cap * stride * 8);
@@ -2032,11 +2055,14 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     pa.nbins = (uint32_t)it.nbr;
     pa.log2_nbins = (uint32_t)__builtin_ctzll(it.nbr);
     pa.log2_rounds = it.log2_rounds;
-    pa.gate_masks = it.gm.as<uint64_t>();
+    pa.gate_masks = ungated ? nullptr : it.gm.as<uint64_t>();
     pa.rw = rw;
     pa.runs = it.runs.as<uint32_t>();
     pa.recs = it.scratch.as<uint64_t>();
     pa.rec_cap = it.rec_cap;
+    pa.ungated = ungated ? 1u : 0u;
+    pa.t_solid = t_solid;
+    pa.filt = filt;
     pa.status = it.st2.as<uint32_t>();          // [0] record overflow, [1] records
     pa.out_count = it.st2.as<uint32_t>() + 2;   // [2] keys emitted from LDS
     pa.stats = reinterpret_cast<unsigned long long*>(it.st2.as<uint32_t>() + 4);  // [4..7]
@@ -2064,7 +2090,8 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
     fill2.push_back({it.st2.as<uint32_t>(), 8, 0u});  // [8..9] are cleared with the table
     runs_fill.push_back({it.runs.as<uint32_t>(), it.nbr * kBinSlices + 1, 0u});
     const uint64_t thr2 = it.bank->n_reads * (uint64_t)it.split2;
-    blkg.push_back(blkg.back() + (uint32_t)((thr2 + kP2GateBlockSize - 1) / kP2GateBlockSize));
+    // the gate kernels take no blocks of an ungated isolate
+    blkg.push_back(blkg.back() + (ungated ? 0u : (uint32_t)((thr2 + kP2GateBlockSize - 1) / kP2GateBlockSize)));
     blks.push_back(blks.back() + (uint32_t)it.n_blocks2);
     blkb.push_back(blkb.back() + (uint32_t)it.nbr);
     b->max_nbr = std::max<uint32_t>(b->max_nbr, (uint32_t)it.nbr);
@@ -2075,7 +2102,8 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t gate, const std::vec
   const uint32_t m = (uint32_t)lds_idx.size();
   b->n_fill2 = m;
   b->nref = nref;
-  const std::vector<uint64_t> sig = {m, blkg.back(), blks.back(), blkb.back(), b->max_nbr, b->max_rounds, b->lds_cap};
+  const std::vector<uint64_t> sig = {m,         blkg.back(),   blks.back(), blkb.back(),
+                                     b->max_nbr, b->max_rounds, b->lds_cap,  b->filt};
   if (b->sig2[k] != sig) *changed = true;
   b->sig2[k] = sig;
   if (!m) return FB_OK;
@@ -2103,7 +2131,7 @@ static int batch_enqueue2(fb_batch* b, uint32_t k) {
   HIP_TRY(c, launch_fill_ranges(b->d_fill2.as<FillRange>(), m, s));
   HIP_TRY(c, launch_table_clear_multi(b->d_tabs.as<DevSolidTable>(), m, s));
   HIP_TRY(c, launch_p2_binned_multi(b->mg, b->ms2, b->mb, b->d_runs_fill.as<FillRange>(), m, k, b->max_nbr,
-                                    b->max_rounds, b->lds_cap, s));
+                                    b->max_rounds, b->lds_cap, b->filt, s));
   HIP_TRY(c, launch_pass3_multi(b->d_p3.as<P3Args>(), m, key_words_for_k(k), s));
   return FB_OK;
 }
@@ -2202,12 +2230,13 @@ extern "C" int fb_batch_run(fb_batch* b, uint32_t k, int32_t target) {
   bool changed = false;
   if (int r = batch_prepare1(b, k, t_solid, &changed)) return r;
   if (int r = run_graph(b, b->g1, b->g1_def, k, changed, [&] { return batch_enqueue1(b, k, t_solid); })) return r;
-  std::vector<uint64_t> occ(n, 0);
+  std::vector<uint32_t> occ((size_t)n * (2 + 256), 0);
   for (uint32_t i = 0; i < n; ++i)
-    HIP_TRY(c, hipMemcpyAsync(&occ[i], b->iso[i]->occ.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(occ.data() + (size_t)i * (2 + 256), b->iso[i]->occ.p, (2 + 256) * 4,
+                              hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   bool changed2 = false;
-  if (int r = batch_prepare2(b, k, gate, occ, &changed2)) return r;
+  if (int r = batch_prepare2(b, k, t_solid, gate, occ, &changed2)) return r;
   if (!b->lds_idx.empty())
     if (int r = run_graph(b, b->g2, b->g2_def, k, changed2, [&] { return batch_enqueue2(b, k); })) return r;
   HIP_TRY(c, hipEventRecord(c->ev[1], s));

@@ -156,6 +156,19 @@ __device__ __forceinline__ T lds_vload(const T* p) {
   return *(lds_ptr)(p);
 }
 
+// Four consecutive 64-bit LDS words from a 32-byte aligned p, fresh, in two
+// 16-byte reads.
+__device__ __forceinline__ void lds_vload_bucket(const unsigned long long* p, unsigned long long (&v)[4]) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  typedef __attribute__((address_space(3))) const volatile u64x2* lds_ptr;
+  const u64x2 a = *(lds_ptr)(p);
+  const u64x2 b = *((lds_ptr)(p) + 1);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+
 // Bits of every window of the share p0..p1 (p1 - p0 <= 63 pairs).
 __device__ __forceinline__ uint64_t all_windows(uint32_t p0, uint32_t p1) {
   return (p1 - p0 >= 63u) ? ~0ull : (2ull << (p1 - p0)) - 1ull;
@@ -683,51 +696,86 @@ __device__ __forceinline__ uint32_t lds_slot(const uint64_t (&w)[KW], uint32_t c
 }
 
 // Finds or claims the LDS entry of key within kLdsMaxProbe slots of its
-// home slot; -1 when all of them hold other keys.  Entries are written once,
-// so a key that finds its probe window full finds it full at every later
-// lookup too and lives in the global table for the whole pass, and a key
-// claimed in the window is found there by every later lookup.  The bound
-// keeps a crowded bin from scanning its whole table for every new key.
+// home bucket; -1 when all of them hold other keys.  Slots are probed in
+// aligned buckets of kLdsBucket: one 32-byte read brings a bucket's first
+// key words, so a wave's slowest lane needs few LDS round trips (probing
+// slot by slot, the longest of 64 probe runs set the wave's time).  Entries
+// are written once, so a key that finds its probe window full finds it
+// full at every later lookup too and lives in the global table for the
+// whole pass, and a key claimed in the window is found there by every
+// later lookup.  The bound keeps a crowded bin from scanning its whole
+// table for every new key.  cap is a multiple of kLdsBucket.
 constexpr uint32_t kLdsMaxProbe = 128;
+constexpr uint32_t kLdsBucket = 4;
 template <int KW>
 __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, const uint64_t (&w)[KW]) {
-  uint32_t s = lds_slot<KW>(w, cap);
-  const uint32_t probes = cap < kLdsMaxProbe ? cap : kLdsMaxProbe;
-  for (uint32_t probe = 0; probe < probes; ++probe) {
-    bool match = true;
+  uint32_t s = lds_slot<KW>(w, cap) & ~(kLdsBucket - 1u);
+  const uint32_t buckets = (cap < kLdsMaxProbe ? cap : kLdsMaxProbe) / kLdsBucket;
+  for (uint32_t b = 0; b < buckets; ++b) {
+    unsigned long long v0[kLdsBucket];
+    lds_vload_bucket(keys + s, v0);
 #pragma unroll
-    for (int j = 0; j < KW; ++j) {
-      if (!match) break;
-      unsigned long long* p = keys + (uint64_t)j * cap + s;
-      unsigned long long v = lds_vload(p);
-      if (v == kEmpty) {
-        const unsigned long long old = atomicCAS(p, (unsigned long long)kEmpty, (unsigned long long)w[j]);
-        v = old == kEmpty ? (unsigned long long)w[j] : old;
+    for (uint32_t i = 0; i < kLdsBucket; ++i) {
+      // another key's slot: its first word is written once and never changes
+      if (v0[i] != w[0] && v0[i] != kEmpty) continue;
+      // the key's slot or a free one: settled word by word, a free word
+      // claimed by compare-and-swap (which returns the winner's word)
+      bool match = true;
+#pragma unroll
+      for (int j = 0; j < KW; ++j) {
+        if (!match) break;
+        unsigned long long* p = keys + (uint64_t)j * cap + s + i;
+        unsigned long long v = j == 0 ? v0[i] : lds_vload(p);
+        if (v == kEmpty) {
+          const unsigned long long old = atomicCAS(p, (unsigned long long)kEmpty, (unsigned long long)w[j]);
+          v = old == kEmpty ? (unsigned long long)w[j] : old;
+        }
+        match = (v == w[j]);
       }
-      match = (v == w[j]);
+      if (match) return (int)(s + i);
     }
-    if (match) return (int)s;
-    if (++s == cap) s = 0;
+    s += kLdsBucket;
+    if (s >= cap) s = 0;
   }
   return -1;
 }
 
-// f(record) for records [0, n) of src, thread t taking t, t + blockDim, ...;
-// each thread has kRecInFlight independent record loads in flight (the
-// record loop is bound by their latency, not by the LDS work).
-constexpr int kRecInFlight = 4;
+// f(record) for records [0, n) of src, thread t taking t, t + blockDim, ...
+// The loop is software-pipelined: the loads of a thread's next group of
+// records go out before it works on the current group, so the load latency
+// (the loop's bound: its LDS work is short) overlaps that work.  Loads are
+// unconditional (indices clamped to the last record, L + 1 words each), so
+// the compiler waits for exactly the group it works on; src must be
+// readable one word past record n - 1 (the record buffers carry a pad).
 template <int L, class F>
 __device__ __forceinline__ void for_records(const uint64_t* __restrict__ src, uint32_t n, uint32_t rw, F&& f) {
-  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kRecInFlight * blockDim.x) {
-    uint64_t r[kRecInFlight][L + 1];
+  constexpr int U = L <= 2 ? 4 : 2;  // records per group (registers)
+  if (n == 0) return;
+  const uint32_t step = blockDim.x;
+  auto load_group = [&](uint32_t i0, uint64_t (&r)[U][L + 1]) {
 #pragma unroll
-    for (int u = 0; u < kRecInFlight; ++u) {
-      const uint32_t i = i0 + (uint32_t)u * blockDim.x;
-      if (i < n) load_record<L>(src + (uint64_t)i * rw, rw, r[u]);
+    for (int u = 0; u < U; ++u) {
+      uint32_t i = i0 + (uint32_t)u * step;
+      i = i < n ? i : n - 1u;
+      const uint64_t* p = src + (uint64_t)i * rw;
+#pragma unroll
+      for (int j = 0; j <= L; ++j) r[u][j] = p[j];
     }
+  };
+  uint64_t nx[U][L + 1];
+  load_group(threadIdx.x, nx);
+  for (uint32_t i0 = threadIdx.x; i0 < n; i0 += U * step) {
+    uint64_t cur[U][L + 1];
 #pragma unroll
-    for (int u = 0; u < kRecInFlight; ++u)
-      if (i0 + (uint32_t)u * blockDim.x < n) f(r[u]);
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int j = 0; j <= L; ++j) cur[u][j] = nx[u][j];
+    load_group(i0 + U * step, nx);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (rw == (uint32_t)L) cur[u][L] = 0ull;  // the word past a record of L words
+      if (i0 + (uint32_t)u * step < n) f(cur[u]);
+    }
   }
 }
 
@@ -1012,7 +1060,7 @@ uint32_t p2_max_bins() { return (lds_budget() - 1024u) / 4u >= 16384u ? 16384u :
 uint32_t p2_lds_cap(uint32_t kw, uint32_t filt) {
   // keys + cov + 8 u16 edges per entry within the workgroup's LDS, after the filter
   const uint32_t entry = kw * 8u + 4u + 16u;
-  return (lds_budget() - 1024u - filt * kP2FiltWords * 4u) / entry;
+  return ((lds_budget() - 1024u - filt * kP2FiltWords * 4u) / entry) & ~(kLdsBucket - 1u);
 }
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap, uint32_t filt) {
   return (size_t)filt * kP2FiltWords * 4u + (size_t)cap * (kw * 8u + 4u + 16u);
@@ -1089,6 +1137,7 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated,
   const size_t scat_lds = p2_scatter_lds(a.nbins);
   const size_t lds = p2_lds_bytes(KW, a.lds_cap, a.filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
+  if (a.lds_cap % kLdsBucket || a.lds_cap == 0) return hipErrorInvalidValue;
   if (a.cflag > 1u || (a.contigs.n_reads && !a.cflag)) return hipErrorInvalidValue;
   if (a.filt > kP2MaxFilt || (a.ungated && (!a.solid_bits || a.t_solid > a.min_trans_cov)))
     return hipErrorInvalidValue;
@@ -1150,6 +1199,7 @@ hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& sc
   const size_t lds = p2_lds_bytes(KW, lds_cap, filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (max_rounds == 0 || max_rounds > (1u << kP2MaxLog2Rounds) || filt > kP2MaxFilt) return hipErrorInvalidValue;
+  if (lds_cap % kLdsBucket || lds_cap == 0) return hipErrorInvalidValue;
   return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, runs_fill, n_fill, max_rounds, (size_t)max_nbins * 4,
                                        scat_lds, lds, s});
 }

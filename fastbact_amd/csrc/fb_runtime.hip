@@ -1164,7 +1164,7 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     DBuf recs, gm, runs, bh, st2, p3s;
     int r = 0;
     if (!r) r = bh.get(c, std::max<uint64_t>(n_blocks2 * nbr, 1) * 4);
-    if (!r) r = recs.get(c, rec_cap * rw * 8);
+    if (!r) r = recs.get(c, rec_cap * rw * 8 + 64);  // the bin kernel reads a word past the last record
     if (!r && !ungated) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
     if (!r) r = runs.get(c, (nbr * kBinSlices + 1) * 4);
     if (!r) r = st2.get(c, 64);
@@ -1626,7 +1626,7 @@ extern "C" int fb_plan_pass2(double est_keys, uint64_t windows, uint64_t n_reads
                              fb_pass2_plan* out) {
   if (!out || k < 1 || k > 127) return FB_ERR_INVALID;
   const uint32_t kw = key_words_for_k(k);
-  const uint32_t lds_cap = (kP2LdsBytes - 1024u) / (kw * 8u + 20u);
+  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / (kw * 8u + 20u)) & ~3u;  // whole 4-slot buckets
   const P2Plan q = plan_pass2(est_keys, windows, n_reads, max_len, k, default_split(k, max_len), 0, lds_cap, 16384u, 0);
   out->nb2 = q.nb2;
   out->nbr = q.nbr;
@@ -1685,7 +1685,7 @@ static IsoBytes iso_bytes(uint64_t genome_size, uint64_t bases, uint64_t reads, 
   const double n_est = (double)windows;
   // host-only sizing: the LDS of a workgroup with the full 160 KB granted
   const uint32_t kw = key_words_for_k(k);
-  const uint32_t lds_cap = (kP2LdsBytes - 1024u) / (kw * 8u + 20u);
+  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / (kw * 8u + 20u)) & ~3u;  // whole 4-slot buckets
   const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u,
                               p->chunks2);
   const uint32_t rw = (2 * k + 8 + 63) / 64;
@@ -1696,7 +1696,7 @@ static IsoBytes iso_bytes(uint64_t genome_size, uint64_t bases, uint64_t reads, 
   const uint64_t n_wg1 = scatter_blocks(thr1, p->chunks1 ? p->chunks1 : p1_chunks(thr1), kP1BlockSize);
   const uint64_t nbins1 = std::max<uint64_t>(1, (W * 4) >> kBinLog2Cells);
   const uint64_t bins = (uint64_t)(4.0 * (double)windows * 1.1 * 2) + 65536 + nbins1 * (n_wg1 + 16) * kP1Chunk * 2;
-  b.scratch = std::max<uint64_t>(bins, q.rec_cap * rw * 8);  // pass-1 bins, then pass-2 records
+  b.scratch = std::max<uint64_t>(bins, q.rec_cap * rw * 8 + 64);  // pass-1 bins, then pass-2 records
   b.masks = q.threads2 * 8;
   b.runs = q.n_blocks2 * q.nbr * 4;
   b.spill = q.spill_cap * table_stride(key_words_for_k(k)) * 8;
@@ -2026,7 +2026,7 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t ga
     it.rec_cap = q.rec_cap;
     b->lds_cap = q.lds_cap;
     auto grow = [&](DBuf& d, uint64_t bytes) -> int { return (d.p && d.bytes >= bytes) ? FB_OK : d.get(c, bytes); };
-    int r = grow(it.scratch, q.rec_cap * rw * 8);
+    int r = grow(it.scratch, q.rec_cap * rw * 8 + 64);  // the bin kernel reads a word past the last record
     if (!r && !ungated) r = grow(it.gm, std::max<uint64_t>(q.threads2, 1) * 8);
     if (!r) r = grow(it.bh, std::max<uint64_t>(q.n_blocks2 * q.nbr, 1) * 4);
     if (!r) r = grow(it.runs, (q.nbr * kBinSlices + 1) * 4);

@@ -1529,7 +1529,8 @@ struct BatchIso {
   bool lds = false;
   uint32_t split2 = 1, chunks2 = 1, log2_rounds = 0;
   uint64_t nb2 = 1, nbr = 1, n_blocks2 = 0, rec_cap = 0, spill_cap = 0, out_cap = 0;
-  double est_keys = 0.0;
+  double est_keys = 0.0;    // keys the LDS tables of pass 2 are sized for
+  double est_gated = 0.0;   // keys passing the per-window gate (the global route's table)
   // results of the last run
   uint64_t n_out = 0;
   uint32_t route = 0;  // as fb_solid_stats[4]
@@ -2008,8 +2009,9 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t ga
     const uint32_t* oi = occ.data() + (size_t)i * (2 + 256);
     uint64_t occ_i = 0;
     memcpy(&occ_i, oi, 8);
+    it.est_gated = est_keys_from_occ(occ_i, it.cms.width, it.windows);
     it.est_keys = ungated ? est_kept_keys(oi + 2, it.cms.width, it.windows, t_solid, std::max(filt, 1u))
-                          : est_keys_from_occ(occ_i, it.cms.width, it.windows);
+                          : it.est_gated;
     const double n_distinct = ungated ? est_distinct_keys(oi + 2, it.cms.width, it.windows) : it.est_keys;
     const P2Plan q = plan_pass2(it.est_keys, it.windows, it.bank->n_reads, it.bank->max_len, k, it.split,
                                 prm.bins2, p2_lds_cap(kw, filt), p2_max_bins(), prm.chunks2, ungated ? 0.6 : 0.5,
@@ -2171,7 +2173,7 @@ static int run_graph(fb_batch* b, std::map<uint32_t, hipGraphExec_t>& ge, std::m
 static int batch_global_route(fb_ctx* c, BatchIso& it, uint32_t k, uint32_t gate) {
   hipStream_t s = c->stream;
   const uint32_t kw = key_words_for_k(k), stride = table_stride(kw), nref = ref_words(k);
-  uint64_t cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * it.est_keys) + 1024));
+  uint64_t cap = next_pow2(std::max<uint64_t>(1ull << 16, (uint64_t)(2.0 * it.est_gated) + 1024));
   DBuf tbuf, status;
   if (int r = status.get(c, 16)) return r;
   DevSolidTable t;

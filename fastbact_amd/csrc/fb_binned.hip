@@ -326,8 +326,8 @@ size_t bin_buffer_lds(uint32_t nbins) { return (size_t)nbins * (3 * sizeof(uint3
 template <int L, bool HLL>
 __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, uint32_t* sh) {
   // windows per thread per round: as many as the walk state leaves room
-  // for in registers (8 for k <= 96, then 6)
-  constexpr int kBufRound = L <= 3 ? 8 : 6;
+  // for in registers (8 for k <= 64, 7 for k <= 96, then 6)
+  constexpr int kBufRound = L <= 2 ? 8 : (L == 3 ? 7 : 6);
   __shared__ uint32_t s_nfull;
   const uint32_t nb = a.nbins;
   uint32_t* cnt = sh;

@@ -151,27 +151,66 @@ __device__ __forceinline__ uint32_t base_at(const uint64_t* __restrict__ data, u
   return (uint32_t)(data[g >> 5] >> ((g & 31u) * 2u)) & 3u;
 }
 
-// Streams the bases of the packed arena from a start position, one word load
-// per 32 bases.
+// Streams the bases of the packed arena from a start position.  The three
+// words from the current base on (at least 65 bases) are loaded together,
+// and again after every 64 bases.  With a load at each word boundary, some
+// lane of a wave would cross one at almost every base and the whole wave
+// would wait for its load; lanes that walk shares of equal length (reads of
+// one length) instead reload at the same base, once per 64.  It reads up to
+// two words past the last base streamed: every device arena carries
+// kArenaPadWords zero words after its data.
 struct BaseStream {
-  const uint64_t* p;
-  uint64_t word;
-  uint32_t sh;
+  const uint64_t* p;  // the word holding the next base
+  uint64_t word, nxt, nxt2;
+  uint32_t sh, left;  // bit offset of the next base in *p; bases before the reload
+  __device__ __forceinline__ void load() {
+    word = p[0] >> sh;
+    nxt = p[1];
+    nxt2 = p[2];
+    left = 64u;
+  }
   __device__ __forceinline__ void init(const uint64_t* __restrict__ data, uint64_t g) {
     p = data + (g >> 5);
     sh = (uint32_t)(g & 31u) * 2u;
-    word = *p >> sh;
+    load();
   }
   __device__ __forceinline__ uint32_t next() {
-    uint32_t b = (uint32_t)word & 3u;
+    const uint32_t b = (uint32_t)word & 3u;
     sh += 2u;
-    if (sh == 64u) {
-      sh = 0u;
-      ++p;
-      word = *p;
-    } else {
-      word >>= 2;
-    }
+    const bool turn = sh == 64u;
+    word = turn ? nxt : word >> 2;
+    nxt = turn ? nxt2 : nxt;
+    p += turn ? 1 : 0;
+    sh = turn ? 0u : sh;
+    if (--left == 0u) load();
+    return b;
+  }
+};
+
+// At most 64 bases from a start position (a pass-2 share's bases after its
+// first window).  The three words they can span are loaded at init, so
+// next() never waits on memory: with a word load at the word boundary, as
+// BaseStream does, some lane of a wave crosses a word at almost every base
+// and the whole wave waits for its load.  It reads up to two words past
+// the last base streamed: every device arena carries kArenaPadWords zero
+// words after its data.
+struct ShortStream {
+  uint64_t word, nxt, nxt2;
+  uint32_t sh;
+  __device__ __forceinline__ void init(const uint64_t* __restrict__ data, uint64_t g) {
+    const uint64_t* p = data + (g >> 5);
+    sh = (uint32_t)(g & 31u) * 2u;
+    word = p[0] >> sh;
+    nxt = p[1];
+    nxt2 = p[2];
+  }
+  __device__ __forceinline__ uint32_t next() {
+    const uint32_t b = (uint32_t)word & 3u;
+    sh += 2u;
+    const bool turn = sh == 64u;
+    word = turn ? nxt : word >> 2;
+    nxt = turn ? nxt2 : nxt;
+    sh = turn ? 0u : sh;
     return b;
   }
 };

@@ -106,7 +106,7 @@ bool bin_buffered(uint32_t nbins);
 uint32_t bin_scatter_resident(uint32_t k, bool hll, uint32_t nbins);
 // mid (optional) is recorded between the scatter and count kernels.
 hipError_t launch_bin_pass1(const BinArgs& a, hipStream_t s, hipEvent_t mid = nullptr);
-constexpr uint32_t kScatterBlockSize = 512;  // block size of the pass-2 count and scatter kernels
+constexpr uint32_t kScatterBlockSize = 1024;  // block size of the pass-2 count and scatter kernels
 constexpr uint32_t kP1BlockSize = 512;       // block size of the pass-1 bin scatter kernel
 constexpr uint32_t kBinLog2Cells = 15;       // pass-1 bin: 2^15 Count-Min cells (128 KB of LDS counters)
 constexpr uint32_t kP1Chunk = 64;            // pass-1 offsets per reserved chunk: one 128-byte line
@@ -233,6 +233,11 @@ struct P2BinArgs {
 };
 constexpr uint32_t kP2FiltWords = 2048;  // u32 words per level of the ungated bin kernel's record filter
 constexpr uint32_t kP2MaxFilt = 3;       // filter levels: records counted up to 3 per position
+constexpr uint32_t kP2BufWords = 8;      // words of a bin's record buffer in the buffered pass-2 scatter
+constexpr uint32_t kP2BufMaxBins = 2048; // most bins a round of the buffered scatter holds in LDS
+// Whether phase A of pass 2 (k, nbins bins a round) writes records through
+// per-bin LDS buffers; else one store per record.
+bool p2_scatter_buffered(uint32_t k, uint32_t nbins);
 uint32_t p2_lds_cap(uint32_t kw, uint32_t filt = 0);
 uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap, uint32_t filt = 0);

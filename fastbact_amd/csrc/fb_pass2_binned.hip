@@ -330,7 +330,7 @@ __device__ __forceinline__ uint64_t gate_row(const P2BinArgs& a, uint64_t r, uin
   Kmer<L> win = window_bits<L>(data, g, k);
   Kmer<L> fwd = fwd_from_window<L>(win, k);
   Kmer<L> rev = rc_from_window<L>(win, k);
-  BaseStream in;
+  ShortStream in;  // p1 - p0 <= 63 bases
   in.init(data, g + k);
   const uint64_t wmask = (1ull << a.log2w) - 1ull;
   const uint64_t rbase = (uint64_t)row << a.log2w;
@@ -383,9 +383,8 @@ __device__ __forceinline__ void p2_walk(const P2BinArgs& a, const DevReads& R, u
   Kmer<L> win = window_bits<L>(data, g, k);
   Kmer<L> fwd = fwd_from_window<L>(win, k);
   Kmer<L> rev = rc_from_window<L>(win, k);
-  BaseStream in, first;
+  ShortStream in;  // p1 - p0 <= 63 bases
   in.init(data, g + k);
-  first.init(data, g);
   bool cur_rc = kmer_lt<L>(rev, fwd);
   Kmer<L> cur = cur_rc ? rev : fwd;
   bool cur_solid = gates & 1ull;
@@ -396,6 +395,8 @@ __device__ __forceinline__ void p2_walk(const P2BinArgs& a, const DevReads& R, u
     uint32_t b = 0;
     bool nxt_rc = false, nxt_solid = false;
     Kmer<L> nxt;
+    // window x's first base: the forward k-mer's most significant
+    const uint32_t first_base = (uint32_t)(fwd.a[L - 1] >> ((2u * k - 2u) & 63u)) & 3u;
     if (x < p1) {
       b = in.next();
       kmer_roll<L>(fwd, rev, b, k);
@@ -404,7 +405,6 @@ __device__ __forceinline__ void p2_walk(const P2BinArgs& a, const DevReads& R, u
       nxt_solid = (gates >> (x + 1 - p0)) & 1ull;
       right = cur_solid && nxt_solid;
     }
-    const uint32_t first_base = first.next();
     if (left || right) {
       // edge_v of pair (x-1, x) and edge_u of pair (x, x+1), build.rs:268-275
       const uint32_t e_left = cur_rc ? left_rcp : left_rcp + 4u;
@@ -444,7 +444,7 @@ __device__ __forceinline__ uint64_t gate3_count(const P2BinArgs& a, uint64_t r, 
   Kmer<L> win = window_bits<L>(data, g, k);
   Kmer<L> fwd = fwd_from_window<L>(win, k);
   Kmer<L> rev = rc_from_window<L>(win, k);
-  BaseStream in;
+  ShortStream in;  // p1 - p0 <= 63 bases
   in.init(data, g + k);
   const uint64_t wmask = (1ull << a.log2w) - 1ull;
   const uint64_t rbase = 3ull << a.log2w;
@@ -678,6 +678,214 @@ __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_m(const P2BinArg
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
   p2_scatter_body<L, KW>(args[iso], blockIdx.x - blk[iso], sh, round);
 }
+
+// The windows of a thread's shares (as p2_walk, one window per take), so
+// that a thread can stop after a few windows and resume after a barrier.
+// A window's record is settled before the window is rolled on: its right
+// pair depends only on the gate bits, and its right edge on the next base.
+template <int L>
+struct RecWalker {
+  uint64_t gid, step;  // next share to open and the distance between a thread's shares
+  uint32_t left;       // shares not yet opened
+  uint32_t i, last;    // window i of the open share (0..last); i > last: none open
+  uint64_t gates;      // gate bit of window i at bit i
+  uint32_t cbit;       // 256 in a prior contig's records
+  bool lpair;          // the pair (i-1, i) joins two gate-passing windows
+  uint32_t lrcp;       // 3 - first base of window i-1
+  Kmer<L> fwd, rev;
+  ShortStream in;  // at most 63 bases after a share's first window
+
+  __device__ __forceinline__ void init(uint64_t gid0, uint64_t stride, uint32_t count) {
+    gid = gid0;
+    step = stride;
+    left = count;
+    i = 1;
+    last = 0;
+  }
+  __device__ __forceinline__ bool has() const { return i <= last; }
+  __device__ __forceinline__ bool alive() const { return i <= last || left != 0u; }
+  __device__ __forceinline__ void open(const P2BinArgs& a) {
+    while (left) {
+      uint64_t r = 0;
+      uint32_t p0 = 0, p1 = 0;
+      const DevReads* src = nullptr;
+      bool contig = false;
+      const uint64_t g = gid;
+      const bool ok = p2_share_any(a, g, src, r, p0, p1, contig);
+      gid += step;
+      --left;
+      if (!ok) continue;
+      gates = a.ungated || contig ? all_windows(p0, p1) : a.gate_masks[g];
+      if (!gates) continue;
+      const uint64_t w0 = src->offs[r] + p0;
+      const Kmer<L> win = window_bits<L>(src->data, w0, a.k);
+      fwd = fwd_from_window<L>(win, a.k);
+      rev = rc_from_window<L>(win, a.k);
+      in.init(src->data, w0 + a.k);
+      i = 0;
+      last = p1 - p0;
+      lpair = false;
+      lrcp = 0;
+      cbit = contig ? 256u : 0u;
+      return;
+    }
+  }
+  // Window i (has() must hold): true when it has a record of this round, in
+  // w (pack_record) with its bin; then moves on to window i + 1.
+  __device__ __forceinline__ bool take(const P2BinArgs& a, uint32_t round, uint64_t (&w)[L + 1], uint32_t& bin) {
+    const uint32_t k = a.k;
+    const bool rc = kmer_lt<L>(rev, fwd);
+    const bool more = i < last;
+    const uint32_t b = more ? in.next() : 0u;
+    const bool rpair = more && ((gates >> i) & (gates >> (i + 1u)) & 1ull);
+    bool rec = lpair || rpair;
+    if (rec) {
+      const Kmer<L> key = rc ? rev : fwd;
+      const uint64_t fold = kmer_fold<L>(key);
+      rec = p2_in_round(a, fold, round);
+      if (rec) {
+        // edge_v of pair (i-1, i) and edge_u of pair (i, i+1), build.rs:268-275
+        const uint32_t el = rc ? lrcp : lrcp + 4u, er = rc ? b + 4u : b;
+        pack_record<L>(key, (lpair ? 1u : 0u) | (el << 1) | (rpair ? 16u : 0u) | (er << 5) | cbit, k, w);
+        bin = p2_bin_of(a, fold);
+      }
+    }
+    if (more) {
+      // the base leaving the window: the forward k-mer's most significant
+      lrcp = 3u - ((uint32_t)(fwd.a[L - 1] >> ((2u * k - 2u) & 63u)) & 3u);
+      kmer_roll<L>(fwd, rev, b, k);
+      lpair = rpair;
+    }
+    ++i;
+    return rec;
+  }
+};
+
+// Phase A through per-bin LDS buffers (at most kP2BufMaxBins bins, L <= 3):
+// every bin holds kP2BufWords words (64 bytes) of records in LDS, and a
+// buffer goes out when it holds the records up to the next multiple of E =
+// kP2BufWords / rw positions of the block's run in the bin (E records once
+// the run is aligned), written by adjacent lanes: one aligned 64-byte store
+// (for rw <= 2) covers what took E scattered stores.  The block's records
+// per bin are those of the count block with the same index (the same
+// shares), so each flush goes to the next positions of the block's exact
+// run.  A thread walks kRecRound windows a round and keeps their records in
+// registers; it appends each with one LDS atomic on the bin's count, and
+// the append that fills a buffer lists the bin.  After a barrier groups of 8
+// lanes write the listed buffers out; records that found their buffer full
+// try again after the flush, and after kP2BufTries steps in a round the
+// rest are stored directly (one store each) at the run's next positions.
+// What is left in the buffers goes out at the end.
+// LDS: buf[nbins * kP2BufWords] (u64), cnt[nbins], pos[nbins] (next
+// position of the block's run), full[nbins].
+constexpr int kP2BufTries = 4;
+size_t p2_scatter_buf_lds(uint32_t nbins) { return (size_t)nbins * (kP2BufWords * 8u + 12u); }
+
+template <int L>
+__device__ __forceinline__ void p2_scatter_buf_body(const P2BinArgs& a, uint64_t blk, uint64_t* smem,
+                                                    uint32_t round) {
+  if ((round >> a.log2_rounds) != 0u) return;
+  constexpr int kRecRound = L == 1 ? 8 : (L == 2 ? 6 : 4);
+  __shared__ uint32_t s_nfull[2];  // full buffers listed in the even / odd steps
+  const uint32_t nb = a.nbins, rw = a.rw, E = kP2BufWords / rw;  // E: 8, 4 or 2
+  uint64_t* buf = smem;
+  uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + (uint64_t)nb * kP2BufWords);
+  uint32_t* pos = cnt + nb;
+  uint32_t* full = pos + nb;
+  const uint32_t* __restrict__ row = a.blk_hist + blk * nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    cnt[b] = 0u;
+    pos[b] = row[b];
+  }
+  if (threadIdx.x < 2) s_nfull[threadIdx.x] = 0;
+  const uint64_t cap = a.rec_cap;
+  uint64_t* __restrict__ recs = a.recs;
+  const uint32_t h = threadIdx.x & 7u;
+  RecWalker<L> wk;
+  wk.init(blk * a.chunks * blockDim.x + threadIdx.x, blockDim.x, a.chunks);
+  uint32_t par = 0;  // parity of the step
+  __syncthreads();
+  for (;;) {
+    if (!wk.has()) wk.open(a);
+    uint64_t w[kRecRound][L + 1];
+    uint32_t bin[kRecRound];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int j = 0; j < kRecRound; ++j)
+      if (wk.has() && wk.take(a, round, w[j], bin[j])) pend |= 1u << j;
+    if (!__syncthreads_or(pend || wk.alive())) break;
+    for (int tries = 0;; ++tries) {
+#pragma unroll
+      for (int j = 0; j < kRecRound; ++j) {
+        if (!((pend >> j) & 1u)) continue;
+        const uint32_t b = bin[j];
+        if (tries == kP2BufTries) {
+          // no flush runs in this step: the run's next position directly
+          const uint32_t p = atomicAdd(pos + b, 1u);
+          if (p < cap) store_record<L>(recs + (uint64_t)p * rw, w[j], rw, p);
+          else a.status[0] = 1u;
+          pend &= ~(1u << j);
+          continue;
+        }
+        // records up to the run's next multiple of E (pos is fixed in this phase)
+        const uint32_t lim = E - (pos[b] & (E - 1u));
+        const uint32_t slot = atomicAdd(cnt + b, 1u);
+        if (slot < lim) {
+          uint64_t* d = buf + (uint64_t)b * kP2BufWords + slot * rw;
+#pragma unroll
+          for (int q = 0; q <= L; ++q)
+            if ((uint32_t)q < rw) d[q] = w[j][q];
+          if (slot == lim - 1u) full[atomicAdd(s_nfull + par, 1u)] = b;
+          pend &= ~(1u << j);
+        }
+      }
+      __syncthreads();
+      const uint32_t nfull = s_nfull[par];
+      // the other step's count: every thread read it before the last barrier
+      if (threadIdx.x == 0) s_nfull[par ^ 1u] = 0;
+      for (uint32_t f = threadIdx.x >> 3; f < nfull; f += blockDim.x >> 3) {
+        const uint32_t b = full[f];
+        const uint32_t p = pos[b], lim = E - (p & (E - 1u));
+        if (h < lim * rw) {
+          if ((uint64_t)p + lim <= cap) recs[(uint64_t)p * rw + h] = buf[(uint64_t)b * kP2BufWords + h];
+          else a.status[0] = 1u;
+        }
+        if (h == 0) {
+          pos[b] = p + lim;
+          cnt[b] = 0u;
+        }
+      }
+      par ^= 1u;
+      if (!__syncthreads_or(pend)) break;
+    }
+  }
+  // every buffer holds fewer records than its limit (a full one went out in its step)
+  for (uint32_t b = threadIdx.x >> 3; b < nb; b += blockDim.x >> 3) {
+    const uint32_t c = cnt[b], p = pos[b];
+    if (h < c * rw) {
+      if ((uint64_t)p + c <= cap) recs[(uint64_t)p * rw + h] = buf[(uint64_t)b * kP2BufWords + h];
+      else a.status[0] = 1u;
+    }
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_buf(P2BinArgs a) {
+  extern __shared__ uint64_t sh64[];
+  p2_scatter_buf_body<L>(a, blockIdx.x, sh64, a.round);
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_buf_m(const P2BinArgs* __restrict__ args,
+                                                                     const uint32_t* __restrict__ blk, uint32_t n_iso,
+                                                                     uint32_t round) {
+  extern __shared__ uint64_t sh64[];
+  const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
+  p2_scatter_buf_body<L>(args[iso], blockIdx.x - blk[iso], sh64, round);
+}
+
+// Whether phase A of nbins bins per round and k uses the buffered scatter.
+bool p2_scatter_buffered(uint32_t k, uint32_t nbins) { return (2u * k + 63u) / 64u <= 3u && nbins <= kP2BufMaxBins; }
 
 // LDS hash of a packed key: its home slot inside the bin's table.  It sets
 // only where a key is placed (probe lengths), never a result, so one
@@ -1036,6 +1244,10 @@ static hipError_t set_pair_attr() {
   if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_bins_m<L, KW>));
   if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter<L, KW>));
   if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_m<L, KW>));
+  if constexpr (L <= 3) {
+    if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf<L>));
+    if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf_m<L>));
+  }
   return e;
 }
 
@@ -1086,6 +1298,7 @@ struct P2Launch {
   uint64_t threads;       // all shares: reads, then prior contigs
   uint64_t read_threads;  // the reads' shares (the gate kernels' grid)
   size_t scat_lds, lds;
+  bool buffered;
   hipStream_t s;
   hipEvent_t gated, scattered;
   template <int L, int KW>
@@ -1113,7 +1326,14 @@ struct P2Launch {
         if (hipError_t e = hipEventRecord(gated, s)) return e;
       if (threads) {
         const unsigned grid = (unsigned)scatter_blocks(threads, a.chunks);
-        hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, ar);
+        if constexpr (L <= 3) {
+          if (buffered)
+            hipLaunchKernelGGL((k_p2_scatter_buf<L>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, ar);
+          else
+            hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, ar);
+        } else {
+          hipLaunchKernelGGL((k_p2_scatter<L, KW>), dim3(grid), dim3(kP2ScatterBlock), scat_lds, s, ar);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
       }
@@ -1134,7 +1354,8 @@ struct P2Launch {
 
 hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated, hipEvent_t scattered) {
   const uint32_t KW = (a.k + 30) / 31;
-  const size_t scat_lds = p2_scatter_lds(a.nbins);
+  const bool buffered = p2_scatter_buffered(a.k, a.nbins);
+  const size_t scat_lds = buffered ? p2_scatter_buf_lds(a.nbins) : p2_scatter_lds(a.nbins);
   const size_t lds = p2_lds_bytes(KW, a.lds_cap, a.filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (a.lds_cap % kLdsBucket || a.lds_cap == 0) return hipErrorInvalidValue;
@@ -1146,7 +1367,7 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated,
   const uint64_t threads = read_threads + a.contigs.n_reads * (uint64_t)a.csplit;
   if (a.n_blocks != scatter_blocks(threads, a.chunks)) return hipErrorInvalidValue;
   if (a.log2_rounds > kP2MaxLog2Rounds || a.nbins != (1u << a.log2_nbins)) return hipErrorInvalidValue;
-  return dispatch_lkw(a.k, P2Launch{a, threads, read_threads, scat_lds, lds, s, gated, scattered});
+  return dispatch_lkw(a.k, P2Launch{a, threads, read_threads, scat_lds, lds, buffered, s, gated, scattered});
 }
 
 struct P2LaunchMulti {
@@ -1154,6 +1375,7 @@ struct P2LaunchMulti {
   const FillRange* runs_fill;
   uint32_t n_fill, max_rounds;
   size_t count_lds, scat_lds, lds;
+  bool buffered;
   hipStream_t s;
   template <int L, int KW>
   hipError_t operator()() {
@@ -1173,9 +1395,20 @@ struct P2LaunchMulti {
         const unsigned ogrid = (unsigned)((count_lds / 4 * kSlices + kP2OffBlock - 1) / kP2OffBlock);
         hipLaunchKernelGGL(k_p2_block_offsets_m, dim3(ogrid, gate.n_iso), dim3(kP2OffBlock), 0, s, ga);
       }
-      if (scatter.total_blocks)
-        hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,
-                           static_cast<const P2BinArgs*>(scatter.args), scatter.blk, scatter.n_iso, round);
+      if (scatter.total_blocks) {
+        const P2BinArgs* sa = static_cast<const P2BinArgs*>(scatter.args);
+        if constexpr (L <= 3) {
+          if (buffered)
+            hipLaunchKernelGGL((k_p2_scatter_buf_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds,
+                               s, sa, scatter.blk, scatter.n_iso, round);
+          else
+            hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds,
+                               s, sa, scatter.blk, scatter.n_iso, round);
+        } else {
+          hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds, s,
+                             sa, scatter.blk, scatter.n_iso, round);
+        }
+      }
       if (hipError_t e = hipGetLastError()) return e;
       if (bins.total_blocks)
         hipLaunchKernelGGL((k_p2_bins_m<L, KW>), dim3(bins.total_blocks), dim3(kP2BinBlock), lds, s,
@@ -1195,13 +1428,14 @@ hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& sc
                                   const FillRange* runs_fill, uint32_t n_fill, uint32_t k, uint32_t max_nbins,
                                   uint32_t max_rounds, uint32_t lds_cap, uint32_t filt, hipStream_t s) {
   const uint32_t KW = (k + 30) / 31;
-  const size_t scat_lds = p2_scatter_lds(max_nbins);
+  const bool buffered = p2_scatter_buffered(k, max_nbins);
+  const size_t scat_lds = buffered ? p2_scatter_buf_lds(max_nbins) : p2_scatter_lds(max_nbins);
   const size_t lds = p2_lds_bytes(KW, lds_cap, filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (max_rounds == 0 || max_rounds > (1u << kP2MaxLog2Rounds) || filt > kP2MaxFilt) return hipErrorInvalidValue;
   if (lds_cap % kLdsBucket || lds_cap == 0) return hipErrorInvalidValue;
   return dispatch_lkw(k, P2LaunchMulti{gate, scatter, bins, runs_fill, n_fill, max_rounds, (size_t)max_nbins * 4,
-                                       scat_lds, lds, s});
+                                       scat_lds, lds, buffered, s});
 }
 
 }  // namespace fb

@@ -786,7 +786,12 @@ __device__ __forceinline__ void p2_scatter_buf_body(const P2BinArgs& a, uint64_t
                                                     uint32_t round) {
   if ((round >> a.log2_rounds) != 0u) return;
   constexpr int kRecRound = L == 1 ? 8 : (L == 2 ? 6 : 4);
-  __shared__ uint32_t s_nfull[2];  // full buffers listed in the even / odd steps
+  // Per step parity: buffers listed full, and whether any thread still holds
+  // records / has windows left.  A step reads its parity's words after its
+  // first barrier and clears the other parity's, which the step before read
+  // before its second barrier and the step after writes after this step's
+  // second barrier.
+  __shared__ uint32_t s_nfull[2], s_pend[2], s_live[2];
   const uint32_t nb = a.nbins, rw = a.rw, E = kP2BufWords / rw;  // E: 8, 4 or 2
   uint64_t* buf = smem;
   uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + (uint64_t)nb * kP2BufWords);
@@ -797,15 +802,15 @@ __device__ __forceinline__ void p2_scatter_buf_body(const P2BinArgs& a, uint64_t
     cnt[b] = 0u;
     pos[b] = row[b];
   }
-  if (threadIdx.x < 2) s_nfull[threadIdx.x] = 0;
+  if (threadIdx.x < 2) s_nfull[threadIdx.x] = s_pend[threadIdx.x] = s_live[threadIdx.x] = 0u;
   const uint64_t cap = a.rec_cap;
   uint64_t* __restrict__ recs = a.recs;
   const uint32_t h = threadIdx.x & 7u;
   RecWalker<L> wk;
   wk.init(blk * a.chunks * blockDim.x + threadIdx.x, blockDim.x, a.chunks);
-  uint32_t par = 0;  // parity of the step
+  uint32_t par = 0;
   __syncthreads();
-  for (;;) {
+  for (bool live = true; live;) {
     if (!wk.has()) wk.open(a);
     uint64_t w[kRecRound][L + 1];
     uint32_t bin[kRecRound];
@@ -813,14 +818,14 @@ __device__ __forceinline__ void p2_scatter_buf_body(const P2BinArgs& a, uint64_t
 #pragma unroll
     for (int j = 0; j < kRecRound; ++j)
       if (wk.has() && wk.take(a, round, w[j], bin[j])) pend |= 1u << j;
-    if (!__syncthreads_or(pend || wk.alive())) break;
+    if (wk.alive()) s_live[par] = 1u;
     for (int tries = 0;; ++tries) {
 #pragma unroll
       for (int j = 0; j < kRecRound; ++j) {
         if (!((pend >> j) & 1u)) continue;
         const uint32_t b = bin[j];
         if (tries == kP2BufTries) {
-          // no flush runs in this step: the run's next position directly
+          // no thread appends in this step: the run's next position directly
           const uint32_t p = atomicAdd(pos + b, 1u);
           if (p < cap) store_record<L>(recs + (uint64_t)p * rw, w[j], rw, p);
           else a.status[0] = 1u;
@@ -839,10 +844,12 @@ __device__ __forceinline__ void p2_scatter_buf_body(const P2BinArgs& a, uint64_t
           pend &= ~(1u << j);
         }
       }
+      if (pend) s_pend[par] = 1u;
       __syncthreads();
       const uint32_t nfull = s_nfull[par];
-      // the other step's count: every thread read it before the last barrier
-      if (threadIdx.x == 0) s_nfull[par ^ 1u] = 0;
+      const bool again = s_pend[par] != 0u;
+      live = s_live[par] != 0u || again;
+      if (threadIdx.x == 0) s_nfull[par ^ 1u] = s_pend[par ^ 1u] = s_live[par ^ 1u] = 0u;
       for (uint32_t f = threadIdx.x >> 3; f < nfull; f += blockDim.x >> 3) {
         const uint32_t b = full[f];
         const uint32_t p = pos[b], lim = E - (p & (E - 1u));
@@ -856,7 +863,11 @@ __device__ __forceinline__ void p2_scatter_buf_body(const P2BinArgs& a, uint64_t
         }
       }
       par ^= 1u;
-      if (!__syncthreads_or(pend)) break;
+      __syncthreads();
+      if (!again) break;
+      // a thread still holding records keeps its walk where it is; live
+      // stays set for the next step
+      if (wk.alive()) s_live[par] = 1u;
     }
   }
   // every buffer holds fewer records than its limit (a full one went out in its step)

@@ -294,10 +294,11 @@ __global__ __launch_bounds__(kScatterBlock, 4) void k_bin_scatter_m(const BinArg
 // round's cells to their bins' buffers with one LDS atomic each.  After a
 // barrier a thread per bin lists the full buffers (reserving a new chunk of
 // 64 offsets, one line, with one global atomic where the bin has none
-// left), and after another half a wave per full bin writes the buffer into
-// the bin's current chunk.  Cells that found their buffer full try again
-// after the flush.  Every store to the bin regions thus writes 64 whole
-// bytes, once; a line is written by one workgroup only, in two halves.
+// left), and after another barrier 8 lanes per full bin write the buffer into
+// the bin's current chunk (8 lanes of 8 bytes a bin).  Cells that found
+// their buffer full try again after the flush.  Every store to the bin
+// regions thus writes 64 whole bytes, once; a line is written by one
+// workgroup only, in two halves.
 // After kBufTries placement steps in one round, the cells still waiting are
 // added to the table with global atomics (heavily repeated k-mers: exact,
 // and no workgroup waits on a bin for long).
@@ -346,7 +347,7 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
   const uint32_t smask = (1u << sb) - 1u;
   uint16_t* __restrict__ out = a.data + sl * cap;
   uint32_t* __restrict__ cursor = a.cursor + sl;
-  const uint32_t h = threadIdx.x & 31u;
+  const uint32_t h = threadIdx.x & 7u;  // a flushing lane's 4 offsets: entries 4h .. 4h+3
   __shared__ NtRollTab s_nt;
   if (HLL && threadIdx.x < 4) s_nt.fill(threadIdx.x, a.k);
   CellWalker<L, HLL> wk;
@@ -399,13 +400,20 @@ __device__ __forceinline__ void bin_buffer_body(const BinArgs& a, uint64_t blk, 
       }
       __syncthreads();
       const uint32_t nf = s_nfull;
-      for (uint32_t f = threadIdx.x >> 5; f < nf; f += blockDim.x >> 5) {
+      // 8 lanes a bin, 8 bytes each.  A half chunk starts at a multiple of
+      // kBufE and cap is a multiple of kChunk, so it lies wholly below cap
+      // or wholly past it.
+      for (uint32_t f = threadIdx.x >> 3; f < nf; f += blockDim.x >> 3) {
         const uint32_t b = full[f];
         const uint32_t cw = cur[b];
         const uint64_t pos = half_pos(cw);
-        const uint16_t e = buf[b * kBufE + h];
-        if (pos + h < cap) out[(uint64_t)b * (kSlices * cap) + pos + h] = e;
-        else atomicAdd(a.counts + ((uint64_t)b << sb) + e, 1u);
+        const uint64_t e4 = reinterpret_cast<const uint64_t*>(buf + b * kBufE)[h];
+        if (pos < cap) {
+          reinterpret_cast<uint64_t*>(out + (uint64_t)b * (kSlices * cap) + pos)[h] = e4;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) atomicAdd(a.counts + ((uint64_t)b << sb) + ((e4 >> (16 * q)) & 0xFFFFu), 1u);
+        }
         if (h == 0) {
           cur[b] = cw + (1u << 2) - 1u;  // next half: position + kBufE, one half fewer
           cnt[b] = 0;

@@ -464,22 +464,36 @@ __device__ __forceinline__ void bin_count_body(const BinArgs& a, uint32_t b, uin
   for (uint32_t sl = 0; sl < kSlices; ++sl) {
     const uint32_t filled = a.cursor[(uint64_t)b * kSlices + sl];
     const uint32_t n = filled < a.cap ? filled : (uint32_t)a.cap;
-    // cap is a multiple of 64, so every region starts on a 128-byte line;
-    // a thread takes 16 offsets (two 16-byte loads in flight) at a time
+    // cap is a multiple of 64, so every region starts on a 128-byte line.
+    // The count is bound by load latency, so a thread has eight 16-byte
+    // loads (64 offsets) in flight before it counts them: load j of a step
+    // takes group j * blockDim + thread of 8 offsets (a wave's loads are
+    // contiguous), its index clamped to the last whole group (always inside
+    // the region) and its offsets skipped when past it.
     const uint16_t* __restrict__ d = a.data + ((uint64_t)b * kSlices + sl) * a.cap;
-    const uint32_t n16 = n & ~15u;
-    for (uint32_t i = threadIdx.x * 16u; i < n16; i += blockDim.x * 16u) {
-      const uint4 v0 = *reinterpret_cast<const uint4*>(d + i);
-      const uint4 v1 = *reinterpret_cast<const uint4*>(d + i + 8);
-      const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+    const uint32_t n8 = n & ~7u;
+    for (uint32_t base = 0; base < n8; base += blockDim.x * 64u) {
+      uint4 v[8];
+      uint32_t ok = 0;  // loads inside the region
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const uint32_t lo = w[j] & 0xFFFFu, hi = w[j] >> 16;
-        if (lo != kNoCell) atomicAdd(lds + lo, 1u);
-        if (hi != kNoCell) atomicAdd(lds + hi, 1u);
+        const uint32_t q = base + ((uint32_t)j * blockDim.x + threadIdx.x) * 8u;
+        v[j] = *reinterpret_cast<const uint4*>(d + (q < n8 ? q : n8 - 8u));
+        ok |= (q < n8 ? 1u : 0u) << j;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (!((ok >> j) & 1u)) continue;
+        const uint32_t w[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const uint32_t lo = w[t] & 0xFFFFu, hi = w[t] >> 16;
+          if (lo != kNoCell) atomicAdd(lds + lo, 1u);
+          if (hi != kNoCell) atomicAdd(lds + hi, 1u);
+        }
       }
     }
-    for (uint32_t i = n16 + threadIdx.x; i < n; i += blockDim.x)
+    for (uint32_t i = n8 + threadIdx.x; i < n; i += blockDim.x)
       if (d[i] != kNoCell) atomicAdd(lds + d[i], 1u);
   }
   __syncthreads();

@@ -232,12 +232,17 @@ struct P2BinArgs {
   unsigned long long* stats = nullptr;  // zeroed; [0] records (sum of raw cov), [1] keys held in LDS
 };
 constexpr uint32_t kP2FiltWords = 2048;  // u32 words per level of the ungated bin kernel's record filter
-constexpr uint32_t kP2MaxFilt = 3;       // filter levels: records counted up to 3 per position
+constexpr uint32_t kP2MaxFilt = 4;       // filter levels: records counted up to 4 per position
+// Target fill of the ungated form's LDS tables at the kept-key estimate
+// (which carries a 1.1 margin; the filter's false positives add a few %).
+constexpr double kP2UngatedLoad = 0.65;
 constexpr uint32_t kP2BufWords = 8;      // words of a bin's record buffer in the buffered pass-2 scatter
 constexpr uint32_t kP2BufMaxBins = 2048; // most bins a round of the buffered scatter holds in LDS
 // Whether phase A of pass 2 (k, nbins bins a round) writes records through
 // per-bin LDS buffers; else one store per record.
 bool p2_scatter_buffered(uint32_t k, uint32_t nbins);
+// LDS bytes of one bin-table entry: kw key words, u32 cov, 8 u8 edge counters.
+constexpr uint32_t p2_entry_bytes(uint32_t kw) { return kw * 8u + 4u + 8u; }
 uint32_t p2_lds_cap(uint32_t kw, uint32_t filt = 0);
 uint32_t p2_max_bins();  // largest pass-2 bin count the scatter kernel's LDS allows
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap, uint32_t filt = 0);

@@ -1011,33 +1011,27 @@ __device__ __forceinline__ void p2_table_add(const P2BinArgs& a, uint32_t* cov, 
   const uint32_t cw = contig ? 100u : 1u;
   if (e >= 0) {
     atomicAdd(cov + e, cw * ((uint32_t)left + (uint32_t)right));
-    // u16 edge counters, two per word.  Only `value >= gate` (gate <= 255)
-    // is ever read from them, and values below the gate are exact.  A read
-    // adds 1 after it has loaded a value below the gate; the load may be
-    // stale, so adds land past the gate, at most two per thread of the block
-    // (the two sides of its record; the LDS serves a wave's operations in
-    // issue order, so its next record's loads see its earlier adds).  A
-    // contig's +1000 saturates by compare and swap at kEdgeCeil = 65535 -
-    // 4 * blockDim and never lowers a value, so every half stays <=
-    // max(kEdgeCeil, gate) + 2 * blockDim < 65535 and a read add can neither
-    // carry into the other half nor wrap.
+    // u8 edge counters, four per word.  Only `value >= gate` (gate <= 255)
+    // and values below the gate are ever read, so a read's side adds 1 by
+    // compare and swap while its byte is below the gate (exact however the
+    // adds interleave, and a byte below 255 cannot carry into the next); a
+    // prior contig's side sets its byte to 255, at or past any gate.  Only a
+    // key's first adds on a slot reach the compare and swap: later ones
+    // find the gate reached and only read.
     for (int side = 0; side < 2; ++side) {
       if (!(side ? right : left)) continue;
       const uint32_t ed = side ? er : el;
-      uint32_t* ew = edges + 4u * (uint32_t)e + (ed >> 1);
-      const uint32_t sh = (ed & 1u) * 16u;
+      uint32_t* ew = edges + 2u * (uint32_t)e + (ed >> 2);
+      const uint32_t sh = (ed & 3u) * 8u;
       if (contig) {
-        constexpr uint32_t kEdgeCeil = 65535u - 4u * (uint32_t)kP2BinBlock;
-        uint32_t old = lds_vload(ew), seen;
-        do {
-          seen = old;
-          const uint32_t hv = (seen >> sh) & 0xffffu;
-          if (hv >= kEdgeCeil) break;
-          const uint32_t nv = hv + 1000u < kEdgeCeil ? hv + 1000u : kEdgeCeil;
-          old = atomicCAS(ew, seen, (seen & ~(0xffffu << sh)) | (nv << sh));
-        } while (old != seen);
-      } else if (((lds_vload(ew) >> sh) & 0xffffu) < gate) {
-        atomicAdd(ew, 1u << sh);
+        atomicOr(ew, 0xffu << sh);
+      } else {
+        uint32_t old = lds_vload(ew);
+        while (((old >> sh) & 0xffu) < gate) {
+          const uint32_t seen = old;
+          old = atomicCAS(ew, seen, seen + (1u << sh));
+          if (old == seen) break;
+        }
       }
     }
   } else {
@@ -1071,13 +1065,13 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   uint32_t* filt = reinterpret_cast<uint32_t*>(smem);                           // nf * kP2FiltWords
   unsigned long long* keys = reinterpret_cast<unsigned long long*>(filt + nf * kP2FiltWords);  // KW * cap
   uint32_t* cov = reinterpret_cast<uint32_t*>(keys + (uint64_t)KW * cap);       // cap
-  uint32_t* edges = cov + cap;                                                  // 4 * cap (u16 pairs)
+  uint32_t* edges = cov + cap;                                                  // 2 * cap (u8 quads)
   __shared__ uint32_t s_wave[kP2BinBlock / 64];
   __shared__ uint32_t s_base;
   __shared__ unsigned long long s_stat[2];
   for (uint32_t i = threadIdx.x; i < nf * kP2FiltWords; i += blockDim.x) filt[i] = 0u;
   for (uint32_t i = threadIdx.x; i < KW * cap; i += blockDim.x) keys[i] = kEmpty;
-  for (uint32_t i = threadIdx.x; i < 5 * cap; i += blockDim.x) cov[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < 3 * cap; i += blockDim.x) cov[i] = 0u;  // cov and edges
   if (threadIdx.x < 2) s_stat[threadIdx.x] = 0ull;
   __syncthreads();
   // the bin's kSlices regions are adjacent: [start of its first slice, start
@@ -1115,7 +1109,7 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
     mask = 0;
 #pragma unroll
     for (uint32_t q = 0; q < 8; ++q) {
-      const uint32_t v = (edges[4u * e + (q >> 1)] >> ((q & 1u) * 16u)) & 0xffffu;
+      const uint32_t v = (edges[2u * e + (q >> 2)] >> ((q & 3u) * 8u)) & 0xffu;
       if (v >= gate) mask |= 1u << q;
     }
     const uint32_t c = cov[e] < 65535u ? cov[e] : 65535u;
@@ -1136,7 +1130,7 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
 #pragma unroll
       for (int j = 0; j < KW; ++j) wk[j] = keys[(uint64_t)j * cap + e];
       cov[e] -= gate_loss<L>(a, unpack62b<L, KW>(wk),
-                             [&](uint32_t q) { return (edges[4u * e + (q >> 1)] >> ((q & 1u) * 16u)) & 0xffffu; });
+                             [&](uint32_t q) { return (edges[2u * e + (q >> 2)] >> ((q & 3u) * 8u)) & 0xffu; });
     }
     ++mine;
   }
@@ -1281,12 +1275,12 @@ static uint32_t lds_budget() {
 uint32_t p2_max_bins() { return (lds_budget() - 1024u) / 4u >= 16384u ? 16384u : 8192u; }
 
 uint32_t p2_lds_cap(uint32_t kw, uint32_t filt) {
-  // keys + cov + 8 u16 edges per entry within the workgroup's LDS, after the filter
-  const uint32_t entry = kw * 8u + 4u + 16u;
+  // keys + cov + 8 u8 edges per entry within the workgroup's LDS, after the filter
+  const uint32_t entry = p2_entry_bytes(kw);
   return ((lds_budget() - 1024u - filt * kP2FiltWords * 4u) / entry) & ~(kLdsBucket - 1u);
 }
 size_t p2_lds_bytes(uint32_t kw, uint32_t cap, uint32_t filt) {
-  return (size_t)filt * kP2FiltWords * 4u + (size_t)cap * (kw * 8u + 4u + 16u);
+  return (size_t)filt * kP2FiltWords * 4u + (size_t)cap * p2_entry_bytes(kw);
 }
 
 // Calls F<L, KW>() for the template pair of k.

@@ -1115,8 +1115,8 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
   const uint32_t kw = tab.kw;
   const uint32_t lds_cap = p2_lds_cap(kw, filt);
   // bins over all rounds: the estimated keys at half the LDS table capacity
-  // (0.6 for the ungated form, whose estimate carries its own margin)
-  const double load = ungated ? 0.6 : 0.5;
+  // (0.65 for the ungated form, whose estimate carries its own margin)
+  const double load = ungated ? kP2UngatedLoad : 0.5;
   uint64_t nb2 = 1;
   while ((double)nb2 * load * (double)lds_cap < n_est_keys) nb2 <<= 1;
   const uint64_t max_bins = (uint64_t)p2_max_bins() << kP2MaxLog2Rounds;
@@ -1594,7 +1594,7 @@ struct P2Plan {
   bool ok = false;
 };
 // lds_cap: LDS table entries of one bin; round_bins: bins one round takes.
-// load: the LDS tables' target fill at the estimate (0.5; 0.6 for the
+// load: the LDS tables' target fill at the estimate (0.5; kP2UngatedLoad for the
 // ungated form, whose estimate has a margin of its own); n_distinct (the
 // ungated form): the spill table's size when bins2 forces fewer bins than
 // the estimate asks for, since their record filters then let most keys in.
@@ -1627,7 +1627,7 @@ extern "C" int fb_plan_pass2(double est_keys, uint64_t windows, uint64_t n_reads
                              fb_pass2_plan* out) {
   if (!out || k < 1 || k > 127) return FB_ERR_INVALID;
   const uint32_t kw = key_words_for_k(k);
-  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / (kw * 8u + 20u)) & ~3u;  // whole 4-slot buckets
+  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / p2_entry_bytes(kw)) & ~3u;  // whole 4-slot buckets
   const P2Plan q = plan_pass2(est_keys, windows, n_reads, max_len, k, default_split(k, max_len), 0, lds_cap, 16384u, 0);
   out->nb2 = q.nb2;
   out->nbr = q.nbr;
@@ -1686,7 +1686,7 @@ static IsoBytes iso_bytes(uint64_t genome_size, uint64_t bases, uint64_t reads, 
   const double n_est = (double)windows;
   // host-only sizing: the LDS of a workgroup with the full 160 KB granted
   const uint32_t kw = key_words_for_k(k);
-  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / (kw * 8u + 20u)) & ~3u;  // whole 4-slot buckets
+  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / p2_entry_bytes(kw)) & ~3u;  // whole 4-slot buckets
   const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u,
                               p->chunks2);
   const uint32_t rw = (2 * k + 8 + 63) / 64;
@@ -2014,7 +2014,7 @@ static int batch_prepare2(fb_batch* b, uint32_t k, uint32_t t_solid, uint32_t ga
                           : it.est_gated;
     const double n_distinct = ungated ? est_distinct_keys(oi + 2, it.cms.width, it.windows) : it.est_keys;
     const P2Plan q = plan_pass2(it.est_keys, it.windows, it.bank->n_reads, it.bank->max_len, k, it.split,
-                                prm.bins2, p2_lds_cap(kw, filt), p2_max_bins(), prm.chunks2, ungated ? 0.6 : 0.5,
+                                prm.bins2, p2_lds_cap(kw, filt), p2_max_bins(), prm.chunks2, ungated ? kP2UngatedLoad : 0.5,
                                 ungated ? n_distinct : 0.0);
     const bool lds = q.ok && prm.placement != 1;
     it.lds = lds;

@@ -796,6 +796,24 @@ __global__ __launch_bounds__(kBlock) void k_gather_records(const uint64_t* __res
   omask[i] = mask[j];
 }
 
+// One-word keys sort with their record as the value: cov | mask << 16.
+__global__ __launch_bounds__(kBlock) void k_pack_cm(const uint16_t* __restrict__ cov, const uint8_t* __restrict__ mask,
+                                                    uint32_t n, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (uint32_t)cov[i] | (uint32_t)mask[i] << 16;
+}
+__global__ __launch_bounds__(kBlock) void k_unpack_cm(const uint64_t* __restrict__ skeys,
+                                                      const uint32_t* __restrict__ cm, uint32_t n,
+                                                      uint64_t* __restrict__ keys, uint16_t* __restrict__ cov,
+                                                      uint8_t* __restrict__ mask) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (skeys != keys) keys[i] = skeys[i];
+  const uint32_t v = cm[i];
+  cov[i] = (uint16_t)v;
+  mask[i] = (uint8_t)(v >> 16);
+}
+
 // Scratch layout: [limb keys x2][perm x2][record copy][cub temp].
 // Most significant 64 bits of every key of more than 64 bits: bits
 // [2k - 64, 2k) of the integer (words most significant first).
@@ -876,6 +894,18 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   uint32_t* ties = (uint32_t*)p; p += al(4);
   void* ctmp = p;
   const unsigned grid = grid_for(n);
+  if (nref == 1) {
+    // one word a key (k <= 32): its cov and mask travel with it as the
+    // sort's value, so no permutation and no gather
+    hipLaunchKernelGGL(k_pack_cm, dim3(grid), dim3(kBlock), 0, s, cov, mask, n, v0);
+    hipcub::DoubleBuffer<uint64_t> bk(keys, k0);
+    hipcub::DoubleBuffer<uint32_t> bv(v0, v1);
+    size_t cb = cub_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(ctmp, cb, bk, bv, (int)n, 0, (int)(2 * k), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_unpack_cm, dim3(grid), dim3(kBlock), 0, s, bk.Current(), bv.Current(), n, keys, cov, mask);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_iota, dim3(grid), dim3(kBlock), 0, s, v0, n);
   // Significant limbs: L = ceil(2k/64); the top limb has 2k - 64(L-1) bits.
   const uint32_t L = (2 * k + 63) / 64;

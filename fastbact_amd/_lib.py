@@ -310,6 +310,7 @@ SIGNATURES = [
     ("fb_diag_atomic_rate", C.c_int, [vp, C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_float)]),
     ("fb_solid_stats", C.c_int, [vp, u64p]),
     ("fb_solid_pass2_gate", C.c_int, [vp]),
+    ("fb_solid_pass2_layout", C.c_int, [vp]),
     ("fb_compact_solid", C.c_int, [vp, C.POINTER(vp)]),
     ("fb_compact_arrays", C.c_int, [vp, u64p, u16p, u8p, C.c_uint64, C.c_uint32, C.POINTER(vp)]),
     ("fb_graph_destroy", None, [vp]),

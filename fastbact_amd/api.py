@@ -398,6 +398,15 @@ class SolidHandle(_Handle):
             self.ctx.check(r, "fb_solid_pass2_gate")
         return ("window", "key")[r]
 
+    def pass2_layout(self) -> str:
+        """'runs': a count walk gave exact record runs; 'chunks': records in
+        chunked regions without a count walk; 'chunks_then_runs': a chunked
+        region filled up and the pass ran again with exact runs."""
+        r = int(self.ctx.lib.fb_solid_pass2_layout(self.h))
+        if r < 0:
+            self.ctx.check(r, "fb_solid_pass2_layout")
+        return ("runs", "chunks", "chunks_then_runs")[r]
+
     def download(self, k: int, with_cms: bool = True, counter_bits: int = 4, hll_p: int = 0) -> SolidSet:
         n, nw = self.count(), self.key_words()
         keys = np.zeros((n, nw), np.uint64)

@@ -193,6 +193,13 @@ struct P2BinArgs {
   // distinct keys of one round fit the LDS tables of nbins bins.
   uint32_t log2_rounds = 0;
   uint32_t round = 0;            // round of this launch sequence
+  // Nonzero: the scatter walks without a count (ungated, one round, the
+  // buffered scatter's bins): the records of bin b from XCD slice sl go to
+  // region b * kBinSlices + sl of region_cap records (a multiple of
+  // kP2ChunkUnits flush units), runs[b * kBinSlices + sl] counts the records
+  // reserved there in whole chunks (tails all-zero records), and status[0]
+  // is raised when a region fills up.
+  uint64_t region_cap = 0;
   uint64_t* gate_masks = nullptr;  // n_reads * split, gate bit of window p0+i at bit i
   uint32_t rw = 1;               // 64-bit words per record: ceil((2k + 8 + cflag) / 64)
   // Prior contigs (the multi-k upgrade): their windows follow the reads' in
@@ -238,6 +245,9 @@ constexpr uint32_t kP2MaxFilt = 4;       // filter levels: records counted up to
 constexpr double kP2UngatedLoad = 0.65;
 constexpr uint32_t kP2BufWords = 8;      // words of a bin's record buffer in the buffered pass-2 scatter
 constexpr uint32_t kP2BufMaxBins = 2048; // most bins a round of the buffered scatter holds in LDS
+constexpr uint32_t kP2ChunkUnits = 8;    // flush units (kP2BufWords words each) a chunked-scatter reservation takes
+// Workgroups of the chunked pass-2 scatter: one per CU (its LDS takes one).
+uint32_t p2_chunk_grid();
 // Whether phase A of pass 2 (k, nbins bins a round) writes records through
 // per-bin LDS buffers; else one store per record.
 bool p2_scatter_buffered(uint32_t k, uint32_t nbins);

@@ -901,6 +901,153 @@ __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_buf_m(const P2Bi
   p2_scatter_buf_body<L>(args[iso], blockIdx.x - blk[iso], sh64, round);
 }
 
+// Phase A without the count walk (P2BinArgs::region_cap != 0; the ungated
+// form, one round, the buffered scatter's bins): a resident grid walks all
+// shares, and a workgroup writes its records of bin b into the region of
+// (b, its XCD's slice), region_cap records, in chunks of kP2ChunkUnits
+// flush units (E records each) that it reserves with one global atomic on
+// runs[b * kSlices + slice].  A bin's buffer goes out when it holds a whole
+// unit, so every store is one aligned unit.  A step: appends (the one that
+// fills a buffer lists the bin), barrier, a new chunk for each listed bin
+// that has none left (a thread per bin, so the reservations' round trips
+// overlap), barrier, the flushes (8 lanes a bin), barrier.  At the end each
+// bin's partial buffer and the rest of its chunk go out with all-zero
+// records after it: the bin kernel skips them (a record always has a left
+// or a right pair).  A chunk past region_cap raises status[0] and its
+// records are dropped; the host then runs the pass again with exact runs.
+// LDS: buf[nbins * kP2BufWords] (u64), cl[nbins] (records in the buffer,
+// low 16 bits (may pass E while records are placed); units left in the
+// current chunk, high 16 bits), pos[nbins] (record index of the next unit),
+// full[nbins].
+constexpr uint32_t kNoPos = 0xFFFFFFFFu;
+template <int L>
+__device__ __forceinline__ void p2_scatter_chunk_body(const P2BinArgs& a, uint32_t blk, uint32_t grid,
+                                                      uint64_t* smem) {
+  constexpr int kRecRound = L == 1 ? 8 : (L == 2 ? 6 : 4);
+  __shared__ uint32_t s_nfull[2], s_pend[2], s_live[2];
+  __shared__ uint32_t s_recs;
+  const uint32_t nb = a.nbins, rw = a.rw, E = kP2BufWords / rw;  // E: 8, 4 or 2
+  const uint32_t CR = kP2ChunkUnits * E;                           // records a chunk
+  uint64_t* buf = smem;
+  uint32_t* cl = reinterpret_cast<uint32_t*>(buf + (uint64_t)nb * kP2BufWords);
+  uint32_t* pos = cl + nb;
+  uint32_t* full = pos + nb;
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    cl[b] = 0u;
+    pos[b] = kNoPos;
+  }
+  if (threadIdx.x < 2) s_nfull[threadIdx.x] = s_pend[threadIdx.x] = s_live[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) s_recs = 0u;
+  const uint32_t sl = xcc_slice();
+  const uint64_t rcap = a.region_cap;
+  uint64_t* __restrict__ recs = a.recs;
+  uint32_t* __restrict__ cursor = a.runs + sl;
+  const uint32_t h = threadIdx.x & 7u;
+  const uint64_t total = a.reads.n_reads * (uint64_t)a.split + a.contigs.n_reads * (uint64_t)a.csplit;
+  const uint64_t gid0 = (uint64_t)blk * blockDim.x + threadIdx.x, stride = (uint64_t)grid * blockDim.x;
+  RecWalker<L> wk;
+  wk.init(gid0, stride, gid0 < total ? (uint32_t)((total - 1 - gid0) / stride + 1) : 0u);
+  uint32_t par = 0, mine = 0;
+  // a new chunk for bin b (its region in this XCD's slice)
+  auto reserve = [&](uint32_t b) {
+    const uint32_t q = atomicAdd(cursor + (uint64_t)b * kSlices, CR);
+    if ((uint64_t)q + CR <= rcap) {
+      pos[b] = (uint32_t)(((uint64_t)b * kSlices + sl) * rcap + q);
+    } else {
+      pos[b] = kNoPos;  // region full: the host runs the pass again with exact runs
+      a.status[0] = 1u;
+    }
+    cl[b] = (cl[b] & 0xFFFFu) | (kP2ChunkUnits << 16);
+  };
+  __syncthreads();
+  for (bool live = true; live;) {
+    if (!wk.has()) wk.open(a);
+    uint64_t w[kRecRound][L + 1];
+    uint32_t bin[kRecRound];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int j = 0; j < kRecRound; ++j)
+      if (wk.has() && wk.take(a, 0u, w[j], bin[j])) pend |= 1u << j;
+    mine += __builtin_popcount(pend);
+    if (wk.alive()) s_live[par] = 1u;
+    for (;;) {
+#pragma unroll
+      for (int j = 0; j < kRecRound; ++j) {
+        if (!((pend >> j) & 1u)) continue;
+        const uint32_t b = bin[j];
+        const uint32_t slot = atomicAdd(cl + b, 1u) & 0xFFFFu;
+        if (slot < E) {
+          uint64_t* d = buf + (uint64_t)b * kP2BufWords + slot * rw;
+#pragma unroll
+          for (int q = 0; q <= L; ++q)
+            if ((uint32_t)q < rw) d[q] = w[j][q];
+          if (slot == E - 1u) full[atomicAdd(s_nfull + par, 1u)] = b;
+          pend &= ~(1u << j);
+        }
+      }
+      if (pend) s_pend[par] = 1u;
+      __syncthreads();
+      const uint32_t nfull = s_nfull[par];
+      const bool again = s_pend[par] != 0u;
+      live = s_live[par] != 0u || again;
+      if (threadIdx.x == 0) s_nfull[par ^ 1u] = s_pend[par ^ 1u] = s_live[par ^ 1u] = 0u;
+      for (uint32_t f = threadIdx.x; f < nfull; f += blockDim.x) {
+        const uint32_t b = full[f];
+        if ((cl[b] >> 16) == 0u) reserve(b);
+      }
+      __syncthreads();
+      for (uint32_t f = threadIdx.x >> 3; f < nfull; f += blockDim.x >> 3) {
+        const uint32_t b = full[f];
+        const uint32_t p = pos[b];
+        if (h < E * rw && p != kNoPos) recs[(uint64_t)p * rw + h] = buf[(uint64_t)b * kP2BufWords + h];
+        if (h == 0) {
+          if (p != kNoPos) pos[b] = p + E;
+          cl[b] = ((cl[b] >> 16) - 1u) << 16;  // one unit fewer, buffer empty
+        }
+      }
+      par ^= 1u;
+      __syncthreads();
+      if (!again) break;
+      if (wk.alive()) s_live[par] = 1u;
+    }
+  }
+  // every buffer holds fewer than E records: a bin with records and no unit
+  // left takes a chunk, then its buffer and zeros fill its current chunk
+  if (mine) atomicAdd(&s_recs, mine);
+  for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x)
+    if ((cl[b] & 0xFFFFu) && (cl[b] >> 16) == 0u) reserve(b);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_recs) atomicAdd(a.status + 1, s_recs);
+  for (uint32_t b = threadIdx.x >> 3; b < nb; b += blockDim.x >> 3) {
+    const uint32_t c = cl[b] & 0xFFFFu, units = cl[b] >> 16, p = pos[b];
+    if (p == kNoPos || h >= E * rw) continue;
+    for (uint32_t u = 0; u < units; ++u) {
+      const uint64_t v = (u == 0 && h < c * rw) ? buf[(uint64_t)b * kP2BufWords + h] : 0ull;
+      recs[((uint64_t)p + (uint64_t)u * E) * rw + h] = v;
+    }
+  }
+}
+
+template <int L>
+__global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_chunk(P2BinArgs a) {
+  extern __shared__ uint64_t sh64[];
+  p2_scatter_chunk_body<L>(a, blockIdx.x, gridDim.x, sh64);
+}
+
+uint32_t p2_chunk_grid() {
+  static uint32_t g = 0;
+  if (!g) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus <= 0) {
+      (void)hipGetLastError();
+      cus = 256;
+    }
+    g = (uint32_t)cus;
+  }
+  return g;
+}
+
 // Whether phase A of nbins bins per round and k uses the buffered scatter.
 bool p2_scatter_buffered(uint32_t k, uint32_t nbins) { return (2u * k + 63u) / 64u <= 3u && nbins <= kP2BufMaxBins; }
 
@@ -1080,27 +1227,43 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
   for (uint32_t i = threadIdx.x; i < 3 * cap; i += blockDim.x) cov[i] = 0u;  // cov and edges
   if (threadIdx.x < 2) s_stat[threadIdx.x] = 0ull;
   __syncthreads();
-  // the bin's kSlices regions are adjacent: [start of its first slice, start
-  // of the next bin's first slice)
-  const uint64_t e0 = a.runs[(uint64_t)bin * kSlices];
-  uint64_t e1 = a.runs[(uint64_t)(bin + 1) * kSlices];
-  if (e1 > a.rec_cap) e1 = a.rec_cap;
-  const uint32_t n = e1 > e0 ? (uint32_t)(e1 - e0) : 0u;
   const uint32_t rw = a.rw;
   const uint32_t gate = a.min_trans_cov;
+  // f(record) over the bin's records: with exact runs the bin's kSlices
+  // regions are adjacent, [start of its first slice, start of the next bin's
+  // first slice); with chunked regions each slice's region holds its
+  // reserved chunks, whose all-zero tail records are skipped
+  auto each_record = [&](auto&& f) {
+    if (a.region_cap) {
+      for (uint32_t sl = 0; sl < kSlices; ++sl) {
+        const uint64_t r = (uint64_t)bin * kSlices + sl;
+        const uint64_t fill = a.runs[r];
+        const uint32_t n = (uint32_t)(fill < a.region_cap ? fill : a.region_cap);
+        for_records<L>(a.recs + r * a.region_cap * rw, n, rw, [&](const uint64_t (&cur)[L + 1]) {
+          Kmer<L> key;
+          uint32_t info;
+          decode_record<L>(cur, a.k, key, info);
+          if (info & 0x11u) f(key, info);  // a left or a right pair: not a tail record
+        });
+      }
+    } else {
+      const uint64_t e0 = a.runs[(uint64_t)bin * kSlices];
+      uint64_t e1 = a.runs[(uint64_t)(bin + 1) * kSlices];
+      if (e1 > a.rec_cap) e1 = a.rec_cap;
+      const uint32_t n = e1 > e0 ? (uint32_t)(e1 - e0) : 0u;
+      for_records<L>(a.recs + e0 * rw, n, rw, [&](const uint64_t (&cur)[L + 1]) {
+        Kmer<L> key;
+        uint32_t info;
+        decode_record<L>(cur, a.k, key, info);
+        f(key, info);
+      });
+    }
+  };
   if (nf) {
-    for_records<L>(a.recs + e0 * rw, n, rw, [&](const uint64_t (&cur)[L + 1]) {
-      Kmer<L> key;
-      uint32_t info;
-      decode_record<L>(cur, a.k, key, info);
-      filt_add<L>(filt, nf, key, (info >> 8) & 1u);
-    });
+    each_record([&](const Kmer<L>& key, uint32_t info) { filt_add<L>(filt, nf, key, (info >> 8) & 1u); });
     __syncthreads();
   }
-  for_records<L>(a.recs + e0 * rw, n, rw, [&](const uint64_t (&cur)[L + 1]) {
-    Kmer<L> key;
-    uint32_t info;
-    decode_record<L>(cur, a.k, key, info);
+  each_record([&](const Kmer<L>& key, uint32_t info) {
     if (nf && !((info >> 8) & 1u) && !filt_pass<L>(filt, nf, key)) return;  // fewer than nf records
     uint64_t w[KW];
     pack62b<L, KW>(key, w);
@@ -1258,6 +1421,7 @@ static hipError_t set_pair_attr() {
   if constexpr (L <= 3) {
     if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf<L>));
     if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf_m<L>));
+    if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_chunk<L>));
   }
   return e;
 }
@@ -1324,6 +1488,21 @@ struct P2Launch {
       P2BinArgs ar = a;
       ar.round = round;
       if (hipError_t e = launch_fill_u32(a.runs, (uint64_t)a.nbins * kSlices + 1, 0u, s)) return e;
+      if constexpr (L <= 3) {
+        if (a.region_cap) {  // no count walk: chunks reserved as the records go out
+          if (gated)
+            if (hipError_t e = hipEventRecord(gated, s)) return e;
+          if (threads)
+            hipLaunchKernelGGL((k_p2_scatter_chunk<L>), dim3(p2_chunk_grid()), dim3(kP2ScatterBlock), scat_lds, s,
+                               ar);
+          if (hipError_t e = hipGetLastError()) return e;
+          if (scattered)
+            if (hipError_t e = hipEventRecord(scattered, s)) return e;
+          hipLaunchKernelGGL((k_p2_bins<L, KW>), dim3(a.nbins), dim3(kP2BinBlock), lds, s, ar);
+          if (hipError_t e = hipGetLastError()) return e;
+          continue;
+        }
+      }
       if (threads)
         hipLaunchKernelGGL((k_p2_count<L>), dim3(scatter_blocks(threads, a.chunks)), dim3(kP2ScatterBlock),
                            (size_t)a.nbins * 4, s, ar);
@@ -1378,6 +1557,12 @@ hipError_t launch_p2_binned(const P2BinArgs& a, hipStream_t s, hipEvent_t gated,
   const uint64_t threads = read_threads + a.contigs.n_reads * (uint64_t)a.csplit;
   if (a.n_blocks != scatter_blocks(threads, a.chunks)) return hipErrorInvalidValue;
   if (a.log2_rounds > kP2MaxLog2Rounds || a.nbins != (1u << a.log2_nbins)) return hipErrorInvalidValue;
+  if (a.region_cap) {
+    const uint32_t unit = kP2BufWords / a.rw;
+    if (!buffered || !a.ungated || a.log2_rounds || a.region_cap % (kP2ChunkUnits * unit) ||
+        (uint64_t)a.nbins * kSlices * a.region_cap >= (1ull << 32))
+      return hipErrorInvalidValue;
+  }
   return dispatch_lkw(a.k, P2Launch{a, threads, read_threads, scat_lds, lds, buffered, s, gated, scattered});
 }
 

@@ -834,6 +834,7 @@ struct fb_solid {
   uint64_t stats[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool sorted = false;  // records sorted by k-mer (the parity form)
   int p2_gate = 0;      // as fb_solid_pass2_gate
+  int p2_layout = 0;    // as fb_solid_pass2_layout
 };
 
 extern "C" void fb_build_params_default(fb_build_params* p, uint32_t k) {
@@ -1161,10 +1162,23 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     const uint64_t out_cap = std::min<uint64_t>(lds_out + spill_cap, 0xFFFFFFFFull);
     const uint32_t chunks2 = prm->chunks2 ? prm->chunks2 : scatter_chunks(threads2);
     const uint64_t n_blocks2 = scatter_blocks(threads2, chunks2);
+    // Without the count walk (ungated, one round, the buffered scatter's
+    // bins): per (bin, slice) regions of the expected records plus a fifth,
+    // and two chunks for each workgroup of the slice's XCD; a region that
+    // fills up sends the pass to a second run with exact runs.
+    uint64_t region_cap = 0;
+    if (ungated && log2_rounds == 0 && p2_scatter_buffered(k, (uint32_t)nbr) && prm->placement != 3) {
+      const uint64_t CR = (uint64_t)kP2ChunkUnits * (kP2BufWords / rw);
+      const double per = (double)rec_cap / (double)(nbr * kBinSlices);
+      uint64_t rc = (uint64_t)(1.2 * per) + (p2_chunk_grid() / kBinSlices + 1) * 2 * CR;
+      rc = (rc + CR - 1) / CR * CR;
+      if (nbr * kBinSlices * rc < (1ull << 32)) region_cap = rc;
+    }
     DBuf recs, gm, runs, bh, st2, p3s;
     int r = 0;
     if (!r) r = bh.get(c, std::max<uint64_t>(n_blocks2 * nbr, 1) * 4);
-    if (!r) r = recs.get(c, rec_cap * rw * 8 + 64);  // the bin kernel reads a word past the last record
+    // the bin kernel reads a word past the last record
+    if (!r) r = recs.get(c, std::max<uint64_t>(rec_cap, nbr * kBinSlices * region_cap) * rw * 8 + 64);
     if (!r && !ungated) r = gm.get(c, std::max<uint64_t>(threads2, 1) * 8);
     if (!r) r = runs.get(c, (nbr * kBinSlices + 1) * 4);
     if (!r) r = st2.get(c, 64);
@@ -1177,8 +1191,6 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     if (r) return fail(r);
     tab.cap = spill_cap;
     tab.slots = tbuf.as<uint64_t>();
-    HIP_TRY(c, launch_table_clear(tab, s));
-    HIP_TRY(c, launch_fill_u32(st2.as<uint32_t>(), 16, 0, s));
     P2BinArgs pa;
     pa.reads = bank->view();
     pa.contigs = cview;
@@ -1213,17 +1225,25 @@ extern "C" int fb_build_solid(fb_ctx* c, const fb_bank* bank, const uint8_t* con
     pa.out_keys = so->keys.as<uint64_t>();
     pa.out_cov = so->cov.as<uint16_t>();
     pa.out_mask = so->mask.as<uint8_t>();
-    HIP_TRY(c, hipEventRecord(c->ev[2], s));
-    HIP_TRY(c, launch_p2_binned(pa, s, c->ev[9], c->ev[10]));
-    HIP_TRY(c, hipEventRecord(c->ev[3], s));
-    HIP_TRY(c, hipEventRecord(c->ev[4], s));
-    HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
-                            so->mask.as<uint8_t>(), p3s.p, s, pa.out_count, (uint32_t)out_cap));
+    pa.region_cap = region_cap;
     uint32_t hb[8] = {0}, sp[4] = {0}, ts[2] = {0};
-    HIP_TRY(c, hipMemcpyAsync(hb, st2.p, 32, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipMemcpyAsync(sp, p3s.p, 16, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipMemcpyAsync(ts, tab.status, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
+    for (;;) {
+      HIP_TRY(c, launch_table_clear(tab, s));
+      HIP_TRY(c, launch_fill_u32(st2.as<uint32_t>(), 16, 0, s));
+      HIP_TRY(c, hipEventRecord(c->ev[2], s));
+      HIP_TRY(c, launch_p2_binned(pa, s, c->ev[9], c->ev[10]));
+      HIP_TRY(c, hipEventRecord(c->ev[3], s));
+      HIP_TRY(c, hipEventRecord(c->ev[4], s));
+      HIP_TRY(c, launch_pass3(tab, k, so->nref, gate, so->keys.as<uint64_t>(), so->cov.as<uint16_t>(),
+                              so->mask.as<uint8_t>(), p3s.p, s, pa.out_count, (uint32_t)out_cap));
+      HIP_TRY(c, hipMemcpyAsync(hb, st2.p, 32, hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipMemcpyAsync(sp, p3s.p, 16, hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipMemcpyAsync(ts, tab.status, 8, hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipStreamSynchronize(s));
+      so->p2_layout = pa.region_cap ? 1 : (region_cap ? 2 : 0);
+      if (!(hb[0] && pa.region_cap)) break;
+      pa.region_cap = 0;  // a chunked region filled up: again with exact runs
+    }
     const bool out_full = (uint64_t)hb[2] + sp[0] > out_cap;
     if (!hb[0] && !ts[0] && !out_full) {
       uint64_t bins_rec = 0, bins_keys = 0, sp_rec = 0;
@@ -1374,6 +1394,7 @@ extern "C" int fb_solid_stats(const fb_solid* s, uint64_t out[11]) {
   return FB_OK;
 }
 extern "C" int fb_solid_pass2_gate(const fb_solid* s) { return s ? s->p2_gate : FB_ERR_INVALID; }
+extern "C" int fb_solid_pass2_layout(const fb_solid* s) { return s ? s->p2_layout : FB_ERR_INVALID; }
 
 // ---------------------------------------------------------------------------
 // Genome estimator

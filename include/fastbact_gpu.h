@@ -241,6 +241,12 @@ int fb_solid_stats(const fb_solid* s, uint64_t out[11]);
  * key).  Both give the reference's set.  Diagnostics, no counterpart in
  * the reference. */
 int fb_solid_pass2_gate(const fb_solid* s);
+/* Record layout of the LDS route's pass 2: 0 exact runs (a count walk
+ * before the scatter), 1 chunked regions (no count walk: the ungated form
+ * with the buffered scatter), 2 chunked regions that filled up, then exact
+ * runs.  Every layout gives the same set.  Diagnostics, no counterpart in
+ * the reference. */
+int fb_solid_pass2_layout(const fb_solid* s);
 
 /* ---- Unitig compaction: build_cdbg_with_contigs_ext step 4-6 (build.rs:330-597)
  * Walks from boundary directed k-mers through 1-in/1-out k-mers, isolated

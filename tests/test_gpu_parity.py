@@ -428,3 +428,41 @@ def test_pass2_rounds_match_oracle(ctx, orc, small_iso, k, bins2):
         raw.append(h3.stats()["records"])
         h3.close()
     assert raw[0] == raw[1]
+
+
+def _repeat_heavy_bank(seed: int, n_random: int = 12_000, n_repeat: int = 12_000):
+    """Reads of a random 150 kb genome plus as many copies of one read made
+    of a period-8 tandem repeat, whose few k-mers carry half the records."""
+    rng = np.random.default_rng(seed)
+    genome = rng.choice(list(b"ACGT"), size=150_000).astype(np.uint8).tobytes()
+    reads = []
+    for _ in range(n_random):
+        s0 = int(rng.integers(0, len(genome) - 150))
+        reads.append(genome[s0:s0 + 150])
+    rep = (b"AACGTTGC" * 19)[:150]
+    reads += [rep] * n_repeat
+    half = len(reads) // 2
+    return F.ReadBank(F.BitSeqArena.from_ascii(reads[:half]), F.BitSeqArena.from_ascii(reads[half:]))
+
+
+@pytest.mark.parametrize("k", [21, 55])
+def test_chunked_regions_fill_up_then_exact_runs(ctx, orc, small_iso, k):
+    """The ungated pass 2 without its count walk writes each bin's records
+    into per-slice regions sized for an even spread.  A tandem-repeat read
+    present 12,000 times sends half of all records to the few bins of its
+    k-mers: their regions fill up, the pass runs again with exact runs, and
+    the set is the oracle's.  An ordinary isolate stays on the chunked layout."""
+    bank = _repeat_heavy_bank(seed=k)
+    dev = F.DeviceBank(ctx, bank)
+    kw = dict(target_solid_cov=3, cms_width=1 << 20, placement=2, bins2=64)
+    g = F.build_solid_kmers(ctx, dev, k, **kw)
+    r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=3, width=1 << 20)
+    _solid_equal(g, r)
+    h = F.build_solid_handle(ctx, dev, k, **kw)
+    assert h.pass2_gate() == "key" and h.pass2_layout() == "chunks_then_runs"
+    h.close()
+    dev.close()
+    dev = F.DeviceBank(ctx, small_iso.bank())
+    h = F.build_solid_handle(ctx, dev, k, target_solid_cov=3, cms_width=1 << 18, placement=2)
+    assert h.pass2_layout() == "chunks"
+    h.close()

@@ -258,17 +258,17 @@ __device__ __forceinline__ uint32_t gate_loss(const P2BinArgs& a, const Kmer<L>&
 // LDS record filter of the ungated bin kernel: kP2FiltWords words per level.
 constexpr uint32_t kFiltLog2Bits = 16;
 static_assert((1u << kFiltLog2Bits) == 32u * kP2FiltWords, "filter level size");
+// The two positions of a key: bits of one multiply of its premixed fold
+// (as the bin hash, with another multiplier).  They only decide which keys
+// the filter lets through, never a result; on real k-mers (k = 21, 55, 77,
+// the benchmark's bins and error rates) this lets through as few error
+// keys as a multiply chain over all the key's limbs (0.2-0.5%).
 template <int L>
 __device__ __forceinline__ void filt_positions(const Kmer<L>& key, uint32_t& p0, uint32_t& p1) {
-  uint64_t h = 0x8cb92ba72f3d8dd7ull;
-#pragma unroll
-  for (int j = 0; j < L; ++j) {
-    h = (h ^ key.a[j]) * 0x9e3779b97f4a7c15ull;
-    h ^= h >> 29;
-  }
-  h *= 0xbf58476d1ce4e5b9ull;
+  const uint64_t f = kmer_fold<L>(key);
+  const uint64_t h = (f ^ (f >> 29)) * 0xbf58476d1ce4e5b9ull;
   p0 = (uint32_t)(h >> (64u - kFiltLog2Bits));
-  p1 = (uint32_t)(h >> 8) & ((1u << kFiltLog2Bits) - 1u);
+  p1 = (uint32_t)(h >> 24) & ((1u << kFiltLog2Bits) - 1u);
 }
 // One record at both positions: it sets the lowest level not yet set there
 // (an atomicOr that finds the bit set moves on to the next level), so after

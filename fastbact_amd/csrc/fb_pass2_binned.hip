@@ -258,14 +258,12 @@ __device__ __forceinline__ uint32_t gate_loss(const P2BinArgs& a, const Kmer<L>&
 // LDS record filter of the ungated bin kernel: kP2FiltWords words per level.
 constexpr uint32_t kFiltLog2Bits = 16;
 static_assert((1u << kFiltLog2Bits) == 32u * kP2FiltWords, "filter level size");
-// The two positions of a key: bits of one multiply of its premixed fold
+// The two positions of a key of fold f: bits of one multiply of the premixed fold
 // (as the bin hash, with another multiplier).  They only decide which keys
 // the filter lets through, never a result; on real k-mers (k = 21, 55, 77,
 // the benchmark's bins and error rates) this lets through as few error
 // keys as a multiply chain over all the key's limbs (0.2-0.5%).
-template <int L>
-__device__ __forceinline__ void filt_positions(const Kmer<L>& key, uint32_t& p0, uint32_t& p1) {
-  const uint64_t f = kmer_fold<L>(key);
+__device__ __forceinline__ void filt_positions(uint64_t f, uint32_t& p0, uint32_t& p1) {
   const uint64_t h = (f ^ (f >> 29)) * 0xbf58476d1ce4e5b9ull;
   p0 = (uint32_t)(h >> (64u - kFiltLog2Bits));
   p1 = (uint32_t)(h >> 24) & ((1u << kFiltLog2Bits) - 1u);
@@ -275,10 +273,9 @@ __device__ __forceinline__ void filt_positions(const Kmer<L>& key, uint32_t& p0,
 // all records level l is set where at least l + 1 records landed.  A prior
 // contig's record sets the top level: its key (and any key sharing both
 // positions) always passes.
-template <int L>
-__device__ __forceinline__ void filt_add(uint32_t* f, uint32_t nf, const Kmer<L>& key, bool contig) {
+__device__ __forceinline__ void filt_add(uint32_t* f, uint32_t nf, uint64_t fold, bool contig) {
   uint32_t pos[2];
-  filt_positions<L>(key, pos[0], pos[1]);
+  filt_positions(fold, pos[0], pos[1]);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     const uint32_t wd = pos[j] >> 5, bit = 1u << (pos[j] & 31u);
@@ -292,10 +289,9 @@ __device__ __forceinline__ void filt_add(uint32_t* f, uint32_t nf, const Kmer<L>
       if (!(atomicOr(f + l * kP2FiltWords + wd, bit) & bit)) break;
   }
 }
-template <int L>
-__device__ __forceinline__ bool filt_pass(const uint32_t* f, uint32_t nf, const Kmer<L>& key) {
+__device__ __forceinline__ bool filt_pass(const uint32_t* f, uint32_t nf, uint64_t fold) {
   uint32_t p0, p1;
-  filt_positions<L>(key, p0, p1);
+  filt_positions(fold, p0, p1);
   const uint32_t* top = f + (nf - 1u) * kP2FiltWords;
   return ((lds_vload(top + (p0 >> 5)) >> (p0 & 31u)) & (lds_vload(top + (p1 >> 5)) >> (p1 & 31u)) & 1u) != 0u;
 }
@@ -1051,24 +1047,20 @@ uint32_t p2_chunk_grid() {
 // Whether phase A of nbins bins per round and k uses the buffered scatter.
 bool p2_scatter_buffered(uint32_t k, uint32_t nbins) { return (2u * k + 63u) / 64u <= 3u && nbins <= kP2BufMaxBins; }
 
-// LDS hash of a packed key: its home slot inside the bin's table.  It sets
-// only where a key is placed (probe lengths), never a result, so one
-// multiply per key word and a fold of the high half suffice; the odd
-// multipliers differ per word, and the bin was chosen by an unrelated hash
-// (bin_hash of the k-mer's fold), so the keys of a bin spread evenly.
-template <int KW>
-__device__ __forceinline__ uint32_t lds_slot(const uint64_t (&w)[KW], uint32_t cap) {
-  constexpr uint64_t kMul[5] = {0x9e3779b97f4a7c15ull, 0xc2b2ae3d27d4eb4full, 0x165667b19e3779f9ull,
-                                0xd6e8feb86659fd93ull, 0xff51afd7ed558ccdull};
-  uint64_t h = 0;
-#pragma unroll
-  for (int j = 0; j < KW; ++j) h ^= (w[j] ^ (w[j] >> 31)) * kMul[j];
+// Home slot of a key of fold f inside its bin's table.  It sets only where
+// a key is placed (probe lengths), never a result, so one multiply of the
+// premixed fold and a fold of the product's halves suffice; its multiplier
+// and shift differ from the bin hash's and the filter's, and over the
+// canonical k-mers of a random genome (k = 21, 55, 77) the keys of one bin
+// fill its 4-slot buckets as unevenly as random keys do, no more.
+__device__ __forceinline__ uint32_t lds_slot(uint64_t f, uint32_t cap) {
+  const uint64_t h = (f ^ (f >> 31)) * 0xc2b2ae3d27d4eb4full;
   const uint32_t x = (uint32_t)(h >> 32) ^ (uint32_t)h;
   return (uint32_t)(((uint64_t)x * (uint64_t)cap) >> 32);
 }
 
 // Finds or claims the LDS entry of key within kLdsMaxProbe slots of its
-// home bucket; -1 when all of them hold other keys.  Slots are probed in
+// home bucket (the bucket of slot home); -1 when all of them hold other keys.  Slots are probed in
 // aligned buckets of kLdsBucket: one 32-byte read brings a bucket's first
 // key words, so a wave's slowest lane needs few LDS round trips (probing
 // slot by slot, the longest of 64 probe runs set the wave's time).  Entries
@@ -1080,8 +1072,8 @@ __device__ __forceinline__ uint32_t lds_slot(const uint64_t (&w)[KW], uint32_t c
 constexpr uint32_t kLdsMaxProbe = 128;
 constexpr uint32_t kLdsBucket = 4;
 template <int KW>
-__device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, const uint64_t (&w)[KW]) {
-  uint32_t s = lds_slot<KW>(w, cap) & ~(kLdsBucket - 1u);
+__device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, const uint64_t (&w)[KW], uint32_t home) {
+  uint32_t s = home & ~(kLdsBucket - 1u);
   const uint32_t buckets = (cap < kLdsMaxProbe ? cap : kLdsMaxProbe) / kLdsBucket;
   for (uint32_t b = 0; b < buckets; ++b) {
     unsigned long long v0[kLdsBucket];
@@ -1260,14 +1252,15 @@ __device__ __forceinline__ void p2_bins_body(const P2BinArgs& a, uint32_t bin, u
     }
   };
   if (nf) {
-    each_record([&](const Kmer<L>& key, uint32_t info) { filt_add<L>(filt, nf, key, (info >> 8) & 1u); });
+    each_record([&](const Kmer<L>& key, uint32_t info) { filt_add(filt, nf, kmer_fold<L>(key), (info >> 8) & 1u); });
     __syncthreads();
   }
   each_record([&](const Kmer<L>& key, uint32_t info) {
-    if (nf && !((info >> 8) & 1u) && !filt_pass<L>(filt, nf, key)) return;  // fewer than nf records
+    const uint64_t fold = kmer_fold<L>(key);
+    if (nf && !((info >> 8) & 1u) && !filt_pass(filt, nf, fold)) return;  // fewer than nf records
     uint64_t w[KW];
     pack62b<L, KW>(key, w);
-    p2_table_add<KW>(a, cov, edges, lds_find<KW>(keys, cap, w), w, info);
+    p2_table_add<KW>(a, cov, edges, lds_find<KW>(keys, cap, w, lds_slot(fold, cap)), w, info);
   });
   __syncthreads();
   // pass 3 on the LDS table: count, one atomic per block, write

@@ -162,17 +162,14 @@ __device__ __forceinline__ T lds_vload(const T* p) {
   return *(lds_ptr)(p);
 }
 
-// Four consecutive 64-bit LDS words from a 32-byte aligned p, fresh, in two
-// 16-byte reads.
-__device__ __forceinline__ void lds_vload_bucket(const unsigned long long* p, unsigned long long (&v)[4]) {
+// Two consecutive 64-bit LDS words from a 16-byte aligned p, fresh, in one
+// 16-byte read.
+__device__ __forceinline__ void lds_vload_bucket(const unsigned long long* p, unsigned long long (&v)[2]) {
   typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   typedef __attribute__((address_space(3))) const volatile u64x2* lds_ptr;
   const u64x2 a = *(lds_ptr)(p);
-  const u64x2 b = *((lds_ptr)(p) + 1);
   v[0] = a.x;
   v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
 }
 
 // Bits of every window of the share p0..p1 (p1 - p0 <= 63 pairs).
@@ -1052,7 +1049,7 @@ bool p2_scatter_buffered(uint32_t k, uint32_t nbins) { return (2u * k + 63u) / 6
 // premixed fold and a fold of the product's halves suffice; its multiplier
 // and shift differ from the bin hash's and the filter's, and over the
 // canonical k-mers of a random genome (k = 21, 55, 77) the keys of one bin
-// fill its 4-slot buckets as unevenly as random keys do, no more.
+// spread over the table's buckets as evenly as random keys do.
 __device__ __forceinline__ uint32_t lds_slot(uint64_t f, uint32_t cap) {
   const uint64_t h = (f ^ (f >> 31)) * 0xc2b2ae3d27d4eb4full;
   const uint32_t x = (uint32_t)(h >> 32) ^ (uint32_t)h;
@@ -1061,16 +1058,18 @@ __device__ __forceinline__ uint32_t lds_slot(uint64_t f, uint32_t cap) {
 
 // Finds or claims the LDS entry of key within kLdsMaxProbe slots of its
 // home bucket (the bucket of slot home); -1 when all of them hold other keys.  Slots are probed in
-// aligned buckets of kLdsBucket: one 32-byte read brings a bucket's first
-// key words, so a wave's slowest lane needs few LDS round trips (probing
-// slot by slot, the longest of 64 probe runs set the wave's time).  Entries
+// aligned buckets of kLdsBucket: one 16-byte read brings a bucket's first
+// key words.  Buckets of two were measured faster than of four (bin kernel
+// 4.61/5.11/3.99 -> 4.41/4.94/3.84 ms at k = 21/55/77) and of eight (5.6 /
+// 6.0 / 4.9 ms): the reads of a probe cost more than the extra round trips
+// of the few lanes that go past their home bucket.  Entries
 // are written once, so a key that finds its probe window full finds it
 // full at every later lookup too and lives in the global table for the
 // whole pass, and a key claimed in the window is found there by every
 // later lookup.  The bound keeps a crowded bin from scanning its whole
 // table for every new key.  cap is a multiple of kLdsBucket.
 constexpr uint32_t kLdsMaxProbe = 128;
-constexpr uint32_t kLdsBucket = 4;
+constexpr uint32_t kLdsBucket = 2;
 template <int KW>
 __device__ __forceinline__ int lds_find(unsigned long long* keys, uint32_t cap, const uint64_t (&w)[KW], uint32_t home) {
   uint32_t s = home & ~(kLdsBucket - 1u);

@@ -1648,7 +1648,7 @@ extern "C" int fb_plan_pass2(double est_keys, uint64_t windows, uint64_t n_reads
                              fb_pass2_plan* out) {
   if (!out || k < 1 || k > 127) return FB_ERR_INVALID;
   const uint32_t kw = key_words_for_k(k);
-  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / p2_entry_bytes(kw)) & ~3u;  // whole 4-slot buckets
+  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / p2_entry_bytes(kw)) & ~1u;  // whole 2-slot buckets
   const P2Plan q = plan_pass2(est_keys, windows, n_reads, max_len, k, default_split(k, max_len), 0, lds_cap, 16384u, 0);
   out->nb2 = q.nb2;
   out->nbr = q.nbr;
@@ -1707,7 +1707,7 @@ static IsoBytes iso_bytes(uint64_t genome_size, uint64_t bases, uint64_t reads, 
   const double n_est = (double)windows;
   // host-only sizing: the LDS of a workgroup with the full 160 KB granted
   const uint32_t kw = key_words_for_k(k);
-  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / p2_entry_bytes(kw)) & ~3u;  // whole 4-slot buckets
+  const uint32_t lds_cap = ((kP2LdsBytes - 1024u) / p2_entry_bytes(kw)) & ~1u;  // whole 2-slot buckets
   const P2Plan q = plan_pass2(n_est, windows, reads, max_len, k, default_split(k, max_len), p->bins2, lds_cap, 16384u,
                               p->chunks2);
   const uint32_t rw = (2 * k + 8 + 63) / 64;

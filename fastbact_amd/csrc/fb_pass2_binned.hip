@@ -885,13 +885,21 @@ __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_buf(P2BinArgs a)
   p2_scatter_buf_body<L>(a, blockIdx.x, sh64, a.round);
 }
 
-template <int L>
+// A batch's scatter: an isolate's workgroups take the buffered body when it
+// has at most kP2BufMaxBins bins and the direct one otherwise (the choice is
+// the workgroup's, so its barriers stay uniform); both write the runs of the
+// count walk.
+template <int L, int KW>
 __global__ __launch_bounds__(kP2ScatterBlock) void k_p2_scatter_buf_m(const P2BinArgs* __restrict__ args,
                                                                      const uint32_t* __restrict__ blk, uint32_t n_iso,
                                                                      uint32_t round) {
   extern __shared__ uint64_t sh64[];
   const uint32_t iso = find_iso(blk, n_iso, blockIdx.x);
-  p2_scatter_buf_body<L>(args[iso], blockIdx.x - blk[iso], sh64, round);
+  const P2BinArgs& a = args[iso];
+  if (a.nbins <= kP2BufMaxBins)
+    p2_scatter_buf_body<L>(a, blockIdx.x - blk[iso], sh64, round);
+  else
+    p2_scatter_body<L, KW>(a, blockIdx.x - blk[iso], reinterpret_cast<uint32_t*>(sh64), round);
 }
 
 // Phase A without the count walk (P2BinArgs::region_cap != 0; the ungated
@@ -1412,7 +1420,7 @@ static hipError_t set_pair_attr() {
   if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_m<L, KW>));
   if constexpr (L <= 3) {
     if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf<L>));
-    if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf_m<L>));
+    if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_buf_m<L, KW>));
     if (e == hipSuccess) e = set_attr(reinterpret_cast<const void*>(&k_p2_scatter_chunk<L>));
   }
   return e;
@@ -1587,8 +1595,8 @@ struct P2LaunchMulti {
         const P2BinArgs* sa = static_cast<const P2BinArgs*>(scatter.args);
         if constexpr (L <= 3) {
           if (buffered)
-            hipLaunchKernelGGL((k_p2_scatter_buf_m<L>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds,
-                               s, sa, scatter.blk, scatter.n_iso, round);
+            hipLaunchKernelGGL((k_p2_scatter_buf_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock),
+                               scat_lds, s, sa, scatter.blk, scatter.n_iso, round);
           else
             hipLaunchKernelGGL((k_p2_scatter_m<L, KW>), dim3(scatter.total_blocks), dim3(kP2ScatterBlock), scat_lds,
                                s, sa, scatter.blk, scatter.n_iso, round);
@@ -1616,8 +1624,14 @@ hipError_t launch_p2_binned_multi(const MultiLaunch& gate, const MultiLaunch& sc
                                   const FillRange* runs_fill, uint32_t n_fill, uint32_t k, uint32_t max_nbins,
                                   uint32_t max_rounds, uint32_t lds_cap, uint32_t filt, hipStream_t s) {
   const uint32_t KW = (k + 30) / 31;
-  const bool buffered = p2_scatter_buffered(k, max_nbins);
-  const size_t scat_lds = buffered ? p2_scatter_buf_lds(max_nbins) : p2_scatter_lds(max_nbins);
+  // every isolate of a batch may take the buffered body when the keys fit
+  // it (an isolate with more bins than kP2BufMaxBins takes the direct one)
+  const bool buffered = p2_scatter_buffered(k, 1u);
+  size_t scat_lds = p2_scatter_lds(max_nbins);
+  if (buffered) {
+    const size_t buf_lds = p2_scatter_buf_lds(max_nbins < kP2BufMaxBins ? max_nbins : kP2BufMaxBins);
+    if (buf_lds > scat_lds) scat_lds = buf_lds;
+  }
   const size_t lds = p2_lds_bytes(KW, lds_cap, filt);
   if (lds + 1024 > lds_budget() || scat_lds + 1024 > lds_budget()) return hipErrorInvalidConfiguration;
   if (max_rounds == 0 || max_rounds > (1u << kP2MaxLog2Rounds) || filt > kP2MaxFilt) return hipErrorInvalidValue;

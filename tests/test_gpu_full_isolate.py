@@ -112,3 +112,28 @@ def test_prefix_300k_pairs_equals_oracle(ctx, orc, full):
             assert np.array_equal(g.hll_registers, r["hll"])
     finally:
         dev.close()
+
+
+def test_batch_of_full_isolate_and_prefix_matches_plain(ctx, full):
+    """One batch holding the full isolate and its 300k-pair prefix: at
+    k = 77 the full isolate's pass 2 takes 4096 bins (the direct record
+    scatter) and the prefix's at most 2048 (the buffered one), both in the
+    same launch; each isolate's set, table and registers equal its plain
+    build's."""
+    bank, dev = full
+    sub = F.DeviceBank(ctx, head_pairs(bank, 300_000))
+    try:
+        batch = F.Batch(ctx, [dev, sub], [GENOME["genome_len"]] * 2, cms_width=WIDTH, counter_bits=16, hll_p=16,
+                        placement=2)
+        for k, tgt in ((77, 4), (55, 4)):
+            batch.run(k, tgt)
+            assert batch.routes() == ["lds", "lds"]
+            for i, d in enumerate((dev, sub)):
+                single = _build(ctx, d, k, tgt)
+                got = batch.result(i, k)
+                assert np.array_equal(got.keys, single.keys), (k, i)
+                assert np.array_equal(got.cov, single.cov) and np.array_equal(got.mask, single.mask)
+                assert np.array_equal(got.cms_table, single.cms_table)
+                assert np.array_equal(got.hll_registers, single.hll_registers)
+    finally:
+        sub.close()

@@ -782,18 +782,30 @@ __global__ __launch_bounds__(kBlock) void k_gather_limb(const uint64_t* __restri
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = keys[(uint64_t)perm[i] * nref + word];
 }
-__global__ __launch_bounds__(kBlock) void k_gather_records(const uint64_t* __restrict__ keys,
-                                                           const uint16_t* __restrict__ cov,
-                                                           const uint8_t* __restrict__ mask,
+// A solid k-mer's record side by side: its nref key words, then
+// cov | mask << 16, so that the gather after the sort reads one run of
+// bytes per record instead of one access per array.
+__global__ __launch_bounds__(kBlock) void k_pack_records(const uint64_t* __restrict__ keys,
+                                                         const uint16_t* __restrict__ cov,
+                                                         const uint8_t* __restrict__ mask, uint32_t n,
+                                                         uint32_t nref, uint64_t* __restrict__ rec) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t* r = rec + (uint64_t)i * (nref + 1);
+  for (uint32_t w = 0; w < nref; ++w) r[w] = keys[(uint64_t)i * nref + w];
+  r[nref] = (uint64_t)cov[i] | (uint64_t)mask[i] << 16;
+}
+__global__ __launch_bounds__(kBlock) void k_gather_records(const uint64_t* __restrict__ rec,
                                                            const uint32_t* __restrict__ perm, uint32_t n,
                                                            uint32_t nref, uint64_t* __restrict__ okeys,
                                                            uint16_t* __restrict__ ocov, uint8_t* __restrict__ omask) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t j = perm[i];
-  for (uint32_t w = 0; w < nref; ++w) okeys[(uint64_t)i * nref + w] = keys[(uint64_t)j * nref + w];
-  ocov[i] = cov[j];
-  omask[i] = mask[j];
+  const uint64_t* r = rec + (uint64_t)perm[i] * (nref + 1);
+  for (uint32_t w = 0; w < nref; ++w) okeys[(uint64_t)i * nref + w] = r[w];
+  const uint64_t cm = r[nref];
+  ocov[i] = (uint16_t)cm;
+  omask[i] = (uint8_t)(cm >> 16);
 }
 
 // One-word keys sort with their record as the value: cov | mask << 16.
@@ -814,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_cm(const uint64_t* __restrict
   mask[i] = (uint8_t)(v >> 16);
 }
 
-// Scratch layout: [limb keys x2][perm x2][record copy][cub temp].
+// Scratch layout: [limb keys x2][perm x2][packed records][tie run][cub temp].
 // Most significant 64 bits of every key of more than 64 bits: bits
 // [2k - 64, 2k) of the integer (words most significant first).
 __global__ __launch_bounds__(kBlock) void k_top64(const uint64_t* __restrict__ keys, uint32_t n, uint32_t nref,
@@ -876,7 +888,7 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   if (e != hipSuccess) return e;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nn = n ? n : 1;
-  const size_t need = al(nn * 8) * 2 + al(nn * 4) * 2 + al(nn * nref * 8) + al(nn * 2) + al(nn) + al(4) + al(cub_bytes);
+  const size_t need = al(nn * 8) * 2 + al(nn * 4) * 2 + al(nn * (nref + 1) * 8) + al(4) + al(cub_bytes);
   if (keys == nullptr) {
     *tmp_bytes = need;
     return hipSuccess;
@@ -888,9 +900,7 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   uint64_t* k1 = (uint64_t*)p; p += al(nn * 8);
   uint32_t* v0 = (uint32_t*)p; p += al(nn * 4);
   uint32_t* v1 = (uint32_t*)p; p += al(nn * 4);
-  uint64_t* ck = (uint64_t*)p; p += al(nn * nref * 8);
-  uint16_t* cc = (uint16_t*)p; p += al(nn * 2);
-  uint8_t* cm = (uint8_t*)p; p += al(nn);
+  uint64_t* rec = (uint64_t*)p; p += al(nn * (nref + 1) * 8);
   uint32_t* ties = (uint32_t*)p; p += al(4);
   void* ctmp = p;
   const unsigned grid = grid_for(n);
@@ -950,10 +960,8 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
       perm = bv.Current();
     }
   }
-  hipMemcpyAsync(ck, keys, (size_t)n * nref * 8, hipMemcpyDeviceToDevice, s);
-  hipMemcpyAsync(cc, cov, (size_t)n * 2, hipMemcpyDeviceToDevice, s);
-  hipMemcpyAsync(cm, mask, (size_t)n, hipMemcpyDeviceToDevice, s);
-  hipLaunchKernelGGL(k_gather_records, dim3(grid), dim3(kBlock), 0, s, ck, cc, cm, perm, n, nref, keys, cov, mask);
+  hipLaunchKernelGGL(k_pack_records, dim3(grid), dim3(kBlock), 0, s, keys, cov, mask, n, nref, rec);
+  hipLaunchKernelGGL(k_gather_records, dim3(grid), dim3(kBlock), 0, s, rec, perm, n, nref, keys, cov, mask);
   return hipGetLastError();
 }
 

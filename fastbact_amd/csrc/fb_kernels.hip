@@ -827,8 +827,12 @@ __global__ __launch_bounds__(kBlock) void k_unpack_cm(const uint64_t* __restrict
 }
 
 // Scratch layout: [limb keys x2][perm x2][packed records][tie run][cub temp].
-// Most significant 64 bits of every key of more than 64 bits: bits
-// [2k - 64, 2k) of the integer (words most significant first).
+// The first sort of keys of more than 64 bits orders them by their top
+// kTopSortBits bits (the first 16 bases; half the radix passes of the top
+// 64 bits); keys sharing those bits form runs for the fix below.
+constexpr uint32_t kTopSortBits = 32;
+// Top kTopSortBits bits of every key: bits [2k - kTopSortBits, 2k) of the
+// integer (words most significant first), as the low bits of out[i].
 __global__ __launch_bounds__(kBlock) void k_top64(const uint64_t* __restrict__ keys, uint32_t n, uint32_t nref,
                                                   uint32_t k, uint64_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -837,7 +841,7 @@ __global__ __launch_bounds__(kBlock) void k_top64(const uint64_t* __restrict__ k
   const uint64_t* key = keys + (uint64_t)i * nref;
   uint64_t v = key[nref - 1 - a] >> off;
   if (off) v |= key[nref - 2 - a] << (64 - off);
-  out[i] = v;
+  out[i] = v >> (64 - kTopSortBits);
 }
 // Runs of equal values in the sorted v: the first element of a run of two
 // or more walks it, up to kTieRunMax + 1 long.
@@ -923,14 +927,15 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   uint32_t* perm_alt = v1;
   uint32_t first = 0;  // limbs to sort, from the least significant
   if (top_first && L > 1) {
-    // One sort by the top 64 bits gives the full order when no two keys
-    // share them; short runs of keys that do are then sorted in place, and
-    // the limb-by-limb sort runs only for a run longer than kTieRunMax.
+    // One sort by the top kTopSortBits bits gives the full order when no
+    // two keys share them; short runs of keys that do are then sorted in
+    // place, and the limb-by-limb sort runs only for a run longer than
+    // kTieRunMax.
     hipLaunchKernelGGL(k_top64, dim3(grid), dim3(kBlock), 0, s, keys, n, nref, k, k0);
     hipcub::DoubleBuffer<uint64_t> bk(k0, k1);
     hipcub::DoubleBuffer<uint32_t> bv(v0, v1);
     size_t cb = cub_bytes;
-    e = hipcub::DeviceRadixSort::SortPairs(ctmp, cb, bk, bv, (int)n, 0, 64, s);
+    e = hipcub::DeviceRadixSort::SortPairs(ctmp, cb, bk, bv, (int)n, 0, (int)kTopSortBits, s);
     if (e != hipSuccess) return e;
     if ((e = launch_fill_u32(ties, 1, 0u, s))) return e;
     hipLaunchKernelGGL(k_tie_runs, dim3(grid), dim3(kBlock), 0, s, bk.Current(), n, ties);

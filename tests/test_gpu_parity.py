@@ -466,3 +466,32 @@ def test_chunked_regions_fill_up_then_exact_runs(ctx, orc, small_iso, k):
     h = F.build_solid_handle(ctx, dev, k, target_solid_cov=3, cms_width=1 << 18, placement=2)
     assert h.pass2_layout() == "chunks"
     h.close()
+
+
+@pytest.mark.parametrize("k", [55, 77])
+@pytest.mark.parametrize("n_shared", [40, 400])
+def test_sort_runs_of_keys_sharing_their_top_bases(ctx, orc, k, n_shared):
+    """Keys of more than 32 bases are sorted by their first 16 bases, then
+    every run of keys sharing those is ordered in place by the whole key;
+    a run longer than 64 sends the set to the word-by-word sort.  Reads that
+    open with 16 A's give their first windows the same 16 leading bases
+    (runs of 47-56 keys for 40 such reads, over 500 for 400): both forms
+    give the oracle's sorted set."""
+    rng = np.random.default_rng(k * 1000 + n_shared)
+    genome = rng.choice(list(b"ACGT"), size=60_000).astype(np.uint8).tobytes()
+    reads = []
+    for _ in range(n_shared):
+        reads.append(b"A" * 16 + rng.choice(list(b"ACGT"), size=134).astype(np.uint8).tobytes())
+    for _ in range(600):
+        s0 = int(rng.integers(0, len(genome) - 150))
+        reads.append(genome[s0:s0 + 150])
+    half = len(reads) // 2
+    bank = F.ReadBank(F.BitSeqArena.from_ascii(reads[:half]), F.BitSeqArena.from_ascii(reads[half:]))
+    dev = F.DeviceBank(ctx, bank)
+    try:
+        for placement in (1, 2):
+            g = F.build_solid_kmers(ctx, dev, k, target_solid_cov=1, cms_width=1 << 18, placement=placement)
+            r = orc.build_solid(oracle_arenas(bank), k, target_solid_cov=1, width=1 << 18)
+            _solid_equal(g, r)
+    finally:
+        dev.close()

@@ -827,33 +827,33 @@ __global__ __launch_bounds__(kBlock) void k_unpack_cm(const uint64_t* __restrict
 }
 
 // Scratch layout: [limb keys x2][perm x2][packed records][tie run][cub temp].
-// The first sort of keys of more than 64 bits orders them by their top
-// kTopSortBits bits (the first 16 bases; half the radix passes of the top
-// 64 bits); keys sharing those bits form runs for the fix below.
+// The first sort of keys of more than 32 bases orders them by their first
+// 16 bases (kTopSortBits bits, 32-bit sort keys: four radix passes);
+// keys sharing those bases form runs for the fix below.
 constexpr uint32_t kTopSortBits = 32;
-// Top kTopSortBits bits of every key: bits [2k - kTopSortBits, 2k) of the
-// integer (words most significant first), as the low bits of out[i].
-__global__ __launch_bounds__(kBlock) void k_top64(const uint64_t* __restrict__ keys, uint32_t n, uint32_t nref,
-                                                  uint32_t k, uint64_t* __restrict__ out) {
+// The top kTopSortBits bits of every key: bits [2k - 32, 2k) of the
+// integer (words most significant first); 2k > 64.
+__global__ __launch_bounds__(kBlock) void k_top_digit(const uint64_t* __restrict__ keys, uint32_t n, uint32_t nref,
+                                                      uint32_t k, uint32_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint32_t sh = 2 * k - 64, a = sh >> 6, off = sh & 63u;
+  const uint32_t sh = 2 * k - kTopSortBits, a = sh >> 6, off = sh & 63u;  // word a from the least significant
   const uint64_t* key = keys + (uint64_t)i * nref;
   uint64_t v = key[nref - 1 - a] >> off;
-  if (off) v |= key[nref - 2 - a] << (64 - off);
-  out[i] = v >> (64 - kTopSortBits);
+  if (off > 64u - kTopSortBits) v |= key[nref - 2 - a] << (64 - off);  // the digit spans two words
+  out[i] = (uint32_t)v;
 }
 // Runs of equal values in the sorted v: the first element of a run of two
 // or more walks it, up to kTieRunMax + 1 long.
 constexpr uint32_t kTieRunMax = 64;
-__device__ __forceinline__ uint32_t tie_run(const uint64_t* __restrict__ v, uint32_t n, uint32_t i) {
+__device__ __forceinline__ uint32_t tie_run(const uint32_t* __restrict__ v, uint32_t n, uint32_t i) {
   if (i + 1 >= n || v[i] != v[i + 1] || (i > 0 && v[i - 1] == v[i])) return 0u;
   uint32_t j = i + 2;
   while (j < n && j - i <= kTieRunMax && v[j] == v[i]) ++j;
   return j - i;
 }
 // *maxrun = the longest run (capped at kTieRunMax + 1; *maxrun zeroed before).
-__global__ __launch_bounds__(kBlock) void k_tie_runs(const uint64_t* __restrict__ v, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_tie_runs(const uint32_t* __restrict__ v, uint32_t n,
                                                      uint32_t* __restrict__ maxrun) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -862,7 +862,7 @@ __global__ __launch_bounds__(kBlock) void k_tie_runs(const uint64_t* __restrict_
 }
 // Sorts the permutation within every run (at most kTieRunMax long) by the
 // whole key, words most significant first.
-__global__ __launch_bounds__(kBlock) void k_fix_ties(const uint64_t* __restrict__ v, uint32_t n,
+__global__ __launch_bounds__(kBlock) void k_fix_ties(const uint32_t* __restrict__ v, uint32_t n,
                                                      const uint64_t* __restrict__ keys, uint32_t nref,
                                                      uint32_t* __restrict__ perm) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -890,6 +890,13 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   hipcub::DoubleBuffer<uint32_t> dv(nullptr, nullptr);
   hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, cub_bytes, dk, dv, (int)(n ? n : 1), 0, 64, s);
   if (e != hipSuccess) return e;
+  {
+    size_t digit_bytes = 0;
+    hipcub::DoubleBuffer<uint32_t> dd(nullptr, nullptr);
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, digit_bytes, dd, dv, (int)(n ? n : 1), 0, (int)kTopSortBits, s);
+    if (e != hipSuccess) return e;
+    if (digit_bytes > cub_bytes) cub_bytes = digit_bytes;
+  }
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nn = n ? n : 1;
   const size_t need = al(nn * 8) * 2 + al(nn * 4) * 2 + al(nn * (nref + 1) * 8) + al(4) + al(cub_bytes);
@@ -908,6 +915,9 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   uint32_t* ties = (uint32_t*)p; p += al(4);
   void* ctmp = p;
   const unsigned grid = grid_for(n);
+  // one-word keys (k <= 32) sort directly with their record as the value:
+  // for them that is faster than a digit sort followed by a gather
+  const bool digits = top_first && nref > 1;
   if (nref == 1) {
     // one word a key (k <= 32): its cov and mask travel with it as the
     // sort's value, so no permutation and no gather
@@ -926,13 +936,14 @@ hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, 
   uint32_t* perm = v0;
   uint32_t* perm_alt = v1;
   uint32_t first = 0;  // limbs to sort, from the least significant
-  if (top_first && L > 1) {
+  if (digits) {
     // One sort by the top kTopSortBits bits gives the full order when no
     // two keys share them; short runs of keys that do are then sorted in
     // place, and the limb-by-limb sort runs only for a run longer than
     // kTieRunMax.
-    hipLaunchKernelGGL(k_top64, dim3(grid), dim3(kBlock), 0, s, keys, n, nref, k, k0);
-    hipcub::DoubleBuffer<uint64_t> bk(k0, k1);
+    uint32_t* d0 = reinterpret_cast<uint32_t*>(k0);
+    hipLaunchKernelGGL(k_top_digit, dim3(grid), dim3(kBlock), 0, s, keys, n, nref, k, d0);
+    hipcub::DoubleBuffer<uint32_t> bk(d0, reinterpret_cast<uint32_t*>(k1));
     hipcub::DoubleBuffer<uint32_t> bv(v0, v1);
     size_t cb = cub_bytes;
     e = hipcub::DeviceRadixSort::SortPairs(ctmp, cb, bk, bv, (int)n, 0, (int)kTopSortBits, s);

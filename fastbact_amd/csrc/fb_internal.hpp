@@ -172,8 +172,9 @@ hipError_t launch_pass3(const DevSolidTable& t, uint32_t k, uint32_t n_ref_words
                         const uint32_t* out_base = nullptr, uint32_t out_cap = 0xFFFFFFFFu);
 // Sort the compacted records by k-mer (integer order).  Uses scratch from
 // the caller: tmp_bytes queried with keys == nullptr.  top_first: for keys
-// over 64 bits, first one sort by the top 64 bits and a check (synchronising
-// the stream) that no two keys share them, else the limb-by-limb sort.
+// over 64 bits, first one sort by their first 16 bases, a check
+// (synchronising the stream) for runs of keys that share them, an in-place
+// sort of runs of at most 64, else the limb-by-limb sort.
 hipError_t sort_solid(uint64_t* keys, uint16_t* cov, uint8_t* mask, uint32_t n, uint32_t n_ref_words,
                       uint32_t k, void* tmp, size_t* tmp_bytes, hipStream_t s, bool top_first = false);
 

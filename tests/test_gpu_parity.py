@@ -314,12 +314,15 @@ def test_1mbp_isolate_100k_pairs(ctx, orc):
 
 @pytest.mark.parametrize("k", [33, 55, 64, 77, 127])
 def test_sort_with_shared_top_bits_matches_oracle(ctx, orc, k):
-    """Keys over 64 bits are sorted by their top 64 bits first; runs of keys
-    that share those bits are then sorted in place when at most 64 long,
+    """Keys over 64 bits are sorted by their first 16 bases first; runs of
+    keys that share those are then sorted in place when at most 64 long,
     else the whole set is sorted limb by limb.  Reads that start with the
-    same 40 bases give solid k-mers sharing their first 32 bases: in runs
-    of up to 20 (20 distinct reads) or of 300 (300 reads); a random isolate
-    gives no runs.  Every order equals the oracle's."""
+    same 40 bases give solid k-mers sharing their first 32 bases, in runs
+    of up to 20 or of 300 over their top 64 bits (asserted below); over the
+    first 16 bases the runs are 46-75 and 2,000-3,700 keys long, so k = 33
+    takes the in-place sort here and the other k the limb-by-limb one (runs
+    of 47-56 at k = 55 and 77: the test below).  A random isolate has no
+    runs over its top 64 bits.  Every order equals the oracle's."""
     rng = np.random.default_rng(k)
     acgt = np.frombuffer(b"ACGT", np.uint8)
     head = np.frombuffer(b"AC" * 20, np.uint8)
@@ -336,7 +339,7 @@ def test_sort_with_shared_top_bits_matches_oracle(ctx, orc, k):
         nref = r["keys"].shape[1]
         top = [sum(int(x) << (64 * (nref - 1 - j)) for j, x in enumerate(row)) >> (2 * k - 64) for row in r["keys"]]
         runs.append(max(np.unique(np.array(top, dtype=object), return_counts=True)[1]))
-    # the three paths: short runs sorted in place, a long run, no run (an
+    # the data's runs over the top 64 bits: short, long, none (an
     # error-free isolate)
     assert 2 <= runs[0] <= 64 and runs[2] == 1, runs
     assert runs[1] > 64 or k == 33, runs
